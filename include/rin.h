@@ -1,0 +1,142 @@
+/*
+ * rin.h — C ABI of the MI355X per-tet implicit-arrangement / material-interface
+ * hot path.
+ *
+ * The reference runs this path inside implicit_arrangement() and
+ * material_interface() (src/implicit_arrangement.h:39-62,
+ * src/material_interface.h:38-61): function signs, the active-function filter,
+ * one simplicial_arrangement cut per active tet, extract_iso_mesh /
+ * extract_MI_mesh and the vertex coordinates.  This library performs those
+ * stages on one GPU and hands back plain arrays; the drop-in C++ entry points
+ * (include/rin/implicit_arrangement.h) call it and run the host graph stages.
+ *
+ * Conventions: every call returns RIN_OK (0) or a negative status; no C++
+ * exception crosses the ABI; rin_last_error() describes the last failure of a
+ * context.  The library owns device memory until the matching *_release; the
+ * caller owns every host buffer.  One context per host thread; contexts are
+ * independent (no process-global state).  Results are deterministic and do not
+ * depend on scheduling.
+ */
+#ifndef RIN_H
+#define RIN_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct rin_ctx rin_ctx;
+typedef struct rin_grid rin_grid;
+typedef struct rin_values rin_values;
+typedef struct rin_result rin_result;
+
+enum rin_status {
+    RIN_OK = 0,
+    RIN_ERR_ARG = -1,         /* bad argument */
+    RIN_ERR_DEVICE = -2,      /* HIP runtime error */
+    RIN_ERR_MEMORY = -3,      /* device allocation failed */
+    RIN_ERR_CAPACITY = -4,    /* input exceeds a device capacity (active functions, keys) */
+    RIN_ERR_UNSUPPORTED = -5, /* feature not available on this path */
+    RIN_ERR_INTERNAL = -6
+};
+
+/* Function descriptor (the reference reads these from a function-file JSON,
+ * app/implicit_arrangement.cpp:59).  type: 0 plane (p0..2 point, p3..5 normal),
+ * 1 sphere (p0..2 center, p3 radius, p4 squared flag), 2 cylinder (p0..2 axis
+ * point, p3..5 unit axis, p6 radius, p7 flipped), 3 torus (p0..2 center, p3..5
+ * unit axis, p6 major radius, p7 minor radius, p8 flipped), 4 zero. */
+typedef struct rin_func_desc {
+    int32_t type;
+    int32_t reserved;
+    double params[12];
+} rin_func_desc;
+
+/* rin_run flags */
+#define RIN_FLAG_MI 1u           /* material interface instead of implicit arrangement */
+#define RIN_FLAG_REVERSE 2u      /* insert active functions in descending order (with CUT_ONLY) */
+#define RIN_FLAG_NO_LOOKUP 4u    /* general cut for every tet */
+#define RIN_FLAG_BATCHED 8u      /* bucketed fused form */
+#define RIN_FLAG_KEEP_RECORDS 16u /* keep per-tet cut records for rin_result_records */
+#define RIN_FLAG_CUT_ONLY 32u    /* stop after the per-tet cut (the -R robustness check) */
+
+typedef struct rin_sizes {
+    uint64_t n_tets, n_funcs;
+    uint64_t num_degenerate;   /* IA: zero (vertex,function) entries; MI: vertices with tied maxima */
+    uint64_t num_intersecting; /* active tets */
+    uint64_t bucket[3];        /* IA: 1, 2, >=3 active functions; MI: 2, 3, >=4 */
+    uint64_t n_func_in_tet;    /* CSR length */
+    uint64_t n_verts, n_faces, n_face_verts, n_tet_faces;
+} rin_sizes;
+
+/* Context: one HIP stream and a device-memory cache on `device`.
+ * Replaces the process-global lookup-table state of the reference
+ * (app/implicit_arrangement.cpp:29-42). */
+int rin_create(rin_ctx** out, int32_t device);
+int rin_destroy(rin_ctx* ctx);
+const char* rin_last_error(const rin_ctx* ctx);
+
+/* Background mesh.  rin_grid_regular restates generate_tet_mesh
+ * (src/io.cpp:95-152): (res+1)^3 vertices, 5*res^3 tets, tets implicit.
+ * rin_grid_explicit takes the arrays load_tet_mesh produces
+ * (src/io.cpp:65-93): V x 3 points, T x 4 corner indices. */
+int rin_grid_regular(rin_ctx* ctx, int32_t res, const double bmin[3], const double bmax[3], rin_grid** out);
+int rin_grid_explicit(rin_ctx* ctx, const double* pts, uint64_t n_pts, const uint64_t* tets, uint64_t n_tets,
+                      rin_grid** out);
+int rin_grid_points(rin_ctx* ctx, const rin_grid* grid, double* pts_out);
+int rin_grid_release(rin_ctx* ctx, rin_grid* grid);
+
+/* Function values at the grid vertices, kept on the device.
+ * rin_values_eval evaluates descriptors on the GPU (replaces load_functions,
+ * app/implicit_arrangement.cpp:59); rin_values_upload takes the reference's
+ * row-major V x F funcVals (src/implicit_arrangement.h:47). */
+int rin_values_eval(rin_ctx* ctx, const rin_grid* grid, const rin_func_desc* funcs, uint32_t n_funcs,
+                    rin_values** out);
+int rin_values_upload(rin_ctx* ctx, const rin_grid* grid, const double* vals_rowmajor, uint32_t n_funcs,
+                      rin_values** out);
+int rin_values_download(rin_ctx* ctx, const rin_values* vals, double* vals_rowmajor);
+int rin_values_release(rin_ctx* ctx, rin_values* vals);
+
+/* The hot path: src/implicit_arrangement.cpp:53-402 (IA) or
+ * src/material_interface.cpp:53-447 (MI, with RIN_FLAG_MI). */
+int rin_run(rin_ctx* ctx, const rin_grid* grid, const rin_values* vals, uint32_t flags, rin_result** out);
+
+int rin_result_sizes(rin_ctx* ctx, const rin_result* res, rin_sizes* sizes);
+/* CSR of active functions: func_in_tet[n_func_in_tet], start_index[n_tets+1]
+ * (src/implicit_arrangement.cpp:82-116). */
+int rin_result_csr(rin_ctx* ctx, const rin_result* res, uint64_t* func_in_tet, uint64_t* start_index);
+/* Vertices in first-occurrence order: 11 words each — tet_index,
+ * tet_vert_index, simplex_size, simplex_vert_indices[4],
+ * func_indices[4] (MI: material_indices) with UINT64_MAX for unused slots —
+ * and n_verts x 3 coordinates (src/mesh.h:28-57, src/extract_mesh.cpp:1446-1637). */
+int rin_result_verts(rin_ctx* ctx, const rin_result* res, uint64_t* rec11, double* xyz);
+/* Faces in first-occurrence order: vert_start[n_faces+1], verts[n_face_verts],
+ * tf_start[n_faces+1], tf[2*n_tet_faces] (tet, local face) pairs,
+ * func[2*n_faces] (func_index; UINT64_MAX when unused) (src/mesh.h:15-25). */
+int rin_result_faces(rin_ctx* ctx, const rin_result* res, uint64_t* vert_start, uint64_t* verts, uint64_t* tf_start,
+                     uint64_t* tf, uint64_t* func);
+/* Per-tet cut records (layout in include/rin/cut_record.h) of the given tets,
+ * packed: rec_start[n+1] word offsets into words[cap_words].  Needs
+ * RIN_FLAG_KEEP_RECORDS.  Empty tets get an empty record. */
+int rin_result_records(rin_ctx* ctx, const rin_result* res, const uint64_t* tets, uint64_t n, uint64_t* rec_start,
+                       uint16_t* words, uint64_t cap_words);
+/* With RIN_FLAG_CUT_ONLY: (vertices, faces, cells) of every active tet's
+ * complex, in tet order, n_active x 3 (the -R comparison,
+ * src/implicit_arrangement.cpp:136-243). */
+int rin_result_cut_counts(rin_ctx* ctx, const rin_result* res, uint64_t* counts);
+/* Device stage times in milliseconds; labels follow the reference's timing
+ * labels where a stage has one.  Returns the number of stages. */
+int rin_result_stage_times(rin_ctx* ctx, const rin_result* res, double* ms, int32_t cap);
+const char* rin_result_stage_label(rin_ctx* ctx, const rin_result* res, int32_t i);
+int rin_result_release(rin_ctx* ctx, rin_result* res);
+
+/* Test hook with no counterpart in the reference: run the device per-tet cut
+ * on independent plane (IA) or material (MI) sets.  rows holds sum(ks) rows of
+ * 4 corner values; each case writes at most cap words at words + i*cap and its
+ * length (or a negative error) to lens[i]. */
+int rin_debug_cuts(rin_ctx* ctx, int32_t mi, const double* rows, const int32_t* ks, int32_t n_cases, int32_t cap,
+                   uint16_t* words, int32_t* lens);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RIN_H */

@@ -1,0 +1,85 @@
+// Device hot path: values -> signs -> active-function CSR -> per-tet cut ->
+// iso-vertex / iso-face extraction with first-occurrence dedup -> xyz.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../devmem.h"
+#include "eval_dev.h"
+#include "grid_dev.h"
+
+namespace rin {
+
+constexpr uint32_t kMaxFuncs = 64;   // function ids are packed in 6 bits
+constexpr uint32_t kMaxActive = 10;  // active functions per tet on the device path
+
+struct Grid {
+    GridView view{};
+    DBuf<double> pts;
+    DBuf<uint32_t> tets, tedge, tface;
+};
+
+struct Values {
+    uint32_t F = 0;
+    uint64_t V = 0;
+    DBuf<double> vals;  // F x V, function-major
+};
+
+enum RunFlags : uint32_t {
+    RUN_MI = 1,
+    RUN_REVERSE = 2,     // insert functions in descending order (cut only)
+    RUN_NO_LOOKUP = 4,   // general cut for every tet
+    RUN_BATCHED = 8,     // bucketed fused form
+    RUN_KEEP_RECORDS = 16,
+    RUN_CUT_ONLY = 32,   // stop after the per-tet cut (robustness check)
+};
+
+struct StageTimes {
+    std::vector<std::pair<std::string, float>> ms;
+};
+
+struct RunOut {
+    bool mi = false;
+    uint32_t flags = 0;
+    uint64_t T = 0, F = 0;
+    uint64_t num_degenerate = 0, num_intersecting = 0;
+    uint64_t nb[3] = {0, 0, 0};
+    uint64_t n_active = 0, n_fit = 0;
+    uint32_t err = 0;  // bit 0 capacity, bit 1 predicate range, bit 2 record overflow
+    DBuf<uint8_t> fit;          // CSR function ids
+    DBuf<uint32_t> csr_off;     // T + 1
+    DBuf<uint32_t> act;         // tet of each active rank
+    DBuf<uint8_t> act_k;
+    DBuf<uint32_t> rank_of_tet; // T (exclusive scan of the active flags)
+    DBuf<uint32_t> rec_off;     // n_active + 1 (words)
+    DBuf<uint16_t> rec;
+    DBuf<uint32_t> cut_counts;  // n_active x 3: (vertices, faces, cells) for the robustness check
+    // mesh
+    uint64_t n_verts = 0, n_faces = 0, n_face_verts = 0, n_tet_faces = 0;
+    DBuf<uint32_t> v_tet;
+    DBuf<uint16_t> v_j;
+    DBuf<uint8_t> v_ss;
+    DBuf<uint32_t> v_sv;  // 4 per vertex
+    DBuf<uint8_t> v_fn;   // 4 per vertex (0xFF unused)
+    DBuf<double> xyz;     // 3 per vertex
+    DBuf<uint32_t> f_vstart, f_verts, f_tfstart, f_tf_tet;
+    DBuf<uint16_t> f_tf_local;
+    DBuf<uint8_t> f_func;  // 2 per face (0xFF unused)
+    StageTimes times;
+};
+
+void grid_regular(DevMem& mem, int res, const double bmin[3], const double bmax[3], Grid& g, hipStream_t s);
+void grid_explicit(DevMem& mem, const double* pts, uint64_t V, const uint64_t* tets, uint64_t T, Grid& g,
+                   hipStream_t s);
+void values_eval(DevMem& mem, const Grid& g, const FuncDesc* funcs, uint32_t F, Values& v, hipStream_t s);
+void values_upload(DevMem& mem, const Grid& g, const double* rowmajor, uint32_t F, Values& v, hipStream_t s);
+void run_pipeline(DevMem& mem, const Grid& g, const Values& v, uint32_t flags, RunOut& out, hipStream_t s);
+
+// diagnostics: run the device cut on independent plane/material sets
+void debug_cuts(DevMem& mem, int mi, const double* rows, const int* ks, int ncase, int cap, uint16_t* words,
+                int* lens, hipStream_t s);
+
+}  // namespace rin

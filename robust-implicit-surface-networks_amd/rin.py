@@ -1,0 +1,262 @@
+"""ctypes binding of the C ABI in include/rin.h (librin.so, built in-tree).
+
+This is the Python face of the drop-in boundary: the same entry points a
+reference-side FFI would bind.  It fails loudly when the HIP library is not
+built; there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "librin.so")
+
+FLAG_MI = 1
+FLAG_REVERSE = 2
+FLAG_NO_LOOKUP = 4
+FLAG_BATCHED = 8
+FLAG_KEEP_RECORDS = 16
+FLAG_CUT_ONLY = 32
+
+STATUS = {0: "ok", -1: "bad argument", -2: "device error", -3: "out of device memory", -4: "capacity exceeded",
+          -5: "unsupported", -6: "internal error"}
+
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+_f64p = ctypes.POINTER(ctypes.c_double)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_u16p = ctypes.POINTER(ctypes.c_uint16)
+_vp = ctypes.c_void_p
+
+
+class FuncDesc(ctypes.Structure):
+    _fields_ = [("type", ctypes.c_int32), ("reserved", ctypes.c_int32), ("params", ctypes.c_double * 12)]
+
+
+class Sizes(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint64) for n in
+                ("n_tets", "n_funcs", "num_degenerate", "num_intersecting", "b0", "b1", "b2", "n_func_in_tet",
+                 "n_verts", "n_faces", "n_face_verts", "n_tet_faces")]
+
+
+class RinError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("rin error %d (%s): %s" % (code, STATUS.get(code, "?"), msg))
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("librin.so is not built (run __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        L.rin_create.argtypes = [ctypes.POINTER(_vp), ctypes.c_int32]
+        L.rin_destroy.argtypes = [_vp]
+        L.rin_last_error.argtypes = [_vp]
+        L.rin_last_error.restype = ctypes.c_char_p
+        L.rin_grid_regular.argtypes = [_vp, ctypes.c_int32, _f64p, _f64p, ctypes.POINTER(_vp)]
+        L.rin_grid_explicit.argtypes = [_vp, _f64p, ctypes.c_uint64, _u64p, ctypes.c_uint64, ctypes.POINTER(_vp)]
+        L.rin_grid_points.argtypes = [_vp, _vp, _f64p]
+        L.rin_grid_release.argtypes = [_vp, _vp]
+        L.rin_values_eval.argtypes = [_vp, _vp, ctypes.POINTER(FuncDesc), ctypes.c_uint32, ctypes.POINTER(_vp)]
+        L.rin_values_upload.argtypes = [_vp, _vp, _f64p, ctypes.c_uint32, ctypes.POINTER(_vp)]
+        L.rin_values_download.argtypes = [_vp, _vp, _f64p]
+        L.rin_values_release.argtypes = [_vp, _vp]
+        L.rin_run.argtypes = [_vp, _vp, _vp, ctypes.c_uint32, ctypes.POINTER(_vp)]
+        L.rin_result_sizes.argtypes = [_vp, _vp, ctypes.POINTER(Sizes)]
+        L.rin_result_csr.argtypes = [_vp, _vp, _u64p, _u64p]
+        L.rin_result_verts.argtypes = [_vp, _vp, _u64p, _f64p]
+        L.rin_result_faces.argtypes = [_vp, _vp, _u64p, _u64p, _u64p, _u64p, _u64p]
+        L.rin_result_records.argtypes = [_vp, _vp, _u64p, ctypes.c_uint64, _u64p, _u16p, ctypes.c_uint64]
+        L.rin_result_cut_counts.argtypes = [_vp, _vp, _u64p]
+        L.rin_result_stage_times.argtypes = [_vp, _vp, _f64p, ctypes.c_int32]
+        L.rin_result_stage_label.argtypes = [_vp, _vp, ctypes.c_int32]
+        L.rin_result_stage_label.restype = ctypes.c_char_p
+        L.rin_result_release.argtypes = [_vp, _vp]
+        L.rin_debug_cuts.argtypes = [_vp, ctypes.c_int32, _f64p, _i32p, ctypes.c_int32, ctypes.c_int32, _u16p,
+                                     _i32p]
+        _lib = L
+    return _lib
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+def descs(types, params):
+    F = len(types)
+    arr = (FuncDesc * F)()
+    for j in range(F):
+        arr[j].type = int(types[j])
+        for q in range(12):
+            arr[j].params[q] = float(params[j][q])
+    return arr
+
+
+class Context:
+    def __init__(self, device=0):
+        self.h = _vp()
+        rc = lib().rin_create(ctypes.byref(self.h), device)
+        if rc != 0:
+            raise RinError(rc, "rin_create failed (no GPU?)")
+
+    def check(self, rc):
+        if rc != 0:
+            raise RinError(rc, lib().rin_last_error(self.h).decode())
+
+    def close(self):
+        if self.h:
+            lib().rin_destroy(self.h)
+            self.h = _vp()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # grids
+    def grid_regular(self, res, bmin=(-1.0, -1.0, -1.0), bmax=(1.0, 1.0, 1.0)):
+        g = _vp()
+        lo = np.asarray(bmin, dtype=np.float64)
+        hi = np.asarray(bmax, dtype=np.float64)
+        self.check(lib().rin_grid_regular(self.h, res, _p(lo, _f64p), _p(hi, _f64p), ctypes.byref(g)))
+        N = res + 1
+        return Grid(self, g, N ** 3, 5 * res ** 3)
+
+    def grid_explicit(self, pts, tets):
+        pts = np.ascontiguousarray(pts, dtype=np.float64)
+        tets = np.ascontiguousarray(tets, dtype=np.uint64)
+        g = _vp()
+        self.check(lib().rin_grid_explicit(self.h, _p(pts, _f64p), pts.shape[0], _p(tets, _u64p), tets.shape[0],
+                                           ctypes.byref(g)))
+        return Grid(self, g, pts.shape[0], tets.shape[0])
+
+    def debug_cuts(self, mi, cases, cap=4096):
+        """cases: list of (k, 4) arrays -> list of uint16 records (or negative codes)."""
+        ks = np.asarray([len(c) for c in cases], dtype=np.int32)
+        rows = np.ascontiguousarray(np.concatenate([np.asarray(c, dtype=np.float64).reshape(-1, 4) for c in cases]))
+        n = len(cases)
+        words = np.zeros(n * cap, dtype=np.uint16)
+        lens = np.zeros(n, dtype=np.int32)
+        self.check(lib().rin_debug_cuts(self.h, int(mi), _p(rows, _f64p), _p(ks, _i32p), n, cap, _p(words, _u16p),
+                                        _p(lens, _i32p)))
+        out = []
+        for i in range(n):
+            L = int(lens[i])
+            out.append(words[i * cap:i * cap + L].copy() if L > 0 else L)
+        return out
+
+
+class Grid:
+    def __init__(self, ctx, h, V, T):
+        self.ctx, self.h, self.V, self.T = ctx, h, V, T
+
+    def points(self):
+        out = np.zeros((self.V, 3), dtype=np.float64)
+        self.ctx.check(lib().rin_grid_points(self.ctx.h, self.h, _p(out, _f64p)))
+        return out
+
+    def eval(self, types, params):
+        d = descs(types, params)
+        v = _vp()
+        self.ctx.check(lib().rin_values_eval(self.ctx.h, self.h, d, len(types), ctypes.byref(v)))
+        return Values(self, v, len(types))
+
+    def upload(self, vals_rowmajor):
+        a = np.ascontiguousarray(vals_rowmajor, dtype=np.float64)
+        v = _vp()
+        self.ctx.check(lib().rin_values_upload(self.ctx.h, self.h, _p(a, _f64p), a.shape[1], ctypes.byref(v)))
+        return Values(self, v, a.shape[1])
+
+    def release(self):
+        if self.h:
+            lib().rin_grid_release(self.ctx.h, self.h)
+            self.h = None
+
+
+class Values:
+    def __init__(self, grid, h, F):
+        self.grid, self.h, self.F = grid, h, F
+
+    def download(self):
+        out = np.zeros((self.grid.V, self.F), dtype=np.float64)
+        self.grid.ctx.check(lib().rin_values_download(self.grid.ctx.h, self.h, _p(out, _f64p)))
+        return out
+
+    def run(self, flags=0):
+        r = _vp()
+        ctx = self.grid.ctx
+        ctx.check(lib().rin_run(ctx.h, self.grid.h, self.h, flags, ctypes.byref(r)))
+        return Result(ctx, r, self.grid.T)
+
+    def release(self):
+        if self.h:
+            lib().rin_values_release(self.grid.ctx.h, self.h)
+            self.h = None
+
+
+class Result:
+    def __init__(self, ctx, h, T):
+        self.ctx, self.h, self.T = ctx, h, T
+        s = Sizes()
+        ctx.check(lib().rin_result_sizes(ctx.h, h, ctypes.byref(s)))
+        self.sizes = s
+        self.num_degenerate = s.num_degenerate
+        self.num_intersecting = s.num_intersecting
+        self.buckets = [s.b0, s.b1, s.b2]
+
+    def csr(self):
+        s = self.sizes
+        fit = np.zeros(s.n_func_in_tet, dtype=np.uint64)
+        st = np.zeros(s.n_tets + 1, dtype=np.uint64)
+        self.ctx.check(lib().rin_result_csr(self.ctx.h, self.h, _p(fit, _u64p), _p(st, _u64p)))
+        return fit, st
+
+    def verts(self):
+        n = self.sizes.n_verts
+        rec = np.zeros((n, 11), dtype=np.uint64)
+        xyz = np.zeros((n, 3), dtype=np.float64)
+        self.ctx.check(lib().rin_result_verts(self.ctx.h, self.h, _p(rec, _u64p), _p(xyz, _f64p)))
+        return rec, xyz
+
+    def faces(self):
+        s = self.sizes
+        vs = np.zeros(s.n_faces + 1, dtype=np.uint64)
+        v = np.zeros(s.n_face_verts, dtype=np.uint64)
+        ts = np.zeros(s.n_faces + 1, dtype=np.uint64)
+        tf = np.zeros((s.n_tet_faces, 2), dtype=np.uint64)
+        fn = np.zeros((s.n_faces, 2), dtype=np.uint64)
+        self.ctx.check(lib().rin_result_faces(self.ctx.h, self.h, _p(vs, _u64p), _p(v, _u64p), _p(ts, _u64p),
+                                              _p(tf, _u64p), _p(fn, _u64p)))
+        return vs, v, ts, tf, fn
+
+    def records(self, tets, cap=None):
+        tets = np.ascontiguousarray(tets, dtype=np.uint64)
+        cap = cap or 8192 * max(1, len(tets))
+        st = np.zeros(len(tets) + 1, dtype=np.uint64)
+        w = np.zeros(cap, dtype=np.uint16)
+        self.ctx.check(lib().rin_result_records(self.ctx.h, self.h, _p(tets, _u64p), len(tets), _p(st, _u64p),
+                                                _p(w, _u16p), cap))
+        return st, w[:int(st[-1])]
+
+    def cut_counts(self):
+        n = self.sizes.num_intersecting
+        c = np.zeros((n, 3), dtype=np.uint64)
+        self.ctx.check(lib().rin_result_cut_counts(self.ctx.h, self.h, _p(c, _u64p)))
+        return c
+
+    def stage_times(self):
+        L = lib()
+        ms = np.zeros(64, dtype=np.float64)
+        n = L.rin_result_stage_times(self.ctx.h, self.h, _p(ms, _f64p), 64)
+        return [(L.rin_result_stage_label(self.ctx.h, self.h, i).decode(), float(ms[i])) for i in range(min(n, 64))]
+
+    def release(self):
+        if self.h:
+            lib().rin_result_release(self.ctx.h, self.h)
+            self.h = None

@@ -1,0 +1,90 @@
+"""Device pipeline (through the C ABI) against the oracle on the same inputs:
+grid points and function values bit for bit, the active-function CSR, the
+bucket counts, iso-vertices (records and coordinates) and iso-faces (vertex
+cycles, tet-face occurrences, function labels) bit for bit."""
+import numpy as np
+import pytest
+
+import support as S
+import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _compare(dev, ref):
+    s = dev.sizes
+    assert dev.num_intersecting == ref.num_intersecting
+    assert dev.buckets == ref.buckets
+    fit, st = dev.csr()
+    np.testing.assert_array_equal(st, ref.start_index)
+    np.testing.assert_array_equal(fit, ref.func_in_tet)
+    rec, xyz = dev.verts()
+    assert rec.shape == ref.verts.shape, (rec.shape, ref.verts.shape)
+    np.testing.assert_array_equal(rec, ref.verts)
+    # bit-exact coordinates (same expression order, IEEE division)
+    np.testing.assert_array_equal(xyz.view(np.uint64), ref.xyz.view(np.uint64))
+    vs, v, ts, tf, fn = dev.faces()
+    np.testing.assert_array_equal(vs, ref.face_start)
+    np.testing.assert_array_equal(v, ref.face_verts)
+    np.testing.assert_array_equal(ts, ref.tf_start)
+    np.testing.assert_array_equal(tf, ref.tf)
+    np.testing.assert_array_equal(fn, ref.face_func)
+
+
+def test_grid_points(ctx):
+    for res in (20, 32):
+        g = ctx.grid_regular(res)
+        pts, _ = S.grid(res)
+        np.testing.assert_array_equal(g.points().view(np.uint64), pts.view(np.uint64))
+        g.release()
+
+
+def test_eval_bit_exact(ctx):
+    res, fs = W.config2()
+    res = 24
+    g = ctx.grid_regular(res)
+    types, params = fs.arrays()
+    # squared and plain spheres both
+    fs2 = W.mixed(np.random.default_rng(9), 2, 3, 2, 2)
+    fs2.sphere([0.1, 0.2, -0.3], 0.45, squared=True)
+    for t, p in (fs.arrays(), fs2.arrays()):
+        vals = g.eval(t, p)
+        pts, _ = S.grid(res)
+        ref = S.evaluate(t, p, pts)
+        np.testing.assert_array_equal(vals.download().view(np.uint64), ref.view(np.uint64))
+        vals.release()
+    g.release()
+
+
+@pytest.mark.parametrize("which", ["config1", "config2_small", "config4", "config4_shuffled"])
+def test_pipeline_ia_matches_oracle(ctx, which):
+    if which == "config1":
+        res, fs = W.config1()
+    elif which == "config2_small":
+        _, fs = W.config2()
+        res = 40
+    elif which == "config4":
+        res, fs = W.config4()
+    else:
+        res, fs = W.config4(shuffle_seed=44)
+    types, params = fs.arrays()
+    g = ctx.grid_regular(res)
+    vals = g.eval(types, params)
+    dev = vals.run(0)
+    pts, tets = S.grid(res)
+    ref = S.OracleRun(0, pts, tets, S.evaluate(types, params, pts))
+    _compare(dev, ref)
+    dev.release()
+    vals.release()
+    g.release()
+
+
+def test_pipeline_ia_explicit_mesh(ctx):
+    m = np.load(S.REF_TESTS + "/mesh.npz")
+    pts, tets = m["pts"], m["tets"].astype(np.uint64)
+    types, params = S.load_ref_funcs("8-sphere.json")
+    g = ctx.grid_explicit(pts, tets)
+    vals = g.upload(S.evaluate(types, params, pts))
+    dev = vals.run(0)
+    ref = S.OracleRun(0, pts, tets, S.evaluate(types, params, pts))
+    _compare(dev, ref)
