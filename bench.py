@@ -90,7 +90,7 @@ def main():
         r = vals.run(flags)
         return r
 
-    modes = {"plain": 0}
+    modes = {"plain": 0, "batched": rin.FLAG_BATCHED}
     results = {}
     for name, flags in modes.items():
         for _ in range(args.warmup):
@@ -119,7 +119,7 @@ def main():
         for name, r in results.items():
             print(name, json.dumps({k: round(v, 4) for k, v in r["stages_ms"].items()}), file=sys.stderr)
     # dominant stage: its algorithmic bytes over its event-measured duration
-    dom = max(stages, key=stages.get)
+    dom = max(stages, key=stages.get) if stages else None
     B = algorithmic_bytes(V, F, sizes)
     achieved_path = B / sps / 1e9
     out = {

@@ -1,0 +1,1013 @@
+// Batched form of the implicit-arrangement hot path.
+//
+//  * Tets are bucketed by active-function count k.  k = 1 (most active tets)
+//    never runs the cut: its complex depends only on the four corner signs,
+//    so an 81-entry table built from the cut itself gives counts, iso
+//    vertices, iso faces and the record; each block stages the table in LDS.
+//  * k = 2, 3..4 and 5..8 run per-bucket cut kernels whose thread-private
+//    state is sized for that bucket, as persistent grids pulling 64-tet
+//    chunks from a work queue; grid sizes come from a small autotuner that
+//    times candidates on the run's own buckets.
+//  * Every stage reads its element count from device memory and checks it
+//    against the workspace capacity, so the whole path is captured once as a
+//    hipGraph and replayed; one small read-back per run returns the sizes and
+//    the overflow flags (on overflow the workspace grows and the graph is
+//    re-captured).
+#include <algorithm>
+#include <cstring>
+#include <functional>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+
+#include "kernels_ia.h"
+
+namespace rin {
+
+namespace {
+
+using CK2 = dev::Caps<16, 24, 6, 6>;
+// CMed (k 3..4) and CBig (k 5..8) come from kernels_ia.h
+
+constexpr uint32_t K1_FLAG = 0x80000000u;
+constexpr uint32_t ERR_OVERFLOW = 8;
+
+// counters in the workspace (device)
+enum Ctr : int {
+    C_ACTIVE = 0, C_FIT, C_VO, C_FO, C_FV, C_VERTS, C_FACES, C_FACEVERTS, C_B1, C_B2, C_B3, C_B5, C_BX, C_DEGEN,
+    C_Q2, C_Q3, C_Q5, C_ABORT, C_N
+};
+
+struct K1Entry {
+    uint8_t niv, nif, nfv, valid;
+    uint8_t vj[4];
+    uint8_t vt[4];  // 0..3 corner, 4 + local edge index
+    uint8_t fl, fb, fn, pad;
+    uint8_t fv[4];
+    uint16_t rec_off, rec_len;
+};
+static_assert(sizeof(K1Entry) == 24, "k1 entry layout");
+
+struct K1Table {
+    K1Entry e[81];
+    std::vector<uint16_t> words;
+};
+
+// one complex per sign pattern, from the cut itself (host build of the
+// device algorithm)
+const K1Table& k1_table() {
+    static K1Table T;
+    static std::once_flag once;
+    std::call_once(once, [] {
+        std::memset(T.e, 0, sizeof(T.e));
+        static dev::Complex<CK2> X;
+        for (int pat = 0; pat < 81; ++pat) {
+            double P[1][4];
+            int np = 0, nn = 0;
+            for (int c = 0, q = pat; c < 4; ++c, q /= 3) {
+                P[0][c] = (double)(q % 3 - 1);
+                np += P[0][c] > 0;
+                nn += P[0][c] < 0;
+            }
+            K1Entry& E = T.e[pat];
+            if (np == 4 || nn == 4) continue;
+            dev::ia_cut<CK2>(P, 1, X);
+            if (X.err) throw std::runtime_error("k1 table: cut failed");
+            uint16_t rec[512];
+            const int len = dev::write_record_ia<CK2>(X, 1, rec, 512);
+            E.valid = 1;
+            E.rec_off = (uint16_t)T.words.size();
+            E.rec_len = (uint16_t)len;
+            T.words.insert(T.words.end(), rec, rec + len);
+            bool iso_v[CK2::MV] = {false};
+            int isof = -1, niso_f = 0;
+            for (int f = 0; f < X.nf; ++f) {
+                bool is = X.fa[f] > 3;
+                if (!is && X.dup) is = X.grp[4] == X.grp[X.fa[f]];
+                if (!is) continue;
+                ++niso_f;
+                isof = f;
+                for (int q = 0; q < X.fn[f]; ++q) iso_v[X.fv[f][q]] = true;
+            }
+            if (niso_f > 1) throw std::runtime_error("k1 table: more than one iso face");
+            int idx[CK2::MV];
+            int n = 0;
+            for (int j = 0; j < X.nv; ++j) {
+                if (!iso_v[j]) continue;
+                if (n >= 4) throw std::runtime_error("k1 table: more than four iso vertices");
+                idx[j] = n;
+                E.vj[n] = (uint8_t)j;
+                int bnd[3], nb = 0;
+                for (int q = 0; q < 3; ++q)
+                    if (X.vl[j][q] <= 3) bnd[nb++] = X.vl[j][q];
+                if (nb == 3) {
+                    int c = 0;
+                    while (c == bnd[0] || c == bnd[1] || c == bnd[2]) ++c;
+                    E.vt[n] = (uint8_t)c;
+                } else if (nb == 2) {
+                    int l[2], m = 0;
+                    for (int c = 0; c < 4; ++c)
+                        if (c != bnd[0] && c != bnd[1]) l[m++] = c;
+                    E.vt[n] = (uint8_t)(4 + dev::local_edge(l[0], l[1]));
+                } else {
+                    throw std::runtime_error("k1 table: unexpected vertex type");
+                }
+                ++n;
+            }
+            E.niv = (uint8_t)n;
+            E.nif = (uint8_t)niso_f;
+            if (isof >= 0) {
+                E.fl = (uint8_t)isof;
+                E.fb = X.fc[isof] == dev::NIL;
+                E.fn = X.fn[isof];
+                for (int q = 0; q < X.fn[isof]; ++q) E.fv[q] = (uint8_t)idx[X.fv[isof][q]];
+                E.nfv = E.fn;
+            }
+        }
+    });
+    return T;
+}
+
+__device__ inline int bucket_of(int k) { return k <= 1 ? 0 : (k == 2 ? 1 : (k <= 4 ? 2 : (k <= 8 ? 3 : 4))); }
+
+// per tet: active count and bucket histogram
+__global__ void k_filter_count(GridView g, const uint64_t* __restrict__ pos, const uint64_t* __restrict__ neg,
+                               uint64_t full, uint32_t* __restrict__ kcnt, uint32_t* __restrict__ isact,
+                               uint32_t* __restrict__ ctr) {
+    __shared__ uint32_t hist[8];
+    if (threadIdx.x < 8) hist[threadIdx.x] = 0;
+    __syncthreads();
+    GRID_STRIDE(t, g.T) {
+        uint32_t v[4];
+        dev::tet_corners(g, t, v);
+        const uint64_t ap = pos[v[0]] & pos[v[1]] & pos[v[2]] & pos[v[3]];
+        const uint64_t an = neg[v[0]] & neg[v[1]] & neg[v[2]] & neg[v[3]];
+        const uint64_t m = full & ~(ap | an);
+        const uint32_t k = (uint32_t)__popcll(m);
+        kcnt[t] = k;
+        isact[t] = k != 0;
+        if (k) atomicAdd(&hist[bucket_of((int)k)], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x < 5 && hist[threadIdx.x]) atomicAdd(&ctr[C_B1 + threadIdx.x], hist[threadIdx.x]);
+}
+
+struct FillArgs {
+    GridView g;
+    const uint64_t *pos, *neg;
+    uint64_t full;
+    const uint32_t *csr_off, *rank;
+    uint8_t* fit;
+    uint32_t *act, *rec_loc, *cnt;
+    uint8_t* act_k;
+    const K1Entry* k1;
+    uint32_t* list[3];      // buckets k=2, 3..4, 5..8
+    uint32_t cap[3];
+    uint32_t rec_base[3];   // word offsets of each bucket's slots
+    uint32_t slot[3];       // words per slot
+    uint32_t* ctr;
+    uint32_t* errs;
+    uint32_t cap_active, cap_fit;
+};
+
+__global__ void k_fill_batched(FillArgs A) {
+    __shared__ K1Entry tab[81];
+    for (int i = threadIdx.x; i < 81; i += blockDim.x) tab[i] = A.k1[i];
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+    const uint64_t T = A.g.T;
+    // whole waves iterate together so the bucket appends can be aggregated
+    for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < T; base += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t t = base + threadIdx.x;
+        uint64_t m = 0;
+        uint32_t v[4] = {0, 0, 0, 0};
+        if (t < T) {
+            dev::tet_corners(A.g, t, v);
+            const uint64_t ap = A.pos[v[0]] & A.pos[v[1]] & A.pos[v[2]] & A.pos[v[3]];
+            const uint64_t an = A.neg[v[0]] & A.neg[v[1]] & A.neg[v[2]] & A.neg[v[3]];
+            m = A.full & ~(ap | an);
+        }
+        const int k = __popcll(m);
+        int b = -1;
+        uint32_t r = 0;
+        if (k) {
+            r = A.rank[t];
+            uint32_t o = A.csr_off[t];
+            if (r >= A.cap_active || o + (uint32_t)k > A.cap_fit) {
+                A.errs[lane] |= ERR_OVERFLOW;
+                A.ctr[C_ABORT] = 1;
+            } else {
+                A.act[r] = (uint32_t)t;
+                A.act_k[r] = (uint8_t)k;
+                for (uint64_t mm = m; mm; mm &= mm - 1) A.fit[o++] = (uint8_t)(__ffsll((unsigned long long)mm) - 1);
+                if (k == 1) {
+                    const uint32_t f = __ffsll((unsigned long long)m) - 1;
+                    int pat = 0, w = 1;
+                    for (int c = 0; c < 4; ++c, w *= 3) {
+                        const int s = ((A.pos[v[c]] >> f) & 1) ? 2 : (((A.neg[v[c]] >> f) & 1) ? 0 : 1);
+                        pat += s * w;
+                    }
+                    const K1Entry& E = tab[pat];
+                    A.cnt[3 * r] = E.niv;
+                    A.cnt[3 * r + 1] = E.nif;
+                    A.cnt[3 * r + 2] = E.nfv;
+                    A.rec_loc[r] = K1_FLAG | (uint32_t)pat;
+                } else {
+                    b = k == 2 ? 0 : (k <= 4 ? 1 : (k <= 8 ? 2 : -1));
+                    if (b < 0) A.errs[lane] |= 1;  // more active functions than the device cut supports
+                }
+            }
+        }
+        // wave-aggregated appends to the bucket lists
+        for (int q = 0; q < 3; ++q) {
+            const uint64_t mask = __ballot(b == q);
+            if (!mask) continue;
+            const int leader = __ffsll((unsigned long long)mask) - 1;
+            uint32_t start = 0;
+            if (lane == leader) start = atomicAdd(&A.ctr[C_Q2 + q], (uint32_t)__popcll(mask));
+            start = __shfl(start, leader, 64);
+            if (b == q) {
+                const uint32_t slot = start + (uint32_t)__popcll(mask & lt);
+                if (slot < A.cap[q]) {
+                    A.list[q][slot] = r;
+                    A.rec_loc[r] = A.rec_base[q] + slot * A.slot[q];
+                } else {
+                    A.errs[lane] |= ERR_OVERFLOW;
+                    A.ctr[C_ABORT] = 1;
+                }
+            }
+        }
+    }
+}
+
+struct CutArgs {
+    GridView g;
+    const double* vals;
+    const uint32_t *act, *csr_off, *list, *rec_loc;
+    const uint8_t *act_k, *fit;
+    const uint32_t* n;   // bucket size (device)
+    const uint32_t* abort;
+    uint32_t* queue;     // work counter
+    uint32_t cap, slot;
+    uint16_t* rec;
+    uint32_t *cnt, *errs;
+};
+
+// persistent per-bucket cut: each wave takes 64 bucket entries at a time
+template <class C>
+__global__ void __launch_bounds__(64) k_cut_bucket(CutArgs A) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t n = *A.abort ? 0u : min(*A.n, A.cap);
+    const uint64_t V = A.g.V;
+    for (;;) {
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(A.queue, 64u);
+        base = __shfl(base, 0, 64);
+        if (base >= n) break;
+        const uint32_t i = base + lane;
+        if (i >= n) continue;
+        const uint32_t r = A.list[i];
+        const int k = A.act_k[r];
+        const uint32_t t = A.act[r];
+        uint32_t v[4];
+        dev::tet_corners(A.g, t, v);
+        const uint32_t s0 = A.csr_off[t];
+        double P[C::MP - 4][4];
+        for (int q = 0; q < k; ++q) {
+            const uint32_t f = A.fit[s0 + q];
+            for (int c = 0; c < 4; ++c) P[q][c] = A.vals[(uint64_t)f * V + v[c]];
+        }
+        dev::Complex<C> X;
+        dev::ia_cut<C>(P, k, X);
+        uint16_t* out = A.rec + A.rec_loc[r];
+        if (X.err) {
+            A.errs[lane] |= 1;
+            out[0] = 0;
+            A.cnt[3 * r] = A.cnt[3 * r + 1] = A.cnt[3 * r + 2] = 0;
+            continue;
+        }
+        if (dev::write_record_ia<C>(X, k, out, (int)A.slot) < 0) A.errs[lane] |= 4;
+        uint32_t a0, a1, a2;
+        ia_counts<C>(X, a0, a1, a2);
+        A.cnt[3 * r] = a0;
+        A.cnt[3 * r + 1] = a1;
+        A.cnt[3 * r + 2] = a2;
+    }
+}
+
+struct ExtractArgs {
+    GridView g;
+    const uint32_t *act, *csr_off, *rec_loc;
+    const uint8_t* fit;
+    const uint16_t* rec;
+    const K1Entry* k1;
+    const uint32_t* n_active;
+    uint32_t cap_active;
+    uint32_t* ctr;
+    const uint32_t *voff, *foff, *fvoff;
+    const uint32_t *tot;  // C_VO, C_FO, C_FV totals live at ctr
+    uint32_t cap_vo, cap_fo, cap_fv;
+    uint64_t* okey;
+    uint32_t* oa;
+    uint16_t* oj;
+    uint32_t* fq_a;
+    uint16_t* fq_l;
+    uint8_t *fq_fn, *fq_bnd;
+    uint32_t *fq_vs, *fve, *errs;
+};
+
+__global__ void k_extract_batched(ExtractArgs A) {
+    __shared__ K1Entry tab[81];
+    for (int i = threadIdx.x; i < 81; i += blockDim.x) tab[i] = A.k1[i];
+    __syncthreads();
+    if (A.ctr[C_ABORT]) return;
+    if (A.tot[C_VO] > A.cap_vo || A.tot[C_FO] > A.cap_fo || A.tot[C_FV] > A.cap_fv) {
+        if (blockIdx.x == 0) {
+            A.errs[threadIdx.x & 63] |= ERR_OVERFLOW;
+            A.ctr[C_ABORT] = 1;
+        }
+        return;
+    }
+    const uint64_t NA = min((uint64_t)*A.n_active, (uint64_t)A.cap_active);
+    const uint8_t le[6][2] = {{0, 1}, {0, 2}, {0, 3}, {1, 2}, {1, 3}, {2, 3}};
+    GRID_STRIDE(a, NA) {
+        const uint32_t loc = A.rec_loc[a];
+        const uint32_t t = A.act[a];
+        uint32_t v[4];
+        dev::tet_corners(A.g, t, v);
+        const uint32_t s0 = A.csr_off[t];
+        if (loc & K1_FLAG) {
+            const K1Entry& E = tab[loc & 0xFF];
+            const uint32_t f = A.fit[s0];
+            uint32_t o = A.voff[a];
+            for (int i = 0; i < E.niv; ++i) {
+                const int vt = E.vt[i];
+                A.okey[o] = vt < 4 ? key_corner(v[vt])
+                                   : key_edge(dev::edge_id(A.g, t, v, le[vt - 4][0], le[vt - 4][1]), f);
+                A.oa[o] = (uint32_t)a;
+                A.oj[o] = E.vj[i];
+                ++o;
+            }
+            if (E.nif) {
+                const uint32_t q = A.foff[a], e = A.fvoff[a], vb = A.voff[a];
+                A.fq_a[q] = (uint32_t)a;
+                A.fq_l[q] = E.fl;
+                A.fq_fn[q] = (uint8_t)f;
+                A.fq_bnd[q] = E.fb;
+                A.fq_vs[q] = e;
+                for (int i = 0; i < E.fn; ++i) A.fve[e + i] = vb + E.fv[i];
+            }
+            continue;
+        }
+        // general record (same layout and rules as the plain extraction)
+        const Rec R(A.rec + loc);
+        if (R.r[0] != 1) continue;
+        uint64_t iso[4] = {0, 0, 0, 0};
+        const uint16_t* F = R.faces();
+        for (int f = 0; f < R.nf; ++f) {
+            const int n = F[3];
+            if (ia_face_is_iso(R, F))
+                for (int q = 0; q < n; ++q) iso[F[4 + q] >> 6] |= 1ull << (F[4 + q] & 63);
+            F += 4 + n;
+        }
+        uint32_t o = A.voff[a];
+        for (int j = 0; j < R.nv; ++j) {
+            if (!((iso[j >> 6] >> (j & 63)) & 1)) continue;
+            const uint16_t* T3 = R.vert(j);
+            int bnd[3], nb = 0;
+            uint32_t fn[3];
+            int ni = 0;
+            for (int q = 0; q < 3; ++q) {
+                if (T3[q] > 3)
+                    fn[ni++] = A.fit[s0 + T3[q] - 4];
+                else
+                    bnd[nb++] = T3[q];
+            }
+            uint64_t key;
+            if (nb == 3) {
+                int c = 0;
+                while (c == bnd[0] || c == bnd[1] || c == bnd[2]) ++c;
+                key = key_corner(v[c]);
+            } else if (nb == 2) {
+                int l[2], nl = 0;
+                for (int c = 0; c < 4; ++c)
+                    if (c != bnd[0] && c != bnd[1]) l[nl++] = c;
+                key = key_edge(dev::edge_id(A.g, t, v, l[0], l[1]), fn[0]);
+            } else if (nb == 1) {
+                key = key_face(dev::face_id(A.g, t, v, bnd[0]), fn[0], fn[1]);
+            } else {
+                key = key_inner(a, j);
+            }
+            A.okey[o] = key;
+            A.oa[o] = (uint32_t)a;
+            A.oj[o] = (uint16_t)j;
+            ++o;
+        }
+        uint32_t q = A.foff[a], e = A.fvoff[a];
+        const uint32_t vb = A.voff[a];
+        F = R.faces();
+        for (int f = 0; f < R.nf; ++f) {
+            const int n = F[3];
+            if (ia_face_is_iso(R, F)) {
+                int sp = F[0];
+                if (sp <= 3) {
+                    const uint16_t* grp = R.groups_ia();
+                    for (int p = 4; p < R.L; ++p)
+                        if (grp[p] == grp[sp]) {
+                            sp = p;
+                            break;
+                        }
+                }
+                A.fq_a[q] = (uint32_t)a;
+                A.fq_l[q] = (uint16_t)f;
+                A.fq_fn[q] = A.fit[s0 + sp - 4];
+                A.fq_bnd[q] = F[2] == 0xFFFF;
+                A.fq_vs[q] = e;
+                for (int i = 0; i < n; ++i) {
+                    const int j = F[4 + i];
+                    uint32_t rank = 0;
+                    for (int w = 0; w < (j >> 6); ++w) rank += __popcll(iso[w]);
+                    rank += __popcll(iso[j >> 6] & ((1ull << (j & 63)) - 1));
+                    A.fve[e++] = vb + rank;
+                }
+                ++q;
+            }
+            F += 4 + n;
+        }
+    }
+}
+
+__global__ void k_face_end(Count nfo, const uint32_t* __restrict__ fq_vs, const uint32_t* __restrict__ nfv,
+                           uint32_t* __restrict__ fq_end) {
+    GRID_STRIDE(q, nfo) fq_end[q] = (q + 1 < q_end) ? fq_vs[q + 1] : *nfv;
+}
+
+// CSR terminators: csr_off[T], face vertex and tet-face offsets at n_faces
+__global__ void k_write_totals(const uint32_t* ctr, uint64_t T, uint32_t* csr_off, uint32_t* f_vstart,
+                               uint32_t* f_tfstart, uint32_t cap_fo) {
+    if (threadIdx.x == 0) {
+        csr_off[T] = ctr[C_FIT];
+        const uint32_t nf = ctr[C_ABORT] ? 0u : min(ctr[C_FACES], cap_fo);
+        f_vstart[nf] = ctr[C_FACEVERTS];
+        f_tfstart[nf] = ctr[C_ABORT] ? 0u : min(ctr[C_FO], cap_fo);
+    }
+}
+
+__global__ void k_zero_ctr(uint32_t* ctr, int n, uint32_t* errs) {
+    if (threadIdx.x < (unsigned)n) ctr[threadIdx.x] = 0;
+    if (threadIdx.x < 64) errs[threadIdx.x] = 0;
+}
+
+// the counters the host reads back after a replay
+__global__ void k_cap_check(const uint32_t* ctr, uint32_t cap_vo, uint32_t cap_fo, uint32_t cap_fv, uint32_t* errs) {
+    const int lane = threadIdx.x & 63;
+    if (lane == 0 && (ctr[C_VO] > cap_vo || ctr[C_FO] > cap_fo || ctr[C_FV] > cap_fv)) errs[lane] |= ERR_OVERFLOW;
+}
+
+int bitsf(uint64_t x) {
+    int b = 0;
+    while (b < 64 && (x >> b)) ++b;
+    return b;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ plan
+
+template <class C>
+constexpr uint32_t rec_words() {
+    return 8 + 4 * C::MV + C::MF * (4 + C::MFV) + 2 * C::MP;
+}
+
+struct Caps_ {
+    uint32_t active = 0, fit = 0, b[3] = {0, 0, 0}, vo = 0, fo = 0, fv = 0;
+};
+
+struct Workspace {
+    DevMem mem;  // every buffer (and every temporary of the captured launches) lives here
+    Caps_ cap;
+    uint32_t rec_base[3] = {0, 0, 0};
+    // words per record slot: the largest record a bucket's capacity allows
+    // (header, vertices, faces with their cycles, coincidence groups)
+    uint32_t slot[3] = {rec_words<CK2>(), rec_words<CMed>(), rec_words<CBig>()};
+    DBuf<uint64_t> pos, neg;
+    DBuf<uint32_t> zeros, kcnt, isact, csr_off, rank, act, rec_loc, cnt, list[3];
+    DBuf<uint8_t> fit, act_k;
+    DBuf<uint16_t> rec;
+    DBuf<uint32_t> cv, cf, cfv;
+    DBuf<uint64_t> okey;
+    DBuf<uint32_t> oa;
+    DBuf<uint16_t> oj;
+    DBuf<uint32_t> fq_a, fq_vs, fq_end, fve;
+    DBuf<uint16_t> fq_l;
+    DBuf<uint8_t> fq_fn, fq_bnd;
+    DBuf<uint32_t> sidx, first, head, hexcl, seg_first, vid, vrank, first_occ;
+    DBuf<uint64_t> khi, klo;
+    DBuf<uint32_t> perm, ffirst, fhead, fhex, fsegfirst, frank, fsegstart, ffirst_occ;
+    // outputs
+    DBuf<uint32_t> v_tet, v_sv, f_vstart, f_verts, f_tfstart, f_tf_tet;
+    DBuf<uint16_t> v_j, f_tf_local;
+    DBuf<uint8_t> v_ss, v_fn, f_func;
+    DBuf<double> xyz;
+    DBuf<uint32_t> ctr, errs;
+    DBuf<K1Entry> k1;
+    DBuf<uint16_t> k1_words;
+    DBuf<uint32_t> k1_off;
+    // the path as a few captured graphs launched back to back; stream events
+    // between them time each part on every replay
+    std::vector<std::pair<std::string, hipGraph_t>> graphs;
+    std::vector<std::pair<std::string, hipGraphExec_t>> parts;
+    std::vector<hipEvent_t> ev;
+    bool exec = false;  // graphs captured for the current capacities
+    bool in_use = false;
+    void drop_graphs() {
+        for (auto& p : parts) (void)hipGraphExecDestroy(p.second);
+        for (auto& g : graphs) (void)hipGraphDestroy(g.second);
+        parts.clear();
+        graphs.clear();
+    }
+    ~Workspace() {
+        drop_graphs();
+        for (auto e : ev) (void)hipEventDestroy(e);
+    }
+};
+
+struct Plan {
+    std::mutex mu;
+    std::vector<std::shared_ptr<Workspace>> ws;
+    int blocks_per_cu[3] = {0, 0, 0};  // autotuned grid of each bucket's cut kernel
+    uint32_t hint_b[5] = {0, 0, 0, 0, 0};
+};
+
+namespace {
+
+void alloc_ws(Workspace& W, const Grid& g, const Caps_& c) {
+    W.cap = c;
+    DevMem& m = W.mem;
+    const uint64_t V = g.view.V, T = g.view.T;
+    W.pos.alloc(m, V);
+    W.neg.alloc(m, V);
+    W.zeros.alloc(m, V + 1);
+    W.kcnt.alloc(m, T + 1);
+    W.isact.alloc(m, T + 1);
+    W.csr_off.alloc(m, T + 1);
+    W.rank.alloc(m, T + 1);
+    W.act.alloc(m, c.active + 1);
+    W.act_k.alloc(m, c.active + 1);
+    W.rec_loc.alloc(m, c.active + 1);
+    W.cnt.alloc(m, 3 * (uint64_t)c.active + 3);
+    W.fit.alloc(m, c.fit + 1);
+    uint64_t words = 0;
+    for (int q = 0; q < 3; ++q) {
+        W.list[q].alloc(m, c.b[q] + 1);
+        W.rec_base[q] = (uint32_t)words;
+        words += (uint64_t)c.b[q] * W.slot[q];
+    }
+    if (words >= (1ull << 31)) throw std::runtime_error("record workspace exceeds capacity");
+    W.rec.alloc(m, words + 1);
+    W.cv.alloc(m, c.active + 1);
+    W.cf.alloc(m, c.active + 1);
+    W.cfv.alloc(m, c.active + 1);
+    W.okey.alloc(m, c.vo + 1);
+    W.oa.alloc(m, c.vo + 1);
+    W.oj.alloc(m, c.vo + 1);
+    for (auto* b : {&W.sidx, &W.first, &W.head, &W.hexcl, &W.seg_first, &W.vid, &W.vrank, &W.first_occ})
+        b->alloc(m, c.vo + 1);
+    W.fq_a.alloc(m, c.fo + 1);
+    W.fq_vs.alloc(m, c.fo + 1);
+    W.fq_end.alloc(m, c.fo + 1);
+    W.fq_l.alloc(m, c.fo + 1);
+    W.fq_fn.alloc(m, c.fo + 1);
+    W.fq_bnd.alloc(m, c.fo + 1);
+    W.fve.alloc(m, c.fv + 1);
+    W.khi.alloc(m, c.fo + 1);
+    W.klo.alloc(m, c.fo + 1);
+    for (auto* b : {&W.perm, &W.ffirst, &W.fhead, &W.fhex, &W.fsegfirst, &W.frank, &W.fsegstart, &W.ffirst_occ})
+        b->alloc(m, c.fo + 1);
+    W.v_tet.alloc(m, c.vo + 1);
+    W.v_j.alloc(m, c.vo + 1);
+    W.v_ss.alloc(m, c.vo + 1);
+    W.v_sv.alloc(m, 4 * (uint64_t)c.vo + 4);
+    W.v_fn.alloc(m, 4 * (uint64_t)c.vo + 4);
+    W.xyz.alloc(m, 3 * (uint64_t)c.vo + 3);
+    W.f_vstart.alloc(m, c.fo + 1);
+    W.f_tfstart.alloc(m, c.fo + 1);
+    W.f_verts.alloc(m, c.fv + 1);
+    W.f_tf_tet.alloc(m, c.fo + 1);
+    W.f_tf_local.alloc(m, c.fo + 1);
+    W.f_func.alloc(m, 2 * (uint64_t)c.fo + 2);
+    W.ctr.alloc(m, 32);
+    W.errs.alloc(m, 64);
+    const K1Table& K = k1_table();
+    W.k1.alloc(m, 81);
+    W.k1_words.alloc(m, K.words.size() + 1);
+    W.k1_off.alloc(m, 81);
+    std::vector<uint32_t> off(81);
+    for (int i = 0; i < 81; ++i) off[i] = K.e[i].rec_off;
+    RIN_HIP(hipMemcpy(W.k1.p, K.e, sizeof(K.e), hipMemcpyHostToDevice));
+    RIN_HIP(hipMemcpy(W.k1_words.p, K.words.data(), 2 * K.words.size(), hipMemcpyHostToDevice));
+    RIN_HIP(hipMemcpy(W.k1_off.p, off.data(), 4 * 81, hipMemcpyHostToDevice));
+}
+
+// stage 1 (also run eagerly to size the workspace): signs, filter, CSR scan
+void stage_count(Workspace& W, const Grid& g, const Values& val, hipStream_t s) {
+    const uint64_t V = g.view.V, T = g.view.T;
+    const uint32_t F = val.F;
+    hipLaunchKernelGGL(k_zero_ctr, dim3(1), dim3(64), 0, s, W.ctr.p, (int)C_N, W.errs.p);
+    hipLaunchKernelGGL(k_signs_ia, dim3(nblk(V)), dim3(256), 0, s, (const double*)val.vals.p, V, F, W.pos.p, W.neg.p,
+                       W.zeros.p);
+    const uint64_t full = (F == 64) ? ~0ull : ((1ull << F) - 1);
+    hipLaunchKernelGGL(k_filter_count, dim3(nblk(T)), dim3(256), 0, s, g.view, (const uint64_t*)W.pos.p,
+                       (const uint64_t*)W.neg.p, full, W.kcnt.p, W.isact.p, W.ctr.p);
+    scan_u32(W.mem, W.zeros.p, W.zeros.p, Count(V), W.ctr.p + C_DEGEN, s);
+    scan_u32(W.mem, W.kcnt.p, W.csr_off.p, Count(T), W.ctr.p + C_FIT, s);
+    scan_u32(W.mem, W.isact.p, W.rank.p, Count(T), W.ctr.p + C_ACTIVE, s);
+}
+
+template <class C>
+void launch_cut(Workspace& W, const Grid& g, const Values& val, int q, int grid, hipStream_t s) {
+    CutArgs A;
+    A.g = g.view;
+    A.vals = val.vals.p;
+    A.act = W.act.p;
+    A.csr_off = W.csr_off.p;
+    A.list = W.list[q].p;
+    A.rec_loc = W.rec_loc.p;
+    A.act_k = W.act_k.p;
+    A.fit = W.fit.p;
+    A.n = W.ctr.p + C_Q2 + q;
+    A.abort = W.ctr.p + C_ABORT;
+    A.queue = W.ctr.p + 24 + q;
+    A.cap = W.cap.b[q];
+    A.slot = W.slot[q];
+    A.rec = W.rec.p;
+    A.cnt = W.cnt.p;
+    A.errs = W.errs.p;
+    fill_u32(W.ctr.p + 24 + q, 0, 1, s);
+    hipLaunchKernelGGL(k_cut_bucket<C>, dim3(grid), dim3(64), 0, s, A);
+}
+
+void stage_cut(Workspace& W, const Grid& g, const Values& val, const int* grid, hipStream_t s) {
+    if (W.cap.b[0]) launch_cut<CK2>(W, g, val, 0, grid[0], s);
+    if (W.cap.b[1]) launch_cut<CMed>(W, g, val, 1, grid[1], s);
+    if (W.cap.b[2]) launch_cut<CBig>(W, g, val, 2, grid[2], s);
+}
+
+void stage_fill(Workspace& W, const Grid& g, const Values& val, hipStream_t s) {
+    const uint64_t T = g.view.T;
+    const uint32_t F = val.F;
+    FillArgs A;
+    A.g = g.view;
+    A.pos = W.pos.p;
+    A.neg = W.neg.p;
+    A.full = (F == 64) ? ~0ull : ((1ull << F) - 1);
+    A.csr_off = W.csr_off.p;
+    A.rank = W.rank.p;
+    A.fit = W.fit.p;
+    A.act = W.act.p;
+    A.rec_loc = W.rec_loc.p;
+    A.cnt = W.cnt.p;
+    A.act_k = W.act_k.p;
+    A.k1 = W.k1.p;
+    for (int q = 0; q < 3; ++q) {
+        A.list[q] = W.list[q].p;
+        A.cap[q] = W.cap.b[q];
+        A.rec_base[q] = W.rec_base[q];
+        A.slot[q] = W.slot[q];
+    }
+    A.ctr = W.ctr.p;
+    A.errs = W.errs.p;
+    A.cap_active = W.cap.active;
+    A.cap_fit = W.cap.fit;
+    hipLaunchKernelGGL(k_fill_batched, dim3(nblk(T)), dim3(256), 0, s, A);
+}
+
+// everything after the cut
+void stage_rest(Workspace& W, const Grid& g, const Values& val, hipStream_t s) {
+    DevMem& m = W.mem;
+    const Caps_& c = W.cap;
+    uint32_t* ctr = W.ctr.p;
+    const uint32_t* abort = ctr + C_ABORT;
+    const Count NA(ctr + C_ACTIVE, c.active, abort);
+    hipLaunchKernelGGL(k_split3, dim3(nblk(c.active)), dim3(256), 0, s, NA, (const uint32_t*)W.cnt.p, W.cv.p, W.cf.p,
+                       W.cfv.p);
+    scan_u32(m, W.cv.p, W.cv.p, NA, ctr + C_VO, s);
+    scan_u32(m, W.cf.p, W.cf.p, NA, ctr + C_FO, s);
+    scan_u32(m, W.cfv.p, W.cfv.p, NA, ctr + C_FV, s);
+    ExtractArgs E;
+    E.g = g.view;
+    E.act = W.act.p;
+    E.csr_off = W.csr_off.p;
+    E.rec_loc = W.rec_loc.p;
+    E.fit = W.fit.p;
+    E.rec = W.rec.p;
+    E.k1 = W.k1.p;
+    E.n_active = ctr + C_ACTIVE;
+    E.cap_active = c.active;
+    E.ctr = ctr;
+    E.voff = W.cv.p;
+    E.foff = W.cf.p;
+    E.fvoff = W.cfv.p;
+    E.tot = ctr;
+    E.cap_vo = c.vo;
+    E.cap_fo = c.fo;
+    E.cap_fv = c.fv;
+    E.okey = W.okey.p;
+    E.oa = W.oa.p;
+    E.oj = W.oj.p;
+    E.fq_a = W.fq_a.p;
+    E.fq_l = W.fq_l.p;
+    E.fq_fn = W.fq_fn.p;
+    E.fq_bnd = W.fq_bnd.p;
+    E.fq_vs = W.fq_vs.p;
+    E.fve = W.fve.p;
+    E.errs = W.errs.p;
+    hipLaunchKernelGGL(k_extract_batched, dim3(nblk(c.active)), dim3(256), 0, s, E);
+
+    const Count NVO(ctr + C_VO, c.vo, abort), NFO(ctr + C_FO, c.fo, abort), NV(ctr + C_VERTS, c.vo, abort),
+        NF(ctr + C_FACES, c.fo, abort);
+    const int kbits = 2 + std::max({bitsf(g.view.V), bitsf(g.view.n_edge_ids) + 6, bitsf(g.view.n_face_ids) + 12,
+                                    bitsf(c.active) + 8});
+    iota_u32(W.sidx.p, NVO, s);
+    sort_pairs(m, W.okey.p, W.sidx.p, NVO, kbits, s);
+    hipLaunchKernelGGL(k_heads, dim3(nblk(c.vo)), dim3(256), 0, s, NVO, (const uint64_t*)W.okey.p,
+                       (const uint32_t*)W.sidx.p, W.first.p, W.head.p);
+    scan_u32(m, W.head.p, W.hexcl.p, NVO, nullptr, s);
+    hipLaunchKernelGGL(k_assign, dim3(nblk(c.vo)), dim3(256), 0, s, NVO, (const uint32_t*)W.sidx.p,
+                       (const uint32_t*)W.head.p, (const uint32_t*)W.hexcl.p, W.seg_first.p);
+    scan_u32(m, W.first.p, W.vrank.p, NVO, ctr + C_VERTS, s);
+    hipLaunchKernelGGL(k_assign2, dim3(nblk(c.vo)), dim3(256), 0, s, NVO, (const uint32_t*)W.sidx.p,
+                       (const uint32_t*)W.head.p, (const uint32_t*)W.hexcl.p, (const uint32_t*)W.seg_first.p,
+                       (const uint32_t*)W.vrank.p, W.vid.p);
+    hipLaunchKernelGGL(k_first_list, dim3(nblk(c.vo)), dim3(256), 0, s, NVO, (const uint32_t*)W.first.p,
+                       (const uint32_t*)W.vrank.p, W.first_occ.p);
+    hipLaunchKernelGGL(k_vert_fields_ia, dim3(nblk(c.vo)), dim3(256), 0, s, g.view, (const double*)val.vals.p,
+                       (const uint32_t*)W.act.p, (const uint32_t*)W.csr_off.p, (const uint8_t*)W.fit.p,
+                       (const uint32_t*)W.rec_loc.p, (const uint16_t*)W.rec.p, (const uint16_t*)W.k1_words.p,
+                       (const uint32_t*)W.k1_off.p, (const uint32_t*)W.first_occ.p,
+                       (const uint32_t*)W.oa.p, (const uint16_t*)W.oj.p, NV, W.v_tet.p, W.v_j.p, W.v_ss.p, W.v_sv.p,
+                       W.v_fn.p, W.xyz.p);
+    // faces
+    hipLaunchKernelGGL(k_face_end, dim3(nblk(c.fo)), dim3(256), 0, s, NFO, (const uint32_t*)W.fq_vs.p,
+                       (const uint32_t*)(ctr + C_FV), W.fq_end.p);
+    const int vb = bitsf((uint64_t)c.vo + 1);
+    hipLaunchKernelGGL(k_face_keys, dim3(nblk(c.fo)), dim3(256), 0, s, NFO, (const uint32_t*)W.fq_vs.p,
+                       (const uint32_t*)W.fq_end.p, W.fve.p, (const uint32_t*)W.vid.p, (const uint8_t*)W.fq_bnd.p, vb,
+                       W.khi.p, W.klo.p);
+    sort_index_by2(m, W.khi.p, W.klo.p, 2 * vb + 1, vb, W.perm.p, NFO, s);
+    hipLaunchKernelGGL(k_face_heads, dim3(nblk(c.fo)), dim3(256), 0, s, NFO, (const uint32_t*)W.perm.p,
+                       (const uint64_t*)W.khi.p, (const uint64_t*)W.klo.p, (const uint8_t*)W.fq_bnd.p, W.ffirst.p,
+                       W.fhead.p);
+    scan_u32(m, W.fhead.p, W.fhex.p, NFO, nullptr, s);
+    hipLaunchKernelGGL(k_assign, dim3(nblk(c.fo)), dim3(256), 0, s, NFO, (const uint32_t*)W.perm.p,
+                       (const uint32_t*)W.fhead.p, (const uint32_t*)W.fhex.p, W.fsegfirst.p);
+    hipLaunchKernelGGL(k_seg_start, dim3(nblk(c.fo)), dim3(256), 0, s, NFO, (const uint32_t*)W.fhead.p,
+                       (const uint32_t*)W.fhex.p, W.fsegstart.p);
+    scan_u32(m, W.ffirst.p, W.frank.p, NFO, ctr + C_FACES, s);
+    hipLaunchKernelGGL(k_first_list, dim3(nblk(c.fo)), dim3(256), 0, s, NFO, (const uint32_t*)W.ffirst.p,
+                       (const uint32_t*)W.frank.p, W.ffirst_occ.p);
+    fill_u32(W.f_tfstart.p, 0, c.fo + 1, s);
+    hipLaunchKernelGGL(k_face_counts, dim3(nblk(c.fo)), dim3(256), 0, s, NFO, (const uint32_t*)W.perm.p,
+                       (const uint32_t*)W.fhead.p, (const uint32_t*)W.fhex.p, (const uint32_t*)W.fsegfirst.p,
+                       (const uint32_t*)W.frank.p, W.f_tfstart.p);
+    scan_u32(m, W.f_tfstart.p, W.f_tfstart.p, NF, nullptr, s);
+    hipLaunchKernelGGL(k_face_tf, dim3(nblk(c.fo)), dim3(256), 0, s, NFO, (const uint32_t*)W.perm.p,
+                       (const uint32_t*)W.fhead.p, (const uint32_t*)W.fhex.p, (const uint32_t*)W.fsegfirst.p,
+                       (const uint32_t*)W.fsegstart.p, (const uint32_t*)W.frank.p, (const uint32_t*)W.f_tfstart.p,
+                       (const uint32_t*)W.act.p, (const uint32_t*)W.fq_a.p, (const uint16_t*)W.fq_l.p, W.f_tf_tet.p,
+                       W.f_tf_local.p);
+    hipLaunchKernelGGL(k_face_vcount, dim3(nblk(c.fo)), dim3(256), 0, s, NF, (const uint32_t*)W.ffirst_occ.p,
+                       (const uint32_t*)W.fq_vs.p, (const uint32_t*)W.fq_end.p, W.f_vstart.p);
+    scan_u32(m, W.f_vstart.p, W.f_vstart.p, NF, ctr + C_FACEVERTS, s);
+    hipLaunchKernelGGL(k_face_out, dim3(nblk(c.fo)), dim3(256), 0, s, NF, (const uint32_t*)W.ffirst_occ.p,
+                       (const uint32_t*)W.fq_vs.p, (const uint32_t*)W.fq_end.p, (const uint32_t*)W.fve.p,
+                       (const uint8_t*)W.fq_fn.p, (const uint32_t*)W.f_vstart.p, W.f_verts.p, W.f_func.p);
+    hipLaunchKernelGGL(k_write_totals, dim3(1), dim3(64), 0, s, (const uint32_t*)ctr, g.view.T, W.csr_off.p,
+                       W.f_vstart.p, W.f_tfstart.p, c.fo);
+}
+
+// conservative per-tet output bounds by active count (maxima of the cut's
+// vertex / face counts, see the capacities of the bucket kernels)
+void caps_from_counts(const uint32_t* ctr, Caps_& c) {
+    const uint64_t b1 = ctr[C_B1], b2 = ctr[C_B2], b3 = ctr[C_B3], b5 = ctr[C_B5];
+    auto clamp = [](uint64_t x) -> uint32_t {
+        if (x >= (1ull << 31)) throw std::runtime_error("workspace capacity exceeds 2^31 entries");
+        return (uint32_t)x;
+    };
+    c.active = clamp(ctr[C_ACTIVE] + 64);
+    c.fit = clamp(ctr[C_FIT] + 64);
+    c.b[0] = clamp(b2 + 64);
+    c.b[1] = clamp(b3 + 64);
+    c.b[2] = clamp(b5 + 64);
+    c.vo = clamp(4 * b1 + 16 * b2 + 40 * b3 + 132 * b5 + 64);
+    c.fo = clamp(1 * b1 + 24 * b2 + 64 * b3 + 284 * b5 + 64);
+    c.fv = clamp(4 * b1 + 24 * 6 * b2 + 64 * 8 * b3 + 284 * 10 * b5 + 64);
+}
+
+bool caps_ok(const Caps_& have, const Caps_& need) {
+    return have.active >= need.active && have.fit >= need.fit && have.b[0] >= need.b[0] && have.b[1] >= need.b[1] &&
+           have.b[2] >= need.b[2] && have.vo >= need.vo && have.fo >= need.fo && have.fv >= need.fv;
+}
+
+float time_ms(hipStream_t s, const std::function<void()>& f) {
+    hipEvent_t a, b;
+    RIN_HIP(hipEventCreate(&a));
+    RIN_HIP(hipEventCreate(&b));
+    RIN_HIP(hipEventRecord(a, s));
+    f();
+    RIN_HIP(hipEventRecord(b, s));
+    RIN_HIP(hipEventSynchronize(b));
+    float ms = 0;
+    RIN_HIP(hipEventElapsedTime(&ms, a, b));
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return ms;
+}
+
+}  // namespace
+
+void run_batched(DevMem& ctx_mem, const Grid& g, Values& val, uint32_t flags, RunOut& out, hipStream_t s, int n_cu) {
+    (void)ctx_mem;
+    if (flags & (RUN_MI | RUN_CUT_ONLY | RUN_REVERSE))
+        throw std::runtime_error("batched form is not available for this mode (use the plain form)");
+    const uint64_t V = g.view.V, T = g.view.T;
+    if (val.V != V) throw std::invalid_argument("values do not match the grid");
+    auto& planp = val.plan[0];
+    if (!planp) planp = std::make_shared<Plan>();
+    Plan& P = *planp;
+    std::lock_guard<std::mutex> lock(P.mu);
+    std::shared_ptr<Workspace> W;
+    for (auto& w : P.ws)
+        if (!w->in_use) {
+            W = w;
+            break;
+        }
+    if (!W) {
+        W = std::make_shared<Workspace>();
+        P.ws.push_back(W);
+    }
+    Workspace& w = *W;
+
+    uint32_t hctr[32];
+    uint32_t herr[64];
+    auto read_back = [&] {
+        RIN_HIP(hipMemcpyAsync(hctr, w.ctr.p, sizeof(hctr), hipMemcpyDeviceToHost, s));
+        RIN_HIP(hipMemcpyAsync(herr, w.errs.p, sizeof(herr), hipMemcpyDeviceToHost, s));
+        RIN_HIP(hipStreamSynchronize(s));
+    };
+    auto errors = [&] {
+        uint32_t e = 0;
+        for (uint32_t x : herr) e |= x;
+        return e;
+    };
+    bool ran = false;
+    for (int attempt = 0; attempt < 3 && !ran; ++attempt) {
+        if (!w.exec) {
+            // size the workspace from an eager counting pass
+            if (!w.ctr.p) {
+                Caps_ tiny;
+                tiny.active = tiny.fit = 64;
+                for (int q = 0; q < 3; ++q) tiny.b[q] = 64;
+                tiny.vo = tiny.fo = tiny.fv = 64;
+                alloc_ws(w, g, tiny);
+            }
+            stage_count(w, g, val, s);
+            read_back();
+            if (hctr[C_BX]) throw std::runtime_error("a tet has more than 8 active functions; the device cut supports at most 8");
+            Caps_ need;
+            caps_from_counts(hctr, need);
+            if (!caps_ok(w.cap, need)) {
+                // grow with head-room, keeping the k=1 table
+                Caps_ grow = need;
+                grow.active = need.active + need.active / 8;
+                grow.fit = need.fit + need.fit / 8;
+                for (int q = 0; q < 3; ++q) grow.b[q] = need.b[q] + need.b[q] / 8;
+                grow.vo = need.vo + need.vo / 8;
+                grow.fo = need.fo + need.fo / 8;
+                grow.fv = need.fv + need.fv / 8;
+                alloc_ws(w, g, grow);
+                stage_count(w, g, val, s);
+            }
+            stage_fill(w, g, val, s);
+            // autotune the bucket cut grids once per plan
+            if (P.blocks_per_cu[0] == 0) {
+                RIN_HIP(hipStreamSynchronize(s));
+                const int cand[5] = {1, 2, 4, 8, 16};
+                for (int q = 0; q < 3; ++q) {
+                    float best = 1e30f;
+                    int bestc = 4;
+                    for (int ci = 0; ci < 5; ++ci) {
+                        const int grid = std::max(1, n_cu * cand[ci]);
+                        float ms = time_ms(s, [&] {
+                            if (q == 0 && w.cap.b[0]) launch_cut<CK2>(w, g, val, 0, grid, s);
+                            if (q == 1 && w.cap.b[1]) launch_cut<CMed>(w, g, val, 1, grid, s);
+                            if (q == 2 && w.cap.b[2]) launch_cut<CBig>(w, g, val, 2, grid, s);
+                        });
+                        if (ms < best) {
+                            best = ms;
+                            bestc = cand[ci];
+                        }
+                    }
+                    P.blocks_per_cu[q] = bestc;
+                }
+            }
+            int grid[3];
+            for (int q = 0; q < 3; ++q) grid[q] = std::max(1, n_cu * P.blocks_per_cu[q]);
+            // capture the whole path
+            RIN_HIP(hipStreamSynchronize(s));
+            auto capture = [&](const char* label, const std::function<void()>& body) {
+                hipGraph_t gr = nullptr;
+                hipGraphExec_t ex = nullptr;
+                RIN_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+                body();
+                RIN_HIP(hipStreamEndCapture(s, &gr));
+                RIN_HIP(hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0));
+                w.graphs.push_back({label, gr});
+                w.parts.push_back({label, ex});
+            };
+            capture("signs + filter + CSR scans", [&] { stage_count(w, g, val, s); });
+            capture("CSR fill + buckets + k=1 table", [&] { stage_fill(w, g, val, s); });
+            if (w.cap.b[0]) capture("cut k=2", [&] { launch_cut<CK2>(w, g, val, 0, grid[0], s); });
+            if (w.cap.b[1]) capture("cut k=3..4", [&] { launch_cut<CMed>(w, g, val, 1, grid[1], s); });
+            if (w.cap.b[2]) capture("cut k=5..8", [&] { launch_cut<CBig>(w, g, val, 2, grid[2], s); });
+            capture("extract + dedup + xyz + faces", [&] { stage_rest(w, g, val, s); });
+            while (w.ev.size() < w.parts.size() + 1) {
+                hipEvent_t e;
+                RIN_HIP(hipEventCreate(&e));
+                w.ev.push_back(e);
+            }
+            w.exec = true;
+        }
+        RIN_HIP(hipEventRecord(w.ev[0], s));
+        for (size_t i = 0; i < w.parts.size(); ++i) {
+            RIN_HIP(hipGraphLaunch(w.parts[i].second, s));
+            RIN_HIP(hipEventRecord(w.ev[i + 1], s));
+        }
+        read_back();
+        const uint32_t e = errors();
+        if (e & ERR_OVERFLOW) {
+            w.drop_graphs();
+            w.exec = false;
+            continue;
+        }
+        if (e) {
+            out.err = e;
+            throw std::runtime_error("device cut failed: complex exceeds capacity or predicate range");
+        }
+        ran = true;
+    }
+    if (!ran) throw std::runtime_error("batched workspace did not converge");
+    w.in_use = true;
+    for (size_t i = 0; i < w.parts.size(); ++i) {
+        float ms = 0;
+        RIN_HIP(hipEventElapsedTime(&ms, w.ev[i], w.ev[i + 1]));
+        out.times.ms.push_back({w.parts[i].first, ms});
+    }
+    // result views into the workspace
+    out.mi = false;
+    out.flags = flags;
+    out.T = T;
+    out.F = val.F;
+    out.num_degenerate = hctr[C_DEGEN];
+    out.n_active = out.num_intersecting = hctr[C_ACTIVE];
+    out.n_fit = hctr[C_FIT];
+    out.nb[0] = hctr[C_B1];
+    out.nb[1] = hctr[C_B2];
+    out.nb[2] = hctr[C_B3] + hctr[C_B5];
+    out.n_verts = hctr[C_VERTS];
+    out.n_faces = hctr[C_FACES];
+    out.n_face_verts = hctr[C_FACEVERTS];
+    out.n_tet_faces = hctr[C_FO];
+    auto& vw = out.view;
+    vw.fit = w.fit.p;
+    vw.csr_off = w.csr_off.p;
+    vw.act = w.act.p;
+    vw.rank_of_tet = w.rank.p;
+    vw.rec = w.rec.p;
+    vw.rec_loc = w.rec_loc.p;
+    vw.k1_rec = w.k1_words.p;
+    vw.k1_rec_off = w.k1_off.p;
+    vw.v_tet = w.v_tet.p;
+    vw.v_j = w.v_j.p;
+    vw.v_ss = w.v_ss.p;
+    vw.v_sv = w.v_sv.p;
+    vw.v_fn = w.v_fn.p;
+    vw.xyz = w.xyz.p;
+    vw.f_vstart = w.f_vstart.p;
+    vw.f_verts = w.f_verts.p;
+    vw.f_tfstart = w.f_tfstart.p;
+    vw.f_tf_tet = w.f_tf_tet.p;
+    vw.f_tf_local = w.f_tf_local.p;
+    vw.f_func = w.f_func.p;
+    std::weak_ptr<Plan> wp = planp;
+    Workspace* raw = W.get();
+    out.hold = std::shared_ptr<void>(nullptr, [W, wp, raw](void*) {
+        if (auto p = wp.lock()) {
+            std::lock_guard<std::mutex> l(p->mu);
+            raw->in_use = false;
+        }
+    });
+}
+
+}  // namespace rin

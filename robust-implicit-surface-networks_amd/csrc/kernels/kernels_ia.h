@@ -1,0 +1,655 @@
+// Kernels of the device hot path shared by the plain and batched drivers.
+// Included once per translation unit (anonymous namespace).
+#pragma once
+#include "cut_dev.h"
+#include "pipeline.h"
+#include "scan_sort.h"
+
+namespace rin {
+namespace {
+
+using dev::Complex;
+using CMed = dev::Caps<40, 64, 8, 8>;      // up to 4 active functions
+using CBig = dev::Caps<132, 284, 10, 12>;  // up to 8 active functions
+
+constexpr uint32_t ERR_CAPACITY = 1, ERR_RECORD = 4;
+
+inline unsigned nblk(uint64_t n, unsigned t = 256) {
+    uint64_t g = (n + t - 1) / t;
+    if (g > 1u << 20) g = 1u << 20;
+    return (unsigned)(g ? g : 1);
+}
+
+__device__ inline uint64_t count_of(uint64_t n) { return n; }
+__device__ inline uint64_t count_of(const Count& n) { return n.get(); }
+#define GRID_STRIDE(i, n) \
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x, i##_end = count_of(n); i < i##_end; \
+         i += (uint64_t)gridDim.x * blockDim.x)
+
+__device__ inline void flag_error(uint32_t* errs, uint32_t bit) { errs[threadIdx.x & 63] |= bit; }
+
+// ------------------------------------------------------------------ setup
+
+__global__ void k_grid_pts(GridView g, double* pts) {
+    const uint64_t N = (uint64_t)g.res + 1;
+    GRID_STRIDE(v, g.V) {
+        const uint64_t i = v / (N * N), j = (v / N) % N, k = v % N;
+        const double d = double(N - 1);
+        pts[3 * v + 0] = (double(i) / d) * (g.bmax[0] - g.bmin[0]) + g.bmin[0];
+        pts[3 * v + 1] = (double(j) / d) * (g.bmax[1] - g.bmin[1]) + g.bmin[1];
+        pts[3 * v + 2] = (double(k) / d) * (g.bmax[2] - g.bmin[2]) + g.bmin[2];
+    }
+}
+
+__global__ void k_eval(const double* __restrict__ pts, uint64_t V, const FuncDesc* __restrict__ fd, uint32_t F,
+                       double* __restrict__ vals) {
+    GRID_STRIDE(v, V) {
+        const double x0 = pts[3 * v], x1 = pts[3 * v + 1], x2 = pts[3 * v + 2];
+        for (uint32_t f = 0; f < F; ++f) vals[(uint64_t)f * V + v] = dev::eval_func(fd[f], x0, x1, x2);
+    }
+}
+
+__global__ void k_transpose_in(const double* __restrict__ rm, uint64_t V, uint32_t F, double* __restrict__ vals) {
+    GRID_STRIDE(i, V * F) {
+        const uint64_t v = i / F, f = i % F;
+        vals[f * V + v] = rm[i];
+    }
+}
+
+// ------------------------------------------------------------------ signs / filter
+
+__global__ void k_signs_ia(const double* __restrict__ vals, uint64_t V, uint32_t F, uint64_t* __restrict__ pos,
+                           uint64_t* __restrict__ neg, uint32_t* __restrict__ zeros) {
+    GRID_STRIDE(v, V) {
+        uint64_t p = 0, n = 0;
+        uint32_t z = 0;
+        for (uint32_t f = 0; f < F; ++f) {
+            const double x = vals[(uint64_t)f * V + v];
+            if (x > 0)
+                p |= 1ull << f;
+            else if (x < 0)
+                n |= 1ull << f;
+            else
+                ++z;
+        }
+        pos[v] = p;
+        neg[v] = n;
+        zeros[v] = z;
+    }
+}
+
+__global__ void k_filter_ia(GridView g, const uint64_t* __restrict__ pos, const uint64_t* __restrict__ neg,
+                            uint64_t full, uint64_t* __restrict__ amask, uint32_t* __restrict__ kcnt,
+                            uint32_t* __restrict__ isact) {
+    GRID_STRIDE(t, g.T) {
+        uint32_t v[4];
+        dev::tet_corners(g, t, v);
+        const uint64_t ap = pos[v[0]] & pos[v[1]] & pos[v[2]] & pos[v[3]];
+        const uint64_t an = neg[v[0]] & neg[v[1]] & neg[v[2]] & neg[v[3]];
+        const uint64_t m = full & ~(ap | an);
+        amask[t] = m;
+        kcnt[t] = (uint32_t)__popcll(m);
+        isact[t] = m != 0;
+    }
+}
+
+__global__ void k_highest_mi(const double* __restrict__ vals, uint64_t V, uint32_t F, uint64_t* __restrict__ hm,
+                             uint32_t* __restrict__ degen) {
+    GRID_STRIDE(v, V) {
+        double mx = vals[v];
+        uint32_t id = 0, cnt = 1;
+        for (uint32_t j = 1; j < F; ++j) {
+            const double x = vals[(uint64_t)j * V + v];
+            if (x > mx) {
+                mx = x;
+                id = j;
+                cnt = 1;
+            } else if (x == mx) {
+                ++cnt;
+            }
+        }
+        uint64_t m = 1ull << id;
+        if (cnt > 1) {
+            m = 0;
+            for (uint32_t j = 0; j < F; ++j)
+                if (vals[(uint64_t)j * V + v] == mx) m |= 1ull << j;
+        }
+        hm[v] = m;
+        degen[v] = cnt > 1;
+    }
+}
+
+__global__ void k_filter_mi(GridView g, const double* __restrict__ vals, uint32_t F, const uint64_t* __restrict__ hm,
+                            uint64_t* __restrict__ amask, uint32_t* __restrict__ kcnt, uint32_t* __restrict__ isact) {
+    const uint64_t V = g.V;
+    GRID_STRIDE(t, g.T) {
+        uint32_t v[4];
+        dev::tet_corners(g, t, v);
+        uint64_t u = hm[v[0]] | hm[v[1]] | hm[v[2]] | hm[v[3]];
+        if (__popcll(u) < 2) {
+            amask[t] = 0;
+            kcnt[t] = 0;
+            isact[t] = 0;
+            continue;
+        }
+        double mn[4] = {1.79769313486231570815e308, 1.79769313486231570815e308, 1.79769313486231570815e308,
+                        1.79769313486231570815e308};
+        for (uint64_t m = u; m; m &= m - 1) {
+            const uint32_t j = __ffsll((unsigned long long)m) - 1;
+            for (int c = 0; c < 4; ++c) {
+                const double x = vals[(uint64_t)j * V + v[c]];
+                if (x < mn[c]) mn[c] = x;
+            }
+        }
+        for (uint32_t j = 0; j < F; ++j) {
+            int gc = 0;
+            for (int c = 0; c < 4; ++c) gc += vals[(uint64_t)j * V + v[c]] > mn[c];
+            if (gc > 1) u |= 1ull << j;
+        }
+        amask[t] = u;
+        kcnt[t] = (uint32_t)__popcll(u);
+        isact[t] = 1;
+    }
+}
+
+__global__ void k_fill(uint64_t T, const uint64_t* __restrict__ amask, const uint32_t* __restrict__ csr_off,
+                       const uint32_t* __restrict__ rank, uint8_t* __restrict__ fit, uint32_t* __restrict__ act,
+                       uint8_t* __restrict__ act_k) {
+    GRID_STRIDE(t, T) {
+        uint64_t m = amask[t];
+        if (!m) continue;
+        uint32_t o = csr_off[t];
+        const uint32_t r = rank[t];
+        act[r] = (uint32_t)t;
+        act_k[r] = (uint8_t)__popcll(m);
+        for (; m; m &= m - 1) fit[o++] = (uint8_t)(__ffsll((unsigned long long)m) - 1);
+    }
+}
+
+// record slot bound per active count
+__device__ inline uint32_t rec_bound(int mi, int k) {
+    const uint32_t ia[11] = {0, 128, 256, 448, 768, 1280, 1920, 2560, 3200, 4096, 5120};
+    const uint32_t m[11] = {0, 64, 128, 256, 320, 400, 480, 560, 640, 720, 800};
+    return (k < 0 || k > 10) ? 6000u : (mi ? m[k] : ia[k]);
+}
+
+__global__ void k_rec_bound(Count n, const uint8_t* __restrict__ act_k, int mi, uint32_t* __restrict__ b) {
+    GRID_STRIDE(a, n) b[a] = rec_bound(mi, act_k[a]);
+}
+
+// ------------------------------------------------------------------ per-tet cut
+
+template <class C>
+__device__ inline void ia_counts(const Complex<C>& X, uint32_t& niv, uint32_t& nif, uint32_t& nfv) {
+    uint64_t iso[4] = {0, 0, 0, 0};
+    nif = nfv = 0;
+    for (int f = 0; f < X.nf; ++f) {
+        bool is = X.fa[f] > 3;
+        if (!is && X.dup) {
+            const uint8_t rep = X.grp[X.fa[f]];
+            for (int q = 4; q < C::MP; ++q) is |= (X.grp[q] == rep);
+        }
+        if (!is) continue;
+        ++nif;
+        nfv += X.fn[f];
+        for (int q = 0; q < X.fn[f]; ++q) iso[X.fv[f][q] >> 6] |= 1ull << (X.fv[f][q] & 63);
+    }
+    niv = (uint32_t)(__popcll(iso[0]) + __popcll(iso[1]) + __popcll(iso[2]) + __popcll(iso[3]));
+}
+
+template <class C>
+__device__ inline void mi_counts(const Complex<C>& X, uint32_t& niv, uint32_t& nif, uint32_t& nfv) {
+    // MI faces: positive label > 3; boundary faces are resolved across tets
+    uint64_t iso[4] = {0, 0, 0, 0};
+    nif = nfv = 0;
+    for (int f = 0; f < X.nf; ++f) {
+        if (X.fa[f] <= 3) continue;
+        ++nif;
+        nfv += X.fn[f];
+        for (int q = 0; q < X.fn[f]; ++q) iso[X.fv[f][q] >> 6] |= 1ull << (X.fv[f][q] & 63);
+    }
+    niv = (uint32_t)(__popcll(iso[0]) + __popcll(iso[1]) + __popcll(iso[2]) + __popcll(iso[3]));
+}
+
+template <class C, bool MI>
+__global__ void __launch_bounds__(64) k_cut(GridView g, const double* __restrict__ vals, const uint32_t* __restrict__ act,
+                                            const uint8_t* __restrict__ act_k, const uint32_t* __restrict__ csr_off,
+                                            const uint8_t* __restrict__ fit, uint64_t n_active, int kmin, int kmax,
+                                            uint32_t flags, const uint32_t* __restrict__ rec_off,
+                                            uint16_t* __restrict__ rec, uint32_t* __restrict__ cnt,
+                                            uint32_t* __restrict__ errs) {
+    const uint64_t V = g.V;
+    GRID_STRIDE(a, n_active) {
+        const int k = act_k[a];
+        if (k < kmin || k > kmax) continue;
+        const uint32_t t = act[a];
+        uint32_t v[4];
+        dev::tet_corners(g, t, v);
+        const uint32_t s0 = csr_off[t];
+        double P[C::MP - 4][4];
+        for (int q = 0; q < k; ++q) {
+            const uint32_t f = fit[s0 + ((flags & RUN_REVERSE) ? k - 1 - q : q)];
+            for (int c = 0; c < 4; ++c) P[q][c] = vals[(uint64_t)f * V + v[c]];
+        }
+        Complex<C> X;
+        if (MI)
+            dev::mi_cut<C>(P, k, X);
+        else
+            dev::ia_cut<C>(P, k, X);
+        if (X.err) {
+            flag_error(errs, ERR_CAPACITY);
+            cnt[3 * a] = cnt[3 * a + 1] = cnt[3 * a + 2] = 0;
+            rec[rec_off[a]] = 0;
+            continue;
+        }
+        const int cap = (int)(rec_off[a + 1] - rec_off[a]);
+        const int w = MI ? dev::write_record_mi<C>(X, k, rec + rec_off[a], cap)
+                         : dev::write_record_ia<C>(X, k, rec + rec_off[a], cap);
+        if (w < 0) flag_error(errs, ERR_RECORD);
+        uint32_t a0, a1, a2;
+        if (flags & RUN_CUT_ONLY) {
+            a0 = X.nv;
+            a1 = X.nf;
+            a2 = X.nc;
+        } else if (MI) {
+            mi_counts<C>(X, a0, a1, a2);
+        } else {
+            ia_counts<C>(X, a0, a1, a2);
+        }
+        cnt[3 * a] = a0;
+        cnt[3 * a + 1] = a1;
+        cnt[3 * a + 2] = a2;
+    }
+}
+
+__global__ void k_split3(Count n, const uint32_t* __restrict__ cnt, uint32_t* a, uint32_t* b, uint32_t* c) {
+    GRID_STRIDE(i, n) {
+        a[i] = cnt[3 * i];
+        b[i] = cnt[3 * i + 1];
+        c[i] = cnt[3 * i + 2];
+    }
+}
+
+// ------------------------------------------------------------------ extraction (IA)
+
+// keys: (payload << 2) | type; type 0 corner, 1 tet edge, 2 tet face, 3 interior
+__device__ inline uint64_t key_corner(uint32_t v) { return ((uint64_t)v << 2) | 0; }
+__device__ inline uint64_t key_edge(uint64_t e, uint32_t f) { return ((e << 6 | f) << 2) | 1; }
+__device__ inline uint64_t key_face(uint64_t fc, uint32_t f0, uint32_t f1) {
+    return ((((fc << 6) | f0) << 6 | f1) << 2) | 2;
+}
+__device__ inline uint64_t key_inner(uint64_t a, uint32_t j) { return ((a << 8 | j) << 2) | 3; }
+
+struct Rec {
+    const uint16_t* r;
+    int L, nv, nf;
+    bool dup;
+    __device__ Rec(const uint16_t* p) : r(p), L(p[1]), nv(p[2]), nf(p[3]), dup(p[5] & 1) {}
+    __device__ const uint16_t* vert(int j) const { return r + 8 + 4 * j; }
+    __device__ const uint16_t* faces() const { return r + 8 + 4 * nv; }
+    __device__ const uint16_t* groups_ia() const { return r + r[6] - 2 * L; }
+};
+
+__device__ inline bool ia_face_is_iso(const Rec& R, const uint16_t* F) {
+    if (F[0] > 3) return true;
+    if (!R.dup) return false;
+    const uint16_t* grp = R.groups_ia();
+    const uint16_t rep = grp[F[0]];
+    for (int q = 4; q < R.L; ++q)
+        if (grp[q] == rep) return true;
+    return false;
+}
+
+__global__ void k_extract_ia(GridView g, const uint32_t* __restrict__ act, const uint32_t* __restrict__ csr_off,
+                             const uint8_t* __restrict__ fit, Count n_active, const uint32_t* __restrict__ rec_off,
+                             const uint16_t* __restrict__ rec, const uint32_t* __restrict__ voff,
+                             const uint32_t* __restrict__ foff, const uint32_t* __restrict__ fvoff,
+                             uint64_t* __restrict__ okey, uint32_t* __restrict__ oa, uint16_t* __restrict__ oj,
+                             uint32_t* __restrict__ fq_a, uint16_t* __restrict__ fq_l, uint8_t* __restrict__ fq_fn,
+                             uint8_t* __restrict__ fq_bnd, uint32_t* __restrict__ fq_vs, uint32_t* __restrict__ fve) {
+    GRID_STRIDE(a, n_active) {
+        const uint32_t t = act[a];
+        const Rec R(rec + rec_off[a]);
+        if (R.r[0] != 1) continue;
+        uint32_t v[4];
+        dev::tet_corners(g, t, v);
+        const uint32_t s0 = csr_off[t];
+        uint64_t iso[4] = {0, 0, 0, 0};
+        const uint16_t* F = R.faces();
+        for (int f = 0; f < R.nf; ++f) {
+            const int n = F[3];
+            if (ia_face_is_iso(R, F))
+                for (int q = 0; q < n; ++q) iso[F[4 + q] >> 6] |= 1ull << (F[4 + q] & 63);
+            F += 4 + n;
+        }
+        // vertex occurrences in local order
+        uint32_t o = voff[a];
+        for (int j = 0; j < R.nv; ++j) {
+            if (!((iso[j >> 6] >> (j & 63)) & 1)) continue;
+            const uint16_t* T3 = R.vert(j);
+            int bnd[3], nb = 0;
+            uint32_t fn[3];
+            int ni = 0;
+            for (int q = 0; q < 3; ++q) {
+                if (T3[q] > 3)
+                    fn[ni++] = fit[s0 + T3[q] - 4];
+                else
+                    bnd[nb++] = T3[q];
+            }
+            uint64_t key;
+            if (nb == 3) {
+                int c = 0;
+                while (c == bnd[0] || c == bnd[1] || c == bnd[2]) ++c;
+                key = key_corner(v[c]);
+            } else if (nb == 2) {
+                int l[2], nl = 0;
+                for (int c = 0; c < 4; ++c)
+                    if (c != bnd[0] && c != bnd[1]) l[nl++] = c;
+                key = key_edge(dev::edge_id(g, t, v, l[0], l[1]), fn[0]);
+            } else if (nb == 1) {
+                key = key_face(dev::face_id(g, t, v, bnd[0]), fn[0], fn[1]);
+            } else {
+                key = key_inner(a, j);
+            }
+            okey[o] = key;
+            oa[o] = (uint32_t)a;
+            oj[o] = (uint16_t)j;
+            ++o;
+        }
+        // face occurrences in local order; vertex slots point at occurrences
+        uint32_t q = foff[a], e = fvoff[a];
+        const uint32_t vb = voff[a];
+        F = R.faces();
+        for (int f = 0; f < R.nf; ++f) {
+            const int n = F[3];
+            if (ia_face_is_iso(R, F)) {
+                int sp = F[0];
+                if (sp <= 3) {
+                    // a tet face coinciding with implicit planes: the smallest of them
+                    const uint16_t* grp = R.groups_ia();
+                    for (int p = 4; p < R.L; ++p)
+                        if (grp[p] == grp[sp]) {
+                            sp = p;
+                            break;
+                        }
+                }
+                fq_a[q] = (uint32_t)a;
+                fq_l[q] = (uint16_t)f;
+                fq_fn[q] = fit[s0 + sp - 4];
+                fq_bnd[q] = F[2] == 0xFFFF;
+                fq_vs[q] = e;
+                for (int i = 0; i < n; ++i) {
+                    const int j = F[4 + i];
+                    uint32_t rank = 0;
+                    for (int w = 0; w < (j >> 6); ++w) rank += __popcll(iso[w]);
+                    rank += __popcll(iso[j >> 6] & ((1ull << (j & 63)) - 1));
+                    fve[e++] = vb + rank;
+                }
+                ++q;
+            }
+            F += 4 + n;
+        }
+    }
+}
+
+// ------------------------------------------------------------------ dedup
+
+__global__ void k_heads(Count n, const uint64_t* __restrict__ skey, const uint32_t* __restrict__ sidx,
+                        uint32_t* __restrict__ first, uint32_t* __restrict__ head) {
+    GRID_STRIDE(p, n) {
+        const uint32_t h = (p == 0 || skey[p] != skey[p - 1]) ? 1u : 0u;
+        head[p] = h;
+        first[sidx[p]] = h;
+    }
+}
+
+// head_excl: exclusive scan of head over sorted positions; rank: exclusive
+// scan of `first` over occurrence order
+__global__ void k_assign(Count n, const uint32_t* __restrict__ sidx, const uint32_t* __restrict__ head,
+                         const uint32_t* __restrict__ head_excl, uint32_t* __restrict__ seg_first_occ) {
+    GRID_STRIDE(p, n) {
+        if (head[p]) seg_first_occ[head_excl[p]] = sidx[p];
+    }
+}
+__global__ void k_assign2(Count n, const uint32_t* __restrict__ sidx, const uint32_t* __restrict__ head,
+                          const uint32_t* __restrict__ head_excl, const uint32_t* __restrict__ seg_first_occ,
+                          const uint32_t* __restrict__ rank, uint32_t* __restrict__ id_of_occ) {
+    GRID_STRIDE(p, n) {
+        const uint32_t seg = head_excl[p] + head[p] - 1;
+        id_of_occ[sidx[p]] = rank[seg_first_occ[seg]];
+    }
+}
+__global__ void k_first_list(Count n, const uint32_t* __restrict__ first, const uint32_t* __restrict__ rank,
+                             uint32_t* __restrict__ first_occ) {
+    GRID_STRIDE(o, n) {
+        if (first[o]) first_occ[rank[o]] = (uint32_t)o;
+    }
+}
+
+// vertex records and coordinates (src/extract_mesh.cpp:95-228, 1446-1538)
+__global__ void k_vert_fields_ia(GridView g, const double* __restrict__ vals, const uint32_t* __restrict__ act,
+                                 const uint32_t* __restrict__ csr_off, const uint8_t* __restrict__ fit,
+                                 const uint32_t* __restrict__ rec_loc, const uint16_t* __restrict__ rec,
+                                 const uint16_t* __restrict__ k1_words, const uint32_t* __restrict__ k1_off,
+                                 const uint32_t* __restrict__ first_occ, const uint32_t* __restrict__ oa,
+                                 const uint16_t* __restrict__ oj, Count nverts, uint32_t* __restrict__ v_tet,
+                                 uint16_t* __restrict__ v_j, uint8_t* __restrict__ v_ss, uint32_t* __restrict__ v_sv,
+                                 uint8_t* __restrict__ v_fn, double* __restrict__ xyz) {
+    const uint64_t V = g.V;
+    GRID_STRIDE(i, nverts) {
+        const uint32_t o = first_occ[i];
+        const uint32_t a = oa[o];
+        const int j = oj[o];
+        const uint32_t t = act[a];
+        // a record in the workspace, or (bit 31 set) the k=1 record of a sign pattern
+        const uint32_t loc = rec_loc[a];
+        const Rec R((loc & 0x80000000u) ? k1_words + k1_off[loc & 0xFF] : rec + loc);
+        uint32_t v[4];
+        dev::tet_corners(g, t, v);
+        const uint32_t s0 = csr_off[t];
+        const uint16_t* T3 = R.vert(j);
+        int bnd[3], nb = 0;
+        uint32_t fn[3] = {0xFF, 0xFF, 0xFF};
+        int ni = 0;
+        for (int q = 0; q < 3; ++q) {
+            if (T3[q] > 3)
+                fn[ni++] = fit[s0 + T3[q] - 4];
+            else
+                bnd[nb++] = T3[q];
+        }
+        uint32_t sv[4] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu};
+        int ss;
+        double X0, X1, X2;
+        auto P = [&](uint32_t vid, int ax) { return g.pts[3 * (uint64_t)vid + ax]; };
+        auto fv = [&](uint32_t vid, uint32_t f) { return vals[(uint64_t)f * V + vid]; };
+        if (nb == 3) {
+            int c = 0;
+            while (c == bnd[0] || c == bnd[1] || c == bnd[2]) ++c;
+            ss = 1;
+            sv[0] = v[c];
+            X0 = P(sv[0], 0);
+            X1 = P(sv[0], 1);
+            X2 = P(sv[0], 2);
+        } else if (nb == 2) {
+            uint32_t e[2];
+            int ne = 0;
+            for (int c = 0; c < 4; ++c)
+                if (c != bnd[0] && c != bnd[1]) e[ne++] = v[c];
+            sv[0] = e[0] < e[1] ? e[0] : e[1];
+            sv[1] = e[0] < e[1] ? e[1] : e[0];
+            ss = 2;
+            const double f1 = fv(sv[0], fn[0]), f2 = fv(sv[1], fn[0]);
+            const double b0 = f2 / (f2 - f1);
+            const double b1 = 1 - b0;
+            X0 = b0 * P(sv[0], 0) + b1 * P(sv[1], 0);
+            X1 = b0 * P(sv[0], 1) + b1 * P(sv[1], 1);
+            X2 = b0 * P(sv[0], 2) + b1 * P(sv[1], 2);
+        } else if (nb == 1) {
+            uint32_t e[3];
+            int ne = 0;
+            for (int c = 0; c < 4; ++c)
+                if (c != bnd[0]) e[ne++] = v[c];
+            for (int x = 1; x < 3; ++x)
+                for (int y = x; y > 0 && e[y - 1] > e[y]; --y) {
+                    uint32_t tmp = e[y];
+                    e[y] = e[y - 1];
+                    e[y - 1] = tmp;
+                }
+            for (int q = 0; q < 3; ++q) sv[q] = e[q];
+            ss = 3;
+            double p1[3], p2[3];
+            for (int q = 0; q < 3; ++q) {
+                p1[q] = fv(sv[q], fn[0]);
+                p2[q] = fv(sv[q], fn[1]);
+            }
+            const double n1 = p1[2] * p2[1] - p1[1] * p2[2];
+            const double n2 = p1[0] * p2[2] - p1[2] * p2[0];
+            const double n3 = p1[1] * p2[0] - p1[0] * p2[1];
+            const double d = n1 + n2 + n3;
+            const double b0 = n1 / d, b1 = n2 / d, b2 = n3 / d;
+            X0 = b0 * P(sv[0], 0) + b1 * P(sv[1], 0) + b2 * P(sv[2], 0);
+            X1 = b0 * P(sv[0], 1) + b1 * P(sv[1], 1) + b2 * P(sv[2], 1);
+            X2 = b0 * P(sv[0], 2) + b1 * P(sv[1], 2) + b2 * P(sv[2], 2);
+        } else {
+            for (int q = 0; q < 4; ++q) sv[q] = v[q];
+            ss = 4;
+            double a1[4], a2[4], a3[4];
+            for (int q = 0; q < 4; ++q) {
+                a1[q] = fv(sv[q], fn[0]);
+                a2[q] = fv(sv[q], fn[1]);
+                a3[q] = fv(sv[q], fn[2]);
+            }
+            const double n1 = a1[3] * (a2[2] * a3[1] - a2[1] * a3[2]) + a1[2] * (a2[1] * a3[3] - a2[3] * a3[1]) +
+                              a1[1] * (a2[3] * a3[2] - a2[2] * a3[3]);
+            const double n2 = a1[3] * (a2[0] * a3[2] - a2[2] * a3[0]) + a1[2] * (a2[3] * a3[0] - a2[0] * a3[3]) +
+                              a1[0] * (a2[2] * a3[3] - a2[3] * a3[2]);
+            const double n3 = a1[3] * (a2[1] * a3[0] - a2[0] * a3[1]) + a1[1] * (a2[0] * a3[3] - a2[3] * a3[0]) +
+                              a1[0] * (a2[3] * a3[1] - a2[1] * a3[3]);
+            const double n4 = a1[2] * (a2[0] * a3[1] - a2[1] * a3[0]) + a1[1] * (a2[2] * a3[0] - a2[0] * a3[2]) +
+                              a1[0] * (a2[1] * a3[2] - a2[2] * a3[1]);
+            const double d = n1 + n2 + n3 + n4;
+            const double b0 = n1 / d, b1 = n2 / d, b2 = n3 / d, b3 = n4 / d;
+            X0 = b0 * P(sv[0], 0) + b1 * P(sv[1], 0) + b2 * P(sv[2], 0) + b3 * P(sv[3], 0);
+            X1 = b0 * P(sv[0], 1) + b1 * P(sv[1], 1) + b2 * P(sv[2], 1) + b3 * P(sv[3], 1);
+            X2 = b0 * P(sv[0], 2) + b1 * P(sv[1], 2) + b2 * P(sv[2], 2) + b3 * P(sv[3], 2);
+        }
+        v_tet[i] = t;
+        v_j[i] = (uint16_t)j;
+        v_ss[i] = (uint8_t)ss;
+        for (int q = 0; q < 4; ++q) {
+            v_sv[4 * i + q] = sv[q];
+            v_fn[4 * i + q] = q < 3 ? (uint8_t)fn[q] : 0xFF;
+        }
+        xyz[3 * i] = X0;
+        xyz[3 * i + 1] = X1;
+        xyz[3 * i + 2] = X2;
+    }
+}
+
+// ------------------------------------------------------------------ faces
+
+// global vertex ids of every face occurrence; boundary faces get their
+// (min, second-min, max) key (src/extract_mesh.h:68-92)
+__global__ void k_face_keys(Count nq, const uint32_t* __restrict__ fq_vs, const uint32_t* __restrict__ fq_vs_end,
+                            uint32_t* __restrict__ fve, const uint32_t* __restrict__ vid_of_occ,
+                            const uint8_t* __restrict__ fq_bnd, int vb, uint64_t* __restrict__ khi,
+                            uint64_t* __restrict__ klo) {
+    GRID_STRIDE(q, nq) {
+        const uint32_t b = fq_vs[q], e = fq_vs_end[q];
+        for (uint32_t i = b; i < e; ++i) fve[i] = vid_of_occ[fve[i]];
+        if (!fq_bnd[q]) {
+            // interior faces are never merged; they sort after every boundary key
+            khi[q] = 1ull << (2 * vb);
+            klo[q] = 0;
+            continue;
+        }
+        uint32_t mn = fve[b], mx = fve[b], mp = b;
+        for (uint32_t i = b + 1; i < e; ++i) {
+            if (fve[i] < mn) {
+                mn = fve[i];
+                mp = i;
+            } else if (fve[i] > mx) {
+                mx = fve[i];
+            }
+        }
+        uint32_t sm = mx + 1;
+        for (uint32_t i = b; i < e; ++i)
+            if (i != mp && fve[i] < sm) sm = fve[i];
+        khi[q] = ((uint64_t)mn << vb) | sm;
+        klo[q] = mx;
+    }
+}
+
+__global__ void k_face_heads(Count n, const uint32_t* __restrict__ perm, const uint64_t* __restrict__ khi,
+                             const uint64_t* __restrict__ klo, const uint8_t* __restrict__ bnd,
+                             uint32_t* __restrict__ first, uint32_t* __restrict__ head) {
+    GRID_STRIDE(p, n) {
+        const uint32_t q = perm[p];
+        uint32_t h = 1;
+        if (p > 0 && bnd[q]) {
+            const uint32_t r = perm[p - 1];
+            h = (bnd[r] && khi[r] == khi[q] && klo[r] == klo[q]) ? 0u : 1u;
+        }
+        head[p] = h;
+        first[q] = h;
+    }
+}
+
+// per unique face: occurrence count and vertex count
+__global__ void k_face_counts(Count n, const uint32_t* __restrict__ perm, const uint32_t* __restrict__ head,
+                              const uint32_t* __restrict__ head_excl, const uint32_t* __restrict__ seg_first_occ,
+                              const uint32_t* __restrict__ rank, uint32_t* __restrict__ tf_cnt) {
+    GRID_STRIDE(p, n) {
+        const uint32_t seg = head_excl[p] + head[p] - 1;
+        atomicAdd(&tf_cnt[rank[seg_first_occ[seg]]], 1u);
+    }
+}
+
+__global__ void k_face_out(Count nfaces, const uint32_t* __restrict__ first_occ, const uint32_t* __restrict__ fq_vs,
+                           const uint32_t* __restrict__ fq_vs_end, const uint32_t* __restrict__ fve,
+                           const uint8_t* __restrict__ fq_fn, const uint32_t* __restrict__ fvstart,
+                           uint32_t* __restrict__ fverts, uint8_t* __restrict__ ffunc) {
+    GRID_STRIDE(i, nfaces) {
+        const uint32_t q = first_occ[i];
+        uint32_t o = fvstart[i];
+        for (uint32_t e = fq_vs[q]; e < fq_vs_end[q]; ++e) fverts[o++] = fve[e];
+        ffunc[2 * i] = fq_fn[q];
+        ffunc[2 * i + 1] = 0xFF;
+    }
+}
+
+__global__ void k_face_vcount(Count nfaces, const uint32_t* __restrict__ first_occ, const uint32_t* __restrict__ fq_vs,
+                              const uint32_t* __restrict__ fq_vs_end, uint32_t* __restrict__ cnt) {
+    GRID_STRIDE(i, nfaces) {
+        const uint32_t q = first_occ[i];
+        cnt[i] = fq_vs_end[q] - fq_vs[q];
+    }
+}
+
+// occurrences of each unique face in occurrence order: sorted positions of a
+// segment are consecutive and (stable sort) already in occurrence order
+__global__ void k_face_tf(Count n, const uint32_t* __restrict__ perm, const uint32_t* __restrict__ head,
+                          const uint32_t* __restrict__ head_excl, const uint32_t* __restrict__ seg_first_occ,
+                          const uint32_t* __restrict__ seg_start, const uint32_t* __restrict__ rank,
+                          const uint32_t* __restrict__ tfstart, const uint32_t* __restrict__ act,
+                          const uint32_t* __restrict__ fq_a, const uint16_t* __restrict__ fq_l,
+                          uint32_t* __restrict__ tf_tet, uint16_t* __restrict__ tf_local) {
+    GRID_STRIDE(p, n) {
+        const uint32_t seg = head_excl[p] + head[p] - 1;
+        const uint32_t fid = rank[seg_first_occ[seg]];
+        const uint32_t pos = tfstart[fid] + ((uint32_t)p - seg_start[seg]);
+        const uint32_t q = perm[p];
+        tf_tet[pos] = act[fq_a[q]];
+        tf_local[pos] = fq_l[q];
+    }
+}
+__global__ void k_seg_start(Count n, const uint32_t* __restrict__ head, const uint32_t* __restrict__ head_excl,
+                            uint32_t* __restrict__ seg_start) {
+    GRID_STRIDE(p, n) {
+        if (head[p]) seg_start[head_excl[p]] = (uint32_t)p;
+    }
+}
+
+
+}  // namespace
+}  // namespace rin

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -22,10 +23,13 @@ struct Grid {
     DBuf<uint32_t> tets, tedge, tface;
 };
 
+struct Plan;  // batched form: workspaces and captured graphs (batched.hip)
+
 struct Values {
     uint32_t F = 0;
     uint64_t V = 0;
     DBuf<double> vals;  // F x V, function-major
+    std::shared_ptr<Plan> plan[2];  // IA, MI
 };
 
 enum RunFlags : uint32_t {
@@ -69,6 +73,27 @@ struct RunOut {
     DBuf<uint16_t> f_tf_local;
     DBuf<uint8_t> f_func;  // 2 per face (0xFF unused)
     StageTimes times;
+
+    // read-only views of the outputs (owned by the DBufs above in the plain
+    // form, by a batched workspace otherwise)
+    struct View {
+        const uint8_t* fit = nullptr;
+        const uint32_t *csr_off = nullptr, *act = nullptr, *rank_of_tet = nullptr, *rec_off = nullptr;
+        const uint16_t* rec = nullptr;
+        const uint32_t* rec_loc = nullptr;       // batched: word offset, or 0x80000000|pattern for k=1
+        const uint16_t* k1_rec = nullptr;        // batched: k=1 records by pattern
+        const uint32_t* k1_rec_off = nullptr;
+        const uint32_t *v_tet = nullptr, *v_sv = nullptr;
+        const uint16_t* v_j = nullptr;
+        const uint8_t *v_ss = nullptr, *v_fn = nullptr;
+        const double* xyz = nullptr;
+        const uint32_t *f_vstart = nullptr, *f_verts = nullptr, *f_tfstart = nullptr, *f_tf_tet = nullptr;
+        const uint16_t* f_tf_local = nullptr;
+        const uint8_t* f_func = nullptr;
+        const uint32_t* cut_counts = nullptr;
+    } view;
+    std::shared_ptr<void> hold;  // keeps a batched workspace checked out
+    void set_views_from_buffers();
 };
 
 void grid_regular(DevMem& mem, int res, const double bmin[3], const double bmax[3], Grid& g, hipStream_t s);
@@ -77,6 +102,11 @@ void grid_explicit(DevMem& mem, const double* pts, uint64_t V, const uint64_t* t
 void values_eval(DevMem& mem, const Grid& g, const FuncDesc* funcs, uint32_t F, Values& v, hipStream_t s);
 void values_upload(DevMem& mem, const Grid& g, const double* rowmajor, uint32_t F, Values& v, hipStream_t s);
 void run_pipeline(DevMem& mem, const Grid& g, const Values& v, uint32_t flags, RunOut& out, hipStream_t s);
+
+// batched form: buckets by active-function count, k=1 lookup table in LDS,
+// persistent per-bucket cut kernels, whole path captured in a hipGraph
+void run_batched(DevMem& ctx_mem, const Grid& g, Values& v, uint32_t flags, RunOut& out, hipStream_t s,
+                 int n_cu);
 
 // diagnostics: run the device cut on independent plane/material sets
 void debug_cuts(DevMem& mem, int mi, const double* rows, const int* ks, int ncase, int cap, uint16_t* words,

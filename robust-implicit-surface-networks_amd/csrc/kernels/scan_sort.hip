@@ -43,7 +43,8 @@ __device__ inline uint32_t block_excl_scan(uint32_t v, uint32_t& total) {
     return off + inc - v;
 }
 
-__global__ void k_scan_reduce(const uint32_t* __restrict__ in, size_t n, uint32_t* __restrict__ sums) {
+__global__ void k_scan_reduce(const uint32_t* __restrict__ in, Count cn, uint32_t* __restrict__ sums) {
+    const size_t n = cn.get();
     const size_t base = (size_t)blockIdx.x * kTile;
     uint32_t acc = 0;
     for (int i = 0; i < kItems; ++i) {
@@ -55,8 +56,9 @@ __global__ void k_scan_reduce(const uint32_t* __restrict__ in, size_t n, uint32_
     if (threadIdx.x == 0) sums[blockIdx.x] = tot;
 }
 
-__global__ void k_scan_apply(const uint32_t* in, uint32_t* out, size_t n, const uint32_t* __restrict__ offs,
+__global__ void k_scan_apply(const uint32_t* in, uint32_t* out, Count cn, const uint32_t* __restrict__ offs,
                              uint32_t* total) {
+    const size_t n = cn.get();
     const size_t base = (size_t)blockIdx.x * kTile + (size_t)threadIdx.x * kItems;
     uint32_t v[kItems];
     uint32_t acc = 0;
@@ -72,7 +74,9 @@ __global__ void k_scan_apply(const uint32_t* in, uint32_t* out, size_t n, const 
         if (idx < n) out[idx] = run;
         run += v[i];
     }
-    if (total && blockIdx.x == gridDim.x - 1 && threadIdx.x == kThreads - 1) *total = run;
+    // the block holding element n-1 (or block 0 when n == 0) reports the total
+    const size_t last_blk = n ? (n - 1) / kTile : 0;
+    if (total && blockIdx.x == last_blk && threadIdx.x == kThreads - 1) *total = run;
 }
 
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t n) {
@@ -81,18 +85,30 @@ __global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t n) {
 __global__ void k_fill_u64(uint64_t* p, uint64_t v, size_t n) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
 }
-__global__ void k_iota_u32(uint32_t* p, size_t n) {
+__global__ void k_iota_u32(uint32_t* p, Count cn) {
+    const size_t n = cn.get();
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
         p[i] = (uint32_t)i;
 }
 __global__ void k_gather_u64(const uint64_t* __restrict__ src, const uint32_t* __restrict__ idx, uint64_t* dst,
-                             size_t n) {
+                             Count cn) {
+    const size_t n = cn.get();
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
         dst[i] = src[idx[i]];
 }
 
-__global__ void k_rs_hist(const uint64_t* __restrict__ keys, size_t n, int shift, uint32_t* __restrict__ hist,
+__global__ void k_copy_u32(uint32_t* __restrict__ d, const uint32_t* __restrict__ s, Count cn) {
+    const size_t n = cn.get();
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) d[i] = s[i];
+}
+__global__ void k_copy_u64(uint64_t* __restrict__ d, const uint64_t* __restrict__ s, Count cn) {
+    const size_t n = cn.get();
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) d[i] = s[i];
+}
+
+__global__ void k_rs_hist(const uint64_t* __restrict__ keys, Count cn, int shift, uint32_t* __restrict__ hist,
                           uint32_t nblocks) {
+    const size_t n = cn.get();
     __shared__ uint32_t cnt[256];
     cnt[threadIdx.x] = 0;
     __syncthreads();
@@ -106,8 +122,9 @@ __global__ void k_rs_hist(const uint64_t* __restrict__ keys, size_t n, int shift
 }
 
 __global__ void k_rs_scatter(const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin,
-                             uint64_t* __restrict__ kout, uint32_t* __restrict__ vout, size_t n, int shift,
+                             uint64_t* __restrict__ kout, uint32_t* __restrict__ vout, Count cn, int shift,
                              const uint32_t* __restrict__ offs, uint32_t nblocks) {
+    const size_t n = cn.get();
     __shared__ uint32_t run[256];
     __shared__ uint32_t wcnt[kWaves][256];
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
@@ -162,32 +179,38 @@ void fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s) {
 void fill_u64(uint64_t* p, uint64_t v, size_t n, hipStream_t s) {
     if (n) hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(n)), dim3(256), 0, s, p, v, n);
 }
-void iota_u32(uint32_t* p, size_t n, hipStream_t s) {
-    if (n) hipLaunchKernelGGL(k_iota_u32, dim3(grid_for(n)), dim3(256), 0, s, p, n);
+void iota_u32(uint32_t* p, Count n, hipStream_t s) {
+    if (n.cap) hipLaunchKernelGGL(k_iota_u32, dim3(grid_for(n.cap)), dim3(256), 0, s, p, n);
+}
+void copy_u32(uint32_t* d, const uint32_t* src, Count n, hipStream_t s) {
+    if (n.cap) hipLaunchKernelGGL(k_copy_u32, dim3(grid_for(n.cap)), dim3(256), 0, s, d, src, n);
+}
+void copy_u64(uint64_t* d, const uint64_t* src, Count n, hipStream_t s) {
+    if (n.cap) hipLaunchKernelGGL(k_copy_u64, dim3(grid_for(n.cap)), dim3(256), 0, s, d, src, n);
 }
 
-void scan_u32(DevMem& mem, const uint32_t* in, uint32_t* out, size_t n, uint32_t* total, hipStream_t s) {
-    if (n == 0) {
+void scan_u32(DevMem& mem, const uint32_t* in, uint32_t* out, Count n, uint32_t* total, hipStream_t s) {
+    if (n.cap == 0) {
         if (total) fill_u32(total, 0, 1, s);
         return;
     }
-    const size_t nb = (n + kTile - 1) / kTile;
+    const size_t nb = (n.cap + kTile - 1) / kTile;
     if (nb == 1) {
         hipLaunchKernelGGL(k_scan_apply, dim3(1), dim3(kThreads), 0, s, in, out, n, (const uint32_t*)nullptr, total);
         return;
     }
     DBuf<uint32_t> sums(mem, nb);
     hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb), dim3(kThreads), 0, s, in, n, sums.p);
-    scan_u32(mem, sums.p, sums.p, nb, nullptr, s);
+    scan_u32(mem, sums.p, sums.p, Count(nb), nullptr, s);
     hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nb), dim3(kThreads), 0, s, in, out, n, (const uint32_t*)sums.p,
                        total);
 }
 
-void sort_pairs(DevMem& mem, uint64_t* keys, uint32_t* vals, size_t n, int bits, hipStream_t s) {
-    if (n <= 1 || bits <= 0) return;
-    const uint32_t nb = (uint32_t)((n + kTile - 1) / kTile);
-    DBuf<uint64_t> k2(mem, n);
-    DBuf<uint32_t> v2(mem, n);
+void sort_pairs(DevMem& mem, uint64_t* keys, uint32_t* vals, Count n, int bits, hipStream_t s) {
+    if (n.cap <= 1 || bits <= 0) return;
+    const uint32_t nb = (uint32_t)((n.cap + kTile - 1) / kTile);
+    DBuf<uint64_t> k2(mem, n.cap);
+    DBuf<uint32_t> v2(mem, n.cap);
     DBuf<uint32_t> hist(mem, (size_t)256 * nb);
     uint64_t* ka = keys;
     uint32_t* va = vals;
@@ -195,28 +218,28 @@ void sort_pairs(DevMem& mem, uint64_t* keys, uint32_t* vals, size_t n, int bits,
     uint32_t* vb = v2.p;
     for (int shift = 0; shift < bits; shift += 8) {
         hipLaunchKernelGGL(k_rs_hist, dim3(nb), dim3(kThreads), 0, s, ka, n, shift, hist.p, nb);
-        scan_u32(mem, hist.p, hist.p, (size_t)256 * nb, nullptr, s);
+        scan_u32(mem, hist.p, hist.p, Count((size_t)256 * nb), nullptr, s);
         hipLaunchKernelGGL(k_rs_scatter, dim3(nb), dim3(kThreads), 0, s, ka, va, kb, vb, n, shift,
                            (const uint32_t*)hist.p, nb);
         std::swap(ka, kb);
         std::swap(va, vb);
     }
     if (ka != keys) {
-        RIN_HIP(hipMemcpyAsync(keys, ka, n * sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
-        RIN_HIP(hipMemcpyAsync(vals, va, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+        copy_u64(keys, ka, n, s);
+        copy_u32(vals, va, n, s);
     }
 }
 
 void sort_index_by2(DevMem& mem, const uint64_t* hi, const uint64_t* lo, int hi_bits, int lo_bits, uint32_t* perm,
-                    size_t n, hipStream_t s) {
+                    Count n, hipStream_t s) {
     iota_u32(perm, n, s);
-    if (n <= 1) return;
-    DBuf<uint64_t> k(mem, n);
+    if (n.cap <= 1) return;
+    DBuf<uint64_t> k(mem, n.cap);
     if (lo) {
-        RIN_HIP(hipMemcpyAsync(k.p, lo, n * sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
+        copy_u64(k.p, lo, n, s);
         sort_pairs(mem, k.p, perm, n, lo_bits, s);
     }
-    hipLaunchKernelGGL(k_gather_u64, dim3(grid_for(n)), dim3(256), 0, s, hi, (const uint32_t*)perm, k.p, n);
+    hipLaunchKernelGGL(k_gather_u64, dim3(grid_for(n.cap)), dim3(256), 0, s, hi, (const uint32_t*)perm, k.p, n);
     sort_pairs(mem, k.p, perm, n, hi_bits, s);
 }
 

@@ -13,6 +13,7 @@
 
 struct rin_ctx {
     int device = 0;
+    int n_cu = 256;
     hipStream_t stream = nullptr;
     rin::DevMem mem;
     std::mutex mu;
@@ -91,6 +92,9 @@ int rin_create(rin_ctx** out, int32_t device) {
         delete c;
         return RIN_ERR_DEVICE;
     }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+        c->n_cu = prop.multiProcessorCount;
     *out = c;
     return RIN_OK;
 }
@@ -182,7 +186,10 @@ int rin_run(rin_ctx* c, const rin_grid* g, const rin_values* v, uint32_t flags, 
     *out = nullptr;
     return guarded(c, [&] {
         auto r = std::make_unique<rin_result>();
-        rin::run_pipeline(c->mem, g->g, v->v, flags, r->r, c->stream);
+        if (flags & RIN_FLAG_BATCHED)
+            rin::run_batched(c->mem, g->g, const_cast<rin_values*>(v)->v, flags, r->r, c->stream, c->n_cu);
+        else
+            rin::run_pipeline(c->mem, g->g, v->v, flags, r->r, c->stream);
         RIN_HIP(hipStreamSynchronize(c->stream));
         *out = r.release();
     });
@@ -208,8 +215,8 @@ int rin_result_csr(rin_ctx* c, const rin_result* res, uint64_t* fit, uint64_t* s
     if (!res || !fit || !start) return fail(c, RIN_ERR_ARG, "null argument");
     return guarded(c, [&] {
         const auto& r = res->r;
-        auto a = fetch(r.fit.p, r.n_fit, c->stream);
-        auto b = fetch(r.csr_off.p, r.T + 1, c->stream);
+        auto a = fetch(r.view.fit, r.n_fit, c->stream);
+        auto b = fetch(r.view.csr_off, r.T + 1, c->stream);
         for (uint64_t i = 0; i < r.n_fit; ++i) fit[i] = a[i];
         for (uint64_t i = 0; i <= r.T; ++i) start[i] = b[i];
     });
@@ -220,12 +227,12 @@ int rin_result_verts(rin_ctx* c, const rin_result* res, uint64_t* rec11, double*
     return guarded(c, [&] {
         const auto& r = res->r;
         const uint64_t n = r.n_verts;
-        auto tet = fetch(r.v_tet.p, n, c->stream);
-        auto j = fetch(r.v_j.p, n, c->stream);
-        auto ss = fetch(r.v_ss.p, n, c->stream);
-        auto sv = fetch(r.v_sv.p, 4 * n, c->stream);
-        auto fn = fetch(r.v_fn.p, 4 * n, c->stream);
-        auto x = fetch(r.xyz.p, 3 * n, c->stream);
+        auto tet = fetch(r.view.v_tet, n, c->stream);
+        auto j = fetch(r.view.v_j, n, c->stream);
+        auto ss = fetch(r.view.v_ss, n, c->stream);
+        auto sv = fetch(r.view.v_sv, 4 * n, c->stream);
+        auto fn = fetch(r.view.v_fn, 4 * n, c->stream);
+        auto x = fetch(r.view.xyz, 3 * n, c->stream);
         for (uint64_t i = 0; i < n; ++i) {
             uint64_t* o = rec11 + 11 * i;
             o[0] = tet[i];
@@ -244,12 +251,12 @@ int rin_result_faces(rin_ctx* c, const rin_result* res, uint64_t* vstart, uint64
     return guarded(c, [&] {
         const auto& r = res->r;
         const uint64_t nf = r.n_faces;
-        auto a = fetch(r.f_vstart.p, nf + 1, c->stream);
-        auto b = fetch(r.f_verts.p, r.n_face_verts, c->stream);
-        auto d = fetch(r.f_tfstart.p, nf + 1, c->stream);
-        auto e = fetch(r.f_tf_tet.p, r.n_tet_faces, c->stream);
-        auto f = fetch(r.f_tf_local.p, r.n_tet_faces, c->stream);
-        auto g = fetch(r.f_func.p, 2 * nf, c->stream);
+        auto a = fetch(r.view.f_vstart, nf + 1, c->stream);
+        auto b = fetch(r.view.f_verts, r.n_face_verts, c->stream);
+        auto d = fetch(r.view.f_tfstart, nf + 1, c->stream);
+        auto e = fetch(r.view.f_tf_tet, r.n_tet_faces, c->stream);
+        auto f = fetch(r.view.f_tf_local, r.n_tet_faces, c->stream);
+        auto g = fetch(r.view.f_func, 2 * nf, c->stream);
         for (uint64_t i = 0; i <= nf; ++i) {
             vstart[i] = a[i];
             tfstart[i] = d[i];
@@ -268,31 +275,31 @@ int rin_result_records(rin_ctx* c, const rin_result* res, const uint64_t* tets, 
     if (!res || (!tets && n) || !rec_start) return fail(c, RIN_ERR_ARG, "null argument");
     return guarded(c, [&] {
         const auto& r = res->r;
-        if (!r.rec.p) throw std::invalid_argument("records were not kept (RIN_FLAG_KEEP_RECORDS)");
+        const auto& vw = r.view;
+        if (!vw.rec) throw std::invalid_argument("records were not kept (RIN_FLAG_KEEP_RECORDS)");
+        auto get32 = [&](const uint32_t* p) {
+            uint32_t x = 0;
+            RIN_HIP(hipMemcpyAsync(&x, p, 4, hipMemcpyDeviceToHost, c->stream));
+            RIN_HIP(hipStreamSynchronize(c->stream));
+            return x;
+        };
         uint64_t o = 0;
         rec_start[0] = 0;
         for (uint64_t i = 0; i < n; ++i) {
             const uint64_t t = tets[i];
             if (t >= r.T) throw std::invalid_argument("tet index out of range");
-            uint32_t rk = 0, at = 0;
-            RIN_HIP(hipMemcpyAsync(&rk, r.rank_of_tet.p + t, 4, hipMemcpyDeviceToHost, c->stream));
-            RIN_HIP(hipStreamSynchronize(c->stream));
-            bool active = rk < r.n_active;
+            const uint32_t rk = get32(vw.rank_of_tet + t);
+            bool active = rk < r.n_active && get32(vw.act + rk) == t;
             if (active) {
-                RIN_HIP(hipMemcpyAsync(&at, r.act.p + rk, 4, hipMemcpyDeviceToHost, c->stream));
-                RIN_HIP(hipStreamSynchronize(c->stream));
-                active = at == t;
-            }
-            if (active) {
-                uint32_t off = 0;
+                const uint32_t loc = get32(vw.rec_loc + rk);
+                const uint16_t* src = (loc & 0x80000000u) ? vw.k1_rec + get32(vw.k1_rec_off + (loc & 0xFF))
+                                                          : vw.rec + loc;
                 uint16_t hdr[8];
-                RIN_HIP(hipMemcpyAsync(&off, r.rec_off.p + rk, 4, hipMemcpyDeviceToHost, c->stream));
-                RIN_HIP(hipStreamSynchronize(c->stream));
-                RIN_HIP(hipMemcpyAsync(hdr, r.rec.p + off, 16, hipMemcpyDeviceToHost, c->stream));
+                RIN_HIP(hipMemcpyAsync(hdr, src, 16, hipMemcpyDeviceToHost, c->stream));
                 RIN_HIP(hipStreamSynchronize(c->stream));
                 const uint64_t len = hdr[6];
                 if (o + len > cap) throw std::invalid_argument("record buffer too small");
-                RIN_HIP(hipMemcpyAsync(words + o, r.rec.p + off, 2 * len, hipMemcpyDeviceToHost, c->stream));
+                RIN_HIP(hipMemcpyAsync(words + o, src, 2 * len, hipMemcpyDeviceToHost, c->stream));
                 RIN_HIP(hipStreamSynchronize(c->stream));
                 o += len;
             }
@@ -305,8 +312,8 @@ int rin_result_cut_counts(rin_ctx* c, const rin_result* res, uint64_t* counts) {
     if (!res || !counts) return fail(c, RIN_ERR_ARG, "null argument");
     return guarded(c, [&] {
         const auto& r = res->r;
-        if (!r.cut_counts.p) throw std::invalid_argument("run was not RIN_FLAG_CUT_ONLY");
-        auto h = fetch(r.cut_counts.p, 3 * r.n_active, c->stream);
+        if (!r.view.cut_counts) throw std::invalid_argument("run was not RIN_FLAG_CUT_ONLY");
+        auto h = fetch(r.view.cut_counts, 3 * r.n_active, c->stream);
         for (uint64_t i = 0; i < 3 * r.n_active; ++i) counts[i] = h[i];
     });
 }

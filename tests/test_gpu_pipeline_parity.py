@@ -88,3 +88,39 @@ def test_pipeline_ia_explicit_mesh(ctx):
     dev = vals.run(0)
     ref = S.OracleRun(0, pts, tets, S.evaluate(types, params, pts))
     _compare(dev, ref)
+
+
+@pytest.mark.parametrize("which", ["config1", "config2_small", "config4"])
+def test_batched_matches_oracle_and_replays(ctx, which):
+    """The batched form (k=1 table, bucket kernels, captured graph) gives the
+    oracle's output bit for bit, on the first (capturing) run and on graph
+    replays with plain-form runs between them."""
+    import rin
+
+    if which == "config1":
+        res, fs = W.config1()
+    elif which == "config2_small":
+        _, fs = W.config2()
+        res = 48
+    else:
+        res, fs = W.config4()
+    types, params = fs.arrays()
+    g = ctx.grid_regular(res)
+    vals = g.eval(types, params)
+    pts, tets = S.grid(res)
+    ref = S.OracleRun(0, pts, tets, S.evaluate(types, params, pts))
+    for rep in range(3):
+        dev = vals.run(rin.FLAG_BATCHED)
+        _compare(dev, ref)
+        dev.release()
+        plain = vals.run(0)
+        plain.release()
+    # a held result forces a second workspace; both stay correct
+    a = vals.run(rin.FLAG_BATCHED)
+    b = vals.run(rin.FLAG_BATCHED)
+    _compare(a, ref)
+    _compare(b, ref)
+    a.release()
+    b.release()
+    vals.release()
+    g.release()
