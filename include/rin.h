@@ -130,9 +130,10 @@ const char* rin_result_stage_label(rin_ctx* ctx, const rin_result* res, int32_t 
 int rin_result_release(rin_ctx* ctx, rin_result* res);
 
 /* Test hook with no counterpart in the reference: run the device per-tet cut
- * on independent plane (IA) or material (MI) sets.  rows holds sum(ks) rows of
- * 4 corner values; each case writes at most cap words at words + i*cap and its
- * length (or a negative error) to lens[i]. */
+ * on independent plane (IA) or material (MI) sets.  mode: 0 IA thread-per-tet
+ * cut, 1 MI thread-per-tet cut, 2 IA wave-cooperative cut.  rows holds
+ * sum(ks) rows of 4 corner values; each case writes at most cap words at
+ * words + i*cap and its length (or a negative error) to lens[i]. */
 int rin_debug_cuts(rin_ctx* ctx, int32_t mi, const double* rows, const int32_t* ks, int32_t n_cases, int32_t cap,
                    uint16_t* words, int32_t* lens);
 

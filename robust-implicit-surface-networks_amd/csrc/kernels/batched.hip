@@ -14,12 +14,14 @@
 //    the overflow flags (on overflow the workspace grows and the graph is
 //    re-captured).
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <functional>
 #include <mutex>
 #include <stdexcept>
 #include <string>
 
+#include "cut_wave.h"
 #include "kernels_ia.h"
 
 namespace rin {
@@ -290,6 +292,44 @@ __global__ void __launch_bounds__(64) k_cut_bucket(CutArgs A) {
         if (dev::write_record_ia<C>(X, k, out, (int)A.slot) < 0) A.errs[lane] |= 4;
         uint32_t a0, a1, a2;
         ia_counts<C>(X, a0, a1, a2);
+        A.cnt[3 * r] = a0;
+        A.cnt[3 * r + 1] = a1;
+        A.cnt[3 * r + 2] = a2;
+    }
+}
+
+// wave-cooperative cut: one block (a single wavefront) per bucket entry, the
+// complex in LDS; lanes share the predicates and the splits of each insertion
+template <class C>
+__global__ void __launch_bounds__(64) k_cut_wave(CutArgs A) {
+    __shared__ dev::WaveIA<C> X;
+    const int lane = threadIdx.x;
+    const uint32_t n = *A.abort ? 0u : min(*A.n, A.cap);
+    const uint32_t i = blockIdx.x;
+    if (i >= n) return;
+    const uint32_t r = A.list[i];
+    const int k = A.act_k[r];
+    const uint32_t t = A.act[r];
+    if (lane < 4 * k && 4 + k <= C::MP) {
+        uint32_t v[4];
+        dev::tet_corners(A.g, t, v);
+        const uint32_t f = A.fit[A.csr_off[t] + (lane >> 2)];
+        X.P[lane >> 2][lane & 3] = A.vals[(uint64_t)f * A.g.V + v[lane & 3]];
+    }
+    dev::wave_ia_cut<C>(X, k);
+    uint16_t* out = A.rec + A.rec_loc[r];
+    if (X.err) {
+        if (lane == 0) {
+            atomicOr(A.errs, 1u);
+            out[0] = 0;
+            A.cnt[3 * r] = A.cnt[3 * r + 1] = A.cnt[3 * r + 2] = 0;
+        }
+        return;
+    }
+    uint32_t a0, a1, a2;
+    const int len = dev::wave_ia_record<C>(X, k, out, (int)A.slot, a0, a1, a2);
+    if (lane == 0) {
+        if (len < 0) atomicOr(A.errs, 4u);
         A.cnt[3 * r] = a0;
         A.cnt[3 * r + 1] = a1;
         A.cnt[3 * r + 2] = a2;
@@ -644,6 +684,10 @@ void launch_cut(Workspace& W, const Grid& g, const Values& val, int q, int grid,
     A.rec = W.rec.p;
     A.cnt = W.cnt.p;
     A.errs = W.errs.p;
+    if (grid < 0) {  // wave form: one block per bucket entry
+        hipLaunchKernelGGL(k_cut_wave<C>, dim3(W.cap.b[q]), dim3(64), 0, s, A);
+        return;
+    }
     fill_u32(W.ctr.p + 24 + q, 0, 1, s);
     hipLaunchKernelGGL(k_cut_bucket<C>, dim3(grid), dim3(64), 0, s, A);
 }
@@ -893,12 +937,18 @@ void run_batched(DevMem& ctx_mem, const Grid& g, Values& val, uint32_t flags, Ru
             // autotune the bucket cut grids once per plan
             if (P.blocks_per_cu[0] == 0) {
                 RIN_HIP(hipStreamSynchronize(s));
-                const int cand[5] = {1, 2, 4, 8, 16};
+                // thread form at 1..16 blocks per CU, or the wave form (-1);
+                // RIN_CUT_FORM=wave|thread restricts the choice
+                const int cand[6] = {1, 2, 4, 8, 16, -1};
+                const char* form = std::getenv("RIN_CUT_FORM");
+                const bool only_wave = form && !std::strcmp(form, "wave");
+                const bool only_thread = form && !std::strcmp(form, "thread");
                 for (int q = 0; q < 3; ++q) {
                     float best = 1e30f;
-                    int bestc = 4;
-                    for (int ci = 0; ci < 5; ++ci) {
-                        const int grid = std::max(1, n_cu * cand[ci]);
+                    int bestc = only_wave ? -1 : 4;
+                    for (int ci = 0; ci < 6; ++ci) {
+                        if ((only_wave && cand[ci] >= 0) || (only_thread && cand[ci] < 0)) continue;
+                        const int grid = cand[ci] < 0 ? -1 : std::max(1, n_cu * cand[ci]);
                         float ms = time_ms(s, [&] {
                             if (q == 0 && w.cap.b[0]) launch_cut<CK2>(w, g, val, 0, grid, s);
                             if (q == 1 && w.cap.b[1]) launch_cut<CMed>(w, g, val, 1, grid, s);
@@ -913,10 +963,10 @@ void run_batched(DevMem& ctx_mem, const Grid& g, Values& val, uint32_t flags, Ru
                 }
             }
             int grid[3];
-            for (int q = 0; q < 3; ++q) grid[q] = std::max(1, n_cu * P.blocks_per_cu[q]);
+            for (int q = 0; q < 3; ++q) grid[q] = P.blocks_per_cu[q] < 0 ? -1 : std::max(1, n_cu * P.blocks_per_cu[q]);
             // capture the whole path
             RIN_HIP(hipStreamSynchronize(s));
-            auto capture = [&](const char* label, const std::function<void()>& body) {
+            auto capture = [&](const std::string& label, const std::function<void()>& body) {
                 hipGraph_t gr = nullptr;
                 hipGraphExec_t ex = nullptr;
                 RIN_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
@@ -928,9 +978,14 @@ void run_batched(DevMem& ctx_mem, const Grid& g, Values& val, uint32_t flags, Ru
             };
             capture("signs + filter + CSR scans", [&] { stage_count(w, g, val, s); });
             capture("CSR fill + buckets + k=1 table", [&] { stage_fill(w, g, val, s); });
-            if (w.cap.b[0]) capture("cut k=2", [&] { launch_cut<CK2>(w, g, val, 0, grid[0], s); });
-            if (w.cap.b[1]) capture("cut k=3..4", [&] { launch_cut<CMed>(w, g, val, 1, grid[1], s); });
-            if (w.cap.b[2]) capture("cut k=5..8", [&] { launch_cut<CBig>(w, g, val, 2, grid[2], s); });
+            auto cut_label = [&](const char* base, int q) {
+                return std::string(base) + (grid[q] < 0 ? " (wave per tet)"
+                                                        : " (thread per tet, " + std::to_string(P.blocks_per_cu[q]) +
+                                                              " blocks/CU)");
+            };
+            if (w.cap.b[0]) capture(cut_label("cut k=2", 0), [&] { launch_cut<CK2>(w, g, val, 0, grid[0], s); });
+            if (w.cap.b[1]) capture(cut_label("cut k=3..4", 1), [&] { launch_cut<CMed>(w, g, val, 1, grid[1], s); });
+            if (w.cap.b[2]) capture(cut_label("cut k=5..8", 2), [&] { launch_cut<CBig>(w, g, val, 2, grid[2], s); });
             capture("extract + dedup + xyz + faces", [&] { stage_rest(w, g, val, s); });
             while (w.ev.size() < w.parts.size() + 1) {
                 hipEvent_t e;

@@ -1,0 +1,492 @@
+// Wave-cooperative implicit-arrangement cut: one 64-lane wavefront builds one
+// tet's complex, held in LDS.  Every insertion runs the steps of ia_cut
+// (cut_dev.h) and produces the same numbering; the work of each step is
+// spread over the lanes:
+//   * vertex classification (the exact predicates): one vertex per lane;
+//   * crossing vertices: the strict sign-change edges are listed in
+//     (face, edge) order by a prefix sum over faces (a convex face has at
+//     most two), the first occurrence of each edge creates its vertex, and a
+//     second prefix sum numbers the new vertices in that order;
+//   * face splits: one face per lane, negative parts appended in face order;
+//   * cell splits: flags by LDS atomics, split cells numbered in cell order,
+//     one new face per split cell chained by one lane.
+// Blocks are exactly one wavefront, so __syncthreads() is a wave barrier.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <climits>
+#include <cstdint>
+
+#include "cut_dev.h"
+
+namespace rin {
+namespace dev {
+
+// C: the thread-cut capacities (vertices, faces, vertices per face, labels);
+// cells are bounded as in ia_cut (8 per label, at most 254)
+template <class C>
+struct WaveIA {
+    static constexpr int MV = C::MV, MF = C::MF, MFV = C::MFV, MP = C::MP;
+    static constexpr int MC = C::MP * 8 < 254 ? C::MP * 8 : 254;
+    static constexpr int CAND = 2 * C::MF;
+    double P[MP - 4][4];
+    uint8_t vl[MV][3];
+    int8_t s[MV];
+    uint8_t fa[MF], fb[MF], fc[MF], fn[MF];
+    uint8_t fv[MF][MFV];
+    uint8_t fe[MF][MFV];
+    uint8_t grp[MP], ori[MP];
+    uint8_t fflag[MF];  // bit 0: a vertex with s > 0, bit 1: s < 0
+    uint16_t fcand[MF];
+    uint16_t frank[MF];
+    uint16_t foff[MF];
+    uint32_t cflag[MC];
+    uint16_t crank[MC];
+    uint16_t ckey[CAND];
+    uint16_t cface[CAND];
+    uint8_t cedge[CAND], cfirst[CAND], cvid[CAND];
+    uint8_t isov[MV];
+    uint8_t tv[64][MFV], te[64][MFV], tb[64][MFV];  // per-lane rows
+    int nv, nf, nc, dup, err;
+};
+
+__device__ inline int wave_lane() { return threadIdx.x & 63; }
+
+__device__ inline uint32_t wave_excl(uint32_t x) {
+    const int lane = wave_lane();
+    uint32_t v = x;
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t o = __shfl_up(v, d, 64);
+        if (lane >= d) v += o;
+    }
+    return v - x;
+}
+__device__ inline uint32_t wave_sum(uint32_t x) {
+    for (int d = 32; d > 0; d >>= 1) x += __shfl_xor(x, d, 64);
+    return x;
+}
+__device__ inline int wave_min(int x) {
+    for (int d = 32; d > 0; d >>= 1) {
+        const int o = __shfl_xor(x, d, 64);
+        x = o < x ? o : x;
+    }
+    return x;
+}
+
+// exclusive prefix of count(i) over items 0..n-1, item i on lane i % 64;
+// store(i, offset) runs on that same lane; returns the total (uniform)
+template <class F, class G>
+__device__ inline uint32_t wave_prefix(int n, F count, G store) {
+    uint32_t base = 0;
+    for (int r0 = 0; r0 < n; r0 += 64) {
+        const int i = r0 + wave_lane();
+        const uint32_t c = i < n ? count(i) : 0u;
+        const uint32_t pre = wave_excl(c);
+        if (i < n) store(i, base + pre);
+        base += wave_sum(c);
+    }
+    return base;
+}
+
+// one side (sgn = +1 / -1) of a face split, streamed over the face's cycle
+// with the crossing vertices inserted; the cycle starts at the first kept
+// position and an edge between two kept positions that were not adjacent
+// lies on the new plane.  Returns the vertex count, or -1 past MFV.
+template <class C>
+__device__ inline int wave_split_side(const WaveIA<C>& X, int f, int sgn, uint8_t pid, uint8_t* dv, uint8_t* de) {
+    const int n = X.fn[f];
+    int cnt = 0, p = 0, first = -1, prev = -1;
+    uint8_t pv = 0, pe = 0;
+    uint32_t ci = X.fcand[f];
+    bool over = false;
+    auto visit = [&](uint8_t v, uint8_t e) {
+        if (X.s[v] * sgn >= 0) {
+            if (prev >= 0) {
+                if (cnt >= C::MFV)
+                    over = true;
+                else {
+                    dv[cnt] = pv;
+                    de[cnt] = (prev + 1 == p) ? pe : pid;
+                }
+                ++cnt;
+            } else {
+                first = p;
+            }
+            prev = p;
+            pv = v;
+            pe = e;
+        }
+        ++p;
+    };
+    for (int i = 0; i < n; ++i) {
+        const uint8_t u = X.fv[f][i], w = X.fv[f][i + 1 == n ? 0 : i + 1], e = X.fe[f][i];
+        visit(u, e);
+        if (X.s[u] * X.s[w] < 0) visit(X.cvid[ci++], e);
+    }
+    if (prev >= 0) {
+        if (cnt >= C::MFV)
+            over = true;
+        else {
+            dv[cnt] = pv;
+            de[cnt] = ((prev + 1) % p == first) ? pe : pid;
+        }
+        ++cnt;
+    }
+    return over ? -1 : cnt;
+}
+
+// Runs the insertion of k planes (X.P[0..k-1], loaded by the caller) into
+// the tet; on return X.err is set when a capacity or the predicates'
+// exponent range was exceeded.
+template <class C>
+__device__ void wave_ia_cut(WaveIA<C>& X, int k) {
+    const int lane = wave_lane();
+    if (lane < 4) {
+        const int i = lane;
+        for (int q = 0; q < 3; ++q) X.vl[i][q] = (uint8_t)(q < i ? q : q + 1);  // corner i: planes != i
+        X.fa[i] = (uint8_t)i;
+        X.fb[i] = 0;
+        X.fc[i] = NIL;
+        X.fn[i] = 3;
+    }
+    if (lane == 0) {
+        const uint8_t cyc[4][3] = {{1, 3, 2}, {0, 2, 3}, {0, 3, 1}, {0, 1, 2}};
+        for (int i = 0; i < 4; ++i)
+            for (int q = 0; q < 3; ++q) {
+                X.fv[i][q] = cyc[i][q];
+                X.fe[i][q] = cyc[i][(q + 2) % 3];
+            }
+        X.nv = 4;
+        X.nf = 4;
+        X.nc = 1;
+        X.dup = 0;
+        X.err = (4 + k > C::MP) ? 1 : 0;
+    }
+    if (lane < C::MP) {
+        X.grp[lane] = lane < 4 ? (uint8_t)lane : NIL;
+        X.ori[lane] = 1;
+    }
+    __syncthreads();
+    if (X.err) return;
+    auto fail = [&]() {
+        if (lane == 0) X.err = 1;
+        __syncthreads();
+    };
+    for (int j = 0; j < k; ++j) {
+        const uint8_t pid = (uint8_t)(4 + j);
+        const int nv0 = X.nv, nf0 = X.nf, nc0 = X.nc;
+
+        // classify every vertex against plane j
+        bool e = false, ap = false, an = false;
+        for (int v = lane; v < nv0; v += 64) {
+            const int sv = orient_ia(X.P, X.vl[v][0], X.vl[v][1], X.vl[v][2], X.P[j], e);
+            X.s[v] = (int8_t)sv;
+            ap |= sv > 0;
+            an |= sv < 0;
+        }
+        const bool any_err = __ballot(e) != 0, any_pos = __ballot(ap) != 0, any_neg = __ballot(an) != 0;
+        if (any_err) return fail();
+        __syncthreads();
+
+        // a plane containing an existing face joins that face's group
+        int cop = INT_MAX;
+        for (int f = lane; f < nf0; f += 64) {
+            bool allz = true;
+            for (int q = 0; q < X.fn[f]; ++q) allz &= X.s[X.fv[f][q]] == 0;
+            if (allz && f < cop) cop = f;
+        }
+        cop = wave_min(cop);
+        if (cop != INT_MAX) {
+            if (lane == 0) {
+                const uint8_t cell = X.fb[cop];
+                int sv = 0;
+                for (int g = 0; g < nf0 && sv == 0; ++g) {
+                    if (X.fb[g] != cell && X.fc[g] != cell) continue;
+                    for (int q = 0; q < X.fn[g]; ++q)
+                        if (X.s[X.fv[g][q]] != 0) {
+                            sv = X.s[X.fv[g][q]];
+                            break;
+                        }
+                }
+                const uint8_t pl = X.fa[cop];
+                X.grp[pid] = X.grp[pl];
+                X.ori[pid] = (sv > 0) ? X.ori[pl] : (uint8_t)(1 - X.ori[pl]);
+                X.dup = 1;
+            }
+            __syncthreads();
+            continue;
+        }
+        if (lane == 0) {
+            X.grp[pid] = pid;
+            X.ori[pid] = 1;
+        }
+        if (!any_pos || !any_neg) {
+            __syncthreads();
+            continue;
+        }
+
+        // strict sign-change edges in (face, edge) order
+        const uint32_t ncand = wave_prefix(
+            nf0,
+            [&](int f) {
+                uint32_t c = 0;
+                const int n = X.fn[f];
+                for (int i = 0; i < n; ++i) c += X.s[X.fv[f][i]] * X.s[X.fv[f][i + 1 == n ? 0 : i + 1]] < 0;
+                return c;
+            },
+            [&](int f, uint32_t o) { X.fcand[f] = (uint16_t)o; });
+        if (ncand > (uint32_t)WaveIA<C>::CAND) return fail();
+        for (int f = lane; f < nf0; f += 64) {
+            uint32_t o = X.fcand[f];
+            const int n = X.fn[f];
+            for (int i = 0; i < n; ++i) {
+                const uint8_t u = X.fv[f][i], w = X.fv[f][i + 1 == n ? 0 : i + 1];
+                if (X.s[u] * X.s[w] >= 0) continue;
+                X.ckey[o] = (uint16_t)((u < w ? u : w) << 8 | (u < w ? w : u));
+                X.cface[o] = (uint16_t)f;
+                X.cedge[o] = (uint8_t)i;
+                ++o;
+            }
+        }
+        __syncthreads();
+        for (int c = lane; c < (int)ncand; c += 64) {
+            const uint16_t key = X.ckey[c];
+            uint8_t first = 1;
+            for (int d = 0; d < c; ++d)
+                if (X.ckey[d] == key) {
+                    first = 0;
+                    break;
+                }
+            X.cfirst[c] = first;
+        }
+        __syncthreads();
+        const uint32_t nnew = wave_prefix(
+            (int)ncand, [&](int c) { return (uint32_t)X.cfirst[c]; },
+            [&](int c, uint32_t o) {
+                if (X.cfirst[c]) X.cvid[c] = (uint8_t)(nv0 + o);
+            });
+        if (nv0 + (int)nnew > C::MV) return fail();
+        __syncthreads();
+        for (int c = lane; c < (int)ncand; c += 64) {
+            if (X.cfirst[c]) {
+                const int f = X.cface[c];
+                uint8_t t0 = X.fa[f], t1 = X.fe[f][X.cedge[c]], t2 = pid;
+                sort3(t0, t1, t2);
+                const int id = X.cvid[c];
+                X.vl[id][0] = t0;
+                X.vl[id][1] = t1;
+                X.vl[id][2] = t2;
+                X.s[id] = 0;
+            } else {
+                const uint16_t key = X.ckey[c];
+                for (int d = 0; d < c; ++d)
+                    if (X.ckey[d] == key) {
+                        X.cvid[c] = X.cvid[d];
+                        break;
+                    }
+            }
+        }
+        __syncthreads();
+
+        // face splits: the positive part keeps the id, the negative part is
+        // appended in face order
+        const uint32_t nmix = wave_prefix(
+            nf0,
+            [&](int f) {
+                uint8_t fl = 0;
+                for (int i = 0; i < X.fn[f]; ++i) {
+                    const int sv = X.s[X.fv[f][i]];
+                    fl |= sv > 0 ? 1 : (sv < 0 ? 2 : 0);
+                }
+                X.fflag[f] = fl;
+                return (uint32_t)(fl == 3);
+            },
+            [&](int f, uint32_t o) { X.frank[f] = (uint16_t)o; });
+        if (nf0 + (int)nmix > C::MF) return fail();
+        bool ferr = false;
+        for (int f = lane; f < nf0; f += 64) {
+            if (X.fflag[f] != 3) continue;
+            const int dn = nf0 + X.frank[f];
+            const int nneg = wave_split_side<C>(X, f, -1, pid, X.fv[dn], X.fe[dn]);
+            const int npos = wave_split_side<C>(X, f, +1, pid, X.tv[lane], X.te[lane]);
+            if (nneg < 0 || npos < 0) {
+                ferr = true;
+                continue;
+            }
+            X.fn[dn] = (uint8_t)nneg;
+            X.fa[dn] = X.fa[f];
+            X.fb[dn] = X.fb[f];
+            X.fc[dn] = X.fc[f];
+            for (int q = 0; q < npos; ++q) {
+                X.fv[f][q] = X.tv[lane][q];
+                X.fe[f][q] = X.te[lane][q];
+            }
+            X.fn[f] = (uint8_t)npos;
+        }
+        if (__ballot(ferr)) return fail();
+        const int nf1 = nf0 + (int)nmix;
+        for (int c = lane; c < nc0; c += 64) X.cflag[c] = 0;
+        __syncthreads();
+
+        // cell splits: the positive side keeps the id, the negative side and
+        // one face on the new plane are appended in cell order
+        for (int g = lane; g < nf1; g += 64) {
+            uint8_t fl = 0;
+            for (int i = 0; i < X.fn[g]; ++i) {
+                const int sv = X.s[X.fv[g][i]];
+                fl |= sv > 0 ? 1 : (sv < 0 ? 2 : 0);
+            }
+            X.fflag[g] = fl;
+            if (fl) {
+                atomicOr(&X.cflag[X.fb[g]], (uint32_t)fl);
+                if (X.fc[g] != NIL) atomicOr(&X.cflag[X.fc[g]], (uint32_t)fl);
+            }
+        }
+        __syncthreads();
+        const uint32_t nsplit = wave_prefix(
+            nc0, [&](int c) { return (uint32_t)(X.cflag[c] == 3); },
+            [&](int c, uint32_t o) { X.crank[c] = X.cflag[c] == 3 ? (uint16_t)o : (uint16_t)0xFFFF; });
+        if (nf1 + (int)nsplit > C::MF || nc0 + (int)nsplit > WaveIA<C>::MC) return fail();
+        __syncthreads();
+        for (int g = lane; g < nf1; g += 64) {
+            if (!(X.fflag[g] & 2)) continue;
+            const uint8_t b = X.fb[g], c = X.fc[g];
+            if (X.crank[b] != 0xFFFF) X.fb[g] = (uint8_t)(nc0 + X.crank[b]);
+            if (c != NIL && X.crank[c] != 0xFFFF) X.fc[g] = (uint8_t)(nc0 + X.crank[c]);
+        }
+        __syncthreads();
+        bool cerr = false;
+        for (int c = lane; c < nc0 && !cerr; c += 64) {
+            if (X.crank[c] == 0xFFFF) continue;
+            uint8_t* ea = X.tv[lane];
+            uint8_t* eb = X.tb[lane];
+            uint8_t* el = X.te[lane];
+            int ne = 0;
+            for (int g = 0; g < nf1 && !cerr; ++g) {
+                const bool pos_side = X.fb[g] == c;
+                if (!pos_side && X.fc[g] != c) continue;
+                const int n = X.fn[g];
+                for (int i = 0; i < n; ++i) {
+                    const uint8_t a = X.fv[g][i], b = X.fv[g][i + 1 == n ? 0 : i + 1];
+                    if (X.s[a] != 0 || X.s[b] != 0) continue;
+                    if (ne >= C::MFV) {
+                        cerr = true;
+                        break;
+                    }
+                    ea[ne] = pos_side ? b : a;
+                    eb[ne] = pos_side ? a : b;
+                    el[ne] = X.fa[g];
+                    ++ne;
+                }
+            }
+            if (cerr || ne < 3) {
+                cerr = true;
+                break;
+            }
+            const int F = nf1 + X.crank[c];
+            int cur = 0, cnt = 0;
+            for (;;) {
+                X.fv[F][cnt] = ea[cur];
+                X.fe[F][cnt] = el[cur];
+                ++cnt;
+                int nxt = -1;
+                for (int q = 0; q < ne; ++q)
+                    if (ea[q] == eb[cur]) {
+                        nxt = q;
+                        break;
+                    }
+                if (nxt < 0 || cnt > ne) {
+                    cerr = true;
+                    break;
+                }
+                cur = nxt;
+                if (cur == 0) break;
+            }
+            if (cerr || cnt != ne) {
+                cerr = true;
+                break;
+            }
+            X.fn[F] = (uint8_t)cnt;
+            X.fa[F] = pid;
+            X.fb[F] = (uint8_t)c;
+            X.fc[F] = (uint8_t)(nc0 + X.crank[c]);
+        }
+        if (__ballot(cerr)) return fail();
+        __syncthreads();
+        if (lane == 0) {
+            X.nv = nv0 + (int)nnew;
+            X.nf = nf1 + (int)nsplit;
+            X.nc = nc0 + (int)nsplit;
+        }
+        __syncthreads();
+    }
+}
+
+// Writes the record (include/rin/cut_record.h layout) with the whole wave and
+// returns its length, or -1 when it exceeds cap; niv / nif / nfv are the
+// tet's iso-vertex, iso-face and iso-face-corner counts (uniform).
+template <class C>
+__device__ int wave_ia_record(WaveIA<C>& X, int k, uint16_t* out, int cap, uint32_t& niv, uint32_t& nif,
+                              uint32_t& nfv) {
+    const int lane = wave_lane();
+    const int L = 4 + k, nv = X.nv, nf = X.nf;
+    const uint32_t fwords = wave_prefix(
+        nf, [&](int f) { return (uint32_t)(4 + X.fn[f]); }, [&](int f, uint32_t o) { X.foff[f] = (uint16_t)o; });
+    const int total = 8 + 4 * nv + (int)fwords + (X.dup ? 2 * L : 0);
+    for (int v = lane; v < nv; v += 64) X.isov[v] = 0;
+    __syncthreads();
+    uint32_t cf = 0, cfv = 0;
+    for (int f = lane; f < nf; f += 64) {
+        bool is = X.fa[f] > 3;
+        if (!is && X.dup) {
+            const uint8_t rep = X.grp[X.fa[f]];
+            for (int q = 4; q < L; ++q) is |= X.grp[q] == rep;
+        }
+        if (!is) continue;
+        ++cf;
+        cfv += X.fn[f];
+        for (int q = 0; q < X.fn[f]; ++q) X.isov[X.fv[f][q]] = 1;
+    }
+    nif = wave_sum(cf);
+    nfv = wave_sum(cfv);
+    __syncthreads();
+    uint32_t cv = 0;
+    for (int v = lane; v < nv; v += 64) cv += X.isov[v];
+    niv = wave_sum(cv);
+    if (total > cap) return -1;
+    for (int v = lane; v < nv; v += 64) {
+        uint16_t* o = out + 8 + 4 * v;
+        o[0] = X.vl[v][0];
+        o[1] = X.vl[v][1];
+        o[2] = X.vl[v][2];
+        o[3] = 0xFFFF;
+    }
+    const int fb0 = 8 + 4 * nv;
+    for (int f = lane; f < nf; f += 64) {
+        uint16_t* o = out + fb0 + X.foff[f];
+        o[0] = X.fa[f];
+        o[1] = X.fb[f];
+        o[2] = X.fc[f] == NIL ? 0xFFFF : X.fc[f];
+        o[3] = X.fn[f];
+        for (int q = 0; q < X.fn[f]; ++q) o[4 + q] = X.fv[f][q];
+    }
+    if (X.dup)
+        for (int l = lane; l < L; l += 64) {
+            out[fb0 + fwords + l] = X.grp[l];
+            out[fb0 + fwords + L + l] = X.ori[l];
+        }
+    if (lane == 0) {
+        out[0] = 1;
+        out[1] = (uint16_t)L;
+        out[2] = (uint16_t)nv;
+        out[3] = (uint16_t)nf;
+        out[4] = (uint16_t)X.nc;
+        out[5] = X.dup ? 1 : 0;
+        out[6] = (uint16_t)total;
+        out[7] = 0;
+    }
+    return total;
+}
+
+}  // namespace dev
+}  // namespace rin

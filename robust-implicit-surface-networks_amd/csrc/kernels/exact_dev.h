@@ -189,7 +189,7 @@ __host__ __device__ inline int det_exp(const double (&M)[N][N], const int* cols,
 }
 
 template <int N>
-__host__ __device__ inline int det_exact(const double (&M)[N][N], bool& err) {
+__host__ __device__ __noinline__ int det_exact(const double (&M)[N][N], bool& err) {
     int cols[N];
     for (int c = 0; c < N; ++c) cols[c] = c;
     double out[kCap];

@@ -16,6 +16,7 @@
 #include <map>
 #include <stdexcept>
 
+#include "cut_wave.h"
 #include "kernels_ia.h"
 
 namespace rin {
@@ -499,6 +500,26 @@ __global__ void k_debug_cut(const double* rows, const int* ks, const int* starts
     lens[i] = MI ? dev::write_record_mi<C>(X, k, words + (size_t)i * cap, cap)
                  : dev::write_record_ia<C>(X, k, words + (size_t)i * cap, cap);
 }
+template <class C>
+__global__ void __launch_bounds__(64) k_debug_cut_wave(const double* rows, const int* ks, const int* starts, int ncase,
+                                                       int cap, uint16_t* words, int* lens) {
+    __shared__ dev::WaveIA<C> X;
+    const int i = blockIdx.x, lane = threadIdx.x;
+    const int k = ks[i];
+    if (k > C::MP - 4) {
+        if (lane == 0) lens[i] = -2;
+        return;
+    }
+    if (lane < 4 * k) X.P[lane >> 2][lane & 3] = rows[4 * starts[i] + lane];
+    dev::wave_ia_cut<C>(X, k);
+    if (X.err) {
+        if (lane == 0) lens[i] = -1;
+        return;
+    }
+    uint32_t a, b, c;
+    const int len = dev::wave_ia_record<C>(X, k, words + (size_t)i * cap, cap, a, b, c);
+    if (lane == 0) lens[i] = len;
+}
 }  // namespace
 
 void debug_cuts(DevMem& mem, int mi, const double* rows, const int* ks, int ncase, int cap, uint16_t* words, int* lens,
@@ -516,7 +537,10 @@ void debug_cuts(DevMem& mem, int mi, const double* rows, const int* ks, int ncas
     RIN_HIP(hipMemcpyAsync(dks.p, ks, 4 * (size_t)ncase, hipMemcpyHostToDevice, s));
     RIN_HIP(hipMemcpyAsync(dst.p, starts.data(), 4 * (size_t)ncase, hipMemcpyHostToDevice, s));
     const unsigned nb = (unsigned)((ncase + 63) / 64);
-    if (mi)
+    if (mi == 2)
+        hipLaunchKernelGGL((k_debug_cut_wave<CBig>), dim3((unsigned)ncase), dim3(64), 0, s, (const double*)drows.p,
+                           (const int*)dks.p, (const int*)dst.p, ncase, cap, dw.p, dlens.p);
+    else if (mi)
         hipLaunchKernelGGL((k_debug_cut<CBig, true>), dim3(nb), dim3(64), 0, s, (const double*)drows.p,
                            (const int*)dks.p, (const int*)dst.p, ncase, cap, dw.p, dlens.p);
     else

@@ -22,14 +22,16 @@ def _cases(rng, k, n, degenerate):
     return out
 
 
-@pytest.mark.parametrize("mi", [0, 1])
+# mode 0: IA thread-per-tet cut, 1: MI thread-per-tet cut, 2: IA wave-cooperative cut
+@pytest.mark.parametrize("mode", [0, 1, 2])
 @pytest.mark.parametrize("degenerate", [False, True])
-def test_cut_records_match_oracle(ctx, mi, degenerate):
+def test_cut_records_match_oracle(ctx, mode, degenerate):
+    mi = 1 if mode == 1 else 0
     rng = np.random.default_rng(100 + 10 * mi + degenerate)
     cases = []
-    for k in range(1, 7):
+    for k in range(1, 9 if mode == 2 else 7):
         cases += _cases(rng, k, 120 if k < 5 else 40, degenerate)
-    dev = ctx.debug_cuts(mi, cases)
+    dev = ctx.debug_cuts(mode, cases)
     checked = 0
     for i, rows in enumerate(cases):
         try:
