@@ -69,6 +69,7 @@ def main():
     ap.add_argument("--res", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--stages", action="store_true", help="print per-stage device times to stderr")
+    ap.add_argument("--form", choices=["both", "plain", "batched"], default="both")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU; this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -91,6 +92,8 @@ def main():
         return r
 
     modes = {"plain": 0, "batched": rin.FLAG_BATCHED}
+    if args.form != "both":
+        modes = {args.form: modes[args.form]}
     results = {}
     for name, flags in modes.items():
         for _ in range(args.warmup):
