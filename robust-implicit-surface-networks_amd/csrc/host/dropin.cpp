@@ -1,0 +1,454 @@
+// Drop-in implicit_arrangement() / material_interface() (declared in
+// include/rin/implicit_arrangement.h): the per-tet path on the GPU through the
+// C ABI, the graph stages on the host (graph.cpp).
+//
+// Driver order and reported labels follow src/implicit_arrangement.cpp:13-647
+// and src/material_interface.cpp:13-694.
+#include <chrono>
+#include <cstring>
+#include <iostream>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "cut_view.h"
+#include "graph.h"
+#include "rin.h"
+#include "rin/implicit_arrangement.h"
+
+namespace {
+
+thread_local int t_device = 0;
+
+// one context per host thread (contexts are independent)
+struct Ctx {
+    rin_ctx* c = nullptr;
+    int device = -1;
+    ~Ctx() {
+        if (c) rin_destroy(c);
+    }
+    rin_ctx* get() {
+        if (!c || device != t_device) {
+            if (c) rin_destroy(c);
+            c = nullptr;
+            if (rin_create(&c, t_device) != RIN_OK) throw std::runtime_error("rin_create failed");
+            device = t_device;
+        }
+        return c;
+    }
+};
+thread_local Ctx t_ctx;
+
+struct Failure : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+void check(rin_ctx* c, int st, const char* what) {
+    if (st != RIN_OK) throw Failure(std::string(what) + ": " + rin_last_error(c));
+}
+
+// RAII owners of the C ABI handles
+struct Handles {
+    rin_ctx* c;
+    rin_grid* g = nullptr;
+    rin_values* v = nullptr;
+    std::vector<rin_result*> res;
+    explicit Handles(rin_ctx* ctx) : c(ctx) {}
+    ~Handles() {
+        for (auto* r : res) rin_result_release(c, r);
+        if (v) rin_values_release(c, v);
+        if (g) rin_grid_release(c, g);
+    }
+    rin_result* run(uint32_t flags) {
+        rin_result* r = nullptr;
+        check(c, rin_run(c, g, v, flags, &r), "rin_run");
+        res.push_back(r);
+        return r;
+    }
+};
+
+// per-tet complexes fetched from the device result on demand, in batches
+struct DeviceCuts : rin::CutSource {
+    rin_ctx* c;
+    const rin_result* r;
+    std::unordered_map<size_t, rin::CutView> cache;
+    DeviceCuts(rin_ctx* ctx, const rin_result* res) : c(ctx), r(res) {}
+    void fetch(const std::vector<size_t>& tets) {
+        std::vector<uint64_t> need;
+        for (size_t t : tets)
+            if (!cache.count(t)) need.push_back(t);
+        if (need.empty()) return;
+        std::vector<uint64_t> start(need.size() + 1);
+        // records are at most a few thousand words; size generously, retry once
+        std::vector<uint16_t> words(need.size() * 1024 + 64);
+        int st = rin_result_records(c, r, need.data(), need.size(), start.data(), words.data(), words.size());
+        if (st == RIN_ERR_ARG && std::string(rin_last_error(c)).find("too small") != std::string::npos) {
+            words.assign(need.size() * 8192 + 64, 0);
+            st = rin_result_records(c, r, need.data(), need.size(), start.data(), words.data(), words.size());
+        }
+        check(c, st, "rin_result_records");
+        for (size_t i = 0; i < need.size(); ++i)
+            if (start[i + 1] > start[i]) cache.emplace(need[i], rin::decode_cut(words.data() + start[i]));
+    }
+    const rin::CutView& get(size_t tet) override {
+        auto it = cache.find(tet);
+        if (it == cache.end()) {
+            fetch({tet});
+            it = cache.find(tet);
+            if (it == cache.end()) throw std::runtime_error("cut requested for an inactive tet");
+        }
+        return it->second;
+    }
+    void prefetch(const std::vector<size_t>& tets) override { fetch(tets); }
+};
+
+bool is_row_major_ok(const rin::RowMajorValues& f, size_t n_pts) {
+    return f.data != nullptr && f.rows == n_pts && f.cols > 0;
+}
+
+void append_stage_times(rin_ctx* c, const rin_result* r, std::vector<std::string>& labels,
+                        std::vector<double>& secs) {
+    const int n = rin_result_stage_times(c, r, nullptr, 0);
+    std::vector<double> ms(n > 0 ? n : 0);
+    if (n > 0) rin_result_stage_times(c, r, ms.data(), n);
+    for (int i = 0; i < n; ++i) {
+        labels.emplace_back(rin_result_stage_label(c, r, i));
+        secs.push_back(ms[i] * 1e-3);
+    }
+}
+
+// -R check: per-tet (vertex, face, cell) counts forward vs reversed
+// (src/implicit_arrangement.cpp:136-243, src/material_interface.cpp:174-287)
+bool robust_check(Handles& H, uint32_t mode, std::vector<std::string>& tl, std::vector<double>& tv) {
+    rin_result* fwd = nullptr;
+    int st = rin_run(H.c, H.g, H.v, mode | RIN_FLAG_CUT_ONLY, &fwd);
+    if (st != RIN_OK) {
+        std::cout << "type 2 failure (crash in the normal order)." << std::endl;
+        return false;
+    }
+    H.res.push_back(fwd);
+    append_stage_times(H.c, fwd, tl, tv);
+    rin_result* rev = nullptr;
+    st = rin_run(H.c, H.g, H.v, mode | RIN_FLAG_CUT_ONLY | RIN_FLAG_REVERSE, &rev);
+    if (st != RIN_OK) {
+        std::cout << "type 3 failure (crash in the reverse order)." << std::endl;
+        return false;
+    }
+    H.res.push_back(rev);
+    rin_sizes s{};
+    check(H.c, rin_result_sizes(H.c, fwd, &s), "rin_result_sizes");
+    std::vector<uint64_t> a(3 * s.num_intersecting), b(3 * s.num_intersecting);
+    check(H.c, rin_result_cut_counts(H.c, fwd, a.data()), "rin_result_cut_counts");
+    check(H.c, rin_result_cut_counts(H.c, rev, b.data()), "rin_result_cut_counts");
+    if (a != b) {
+        std::cout << "type 1 failure (inconsistency)." << std::endl;
+        return false;
+    }
+    std::cout << "success." << std::endl;
+    return true;
+}
+
+struct DeviceMesh {
+    rin_sizes s{};
+    std::vector<uint64_t> fit, start, rec11, vstart, verts, tfstart, tf, func;
+    std::vector<double> xyz;
+};
+
+void fetch_mesh(rin_ctx* c, const rin_result* r, DeviceMesh& m) {
+    check(c, rin_result_sizes(c, r, &m.s), "rin_result_sizes");
+    m.fit.resize(m.s.n_func_in_tet + 1);
+    m.start.resize(m.s.n_tets + 1);
+    check(c, rin_result_csr(c, r, m.fit.data(), m.start.data()), "rin_result_csr");
+    m.rec11.resize(11 * m.s.n_verts + 1);
+    m.xyz.resize(3 * m.s.n_verts + 1);
+    check(c, rin_result_verts(c, r, m.rec11.data(), m.xyz.data()), "rin_result_verts");
+    m.vstart.resize(m.s.n_faces + 1);
+    m.verts.resize(m.s.n_face_verts + 1);
+    m.tfstart.resize(m.s.n_faces + 1);
+    m.tf.resize(2 * m.s.n_tet_faces + 1);
+    m.func.resize(2 * m.s.n_faces + 1);
+    check(c, rin_result_faces(c, r, m.vstart.data(), m.verts.data(), m.tfstart.data(), m.tf.data(), m.func.data()),
+          "rin_result_faces");
+}
+
+void to_faces(const DeviceMesh& m, std::vector<PolygonFace>& faces) {
+    const size_t base = faces.size();
+    faces.resize(base + m.s.n_faces);
+    for (uint64_t f = 0; f < m.s.n_faces; ++f) {
+        PolygonFace& P = faces[base + f];
+        P.vert_indices.assign(m.verts.begin() + m.vstart[f], m.verts.begin() + m.vstart[f + 1]);
+        for (uint64_t q = m.tfstart[f]; q < m.tfstart[f + 1]; ++q) P.tet_face_indices.push_back({m.tf[2 * q], m.tf[2 * q + 1]});
+        P.func_index = {m.func[2 * f], m.func[2 * f + 1]};
+    }
+}
+
+template <class T>
+void push_stat(std::vector<std::string>& sl, std::vector<size_t>& sv, const char* label, T v) {
+    sl.emplace_back(label);
+    sv.push_back((size_t)v);
+}
+
+}  // namespace
+
+namespace rin {
+void set_device(int device) { t_device = device; }
+}  // namespace rin
+
+bool implicit_arrangement(bool robust_test, bool use_lookup, bool use_secondary_lookup, bool use_topo_ray_shooting,
+                          const std::vector<std::array<double, 3>>& pts,
+                          const std::vector<std::array<size_t, 4>>& tets, const rin::RowMajorValues& funcVals,
+                          std::vector<std::array<double, 3>>& iso_pts, std::vector<PolygonFace>& iso_faces,
+                          std::vector<std::vector<size_t>>& patches, std::vector<size_t>& patch_function_label,
+                          std::vector<Edge>& iso_edges, std::vector<std::vector<size_t>>& chains,
+                          std::vector<std::vector<size_t>>& non_manifold_edges_of_vert,
+                          std::vector<std::vector<size_t>>& shells,
+                          std::vector<std::vector<size_t>>& arrangement_cells,
+                          std::vector<std::vector<bool>>& cell_function_label, std::vector<std::string>& timing_labels,
+                          std::vector<double>& timings, std::vector<std::string>& stats_labels,
+                          std::vector<size_t>& stats) {
+    (void)use_secondary_lookup;   // the 2-function table changes speed, not results
+    (void)use_topo_ray_shooting;  // cells always come from topological ray shooting
+    const size_t n_pts = pts.size(), n_tets = tets.size();
+    std::cout << "tet mesh: " << n_pts << " verts, " << n_tets << " tets." << std::endl;
+    push_stat(stats_labels, stats, "num_pts", n_pts);
+    push_stat(stats_labels, stats, "num_tets", n_tets);
+    if (!is_row_major_ok(funcVals, n_pts)) {
+        std::cout << "funcVals must be (num_pts x num_funcs)" << std::endl;
+        return false;
+    }
+    const size_t F = funcVals.cols;
+    try {
+        rin_ctx* c = t_ctx.get();
+        Handles H(c);
+        std::vector<uint64_t> t64(4 * n_tets);
+        for (size_t t = 0; t < n_tets; ++t)
+            for (int q = 0; q < 4; ++q) t64[4 * t + q] = tets[t][q];
+        check(c, rin_grid_explicit(c, &pts[0][0], n_pts, t64.data(), n_tets, &H.g), "rin_grid_explicit");
+        check(c, rin_values_upload(c, H.g, funcVals.data, (uint32_t)F, &H.v), "rin_values_upload");
+        const uint32_t lookup = use_lookup ? 0u : RIN_FLAG_NO_LOOKUP;
+        if (robust_test) {
+            const bool ok = robust_check(H, lookup, timing_labels, timings);
+            rin_sizes s{};
+            check(c, rin_result_sizes(c, H.res[0], &s), "rin_result_sizes");
+            push_stat(stats_labels, stats, "num_degenerate_vertex", s.num_degenerate);
+            push_stat(stats_labels, stats, "num_intersecting_tet", s.num_intersecting);
+            push_stat(stats_labels, stats, "num_1_func", s.bucket[0]);
+            push_stat(stats_labels, stats, "num_2_func", s.bucket[1]);
+            push_stat(stats_labels, stats, "num_more_func", s.bucket[2]);
+            return ok;
+        }
+        rin_result* r = H.run(RIN_FLAG_BATCHED | RIN_FLAG_KEEP_RECORDS | lookup);
+        append_stage_times(c, r, timing_labels, timings);
+        DeviceMesh m;
+        fetch_mesh(c, r, m);
+        std::cout << "num_degenerate_vertex = " << m.s.num_degenerate << std::endl;
+        std::cout << "num_intersecting_tet = " << m.s.num_intersecting << std::endl;
+        push_stat(stats_labels, stats, "num_degenerate_vertex", m.s.num_degenerate);
+        push_stat(stats_labels, stats, "num_intersecting_tet", m.s.num_intersecting);
+        push_stat(stats_labels, stats, "num_1_func", m.s.bucket[0]);
+        push_stat(stats_labels, stats, "num_2_func", m.s.bucket[1]);
+        push_stat(stats_labels, stats, "num_more_func", m.s.bucket[2]);
+        std::cout << "num iso-vertices = " << m.s.n_verts << std::endl;
+        std::cout << "num iso-faces = " << m.s.n_faces << std::endl;
+        push_stat(stats_labels, stats, "num_iso_verts", m.s.n_verts);
+        push_stat(stats_labels, stats, "num_iso_faces", m.s.n_faces);
+
+        std::vector<IsoVert> iv(m.s.n_verts);
+        for (uint64_t i = 0; i < m.s.n_verts; ++i) {
+            const uint64_t* w = &m.rec11[11 * i];
+            IsoVert& v = iv[i];
+            v.tet_index = w[0];
+            v.tet_vert_index = w[1];
+            v.simplex_size = w[2];
+            for (int q = 0; q < 4; ++q) v.simplex_vert_indices[q] = w[3 + q];
+            for (int q = 0; q < 3; ++q) v.func_indices[q] = w[7 + q];
+        }
+        const size_t p0 = iso_pts.size();
+        iso_pts.resize(p0 + m.s.n_verts);
+        for (uint64_t i = 0; i < m.s.n_verts; ++i)
+            for (int d = 0; d < 3; ++d) iso_pts[p0 + i][d] = m.xyz[3 * i + d];
+        std::vector<PolygonFace> faces;
+        to_faces(m, faces);
+
+        // host graph stages (src/implicit_arrangement.cpp:404-645)
+        std::vector<size_t> fit(m.fit.begin(), m.fit.begin() + m.s.n_func_in_tet);
+        std::vector<size_t> sidx(m.start.begin(), m.start.end());
+        std::vector<uint8_t> degen(n_pts, 0);
+        bool found = false;
+        for (size_t v = 0; v < n_pts; ++v)
+            for (size_t f = 0; f < F; ++f)
+                if (funcVals.data[v * F + f] == 0) {
+                    degen[v] = 1;
+                    found = true;
+                    break;
+                }
+        DeviceCuts cuts(c, r);
+        rin::GraphIn in;
+        in.mi = false;
+        in.pts = &pts;
+        in.tets = &tets;
+        in.iso_verts = &iv;
+        in.faces = &faces;
+        in.func_in_tet = &fit;
+        in.start_index = &sidx;
+        in.degenerate_vertex = &degen;
+        in.found_degenerate = found;
+        in.n_func = F;
+        for (size_t f = 0; f < F; ++f) in.sample_sign.push_back(funcVals.data[f] > 0);
+        in.cuts = &cuts;
+        rin::GraphOut g;
+        const bool ok = rin::run_graph(in, g);
+        for (auto& t : g.timings) {
+            timing_labels.push_back(t.first);
+            timings.push_back(t.second);
+        }
+        for (auto& s : g.stats) push_stat(stats_labels, stats, s.first.c_str(), s.second);
+        iso_faces.insert(iso_faces.end(), faces.begin(), faces.end());
+        iso_edges.insert(iso_edges.end(), g.edges.begin(), g.edges.end());
+        patches.insert(patches.end(), g.patches.begin(), g.patches.end());
+        patch_function_label.insert(patch_function_label.end(), g.patch_label.begin(), g.patch_label.end());
+        chains.insert(chains.end(), g.chains.begin(), g.chains.end());
+        non_manifold_edges_of_vert.insert(non_manifold_edges_of_vert.end(), g.nm_edges_of_vert.begin(),
+                                          g.nm_edges_of_vert.end());
+        shells.insert(shells.end(), g.shells.begin(), g.shells.end());
+        arrangement_cells.insert(arrangement_cells.end(), g.cells.begin(), g.cells.end());
+        cell_function_label.insert(cell_function_label.end(), g.cell_label.begin(), g.cell_label.end());
+        if (!ok) std::cout << g.error << std::endl;
+        return ok;
+    } catch (std::exception& e) {
+        std::cout << e.what() << std::endl;
+        return false;
+    }
+}
+
+bool material_interface(bool robust_test, bool use_lookup, bool use_secondary_lookup, bool use_topo_ray_shooting,
+                        const std::vector<std::array<double, 3>>& pts, const std::vector<std::array<size_t, 4>>& tets,
+                        const rin::RowMajorValues& funcVals, std::vector<std::array<double, 3>>& MI_pts,
+                        std::vector<PolygonFace>& MI_faces, std::vector<std::vector<size_t>>& patches,
+                        std::vector<std::pair<size_t, size_t>>& patch_function_label, std::vector<Edge>& MI_edges,
+                        std::vector<std::vector<size_t>>& chains,
+                        std::vector<std::vector<size_t>>& non_manifold_edges_of_vert,
+                        std::vector<std::vector<size_t>>& shells, std::vector<std::vector<size_t>>& material_cells,
+                        std::vector<size_t>& cell_function_label, std::vector<std::string>& timing_labels,
+                        std::vector<double>& timings, std::vector<std::string>& stats_labels,
+                        std::vector<size_t>& stats) {
+    (void)use_secondary_lookup;
+    (void)use_topo_ray_shooting;
+    const size_t n_pts = pts.size(), n_tets = tets.size();
+    std::cout << "tet mesh: " << n_pts << " verts, " << n_tets << " tets." << std::endl;
+    push_stat(stats_labels, stats, "num_pts", n_pts);
+    push_stat(stats_labels, stats, "num_tets", n_tets);
+    if (!is_row_major_ok(funcVals, n_pts)) {
+        std::cout << "funcVals must be (num_pts x num_funcs)" << std::endl;
+        return false;
+    }
+    const size_t F = funcVals.cols;
+    try {
+        rin_ctx* c = t_ctx.get();
+        Handles H(c);
+        std::vector<uint64_t> t64(4 * n_tets);
+        for (size_t t = 0; t < n_tets; ++t)
+            for (int q = 0; q < 4; ++q) t64[4 * t + q] = tets[t][q];
+        check(c, rin_grid_explicit(c, &pts[0][0], n_pts, t64.data(), n_tets, &H.g), "rin_grid_explicit");
+        check(c, rin_values_upload(c, H.g, funcVals.data, (uint32_t)F, &H.v), "rin_values_upload");
+        const uint32_t mode = RIN_FLAG_MI | (use_lookup ? 0u : RIN_FLAG_NO_LOOKUP);
+        if (robust_test) {
+            const bool ok = robust_check(H, mode, timing_labels, timings);
+            rin_sizes s{};
+            check(c, rin_result_sizes(c, H.res[0], &s), "rin_result_sizes");
+            push_stat(stats_labels, stats, "num_intersecting_tet", s.num_intersecting);
+            push_stat(stats_labels, stats, "num_2_func", s.bucket[0]);
+            push_stat(stats_labels, stats, "num_3_func", s.bucket[1]);
+            push_stat(stats_labels, stats, "num_more_func", s.bucket[2]);
+            return ok;
+        }
+        rin_result* r = H.run(RIN_FLAG_KEEP_RECORDS | mode);
+        append_stage_times(c, r, timing_labels, timings);
+        DeviceMesh m;
+        fetch_mesh(c, r, m);
+        push_stat(stats_labels, stats, "num_intersecting_tet", m.s.num_intersecting);
+        push_stat(stats_labels, stats, "num_2_func", m.s.bucket[0]);
+        push_stat(stats_labels, stats, "num_3_func", m.s.bucket[1]);
+        push_stat(stats_labels, stats, "num_more_func", m.s.bucket[2]);
+        std::cout << "num MI-vertices = " << m.s.n_verts << std::endl;
+        std::cout << "num MI-faces = " << m.s.n_faces << std::endl;
+        push_stat(stats_labels, stats, "num_MI_verts", m.s.n_verts);
+        push_stat(stats_labels, stats, "num_MI_faces", m.s.n_faces);
+
+        std::vector<MI_Vert> mv(m.s.n_verts);
+        for (uint64_t i = 0; i < m.s.n_verts; ++i) {
+            const uint64_t* w = &m.rec11[11 * i];
+            MI_Vert& v = mv[i];
+            v.tet_index = w[0];
+            v.tet_vert_index = w[1];
+            v.simplex_size = w[2];
+            for (int q = 0; q < 4; ++q) {
+                v.simplex_vert_indices[q] = w[3 + q];
+                v.material_indices[q] = w[7 + q];
+            }
+        }
+        const size_t p0 = MI_pts.size();
+        MI_pts.resize(p0 + m.s.n_verts);
+        for (uint64_t i = 0; i < m.s.n_verts; ++i)
+            for (int d = 0; d < 3; ++d) MI_pts[p0 + i][d] = m.xyz[3 * i + d];
+        std::vector<PolygonFace> faces;
+        to_faces(m, faces);
+
+        // host graph stages (src/material_interface.cpp:449-694)
+        std::vector<size_t> fit(m.fit.begin(), m.fit.begin() + m.s.n_func_in_tet);
+        std::vector<size_t> sidx(m.start.begin(), m.start.end());
+        // degenerate vertex: tied maximum (src/material_interface.cpp:53-92)
+        std::vector<uint8_t> degen(n_pts, 0);
+        bool found = false;
+        for (size_t v = 0; v < n_pts; ++v) {
+            const double* row = funcVals.data + v * F;
+            size_t best = 0;
+            for (size_t f = 1; f < F; ++f)
+                if (row[f] > row[best]) best = f;
+            for (size_t f = 0; f < F; ++f)
+                if (f != best && row[f] == row[best]) {
+                    degen[v] = 1;
+                    found = true;
+                    break;
+                }
+        }
+        DeviceCuts cuts(c, r);
+        rin::GraphIn in;
+        in.mi = true;
+        in.pts = &pts;
+        in.tets = &tets;
+        in.mi_verts = &mv;
+        in.faces = &faces;
+        in.func_in_tet = &fit;
+        in.start_index = &sidx;
+        in.degenerate_vertex = &degen;
+        in.found_degenerate = found;
+        in.n_func = F;
+        for (size_t f = 0; f < F; ++f) in.sample_value.push_back(funcVals.data[f]);
+        in.cuts = &cuts;
+        rin::GraphOut g;
+        const bool ok = rin::run_graph(in, g);
+        for (auto& t : g.timings) {
+            timing_labels.push_back(t.first);
+            timings.push_back(t.second);
+        }
+        for (auto& s : g.stats) push_stat(stats_labels, stats, s.first.c_str(), s.second);
+        MI_faces.insert(MI_faces.end(), faces.begin(), faces.end());
+        MI_edges.insert(MI_edges.end(), g.edges.begin(), g.edges.end());
+        patches.insert(patches.end(), g.patches.begin(), g.patches.end());
+        patch_function_label.insert(patch_function_label.end(), g.patch_label_mi.begin(), g.patch_label_mi.end());
+        chains.insert(chains.end(), g.chains.begin(), g.chains.end());
+        non_manifold_edges_of_vert.insert(non_manifold_edges_of_vert.end(), g.nm_edges_of_vert.begin(),
+                                          g.nm_edges_of_vert.end());
+        shells.insert(shells.end(), g.shells.begin(), g.shells.end());
+        material_cells.insert(material_cells.end(), g.cells.begin(), g.cells.end());
+        cell_function_label.insert(cell_function_label.end(), g.cell_label_mi.begin(), g.cell_label_mi.end());
+        if (!ok) std::cout << g.error << std::endl;
+        return ok;
+    } catch (std::exception& e) {
+        std::cout << e.what() << std::endl;
+        return false;
+    }
+}

@@ -108,6 +108,11 @@ void run_pipeline(DevMem& mem, const Grid& g, const Values& v, uint32_t flags, R
 void run_batched(DevMem& ctx_mem, const Grid& g, Values& v, uint32_t flags, RunOut& out, hipStream_t s,
                  int n_cu);
 
+// copy the cut records of the given tets to the host, packed (rec_start has
+// n + 1 word offsets; inactive tets get empty ranges)
+void gather_records(DevMem& mem, const RunOut& r, const uint64_t* tets, uint64_t n, uint64_t* rec_start,
+                    uint16_t* words, uint64_t cap, hipStream_t s);
+
 // diagnostics: run the device cut on independent plane/material sets
 void debug_cuts(DevMem& mem, int mi, const double* rows, const int* ks, int ncase, int cap, uint16_t* words,
                 int* lens, hipStream_t s);

@@ -520,7 +520,84 @@ __global__ void __launch_bounds__(64) k_debug_cut_wave(const double* rows, const
     const int len = dev::wave_ia_record<C>(X, k, words + (size_t)i * cap, cap, a, b, c);
     if (lane == 0) lens[i] = len;
 }
+// record lookup for a list of tets: source word offset (bit 63: k=1 table)
+// and length, 0 for inactive tets
+__global__ void k_rec_lookup(const uint32_t* __restrict__ tets, uint64_t n, uint64_t T, uint64_t n_active,
+                             const uint32_t* __restrict__ rank, const uint32_t* __restrict__ act,
+                             const uint32_t* __restrict__ rec_loc, const uint16_t* __restrict__ rec,
+                             const uint16_t* __restrict__ k1_rec, const uint32_t* __restrict__ k1_off,
+                             uint64_t* __restrict__ src, uint32_t* __restrict__ len) {
+    GRID_STRIDE(i, n) {
+        const uint32_t t = tets[i];
+        uint32_t L = 0;
+        uint64_t o = 0;
+        if (t < T) {
+            const uint32_t rk = rank[t];
+            if (rk < n_active && act[rk] == t) {
+                const uint32_t loc = rec_loc ? rec_loc[rk] : 0u;
+                if (rec_loc && (loc & 0x80000000u)) {
+                    o = (1ull << 63) | k1_off[loc & 0xFFu];
+                    L = k1_rec[k1_off[loc & 0xFFu] + 6];
+                } else {
+                    o = loc;
+                    L = rec[loc + 6];
+                }
+            }
+        }
+        src[i] = o;
+        len[i] = L;
+    }
+}
+
+__global__ void k_rec_copy(uint64_t n, const uint64_t* __restrict__ src, const uint32_t* __restrict__ len,
+                           const uint32_t* __restrict__ dst, const uint16_t* __restrict__ rec,
+                           const uint16_t* __restrict__ k1_rec, uint16_t* __restrict__ out) {
+    // one wave per tet
+    const uint64_t w = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    if (w >= n) return;
+    const uint16_t* s = (src[w] >> 63) ? k1_rec + (src[w] & 0xFFFFFFFFull) : rec + src[w];
+    for (uint32_t q = lane; q < len[w]; q += 64) out[dst[w] + q] = s[q];
+}
 }  // namespace
+
+void gather_records(DevMem& mem, const RunOut& r, const uint64_t* tets, uint64_t n, uint64_t* rec_start,
+                    uint16_t* words, uint64_t cap, hipStream_t s) {
+    const auto& vw = r.view;
+    rec_start[0] = 0;
+    if (n == 0) return;
+    std::vector<uint32_t> t32(n);
+    for (uint64_t i = 0; i < n; ++i) {
+        if (tets[i] >= r.T) throw std::invalid_argument("tet index out of range");
+        t32[i] = (uint32_t)tets[i];
+    }
+    DBuf<uint32_t> dt(mem, n), dlen(mem, n), ddst(mem, n);
+    DBuf<uint64_t> dsrc(mem, n);
+    RIN_HIP(hipMemcpyAsync(dt.p, t32.data(), 4 * n, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_rec_lookup, dim3(nblk(n)), dim3(256), 0, s, (const uint32_t*)dt.p, n, r.T, r.n_active,
+                       vw.rank_of_tet, vw.act, vw.rec_loc, vw.rec, vw.k1_rec, vw.k1_rec_off, dsrc.p, dlen.p);
+    std::vector<uint32_t> len(n);
+    RIN_HIP(hipMemcpyAsync(len.data(), dlen.p, 4 * n, hipMemcpyDeviceToHost, s));
+    RIN_HIP(hipStreamSynchronize(s));
+    std::vector<uint32_t> dst(n);
+    uint64_t o = 0;
+    for (uint64_t i = 0; i < n; ++i) {
+        dst[i] = (uint32_t)o;
+        o += len[i];
+        rec_start[i + 1] = o;
+    }
+    if (o > cap) throw std::invalid_argument("record buffer too small");
+    if (o >= (1ull << 32)) throw std::runtime_error("record gather exceeds 2^32 words");
+    DBuf<uint16_t> dout(mem, o + 1);
+    RIN_HIP(hipMemcpyAsync(ddst.p, dst.data(), 4 * n, hipMemcpyHostToDevice, s));
+    const uint64_t threads = n * 64;
+    hipLaunchKernelGGL(k_rec_copy, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s, n,
+                       (const uint64_t*)dsrc.p, (const uint32_t*)dlen.p, (const uint32_t*)ddst.p, vw.rec, vw.k1_rec,
+                       dout.p);
+    RIN_HIP(hipGetLastError());
+    RIN_HIP(hipMemcpyAsync(words, dout.p, 2 * o, hipMemcpyDeviceToHost, s));
+    RIN_HIP(hipStreamSynchronize(s));
+}
 
 void debug_cuts(DevMem& mem, int mi, const double* rows, const int* ks, int ncase, int cap, uint16_t* words, int* lens,
                 hipStream_t s) {

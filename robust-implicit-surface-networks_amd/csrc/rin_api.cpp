@@ -275,36 +275,8 @@ int rin_result_records(rin_ctx* c, const rin_result* res, const uint64_t* tets, 
     if (!res || (!tets && n) || !rec_start) return fail(c, RIN_ERR_ARG, "null argument");
     return guarded(c, [&] {
         const auto& r = res->r;
-        const auto& vw = r.view;
-        if (!vw.rec) throw std::invalid_argument("records were not kept (RIN_FLAG_KEEP_RECORDS)");
-        auto get32 = [&](const uint32_t* p) {
-            uint32_t x = 0;
-            RIN_HIP(hipMemcpyAsync(&x, p, 4, hipMemcpyDeviceToHost, c->stream));
-            RIN_HIP(hipStreamSynchronize(c->stream));
-            return x;
-        };
-        uint64_t o = 0;
-        rec_start[0] = 0;
-        for (uint64_t i = 0; i < n; ++i) {
-            const uint64_t t = tets[i];
-            if (t >= r.T) throw std::invalid_argument("tet index out of range");
-            const uint32_t rk = get32(vw.rank_of_tet + t);
-            bool active = rk < r.n_active && get32(vw.act + rk) == t;
-            if (active) {
-                const uint32_t loc = get32(vw.rec_loc + rk);
-                const uint16_t* src = (loc & 0x80000000u) ? vw.k1_rec + get32(vw.k1_rec_off + (loc & 0xFF))
-                                                          : vw.rec + loc;
-                uint16_t hdr[8];
-                RIN_HIP(hipMemcpyAsync(hdr, src, 16, hipMemcpyDeviceToHost, c->stream));
-                RIN_HIP(hipStreamSynchronize(c->stream));
-                const uint64_t len = hdr[6];
-                if (o + len > cap) throw std::invalid_argument("record buffer too small");
-                RIN_HIP(hipMemcpyAsync(words + o, src, 2 * len, hipMemcpyDeviceToHost, c->stream));
-                RIN_HIP(hipStreamSynchronize(c->stream));
-                o += len;
-            }
-            rec_start[i + 1] = o;
-        }
+        if (!r.view.rec) throw std::invalid_argument("records were not kept (RIN_FLAG_KEEP_RECORDS)");
+        rin::gather_records(c->mem, r, tets, n, rec_start, words, cap, c->stream);
     });
 }
 

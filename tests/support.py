@@ -216,3 +216,23 @@ def known_answer_run(mi, pts, tets, vals):
     out = _ka.ka_run(int(mi), _ptr(pts, _f64p), pts.shape[0], _ptr(tets, _u64p), tets.shape[0],
                      _ptr(vals, _f64p), vals.shape[1])
     return json.loads(out.decode())
+
+
+_dropin = None
+
+
+def dropin_run(mi, pts, tets, vals, robust=False):
+    """The drop-in implicit_arrangement() / material_interface() (GPU path and
+    host graph stages) through tests/native/libdropin.so."""
+    global _dropin
+    if _dropin is None:
+        _dropin = ctypes.CDLL(os.path.join(ROOT, "tests", "native", "libdropin.so"))
+        _dropin.dropin_run.restype = ctypes.c_char_p
+        _dropin.dropin_run.argtypes = [ctypes.c_int, ctypes.c_int, _f64p, ctypes.c_uint64, _u64p, ctypes.c_uint64,
+                                       _f64p, ctypes.c_uint64]
+    pts = np.ascontiguousarray(pts, dtype=np.float64)
+    tets = np.ascontiguousarray(tets, dtype=np.uint64)
+    vals = np.ascontiguousarray(vals, dtype=np.float64)
+    out = _dropin.dropin_run(int(mi), int(robust), _ptr(pts, _f64p), pts.shape[0], _ptr(tets, _u64p), tets.shape[0],
+                             _ptr(vals, _f64p), vals.shape[1])
+    return json.loads(out.decode())
