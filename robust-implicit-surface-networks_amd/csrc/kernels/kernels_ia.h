@@ -11,6 +11,10 @@ namespace {
 using dev::Complex;
 using CMed = dev::Caps<40, 64, 8, 8>;      // up to 4 active functions
 using CBig = dev::Caps<132, 284, 10, 12>;  // up to 8 active functions
+// more than 8 active functions (degenerate inputs: ties, coincident
+// functions): up to 28, with the complex held in device memory
+using CHuge = dev::Caps<255, 255, 32, 32>;
+constexpr int kMaxHuge = CHuge::MP - 4;
 
 constexpr uint32_t ERR_CAPACITY = 1, ERR_RECORD = 4;
 
@@ -168,9 +172,11 @@ __global__ void k_fill(uint64_t T, const uint64_t* __restrict__ amask, const uin
 
 // record slot bound per active count
 __device__ inline uint32_t rec_bound(int mi, int k) {
-    const uint32_t ia[11] = {0, 128, 256, 448, 768, 1280, 1920, 2560, 3200, 4096, 5120};
-    const uint32_t m[11] = {0, 64, 128, 256, 320, 400, 480, 560, 640, 720, 800};
-    return (k < 0 || k > 10) ? 6000u : (mi ? m[k] : ia[k]);
+    const uint32_t ia[9] = {0, 128, 256, 448, 768, 1280, 1920, 2560, 3200};
+    const uint32_t m[9] = {0, 64, 128, 256, 320, 400, 480, 560, 640};
+    // beyond 8: everything a CHuge complex can hold
+    constexpr uint32_t huge = 8 + 4 * CHuge::MV + CHuge::MF * (4 + CHuge::MFV) + 2 * CHuge::MP;
+    return (k < 0 || k > 8) ? huge : (mi ? m[k] : ia[k]);
 }
 
 __global__ void k_rec_bound(Count n, const uint8_t* __restrict__ act_k, int mi, uint32_t* __restrict__ b) {
@@ -232,6 +238,62 @@ __global__ void __launch_bounds__(64) k_cut(GridView g, const double* __restrict
             for (int c = 0; c < 4; ++c) P[q][c] = vals[(uint64_t)f * V + v[c]];
         }
         Complex<C> X;
+        if (MI)
+            dev::mi_cut<C>(P, k, X);
+        else
+            dev::ia_cut<C>(P, k, X);
+        if (X.err) {
+            flag_error(errs, ERR_CAPACITY);
+            cnt[3 * a] = cnt[3 * a + 1] = cnt[3 * a + 2] = 0;
+            rec[rec_off[a]] = 0;
+            continue;
+        }
+        const int cap = (int)(rec_off[a + 1] - rec_off[a]);
+        const int w = MI ? dev::write_record_mi<C>(X, k, rec + rec_off[a], cap)
+                         : dev::write_record_ia<C>(X, k, rec + rec_off[a], cap);
+        if (w < 0) flag_error(errs, ERR_RECORD);
+        uint32_t a0, a1, a2;
+        if (flags & RUN_CUT_ONLY) {
+            a0 = X.nv;
+            a1 = X.nf;
+            a2 = X.nc;
+        } else if (MI) {
+            mi_counts<C>(X, a0, a1, a2);
+        } else {
+            ia_counts<C>(X, a0, a1, a2);
+        }
+        cnt[3 * a] = a0;
+        cnt[3 * a + 1] = a1;
+        cnt[3 * a + 2] = a2;
+    }
+}
+
+// active tets with more than 8 functions, one lane each; ws holds one complex
+// per lane of the grid
+template <bool MI>
+__global__ void __launch_bounds__(64) k_cut_huge(GridView g, const double* __restrict__ vals,
+                                                 const uint32_t* __restrict__ act, const uint8_t* __restrict__ act_k,
+                                                 const uint32_t* __restrict__ csr_off, const uint8_t* __restrict__ fit,
+                                                 const uint32_t* __restrict__ list, uint64_t n, uint32_t flags,
+                                                 const uint32_t* __restrict__ rec_off, uint16_t* __restrict__ rec,
+                                                 uint32_t* __restrict__ cnt, uint32_t* __restrict__ errs,
+                                                 Complex<CHuge>* __restrict__ ws) {
+    using C = CHuge;
+    const uint64_t V = g.V;
+    Complex<C>& X = ws[(uint64_t)blockIdx.x * blockDim.x + threadIdx.x];
+    GRID_STRIDE(i, n) {
+        const uint32_t a = list[i];
+        const int k = act_k[a];
+        const uint32_t t = act[a];
+        uint32_t v[4];
+        dev::tet_corners(g, t, v);
+        const uint32_t s0 = csr_off[t];
+        double P[C::MP - 4][4];
+        const int kk = k < C::MP - 4 ? k : C::MP - 4;
+        for (int q = 0; q < kk; ++q) {
+            const uint32_t f = fit[s0 + ((flags & RUN_REVERSE) ? k - 1 - q : q)];
+            for (int c = 0; c < 4; ++c) P[q][c] = vals[(uint64_t)f * V + v[c]];
+        }
         if (MI)
             dev::mi_cut<C>(P, k, X);
         else

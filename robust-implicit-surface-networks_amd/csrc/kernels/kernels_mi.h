@@ -1,0 +1,438 @@
+// Material-interface extraction on the device (extract_MI_mesh,
+// src/extract_mesh.cpp:569-986, and compute_MI_vert_xyz, :1541-1637).
+//
+// Vertex numbering is the reference's first-creation order.  Per active tet,
+// in tet order, the creation events are:
+//   phase 0: every local vertex on an interface face (positive label > 3),
+//            in local order: tet corners, tet-edge points (edge, two
+//            materials), tet-face points (face, three materials) are shared
+//            with other tets by key; interior points are always new;
+//   phase 1: the corners of the tet-boundary faces this tet emits (below), in
+//            face order and cycle order, created if not yet present.
+// Each event becomes an occurrence with a 64-bit key; a stable sort by key and
+// first-occurrence ranks over occurrence order give the ids (as for the
+// implicit arrangement).
+//
+// A tet-boundary face lies on the tet face shared with the neighbouring tet.
+// The reference pairs the two copies through a map keyed by (min, second-min,
+// max) vertex id (:833-951): the first copy (lower tet) is stored, the second
+// becomes an interface face when the material sets on the two sides share no
+// material, with func_index (material of the first copy, material of the
+// second).  The device finds the copy directly in the neighbour's record, so
+// faces need no sort: face ids are a prefix sum in (tet, local face) order.
+#pragma once
+#include "kernels_ia.h"
+
+namespace rin {
+namespace {
+
+constexpr uint32_t NO_NBR = 0xFFFFFFFFu;
+
+__device__ inline uint64_t key_mi_edge(uint64_t e, uint32_t m1, uint32_t m2) {
+    return ((((e << 6) | m1) << 6 | m2) << 2) | 1;
+}
+__device__ inline uint64_t key_mi_face(uint64_t fc, uint32_t m1, uint32_t m2, uint32_t m3) {
+    return (((((fc << 6) | m1) << 6 | m2) << 6 | m3) << 2) | 2;
+}
+
+// ------------------------------------------------------------------ tet adjacency
+
+// (global face id, 4 t + local face) of every tet face
+__global__ void k_face_slots(GridView g, uint64_t* __restrict__ key, uint32_t* __restrict__ slot) {
+    GRID_STRIDE(t, g.T) {
+        uint32_t v[4];
+        dev::tet_corners(g, t, v);
+        for (int c = 0; c < 4; ++c) {
+            key[4 * t + c] = dev::face_id(g, t, v, c);
+            slot[4 * t + c] = (uint32_t)(4 * t + c);
+        }
+    }
+}
+// equal keys are adjacent after the sort; a face is shared by at most two tets
+__global__ void k_pair_faces(uint64_t n, const uint64_t* __restrict__ key, const uint32_t* __restrict__ slot,
+                             uint32_t* __restrict__ nbr) {
+    GRID_STRIDE(p, n) {
+        if (p + 1 < n && key[p] == key[p + 1]) {
+            nbr[slot[p]] = slot[p + 1];
+            nbr[slot[p + 1]] = slot[p];
+        }
+    }
+}
+
+// ------------------------------------------------------------------ per-tet helpers
+
+struct MITet {
+    Rec R;
+    uint32_t t, s0;
+    uint32_t v[4];
+    __device__ MITet(const GridView& g, const uint16_t* rec, uint32_t tet, uint32_t csr0) : R(rec), t(tet), s0(csr0) {
+        dev::tet_corners(g, t, v);
+    }
+    __device__ const uint16_t* groups() const { return R.r + R.r[6] - R.L; }
+};
+
+// key of local vertex j (labels sorted ascending: tet faces first); nb = number
+// of tet-face labels
+__device__ inline uint64_t mi_vkey(const GridView& g, const MITet& M, const uint8_t* __restrict__ fit, uint64_t a,
+                                   int j, int& nb) {
+    const uint16_t* L4 = M.R.vert(j);
+    int bnd[4];
+    uint32_t m[4];
+    int nm = 0;
+    nb = 0;
+    for (int q = 0; q < 4; ++q) {
+        if (L4[q] > 3)
+            m[nm++] = fit[M.s0 + L4[q] - 4];
+        else
+            bnd[nb++] = L4[q];
+    }
+    if (nb == 3) {
+        int c = 0;
+        while (c == bnd[0] || c == bnd[1] || c == bnd[2]) ++c;
+        return key_corner(M.v[c]);
+    }
+    if (nb == 2) {
+        int l[2], nl = 0;
+        for (int c = 0; c < 4; ++c)
+            if (c != bnd[0] && c != bnd[1]) l[nl++] = c;
+        return key_mi_edge(dev::edge_id(g, M.t, M.v, l[0], l[1]), m[0], m[1]);
+    }
+    if (nb == 1) return key_mi_face(dev::face_id(g, M.t, M.v, bnd[0]), m[0], m[1], m[2]);
+    return key_inner(a, j);
+}
+
+// material set on the inner side of a tet-boundary face: its negative
+// label's material, or every material of that label's coincidence group
+__device__ inline int mi_side_set(const MITet& M, const uint8_t* __restrict__ fit, int neg, uint8_t* S) {
+    if (!M.R.dup) {
+        S[0] = fit[M.s0 + neg - 4];
+        return 1;
+    }
+    const uint16_t* grp = M.groups();
+    int n = 0;
+    for (int l = 4; l < M.R.L; ++l)
+        if (grp[l] == grp[neg]) S[n++] = fit[M.s0 + l - 4];
+    if (n <= 1) {
+        S[0] = fit[M.s0 + neg - 4];
+        n = 1;
+    }
+    return n;
+}
+
+// (min, second-min, max) of a face's vertex keys
+__device__ inline void mi_face_triple(const GridView& g, const MITet& M, const uint8_t* __restrict__ fit, uint64_t a,
+                                      const uint16_t* F, uint64_t k3[3]) {
+    const int n = F[3];
+    uint64_t mn = ~0ull, sm = ~0ull, mx = 0;
+    for (int q = 0; q < n; ++q) {
+        int nb;
+        const uint64_t k = mi_vkey(g, M, fit, a, F[4 + q], nb);
+        if (k < mn) {
+            sm = mn;
+            mn = k;
+        } else if (k < sm) {
+            sm = k;
+        }
+        if (k > mx) mx = k;
+    }
+    k3[0] = mn;
+    k3[1] = sm;
+    k3[2] = mx;
+}
+
+struct MIArgs {
+    GridView g;
+    const uint32_t *act, *csr_off, *rank, *rec_off, *nbr;
+    const uint8_t* fit;
+    const uint16_t* rec;
+    uint64_t n_active;
+};
+
+// whether tet-boundary face F of active tet a becomes an interface face
+// here (the second copy); `other` is the first copy's material
+__device__ inline bool mi_bface_emits(const MIArgs& A, const MITet& M, uint64_t a, const uint16_t* F,
+                                      uint8_t& other) {
+    const int l = F[0];
+    const uint32_t nb = A.nbr[4 * (uint64_t)M.t + l];
+    if (nb == NO_NBR) return false;
+    const uint32_t t2 = nb >> 2, l2 = nb & 3;
+    if (t2 >= M.t) return false;
+    const uint32_t r2 = A.rank[t2];
+    if (!(r2 < A.n_active && A.act[r2] == t2)) return false;
+    uint64_t k3[3];
+    mi_face_triple(A.g, M, A.fit, a, F, k3);
+    const MITet N(A.g, A.rec + A.rec_off[r2], t2, A.csr_off[t2]);
+    const uint16_t* G = N.R.faces();
+    for (int f = 0; f < N.R.nf; ++f) {
+        const int n = G[3];
+        if (G[0] == l2) {
+            uint64_t q3[3];
+            mi_face_triple(A.g, N, A.fit, r2, G, q3);
+            if (q3[0] == k3[0] && q3[1] == k3[1] && q3[2] == k3[2]) {
+                uint8_t S1[CHuge::MP], S2[CHuge::MP];
+                const int n1 = mi_side_set(N, A.fit, G[1], S1), n2 = mi_side_set(M, A.fit, F[1], S2);
+                for (int x = 0; x < n1; ++x)
+                    for (int y = 0; y < n2; ++y)
+                        if (S1[x] == S2[y]) return false;
+                other = S1[0];
+                return true;
+            }
+        }
+        G += 4 + n;
+    }
+    return false;
+}
+
+// interface-vertex mask of a tet (vertices of faces with positive label > 3)
+__device__ inline void mi_iface_mask(const Rec& R, uint64_t m[4]) {
+    m[0] = m[1] = m[2] = m[3] = 0;
+    const uint16_t* F = R.faces();
+    for (int f = 0; f < R.nf; ++f) {
+        const int n = F[3];
+        if (F[0] > 3)
+            for (int q = 0; q < n; ++q) m[F[4 + q] >> 6] |= 1ull << (F[4 + q] & 63);
+        F += 4 + n;
+    }
+}
+__device__ inline uint32_t mask_rank(const uint64_t m[4], int j) {
+    uint32_t r = 0;
+    for (int w = 0; w < (j >> 6); ++w) r += __popcll(m[w]);
+    return r + __popcll(m[j >> 6] & ((1ull << (j & 63)) - 1));
+}
+__device__ inline bool is_corner(const Rec& R, int j) {
+    const uint16_t* L4 = R.vert(j);
+    return L4[2] <= 3;  // sorted labels: three tet faces
+}
+// the tet corner a corner vertex sits at: the one tet face not among its labels
+__device__ inline int corner_of(const Rec& R, int j) {
+    const uint16_t* L4 = R.vert(j);
+    int c = 0;
+    while (c == L4[0] || c == L4[1] || c == L4[2]) ++c;
+    return c;
+}
+
+// per active tet: vertex occurrences (phase 0 + phase 1), faces, face corners
+__global__ void k_mi_count(MIArgs A, uint32_t* __restrict__ cnt) {
+    GRID_STRIDE(a, A.n_active) {
+        const uint32_t t = A.act[a];
+        const MITet M(A.g, A.rec + A.rec_off[a], t, A.csr_off[t]);
+        uint32_t nv = 0, nf = 0, nfv = 0;
+        if (M.R.r[0] == 2) {
+            uint64_t m[4];
+            mi_iface_mask(M.R, m);
+            nv = __popcll(m[0]) + __popcll(m[1]) + __popcll(m[2]) + __popcll(m[3]);
+            const uint16_t* F = M.R.faces();
+            for (int f = 0; f < M.R.nf; ++f) {
+                const int n = F[3];
+                uint8_t other;
+                if (F[0] > 3) {
+                    ++nf;
+                    nfv += n;
+                } else if (mi_bface_emits(A, M, a, F, other)) {
+                    ++nf;
+                    nfv += n;
+                    for (int q = 0; q < n; ++q) nv += is_corner(M.R, F[4 + q]);
+                }
+                F += 4 + n;
+            }
+        }
+        cnt[3 * a] = nv;
+        cnt[3 * a + 1] = nf;
+        cnt[3 * a + 2] = nfv;
+    }
+}
+
+__global__ void k_mi_extract(MIArgs A, const uint32_t* __restrict__ voff, const uint32_t* __restrict__ foff,
+                             const uint32_t* __restrict__ fvoff, uint64_t* __restrict__ okey,
+                             uint32_t* __restrict__ oa, uint16_t* __restrict__ oj, uint32_t* __restrict__ fq_a,
+                             uint16_t* __restrict__ fq_l, uint8_t* __restrict__ fq_m, uint32_t* __restrict__ fq_vs,
+                             uint32_t* __restrict__ fve) {
+    GRID_STRIDE(a, A.n_active) {
+        const uint32_t t = A.act[a];
+        const MITet M(A.g, A.rec + A.rec_off[a], t, A.csr_off[t]);
+        if (M.R.r[0] != 2) continue;
+        uint64_t m[4];
+        mi_iface_mask(M.R, m);
+        const uint32_t vb = voff[a];
+        uint32_t o = vb;
+        for (int j = 0; j < M.R.nv; ++j) {
+            if (!((m[j >> 6] >> (j & 63)) & 1)) continue;
+            int nb;
+            okey[o] = mi_vkey(A.g, M, A.fit, a, j, nb);
+            oa[o] = (uint32_t)a;
+            oj[o] = (uint16_t)j;
+            ++o;
+        }
+        uint32_t q = foff[a], e = fvoff[a];
+        const uint16_t* F = M.R.faces();
+        for (int f = 0; f < M.R.nf; ++f) {
+            const int n = F[3];
+            uint8_t other = 0xFF;
+            const bool iface = F[0] > 3;
+            if (iface || mi_bface_emits(A, M, a, F, other)) {
+                fq_a[q] = (uint32_t)a;
+                fq_l[q] = (uint16_t)f;
+                fq_m[2 * q] = iface ? A.fit[M.s0 + F[0] - 4] : other;
+                fq_m[2 * q + 1] = A.fit[M.s0 + F[1] - 4];
+                fq_vs[q] = e;
+                for (int i = 0; i < n; ++i) {
+                    const int j = F[4 + i];
+                    if (!iface && is_corner(M.R, j)) {
+                        // phase 1: the corner is created here if it is new
+                        okey[o] = key_corner(M.v[corner_of(M.R, j)]);
+                        oa[o] = (uint32_t)a;
+                        oj[o] = (uint16_t)j;
+                        fve[e++] = o++;
+                    } else {
+                        fve[e++] = vb + mask_rank(m, j);
+                    }
+                }
+                ++q;
+            }
+            F += 4 + n;
+        }
+    }
+}
+
+// vertex records and coordinates (src/extract_mesh.cpp:640-815, 1541-1637)
+__global__ void k_vert_fields_mi(GridView g, const double* __restrict__ vals, const uint32_t* __restrict__ act,
+                                 const uint32_t* __restrict__ csr_off, const uint8_t* __restrict__ fit,
+                                 const uint32_t* __restrict__ rec_off, const uint16_t* __restrict__ rec,
+                                 const uint32_t* __restrict__ first_occ, const uint32_t* __restrict__ oa,
+                                 const uint16_t* __restrict__ oj, Count nverts, uint32_t* __restrict__ v_tet,
+                                 uint16_t* __restrict__ v_j, uint8_t* __restrict__ v_ss, uint32_t* __restrict__ v_sv,
+                                 uint8_t* __restrict__ v_fn, double* __restrict__ xyz) {
+    const uint64_t V = g.V;
+    GRID_STRIDE(i, nverts) {
+        const uint32_t o = first_occ[i];
+        const uint32_t a = oa[o];
+        const int j = oj[o];
+        const uint32_t t = act[a];
+        const Rec R(rec + rec_off[a]);
+        uint32_t v[4];
+        dev::tet_corners(g, t, v);
+        const uint32_t s0 = csr_off[t];
+        const uint16_t* L4 = R.vert(j);
+        int bnd[4], nb = 0;
+        uint32_t fn[4] = {0xFF, 0xFF, 0xFF, 0xFF};
+        int nm = 0;
+        for (int q = 0; q < 4; ++q) {
+            if (L4[q] > 3)
+                fn[nm++] = fit[s0 + L4[q] - 4];
+            else
+                bnd[nb++] = L4[q];
+        }
+        uint32_t sv[4] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu};
+        int ss;
+        double X0, X1, X2;
+        auto P = [&](uint32_t vid, int ax) { return g.pts[3 * (uint64_t)vid + ax]; };
+        auto fv = [&](uint32_t vid, uint32_t f) { return vals[(uint64_t)f * V + vid]; };
+        if (nb == 3) {
+            int c = 0;
+            while (c == bnd[0] || c == bnd[1] || c == bnd[2]) ++c;
+            ss = 1;
+            sv[0] = v[c];
+            for (int q = 0; q < 4; ++q) fn[q] = 0xFF;  // a tet corner carries no materials
+            X0 = P(sv[0], 0);
+            X1 = P(sv[0], 1);
+            X2 = P(sv[0], 2);
+        } else if (nb == 2) {
+            uint32_t e[2];
+            int ne = 0;
+            for (int c = 0; c < 4; ++c)
+                if (c != bnd[0] && c != bnd[1]) e[ne++] = v[c];
+            sv[0] = e[0] < e[1] ? e[0] : e[1];
+            sv[1] = e[0] < e[1] ? e[1] : e[0];
+            ss = 2;
+            const double f1 = fv(sv[0], fn[0]) - fv(sv[0], fn[1]);
+            const double f2 = fv(sv[1], fn[0]) - fv(sv[1], fn[1]);
+            const double b0 = f2 / (f2 - f1);
+            const double b1 = 1 - b0;
+            X0 = b0 * P(sv[0], 0) + b1 * P(sv[1], 0);
+            X1 = b0 * P(sv[0], 1) + b1 * P(sv[1], 1);
+            X2 = b0 * P(sv[0], 2) + b1 * P(sv[1], 2);
+        } else if (nb == 1) {
+            uint32_t e[3];
+            int ne = 0;
+            for (int c = 0; c < 4; ++c)
+                if (c != bnd[0]) e[ne++] = v[c];
+            for (int x = 1; x < 3; ++x)
+                for (int y = x; y > 0 && e[y - 1] > e[y]; --y) {
+                    uint32_t tmp = e[y];
+                    e[y] = e[y - 1];
+                    e[y - 1] = tmp;
+                }
+            for (int q = 0; q < 3; ++q) sv[q] = e[q];
+            ss = 3;
+            double p1[3], p2[3];
+            for (int q = 0; q < 3; ++q) {
+                p1[q] = fv(sv[q], fn[0]) - fv(sv[q], fn[1]);
+                p2[q] = fv(sv[q], fn[1]) - fv(sv[q], fn[2]);
+            }
+            const double n1 = p1[2] * p2[1] - p1[1] * p2[2];
+            const double n2 = p1[0] * p2[2] - p1[2] * p2[0];
+            const double n3 = p1[1] * p2[0] - p1[0] * p2[1];
+            const double d = n1 + n2 + n3;
+            const double b0 = n1 / d, b1 = n2 / d, b2 = n3 / d;
+            X0 = b0 * P(sv[0], 0) + b1 * P(sv[1], 0) + b2 * P(sv[2], 0);
+            X1 = b0 * P(sv[0], 1) + b1 * P(sv[1], 1) + b2 * P(sv[2], 1);
+            X2 = b0 * P(sv[0], 2) + b1 * P(sv[1], 2) + b2 * P(sv[2], 2);
+        } else {
+            for (int q = 0; q < 4; ++q) sv[q] = v[q];
+            ss = 4;
+            double a1[4], a2[4], a3[4];
+            for (int q = 0; q < 4; ++q) {
+                a1[q] = fv(sv[q], fn[0]) - fv(sv[q], fn[1]);
+                a2[q] = fv(sv[q], fn[1]) - fv(sv[q], fn[2]);
+                a3[q] = fv(sv[q], fn[2]) - fv(sv[q], fn[3]);
+            }
+            const double n1 = a1[3] * (a2[2] * a3[1] - a2[1] * a3[2]) + a1[2] * (a2[1] * a3[3] - a2[3] * a3[1]) +
+                              a1[1] * (a2[3] * a3[2] - a2[2] * a3[3]);
+            const double n2 = a1[3] * (a2[0] * a3[2] - a2[2] * a3[0]) + a1[2] * (a2[3] * a3[0] - a2[0] * a3[3]) +
+                              a1[0] * (a2[2] * a3[3] - a2[3] * a3[2]);
+            const double n3 = a1[3] * (a2[1] * a3[0] - a2[0] * a3[1]) + a1[1] * (a2[0] * a3[3] - a2[3] * a3[0]) +
+                              a1[0] * (a2[3] * a3[1] - a2[1] * a3[3]);
+            const double n4 = a1[2] * (a2[0] * a3[1] - a2[1] * a3[0]) + a1[1] * (a2[2] * a3[0] - a2[0] * a3[2]) +
+                              a1[0] * (a2[1] * a3[2] - a2[2] * a3[1]);
+            const double d = n1 + n2 + n3 + n4;
+            const double b0 = n1 / d, b1 = n2 / d, b2 = n3 / d, b3 = n4 / d;
+            X0 = b0 * P(sv[0], 0) + b1 * P(sv[1], 0) + b2 * P(sv[2], 0) + b3 * P(sv[3], 0);
+            X1 = b0 * P(sv[0], 1) + b1 * P(sv[1], 1) + b2 * P(sv[2], 1) + b3 * P(sv[3], 1);
+            X2 = b0 * P(sv[0], 2) + b1 * P(sv[1], 2) + b2 * P(sv[2], 2) + b3 * P(sv[3], 2);
+        }
+        v_tet[i] = t;
+        v_j[i] = (uint16_t)j;
+        v_ss[i] = (uint8_t)ss;
+        for (int q = 0; q < 4; ++q) {
+            v_sv[4 * i + q] = sv[q];
+            v_fn[4 * i + q] = (uint8_t)fn[q];
+        }
+        xyz[3 * i] = X0;
+        xyz[3 * i + 1] = X1;
+        xyz[3 * i + 2] = X2;
+    }
+}
+
+// faces are never merged: face q is occurrence q
+__global__ void k_mi_face_out(Count nf, const uint32_t* __restrict__ act, const uint32_t* __restrict__ fq_a,
+                              const uint16_t* __restrict__ fq_l, const uint8_t* __restrict__ fq_m,
+                              const uint32_t* __restrict__ fq_vs, const uint32_t* __restrict__ fve,
+                              const uint32_t* __restrict__ vid_of_occ, uint64_t nfv, uint32_t* __restrict__ f_vstart,
+                              uint32_t* __restrict__ f_verts, uint32_t* __restrict__ f_tfstart,
+                              uint32_t* __restrict__ f_tf_tet, uint16_t* __restrict__ f_tf_local,
+                              uint8_t* __restrict__ f_func) {
+    const uint64_t n = count_of(nf);
+    GRID_STRIDE(q, nf) {
+        const uint32_t b = fq_vs[q], e = (q + 1 < n) ? fq_vs[q + 1] : (uint32_t)nfv;
+        f_vstart[q] = b;
+        for (uint32_t i = b; i < e; ++i) f_verts[i] = vid_of_occ[fve[i]];
+        f_tfstart[q] = (uint32_t)q;
+        f_tf_tet[q] = act[fq_a[q]];
+        f_tf_local[q] = fq_l[q];
+        f_func[2 * q] = fq_m[2 * q];
+        f_func[2 * q + 1] = fq_m[2 * q + 1];
+    }
+}
+
+}  // namespace
+}  // namespace rin

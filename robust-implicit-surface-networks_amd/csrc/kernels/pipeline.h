@@ -21,6 +21,9 @@ struct Grid {
     GridView view{};
     DBuf<double> pts;
     DBuf<uint32_t> tets, tedge, tface;
+    // tet across each tet face (4 t + local face, or all ones on the mesh
+    // boundary); built on the first material-interface run
+    mutable DBuf<uint32_t> nbr;
 };
 
 struct Plan;  // batched form: workspaces and captured graphs (batched.hip)

@@ -18,6 +18,7 @@
 
 #include "cut_wave.h"
 #include "kernels_ia.h"
+#include "kernels_mi.h"
 
 namespace rin {
 
@@ -58,6 +59,126 @@ inline int bits_for(uint64_t x) {
     int b = 0;
     while (b < 64 && (x >> b)) ++b;
     return b;
+}
+
+// vertex dedup shared by both modes: stable sort of the occurrence keys, then
+// each key's id is the rank of its first occurrence in occurrence order
+struct Dedup {
+    DBuf<uint32_t> vid;        // id of every occurrence
+    DBuf<uint32_t> first_occ;  // first occurrence of every id
+    uint64_t n = 0;
+};
+void dedup_vertices(DevMem& mem, DBuf<uint64_t>& okey, uint64_t NVO, int kbits, Dedup& d, hipStream_t s) {
+    if (kbits > 64) throw std::runtime_error("simplex keys exceed 64 bits for this mesh");
+    DBuf<uint32_t> sidx(mem, NVO + 1), first(mem, NVO + 1), head(mem, NVO + 1), hexcl(mem, NVO + 1),
+        seg_first(mem, NVO + 1), vrank(mem, NVO + 1), nvt(mem, 2);
+    d.vid.alloc(mem, NVO + 1);
+    iota_u32(sidx.p, NVO, s);
+    sort_pairs(mem, okey.p, sidx.p, NVO, kbits, s);
+    hipLaunchKernelGGL(k_heads, dim3(nblk(NVO)), dim3(256), 0, s, NVO, (const uint64_t*)okey.p,
+                       (const uint32_t*)sidx.p, first.p, head.p);
+    scan_u32(mem, head.p, hexcl.p, NVO, nullptr, s);
+    hipLaunchKernelGGL(k_assign, dim3(nblk(NVO)), dim3(256), 0, s, NVO, (const uint32_t*)sidx.p,
+                       (const uint32_t*)head.p, (const uint32_t*)hexcl.p, seg_first.p);
+    scan_u32(mem, first.p, vrank.p, NVO, nvt.p, s);
+    hipLaunchKernelGGL(k_assign2, dim3(nblk(NVO)), dim3(256), 0, s, NVO, (const uint32_t*)sidx.p,
+                       (const uint32_t*)head.p, (const uint32_t*)hexcl.p, (const uint32_t*)seg_first.p,
+                       (const uint32_t*)vrank.p, d.vid.p);
+    d.n = read1(nvt.p, s);
+    d.first_occ.alloc(mem, d.n + 1);
+    hipLaunchKernelGGL(k_first_list, dim3(nblk(NVO)), dim3(256), 0, s, NVO, (const uint32_t*)first.p,
+                       (const uint32_t*)vrank.p, d.first_occ.p);
+    RIN_HIP(hipGetLastError());
+}
+
+// tet adjacency across faces, once per grid
+void ensure_nbr(DevMem& mem, const Grid& g, hipStream_t s) {
+    if (g.nbr.p) return;
+    const uint64_t n = 4 * g.view.T;
+    if (n >= (1ull << 32)) throw std::runtime_error("too many tets for the adjacency table");
+    DBuf<uint64_t> key(mem, n + 1);
+    DBuf<uint32_t> slot(mem, n + 1);
+    g.nbr.alloc(mem, n + 1);
+    fill_u32(g.nbr.p, NO_NBR, n + 1, s);
+    hipLaunchKernelGGL(k_face_slots, dim3(nblk(g.view.T)), dim3(256), 0, s, g.view, key.p, slot.p);
+    sort_pairs(mem, key.p, slot.p, Count(n), bits_for(g.view.n_face_ids), s);
+    hipLaunchKernelGGL(k_pair_faces, dim3(nblk(n)), dim3(256), 0, s, n, (const uint64_t*)key.p,
+                       (const uint32_t*)slot.p, g.nbr.p);
+    RIN_HIP(hipGetLastError());
+    RIN_HIP(hipStreamSynchronize(s));
+}
+
+// material-interface extraction and coordinates (src/extract_mesh.cpp:569-986,
+// 1541-1637); faces are never merged, so they need no sort
+void extract_mi(DevMem& mem, const Grid& g, const Values& val, RunOut& out, Timer& tm, hipStream_t s) {
+    const uint64_t NA = out.n_active;
+    ensure_nbr(mem, g, s);
+    MIArgs A;
+    A.g = g.view;
+    A.act = out.act.p;
+    A.csr_off = out.csr_off.p;
+    A.rank = out.rank_of_tet.p;
+    A.rec_off = out.rec_off.p;
+    A.nbr = g.nbr.p;
+    A.fit = out.fit.p;
+    A.rec = out.rec.p;
+    A.n_active = NA;
+    DBuf<uint32_t> cnt(mem, 3 * NA + 3), cv(mem, NA + 1), cf(mem, NA + 1), cfv(mem, NA + 1);
+    if (NA) hipLaunchKernelGGL(k_mi_count, dim3(nblk(NA)), dim3(256), 0, s, A, cnt.p);
+    hipLaunchKernelGGL(k_split3, dim3(nblk(NA)), dim3(256), 0, s, Count(NA), (const uint32_t*)cnt.p, cv.p, cf.p,
+                       cfv.p);
+    scan_u32(mem, cv.p, cv.p, NA, cv.p + NA, s);
+    scan_u32(mem, cf.p, cf.p, NA, cf.p + NA, s);
+    scan_u32(mem, cfv.p, cfv.p, NA, cfv.p + NA, s);
+    const uint64_t NVO = read1(cv.p + NA, s), NF = read1(cf.p + NA, s), NFV = read1(cfv.p + NA, s);
+    DBuf<uint64_t> okey(mem, NVO + 1);
+    DBuf<uint32_t> oa(mem, NVO + 1), fq_a(mem, NF + 1), fq_vs(mem, NF + 1), fve(mem, NFV + 1);
+    DBuf<uint16_t> oj(mem, NVO + 1), fq_l(mem, NF + 1);
+    DBuf<uint8_t> fq_m(mem, 2 * NF + 2);
+    if (NA)
+        hipLaunchKernelGGL(k_mi_extract, dim3(nblk(NA)), dim3(256), 0, s, A, (const uint32_t*)cv.p,
+                           (const uint32_t*)cf.p, (const uint32_t*)cfv.p, okey.p, oa.p, oj.p, fq_a.p, fq_l.p, fq_m.p,
+                           fq_vs.p, fve.p);
+    RIN_HIP(hipGetLastError());
+    tm.mark("extract occurrences");
+    const int kbits = 2 + std::max({bits_for(g.view.V), bits_for(g.view.n_edge_ids) + 12,
+                                    bits_for(g.view.n_face_ids) + 18, bits_for(NA) + 8});
+    Dedup d;
+    dedup_vertices(mem, okey, NVO, kbits, d, s);
+    tm.mark("dedup verts");
+    const uint64_t NV = d.n;
+    out.n_verts = NV;
+    out.v_tet.alloc(mem, NV + 1);
+    out.v_j.alloc(mem, NV + 1);
+    out.v_ss.alloc(mem, NV + 1);
+    out.v_sv.alloc(mem, 4 * NV + 4);
+    out.v_fn.alloc(mem, 4 * NV + 4);
+    out.xyz.alloc(mem, 3 * NV + 3);
+    hipLaunchKernelGGL(k_vert_fields_mi, dim3(nblk(NV)), dim3(256), 0, s, g.view, (const double*)val.vals.p,
+                       (const uint32_t*)out.act.p, (const uint32_t*)out.csr_off.p, (const uint8_t*)out.fit.p,
+                       (const uint32_t*)out.rec_off.p, (const uint16_t*)out.rec.p, (const uint32_t*)d.first_occ.p,
+                       (const uint32_t*)oa.p, (const uint16_t*)oj.p, Count(NV), out.v_tet.p, out.v_j.p, out.v_ss.p,
+                       out.v_sv.p, out.v_fn.p, out.xyz.p);
+    RIN_HIP(hipGetLastError());
+    tm.mark("compute xyz");
+    out.n_faces = NF;
+    out.n_face_verts = NFV;
+    out.n_tet_faces = NF;
+    out.f_vstart.alloc(mem, NF + 1);
+    out.f_verts.alloc(mem, NFV + 1);
+    out.f_tfstart.alloc(mem, NF + 1);
+    out.f_tf_tet.alloc(mem, NF + 1);
+    out.f_tf_local.alloc(mem, NF + 1);
+    out.f_func.alloc(mem, 2 * NF + 2);
+    hipLaunchKernelGGL(k_mi_face_out, dim3(nblk(NF)), dim3(256), 0, s, Count(NF), (const uint32_t*)out.act.p,
+                       (const uint32_t*)fq_a.p, (const uint16_t*)fq_l.p, (const uint8_t*)fq_m.p,
+                       (const uint32_t*)fq_vs.p, (const uint32_t*)fve.p, (const uint32_t*)d.vid.p, NFV,
+                       out.f_vstart.p, out.f_verts.p, out.f_tfstart.p, out.f_tf_tet.p, out.f_tf_local.p,
+                       out.f_func.p);
+    fill_u32(out.f_vstart.p + NF, (uint32_t)NFV, 1, s);
+    fill_u32(out.f_tfstart.p + NF, (uint32_t)NF, 1, s);
+    RIN_HIP(hipGetLastError());
+    tm.mark("faces");
 }
 
 }  // namespace
@@ -246,20 +367,24 @@ void run_pipeline(DevMem& mem, const Grid& g, const Values& val, uint32_t flags,
     vflag.reset();
 
     // bucket counts (reference labels: IA 1/2/>=3, MI 2/3/>=4)
+    std::vector<uint32_t> huge;  // active ranks with more than 8 functions
     {
         std::vector<uint8_t> hk(NA);
         if (NA) RIN_HIP(hipMemcpyAsync(hk.data(), out.act_k.p, NA, hipMemcpyDeviceToHost, s));
         RIN_HIP(hipStreamSynchronize(s));
         int kmax = 0;
-        for (uint8_t k : hk) {
+        for (uint64_t a = 0; a < NA; ++a) {
+            const int k = hk[a];
             const int lo = MI ? 2 : 1;
             out.nb[k <= lo ? 0 : (k == lo + 1 ? 1 : 2)]++;
             kmax = std::max<int>(kmax, k);
+            if (k > 8) huge.push_back((uint32_t)a);
         }
-        if (kmax > 8) {
+        if (kmax > kMaxHuge) {
             out.err |= ERR_CAPACITY;
             throw std::runtime_error("a tet has " + std::to_string(kmax) +
-                                     " active functions; the device cut supports at most 8");
+                                     " active functions; the device cut supports at most " +
+                                     std::to_string(kMaxHuge));
         }
     }
 
@@ -296,6 +421,26 @@ void run_pipeline(DevMem& mem, const Grid& g, const Values& val, uint32_t flags,
         }
         RIN_HIP(hipGetLastError());
     }
+    if (!huge.empty()) {
+        // rare degenerate tets: one lane each, complexes in device memory
+        DBuf<uint32_t> hl(mem, huge.size());
+        RIN_HIP(hipMemcpyAsync(hl.p, huge.data(), 4 * huge.size(), hipMemcpyHostToDevice, s));
+        const unsigned blocks = (unsigned)std::min<size_t>((huge.size() + 63) / 64, 64);
+        DBuf<uint8_t> ws(mem, sizeof(Complex<CHuge>) * 64 * blocks);
+        auto* wsp = reinterpret_cast<Complex<CHuge>*>(ws.p);
+        if (MI)
+            hipLaunchKernelGGL(k_cut_huge<true>, dim3(blocks), dim3(64), 0, s, g.view, (const double*)val.vals.p,
+                               (const uint32_t*)out.act.p, (const uint8_t*)out.act_k.p, (const uint32_t*)out.csr_off.p,
+                               (const uint8_t*)out.fit.p, (const uint32_t*)hl.p, (uint64_t)huge.size(), flags,
+                               (const uint32_t*)out.rec_off.p, out.rec.p, cnt.p, errs.p, wsp);
+        else
+            hipLaunchKernelGGL(k_cut_huge<false>, dim3(blocks), dim3(64), 0, s, g.view, (const double*)val.vals.p,
+                               (const uint32_t*)out.act.p, (const uint8_t*)out.act_k.p, (const uint32_t*)out.csr_off.p,
+                               (const uint8_t*)out.fit.p, (const uint32_t*)hl.p, (uint64_t)huge.size(), flags,
+                               (const uint32_t*)out.rec_off.p, out.rec.p, cnt.p, errs.p, wsp);
+        RIN_HIP(hipGetLastError());
+        RIN_HIP(hipStreamSynchronize(s));
+    }
     tm.mark(MI ? "MI cut" : "simp_arr");
     {
         uint32_t he[64];
@@ -317,8 +462,11 @@ void run_pipeline(DevMem& mem, const Grid& g, const Values& val, uint32_t flags,
         return;
     }
     if (MI) {
+        extract_mi(mem, g, val, out, tm, s);
         tm.finish(out.times);
-        throw std::runtime_error("material-interface extraction is not available on the device path yet");
+        if (!(flags & RUN_KEEP_RECORDS)) out.rec.reset();
+        out.set_views_from_buffers();
+        return;
     }
 
     // ---- extraction
@@ -346,27 +494,12 @@ void run_pipeline(DevMem& mem, const Grid& g, const Values& val, uint32_t flags,
     // ---- vertex dedup: stable sort by key, first occurrence ranks
     const int kbits = 2 + std::max({bits_for(V), bits_for(g.view.n_edge_ids) + 6, bits_for(g.view.n_face_ids) + 12,
                                     bits_for(NA) + 8});
-    if (kbits > 64) throw std::runtime_error("simplex keys exceed 64 bits for this mesh");
-    DBuf<uint32_t> sidx(mem, NVO + 1), first(mem, NVO + 1), head(mem, NVO + 1), hexcl(mem, NVO + 1),
-        seg_first(mem, NVO + 1), vid(mem, NVO + 1);
-    DBuf<uint32_t> nvt(mem, 2);
-    iota_u32(sidx.p, NVO, s);
-    sort_pairs(mem, okey.p, sidx.p, NVO, kbits, s);
-    hipLaunchKernelGGL(k_heads, dim3(nblk(NVO)), dim3(256), 0, s, NVO, (const uint64_t*)okey.p,
-                       (const uint32_t*)sidx.p, first.p, head.p);
-    scan_u32(mem, head.p, hexcl.p, NVO, nullptr, s);
-    hipLaunchKernelGGL(k_assign, dim3(nblk(NVO)), dim3(256), 0, s, NVO, (const uint32_t*)sidx.p,
-                       (const uint32_t*)head.p, (const uint32_t*)hexcl.p, seg_first.p);
-    DBuf<uint32_t> vrank(mem, NVO + 1);
-    scan_u32(mem, first.p, vrank.p, NVO, nvt.p, s);
-    hipLaunchKernelGGL(k_assign2, dim3(nblk(NVO)), dim3(256), 0, s, NVO, (const uint32_t*)sidx.p,
-                       (const uint32_t*)head.p, (const uint32_t*)hexcl.p, (const uint32_t*)seg_first.p,
-                       (const uint32_t*)vrank.p, vid.p);
-    out.n_verts = read1(nvt.p, s);
+    Dedup dd;
+    dedup_vertices(mem, okey, NVO, kbits, dd, s);
+    DBuf<uint32_t>& vid = dd.vid;
+    DBuf<uint32_t>& first_occ = dd.first_occ;
+    out.n_verts = dd.n;
     const uint64_t NV = out.n_verts;
-    DBuf<uint32_t> first_occ(mem, NV + 1);
-    hipLaunchKernelGGL(k_first_list, dim3(nblk(NVO)), dim3(256), 0, s, NVO, (const uint32_t*)first.p,
-                       (const uint32_t*)vrank.p, first_occ.p);
     tm.mark("dedup verts");
     out.v_tet.alloc(mem, NV + 1);
     out.v_j.alloc(mem, NV + 1);

@@ -51,3 +51,15 @@ def test_dropin_robust_mode(grid101):
     out = S.dropin_run(0, pts, tets, vals, robust=True)
     assert out["ok"], out
     assert out["num_verts"] == 0 and out["patches"] == 0  # the -R run stops after the cut
+
+
+@pytest.mark.parametrize("case", KA["mi"], ids=[c["name"] for c in KA["mi"]])
+def test_dropin_mi_known_answer(case, grid101):
+    if "mesh" in case:
+        m = np.load(os.path.join(S.REF_TESTS, case["mesh"]))
+        pts, tets = m["pts"], m["tets"].astype(np.uint64)
+    else:
+        pts, tets = grid101
+    types, params = S.load_ref_funcs(case["funcs"])
+    vals = S.evaluate(types, params, pts)
+    _check(case, S.dropin_run(1, pts, tets, vals))

@@ -90,6 +90,50 @@ def test_pipeline_ia_explicit_mesh(ctx):
     _compare(dev, ref)
 
 
+@pytest.mark.parametrize("which", ["config3_small", "config3", "config4", "config4_shuffled", "config1"])
+def test_pipeline_mi_matches_oracle(ctx, which):
+    """Material interface: highest material, filter, MI cut, extract_MI_mesh
+    (lazy corner vertices, tet-boundary face pairing) and coordinates."""
+    import rin
+
+    if which == "config3_small":
+        _, fs = W.config3()
+        res = 24
+    elif which == "config3":
+        res, fs = W.config3()
+    elif which == "config4":
+        res, fs = W.config4()
+    elif which == "config4_shuffled":
+        res, fs = W.config4(shuffle_seed=44)
+    else:
+        res, fs = W.config1()
+    types, params = fs.arrays()
+    g = ctx.grid_regular(res)
+    vals = g.eval(types, params)
+    dev = vals.run(rin.FLAG_MI)
+    pts, tets = S.grid(res)
+    ref = S.OracleRun(1, pts, tets, S.evaluate(types, params, pts))
+    assert dev.sizes.n_faces > 0
+    _compare(dev, ref)
+    dev.release()
+    vals.release()
+    g.release()
+
+
+def test_pipeline_mi_explicit_mesh(ctx):
+    """The reference's unstructured eight-sphere material-interface case."""
+    import rin
+
+    m = np.load(S.REF_TESTS + "/mesh.npz")
+    pts, tets = m["pts"], m["tets"].astype(np.uint64)
+    types, params = S.load_ref_funcs("8-sphere.json")
+    g = ctx.grid_explicit(pts, tets)
+    vals = g.upload(S.evaluate(types, params, pts))
+    dev = vals.run(rin.FLAG_MI)
+    ref = S.OracleRun(1, pts, tets, S.evaluate(types, params, pts))
+    _compare(dev, ref)
+
+
 @pytest.mark.parametrize("form", ["auto", "wave", "thread"])
 @pytest.mark.parametrize("which", ["config1", "config2_small", "config4"])
 def test_batched_matches_oracle_and_replays(ctx, which, form, monkeypatch):
