@@ -130,6 +130,127 @@ const K1Table& k1_table() {
     return T;
 }
 
+// k = 2: the first plane's complex depends on its corner signs only (the
+// k = 1 table); inserting the second plane depends on that complex and the
+// second plane's sign at each of its vertices only.  Those signs are the
+// second plane's corner signs and, at each vertex where the first plane
+// crosses a tet edge, a sign fixed by the corner signs unless they differ
+// across that edge (then one exact predicate).  Every sign vector the
+// corner signs allow is enumerated once on the host, running the insertion
+// itself; a tet then needs at most four predicates and one table read.
+struct K2Pat {
+    uint32_t base;  // first slot of this first-plane pattern
+    uint8_t nv, valid, pad0, pad1;
+    uint8_t e[4][2];  // corners of the tet edge under vertex 4 + q
+};
+static_assert(sizeof(K2Pat) == 16, "k2 pattern layout");
+
+struct K2Table {
+    K2Pat pat[81];
+    uint32_t nslots = 0;
+    std::vector<uint32_t> off;  // word offset of each slot's record, or ~0u
+    std::vector<uint32_t> cnt;  // iso vertices | iso faces << 8 | iso-face corners << 16
+    std::vector<uint16_t> words;
+};
+constexpr uint32_t K2_NONE = 0xFFFFFFFFu;
+
+const K2Table& k2_table() {
+    static K2Table T;
+    static std::once_flag once;
+    std::call_once(once, [] {
+        std::memset(T.pat, 0, sizeof(T.pat));
+        static dev::Complex<CK2> X1, X2;
+        auto first_plane = [&](int pat) {
+            double P[2][4] = {};
+            for (int c = 0, q = pat; c < 4; ++c, q /= 3) P[0][c] = (double)(q % 3 - 1);
+            dev::ia_init<CK2>(X1, 2);
+            int8_t sg[CK2::MV];
+            dev::ia_classify<CK2>(P, 0, X1, sg);
+            dev::ia_insert<CK2>(X1, 4, sg);
+            if (X1.err) throw std::runtime_error("k2 table: first insertion failed");
+        };
+        uint32_t base = 0;
+        for (int pat = 0; pat < 81; ++pat) {
+            int np = 0, nn = 0;
+            for (int c = 0, q = pat; c < 4; ++c, q /= 3) {
+                np += q % 3 == 2;
+                nn += q % 3 == 0;
+            }
+            K2Pat& K = T.pat[pat];
+            if (np == 4 || nn == 4) continue;
+            first_plane(pat);
+            K.valid = 1;
+            K.nv = (uint8_t)X1.nv;
+            K.base = base;
+            for (int v = 4; v < X1.nv; ++v) {
+                // labels {f1, f2, 4}: the edge joins the two corners other than f1, f2
+                int n = 0;
+                for (int c = 0; c < 4; ++c)
+                    if (c != X1.vl[v][0] && c != X1.vl[v][1]) K.e[v - 4][n++] = (uint8_t)c;
+            }
+            uint32_t p3 = 1;
+            for (int v = 0; v < X1.nv; ++v) p3 *= 3;
+            base += p3;
+        }
+        T.nslots = base;
+        T.off.assign(base, K2_NONE);
+        T.cnt.assign(base, 0);
+        for (int pat = 0; pat < 81; ++pat) {
+            const K2Pat& K = T.pat[pat];
+            if (!K.valid) continue;
+            first_plane(pat);
+            const int ncr = K.nv - 4;
+            for (int sb = 0; sb < 81; ++sb) {
+                int sB[4];
+                for (int c = 0, q = sb; c < 4; ++c, q /= 3) sB[c] = q % 3 - 1;
+                int forced[4], nfree = 0;
+                for (int q = 0; q < ncr; ++q) {
+                    const int a = sB[K.e[q][0]], b = sB[K.e[q][1]];
+                    forced[q] = (a * b < 0) ? 2 : (a == b ? a : (a != 0 ? a : b));
+                    nfree += forced[q] == 2;
+                }
+                int p3f = 1;
+                for (int q = 0; q < nfree; ++q) p3f *= 3;
+                for (int fc = 0; fc < p3f; ++fc) {
+                    int8_t sg[CK2::MV];
+                    for (int c = 0; c < 4; ++c) sg[c] = (int8_t)sB[c];
+                    for (int q = 0, r = fc; q < ncr; ++q) {
+                        if (forced[q] == 2) {
+                            sg[4 + q] = (int8_t)(r % 3 - 1);
+                            r /= 3;
+                        } else {
+                            sg[4 + q] = (int8_t)forced[q];
+                        }
+                    }
+                    uint32_t slot = K.base, w = 1;
+                    for (int v = 0; v < K.nv; ++v, w *= 3) slot += (uint32_t)(sg[v] + 1) * w;
+                    X2 = X1;
+                    dev::ia_insert<CK2>(X2, 5, sg);
+                    if (X2.err) continue;  // the cut fails the same way; the device reports it
+                    uint16_t rec[512];
+                    const int len = dev::write_record_ia<CK2>(X2, 2, rec, 512);
+                    if (len < 0) throw std::runtime_error("k2 table: record overflow");
+                    bool iso_v[CK2::MV] = {false};
+                    uint32_t nif = 0, nfv = 0, niv = 0;
+                    for (int f = 0; f < X2.nf; ++f) {
+                        bool is = X2.fa[f] > 3;
+                        if (!is && X2.dup) is = X2.grp[4] == X2.grp[X2.fa[f]] || X2.grp[5] == X2.grp[X2.fa[f]];
+                        if (!is) continue;
+                        ++nif;
+                        nfv += X2.fn[f];
+                        for (int q = 0; q < X2.fn[f]; ++q) iso_v[X2.fv[f][q]] = true;
+                    }
+                    for (int v = 0; v < X2.nv; ++v) niv += iso_v[v];
+                    T.off[slot] = (uint32_t)T.words.size();
+                    T.cnt[slot] = niv | nif << 8 | nfv << 16;
+                    T.words.insert(T.words.end(), rec, rec + len);
+                }
+            }
+        }
+    });
+    return T;
+}
+
 __device__ inline int bucket_of(int k) { return k <= 1 ? 0 : (k == 2 ? 1 : (k <= 4 ? 2 : (k <= 8 ? 3 : 4))); }
 
 // per tet: active count and bucket histogram
@@ -336,12 +457,81 @@ __global__ void __launch_bounds__(64) k_cut_wave(CutArgs A) {
     }
 }
 
+struct K2Args {
+    GridView g;
+    const double* vals;
+    const uint64_t *pos, *neg;
+    const uint32_t *act, *csr_off, *list;
+    const uint8_t* fit;
+    const uint32_t* n;  // bucket size (device)
+    const uint32_t* abort;
+    uint32_t cap;
+    const K2Pat* pat;
+    const uint32_t *slot_off, *slot_cnt;
+    uint32_t *rec_loc, *cnt, *errs;
+    uint16_t* rec;
+};
+
+// k = 2 by table: sign vector of the second plane over the first plane's
+// complex -> slot -> record and counts (no per-tet record is written)
+__global__ void k_k2_lookup(K2Args A) {
+    __shared__ K2Pat pats[81];
+    for (int i = threadIdx.x; i < 81; i += blockDim.x) pats[i] = A.pat[i];
+    __syncthreads();
+    const uint32_t n = *A.abort ? 0u : min(*A.n, A.cap);
+    const uint64_t V = A.g.V;
+    GRID_STRIDE(i, n) {
+        const uint32_t r = A.list[i];
+        const uint32_t t = A.act[r];
+        const uint32_t s0 = A.csr_off[t];
+        const uint32_t fa = A.fit[s0], fb = A.fit[s0 + 1];
+        uint32_t v[4];
+        dev::tet_corners(A.g, t, v);
+        int pa = 0, sB[4];
+        for (int c = 0, w = 1; c < 4; ++c, w *= 3) {
+            const uint64_t p = A.pos[v[c]], q = A.neg[v[c]];
+            pa += (((p >> fa) & 1) ? 2 : (((q >> fa) & 1) ? 0 : 1)) * w;
+            sB[c] = ((p >> fb) & 1) ? 1 : (((q >> fb) & 1) ? -1 : 0);
+        }
+        const K2Pat& K = pats[pa];
+        uint32_t slot = K.base, w = 81;
+        for (int c = 0, w3 = 1; c < 4; ++c, w3 *= 3) slot += (uint32_t)(sB[c] + 1) * w3;
+        bool err = !K.valid;
+        for (int q = 0; q + 4 < K.nv; ++q, w *= 3) {
+            const int c1 = K.e[q][0], c2 = K.e[q][1];
+            int sg;
+            if (sB[c1] * sB[c2] < 0) {
+                // second plane at the point of edge c1 c2 where the first vanishes
+                sg = dev::orient_ia_edge(A.vals[(uint64_t)fa * V + v[c1]], A.vals[(uint64_t)fa * V + v[c2]],
+                                         A.vals[(uint64_t)fb * V + v[c1]], A.vals[(uint64_t)fb * V + v[c2]], err);
+            } else {
+                sg = sB[c1] == sB[c2] ? sB[c1] : (sB[c1] != 0 ? sB[c1] : sB[c2]);
+            }
+            slot += (uint32_t)(sg + 1) * w;
+        }
+        const uint32_t off = err ? K2_NONE : A.slot_off[slot];
+        if (off == K2_NONE) {
+            atomicOr(A.errs, (uint32_t)ERR_CAPACITY);
+            A.rec[A.rec_loc[r]] = 0;
+            A.cnt[3 * r] = A.cnt[3 * r + 1] = A.cnt[3 * r + 2] = 0;
+            continue;
+        }
+        const uint32_t c3 = A.slot_cnt[slot];
+        A.rec_loc[r] = K1_FLAG | (81u + slot);
+        A.cnt[3 * r] = c3 & 0xFF;
+        A.cnt[3 * r + 1] = (c3 >> 8) & 0xFF;
+        A.cnt[3 * r + 2] = c3 >> 16;
+    }
+}
+
 struct ExtractArgs {
     GridView g;
     const uint32_t *act, *csr_off, *rec_loc;
     const uint8_t* fit;
     const uint16_t* rec;
     const K1Entry* k1;
+    const uint16_t* tab;       // table records (k = 1 patterns, then k = 2 slots)
+    const uint32_t* tab_off;
     const uint32_t* n_active;
     uint32_t cap_active;
     uint32_t* ctr;
@@ -377,7 +567,7 @@ __global__ void k_extract_batched(ExtractArgs A) {
         uint32_t v[4];
         dev::tet_corners(A.g, t, v);
         const uint32_t s0 = A.csr_off[t];
-        if (loc & K1_FLAG) {
+        if ((loc & K1_FLAG) && (loc & ~K1_FLAG) < 81) {
             const K1Entry& E = tab[loc & 0xFF];
             const uint32_t f = A.fit[s0];
             uint32_t o = A.voff[a];
@@ -401,7 +591,7 @@ __global__ void k_extract_batched(ExtractArgs A) {
             continue;
         }
         // general record (same layout and rules as the plain extraction)
-        const Rec R(A.rec + loc);
+        const Rec R((loc & K1_FLAG) ? A.tab + A.tab_off[loc & ~K1_FLAG] : A.rec + loc);
         if (R.r[0] != 1) continue;
         uint64_t iso[4] = {0, 0, 0, 0};
         const uint16_t* F = R.faces();
@@ -552,8 +742,10 @@ struct Workspace {
     DBuf<double> xyz;
     DBuf<uint32_t> ctr, errs;
     DBuf<K1Entry> k1;
-    DBuf<uint16_t> k1_words;
-    DBuf<uint32_t> k1_off;
+    DBuf<uint16_t> k1_words;  // table records: k = 1 patterns, then k = 2 slots
+    DBuf<uint32_t> k1_off;    // word offset per table index (81 + k = 2 slots)
+    DBuf<K2Pat> k2pat;
+    DBuf<uint32_t> k2cnt;
     // the path as a few captured graphs launched back to back; stream events
     // between them time each part on every replay
     std::vector<std::pair<std::string, hipGraph_t>> graphs;
@@ -640,14 +832,22 @@ void alloc_ws(Workspace& W, const Grid& g, const Caps_& c) {
     W.ctr.alloc(m, 32);
     W.errs.alloc(m, 64);
     const K1Table& K = k1_table();
+    const K2Table& K2 = k2_table();
     W.k1.alloc(m, 81);
-    W.k1_words.alloc(m, K.words.size() + 1);
-    W.k1_off.alloc(m, 81);
-    std::vector<uint32_t> off(81);
+    W.k1_words.alloc(m, K.words.size() + K2.words.size() + 1);
+    W.k1_off.alloc(m, 81 + K2.nslots);
+    W.k2pat.alloc(m, 81);
+    W.k2cnt.alloc(m, K2.nslots + 1);
+    std::vector<uint32_t> off(81 + K2.nslots);
     for (int i = 0; i < 81; ++i) off[i] = K.e[i].rec_off;
+    for (uint32_t i = 0; i < K2.nslots; ++i)
+        off[81 + i] = K2.off[i] == K2_NONE ? K2_NONE : (uint32_t)K.words.size() + K2.off[i];
     RIN_HIP(hipMemcpy(W.k1.p, K.e, sizeof(K.e), hipMemcpyHostToDevice));
     RIN_HIP(hipMemcpy(W.k1_words.p, K.words.data(), 2 * K.words.size(), hipMemcpyHostToDevice));
-    RIN_HIP(hipMemcpy(W.k1_off.p, off.data(), 4 * 81, hipMemcpyHostToDevice));
+    RIN_HIP(hipMemcpy(W.k1_words.p + K.words.size(), K2.words.data(), 2 * K2.words.size(), hipMemcpyHostToDevice));
+    RIN_HIP(hipMemcpy(W.k1_off.p, off.data(), 4 * off.size(), hipMemcpyHostToDevice));
+    RIN_HIP(hipMemcpy(W.k2pat.p, K2.pat, sizeof(K2.pat), hipMemcpyHostToDevice));
+    RIN_HIP(hipMemcpy(W.k2cnt.p, K2.cnt.data(), 4 * K2.cnt.size(), hipMemcpyHostToDevice));
 }
 
 // stage 1 (also run eagerly to size the workspace): signs, filter, CSR scan
@@ -684,6 +884,29 @@ void launch_cut(Workspace& W, const Grid& g, const Values& val, int q, int grid,
     A.rec = W.rec.p;
     A.cnt = W.cnt.p;
     A.errs = W.errs.p;
+    if (grid == -2) {  // k = 2 table
+        K2Args K;
+        K.g = g.view;
+        K.vals = val.vals.p;
+        K.pos = W.pos.p;
+        K.neg = W.neg.p;
+        K.act = W.act.p;
+        K.csr_off = W.csr_off.p;
+        K.list = W.list[q].p;
+        K.fit = W.fit.p;
+        K.n = A.n;
+        K.abort = A.abort;
+        K.cap = A.cap;
+        K.pat = W.k2pat.p;
+        K.slot_off = W.k1_off.p + 81;
+        K.slot_cnt = W.k2cnt.p;
+        K.rec_loc = W.rec_loc.p;
+        K.cnt = W.cnt.p;
+        K.errs = W.errs.p;
+        K.rec = W.rec.p;
+        hipLaunchKernelGGL(k_k2_lookup, dim3(nblk(W.cap.b[q])), dim3(256), 0, s, K);
+        return;
+    }
     if (grid < 0) {  // wave form: one block per bucket entry
         hipLaunchKernelGGL(k_cut_wave<C>, dim3(W.cap.b[q]), dim3(64), 0, s, A);
         return;
@@ -747,6 +970,8 @@ void stage_rest(Workspace& W, const Grid& g, const Values& val, hipStream_t s) {
     E.fit = W.fit.p;
     E.rec = W.rec.p;
     E.k1 = W.k1.p;
+    E.tab = W.k1_words.p;
+    E.tab_off = W.k1_off.p;
     E.n_active = ctr + C_ACTIVE;
     E.cap_active = c.active;
     E.ctr = ctr;
@@ -939,16 +1164,21 @@ void run_batched(DevMem& ctx_mem, const Grid& g, Values& val, uint32_t flags, Ru
                 RIN_HIP(hipStreamSynchronize(s));
                 // thread form at 1..16 blocks per CU, or the wave form (-1);
                 // RIN_CUT_FORM=wave|thread restricts the choice
-                const int cand[6] = {1, 2, 4, 8, 16, -1};
+                // k = 2 also by table (-2); RIN_CUT_FORM=table forces that
+                const int cand[7] = {1, 2, 4, 8, 16, -1, -2};
                 const char* form = std::getenv("RIN_CUT_FORM");
                 const bool only_wave = form && !std::strcmp(form, "wave");
                 const bool only_thread = form && !std::strcmp(form, "thread");
+                const bool only_table = form && !std::strcmp(form, "table");
                 for (int q = 0; q < 3; ++q) {
                     float best = 1e30f;
-                    int bestc = only_wave ? -1 : 4;
-                    for (int ci = 0; ci < 6; ++ci) {
-                        if ((only_wave && cand[ci] >= 0) || (only_thread && cand[ci] < 0)) continue;
-                        const int grid = cand[ci] < 0 ? -1 : std::max(1, n_cu * cand[ci]);
+                    int bestc = only_wave ? -1 : (only_table && q == 0 ? -2 : 4);
+                    for (int ci = 0; ci < 7; ++ci) {
+                        const int c = cand[ci];
+                        if (c == -2 && (q != 0 || only_wave || only_thread)) continue;
+                        if (only_table && q == 0 && c != -2) continue;
+                        if ((only_wave && c >= 0) || (only_thread && c < 0)) continue;
+                        const int grid = c < 0 ? c : std::max(1, n_cu * c);
                         float ms = time_ms(s, [&] {
                             if (q == 0 && w.cap.b[0]) launch_cut<CK2>(w, g, val, 0, grid, s);
                             if (q == 1 && w.cap.b[1]) launch_cut<CMed>(w, g, val, 1, grid, s);
@@ -963,7 +1193,8 @@ void run_batched(DevMem& ctx_mem, const Grid& g, Values& val, uint32_t flags, Ru
                 }
             }
             int grid[3];
-            for (int q = 0; q < 3; ++q) grid[q] = P.blocks_per_cu[q] < 0 ? -1 : std::max(1, n_cu * P.blocks_per_cu[q]);
+            for (int q = 0; q < 3; ++q)
+                grid[q] = P.blocks_per_cu[q] < 0 ? P.blocks_per_cu[q] : std::max(1, n_cu * P.blocks_per_cu[q]);
             // capture the whole path
             RIN_HIP(hipStreamSynchronize(s));
             auto capture = [&](const std::string& label, const std::function<void()>& body) {
@@ -979,9 +1210,10 @@ void run_batched(DevMem& ctx_mem, const Grid& g, Values& val, uint32_t flags, Ru
             capture("signs + filter + CSR scans", [&] { stage_count(w, g, val, s); });
             capture("CSR fill + buckets + k=1 table", [&] { stage_fill(w, g, val, s); });
             auto cut_label = [&](const char* base, int q) {
-                return std::string(base) + (grid[q] < 0 ? " (wave per tet)"
-                                                        : " (thread per tet, " + std::to_string(P.blocks_per_cu[q]) +
-                                                              " blocks/CU)");
+                return std::string(base) + (grid[q] == -2 ? " (sign-vector table)"
+                                            : grid[q] < 0 ? " (wave per tet)"
+                                                          : " (thread per tet, " + std::to_string(P.blocks_per_cu[q]) +
+                                                                " blocks/CU)");
             };
             if (w.cap.b[0]) capture(cut_label("cut k=2", 0), [&] { launch_cut<CK2>(w, g, val, 0, grid[0], s); });
             if (w.cap.b[1]) capture(cut_label("cut k=3..4", 1), [&] { launch_cut<CMed>(w, g, val, 1, grid[1], s); });

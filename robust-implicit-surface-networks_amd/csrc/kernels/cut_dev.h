@@ -69,8 +69,10 @@ __host__ __device__ inline void sort4(uint8_t* v) {
 
 // ------------------------------------------------------------------ IA
 
+// the tet before any insertion: 4 corners, 4 faces, 1 cell; err when k
+// planes exceed the label capacity
 template <class C>
-__host__ __device__ void ia_cut(const double (*P)[4], int k, Complex<C>& X) {
+__host__ __device__ void ia_init(Complex<C>& X, int k) {
     X.err = false;
     X.dup = false;
     X.nv = 4;
@@ -96,22 +98,25 @@ __host__ __device__ void ia_cut(const double (*P)[4], int k, Complex<C>& X) {
         X.grp[p] = p < 4 ? p : NIL;
         X.ori[p] = 1;
     }
-    if (4 + k > C::MP) {
-        X.err = true;
-        return;
-    }
-    int8_t s[C::MV];
-    for (int j = 0; j < k; ++j) {
-        const uint8_t Pid = 4 + j;
-        const int nv0 = X.nv;
+    if (4 + k > C::MP) X.err = true;
+}
+
+// sign of plane j at every vertex (exact)
+template <class C>
+__host__ __device__ void ia_classify(const double (*P)[4], int j, Complex<C>& X, int8_t* s) {
+    for (int v = 0; v < X.nv; ++v) s[v] = (int8_t)orient_ia(P, X.vl[v][0], X.vl[v][1], X.vl[v][2], P[j], X.err);
+}
+
+// inserts plane Pid given its sign s[v] at every vertex; the result depends
+// on the complex and the signs only
+template <class C>
+__host__ __device__ void ia_insert(Complex<C>& X, uint8_t Pid, int8_t* s) {
+    {
         bool any_pos = false, any_neg = false;
-        for (int v = 0; v < nv0; ++v) {
-            int sv = orient_ia(P, X.vl[v][0], X.vl[v][1], X.vl[v][2], P[j], X.err);
-            s[v] = (int8_t)sv;
-            any_pos |= sv > 0;
-            any_neg |= sv < 0;
+        for (int v = 0; v < X.nv; ++v) {
+            any_pos |= s[v] > 0;
+            any_neg |= s[v] < 0;
         }
-        if (X.err) return;
         int cop = -1;
         for (int f = 0; f < X.nf && cop < 0; ++f) {
             bool allz = true;
@@ -133,186 +138,198 @@ __host__ __device__ void ia_cut(const double (*P)[4], int k, Complex<C>& X) {
             X.grp[Pid] = X.grp[pl];
             X.ori[Pid] = (sv > 0) ? X.ori[pl] : (uint8_t)(1 - X.ori[pl]);
             X.dup = true;
-            continue;
+            return;
         }
         X.grp[Pid] = Pid;
         X.ori[Pid] = 1;
-        if (!any_pos || !any_neg) continue;
-
-        // one new vertex per edge with a strict sign change
-        uint8_t cu[C::MV], cw[C::MV], cx[C::MV];
-        int ncross = 0;
-        for (int f = 0; f < X.nf; ++f) {
-            const int n = X.fn[f];
-            for (int i = 0; i < n; ++i) {
-                uint8_t u = X.fv[f][i], w = X.fv[f][(i + 1) % n];
-                if (s[u] * s[w] >= 0) continue;
-                uint8_t a = u < w ? u : w, b = u < w ? w : u;
-                bool have = false;
-                for (int q = 0; q < ncross; ++q) have |= (cu[q] == a && cw[q] == b);
-                if (have) continue;
-                if (X.nv >= C::MV) {
-                    X.err = true;
-                    return;
-                }
-                uint8_t t0 = X.fa[f], t1 = X.fe[f][i], t2 = Pid;
-                sort3(t0, t1, t2);
-                X.vl[X.nv][0] = t0;
-                X.vl[X.nv][1] = t1;
-                X.vl[X.nv][2] = t2;
-                X.vl[X.nv][3] = NIL;
-                s[X.nv] = 0;
-                cu[ncross] = a;
-                cw[ncross] = b;
-                cx[ncross] = (uint8_t)X.nv;
-                ++ncross;
-                ++X.nv;
-            }
-        }
-        auto cross_of = [&](uint8_t u, uint8_t w) -> uint8_t {
+        if (!any_pos || !any_neg) return;
+    }
+    // one new vertex per edge with a strict sign change
+    uint8_t cu[C::MV], cw[C::MV], cx[C::MV];
+    int ncross = 0;
+    for (int f = 0; f < X.nf; ++f) {
+        const int n = X.fn[f];
+        for (int i = 0; i < n; ++i) {
+            uint8_t u = X.fv[f][i], w = X.fv[f][(i + 1) % n];
+            if (s[u] * s[w] >= 0) continue;
             uint8_t a = u < w ? u : w, b = u < w ? w : u;
-            for (int q = 0; q < ncross; ++q)
-                if (cu[q] == a && cw[q] == b) return cx[q];
-            return NIL;
-        };
-
-        // split faces: positive part keeps the id, negative part appended
-        const int nf0 = X.nf;
-        for (int f = 0; f < nf0; ++f) {
-            const int n = X.fn[f];
-            bool fp = false, fq = false;
-            for (int q = 0; q < n; ++q) {
-                fp |= s[X.fv[f][q]] > 0;
-                fq |= s[X.fv[f][q]] < 0;
-            }
-            if (!(fp && fq)) continue;
-            if (X.nf >= C::MF) {
+            bool have = false;
+            for (int q = 0; q < ncross; ++q) have |= (cu[q] == a && cw[q] == b);
+            if (have) continue;
+            if (X.nv >= C::MV) {
                 X.err = true;
                 return;
             }
-            uint8_t av[2 * C::MFV], ae[2 * C::MFV];
-            int m = 0;
-            for (int i = 0; i < n; ++i) {
-                uint8_t u = X.fv[f][i], w = X.fv[f][(i + 1) % n];
-                av[m] = u;
+            uint8_t t0 = X.fa[f], t1 = X.fe[f][i], t2 = Pid;
+            sort3(t0, t1, t2);
+            X.vl[X.nv][0] = t0;
+            X.vl[X.nv][1] = t1;
+            X.vl[X.nv][2] = t2;
+            X.vl[X.nv][3] = NIL;
+            s[X.nv] = 0;
+            cu[ncross] = a;
+            cw[ncross] = b;
+            cx[ncross] = (uint8_t)X.nv;
+            ++ncross;
+            ++X.nv;
+        }
+    }
+    auto cross_of = [&](uint8_t u, uint8_t w) -> uint8_t {
+        uint8_t a = u < w ? u : w, b = u < w ? w : u;
+        for (int q = 0; q < ncross; ++q)
+            if (cu[q] == a && cw[q] == b) return cx[q];
+        return NIL;
+    };
+
+    // split faces: positive part keeps the id, negative part appended
+    const int nf0 = X.nf;
+    for (int f = 0; f < nf0; ++f) {
+        const int n = X.fn[f];
+        bool fp = false, fq = false;
+        for (int q = 0; q < n; ++q) {
+            fp |= s[X.fv[f][q]] > 0;
+            fq |= s[X.fv[f][q]] < 0;
+        }
+        if (!(fp && fq)) continue;
+        if (X.nf >= C::MF) {
+            X.err = true;
+            return;
+        }
+        uint8_t av[2 * C::MFV], ae[2 * C::MFV];
+        int m = 0;
+        for (int i = 0; i < n; ++i) {
+            uint8_t u = X.fv[f][i], w = X.fv[f][(i + 1) % n];
+            av[m] = u;
+            ae[m] = X.fe[f][i];
+            ++m;
+            if (s[u] * s[w] < 0) {
+                av[m] = cross_of(u, w);
                 ae[m] = X.fe[f][i];
                 ++m;
-                if (s[u] * s[w] < 0) {
-                    av[m] = cross_of(u, w);
-                    ae[m] = X.fe[f][i];
-                    ++m;
-                }
             }
-            for (int sgn = -1; sgn <= 1; sgn += 2) {
-                const int dst = (sgn > 0) ? f : X.nf;
-                int idx[2 * C::MFV], ni = 0;
-                for (int i = 0; i < m; ++i)
-                    if (s[av[i]] * sgn >= 0) idx[ni++] = i;
-                if (ni > C::MFV) {
-                    X.err = true;
-                    return;
-                }
-                uint8_t nvv[C::MFV], nee[C::MFV];
-                for (int q = 0; q < ni; ++q) {
-                    int ia = idx[q], ib = idx[(q + 1) % ni];
-                    nvv[q] = av[ia];
-                    nee[q] = ((ia + 1) % m == ib) ? ae[ia] : Pid;
-                }
-                for (int q = 0; q < ni; ++q) {
-                    X.fv[dst][q] = nvv[q];
-                    X.fe[dst][q] = nee[q];
-                }
-                X.fn[dst] = (uint8_t)ni;
-                if (sgn < 0) {
-                    X.fa[dst] = X.fa[f];
-                    X.fb[dst] = X.fb[f];
-                    X.fc[dst] = X.fc[f];
-                }
-            }
-            ++X.nf;
         }
+        for (int sgn = -1; sgn <= 1; sgn += 2) {
+            const int dst = (sgn > 0) ? f : X.nf;
+            int idx[2 * C::MFV], ni = 0;
+            for (int i = 0; i < m; ++i)
+                if (s[av[i]] * sgn >= 0) idx[ni++] = i;
+            if (ni > C::MFV) {
+                X.err = true;
+                return;
+            }
+            uint8_t nvv[C::MFV], nee[C::MFV];
+            for (int q = 0; q < ni; ++q) {
+                int ia = idx[q], ib = idx[(q + 1) % ni];
+                nvv[q] = av[ia];
+                nee[q] = ((ia + 1) % m == ib) ? ae[ia] : Pid;
+            }
+            for (int q = 0; q < ni; ++q) {
+                X.fv[dst][q] = nvv[q];
+                X.fe[dst][q] = nee[q];
+            }
+            X.fn[dst] = (uint8_t)ni;
+            if (sgn < 0) {
+                X.fa[dst] = X.fa[f];
+                X.fb[dst] = X.fb[f];
+                X.fc[dst] = X.fc[f];
+            }
+        }
+        ++X.nf;
+    }
 
-        // split cells: positive side keeps the id, negative side appended,
-        // one face on the new plane per split cell
-        const int nc0 = X.nc;
-        for (int c = 0; c < nc0; ++c) {
-            bool cp = false, cq = false;
-            for (int g = 0; g < X.nf; ++g) {
-                if (X.fb[g] != c && X.fc[g] != c) continue;
-                for (int q = 0; q < X.fn[g]; ++q) {
-                    cp |= s[X.fv[g][q]] > 0;
-                    cq |= s[X.fv[g][q]] < 0;
-                }
+    // split cells: positive side keeps the id, negative side appended,
+    // one face on the new plane per split cell
+    const int nc0 = X.nc;
+    for (int c = 0; c < nc0; ++c) {
+        bool cp = false, cq = false;
+        for (int g = 0; g < X.nf; ++g) {
+            if (X.fb[g] != c && X.fc[g] != c) continue;
+            for (int q = 0; q < X.fn[g]; ++q) {
+                cp |= s[X.fv[g][q]] > 0;
+                cq |= s[X.fv[g][q]] < 0;
             }
-            if (!(cp && cq)) continue;
-            if (X.nf >= C::MF || X.nc >= C::MP * 8 || X.nc >= 254) {
-                X.err = true;
-                return;
-            }
-            const uint8_t c2 = (uint8_t)X.nc++;
-            for (int g = 0; g < X.nf; ++g) {
-                if (X.fb[g] != c && X.fc[g] != c) continue;
-                bool gn = false;
-                for (int q = 0; q < X.fn[g]; ++q) gn |= s[X.fv[g][q]] < 0;
-                if (!gn) continue;
-                if (X.fb[g] == c) X.fb[g] = c2;
-                if (X.fc[g] == c) X.fc[g] = c2;
-            }
-            uint8_t ea[C::MFV], eb[C::MFV], el[C::MFV];
-            int ne = 0;
-            for (int g = 0; g < X.nf; ++g) {
-                if (X.fb[g] != c && X.fc[g] != c) continue;
-                const int n = X.fn[g];
-                for (int i = 0; i < n; ++i) {
-                    uint8_t a = X.fv[g][i], b = X.fv[g][(i + 1) % n];
-                    if (s[a] != 0 || s[b] != 0) continue;
-                    if (ne >= C::MFV) {
-                        X.err = true;
-                        return;
-                    }
-                    if (X.fb[g] == c) {
-                        ea[ne] = b;
-                        eb[ne] = a;
-                    } else {
-                        ea[ne] = a;
-                        eb[ne] = b;
-                    }
-                    el[ne] = X.fa[g];
-                    ++ne;
-                }
-            }
-            if (ne < 3) {
-                X.err = true;
-                return;
-            }
-            const int F = X.nf++;
-            X.fa[F] = Pid;
-            X.fb[F] = (uint8_t)c;
-            X.fc[F] = c2;
-            int cur = 0, cnt = 0;
-            for (;;) {
-                X.fv[F][cnt] = ea[cur];
-                X.fe[F][cnt] = el[cur];
-                ++cnt;
-                int nxt = -1;
-                for (int q = 0; q < ne; ++q)
-                    if (ea[q] == eb[cur]) {
-                        nxt = q;
-                        break;
-                    }
-                if (nxt < 0 || cnt > ne) {
+        }
+        if (!(cp && cq)) continue;
+        if (X.nf >= C::MF || X.nc >= C::MP * 8 || X.nc >= 254) {
+            X.err = true;
+            return;
+        }
+        const uint8_t c2 = (uint8_t)X.nc++;
+        for (int g = 0; g < X.nf; ++g) {
+            if (X.fb[g] != c && X.fc[g] != c) continue;
+            bool gn = false;
+            for (int q = 0; q < X.fn[g]; ++q) gn |= s[X.fv[g][q]] < 0;
+            if (!gn) continue;
+            if (X.fb[g] == c) X.fb[g] = c2;
+            if (X.fc[g] == c) X.fc[g] = c2;
+        }
+        uint8_t ea[C::MFV], eb[C::MFV], el[C::MFV];
+        int ne = 0;
+        for (int g = 0; g < X.nf; ++g) {
+            if (X.fb[g] != c && X.fc[g] != c) continue;
+            const int n = X.fn[g];
+            for (int i = 0; i < n; ++i) {
+                uint8_t a = X.fv[g][i], b = X.fv[g][(i + 1) % n];
+                if (s[a] != 0 || s[b] != 0) continue;
+                if (ne >= C::MFV) {
                     X.err = true;
                     return;
                 }
-                cur = nxt;
-                if (cur == 0) break;
+                if (X.fb[g] == c) {
+                    ea[ne] = b;
+                    eb[ne] = a;
+                } else {
+                    ea[ne] = a;
+                    eb[ne] = b;
+                }
+                el[ne] = X.fa[g];
+                ++ne;
             }
-            if (cnt != ne) {
+        }
+        if (ne < 3) {
+            X.err = true;
+            return;
+        }
+        const int F = X.nf++;
+        X.fa[F] = Pid;
+        X.fb[F] = (uint8_t)c;
+        X.fc[F] = c2;
+        int cur = 0, cnt = 0;
+        for (;;) {
+            X.fv[F][cnt] = ea[cur];
+            X.fe[F][cnt] = el[cur];
+            ++cnt;
+            int nxt = -1;
+            for (int q = 0; q < ne; ++q)
+                if (ea[q] == eb[cur]) {
+                    nxt = q;
+                    break;
+                }
+            if (nxt < 0 || cnt > ne) {
                 X.err = true;
                 return;
             }
-            X.fn[F] = (uint8_t)cnt;
+            cur = nxt;
+            if (cur == 0) break;
         }
+        if (cnt != ne) {
+            X.err = true;
+            return;
+        }
+        X.fn[F] = (uint8_t)cnt;
+    }
+}
+
+template <class C>
+__host__ __device__ void ia_cut(const double (*P)[4], int k, Complex<C>& X) {
+    ia_init(X, k);
+    if (X.err) return;
+    int8_t s[C::MV];
+    for (int j = 0; j < k; ++j) {
+        ia_classify(P, j, X, s);
+        if (X.err) return;
+        ia_insert(X, (uint8_t)(4 + j), s);
+        if (X.err) return;
     }
 }
 

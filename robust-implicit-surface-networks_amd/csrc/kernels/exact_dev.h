@@ -226,6 +226,16 @@ __host__ __device__ inline int det_sign(double (&M)[N][N], bool& err) {
 // Reduced by the boundary planes: the free corners are those whose
 // coordinate is not pinned, the vertex is the null vector of the implicit
 // rows restricted to them, and sign = sign(det[rows; q]) * sign(det[rows; 1]).
+// Sign of plane q at the point of tet edge (c1, c2) where plane f vanishes,
+// from their values at the edge's corners (c1 < c2).
+__host__ __device__ inline int orient_ia_edge(double f1, double f2, double q1, double q2, bool& err) {
+    double A[2][2] = {{f1, f2}, {q1, q2}};
+    double B[2][2] = {{f1, f2}, {1.0, 1.0}};
+    int a = det_sign<2>(A, err), b = det_sign<2>(B, err);
+    if (b == 0) err = true;
+    return a * b;
+}
+
 __host__ __device__ inline int orient_ia(const double (*P)[4], int t0, int t1, int t2, const double* q, bool& err) {
     const int t[3] = {t0, t1, t2};
     bool pinned[4] = {false, false, false, false};
@@ -244,14 +254,7 @@ __host__ __device__ inline int orient_ia(const double (*P)[4], int t0, int t1, i
         double v = q[fr[0]];
         return v > 0 ? 1 : (v < 0 ? -1 : 0);
     }
-    if (ni == 1) {
-        const double* f = P[impl[0]];
-        double A[2][2] = {{f[fr[0]], f[fr[1]]}, {q[fr[0]], q[fr[1]]}};
-        double B[2][2] = {{f[fr[0]], f[fr[1]]}, {1.0, 1.0}};
-        int a = det_sign<2>(A, err), b = det_sign<2>(B, err);
-        if (b == 0) err = true;
-        return a * b;
-    }
+    if (ni == 1) return orient_ia_edge(P[impl[0]][fr[0]], P[impl[0]][fr[1]], q[fr[0]], q[fr[1]], err);
     if (ni == 2) {
         double A[3][3], B[3][3];
         for (int r = 0; r < 2; ++r)

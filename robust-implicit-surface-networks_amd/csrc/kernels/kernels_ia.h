@@ -503,9 +503,10 @@ __global__ void k_vert_fields_ia(GridView g, const double* __restrict__ vals, co
         const uint32_t a = oa[o];
         const int j = oj[o];
         const uint32_t t = act[a];
-        // a record in the workspace, or (bit 31 set) the k=1 record of a sign pattern
+        // a record in the workspace, or (bit 31 set) a table record: the k = 1
+        // record of a sign pattern or the k = 2 record of a sign-vector slot
         const uint32_t loc = rec_loc[a];
-        const Rec R((loc & 0x80000000u) ? k1_words + k1_off[loc & 0xFF] : rec + loc);
+        const Rec R((loc & 0x80000000u) ? k1_words + k1_off[loc & 0x7FFFFFFFu] : rec + loc);
         uint32_t v[4];
         dev::tet_corners(g, t, v);
         const uint32_t s0 = csr_off[t];

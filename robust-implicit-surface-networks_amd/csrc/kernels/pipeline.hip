@@ -669,8 +669,8 @@ __global__ void k_rec_lookup(const uint32_t* __restrict__ tets, uint64_t n, uint
             if (rk < n_active && act[rk] == t) {
                 const uint32_t loc = rec_loc ? rec_loc[rk] : 0u;
                 if (rec_loc && (loc & 0x80000000u)) {
-                    o = (1ull << 63) | k1_off[loc & 0xFFu];
-                    L = k1_rec[k1_off[loc & 0xFFu] + 6];
+                    o = (1ull << 63) | k1_off[loc & 0x7FFFFFFFu];
+                    L = k1_rec[k1_off[loc & 0x7FFFFFFFu] + 6];
                 } else {
                     o = loc;
                     L = rec[loc + 6];

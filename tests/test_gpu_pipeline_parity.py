@@ -134,13 +134,14 @@ def test_pipeline_mi_explicit_mesh(ctx):
     _compare(dev, ref)
 
 
-@pytest.mark.parametrize("form", ["auto", "wave", "thread"])
+@pytest.mark.parametrize("form", ["auto", "wave", "thread", "table"])
 @pytest.mark.parametrize("which", ["config1", "config2_small", "config4"])
 def test_batched_matches_oracle_and_replays(ctx, which, form, monkeypatch):
     """The batched form (k=1 table, bucket kernels, captured graph) gives the
     oracle's output bit for bit, on the first (capturing) run and on graph
     replays with plain-form runs between them; the bucket cut kernels are the
-    autotuned choice, or forced to the wave-per-tet or thread-per-tet form."""
+    autotuned choice, or forced to the wave-per-tet or thread-per-tet form, or
+    k = 2 by the sign-vector table."""
     import rin
 
     if form != "auto":
