@@ -37,7 +37,7 @@ constexpr uint32_t ERR_OVERFLOW = 8;
 // counters in the workspace (device)
 enum Ctr : int {
     C_ACTIVE = 0, C_FIT, C_VO, C_FO, C_FV, C_VERTS, C_FACES, C_FACEVERTS, C_B1, C_B2, C_B3, C_B5, C_BX, C_DEGEN,
-    C_Q2, C_Q3, C_Q5, C_ABORT, C_N
+    C_Q2, C_Q3, C_Q5, C_ABORT, C_TETFACES, C_N
 };
 
 struct K1Entry {
@@ -120,7 +120,7 @@ const K1Table& k1_table() {
             E.nif = (uint8_t)niso_f;
             if (isof >= 0) {
                 E.fl = (uint8_t)isof;
-                E.fb = X.fc[isof] == dev::NIL;
+                E.fb = X.fc[isof] == dev::NIL ? (uint8_t)(1 + X.fa[isof]) : 0;
                 E.fn = X.fn[isof];
                 for (int q = 0; q < X.fn[isof]; ++q) E.fv[q] = (uint8_t)idx[X.fv[isof][q]];
                 E.nfv = E.fn;
@@ -652,7 +652,7 @@ __global__ void k_extract_batched(ExtractArgs A) {
                 A.fq_a[q] = (uint32_t)a;
                 A.fq_l[q] = (uint16_t)f;
                 A.fq_fn[q] = A.fit[s0 + sp - 4];
-                A.fq_bnd[q] = F[2] == 0xFFFF;
+                A.fq_bnd[q] = F[2] == 0xFFFF ? (uint8_t)(1 + F[0]) : 0;
                 A.fq_vs[q] = e;
                 for (int i = 0; i < n; ++i) {
                     const int j = F[4 + i];
@@ -668,19 +668,13 @@ __global__ void k_extract_batched(ExtractArgs A) {
     }
 }
 
-__global__ void k_face_end(Count nfo, const uint32_t* __restrict__ fq_vs, const uint32_t* __restrict__ nfv,
-                           uint32_t* __restrict__ fq_end) {
-    GRID_STRIDE(q, nfo) fq_end[q] = (q + 1 < q_end) ? fq_vs[q + 1] : *nfv;
-}
-
-// CSR terminators: csr_off[T], face vertex and tet-face offsets at n_faces
 __global__ void k_write_totals(const uint32_t* ctr, uint64_t T, uint32_t* csr_off, uint32_t* f_vstart,
                                uint32_t* f_tfstart, uint32_t cap_fo) {
     if (threadIdx.x == 0) {
         csr_off[T] = ctr[C_FIT];
         const uint32_t nf = ctr[C_ABORT] ? 0u : min(ctr[C_FACES], cap_fo);
         f_vstart[nf] = ctr[C_FACEVERTS];
-        f_tfstart[nf] = ctr[C_ABORT] ? 0u : min(ctr[C_FO], cap_fo);
+        f_tfstart[nf] = ctr[C_ABORT] ? 0u : min(ctr[C_TETFACES], cap_fo);
     }
 }
 
@@ -729,12 +723,11 @@ struct Workspace {
     DBuf<uint64_t> okey;
     DBuf<uint32_t> oa;
     DBuf<uint16_t> oj;
-    DBuf<uint32_t> fq_a, fq_vs, fq_end, fve;
+    DBuf<uint32_t> fq_a, fq_vs, fve;
     DBuf<uint16_t> fq_l;
     DBuf<uint8_t> fq_fn, fq_bnd;
     DBuf<uint32_t> sidx, first, head, hexcl, seg_first, vid, vrank, first_occ;
-    DBuf<uint64_t> khi, klo;
-    DBuf<uint32_t> perm, ffirst, fhead, fhex, fsegfirst, frank, fsegstart, ffirst_occ;
+    DBuf<uint32_t> ffirst, fmatch, ftfc, fvc, frank, ftpos, fvpos;
     // outputs
     DBuf<uint32_t> v_tet, v_sv, f_vstart, f_verts, f_tfstart, f_tf_tet;
     DBuf<uint16_t> v_j, f_tf_local;
@@ -808,14 +801,11 @@ void alloc_ws(Workspace& W, const Grid& g, const Caps_& c) {
         b->alloc(m, c.vo + 1);
     W.fq_a.alloc(m, c.fo + 1);
     W.fq_vs.alloc(m, c.fo + 1);
-    W.fq_end.alloc(m, c.fo + 1);
     W.fq_l.alloc(m, c.fo + 1);
     W.fq_fn.alloc(m, c.fo + 1);
     W.fq_bnd.alloc(m, c.fo + 1);
     W.fve.alloc(m, c.fv + 1);
-    W.khi.alloc(m, c.fo + 1);
-    W.klo.alloc(m, c.fo + 1);
-    for (auto* b : {&W.perm, &W.ffirst, &W.fhead, &W.fhex, &W.fsegfirst, &W.frank, &W.fsegstart, &W.ffirst_occ})
+    for (auto* b : {&W.ffirst, &W.fmatch, &W.ftfc, &W.fvc, &W.frank, &W.ftpos, &W.fvpos})
         b->alloc(m, c.fo + 1);
     W.v_tet.alloc(m, c.vo + 1);
     W.v_j.alloc(m, c.vo + 1);
@@ -1017,41 +1007,55 @@ void stage_rest(Workspace& W, const Grid& g, const Values& val, hipStream_t s) {
                        (const uint32_t*)W.k1_off.p, (const uint32_t*)W.first_occ.p,
                        (const uint32_t*)W.oa.p, (const uint16_t*)W.oj.p, NV, W.v_tet.p, W.v_j.p, W.v_ss.p, W.v_sv.p,
                        W.v_fn.p, W.xyz.p);
-    // faces
-    hipLaunchKernelGGL(k_face_end, dim3(nblk(c.fo)), dim3(256), 0, s, NFO, (const uint32_t*)W.fq_vs.p,
-                       (const uint32_t*)(ctr + C_FV), W.fq_end.p);
-    const int vb = bitsf((uint64_t)c.vo + 1);
-    hipLaunchKernelGGL(k_face_keys, dim3(nblk(c.fo)), dim3(256), 0, s, NFO, (const uint32_t*)W.fq_vs.p,
-                       (const uint32_t*)W.fq_end.p, W.fve.p, (const uint32_t*)W.vid.p, (const uint8_t*)W.fq_bnd.p, vb,
-                       W.khi.p, W.klo.p);
-    sort_index_by2(m, W.khi.p, W.klo.p, 2 * vb + 1, vb, W.perm.p, NFO, s);
-    hipLaunchKernelGGL(k_face_heads, dim3(nblk(c.fo)), dim3(256), 0, s, NFO, (const uint32_t*)W.perm.p,
-                       (const uint64_t*)W.khi.p, (const uint64_t*)W.klo.p, (const uint8_t*)W.fq_bnd.p, W.ffirst.p,
-                       W.fhead.p);
-    scan_u32(m, W.fhead.p, W.fhex.p, NFO, nullptr, s);
-    hipLaunchKernelGGL(k_assign, dim3(nblk(c.fo)), dim3(256), 0, s, NFO, (const uint32_t*)W.perm.p,
-                       (const uint32_t*)W.fhead.p, (const uint32_t*)W.fhex.p, W.fsegfirst.p);
-    hipLaunchKernelGGL(k_seg_start, dim3(nblk(c.fo)), dim3(256), 0, s, NFO, (const uint32_t*)W.fhead.p,
-                       (const uint32_t*)W.fhex.p, W.fsegstart.p);
+    // faces: a boundary face may repeat in the tet across its tet face
+    hipLaunchKernelGGL(k_face_map, dim3(nblk(c.fv)), dim3(256), 0, s, Count(ctr + C_FV, c.fv, abort), W.fve.p,
+                       (const uint32_t*)W.vid.p);
+    FaceArgs FA;
+    FA.fq_a = W.fq_a.p;
+    FA.fq_vs = W.fq_vs.p;
+    FA.fve = W.fve.p;
+    FA.act = W.act.p;
+    FA.rank = W.rank.p;
+    FA.foff = W.cf.p;
+    FA.nbr = g.nbr.p;
+    FA.fq_bnd = W.fq_bnd.p;
+    FA.fq_l = W.fq_l.p;
+    FA.nfv_total = ctr + C_FV;
+    FA.nfo_total = ctr + C_FO;
+    FA.nfv_cap = c.fv;
+    FA.nfo_cap = c.fo;
+    FA.nq = NFO;
+    FA.n_active = NA;
+    FA.first = W.ffirst.p;
+    FA.match = W.fmatch.p;
+    FA.tfc = W.ftfc.p;
+    FA.vcnt = W.fvc.p;
+    hipLaunchKernelGGL(k_face_first, dim3(nblk(c.fo)), dim3(256), 0, s, FA);
     scan_u32(m, W.ffirst.p, W.frank.p, NFO, ctr + C_FACES, s);
-    hipLaunchKernelGGL(k_first_list, dim3(nblk(c.fo)), dim3(256), 0, s, NFO, (const uint32_t*)W.ffirst.p,
-                       (const uint32_t*)W.frank.p, W.ffirst_occ.p);
-    fill_u32(W.f_tfstart.p, 0, c.fo + 1, s);
-    hipLaunchKernelGGL(k_face_counts, dim3(nblk(c.fo)), dim3(256), 0, s, NFO, (const uint32_t*)W.perm.p,
-                       (const uint32_t*)W.fhead.p, (const uint32_t*)W.fhex.p, (const uint32_t*)W.fsegfirst.p,
-                       (const uint32_t*)W.frank.p, W.f_tfstart.p);
-    scan_u32(m, W.f_tfstart.p, W.f_tfstart.p, NF, nullptr, s);
-    hipLaunchKernelGGL(k_face_tf, dim3(nblk(c.fo)), dim3(256), 0, s, NFO, (const uint32_t*)W.perm.p,
-                       (const uint32_t*)W.fhead.p, (const uint32_t*)W.fhex.p, (const uint32_t*)W.fsegfirst.p,
-                       (const uint32_t*)W.fsegstart.p, (const uint32_t*)W.frank.p, (const uint32_t*)W.f_tfstart.p,
-                       (const uint32_t*)W.act.p, (const uint32_t*)W.fq_a.p, (const uint16_t*)W.fq_l.p, W.f_tf_tet.p,
-                       W.f_tf_local.p);
-    hipLaunchKernelGGL(k_face_vcount, dim3(nblk(c.fo)), dim3(256), 0, s, NF, (const uint32_t*)W.ffirst_occ.p,
-                       (const uint32_t*)W.fq_vs.p, (const uint32_t*)W.fq_end.p, W.f_vstart.p);
-    scan_u32(m, W.f_vstart.p, W.f_vstart.p, NF, ctr + C_FACEVERTS, s);
-    hipLaunchKernelGGL(k_face_out, dim3(nblk(c.fo)), dim3(256), 0, s, NF, (const uint32_t*)W.ffirst_occ.p,
-                       (const uint32_t*)W.fq_vs.p, (const uint32_t*)W.fq_end.p, (const uint32_t*)W.fve.p,
-                       (const uint8_t*)W.fq_fn.p, (const uint32_t*)W.f_vstart.p, W.f_verts.p, W.f_func.p);
+    scan_u32(m, W.ftfc.p, W.ftpos.p, NFO, ctr + C_TETFACES, s);
+    scan_u32(m, W.fvc.p, W.fvpos.p, NFO, ctr + C_FACEVERTS, s);
+    FaceOutArgs FO;
+    FO.fq_a = W.fq_a.p;
+    FO.fq_vs = W.fq_vs.p;
+    FO.fve = W.fve.p;
+    FO.act = W.act.p;
+    FO.first = W.ffirst.p;
+    FO.match = W.fmatch.p;
+    FO.frank = W.frank.p;
+    FO.tfpos = W.ftpos.p;
+    FO.vpos = W.fvpos.p;
+    FO.fq_l = W.fq_l.p;
+    FO.fq_fn = W.fq_fn.p;
+    FO.nfv_total = ctr + C_FV;
+    FO.nfv_cap = c.fv;
+    FO.nq = NFO;
+    FO.f_vstart = W.f_vstart.p;
+    FO.f_verts = W.f_verts.p;
+    FO.f_tfstart = W.f_tfstart.p;
+    FO.f_tf_tet = W.f_tf_tet.p;
+    FO.f_tf_local = W.f_tf_local.p;
+    FO.f_func = W.f_func.p;
+    hipLaunchKernelGGL(k_face_emit, dim3(nblk(c.fo)), dim3(256), 0, s, FO);
     hipLaunchKernelGGL(k_write_totals, dim3(1), dim3(64), 0, s, (const uint32_t*)ctr, g.view.T, W.csr_off.p,
                        W.f_vstart.p, W.f_tfstart.p, c.fo);
 }
@@ -1097,11 +1101,11 @@ float time_ms(hipStream_t s, const std::function<void()>& f) {
 }  // namespace
 
 void run_batched(DevMem& ctx_mem, const Grid& g, Values& val, uint32_t flags, RunOut& out, hipStream_t s, int n_cu) {
-    (void)ctx_mem;
     if (flags & (RUN_MI | RUN_CUT_ONLY | RUN_REVERSE))
         throw std::runtime_error("batched form is not available for this mode (use the plain form)");
     const uint64_t V = g.view.V, T = g.view.T;
     if (val.V != V) throw std::invalid_argument("values do not match the grid");
+    ensure_nbr(ctx_mem, g, s);
     auto& planp = val.plan[0];
     if (!planp) planp = std::make_shared<Plan>();
     Plan& P = *planp;
@@ -1265,7 +1269,7 @@ void run_batched(DevMem& ctx_mem, const Grid& g, Values& val, uint32_t flags, Ru
     out.n_verts = hctr[C_VERTS];
     out.n_faces = hctr[C_FACES];
     out.n_face_verts = hctr[C_FACEVERTS];
-    out.n_tet_faces = hctr[C_FO];
+    out.n_tet_faces = hctr[C_TETFACES];
     auto& vw = out.view;
     vw.fit = w.fit.p;
     vw.csr_off = w.csr_off.p;

@@ -438,7 +438,7 @@ __global__ void k_extract_ia(GridView g, const uint32_t* __restrict__ act, const
                 fq_a[q] = (uint32_t)a;
                 fq_l[q] = (uint16_t)f;
                 fq_fn[q] = fit[s0 + sp - 4];
-                fq_bnd[q] = F[2] == 0xFFFF;
+                fq_bnd[q] = F[2] == 0xFFFF ? (uint8_t)(1 + F[0]) : 0;
                 fq_vs[q] = e;
                 for (int i = 0; i < n; ++i) {
                     const int j = F[4 + i];
@@ -610,109 +610,132 @@ __global__ void k_vert_fields_ia(GridView g, const double* __restrict__ vals, co
 }
 
 // ------------------------------------------------------------------ faces
+//
+// Face occurrences come out of extraction in (tet, local face) order with
+// fq_bnd = 0 for faces inside the tet and 1 + l for faces on tet face l.
+// Interior faces are never merged.  A boundary face is merged by the
+// reference through its (min, second-min, max) vertex-id key
+// (src/extract_mesh.h:68-92, src/extract_mesh.cpp:240-253); two faces on
+// different tet faces share at most two vertices, so the only possible copy
+// lies in the tet across l, and is found there directly.
 
-// global vertex ids of every face occurrence; boundary faces get their
-// (min, second-min, max) key (src/extract_mesh.h:68-92)
-__global__ void k_face_keys(Count nq, const uint32_t* __restrict__ fq_vs, const uint32_t* __restrict__ fq_vs_end,
-                            uint32_t* __restrict__ fve, const uint32_t* __restrict__ vid_of_occ,
-                            const uint8_t* __restrict__ fq_bnd, int vb, uint64_t* __restrict__ khi,
-                            uint64_t* __restrict__ klo) {
-    GRID_STRIDE(q, nq) {
-        const uint32_t b = fq_vs[q], e = fq_vs_end[q];
-        for (uint32_t i = b; i < e; ++i) fve[i] = vid_of_occ[fve[i]];
-        if (!fq_bnd[q]) {
-            // interior faces are never merged; they sort after every boundary key
-            khi[q] = 1ull << (2 * vb);
-            klo[q] = 0;
-            continue;
+__global__ void k_face_map(Count nfv, uint32_t* __restrict__ fve, const uint32_t* __restrict__ vid_of_occ) {
+    GRID_STRIDE(i, nfv) fve[i] = vid_of_occ[fve[i]];
+}
+
+__device__ inline void face_triple(const uint32_t* __restrict__ v, uint32_t n, uint32_t k3[3]) {
+    uint32_t mn = v[0], mx = v[0], mp = 0;
+    for (uint32_t i = 1; i < n; ++i) {
+        if (v[i] < mn) {
+            mn = v[i];
+            mp = i;
+        } else if (v[i] > mx) {
+            mx = v[i];
         }
-        uint32_t mn = fve[b], mx = fve[b], mp = b;
-        for (uint32_t i = b + 1; i < e; ++i) {
-            if (fve[i] < mn) {
-                mn = fve[i];
-                mp = i;
-            } else if (fve[i] > mx) {
-                mx = fve[i];
+    }
+    uint32_t sm = 0xFFFFFFFFu;
+    for (uint32_t i = 0; i < n; ++i)
+        if (i != mp && v[i] < sm) sm = v[i];
+    k3[0] = mn;
+    k3[1] = sm;
+    k3[2] = mx;
+}
+
+struct FaceArgs {
+    const uint32_t *fq_a, *fq_vs, *fve, *act, *rank, *foff, *nbr;
+    const uint8_t* fq_bnd;
+    const uint16_t* fq_l;
+    const uint32_t* nfv_total;  // face-vertex entries (device)
+    const uint32_t* nfo_total;  // face occurrences (device)
+    uint64_t nfv_cap, nfo_cap;
+    Count nq, n_active;
+    uint32_t *first, *match, *tfc, *vcnt;
+};
+
+__device__ inline uint32_t clamp_count(const uint32_t* d, uint64_t cap) {
+    const uint64_t n = *d;
+    return (uint32_t)(n < cap ? n : cap);
+}
+
+// first-occurrence flag, the first occurrence of the same face (match), the
+// face's occurrence count (on its first occurrence) and vertex count
+__global__ void k_face_first(FaceArgs A) {
+    const uint64_t NA = A.n_active.get(), NQ = A.nq.get();
+    const uint32_t nfv = clamp_count(A.nfv_total, A.nfv_cap);
+    GRID_STRIDE(q, A.nq) {
+        const uint32_t b = A.fq_vs[q], e = (q + 1 < NQ) ? A.fq_vs[q + 1] : nfv;
+        uint32_t first = 1, match = (uint32_t)q, dup = 0;
+        if (A.fq_bnd[q]) {
+            const uint32_t a = A.fq_a[q], t = A.act[a], l = A.fq_bnd[q] - 1u;
+            const uint32_t nb = A.nbr[4 * (uint64_t)t + l];
+            if (nb != 0xFFFFFFFFu) {
+                const uint32_t t2 = nb >> 2, l2 = nb & 3u, r2 = A.rank[t2];
+                if (r2 < NA && A.act[r2] == t2) {
+                    uint32_t k3[3];
+                    face_triple(A.fve + b, e - b, k3);
+                    const uint32_t q0 = A.foff[r2];
+                    const uint32_t q1 = (r2 + 1 < NA) ? A.foff[r2 + 1] : clamp_count(A.nfo_total, A.nfo_cap);
+                    for (uint32_t p = q0; p < q1; ++p) {
+                        if (A.fq_bnd[p] != 1 + l2) continue;
+                        const uint32_t pb = A.fq_vs[p], pe = (p + 1 < NQ) ? A.fq_vs[p + 1] : nfv;
+                        uint32_t p3[3];
+                        face_triple(A.fve + pb, pe - pb, p3);
+                        if (p3[0] == k3[0] && p3[1] == k3[1] && p3[2] == k3[2]) {
+                            if (t2 < t) {
+                                first = 0;
+                                match = p;
+                            } else {
+                                dup = 1;
+                            }
+                            break;
+                        }
+                    }
+                }
             }
         }
-        uint32_t sm = mx + 1;
-        for (uint32_t i = b; i < e; ++i)
-            if (i != mp && fve[i] < sm) sm = fve[i];
-        khi[q] = ((uint64_t)mn << vb) | sm;
-        klo[q] = mx;
+        A.first[q] = first;
+        A.match[q] = match;
+        A.tfc[q] = first ? 1 + dup : 0;
+        A.vcnt[q] = first ? e - b : 0;
     }
 }
 
-__global__ void k_face_heads(Count n, const uint32_t* __restrict__ perm, const uint64_t* __restrict__ khi,
-                             const uint64_t* __restrict__ klo, const uint8_t* __restrict__ bnd,
-                             uint32_t* __restrict__ first, uint32_t* __restrict__ head) {
-    GRID_STRIDE(p, n) {
-        const uint32_t q = perm[p];
-        uint32_t h = 1;
-        if (p > 0 && bnd[q]) {
-            const uint32_t r = perm[p - 1];
-            h = (bnd[r] && khi[r] == khi[q] && klo[r] == klo[q]) ? 0u : 1u;
+struct FaceOutArgs {
+    const uint32_t *fq_a, *fq_vs, *fve, *act, *first, *match, *frank, *tfpos, *vpos;
+    const uint16_t* fq_l;
+    const uint8_t* fq_fn;
+    const uint32_t* nfv_total;
+    uint64_t nfv_cap;
+    Count nq;
+    uint32_t *f_vstart, *f_verts, *f_tfstart, *f_tf_tet;
+    uint16_t* f_tf_local;
+    uint8_t* f_func;
+};
+
+// face records; tet-face occurrences in occurrence order (first, then the copy)
+__global__ void k_face_emit(FaceOutArgs A) {
+    const uint64_t NQ = A.nq.get();
+    const uint32_t nfv = clamp_count(A.nfv_total, A.nfv_cap);
+    GRID_STRIDE(q, A.nq) {
+        const uint32_t t = A.act[A.fq_a[q]];
+        if (A.first[q]) {
+            const uint32_t fid = A.frank[q];
+            const uint32_t b = A.fq_vs[q], e = (q + 1 < NQ) ? A.fq_vs[q + 1] : nfv;
+            uint32_t o = A.vpos[q];
+            A.f_vstart[fid] = o;
+            for (uint32_t i = b; i < e; ++i) A.f_verts[o++] = A.fve[i];
+            A.f_tfstart[fid] = A.tfpos[q];
+            A.f_tf_tet[A.tfpos[q]] = t;
+            A.f_tf_local[A.tfpos[q]] = A.fq_l[q];
+            A.f_func[2 * fid] = A.fq_fn[q];
+            A.f_func[2 * fid + 1] = 0xFF;
+        } else {
+            const uint32_t p = A.tfpos[A.match[q]] + 1;
+            A.f_tf_tet[p] = t;
+            A.f_tf_local[p] = A.fq_l[q];
         }
-        head[p] = h;
-        first[q] = h;
     }
 }
-
-// per unique face: occurrence count and vertex count
-__global__ void k_face_counts(Count n, const uint32_t* __restrict__ perm, const uint32_t* __restrict__ head,
-                              const uint32_t* __restrict__ head_excl, const uint32_t* __restrict__ seg_first_occ,
-                              const uint32_t* __restrict__ rank, uint32_t* __restrict__ tf_cnt) {
-    GRID_STRIDE(p, n) {
-        const uint32_t seg = head_excl[p] + head[p] - 1;
-        atomicAdd(&tf_cnt[rank[seg_first_occ[seg]]], 1u);
-    }
-}
-
-__global__ void k_face_out(Count nfaces, const uint32_t* __restrict__ first_occ, const uint32_t* __restrict__ fq_vs,
-                           const uint32_t* __restrict__ fq_vs_end, const uint32_t* __restrict__ fve,
-                           const uint8_t* __restrict__ fq_fn, const uint32_t* __restrict__ fvstart,
-                           uint32_t* __restrict__ fverts, uint8_t* __restrict__ ffunc) {
-    GRID_STRIDE(i, nfaces) {
-        const uint32_t q = first_occ[i];
-        uint32_t o = fvstart[i];
-        for (uint32_t e = fq_vs[q]; e < fq_vs_end[q]; ++e) fverts[o++] = fve[e];
-        ffunc[2 * i] = fq_fn[q];
-        ffunc[2 * i + 1] = 0xFF;
-    }
-}
-
-__global__ void k_face_vcount(Count nfaces, const uint32_t* __restrict__ first_occ, const uint32_t* __restrict__ fq_vs,
-                              const uint32_t* __restrict__ fq_vs_end, uint32_t* __restrict__ cnt) {
-    GRID_STRIDE(i, nfaces) {
-        const uint32_t q = first_occ[i];
-        cnt[i] = fq_vs_end[q] - fq_vs[q];
-    }
-}
-
-// occurrences of each unique face in occurrence order: sorted positions of a
-// segment are consecutive and (stable sort) already in occurrence order
-__global__ void k_face_tf(Count n, const uint32_t* __restrict__ perm, const uint32_t* __restrict__ head,
-                          const uint32_t* __restrict__ head_excl, const uint32_t* __restrict__ seg_first_occ,
-                          const uint32_t* __restrict__ seg_start, const uint32_t* __restrict__ rank,
-                          const uint32_t* __restrict__ tfstart, const uint32_t* __restrict__ act,
-                          const uint32_t* __restrict__ fq_a, const uint16_t* __restrict__ fq_l,
-                          uint32_t* __restrict__ tf_tet, uint16_t* __restrict__ tf_local) {
-    GRID_STRIDE(p, n) {
-        const uint32_t seg = head_excl[p] + head[p] - 1;
-        const uint32_t fid = rank[seg_first_occ[seg]];
-        const uint32_t pos = tfstart[fid] + ((uint32_t)p - seg_start[seg]);
-        const uint32_t q = perm[p];
-        tf_tet[pos] = act[fq_a[q]];
-        tf_local[pos] = fq_l[q];
-    }
-}
-__global__ void k_seg_start(Count n, const uint32_t* __restrict__ head, const uint32_t* __restrict__ head_excl,
-                            uint32_t* __restrict__ seg_start) {
-    GRID_STRIDE(p, n) {
-        if (head[p]) seg_start[head_excl[p]] = (uint32_t)p;
-    }
-}
-
 
 }  // namespace
 }  // namespace rin

@@ -111,6 +111,10 @@ void run_pipeline(DevMem& mem, const Grid& g, const Values& v, uint32_t flags, R
 void run_batched(DevMem& ctx_mem, const Grid& g, Values& v, uint32_t flags, RunOut& out, hipStream_t s,
                  int n_cu);
 
+// the tet across every tet face (Grid::nbr), built on first use; synchronises
+// the stream, so call it outside graph capture
+void ensure_nbr(DevMem& mem, const Grid& g, hipStream_t s);
+
 // copy the cut records of the given tets to the host, packed (rec_start has
 // n + 1 word offsets; inactive tets get empty ranges)
 void gather_records(DevMem& mem, const RunOut& r, const uint64_t* tets, uint64_t n, uint64_t* rec_start,

@@ -91,6 +91,8 @@ void dedup_vertices(DevMem& mem, DBuf<uint64_t>& okey, uint64_t NVO, int kbits, 
     RIN_HIP(hipGetLastError());
 }
 
+}  // namespace
+
 // tet adjacency across faces, once per grid
 void ensure_nbr(DevMem& mem, const Grid& g, hipStream_t s) {
     if (g.nbr.p) return;
@@ -107,6 +109,8 @@ void ensure_nbr(DevMem& mem, const Grid& g, hipStream_t s) {
     RIN_HIP(hipGetLastError());
     RIN_HIP(hipStreamSynchronize(s));
 }
+
+namespace {
 
 // material-interface extraction and coordinates (src/extract_mesh.cpp:569-986,
 // 1541-1637); faces are never merged, so they need no sort
@@ -516,60 +520,76 @@ void run_pipeline(DevMem& mem, const Grid& g, const Values& val, uint32_t flags,
     RIN_HIP(hipGetLastError());
     tm.mark("compute xyz");
 
-    // ---- faces: boundary faces dedup by key, interior faces always new
-    // face occurrences are laid out back to back: each ends where the next starts
-    DBuf<uint32_t> fq_end(mem, NFO + 1);
-    if (NFO) {
-        RIN_HIP(hipMemcpyAsync(fq_end.p, fq_vs.p + 1, sizeof(uint32_t) * (NFO - 1), hipMemcpyDeviceToDevice, s));
-        fill_u32(fq_end.p + NFO - 1, (uint32_t)NFV, 1, s);
-    }
-    DBuf<uint64_t> khi(mem, NFO + 1), klo(mem, NFO + 1);
-    const int vb = bits_for(NV + 1);
-    if (2 * vb + 1 > 64) throw std::runtime_error("boundary face keys exceed 64 bits");
-    hipLaunchKernelGGL(k_face_keys, dim3(nblk(NFO)), dim3(256), 0, s, NFO, (const uint32_t*)fq_vs.p,
-                       (const uint32_t*)fq_end.p, fve.p, (const uint32_t*)vid.p, (const uint8_t*)fq_bnd.p, vb, khi.p,
-                       klo.p);
-    DBuf<uint32_t> perm(mem, NFO + 1), ffirst(mem, NFO + 1), fhead(mem, NFO + 1), fhex(mem, NFO + 1),
-        fsegfirst(mem, NFO + 1), frank(mem, NFO + 1), fsegstart(mem, NFO + 1);
-    sort_index_by2(mem, khi.p, klo.p, 2 * vb + 1, vb, perm.p, NFO, s);
-    hipLaunchKernelGGL(k_face_heads, dim3(nblk(NFO)), dim3(256), 0, s, NFO, (const uint32_t*)perm.p,
-                       (const uint64_t*)khi.p, (const uint64_t*)klo.p, (const uint8_t*)fq_bnd.p, ffirst.p, fhead.p);
-    scan_u32(mem, fhead.p, fhex.p, NFO, nullptr, s);
-    hipLaunchKernelGGL(k_assign, dim3(nblk(NFO)), dim3(256), 0, s, NFO, (const uint32_t*)perm.p,
-                       (const uint32_t*)fhead.p, (const uint32_t*)fhex.p, fsegfirst.p);
-    hipLaunchKernelGGL(k_seg_start, dim3(nblk(NFO)), dim3(256), 0, s, NFO, (const uint32_t*)fhead.p,
-                       (const uint32_t*)fhex.p, fsegstart.p);
-    DBuf<uint32_t> nft(mem, 2);
+    // ---- faces: a boundary face may repeat in the tet across its tet face
+    ensure_nbr(mem, g, s);
+    DBuf<uint32_t> tot3(mem, 4);
+    fill_u32(tot3.p, (uint32_t)NFV, 1, s);
+    fill_u32(tot3.p + 1, (uint32_t)NFO, 1, s);
+    hipLaunchKernelGGL(k_face_map, dim3(nblk(NFV)), dim3(256), 0, s, Count(NFV), fve.p, (const uint32_t*)vid.p);
+    DBuf<uint32_t> ffirst(mem, NFO + 1), fmatch(mem, NFO + 1), ftfc(mem, NFO + 1), fvc(mem, NFO + 1),
+        frank(mem, NFO + 1), ftpos(mem, NFO + 1), fvpos(mem, NFO + 1);
+    FaceArgs FA;
+    FA.fq_a = fq_a.p;
+    FA.fq_vs = fq_vs.p;
+    FA.fve = fve.p;
+    FA.act = out.act.p;
+    FA.rank = out.rank_of_tet.p;
+    FA.foff = cf.p;
+    FA.nbr = g.nbr.p;
+    FA.fq_bnd = fq_bnd.p;
+    FA.fq_l = fq_l.p;
+    FA.nfv_total = tot3.p;
+    FA.nfo_total = tot3.p + 1;
+    FA.nfv_cap = NFV;
+    FA.nfo_cap = NFO;
+    FA.nq = Count(NFO);
+    FA.n_active = Count(NA);
+    FA.first = ffirst.p;
+    FA.match = fmatch.p;
+    FA.tfc = ftfc.p;
+    FA.vcnt = fvc.p;
+    hipLaunchKernelGGL(k_face_first, dim3(nblk(NFO)), dim3(256), 0, s, FA);
+    DBuf<uint32_t> nft(mem, 4);
     scan_u32(mem, ffirst.p, frank.p, NFO, nft.p, s);
-    out.n_faces = read1(nft.p, s);
+    scan_u32(mem, ftfc.p, ftpos.p, NFO, nft.p + 1, s);
+    scan_u32(mem, fvc.p, fvpos.p, NFO, nft.p + 2, s);
+    uint32_t hn[3];
+    RIN_HIP(hipMemcpyAsync(hn, nft.p, 12, hipMemcpyDeviceToHost, s));
+    RIN_HIP(hipStreamSynchronize(s));
+    out.n_faces = hn[0];
+    out.n_tet_faces = hn[1];
+    out.n_face_verts = hn[2];
     const uint64_t NF = out.n_faces;
-    DBuf<uint32_t> ffirst_occ(mem, NF + 1);
-    hipLaunchKernelGGL(k_first_list, dim3(nblk(NFO)), dim3(256), 0, s, NFO, (const uint32_t*)ffirst.p,
-                       (const uint32_t*)frank.p, ffirst_occ.p);
-    out.f_tfstart.alloc(mem, NF + 1);
-    fill_u32(out.f_tfstart.p, 0, NF + 1, s);
-    hipLaunchKernelGGL(k_face_counts, dim3(nblk(NFO)), dim3(256), 0, s, NFO, (const uint32_t*)perm.p,
-                       (const uint32_t*)fhead.p, (const uint32_t*)fhex.p, (const uint32_t*)fsegfirst.p,
-                       (const uint32_t*)frank.p, out.f_tfstart.p);
-    scan_u32(mem, out.f_tfstart.p, out.f_tfstart.p, NF, out.f_tfstart.p + NF, s);
-    out.n_tet_faces = NFO;
-    out.f_tf_tet.alloc(mem, NFO + 1);
-    out.f_tf_local.alloc(mem, NFO + 1);
-    hipLaunchKernelGGL(k_face_tf, dim3(nblk(NFO)), dim3(256), 0, s, NFO, (const uint32_t*)perm.p,
-                       (const uint32_t*)fhead.p, (const uint32_t*)fhex.p, (const uint32_t*)fsegfirst.p,
-                       (const uint32_t*)fsegstart.p, (const uint32_t*)frank.p, (const uint32_t*)out.f_tfstart.p,
-                       (const uint32_t*)out.act.p, (const uint32_t*)fq_a.p, (const uint16_t*)fq_l.p, out.f_tf_tet.p,
-                       out.f_tf_local.p);
     out.f_vstart.alloc(mem, NF + 1);
-    hipLaunchKernelGGL(k_face_vcount, dim3(nblk(NF)), dim3(256), 0, s, NF, (const uint32_t*)ffirst_occ.p,
-                       (const uint32_t*)fq_vs.p, (const uint32_t*)fq_end.p, out.f_vstart.p);
-    scan_u32(mem, out.f_vstart.p, out.f_vstart.p, NF, out.f_vstart.p + NF, s);
-    out.n_face_verts = read1(out.f_vstart.p + NF, s);
+    out.f_tfstart.alloc(mem, NF + 1);
     out.f_verts.alloc(mem, out.n_face_verts + 1);
+    out.f_tf_tet.alloc(mem, out.n_tet_faces + 1);
+    out.f_tf_local.alloc(mem, out.n_tet_faces + 1);
     out.f_func.alloc(mem, 2 * NF + 2);
-    hipLaunchKernelGGL(k_face_out, dim3(nblk(NF)), dim3(256), 0, s, NF, (const uint32_t*)ffirst_occ.p,
-                       (const uint32_t*)fq_vs.p, (const uint32_t*)fq_end.p, (const uint32_t*)fve.p,
-                       (const uint8_t*)fq_fn.p, (const uint32_t*)out.f_vstart.p, out.f_verts.p, out.f_func.p);
+    FaceOutArgs FO;
+    FO.fq_a = fq_a.p;
+    FO.fq_vs = fq_vs.p;
+    FO.fve = fve.p;
+    FO.act = out.act.p;
+    FO.first = ffirst.p;
+    FO.match = fmatch.p;
+    FO.frank = frank.p;
+    FO.tfpos = ftpos.p;
+    FO.vpos = fvpos.p;
+    FO.fq_l = fq_l.p;
+    FO.fq_fn = fq_fn.p;
+    FO.nfv_total = tot3.p;
+    FO.nfv_cap = NFV;
+    FO.nq = Count(NFO);
+    FO.f_vstart = out.f_vstart.p;
+    FO.f_verts = out.f_verts.p;
+    FO.f_tfstart = out.f_tfstart.p;
+    FO.f_tf_tet = out.f_tf_tet.p;
+    FO.f_tf_local = out.f_tf_local.p;
+    FO.f_func = out.f_func.p;
+    hipLaunchKernelGGL(k_face_emit, dim3(nblk(NFO)), dim3(256), 0, s, FO);
+    fill_u32(out.f_vstart.p + NF, (uint32_t)out.n_face_verts, 1, s);
+    fill_u32(out.f_tfstart.p + NF, (uint32_t)out.n_tet_faces, 1, s);
     RIN_HIP(hipGetLastError());
     tm.mark("faces");
     tm.finish(out.times);
