@@ -8,17 +8,21 @@ signs -> active-function filter and CSR -> per-tet cut -> iso-vertex and
 iso-face dedup -> vertex coordinates (the reference's timed driver stages,
 src/implicit_arrangement.cpp:53-402).
 
-Prints one JSON line.  The CPU baseline is the oracle (a C++ restatement of
-the same stages) on the same workload, on this host's cores.
+Prints one JSON line.  `roofline` is for the dominant single-kernel stage:
+its algorithmic bytes per launch over its duration, taken from the stream
+events the library records around that stage on every step.  `traffic` comes
+from the committed hardware-counter summary of the same kernel
+(profiles/traffic_*.json, written by --pmc-summary).  The CPU baseline is the
+oracle (a C++ restatement of the same stages) on the same workload, on this
+host's cores.
 """
 import argparse
-import ctypes
+import csv
 import json
 import os
+import re
 import sys
 import time
-
-import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "robust-implicit-surface-networks_amd"))
@@ -26,39 +30,136 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 METRIC = "tets/s (101³ grid, 20 fns) batched vs plain GPU vs CPU; p50/p99 job latency"
 HBM_PEAK_GBS = 8000.0
+WORKLOAD = "ia_res100_f20"
 
 
-def algorithmic_bytes(V, F, sizes):
-    """Bytes the path must move at minimum: read the F x V values once, write
-    the CSR (4 B per entry + 4 B per tet offset of active tets), the iso-vertex
+def path_bytes(V, F, s):
+    """Whole-path algorithmic bytes: read the F x V values once; write the CSR
+    (1 B per function id + 4 B per tet offset of active tets), the iso-vertex
     records (4 B tet + 2 B local + 1 B size + 16 B simplex + 4 B functions) and
     coordinates (24 B), and the faces (4 B per vertex entry, 4 B offsets,
     6 B per tet-face occurrence, 2 B labels)."""
     b = 8 * F * V
-    b += 4 * sizes.n_func_in_tet + 4 * sizes.num_intersecting
-    b += (4 + 2 + 1 + 16 + 4 + 24) * sizes.n_verts
-    b += 4 * sizes.n_face_verts + 4 * sizes.n_faces + 6 * sizes.n_tet_faces + 2 * sizes.n_faces
+    b += s.n_func_in_tet + 4 * s.num_intersecting
+    b += (4 + 2 + 1 + 16 + 4 + 24) * s.n_verts
+    b += 4 * s.n_face_verts + 4 * s.n_faces + 6 * s.n_tet_faces + 2 * s.n_faces
     return b
 
 
-def cpu_baseline(res, fs, budget_s):
+def extract_bytes(s):
+    """k_extract_batched, per launch: per active tet 24 B of indices (tet,
+    record location, CSR offset, three output offsets) and its function ids;
+    the cut records it reads (2 B a word; k = 1 tets read an LDS table); per
+    vertex occurrence a 14 B (key, active rank, local index) row; per face
+    occurrence a 12 B row and 4 B per face corner."""
+    return (24 * s.num_intersecting + s.n_func_in_tet + 2 * s.n_rec_words + 14 * s.n_vert_occ +
+            12 * s.n_face_occ + 4 * s.n_face_occ_verts)
+
+
+# single-kernel stages with an algorithmic byte count: stage label -> (kernel, bytes(sizes))
+KERNEL_STAGES = {
+    "extract occurrences (k_extract_batched)": ("k_extract_batched", extract_bytes),
+}
+
+
+def cpu_baseline(res, fs):
     import support as S
 
     types, params = fs.arrays()
     pts, tets = S.grid(res)
     vals = S.evaluate(types, params, pts)
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    t0 = time.perf_counter()
     r = S.OracleRun(0, pts, tets, vals)
-    wall = time.perf_counter() - t0
     # stage wall time: signs + filter + cut (wall) + extract + xyz
     stage = (r.timings["func signs"] + r.timings["filter"] + r.timings["simp_arr(wall)"] +
              r.timings["extract mesh"] + r.timings["compute xyz"])
     T = tets.shape[0]
     return {"value": T / stage, "unit": "tets/s", "cores": cores, "kind": "port",
             "sample": "the full workload once (%d tets): oracle stages signs, filter, per-tet cut (OpenMP over "
-                      "tets), extraction and xyz, %.2f s of stage time" % (T, stage),
+                      "tets, %d threads), extraction and xyz (one thread); %.2f s of stage time" % (T, cores, stage),
             "stage_s": {k: round(v, 4) for k, v in r.timings.items()}}
+
+
+def traffic_for(kernel):
+    """HBM bytes per launch from the committed counter summary, or None."""
+    p = os.path.join(ROOT, "profiles", "traffic_%s_%s.json" % (kernel, WORKLOAD))
+    if not os.path.exists(p):
+        return None, None
+    d = json.load(open(p))
+    return d.get("hbm_bytes_per_launch"), os.path.relpath(p, ROOT)
+
+
+def _short(name):
+    name = re.sub(r"rin::\(anonymous namespace\)::", "", name)
+    name = re.sub(r"rin::dev::Caps<(\d+), (\d+), (\d+), (\d+) >", r"Caps<\1,\2,\3,\4>", name)
+    return re.sub(r"\(.*\)$", "", name).strip()
+
+
+def pmc_summary(fetch_csv, write_csv, trace_csv, copy_bytes, command):
+    """Summarise rocprofv3 counter runs (FETCH_SIZE and WRITE_SIZE, KiB per
+    dispatch) into profiles/: per-kernel KiB per dispatch, the copy probe's
+    scale factors, and each measured kernel's HBM bytes per launch."""
+
+    def per_kernel(path):
+        out = {}
+        for r in csv.DictReader(open(path)):
+            out.setdefault(_short(r["Kernel_Name"]), []).append(float(r["Counter_Value"]))
+        return out
+
+    fetch, write = per_kernel(fetch_csv), per_kernel(write_csv)
+    # the copy probe reads and writes copy_bytes per dispatch (its largest dispatches)
+    cf = max(fetch["k_copy_u64"]) * 1024.0
+    cw = max(write["k_copy_u64"]) * 1024.0
+    sf, sw = copy_bytes / cf, copy_bytes / cw
+    durs = {}
+    for r in csv.DictReader(open(trace_csv)):
+        durs.setdefault(_short(r["Kernel_Name"]), []).append(int(r["End_Timestamp"]) - int(r["Start_Timestamp"]))
+    os.makedirs(os.path.join(ROOT, "profiles"), exist_ok=True)
+    lines = ["# HBM traffic per dispatch (rocprofv3 FETCH_SIZE / WRITE_SIZE), %s" % WORKLOAD, "",
+             "Commands (from the repository root on the GPU box):", "",
+             "    " + command.replace("COUNTER", "FETCH_SIZE"), "    " + command.replace("COUNTER", "WRITE_SIZE"), "",
+             "Copy probe (k_copy_u64, %d bytes read + %d written per dispatch): FETCH_SIZE reads %.0f KiB, "
+             "WRITE_SIZE %.0f KiB -> scale factors %.4f (fetch), %.4f (write)." % (
+                 copy_bytes, copy_bytes, cf / 1024, cw / 1024, sf, sw), "",
+             "Dispatches include the one-time work of the first run (counting pass, autotuning, tet adjacency).", "",
+             "| kernel | dispatches | FETCH KiB/dispatch | WRITE KiB/dispatch | scaled MB/dispatch | avg us (trace) |",
+             "|---|---|---|---|---|---|"]
+    rows = []
+    for k in fetch:
+        f = sum(fetch[k]) / len(fetch[k])
+        w = sum(write.get(k, [0])) / max(1, len(write.get(k, [0])))
+        d = durs.get(k)
+        rows.append((f * sf + w * sw, k, len(fetch[k]), f, w, (sum(d) / len(d) / 1e3) if d else float("nan")))
+    for mb, k, n, f, w, us in sorted(rows, reverse=True)[:25]:
+        lines.append("| %s | %d | %.0f | %.0f | %.2f | %.1f |" % (k, n, f, w, mb * 1024 / 1e6, us))
+    md = os.path.join(ROOT, "profiles", "hbm_traffic_%s.md" % WORKLOAD)
+    open(md, "w").write("\n".join(lines) + "\n")
+    for kernel in {v[0] for v in KERNEL_STAGES.values()}:
+        if kernel not in fetch:
+            continue
+        f = sum(fetch[kernel]) / len(fetch[kernel]) * 1024 * sf
+        w = sum(write[kernel]) / len(write[kernel]) * 1024 * sw
+        json.dump({"kernel": kernel, "workload": WORKLOAD, "fetch_bytes_per_launch": f, "write_bytes_per_launch": w,
+                   "hbm_bytes_per_launch": f + w, "copy_scale_fetch": sf, "copy_scale_write": sw,
+                   "source": os.path.relpath(md, ROOT)},
+                  open(os.path.join(ROOT, "profiles", "traffic_%s_%s.json" % (kernel, WORKLOAD)), "w"), indent=1)
+    print(open(md).read())
+
+
+def trace_summary(stats_csv, command, title, out_name):
+    """Markdown table of a rocprofv3 --kernel-trace --stats run into profiles/."""
+    rows = list(csv.DictReader(open(stats_csv)))
+    lines = ["# Kernel trace: %s" % title, "", "Command (from the repository root on the GPU box):", "",
+             "    " + command, "",
+             "Durations in microseconds over every dispatch of the run, one-time work included (first-run counting "
+             "pass, autotuning candidates, tet adjacency).", "",
+             "| kernel | calls | total us | avg us | % |", "|---|---|---|---|---|"]
+    for r in rows[:30]:
+        lines.append("| %s | %s | %.1f | %.1f | %.1f |" % (_short(r["Name"]), r["Calls"], float(r["TotalDurationNs"]) / 1e3,
+                                                          float(r["AverageNs"]) / 1e3, float(r["Percentage"])))
+    path = os.path.join(ROOT, "profiles", out_name)
+    open(path, "w").write("\n".join(lines) + "\n")
+    print(open(path).read())
 
 
 def main():
@@ -70,7 +171,21 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--stages", action="store_true", help="print per-stage device times to stderr")
     ap.add_argument("--form", choices=["both", "plain", "batched"], default="both")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="also run plain device copies of MIB MiB (reference kernel for counter scaling)")
+    ap.add_argument("--pmc-summary", nargs=3, metavar=("FETCH_CSV", "WRITE_CSV", "TRACE_CSV"),
+                    help="write profiles/ summaries from rocprofv3 counter runs and exit (no GPU use)")
+    ap.add_argument("--pmc-copy-mib", type=int, default=256)
+    ap.add_argument("--pmc-command", default="")
+    ap.add_argument("--trace-summary", nargs=4, metavar=("STATS_CSV", "COMMAND", "TITLE", "OUT_MD"),
+                    help="write a profiles/ markdown table from a rocprofv3 kernel_stats.csv and exit")
     args = ap.parse_args()
+    if args.trace_summary:
+        trace_summary(*args.trace_summary)
+        return
+    if args.pmc_summary:
+        pmc_summary(*args.pmc_summary, args.pmc_copy_mib << 20, args.pmc_command)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU; this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
@@ -87,22 +202,19 @@ def main():
     vals = grid.eval(types, params)
     V, T = grid.V, grid.T
 
-    def step(flags):
-        r = vals.run(flags)
-        return r
-
     modes = {"plain": 0, "batched": rin.FLAG_BATCHED}
     if args.form != "both":
         modes = {args.form: modes[args.form]}
     results = {}
     for name, flags in modes.items():
         for _ in range(args.warmup):
-            step(flags).release()
+            vals.run(flags).release()
         t0 = time.perf_counter()
         last = None
         stage_acc = {}
         for i in range(args.steps):
-            r = step(flags)
+            # rin_run returns after the device work and the size read-back
+            r = vals.run(flags)
             for k, ms in r.stage_times():
                 stage_acc[k] = stage_acc.get(k, 0.0) + ms
             if i + 1 < args.steps:
@@ -114,6 +226,12 @@ def main():
                          "stages_ms": {k: v / args.steps for k, v in stage_acc.items()}}
         last.release()
 
+    if args.copy_probe:
+        nbytes = args.copy_probe << 20
+        ms = ctx.debug_copy(nbytes, reps=5)
+        print("copy probe: %d bytes per copy, %.4f ms, %.1f GB/s" % (nbytes, ms, 2 * nbytes / ms / 1e6),
+              file=sys.stderr)
+
     best = min(results, key=lambda k: results[k]["s_per_step"])
     sps = results[best]["s_per_step"]
     sizes = results[best]["sizes"]
@@ -121,10 +239,21 @@ def main():
     if args.stages:
         for name, r in results.items():
             print(name, json.dumps({k: round(v, 4) for k, v in r["stages_ms"].items()}), file=sys.stderr)
-    # dominant stage: its algorithmic bytes over its event-measured duration
-    dom = max(stages, key=stages.get) if stages else None
-    B = algorithmic_bytes(V, F, sizes)
-    achieved_path = B / sps / 1e9
+    dom_stage = max(stages, key=stages.get) if stages else None
+    # dominant single-kernel stage with an algorithmic byte count
+    kstages = [k for k in stages if k in KERNEL_STAGES]
+    roof = None
+    if kstages:
+        ks = max(kstages, key=stages.get)
+        kernel, fbytes = KERNEL_STAGES[ks]
+        B = fbytes(sizes)
+        achieved = B / (stages[ks] * 1e-3) / 1e9
+        traffic, tsrc = traffic_for(kernel)
+        roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": kernel,
+                "kernel_ms": stages[ks], "algorithmic_bytes": B, "traffic_source": tsrc,
+                "timing": "stream events around the kernel's own captured graph, every timed step"}
+    pb = path_bytes(V, F, sizes)
     out = {
         "metric": METRIC,
         "value": T / sps,
@@ -146,12 +275,14 @@ def main():
         "forms": {k: {"tets_per_s": T / v["s_per_step"], "ms_per_step": v["s_per_step"] * 1e3}
                   for k, v in results.items()},
         "stages_ms": {k: round(v, 4) for k, v in stages.items()},
-        "roofline": {"bound": "hbm", "achieved": achieved_path, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved_path / HBM_PEAK_GBS, "traffic": None, "kernel": "whole path",
-                     "algorithmic_bytes": B, "dominant_stage": dom},
+        "dominant_stage": dom_stage,
+        "roofline": roof,
+        "path_roofline": {"algorithmic_bytes": pb, "achieved_GBs": pb / sps / 1e9,
+                          "frac": pb / sps / 1e9 / HBM_PEAK_GBS},
     }
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(res, fs, 30.0)
+        out["cpu_baseline"] = cpu_baseline(res, fs)
+        out["vs_cpu"] = out["value"] / out["cpu_baseline"]["value"]
     print(json.dumps(out))
     vals.release()
     grid.release()

@@ -66,6 +66,12 @@ typedef struct rin_sizes {
     uint64_t bucket[3];        /* IA: 1, 2, >=3 active functions; MI: 2, 3, >=4 */
     uint64_t n_func_in_tet;    /* CSR length */
     uint64_t n_verts, n_faces, n_face_verts, n_tet_faces;
+    /* work sizes of the extraction (for throughput accounting) */
+    uint64_t n_vert_occ;      /* vertex occurrences before dedup */
+    uint64_t n_face_occ;      /* face occurrences */
+    uint64_t n_face_occ_verts;
+    uint64_t n_cut[2];        /* active tets with 3..4 and 5..8 functions */
+    uint64_t n_rec_words;     /* cut-record words read by the extraction (batched form; 0 otherwise) */
 } rin_sizes;
 
 /* Context: one HIP stream and a device-memory cache on `device`.
@@ -136,6 +142,11 @@ int rin_result_release(rin_ctx* ctx, rin_result* res);
  * words + i*cap and its length (or a negative error) to lens[i]. */
 int rin_debug_cuts(rin_ctx* ctx, int32_t mi, const double* rows, const int32_t* ks, int32_t n_cases, int32_t cap,
                    uint16_t* words, int32_t* lens);
+
+/* Test hook with no counterpart in the reference: `reps` plain device copies
+ * of `bytes` (a reference kernel of known traffic for hardware-counter
+ * scaling); ms receives the average duration of one copy. */
+int rin_debug_copy(rin_ctx* ctx, uint64_t bytes, int32_t reps, double* ms);
 
 #ifdef __cplusplus
 }

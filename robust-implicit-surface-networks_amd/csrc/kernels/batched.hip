@@ -37,7 +37,7 @@ constexpr uint32_t ERR_OVERFLOW = 8;
 // counters in the workspace (device)
 enum Ctr : int {
     C_ACTIVE = 0, C_FIT, C_VO, C_FO, C_FV, C_VERTS, C_FACES, C_FACEVERTS, C_B1, C_B2, C_B3, C_B5, C_BX, C_DEGEN,
-    C_Q2, C_Q3, C_Q5, C_ABORT, C_TETFACES, C_N
+    C_Q2, C_Q3, C_Q5, C_ABORT, C_TETFACES, C_RECW, C_N
 };
 
 struct K1Entry {
@@ -561,6 +561,7 @@ __global__ void k_extract_batched(ExtractArgs A) {
     }
     const uint64_t NA = min((uint64_t)*A.n_active, (uint64_t)A.cap_active);
     const uint8_t le[6][2] = {{0, 1}, {0, 2}, {0, 3}, {1, 2}, {1, 3}, {2, 3}};
+    uint32_t recw = 0;  // record words read (throughput accounting)
     GRID_STRIDE(a, NA) {
         const uint32_t loc = A.rec_loc[a];
         const uint32_t t = A.act[a];
@@ -593,6 +594,7 @@ __global__ void k_extract_batched(ExtractArgs A) {
         // general record (same layout and rules as the plain extraction)
         const Rec R((loc & K1_FLAG) ? A.tab + A.tab_off[loc & ~K1_FLAG] : A.rec + loc);
         if (R.r[0] != 1) continue;
+        recw += R.r[6];
         uint64_t iso[4] = {0, 0, 0, 0};
         const uint16_t* F = R.faces();
         for (int f = 0; f < R.nf; ++f) {
@@ -666,6 +668,8 @@ __global__ void k_extract_batched(ExtractArgs A) {
             F += 4 + n;
         }
     }
+    for (int d = 32; d > 0; d >>= 1) recw += __shfl_xor(recw, d, 64);
+    if ((threadIdx.x & 63) == 0 && recw) atomicAdd(A.ctr + C_RECW, recw);
 }
 
 __global__ void k_write_totals(const uint32_t* ctr, uint64_t T, uint32_t* csr_off, uint32_t* f_vstart,
@@ -941,123 +945,137 @@ void stage_fill(Workspace& W, const Grid& g, const Values& val, hipStream_t s) {
 }
 
 // everything after the cut
-void stage_rest(Workspace& W, const Grid& g, const Values& val, hipStream_t s) {
+// the stages after the cut, each captured as its own graph so that the
+// stream events between graphs time them separately
+enum RestPart { REST_SCANS = 1, REST_EXTRACT = 2, REST_DEDUP = 4, REST_XYZ = 8, REST_FACES = 16, REST_ALL = 31 };
+
+void stage_rest(Workspace& W, const Grid& g, const Values& val, hipStream_t s, int part) {
     DevMem& m = W.mem;
     const Caps_& c = W.cap;
     uint32_t* ctr = W.ctr.p;
     const uint32_t* abort = ctr + C_ABORT;
     const Count NA(ctr + C_ACTIVE, c.active, abort);
-    hipLaunchKernelGGL(k_split3, dim3(nblk(c.active)), dim3(256), 0, s, NA, (const uint32_t*)W.cnt.p, W.cv.p, W.cf.p,
-                       W.cfv.p);
-    scan_u32(m, W.cv.p, W.cv.p, NA, ctr + C_VO, s);
-    scan_u32(m, W.cf.p, W.cf.p, NA, ctr + C_FO, s);
-    scan_u32(m, W.cfv.p, W.cfv.p, NA, ctr + C_FV, s);
-    ExtractArgs E;
-    E.g = g.view;
-    E.act = W.act.p;
-    E.csr_off = W.csr_off.p;
-    E.rec_loc = W.rec_loc.p;
-    E.fit = W.fit.p;
-    E.rec = W.rec.p;
-    E.k1 = W.k1.p;
-    E.tab = W.k1_words.p;
-    E.tab_off = W.k1_off.p;
-    E.n_active = ctr + C_ACTIVE;
-    E.cap_active = c.active;
-    E.ctr = ctr;
-    E.voff = W.cv.p;
-    E.foff = W.cf.p;
-    E.fvoff = W.cfv.p;
-    E.tot = ctr;
-    E.cap_vo = c.vo;
-    E.cap_fo = c.fo;
-    E.cap_fv = c.fv;
-    E.okey = W.okey.p;
-    E.oa = W.oa.p;
-    E.oj = W.oj.p;
-    E.fq_a = W.fq_a.p;
-    E.fq_l = W.fq_l.p;
-    E.fq_fn = W.fq_fn.p;
-    E.fq_bnd = W.fq_bnd.p;
-    E.fq_vs = W.fq_vs.p;
-    E.fve = W.fve.p;
-    E.errs = W.errs.p;
-    hipLaunchKernelGGL(k_extract_batched, dim3(nblk(c.active)), dim3(256), 0, s, E);
+    if (part & REST_SCANS) {
+        hipLaunchKernelGGL(k_split3, dim3(nblk(c.active)), dim3(256), 0, s, NA, (const uint32_t*)W.cnt.p, W.cv.p, W.cf.p,
+                           W.cfv.p);
+        scan_u32(m, W.cv.p, W.cv.p, NA, ctr + C_VO, s);
+        scan_u32(m, W.cf.p, W.cf.p, NA, ctr + C_FO, s);
+        scan_u32(m, W.cfv.p, W.cfv.p, NA, ctr + C_FV, s);
+    }
+    if (part & REST_EXTRACT) {
+        ExtractArgs E;
+        E.g = g.view;
+        E.act = W.act.p;
+        E.csr_off = W.csr_off.p;
+        E.rec_loc = W.rec_loc.p;
+        E.fit = W.fit.p;
+        E.rec = W.rec.p;
+        E.k1 = W.k1.p;
+        E.tab = W.k1_words.p;
+        E.tab_off = W.k1_off.p;
+        E.n_active = ctr + C_ACTIVE;
+        E.cap_active = c.active;
+        E.ctr = ctr;
+        E.voff = W.cv.p;
+        E.foff = W.cf.p;
+        E.fvoff = W.cfv.p;
+        E.tot = ctr;
+        E.cap_vo = c.vo;
+        E.cap_fo = c.fo;
+        E.cap_fv = c.fv;
+        E.okey = W.okey.p;
+        E.oa = W.oa.p;
+        E.oj = W.oj.p;
+        E.fq_a = W.fq_a.p;
+        E.fq_l = W.fq_l.p;
+        E.fq_fn = W.fq_fn.p;
+        E.fq_bnd = W.fq_bnd.p;
+        E.fq_vs = W.fq_vs.p;
+        E.fve = W.fve.p;
+        E.errs = W.errs.p;
+        hipLaunchKernelGGL(k_extract_batched, dim3(nblk(c.active)), dim3(256), 0, s, E);
+    }
 
     const Count NVO(ctr + C_VO, c.vo, abort), NFO(ctr + C_FO, c.fo, abort), NV(ctr + C_VERTS, c.vo, abort),
         NF(ctr + C_FACES, c.fo, abort);
     const int kbits = 2 + std::max({bitsf(g.view.V), bitsf(g.view.n_edge_ids) + 6, bitsf(g.view.n_face_ids) + 12,
                                     bitsf(c.active) + 8});
-    iota_u32(W.sidx.p, NVO, s);
-    sort_pairs(m, W.okey.p, W.sidx.p, NVO, kbits, s);
-    hipLaunchKernelGGL(k_heads, dim3(nblk(c.vo)), dim3(256), 0, s, NVO, (const uint64_t*)W.okey.p,
-                       (const uint32_t*)W.sidx.p, W.first.p, W.head.p);
-    scan_u32(m, W.head.p, W.hexcl.p, NVO, nullptr, s);
-    hipLaunchKernelGGL(k_assign, dim3(nblk(c.vo)), dim3(256), 0, s, NVO, (const uint32_t*)W.sidx.p,
-                       (const uint32_t*)W.head.p, (const uint32_t*)W.hexcl.p, W.seg_first.p);
-    scan_u32(m, W.first.p, W.vrank.p, NVO, ctr + C_VERTS, s);
-    hipLaunchKernelGGL(k_assign2, dim3(nblk(c.vo)), dim3(256), 0, s, NVO, (const uint32_t*)W.sidx.p,
-                       (const uint32_t*)W.head.p, (const uint32_t*)W.hexcl.p, (const uint32_t*)W.seg_first.p,
-                       (const uint32_t*)W.vrank.p, W.vid.p);
-    hipLaunchKernelGGL(k_first_list, dim3(nblk(c.vo)), dim3(256), 0, s, NVO, (const uint32_t*)W.first.p,
-                       (const uint32_t*)W.vrank.p, W.first_occ.p);
-    hipLaunchKernelGGL(k_vert_fields_ia, dim3(nblk(c.vo)), dim3(256), 0, s, g.view, (const double*)val.vals.p,
-                       (const uint32_t*)W.act.p, (const uint32_t*)W.csr_off.p, (const uint8_t*)W.fit.p,
-                       (const uint32_t*)W.rec_loc.p, (const uint16_t*)W.rec.p, (const uint16_t*)W.k1_words.p,
-                       (const uint32_t*)W.k1_off.p, (const uint32_t*)W.first_occ.p,
-                       (const uint32_t*)W.oa.p, (const uint16_t*)W.oj.p, NV, W.v_tet.p, W.v_j.p, W.v_ss.p, W.v_sv.p,
-                       W.v_fn.p, W.xyz.p);
-    // faces: a boundary face may repeat in the tet across its tet face
-    hipLaunchKernelGGL(k_face_map, dim3(nblk(c.fv)), dim3(256), 0, s, Count(ctr + C_FV, c.fv, abort), W.fve.p,
-                       (const uint32_t*)W.vid.p);
-    FaceArgs FA;
-    FA.fq_a = W.fq_a.p;
-    FA.fq_vs = W.fq_vs.p;
-    FA.fve = W.fve.p;
-    FA.act = W.act.p;
-    FA.rank = W.rank.p;
-    FA.foff = W.cf.p;
-    FA.nbr = g.nbr.p;
-    FA.fq_bnd = W.fq_bnd.p;
-    FA.fq_l = W.fq_l.p;
-    FA.nfv_total = ctr + C_FV;
-    FA.nfo_total = ctr + C_FO;
-    FA.nfv_cap = c.fv;
-    FA.nfo_cap = c.fo;
-    FA.nq = NFO;
-    FA.n_active = NA;
-    FA.first = W.ffirst.p;
-    FA.match = W.fmatch.p;
-    FA.tfc = W.ftfc.p;
-    FA.vcnt = W.fvc.p;
-    hipLaunchKernelGGL(k_face_first, dim3(nblk(c.fo)), dim3(256), 0, s, FA);
-    scan_u32(m, W.ffirst.p, W.frank.p, NFO, ctr + C_FACES, s);
-    scan_u32(m, W.ftfc.p, W.ftpos.p, NFO, ctr + C_TETFACES, s);
-    scan_u32(m, W.fvc.p, W.fvpos.p, NFO, ctr + C_FACEVERTS, s);
-    FaceOutArgs FO;
-    FO.fq_a = W.fq_a.p;
-    FO.fq_vs = W.fq_vs.p;
-    FO.fve = W.fve.p;
-    FO.act = W.act.p;
-    FO.first = W.ffirst.p;
-    FO.match = W.fmatch.p;
-    FO.frank = W.frank.p;
-    FO.tfpos = W.ftpos.p;
-    FO.vpos = W.fvpos.p;
-    FO.fq_l = W.fq_l.p;
-    FO.fq_fn = W.fq_fn.p;
-    FO.nfv_total = ctr + C_FV;
-    FO.nfv_cap = c.fv;
-    FO.nq = NFO;
-    FO.f_vstart = W.f_vstart.p;
-    FO.f_verts = W.f_verts.p;
-    FO.f_tfstart = W.f_tfstart.p;
-    FO.f_tf_tet = W.f_tf_tet.p;
-    FO.f_tf_local = W.f_tf_local.p;
-    FO.f_func = W.f_func.p;
-    hipLaunchKernelGGL(k_face_emit, dim3(nblk(c.fo)), dim3(256), 0, s, FO);
-    hipLaunchKernelGGL(k_write_totals, dim3(1), dim3(64), 0, s, (const uint32_t*)ctr, g.view.T, W.csr_off.p,
-                       W.f_vstart.p, W.f_tfstart.p, c.fo);
+    if (part & REST_DEDUP) {
+        iota_u32(W.sidx.p, NVO, s);
+        sort_pairs(m, W.okey.p, W.sidx.p, NVO, kbits, s);
+        hipLaunchKernelGGL(k_heads, dim3(nblk(c.vo)), dim3(256), 0, s, NVO, (const uint64_t*)W.okey.p,
+                           (const uint32_t*)W.sidx.p, W.first.p, W.head.p);
+        scan_u32(m, W.head.p, W.hexcl.p, NVO, nullptr, s);
+        hipLaunchKernelGGL(k_assign, dim3(nblk(c.vo)), dim3(256), 0, s, NVO, (const uint32_t*)W.sidx.p,
+                           (const uint32_t*)W.head.p, (const uint32_t*)W.hexcl.p, W.seg_first.p);
+        scan_u32(m, W.first.p, W.vrank.p, NVO, ctr + C_VERTS, s);
+        hipLaunchKernelGGL(k_assign2, dim3(nblk(c.vo)), dim3(256), 0, s, NVO, (const uint32_t*)W.sidx.p,
+                           (const uint32_t*)W.head.p, (const uint32_t*)W.hexcl.p, (const uint32_t*)W.seg_first.p,
+                           (const uint32_t*)W.vrank.p, W.vid.p);
+        hipLaunchKernelGGL(k_first_list, dim3(nblk(c.vo)), dim3(256), 0, s, NVO, (const uint32_t*)W.first.p,
+                           (const uint32_t*)W.vrank.p, W.first_occ.p);
+    }
+    if (part & REST_XYZ) {
+        hipLaunchKernelGGL(k_vert_fields_ia, dim3(nblk(c.vo)), dim3(256), 0, s, g.view, (const double*)val.vals.p,
+                           (const uint32_t*)W.act.p, (const uint32_t*)W.csr_off.p, (const uint8_t*)W.fit.p,
+                           (const uint32_t*)W.rec_loc.p, (const uint16_t*)W.rec.p, (const uint16_t*)W.k1_words.p,
+                           (const uint32_t*)W.k1_off.p, (const uint32_t*)W.first_occ.p,
+                           (const uint32_t*)W.oa.p, (const uint16_t*)W.oj.p, NV, W.v_tet.p, W.v_j.p, W.v_ss.p, W.v_sv.p,
+                           W.v_fn.p, W.xyz.p);
+    }
+    if (part & REST_FACES) {
+        // faces: a boundary face may repeat in the tet across its tet face
+        hipLaunchKernelGGL(k_face_map, dim3(nblk(c.fv)), dim3(256), 0, s, Count(ctr + C_FV, c.fv, abort), W.fve.p,
+                           (const uint32_t*)W.vid.p);
+        FaceArgs FA;
+        FA.fq_a = W.fq_a.p;
+        FA.fq_vs = W.fq_vs.p;
+        FA.fve = W.fve.p;
+        FA.act = W.act.p;
+        FA.rank = W.rank.p;
+        FA.foff = W.cf.p;
+        FA.nbr = g.nbr.p;
+        FA.fq_bnd = W.fq_bnd.p;
+        FA.fq_l = W.fq_l.p;
+        FA.nfv_total = ctr + C_FV;
+        FA.nfo_total = ctr + C_FO;
+        FA.nfv_cap = c.fv;
+        FA.nfo_cap = c.fo;
+        FA.nq = NFO;
+        FA.n_active = NA;
+        FA.first = W.ffirst.p;
+        FA.match = W.fmatch.p;
+        FA.tfc = W.ftfc.p;
+        FA.vcnt = W.fvc.p;
+        hipLaunchKernelGGL(k_face_first, dim3(nblk(c.fo)), dim3(256), 0, s, FA);
+        scan_u32(m, W.ffirst.p, W.frank.p, NFO, ctr + C_FACES, s);
+        scan_u32(m, W.ftfc.p, W.ftpos.p, NFO, ctr + C_TETFACES, s);
+        scan_u32(m, W.fvc.p, W.fvpos.p, NFO, ctr + C_FACEVERTS, s);
+        FaceOutArgs FO;
+        FO.fq_a = W.fq_a.p;
+        FO.fq_vs = W.fq_vs.p;
+        FO.fve = W.fve.p;
+        FO.act = W.act.p;
+        FO.first = W.ffirst.p;
+        FO.match = W.fmatch.p;
+        FO.frank = W.frank.p;
+        FO.tfpos = W.ftpos.p;
+        FO.vpos = W.fvpos.p;
+        FO.fq_l = W.fq_l.p;
+        FO.fq_fn = W.fq_fn.p;
+        FO.nfv_total = ctr + C_FV;
+        FO.nfv_cap = c.fv;
+        FO.nq = NFO;
+        FO.f_vstart = W.f_vstart.p;
+        FO.f_verts = W.f_verts.p;
+        FO.f_tfstart = W.f_tfstart.p;
+        FO.f_tf_tet = W.f_tf_tet.p;
+        FO.f_tf_local = W.f_tf_local.p;
+        FO.f_func = W.f_func.p;
+        hipLaunchKernelGGL(k_face_emit, dim3(nblk(c.fo)), dim3(256), 0, s, FO);
+        hipLaunchKernelGGL(k_write_totals, dim3(1), dim3(64), 0, s, (const uint32_t*)ctr, g.view.T, W.csr_off.p,
+                           W.f_vstart.p, W.f_tfstart.p, c.fo);
+    }
 }
 
 // conservative per-tet output bounds by active count (maxima of the cut's
@@ -1222,7 +1240,11 @@ void run_batched(DevMem& ctx_mem, const Grid& g, Values& val, uint32_t flags, Ru
             if (w.cap.b[0]) capture(cut_label("cut k=2", 0), [&] { launch_cut<CK2>(w, g, val, 0, grid[0], s); });
             if (w.cap.b[1]) capture(cut_label("cut k=3..4", 1), [&] { launch_cut<CMed>(w, g, val, 1, grid[1], s); });
             if (w.cap.b[2]) capture(cut_label("cut k=5..8", 2), [&] { launch_cut<CBig>(w, g, val, 2, grid[2], s); });
-            capture("extract + dedup + xyz + faces", [&] { stage_rest(w, g, val, s); });
+            capture("extract: per-tet output scans", [&] { stage_rest(w, g, val, s, REST_SCANS); });
+            capture("extract occurrences (k_extract_batched)", [&] { stage_rest(w, g, val, s, REST_EXTRACT); });
+            capture("dedup verts (sort by key)", [&] { stage_rest(w, g, val, s, REST_DEDUP); });
+            capture("compute xyz (k_vert_fields_ia)", [&] { stage_rest(w, g, val, s, REST_XYZ); });
+            capture("faces", [&] { stage_rest(w, g, val, s, REST_FACES); });
             while (w.ev.size() < w.parts.size() + 1) {
                 hipEvent_t e;
                 RIN_HIP(hipEventCreate(&e));
@@ -1270,6 +1292,12 @@ void run_batched(DevMem& ctx_mem, const Grid& g, Values& val, uint32_t flags, Ru
     out.n_faces = hctr[C_FACES];
     out.n_face_verts = hctr[C_FACEVERTS];
     out.n_tet_faces = hctr[C_TETFACES];
+    out.n_vert_occ = hctr[C_VO];
+    out.n_face_occ = hctr[C_FO];
+    out.n_face_occ_verts = hctr[C_FV];
+    out.n_cut34 = hctr[C_B3];
+    out.n_cut58 = hctr[C_B5];
+    out.n_rec_words = hctr[C_RECW];
     auto& vw = out.view;
     vw.fit = w.fit.p;
     vw.csr_off = w.csr_off.p;

@@ -66,6 +66,7 @@ struct RunOut {
     DBuf<uint32_t> cut_counts;  // n_active x 3: (vertices, faces, cells) for the robustness check
     // mesh
     uint64_t n_verts = 0, n_faces = 0, n_face_verts = 0, n_tet_faces = 0;
+    uint64_t n_vert_occ = 0, n_face_occ = 0, n_face_occ_verts = 0, n_cut34 = 0, n_cut58 = 0, n_rec_words = 0;
     DBuf<uint32_t> v_tet;
     DBuf<uint16_t> v_j;
     DBuf<uint8_t> v_ss;
@@ -119,6 +120,10 @@ void ensure_nbr(DevMem& mem, const Grid& g, hipStream_t s);
 // n + 1 word offsets; inactive tets get empty ranges)
 void gather_records(DevMem& mem, const RunOut& r, const uint64_t* tets, uint64_t n, uint64_t* rec_start,
                     uint16_t* words, uint64_t cap, hipStream_t s);
+
+// diagnostics: a plain device copy of `bytes` (reference kernel for
+// hardware-counter scaling); returns the average milliseconds per copy
+float debug_copy(DevMem& mem, uint64_t bytes, int reps, hipStream_t s);
 
 // diagnostics: run the device cut on independent plane/material sets
 void debug_cuts(DevMem& mem, int mi, const double* rows, const int* ks, int ncase, int cap, uint16_t* words,

@@ -135,6 +135,9 @@ void extract_mi(DevMem& mem, const Grid& g, const Values& val, RunOut& out, Time
     scan_u32(mem, cf.p, cf.p, NA, cf.p + NA, s);
     scan_u32(mem, cfv.p, cfv.p, NA, cfv.p + NA, s);
     const uint64_t NVO = read1(cv.p + NA, s), NF = read1(cf.p + NA, s), NFV = read1(cfv.p + NA, s);
+    out.n_vert_occ = NVO;
+    out.n_face_occ = NF;
+    out.n_face_occ_verts = NFV;
     DBuf<uint64_t> okey(mem, NVO + 1);
     DBuf<uint32_t> oa(mem, NVO + 1), fq_a(mem, NF + 1), fq_vs(mem, NF + 1), fve(mem, NFV + 1);
     DBuf<uint16_t> oj(mem, NVO + 1), fq_l(mem, NF + 1);
@@ -383,6 +386,8 @@ void run_pipeline(DevMem& mem, const Grid& g, const Values& val, uint32_t flags,
             out.nb[k <= lo ? 0 : (k == lo + 1 ? 1 : 2)]++;
             kmax = std::max<int>(kmax, k);
             if (k > 8) huge.push_back((uint32_t)a);
+            out.n_cut34 += k >= 3 && k <= 4;
+            out.n_cut58 += k >= 5 && k <= 8;
         }
         if (kmax > kMaxHuge) {
             out.err |= ERR_CAPACITY;
@@ -480,6 +485,9 @@ void run_pipeline(DevMem& mem, const Grid& g, const Values& val, uint32_t flags,
     scan_u32(mem, cf.p, cf.p, NA, cf.p + NA, s);
     scan_u32(mem, cfv.p, cfv.p, NA, cfv.p + NA, s);
     const uint64_t NVO = read1(cv.p + NA, s), NFO = read1(cf.p + NA, s), NFV = read1(cfv.p + NA, s);
+    out.n_vert_occ = NVO;
+    out.n_face_occ = NFO;
+    out.n_face_occ_verts = NFV;
     DBuf<uint64_t> okey(mem, NVO + 1);
     DBuf<uint32_t> oa(mem, NVO + 1);
     DBuf<uint16_t> oj(mem, NVO + 1);
@@ -750,6 +758,26 @@ void gather_records(DevMem& mem, const RunOut& r, const uint64_t* tets, uint64_t
     RIN_HIP(hipGetLastError());
     RIN_HIP(hipMemcpyAsync(words, dout.p, 2 * o, hipMemcpyDeviceToHost, s));
     RIN_HIP(hipStreamSynchronize(s));
+}
+
+float debug_copy(DevMem& mem, uint64_t bytes, int reps, hipStream_t s) {
+    const uint64_t n = bytes / 8;
+    if (n == 0 || reps < 1) throw std::invalid_argument("copy size and repetitions must be positive");
+    DBuf<uint64_t> a(mem, n), b(mem, n);
+    fill_u64(a.p, 0x0123456789abcdefull, n, s);
+    copy_u64(b.p, a.p, Count(n), s);  // warm-up
+    hipEvent_t e0, e1;
+    RIN_HIP(hipEventCreate(&e0));
+    RIN_HIP(hipEventCreate(&e1));
+    RIN_HIP(hipEventRecord(e0, s));
+    for (int r = 0; r < reps; ++r) copy_u64(b.p, a.p, Count(n), s);
+    RIN_HIP(hipEventRecord(e1, s));
+    RIN_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    RIN_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return ms / reps;
 }
 
 void debug_cuts(DevMem& mem, int mi, const double* rows, const int* ks, int ncase, int cap, uint16_t* words, int* lens,

@@ -208,6 +208,12 @@ int rin_result_sizes(rin_ctx* c, const rin_result* res, rin_sizes* s) {
     s->n_faces = r.n_faces;
     s->n_face_verts = r.n_face_verts;
     s->n_tet_faces = r.n_tet_faces;
+    s->n_vert_occ = r.n_vert_occ;
+    s->n_face_occ = r.n_face_occ;
+    s->n_face_occ_verts = r.n_face_occ_verts;
+    s->n_cut[0] = r.n_cut34;
+    s->n_cut[1] = r.n_cut58;
+    s->n_rec_words = r.n_rec_words;
     return RIN_OK;
 }
 
@@ -316,6 +322,11 @@ int rin_debug_cuts(rin_ctx* c, int32_t mi, const double* rows, const int32_t* ks
                    uint16_t* words, int32_t* lens) {
     if (!rows || !ks || !words || !lens || n < 0 || cap <= 8) return fail(c, RIN_ERR_ARG, "bad argument");
     return guarded(c, [&] { rin::debug_cuts(c->mem, mi, rows, ks, n, cap, words, lens, c->stream); });
+}
+
+int rin_debug_copy(rin_ctx* c, uint64_t bytes, int32_t reps, double* ms) {
+    if (!ms) return fail(c, RIN_ERR_ARG, "null argument");
+    return guarded(c, [&] { *ms = rin::debug_copy(c->mem, bytes, reps, c->stream); });
 }
 
 }  // extern "C"

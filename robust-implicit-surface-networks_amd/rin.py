@@ -36,7 +36,8 @@ class FuncDesc(ctypes.Structure):
 class Sizes(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint64) for n in
                 ("n_tets", "n_funcs", "num_degenerate", "num_intersecting", "b0", "b1", "b2", "n_func_in_tet",
-                 "n_verts", "n_faces", "n_face_verts", "n_tet_faces")]
+                 "n_verts", "n_faces", "n_face_verts", "n_tet_faces", "n_vert_occ", "n_face_occ",
+                 "n_face_occ_verts", "n_cut34", "n_cut58", "n_rec_words")]
 
 
 class RinError(RuntimeError):
@@ -79,6 +80,7 @@ def lib():
         L.rin_result_release.argtypes = [_vp, _vp]
         L.rin_debug_cuts.argtypes = [_vp, ctypes.c_int32, _f64p, _i32p, ctypes.c_int32, ctypes.c_int32, _u16p,
                                      _i32p]
+        L.rin_debug_copy.argtypes = [_vp, ctypes.c_uint64, ctypes.c_int32, ctypes.POINTER(ctypes.c_double)]
         _lib = L
     return _lib
 
@@ -135,6 +137,12 @@ class Context:
         self.check(lib().rin_grid_explicit(self.h, _p(pts, _f64p), pts.shape[0], _p(tets, _u64p), tets.shape[0],
                                            ctypes.byref(g)))
         return Grid(self, g, pts.shape[0], tets.shape[0])
+
+    def debug_copy(self, nbytes, reps=5):
+        """Average milliseconds of one plain device copy of nbytes."""
+        ms = ctypes.c_double(0)
+        self.check(lib().rin_debug_copy(self.h, int(nbytes), int(reps), ctypes.byref(ms)))
+        return ms.value
 
     def debug_cuts(self, mi, cases, cap=4096):
         """cases: list of (k, 4) arrays -> list of uint16 records (or negative codes)."""
