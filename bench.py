@@ -171,6 +171,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--stages", action="store_true", help="print per-stage device times to stderr")
     ap.add_argument("--form", choices=["both", "plain", "batched"], default="both")
+    ap.add_argument("--serve-jobs", type=int, default=48, help="serving stream length (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=20.0, help="serving arrivals per second (Poisson)")
+    ap.add_argument("--serve-slots", type=int, default=1,
+                    help="server slots (the library currently runs one job at a time)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also run plain device copies of MIB MiB (reference kernel for counter scaling)")
     ap.add_argument("--pmc-summary", nargs=3, metavar=("FETCH_CSV", "WRITE_CSV", "TRACE_CSV"),
@@ -280,6 +284,15 @@ def main():
         "path_roofline": {"algorithmic_bytes": pb, "achieved_GBs": pb / sps / 1e9,
                           "frac": pb / sps / 1e9 / HBM_PEAK_GBS},
     }
+    if args.serve_jobs:
+        import serving
+
+        jobs = serving.make_jobs(args.serve_jobs)
+        serving.run_stream(jobs[:4], rate=0.0, slots=args.serve_slots)  # warm-up: grids, code objects
+        summary, _, _ = serving.run_stream(jobs, rate=args.serve_rate, slots=args.serve_slots)
+        summary["workload"] = ("Poisson arrivals; each job a %s grid, 2..32 mixed functions, 70%% implicit "
+                               "arrangement / 30%% material interface (seeded)" % "{17^3, 33^3, 65^3}")
+        out["serving"] = summary
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(res, fs)
         out["vs_cpu"] = out["value"] / out["cpu_baseline"]["value"]

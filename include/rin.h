@@ -78,6 +78,8 @@ typedef struct rin_sizes {
  * Replaces the process-global lookup-table state of the reference
  * (app/implicit_arrangement.cpp:29-42). */
 int rin_create(rin_ctx** out, int32_t device);
+/* Free and total device memory of the context's GPU, in bytes. */
+int rin_device_memory(rin_ctx* ctx, uint64_t* free_bytes, uint64_t* total_bytes);
 int rin_destroy(rin_ctx* ctx);
 const char* rin_last_error(const rin_ctx* ctx);
 

@@ -999,7 +999,7 @@ void stage_rest(Workspace& W, const Grid& g, const Values& val, hipStream_t s, i
     const Count NVO(ctr + C_VO, c.vo, abort), NFO(ctr + C_FO, c.fo, abort), NV(ctr + C_VERTS, c.vo, abort),
         NF(ctr + C_FACES, c.fo, abort);
     const int kbits = 2 + std::max({bitsf(g.view.V), bitsf(g.view.n_edge_ids) + 6, bitsf(g.view.n_face_ids) + 12,
-                                    bitsf(c.active) + 8});
+                                    bitsf(c.active) + 10});
     if (part & REST_DEDUP) {
         iota_u32(W.sidx.p, NVO, s);
         sort_pairs(m, W.okey.p, W.sidx.p, NVO, kbits, s);
@@ -1165,7 +1165,7 @@ void run_batched(DevMem& ctx_mem, const Grid& g, Values& val, uint32_t flags, Ru
             }
             stage_count(w, g, val, s);
             read_back();
-            if (hctr[C_BX]) throw std::runtime_error("a tet has more than 8 active functions; the device cut supports at most 8");
+            if (hctr[C_BX]) throw NeedsPlainForm("a tet has more than 8 active functions");
             Caps_ need;
             caps_from_counts(hctr, need);
             if (!caps_ok(w.cap, need)) {
@@ -1200,6 +1200,10 @@ void run_batched(DevMem& ctx_mem, const Grid& g, Values& val, uint32_t flags, Ru
                         if (c == -2 && (q != 0 || only_wave || only_thread)) continue;
                         if (only_table && q == 0 && c != -2) continue;
                         if ((only_wave && c >= 0) || (only_thread && c < 0)) continue;
+                        // thread forms keep the complex in private memory: the
+                        // runtime sizes scratch by the dispatch's waves, so the
+                        // larger complexes get smaller grids
+                        if (c > (q == 2 ? 2 : (q == 1 ? 4 : 16))) continue;
                         const int grid = c < 0 ? c : std::max(1, n_cu * c);
                         float ms = time_ms(s, [&] {
                             if (q == 0 && w.cap.b[0]) launch_cut<CK2>(w, g, val, 0, grid, s);
@@ -1266,7 +1270,7 @@ void run_batched(DevMem& ctx_mem, const Grid& g, Values& val, uint32_t flags, Ru
         }
         if (e) {
             out.err = e;
-            throw std::runtime_error("device cut failed: complex exceeds capacity or predicate range");
+            throw NeedsPlainForm("a complex exceeds the bucket capacities or the predicate range");
         }
         ran = true;
     }

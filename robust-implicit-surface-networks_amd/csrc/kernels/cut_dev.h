@@ -32,36 +32,43 @@ namespace dev {
 
 constexpr uint8_t NIL = 0xFF;
 
-template <int MV_, int MF_, int MFV_, int MP_>
+template <int MV_, int MF_, int MFV_, int MP_, class Idx_ = uint8_t>
 struct Caps {
     static constexpr int MV = MV_;    // vertices
     static constexpr int MF = MF_;    // faces
     static constexpr int MFV = MFV_;  // vertices per face
     static constexpr int MP = MP_;    // labels (4 + active functions)
+    using Idx = Idx_;                 // vertex / face / cell / label ids
+    static constexpr Idx NIL = Idx(~Idx(0));
+    // cells (IA): 8 per label with 8-bit ids, 64 per label with 16-bit ids
+    static constexpr int MC = sizeof(Idx) == 1 ? (MP * 8 < 254 ? MP * 8 : 254) : (MP * 64 < 65534 ? MP * 64 : 65534);
 };
 
 template <class C>
 struct Complex {
+    using I = typename C::Idx;
     int nv, nf, nc;
     bool dup, err;
-    uint8_t vl[C::MV][4];          // IA: 3 plane ids (+NIL); MI: 4 labels
-    uint8_t fa[C::MF], fb[C::MF], fc[C::MF], fn[C::MF];  // IA: plane,pos,neg ; MI: pos,neg,-
-    uint8_t fv[C::MF][C::MFV];
-    uint8_t fe[C::MF][C::MFV];     // third plane / label of edge fv[i] -> fv[i+1]
-    uint8_t cells[C::MP];          // MI: material label per cell
-    uint8_t grp[C::MP], ori[C::MP];
+    I vl[C::MV][4];          // IA: 3 plane ids (+NIL); MI: 4 labels
+    I fa[C::MF], fb[C::MF], fc[C::MF], fn[C::MF];  // IA: plane,pos,neg ; MI: pos,neg,-
+    I fv[C::MF][C::MFV];
+    I fe[C::MF][C::MFV];     // third plane / label of edge fv[i] -> fv[i+1]
+    I cells[C::MP];          // MI: material label per cell
+    I grp[C::MP], ori[C::MP];
 };
 
-__host__ __device__ inline void sort3(uint8_t& a, uint8_t& b, uint8_t& c) {
-    uint8_t t;
+template <class T>
+__host__ __device__ inline void sort3(T& a, T& b, T& c) {
+    T t;
     if (a > b) { t = a; a = b; b = t; }
     if (b > c) { t = b; b = c; c = t; }
     if (a > b) { t = a; a = b; b = t; }
 }
-__host__ __device__ inline void sort4(uint8_t* v) {
+template <class T>
+__host__ __device__ inline void sort4(T* v) {
     for (int i = 1; i < 4; ++i)
         for (int j = i; j > 0 && v[j - 1] > v[j]; --j) {
-            uint8_t t = v[j];
+            T t = v[j];
             v[j] = v[j - 1];
             v[j - 1] = t;
         }
@@ -73,20 +80,21 @@ __host__ __device__ inline void sort4(uint8_t* v) {
 // planes exceed the label capacity
 template <class C>
 __host__ __device__ void ia_init(Complex<C>& X, int k) {
+    using I = typename C::Idx;
     X.err = false;
     X.dup = false;
     X.nv = 4;
-    const uint8_t corner[4][3] = {{1, 2, 3}, {0, 2, 3}, {0, 1, 3}, {0, 1, 2}};
+    const unsigned char corner[4][3] = {{1, 2, 3}, {0, 2, 3}, {0, 1, 3}, {0, 1, 2}};
     for (int v = 0; v < 4; ++v) {
         for (int q = 0; q < 3; ++q) X.vl[v][q] = corner[v][q];
-        X.vl[v][3] = NIL;
+        X.vl[v][3] = C::NIL;
     }
-    const uint8_t cyc[4][3] = {{1, 3, 2}, {0, 2, 3}, {0, 3, 1}, {0, 1, 2}};
+    const unsigned char cyc[4][3] = {{1, 3, 2}, {0, 2, 3}, {0, 3, 1}, {0, 1, 2}};
     X.nf = 4;
     for (int i = 0; i < 4; ++i) {
         X.fa[i] = i;
         X.fb[i] = 0;
-        X.fc[i] = NIL;
+        X.fc[i] = C::NIL;
         X.fn[i] = 3;
         for (int q = 0; q < 3; ++q) {
             X.fv[i][q] = cyc[i][q];
@@ -95,7 +103,7 @@ __host__ __device__ void ia_init(Complex<C>& X, int k) {
     }
     X.nc = 1;
     for (int p = 0; p < C::MP; ++p) {
-        X.grp[p] = p < 4 ? p : NIL;
+        X.grp[p] = p < 4 ? p : C::NIL;
         X.ori[p] = 1;
     }
     if (4 + k > C::MP) X.err = true;
@@ -110,7 +118,8 @@ __host__ __device__ void ia_classify(const double (*P)[4], int j, Complex<C>& X,
 // inserts plane Pid given its sign s[v] at every vertex; the result depends
 // on the complex and the signs only
 template <class C>
-__host__ __device__ void ia_insert(Complex<C>& X, uint8_t Pid, int8_t* s) {
+__host__ __device__ void ia_insert(Complex<C>& X, typename C::Idx Pid, int8_t* s) {
+    using I = typename C::Idx;
     {
         bool any_pos = false, any_neg = false;
         for (int v = 0; v < X.nv; ++v) {
@@ -124,7 +133,7 @@ __host__ __device__ void ia_insert(Complex<C>& X, uint8_t Pid, int8_t* s) {
             if (allz) cop = f;
         }
         if (cop >= 0) {
-            const uint8_t cell = X.fb[cop];
+            const I cell = X.fb[cop];
             int sv = 0;
             for (int g = 0; g < X.nf && sv == 0; ++g) {
                 if (X.fb[g] != cell && X.fc[g] != cell) continue;
@@ -134,9 +143,9 @@ __host__ __device__ void ia_insert(Complex<C>& X, uint8_t Pid, int8_t* s) {
                         break;
                     }
             }
-            const uint8_t pl = X.fa[cop];
+            const I pl = X.fa[cop];
             X.grp[Pid] = X.grp[pl];
-            X.ori[Pid] = (sv > 0) ? X.ori[pl] : (uint8_t)(1 - X.ori[pl]);
+            X.ori[Pid] = (sv > 0) ? X.ori[pl] : (I)(1 - X.ori[pl]);
             X.dup = true;
             return;
         }
@@ -145,14 +154,14 @@ __host__ __device__ void ia_insert(Complex<C>& X, uint8_t Pid, int8_t* s) {
         if (!any_pos || !any_neg) return;
     }
     // one new vertex per edge with a strict sign change
-    uint8_t cu[C::MV], cw[C::MV], cx[C::MV];
+    I cu[C::MV], cw[C::MV], cx[C::MV];
     int ncross = 0;
     for (int f = 0; f < X.nf; ++f) {
         const int n = X.fn[f];
         for (int i = 0; i < n; ++i) {
-            uint8_t u = X.fv[f][i], w = X.fv[f][(i + 1) % n];
+            I u = X.fv[f][i], w = X.fv[f][(i + 1) % n];
             if (s[u] * s[w] >= 0) continue;
-            uint8_t a = u < w ? u : w, b = u < w ? w : u;
+            I a = u < w ? u : w, b = u < w ? w : u;
             bool have = false;
             for (int q = 0; q < ncross; ++q) have |= (cu[q] == a && cw[q] == b);
             if (have) continue;
@@ -160,25 +169,25 @@ __host__ __device__ void ia_insert(Complex<C>& X, uint8_t Pid, int8_t* s) {
                 X.err = true;
                 return;
             }
-            uint8_t t0 = X.fa[f], t1 = X.fe[f][i], t2 = Pid;
+            I t0 = X.fa[f], t1 = X.fe[f][i], t2 = Pid;
             sort3(t0, t1, t2);
             X.vl[X.nv][0] = t0;
             X.vl[X.nv][1] = t1;
             X.vl[X.nv][2] = t2;
-            X.vl[X.nv][3] = NIL;
+            X.vl[X.nv][3] = C::NIL;
             s[X.nv] = 0;
             cu[ncross] = a;
             cw[ncross] = b;
-            cx[ncross] = (uint8_t)X.nv;
+            cx[ncross] = (I)X.nv;
             ++ncross;
             ++X.nv;
         }
     }
-    auto cross_of = [&](uint8_t u, uint8_t w) -> uint8_t {
-        uint8_t a = u < w ? u : w, b = u < w ? w : u;
+    auto cross_of = [&](I u, I w) -> I {
+        I a = u < w ? u : w, b = u < w ? w : u;
         for (int q = 0; q < ncross; ++q)
             if (cu[q] == a && cw[q] == b) return cx[q];
-        return NIL;
+        return C::NIL;
     };
 
     // split faces: positive part keeps the id, negative part appended
@@ -195,10 +204,10 @@ __host__ __device__ void ia_insert(Complex<C>& X, uint8_t Pid, int8_t* s) {
             X.err = true;
             return;
         }
-        uint8_t av[2 * C::MFV], ae[2 * C::MFV];
+        I av[2 * C::MFV], ae[2 * C::MFV];
         int m = 0;
         for (int i = 0; i < n; ++i) {
-            uint8_t u = X.fv[f][i], w = X.fv[f][(i + 1) % n];
+            I u = X.fv[f][i], w = X.fv[f][(i + 1) % n];
             av[m] = u;
             ae[m] = X.fe[f][i];
             ++m;
@@ -217,7 +226,7 @@ __host__ __device__ void ia_insert(Complex<C>& X, uint8_t Pid, int8_t* s) {
                 X.err = true;
                 return;
             }
-            uint8_t nvv[C::MFV], nee[C::MFV];
+            I nvv[C::MFV], nee[C::MFV];
             for (int q = 0; q < ni; ++q) {
                 int ia = idx[q], ib = idx[(q + 1) % ni];
                 nvv[q] = av[ia];
@@ -227,7 +236,7 @@ __host__ __device__ void ia_insert(Complex<C>& X, uint8_t Pid, int8_t* s) {
                 X.fv[dst][q] = nvv[q];
                 X.fe[dst][q] = nee[q];
             }
-            X.fn[dst] = (uint8_t)ni;
+            X.fn[dst] = (I)ni;
             if (sgn < 0) {
                 X.fa[dst] = X.fa[f];
                 X.fb[dst] = X.fb[f];
@@ -250,11 +259,11 @@ __host__ __device__ void ia_insert(Complex<C>& X, uint8_t Pid, int8_t* s) {
             }
         }
         if (!(cp && cq)) continue;
-        if (X.nf >= C::MF || X.nc >= C::MP * 8 || X.nc >= 254) {
+        if (X.nf >= C::MF || X.nc >= C::MC) {
             X.err = true;
             return;
         }
-        const uint8_t c2 = (uint8_t)X.nc++;
+        const I c2 = (I)X.nc++;
         for (int g = 0; g < X.nf; ++g) {
             if (X.fb[g] != c && X.fc[g] != c) continue;
             bool gn = false;
@@ -263,13 +272,13 @@ __host__ __device__ void ia_insert(Complex<C>& X, uint8_t Pid, int8_t* s) {
             if (X.fb[g] == c) X.fb[g] = c2;
             if (X.fc[g] == c) X.fc[g] = c2;
         }
-        uint8_t ea[C::MFV], eb[C::MFV], el[C::MFV];
+        I ea[C::MFV], eb[C::MFV], el[C::MFV];
         int ne = 0;
         for (int g = 0; g < X.nf; ++g) {
             if (X.fb[g] != c && X.fc[g] != c) continue;
             const int n = X.fn[g];
             for (int i = 0; i < n; ++i) {
-                uint8_t a = X.fv[g][i], b = X.fv[g][(i + 1) % n];
+                I a = X.fv[g][i], b = X.fv[g][(i + 1) % n];
                 if (s[a] != 0 || s[b] != 0) continue;
                 if (ne >= C::MFV) {
                     X.err = true;
@@ -292,7 +301,7 @@ __host__ __device__ void ia_insert(Complex<C>& X, uint8_t Pid, int8_t* s) {
         }
         const int F = X.nf++;
         X.fa[F] = Pid;
-        X.fb[F] = (uint8_t)c;
+        X.fb[F] = (I)c;
         X.fc[F] = c2;
         int cur = 0, cnt = 0;
         for (;;) {
@@ -316,7 +325,7 @@ __host__ __device__ void ia_insert(Complex<C>& X, uint8_t Pid, int8_t* s) {
             X.err = true;
             return;
         }
-        X.fn[F] = (uint8_t)cnt;
+        X.fn[F] = (I)cnt;
     }
 }
 
@@ -328,7 +337,7 @@ __host__ __device__ void ia_cut(const double (*P)[4], int k, Complex<C>& X) {
     for (int j = 0; j < k; ++j) {
         ia_classify(P, j, X, s);
         if (X.err) return;
-        ia_insert(X, (uint8_t)(4 + j), s);
+        ia_insert(X, (typename C::Idx)(4 + j), s);
         if (X.err) return;
     }
 }
@@ -336,6 +345,7 @@ __host__ __device__ void ia_cut(const double (*P)[4], int k, Complex<C>& X) {
 // record layout: include/rin/cut_record.h; returns words written or -1
 template <class C>
 __host__ __device__ int write_record_ia(const Complex<C>& X, int k, uint16_t* out, int cap) {
+    using I = typename C::Idx;
     int p = 8;
     const int L = 4 + k;
     auto put = [&](uint16_t w) {
@@ -351,7 +361,7 @@ __host__ __device__ int write_record_ia(const Complex<C>& X, int k, uint16_t* ou
     for (int f = 0; f < X.nf; ++f) {
         put(X.fa[f]);
         put(X.fb[f]);
-        put(X.fc[f] == NIL ? 0xFFFF : X.fc[f]);
+        put(X.fc[f] == C::NIL ? 0xFFFF : X.fc[f]);
         put(X.fn[f]);
         for (int q = 0; q < X.fn[f]; ++q) put(X.fv[f][q]);
     }
@@ -374,9 +384,8 @@ __host__ __device__ int write_record_ia(const Complex<C>& X, int k, uint16_t* ou
 // ------------------------------------------------------------------ MI
 
 // chain directed edges (ea->eb, label el) into one cycle from edge 0
-template <int CAP>
-__host__ __device__ inline int chain_edges(const uint8_t* ea, const uint8_t* eb, const uint8_t* el, int ne, uint8_t* ov,
-                                  uint8_t* oe) {
+template <int CAP, class T>
+__host__ __device__ inline int chain_edges(const T* ea, const T* eb, const T* el, int ne, T* ov, T* oe) {
     if (ne <= 0 || ne > CAP) return -1;
     int cur = 0, cnt = 0;
     for (;;) {
@@ -398,9 +407,10 @@ __host__ __device__ inline int chain_edges(const uint8_t* ea, const uint8_t* eb,
 
 template <class C>
 __host__ __device__ void mi_cut(const double (*P)[4], int k, Complex<C>& X) {
+    using I = typename C::Idx;
     X.err = false;
     X.dup = false;
-    for (int p = 0; p < C::MP; ++p) X.grp[p] = p < 4 ? p : NIL;
+    for (int p = 0; p < C::MP; ++p) X.grp[p] = p < 4 ? p : C::NIL;
     if (4 + k > C::MP || k < 1) {
         X.err = k > 0;
         X.nv = X.nf = X.nc = 0;
@@ -413,12 +423,12 @@ __host__ __device__ void mi_cut(const double (*P)[4], int k, Complex<C>& X) {
             if (q != i) X.vl[i][n++] = q;
         X.vl[i][3] = 4;
     }
-    const uint8_t cyc[4][3] = {{1, 2, 3}, {0, 3, 2}, {0, 1, 3}, {0, 2, 1}};
+    const unsigned char cyc[4][3] = {{1, 2, 3}, {0, 3, 2}, {0, 1, 3}, {0, 2, 1}};
     X.nf = 4;
     for (int i = 0; i < 4; ++i) {
         X.fa[i] = i;
         X.fb[i] = 4;
-        X.fc[i] = NIL;
+        X.fc[i] = C::NIL;
         X.fn[i] = 3;
         for (int q = 0; q < 3; ++q) {
             X.fv[i][q] = cyc[i][q];
@@ -431,7 +441,7 @@ __host__ __device__ void mi_cut(const double (*P)[4], int k, Complex<C>& X) {
     int8_t s[C::MV];
     constexpr int PE = 4 * C::MFV * 4;
     for (int j = 1; j < k; ++j) {
-        const uint8_t m = 4 + j;
+        const I m = 4 + j;
         const int nv0 = X.nv;
         bool any_pos = false;
         for (int v = 0; v < nv0; ++v) {
@@ -445,7 +455,7 @@ __host__ __device__ void mi_cut(const double (*P)[4], int k, Complex<C>& X) {
         for (int l = 0; l < C::MP; ++l) cpos[l] = cneg[l] = false;
         for (int f = 0; f < X.nf; ++f)
             for (int side = 0; side < 2; ++side) {
-                uint8_t lab = side ? X.fb[f] : X.fa[f];
+                I lab = side ? X.fb[f] : X.fa[f];
                 if (lab < 4) continue;
                 for (int q = 0; q < X.fn[f]; ++q) {
                     cpos[lab] |= s[X.fv[f][q]] > 0;
@@ -455,7 +465,7 @@ __host__ __device__ void mi_cut(const double (*P)[4], int k, Complex<C>& X) {
         if (!any_pos) {
             X.grp[m] = m;
             for (int c = 0; c < X.nc; ++c) {
-                uint8_t L = X.cells[c];
+                I L = X.cells[c];
                 if (!cpos[L] && !cneg[L]) {
                     X.grp[m] = X.grp[L];
                     X.dup = true;
@@ -470,14 +480,14 @@ __host__ __device__ void mi_cut(const double (*P)[4], int k, Complex<C>& X) {
             consumed[l] = cpos[l] && !cneg[l];
             split[l] = cpos[l] && cneg[l];
         }
-        uint8_t cu[C::MV], cw[C::MV], cx[C::MV];
+        I cu[C::MV], cw[C::MV], cx[C::MV];
         int ncross = 0;
         for (int f = 0; f < X.nf; ++f) {
             const int n = X.fn[f];
             for (int i = 0; i < n; ++i) {
-                uint8_t u = X.fv[f][i], w = X.fv[f][(i + 1) % n];
+                I u = X.fv[f][i], w = X.fv[f][(i + 1) % n];
                 if (s[u] * s[w] >= 0) continue;
-                uint8_t a = u < w ? u : w, b = u < w ? w : u;
+                I a = u < w ? u : w, b = u < w ? w : u;
                 bool have = false;
                 for (int q = 0; q < ncross; ++q) have |= (cu[q] == a && cw[q] == b);
                 if (have) continue;
@@ -485,27 +495,27 @@ __host__ __device__ void mi_cut(const double (*P)[4], int k, Complex<C>& X) {
                     X.err = true;
                     return;
                 }
-                uint8_t t[4] = {X.fa[f], X.fb[f], X.fe[f][i], m};
+                I t[4] = {X.fa[f], X.fb[f], X.fe[f][i], m};
                 sort4(t);
                 for (int q = 0; q < 4; ++q) X.vl[X.nv][q] = t[q];
                 s[X.nv] = 0;
                 cu[ncross] = a;
                 cw[ncross] = b;
-                cx[ncross] = (uint8_t)X.nv;
+                cx[ncross] = (I)X.nv;
                 ++ncross;
                 ++X.nv;
             }
         }
-        auto cross_of = [&](uint8_t u, uint8_t w) -> uint8_t {
-            uint8_t a = u < w ? u : w, b = u < w ? w : u;
+        auto cross_of = [&](I u, I w) -> I {
+            I a = u < w ? u : w, b = u < w ? w : u;
             for (int q = 0; q < ncross; ++q)
                 if (cu[q] == a && cw[q] == b) return cx[q];
-            return NIL;
+            return C::NIL;
         };
         // tet-face edges won by m, per tet face: (plane, from, to, label)
-        uint8_t pp[PE], pa[PE], pb[PE], pl[PE];
+        I pp[PE], pa[PE], pb[PE], pl[PE];
         int npe = 0;
-        auto add_plus = [&](uint8_t plane, uint8_t a, uint8_t b, uint8_t l) {
+        auto add_plus = [&](I plane, I a, I b, I l) {
             if (npe >= PE) {
                 X.err = true;
                 return;
@@ -526,10 +536,10 @@ __host__ __device__ void mi_cut(const double (*P)[4], int k, Complex<C>& X) {
                 fp |= s[X.fv[f][q]] > 0;
                 fq |= s[X.fv[f][q]] < 0;
             }
-            const uint8_t P0 = X.fa[f], N0 = X.fb[f];
-            uint8_t nv_[C::MFV], ne_[C::MFV];
+            const I P0 = X.fa[f], N0 = X.fb[f];
+            I nv_[C::MFV], ne_[C::MFV];
             int nn = -1;
-            uint8_t oP = P0, oN = N0;
+            I oP = P0, oN = N0;
             if (fq && !fp) {
                 nn = n;
                 for (int q = 0; q < n; ++q) {
@@ -537,10 +547,10 @@ __host__ __device__ void mi_cut(const double (*P)[4], int k, Complex<C>& X) {
                     ne_[q] = X.fe[f][q];
                 }
             } else if (fq && fp) {
-                uint8_t av[2 * C::MFV], ae[2 * C::MFV];
+                I av[2 * C::MFV], ae[2 * C::MFV];
                 int mm = 0;
                 for (int i = 0; i < n; ++i) {
-                    uint8_t u = X.fv[f][i], ww = X.fv[f][(i + 1) % n];
+                    I u = X.fv[f][i], ww = X.fv[f][(i + 1) % n];
                     av[mm] = u;
                     ae[mm] = X.fe[f][i];
                     ++mm;
@@ -559,10 +569,10 @@ __host__ __device__ void mi_cut(const double (*P)[4], int k, Complex<C>& X) {
                         X.err = true;
                         return;
                     }
-                    const uint8_t cutlab = sgn < 0 ? m : N0;
+                    const I cutlab = sgn < 0 ? m : N0;
                     for (int q = 0; q < ni; ++q) {
                         int ia = idx[q], ib = idx[(q + 1) % ni];
-                        uint8_t lab = ((ia + 1) % mm == ib) ? ae[ia] : cutlab;
+                        I lab = ((ia + 1) % mm == ib) ? ae[ia] : cutlab;
                         if (sgn < 0) {
                             nv_[q] = av[ia];
                             ne_[q] = lab;
@@ -611,8 +621,8 @@ __host__ __device__ void mi_cut(const double (*P)[4], int k, Complex<C>& X) {
             if (nn >= 0) {
                 X.fa[w] = oP;
                 X.fb[w] = oN;
-                X.fc[w] = NIL;
-                X.fn[w] = (uint8_t)nn;
+                X.fc[w] = C::NIL;
+                X.fn[w] = (I)nn;
                 for (int q = 0; q < nn; ++q) {
                     X.fv[w][q] = nv_[q];
                     X.fe[w][q] = ne_[q];
@@ -623,7 +633,7 @@ __host__ __device__ void mi_cut(const double (*P)[4], int k, Complex<C>& X) {
         X.nf = w;
         // one merged face per tet face won by m
         for (int i = 0; i < 4; ++i) {
-            uint8_t ea[C::MFV * 2], eb[C::MFV * 2], el[C::MFV * 2];
+            I ea[C::MFV * 2], eb[C::MFV * 2], el[C::MFV * 2];
             int ne = 0, any = 0;
             for (int q = 0; q < npe; ++q) {
                 if (pp[q] != i) continue;
@@ -645,7 +655,7 @@ __host__ __device__ void mi_cut(const double (*P)[4], int k, Complex<C>& X) {
                 ++ne;
             }
             if (!any) continue;
-            uint8_t vs[C::MFV * 2], es[C::MFV * 2];
+            I vs[C::MFV * 2], es[C::MFV * 2];
             int n = chain_edges<2 * C::MFV>(ea, eb, el, ne, vs, es);
             if (n < 0 || X.nf >= C::MF) {
                 X.err = true;
@@ -666,21 +676,21 @@ __host__ __device__ void mi_cut(const double (*P)[4], int k, Complex<C>& X) {
             }
             X.fa[F] = i;
             X.fb[F] = m;
-            X.fc[F] = NIL;
-            X.fn[F] = (uint8_t)cnt;
+            X.fc[F] = C::NIL;
+            X.fn[F] = (I)cnt;
         }
         // one face (m | X) per cell crossed by f_m = f_X
         for (int c = 0; c < X.nc; ++c) {
-            const uint8_t L = X.cells[c];
+            const I L = X.cells[c];
             if (!split[L]) continue;
-            uint8_t ea[C::MFV], eb[C::MFV], el[C::MFV];
+            I ea[C::MFV], eb[C::MFV], el[C::MFV];
             int ne = 0;
             for (int g = 0; g < X.nf; ++g) {
                 if (X.fa[g] != L && X.fb[g] != L) continue;
                 const int n = X.fn[g];
-                const uint8_t other = (X.fa[g] == L) ? X.fb[g] : X.fa[g];
+                const I other = (X.fa[g] == L) ? X.fb[g] : X.fa[g];
                 for (int i = 0; i < n; ++i) {
-                    uint8_t a = X.fv[g][i], b = X.fv[g][(i + 1) % n];
+                    I a = X.fv[g][i], b = X.fv[g][(i + 1) % n];
                     if (s[a] != 0 || s[b] != 0) continue;
                     if (ne >= C::MFV) {
                         X.err = true;
@@ -709,8 +719,8 @@ __host__ __device__ void mi_cut(const double (*P)[4], int k, Complex<C>& X) {
             }
             X.fa[F] = m;
             X.fb[F] = L;
-            X.fc[F] = NIL;
-            X.fn[F] = (uint8_t)n;
+            X.fc[F] = C::NIL;
+            X.fn[F] = (I)n;
             ++X.nf;
         }
         // cells: survivors in order, then m
@@ -727,7 +737,7 @@ __host__ __device__ void mi_cut(const double (*P)[4], int k, Complex<C>& X) {
                     if (q != c) X.vl[c][n++] = q;
                 X.vl[c][3] = m;
             }
-        uint8_t remap[C::MV];
+        I remap[C::MV];
         bool used[C::MV];
         for (int v = 0; v < X.nv; ++v) used[v] = v < 4;
         for (int f = 0; f < X.nf; ++f)
@@ -735,10 +745,10 @@ __host__ __device__ void mi_cut(const double (*P)[4], int k, Complex<C>& X) {
         int nvn = 0;
         for (int v = 0; v < X.nv; ++v) {
             if (!used[v]) {
-                remap[v] = NIL;
+                remap[v] = C::NIL;
                 continue;
             }
-            remap[v] = (uint8_t)nvn;
+            remap[v] = (I)nvn;
             if (nvn != v)
                 for (int q = 0; q < 4; ++q) X.vl[nvn][q] = X.vl[v][q];
             ++nvn;
@@ -751,6 +761,7 @@ __host__ __device__ void mi_cut(const double (*P)[4], int k, Complex<C>& X) {
 
 template <class C>
 __host__ __device__ int write_record_mi(const Complex<C>& X, int k, uint16_t* out, int cap) {
+    using I = typename C::Idx;
     int p = 8;
     const int L = 4 + k;
     auto put = [&](uint16_t w) {

@@ -12,11 +12,17 @@ using dev::Complex;
 using CMed = dev::Caps<40, 64, 8, 8>;      // up to 4 active functions
 using CBig = dev::Caps<132, 284, 10, 12>;  // up to 8 active functions
 // more than 8 active functions (degenerate inputs: ties, coincident
-// functions): up to 28, with the complex held in device memory
-using CHuge = dev::Caps<255, 255, 32, 32>;
+// functions; many functions on coarse grids) and complexes beyond a bucket's
+// capacities: up to 28 functions, 16-bit ids, the complex in device memory
+using CHuge = dev::Caps<1023, 1500, 32, 32, uint16_t>;
+constexpr int kMaskWords = (CHuge::MV + 63) / 64;  // vertex masks of any record
 constexpr int kMaxHuge = CHuge::MP - 4;
 
 constexpr uint32_t ERR_CAPACITY = 1, ERR_RECORD = 4;
+// a bucket kernel's complex outgrew its capacities: the tet is cut again
+// with the large-capacity kernel (plain form)
+constexpr uint32_t ERR_RETRY = 16;
+constexpr uint32_t CNT_RETRY = 0xFFFFFFFFu;
 
 inline unsigned nblk(uint64_t n, unsigned t = 256) {
     uint64_t g = (n + t - 1) / t;
@@ -187,7 +193,9 @@ __global__ void k_rec_bound(Count n, const uint8_t* __restrict__ act_k, int mi, 
 
 template <class C>
 __device__ inline void ia_counts(const Complex<C>& X, uint32_t& niv, uint32_t& nif, uint32_t& nfv) {
-    uint64_t iso[4] = {0, 0, 0, 0};
+    constexpr int W = (C::MV + 63) / 64;
+    uint64_t iso[W];
+    for (int w = 0; w < W; ++w) iso[w] = 0;
     nif = nfv = 0;
     for (int f = 0; f < X.nf; ++f) {
         bool is = X.fa[f] > 3;
@@ -200,13 +208,16 @@ __device__ inline void ia_counts(const Complex<C>& X, uint32_t& niv, uint32_t& n
         nfv += X.fn[f];
         for (int q = 0; q < X.fn[f]; ++q) iso[X.fv[f][q] >> 6] |= 1ull << (X.fv[f][q] & 63);
     }
-    niv = (uint32_t)(__popcll(iso[0]) + __popcll(iso[1]) + __popcll(iso[2]) + __popcll(iso[3]));
+    niv = 0;
+    for (int w = 0; w < W; ++w) niv += (uint32_t)__popcll(iso[w]);
 }
 
 template <class C>
 __device__ inline void mi_counts(const Complex<C>& X, uint32_t& niv, uint32_t& nif, uint32_t& nfv) {
     // MI faces: positive label > 3; boundary faces are resolved across tets
-    uint64_t iso[4] = {0, 0, 0, 0};
+    constexpr int W = (C::MV + 63) / 64;
+    uint64_t iso[W];
+    for (int w = 0; w < W; ++w) iso[w] = 0;
     nif = nfv = 0;
     for (int f = 0; f < X.nf; ++f) {
         if (X.fa[f] <= 3) continue;
@@ -214,7 +225,8 @@ __device__ inline void mi_counts(const Complex<C>& X, uint32_t& niv, uint32_t& n
         nfv += X.fn[f];
         for (int q = 0; q < X.fn[f]; ++q) iso[X.fv[f][q] >> 6] |= 1ull << (X.fv[f][q] & 63);
     }
-    niv = (uint32_t)(__popcll(iso[0]) + __popcll(iso[1]) + __popcll(iso[2]) + __popcll(iso[3]));
+    niv = 0;
+    for (int w = 0; w < W; ++w) niv += (uint32_t)__popcll(iso[w]);
 }
 
 template <class C, bool MI>
@@ -223,9 +235,10 @@ __global__ void __launch_bounds__(64) k_cut(GridView g, const double* __restrict
                                             const uint8_t* __restrict__ fit, uint64_t n_active, int kmin, int kmax,
                                             uint32_t flags, const uint32_t* __restrict__ rec_off,
                                             uint16_t* __restrict__ rec, uint32_t* __restrict__ cnt,
-                                            uint32_t* __restrict__ errs) {
+                                            uint32_t* __restrict__ errs, const uint32_t* __restrict__ list) {
     const uint64_t V = g.V;
-    GRID_STRIDE(a, n_active) {
+    GRID_STRIDE(i, n_active) {
+        const uint64_t a = list ? list[i] : i;
         const int k = act_k[a];
         if (k < kmin || k > kmax) continue;
         const uint32_t t = act[a];
@@ -243,15 +256,23 @@ __global__ void __launch_bounds__(64) k_cut(GridView g, const double* __restrict
         else
             dev::ia_cut<C>(P, k, X);
         if (X.err) {
-            flag_error(errs, ERR_CAPACITY);
-            cnt[3 * a] = cnt[3 * a + 1] = cnt[3 * a + 2] = 0;
+            // capacities or predicate range: cut again with the large kernel
+            flag_error(errs, ERR_RETRY);
+            cnt[3 * a] = CNT_RETRY;
+            cnt[3 * a + 1] = cnt[3 * a + 2] = 0;
             rec[rec_off[a]] = 0;
             continue;
         }
         const int cap = (int)(rec_off[a + 1] - rec_off[a]);
         const int w = MI ? dev::write_record_mi<C>(X, k, rec + rec_off[a], cap)
                          : dev::write_record_ia<C>(X, k, rec + rec_off[a], cap);
-        if (w < 0) flag_error(errs, ERR_RECORD);
+        if (w < 0) {
+            flag_error(errs, ERR_RETRY);
+            cnt[3 * a] = CNT_RETRY;
+            cnt[3 * a + 1] = cnt[3 * a + 2] = 0;
+            rec[rec_off[a]] = 0;
+            continue;
+        }
         uint32_t a0, a1, a2;
         if (flags & RUN_CUT_ONLY) {
             a0 = X.nv;
@@ -268,7 +289,8 @@ __global__ void __launch_bounds__(64) k_cut(GridView g, const double* __restrict
     }
 }
 
-// active tets with more than 8 functions, one lane each; ws holds one complex
+// active tets with more than 8 functions, and tets a bucket kernel could not
+// hold, one lane each; ws holds one complex
 // per lane of the grid
 template <bool MI>
 __global__ void __launch_bounds__(64) k_cut_huge(GridView g, const double* __restrict__ vals,
@@ -277,7 +299,8 @@ __global__ void __launch_bounds__(64) k_cut_huge(GridView g, const double* __res
                                                  const uint32_t* __restrict__ list, uint64_t n, uint32_t flags,
                                                  const uint32_t* __restrict__ rec_off, uint16_t* __restrict__ rec,
                                                  uint32_t* __restrict__ cnt, uint32_t* __restrict__ errs,
-                                                 Complex<CHuge>* __restrict__ ws) {
+                                                 Complex<CHuge>* __restrict__ ws, const uint32_t* __restrict__ rec_loc,
+                                                 uint32_t fixed_cap) {
     using C = CHuge;
     const uint64_t V = g.V;
     Complex<C>& X = ws[(uint64_t)blockIdx.x * blockDim.x + threadIdx.x];
@@ -298,15 +321,16 @@ __global__ void __launch_bounds__(64) k_cut_huge(GridView g, const double* __res
             dev::mi_cut<C>(P, k, X);
         else
             dev::ia_cut<C>(P, k, X);
+        // the record goes to rec_loc[a] (retries) or to the tet's own slot
+        const uint32_t at = rec_loc ? rec_loc[a] : rec_off[a];
         if (X.err) {
             flag_error(errs, ERR_CAPACITY);
             cnt[3 * a] = cnt[3 * a + 1] = cnt[3 * a + 2] = 0;
-            rec[rec_off[a]] = 0;
+            rec[at] = 0;
             continue;
         }
-        const int cap = (int)(rec_off[a + 1] - rec_off[a]);
-        const int w = MI ? dev::write_record_mi<C>(X, k, rec + rec_off[a], cap)
-                         : dev::write_record_ia<C>(X, k, rec + rec_off[a], cap);
+        const int cap = fixed_cap ? (int)fixed_cap : (int)(rec_off[a + 1] - rec_off[a]);
+        const int w = MI ? dev::write_record_mi<C>(X, k, rec + at, cap) : dev::write_record_ia<C>(X, k, rec + at, cap);
         if (w < 0) flag_error(errs, ERR_RECORD);
         uint32_t a0, a1, a2;
         if (flags & RUN_CUT_ONLY) {
@@ -340,7 +364,7 @@ __device__ inline uint64_t key_edge(uint64_t e, uint32_t f) { return ((e << 6 | 
 __device__ inline uint64_t key_face(uint64_t fc, uint32_t f0, uint32_t f1) {
     return ((((fc << 6) | f0) << 6 | f1) << 2) | 2;
 }
-__device__ inline uint64_t key_inner(uint64_t a, uint32_t j) { return ((a << 8 | j) << 2) | 3; }
+__device__ inline uint64_t key_inner(uint64_t a, uint32_t j) { return ((a << 10 | j) << 2) | 3; }
 
 struct Rec {
     const uint16_t* r;
@@ -363,7 +387,7 @@ __device__ inline bool ia_face_is_iso(const Rec& R, const uint16_t* F) {
 }
 
 __global__ void k_extract_ia(GridView g, const uint32_t* __restrict__ act, const uint32_t* __restrict__ csr_off,
-                             const uint8_t* __restrict__ fit, Count n_active, const uint32_t* __restrict__ rec_off,
+                             const uint8_t* __restrict__ fit, Count n_active, const uint32_t* __restrict__ rec_loc,
                              const uint16_t* __restrict__ rec, const uint32_t* __restrict__ voff,
                              const uint32_t* __restrict__ foff, const uint32_t* __restrict__ fvoff,
                              uint64_t* __restrict__ okey, uint32_t* __restrict__ oa, uint16_t* __restrict__ oj,
@@ -371,12 +395,13 @@ __global__ void k_extract_ia(GridView g, const uint32_t* __restrict__ act, const
                              uint8_t* __restrict__ fq_bnd, uint32_t* __restrict__ fq_vs, uint32_t* __restrict__ fve) {
     GRID_STRIDE(a, n_active) {
         const uint32_t t = act[a];
-        const Rec R(rec + rec_off[a]);
+        const Rec R(rec + rec_loc[a]);
         if (R.r[0] != 1) continue;
         uint32_t v[4];
         dev::tet_corners(g, t, v);
         const uint32_t s0 = csr_off[t];
-        uint64_t iso[4] = {0, 0, 0, 0};
+        uint64_t iso[kMaskWords];
+        for (int w = 0; w < kMaskWords; ++w) iso[w] = 0;
         const uint16_t* F = R.faces();
         for (int f = 0; f < R.nf; ++f) {
             const int n = F[3];

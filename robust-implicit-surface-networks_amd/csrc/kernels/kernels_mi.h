@@ -142,7 +142,7 @@ __device__ inline void mi_face_triple(const GridView& g, const MITet& M, const u
 
 struct MIArgs {
     GridView g;
-    const uint32_t *act, *csr_off, *rank, *rec_off, *nbr;
+    const uint32_t *act, *csr_off, *rank, *rec_loc, *nbr;
     const uint8_t* fit;
     const uint16_t* rec;
     uint64_t n_active;
@@ -161,7 +161,7 @@ __device__ inline bool mi_bface_emits(const MIArgs& A, const MITet& M, uint64_t 
     if (!(r2 < A.n_active && A.act[r2] == t2)) return false;
     uint64_t k3[3];
     mi_face_triple(A.g, M, A.fit, a, F, k3);
-    const MITet N(A.g, A.rec + A.rec_off[r2], t2, A.csr_off[t2]);
+    const MITet N(A.g, A.rec + A.rec_loc[r2], t2, A.csr_off[t2]);
     const uint16_t* G = N.R.faces();
     for (int f = 0; f < N.R.nf; ++f) {
         const int n = G[3];
@@ -184,8 +184,8 @@ __device__ inline bool mi_bface_emits(const MIArgs& A, const MITet& M, uint64_t 
 }
 
 // interface-vertex mask of a tet (vertices of faces with positive label > 3)
-__device__ inline void mi_iface_mask(const Rec& R, uint64_t m[4]) {
-    m[0] = m[1] = m[2] = m[3] = 0;
+__device__ inline void mi_iface_mask(const Rec& R, uint64_t m[kMaskWords]) {
+    for (int w = 0; w < kMaskWords; ++w) m[w] = 0;
     const uint16_t* F = R.faces();
     for (int f = 0; f < R.nf; ++f) {
         const int n = F[3];
@@ -194,7 +194,7 @@ __device__ inline void mi_iface_mask(const Rec& R, uint64_t m[4]) {
         F += 4 + n;
     }
 }
-__device__ inline uint32_t mask_rank(const uint64_t m[4], int j) {
+__device__ inline uint32_t mask_rank(const uint64_t m[kMaskWords], int j) {
     uint32_t r = 0;
     for (int w = 0; w < (j >> 6); ++w) r += __popcll(m[w]);
     return r + __popcll(m[j >> 6] & ((1ull << (j & 63)) - 1));
@@ -215,12 +215,12 @@ __device__ inline int corner_of(const Rec& R, int j) {
 __global__ void k_mi_count(MIArgs A, uint32_t* __restrict__ cnt) {
     GRID_STRIDE(a, A.n_active) {
         const uint32_t t = A.act[a];
-        const MITet M(A.g, A.rec + A.rec_off[a], t, A.csr_off[t]);
+        const MITet M(A.g, A.rec + A.rec_loc[a], t, A.csr_off[t]);
         uint32_t nv = 0, nf = 0, nfv = 0;
         if (M.R.r[0] == 2) {
-            uint64_t m[4];
+            uint64_t m[kMaskWords];
             mi_iface_mask(M.R, m);
-            nv = __popcll(m[0]) + __popcll(m[1]) + __popcll(m[2]) + __popcll(m[3]);
+            for (int w = 0; w < kMaskWords; ++w) nv += __popcll(m[w]);
             const uint16_t* F = M.R.faces();
             for (int f = 0; f < M.R.nf; ++f) {
                 const int n = F[3];
@@ -249,9 +249,9 @@ __global__ void k_mi_extract(MIArgs A, const uint32_t* __restrict__ voff, const 
                              uint32_t* __restrict__ fve) {
     GRID_STRIDE(a, A.n_active) {
         const uint32_t t = A.act[a];
-        const MITet M(A.g, A.rec + A.rec_off[a], t, A.csr_off[t]);
+        const MITet M(A.g, A.rec + A.rec_loc[a], t, A.csr_off[t]);
         if (M.R.r[0] != 2) continue;
-        uint64_t m[4];
+        uint64_t m[kMaskWords];
         mi_iface_mask(M.R, m);
         const uint32_t vb = voff[a];
         uint32_t o = vb;
@@ -297,7 +297,7 @@ __global__ void k_mi_extract(MIArgs A, const uint32_t* __restrict__ voff, const 
 // vertex records and coordinates (src/extract_mesh.cpp:640-815, 1541-1637)
 __global__ void k_vert_fields_mi(GridView g, const double* __restrict__ vals, const uint32_t* __restrict__ act,
                                  const uint32_t* __restrict__ csr_off, const uint8_t* __restrict__ fit,
-                                 const uint32_t* __restrict__ rec_off, const uint16_t* __restrict__ rec,
+                                 const uint32_t* __restrict__ rec_loc, const uint16_t* __restrict__ rec,
                                  const uint32_t* __restrict__ first_occ, const uint32_t* __restrict__ oa,
                                  const uint16_t* __restrict__ oj, Count nverts, uint32_t* __restrict__ v_tet,
                                  uint16_t* __restrict__ v_j, uint8_t* __restrict__ v_ss, uint32_t* __restrict__ v_sv,
@@ -308,7 +308,7 @@ __global__ void k_vert_fields_mi(GridView g, const double* __restrict__ vals, co
         const uint32_t a = oa[o];
         const int j = oj[o];
         const uint32_t t = act[a];
-        const Rec R(rec + rec_off[a]);
+        const Rec R(rec + rec_loc[a]);
         uint32_t v[4];
         dev::tet_corners(g, t, v);
         const uint32_t s0 = csr_off[t];

@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <memory>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -61,7 +62,8 @@ struct RunOut {
     DBuf<uint32_t> act;         // tet of each active rank
     DBuf<uint8_t> act_k;
     DBuf<uint32_t> rank_of_tet; // T (exclusive scan of the active flags)
-    DBuf<uint32_t> rec_off;     // n_active + 1 (words)
+    DBuf<uint32_t> rec_off;     // n_active + 1 (words): each tet's slot and its capacity
+    DBuf<uint32_t> rec_loc;     // n_active: where each tet's record is (its slot, or a retry slot)
     DBuf<uint16_t> rec;
     DBuf<uint32_t> cut_counts;  // n_active x 3: (vertices, faces, cells) for the robustness check
     // mesh
@@ -106,6 +108,13 @@ void grid_explicit(DevMem& mem, const double* pts, uint64_t V, const uint64_t* t
 void values_eval(DevMem& mem, const Grid& g, const FuncDesc* funcs, uint32_t F, Values& v, hipStream_t s);
 void values_upload(DevMem& mem, const Grid& g, const double* rowmajor, uint32_t F, Values& v, hipStream_t s);
 void run_pipeline(DevMem& mem, const Grid& g, const Values& v, uint32_t flags, RunOut& out, hipStream_t s);
+
+// thrown by the batched form when a tet needs more than its buckets hold
+// (more than 8 active functions, or a complex beyond the bucket capacities);
+// the caller runs the plain form, which handles those tets
+struct NeedsPlainForm : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
 
 // batched form: buckets by active-function count, k=1 lookup table in LDS,
 // persistent per-bucket cut kernels, whole path captured in a hipGraph

@@ -122,7 +122,7 @@ void extract_mi(DevMem& mem, const Grid& g, const Values& val, RunOut& out, Time
     A.act = out.act.p;
     A.csr_off = out.csr_off.p;
     A.rank = out.rank_of_tet.p;
-    A.rec_off = out.rec_off.p;
+    A.rec_loc = out.rec_loc.p;
     A.nbr = g.nbr.p;
     A.fit = out.fit.p;
     A.rec = out.rec.p;
@@ -149,7 +149,7 @@ void extract_mi(DevMem& mem, const Grid& g, const Values& val, RunOut& out, Time
     RIN_HIP(hipGetLastError());
     tm.mark("extract occurrences");
     const int kbits = 2 + std::max({bits_for(g.view.V), bits_for(g.view.n_edge_ids) + 12,
-                                    bits_for(g.view.n_face_ids) + 18, bits_for(NA) + 8});
+                                    bits_for(g.view.n_face_ids) + 18, bits_for(NA) + 10});
     Dedup d;
     dedup_vertices(mem, okey, NVO, kbits, d, s);
     tm.mark("dedup verts");
@@ -163,7 +163,7 @@ void extract_mi(DevMem& mem, const Grid& g, const Values& val, RunOut& out, Time
     out.xyz.alloc(mem, 3 * NV + 3);
     hipLaunchKernelGGL(k_vert_fields_mi, dim3(nblk(NV)), dim3(256), 0, s, g.view, (const double*)val.vals.p,
                        (const uint32_t*)out.act.p, (const uint32_t*)out.csr_off.p, (const uint8_t*)out.fit.p,
-                       (const uint32_t*)out.rec_off.p, (const uint16_t*)out.rec.p, (const uint32_t*)d.first_occ.p,
+                       (const uint32_t*)out.rec_loc.p, (const uint16_t*)out.rec.p, (const uint32_t*)d.first_occ.p,
                        (const uint32_t*)oa.p, (const uint16_t*)oj.p, Count(NV), out.v_tet.p, out.v_j.p, out.v_ss.p,
                        out.v_sv.p, out.v_fn.p, out.xyz.p);
     RIN_HIP(hipGetLastError());
@@ -374,7 +374,7 @@ void run_pipeline(DevMem& mem, const Grid& g, const Values& val, uint32_t flags,
     vflag.reset();
 
     // bucket counts (reference labels: IA 1/2/>=3, MI 2/3/>=4)
-    std::vector<uint32_t> huge;  // active ranks with more than 8 functions
+    std::vector<uint32_t> huge, big, med;  // active ranks with > 8, 5..8 and 1..4 functions
     {
         std::vector<uint8_t> hk(NA);
         if (NA) RIN_HIP(hipMemcpyAsync(hk.data(), out.act_k.p, NA, hipMemcpyDeviceToHost, s));
@@ -386,6 +386,8 @@ void run_pipeline(DevMem& mem, const Grid& g, const Values& val, uint32_t flags,
             out.nb[k <= lo ? 0 : (k == lo + 1 ? 1 : 2)]++;
             kmax = std::max<int>(kmax, k);
             if (k > 8) huge.push_back((uint32_t)a);
+            else if (k > 4) big.push_back((uint32_t)a);
+            else med.push_back((uint32_t)a);
             out.n_cut34 += k >= 3 && k <= 4;
             out.n_cut58 += k >= 5 && k <= 8;
         }
@@ -408,61 +410,90 @@ void run_pipeline(DevMem& mem, const Grid& g, const Values& val, uint32_t flags,
     const uint64_t nwords = read1(out.rec_off.p + NA, s);
     out.rec.alloc(mem, nwords + 1);
     DBuf<uint32_t> cnt(mem, 3 * NA + 3);
-    if (NA) {
-        if (!MI) {
-            hipLaunchKernelGGL((k_cut<CMed, false>), dim3(nblk(NA, 64)), dim3(64), 0, s, g.view,
-                               (const double*)val.vals.p, (const uint32_t*)out.act.p, (const uint8_t*)out.act_k.p,
-                               (const uint32_t*)out.csr_off.p, (const uint8_t*)out.fit.p, NA, 1, 4, flags,
-                               (const uint32_t*)out.rec_off.p, out.rec.p, cnt.p, errs.p);
-            hipLaunchKernelGGL((k_cut<CBig, false>), dim3(nblk(NA, 64)), dim3(64), 0, s, g.view,
-                               (const double*)val.vals.p, (const uint32_t*)out.act.p, (const uint8_t*)out.act_k.p,
-                               (const uint32_t*)out.csr_off.p, (const uint8_t*)out.fit.p, NA, 5, 8, flags,
-                               (const uint32_t*)out.rec_off.p, out.rec.p, cnt.p, errs.p);
-        } else {
-            hipLaunchKernelGGL((k_cut<CMed, true>), dim3(nblk(NA, 64)), dim3(64), 0, s, g.view,
-                               (const double*)val.vals.p, (const uint32_t*)out.act.p, (const uint8_t*)out.act_k.p,
-                               (const uint32_t*)out.csr_off.p, (const uint8_t*)out.fit.p, NA, 1, 4, flags,
-                               (const uint32_t*)out.rec_off.p, out.rec.p, cnt.p, errs.p);
-            hipLaunchKernelGGL((k_cut<CBig, true>), dim3(nblk(NA, 64)), dim3(64), 0, s, g.view,
-                               (const double*)val.vals.p, (const uint32_t*)out.act.p, (const uint8_t*)out.act_k.p,
-                               (const uint32_t*)out.csr_off.p, (const uint8_t*)out.fit.p, NA, 5, 8, flags,
-                               (const uint32_t*)out.rec_off.p, out.rec.p, cnt.p, errs.p);
-        }
+    // each thread-per-tet cut kernel runs over its own k range only
+    for (int q = 0; q < 2 && NA; ++q) {
+        const std::vector<uint32_t>& L = q == 0 ? med : big;
+        if (L.empty()) continue;
+        DBuf<uint32_t> dl(mem, L.size());
+        RIN_HIP(hipMemcpyAsync(dl.p, L.data(), 4 * L.size(), hipMemcpyHostToDevice, s));
+        const unsigned blocks = (unsigned)((L.size() + 63) / 64);
+        const int kmin = q == 0 ? 1 : 5, kmax = q == 0 ? 4 : 8;
+        auto go = [&](auto kernel) {
+            hipLaunchKernelGGL(kernel, dim3(blocks), dim3(64), 0, s, g.view, (const double*)val.vals.p,
+                               (const uint32_t*)out.act.p, (const uint8_t*)out.act_k.p,
+                               (const uint32_t*)out.csr_off.p, (const uint8_t*)out.fit.p, (uint64_t)L.size(), kmin,
+                               kmax, flags, (const uint32_t*)out.rec_off.p, out.rec.p, cnt.p, errs.p,
+                               (const uint32_t*)dl.p);
+        };
+        if (q == 0)
+            MI ? go(k_cut<CMed, true>) : go(k_cut<CMed, false>);
+        else
+            MI ? go(k_cut<CBig, true>) : go(k_cut<CBig, false>);
         RIN_HIP(hipGetLastError());
+        RIN_HIP(hipStreamSynchronize(s));  // the list buffer returns to the pool
     }
-    if (!huge.empty()) {
-        // rare degenerate tets: one lane each, complexes in device memory
-        DBuf<uint32_t> hl(mem, huge.size());
-        RIN_HIP(hipMemcpyAsync(hl.p, huge.data(), 4 * huge.size(), hipMemcpyHostToDevice, s));
-        const unsigned blocks = (unsigned)std::min<size_t>((huge.size() + 63) / 64, 64);
+    // rare tets: one lane each, complexes in device memory
+    auto cut_large = [&](const std::vector<uint32_t>& list, const uint32_t* loc, uint32_t fixed_cap) {
+        DBuf<uint32_t> hl(mem, list.size());
+        RIN_HIP(hipMemcpyAsync(hl.p, list.data(), 4 * list.size(), hipMemcpyHostToDevice, s));
+        // ~200 KB of complex per lane: a few blocks
+        const unsigned blocks = (unsigned)std::min<size_t>((list.size() + 63) / 64, 4);
         DBuf<uint8_t> ws(mem, sizeof(Complex<CHuge>) * 64 * blocks);
         auto* wsp = reinterpret_cast<Complex<CHuge>*>(ws.p);
-        if (MI)
-            hipLaunchKernelGGL(k_cut_huge<true>, dim3(blocks), dim3(64), 0, s, g.view, (const double*)val.vals.p,
+        auto go = [&](auto kernel) {
+            hipLaunchKernelGGL(kernel, dim3(blocks), dim3(64), 0, s, g.view, (const double*)val.vals.p,
                                (const uint32_t*)out.act.p, (const uint8_t*)out.act_k.p, (const uint32_t*)out.csr_off.p,
-                               (const uint8_t*)out.fit.p, (const uint32_t*)hl.p, (uint64_t)huge.size(), flags,
-                               (const uint32_t*)out.rec_off.p, out.rec.p, cnt.p, errs.p, wsp);
-        else
-            hipLaunchKernelGGL(k_cut_huge<false>, dim3(blocks), dim3(64), 0, s, g.view, (const double*)val.vals.p,
-                               (const uint32_t*)out.act.p, (const uint8_t*)out.act_k.p, (const uint32_t*)out.csr_off.p,
-                               (const uint8_t*)out.fit.p, (const uint32_t*)hl.p, (uint64_t)huge.size(), flags,
-                               (const uint32_t*)out.rec_off.p, out.rec.p, cnt.p, errs.p, wsp);
+                               (const uint8_t*)out.fit.p, (const uint32_t*)hl.p, (uint64_t)list.size(), flags,
+                               (const uint32_t*)out.rec_off.p, out.rec.p, cnt.p, errs.p, wsp, loc, fixed_cap);
+        };
+        MI ? go(k_cut_huge<true>) : go(k_cut_huge<false>);
         RIN_HIP(hipGetLastError());
         RIN_HIP(hipStreamSynchronize(s));
-    }
-    tm.mark(MI ? "MI cut" : "simp_arr");
-    {
-        uint32_t he[64];
+    };
+    if (!huge.empty()) cut_large(huge, nullptr, 0);
+    // record location of every active tet: its slot, or (after a retry) a slot
+    // appended for a large complex
+    out.rec_loc.alloc(mem, NA + 1);
+    if (NA) copy_u32(out.rec_loc.p, out.rec_off.p, Count(NA), s);
+    auto read_errs = [&] {
+        uint32_t he[64], e = 0;
         RIN_HIP(hipMemcpyAsync(he, errs.p, sizeof(he), hipMemcpyDeviceToHost, s));
         RIN_HIP(hipStreamSynchronize(s));
-        for (uint32_t e : he) out.err |= e;
-        if (out.err) {
-            tm.finish(out.times);
-            throw std::runtime_error(std::string("device cut failed: ") +
-                                     ((out.err & ERR_CAPACITY) ? "complex exceeds capacity or predicate range; "
-                                                               : "") +
-                                     ((out.err & ERR_RECORD) ? "record overflow" : ""));
-        }
+        for (uint32_t x : he) e |= x;
+        return e;
+    };
+    uint32_t e = read_errs();
+    if (e == ERR_RETRY) {
+        // complexes that outgrew a bucket kernel: cut again with the
+        // large-capacity kernel into slots appended to the record buffer
+        std::vector<uint32_t> hc(3 * NA);
+        RIN_HIP(hipMemcpyAsync(hc.data(), cnt.p, 4 * hc.size(), hipMemcpyDeviceToHost, s));
+        std::vector<uint32_t> hl(NA);
+        RIN_HIP(hipMemcpyAsync(hl.data(), out.rec_off.p, 4 * NA, hipMemcpyDeviceToHost, s));
+        RIN_HIP(hipStreamSynchronize(s));
+        std::vector<uint32_t> retry;
+        for (uint64_t a = 0; a < NA; ++a)
+            if (hc[3 * a] == CNT_RETRY) retry.push_back((uint32_t)a);
+        constexpr uint32_t slot = 8 + 4 * CHuge::MV + CHuge::MF * (4 + CHuge::MFV) + 2 * CHuge::MP;
+        const uint64_t total = nwords + (uint64_t)slot * retry.size();
+        if (total >= (1ull << 31)) throw std::runtime_error("record buffer exceeds 2^31 words");
+        DBuf<uint16_t> rec2(mem, total + 1);
+        RIN_HIP(hipMemcpyAsync(rec2.p, out.rec.p, 2 * nwords, hipMemcpyDeviceToDevice, s));
+        for (size_t i = 0; i < retry.size(); ++i) hl[retry[i]] = (uint32_t)(nwords + slot * i);
+        RIN_HIP(hipMemcpyAsync(out.rec_loc.p, hl.data(), 4 * NA, hipMemcpyHostToDevice, s));
+        RIN_HIP(hipStreamSynchronize(s));
+        out.rec = std::move(rec2);
+        fill_u32(errs.p, 0, 64, s);
+        cut_large(retry, out.rec_loc.p, slot);
+        e = read_errs();
+    }
+    tm.mark(MI ? "MI cut" : "simp_arr");
+    out.err |= e;
+    if (out.err) {
+        tm.finish(out.times);
+        throw std::runtime_error(std::string("device cut failed: ") +
+                                 ((out.err & ERR_CAPACITY) ? "complex exceeds capacity or predicate range; " : "") +
+                                 ((out.err & ERR_RECORD) ? "record overflow" : ""));
     }
     if (flags & RUN_CUT_ONLY) {
         out.cut_counts = std::move(cnt);
@@ -497,7 +528,7 @@ void run_pipeline(DevMem& mem, const Grid& g, const Values& val, uint32_t flags,
     if (NA)
         hipLaunchKernelGGL(k_extract_ia, dim3(nblk(NA)), dim3(256), 0, s, g.view, (const uint32_t*)out.act.p,
                            (const uint32_t*)out.csr_off.p, (const uint8_t*)out.fit.p, NA,
-                           (const uint32_t*)out.rec_off.p, (const uint16_t*)out.rec.p, (const uint32_t*)cv.p,
+                           (const uint32_t*)out.rec_loc.p, (const uint16_t*)out.rec.p, (const uint32_t*)cv.p,
                            (const uint32_t*)cf.p, (const uint32_t*)cfv.p, okey.p, oa.p, oj.p, fq_a.p, fq_l.p,
                            fq_fn.p, fq_bnd.p, fq_vs.p, fve.p);
     RIN_HIP(hipGetLastError());
@@ -505,7 +536,7 @@ void run_pipeline(DevMem& mem, const Grid& g, const Values& val, uint32_t flags,
 
     // ---- vertex dedup: stable sort by key, first occurrence ranks
     const int kbits = 2 + std::max({bits_for(V), bits_for(g.view.n_edge_ids) + 6, bits_for(g.view.n_face_ids) + 12,
-                                    bits_for(NA) + 8});
+                                    bits_for(NA) + 10});
     Dedup dd;
     dedup_vertices(mem, okey, NVO, kbits, dd, s);
     DBuf<uint32_t>& vid = dd.vid;
@@ -521,7 +552,7 @@ void run_pipeline(DevMem& mem, const Grid& g, const Values& val, uint32_t flags,
     out.xyz.alloc(mem, 3 * NV + 3);
     hipLaunchKernelGGL(k_vert_fields_ia, dim3(nblk(NV)), dim3(256), 0, s, g.view, (const double*)val.vals.p,
                        (const uint32_t*)out.act.p, (const uint32_t*)out.csr_off.p, (const uint8_t*)out.fit.p,
-                       (const uint32_t*)out.rec_off.p, (const uint16_t*)out.rec.p, (const uint16_t*)nullptr,
+                       (const uint32_t*)out.rec_loc.p, (const uint16_t*)out.rec.p, (const uint16_t*)nullptr,
                        (const uint32_t*)nullptr, (const uint32_t*)first_occ.p,
                        (const uint32_t*)oa.p, (const uint16_t*)oj.p, NV, out.v_tet.p, out.v_j.p, out.v_ss.p,
                        out.v_sv.p, out.v_fn.p, out.xyz.p);
@@ -615,7 +646,7 @@ void RunOut::set_views_from_buffers() {
     view.rank_of_tet = rank_of_tet.p;
     view.rec_off = rec_off.p;
     view.rec = rec.p;
-    view.rec_loc = rec_off.p;
+    view.rec_loc = rec_loc.p ? rec_loc.p : rec_off.p;
     view.k1_rec = nullptr;
     view.k1_rec_off = nullptr;
     view.v_tet = v_tet.p;

@@ -99,6 +99,16 @@ int rin_create(rin_ctx** out, int32_t device) {
     return RIN_OK;
 }
 
+int rin_device_memory(rin_ctx* c, uint64_t* free_bytes, uint64_t* total_bytes) {
+    if (!free_bytes || !total_bytes) return fail(c, RIN_ERR_ARG, "null argument");
+    return guarded(c, [&] {
+        size_t f = 0, t = 0;
+        RIN_HIP(hipMemGetInfo(&f, &t));
+        *free_bytes = f;
+        *total_bytes = t;
+    });
+}
+
 int rin_destroy(rin_ctx* c) {
     if (!c) return RIN_ERR_ARG;
     (void)hipSetDevice(c->device);
@@ -186,10 +196,19 @@ int rin_run(rin_ctx* c, const rin_grid* g, const rin_values* v, uint32_t flags, 
     *out = nullptr;
     return guarded(c, [&] {
         auto r = std::make_unique<rin_result>();
-        if (flags & RIN_FLAG_BATCHED)
-            rin::run_batched(c->mem, g->g, const_cast<rin_values*>(v)->v, flags, r->r, c->stream, c->n_cu);
-        else
+        if (flags & RIN_FLAG_BATCHED) {
+            try {
+                rin::run_batched(c->mem, g->g, const_cast<rin_values*>(v)->v, flags, r->r, c->stream, c->n_cu);
+            } catch (rin::NeedsPlainForm& e) {
+                // the plain form handles every tet the buckets cannot
+                RIN_HIP(hipStreamSynchronize(c->stream));
+                r = std::make_unique<rin_result>();
+                rin::run_pipeline(c->mem, g->g, v->v, flags & ~(uint32_t)RIN_FLAG_BATCHED, r->r, c->stream);
+                r->r.times.ms.insert(r->r.times.ms.begin(), {std::string("batched form fell back: ") + e.what(), 0.f});
+            }
+        } else {
             rin::run_pipeline(c->mem, g->g, v->v, flags, r->r, c->stream);
+        }
         RIN_HIP(hipStreamSynchronize(c->stream));
         *out = r.release();
     });

@@ -268,3 +268,113 @@ class Result:
         if self.h:
             lib().rin_result_release(self.ctx.h, self.h)
             self.h = None
+
+
+# ---------------------------------------------------------------- serving loop (include/rin/serve.h)
+
+SERVE_LIB_PATH = os.path.join(HERE, "librin_serve.so")
+_serve = None
+
+
+class JobDesc(ctypes.Structure):
+    _fields_ = [("mode", ctypes.c_int32), ("res", ctypes.c_int32), ("n_funcs", ctypes.c_uint32),
+                ("flags", ctypes.c_uint32), ("funcs", ctypes.POINTER(FuncDesc))]
+
+
+class JobInfo(ctypes.Structure):
+    _fields_ = [("id", ctypes.c_uint64), ("status", ctypes.c_int32), ("slot", ctypes.c_int32),
+                ("t_submit", ctypes.c_double), ("t_start", ctypes.c_double), ("t_end", ctypes.c_double),
+                ("hbm_estimate", ctypes.c_uint64), ("sizes", Sizes)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_uint64), ("completed", ctypes.c_uint64), ("failed", ctypes.c_uint64),
+                ("max_in_flight", ctypes.c_uint64), ("tets", ctypes.c_uint64), ("first_submit", ctypes.c_double),
+                ("last_end", ctypes.c_double), ("p50_latency", ctypes.c_double), ("p99_latency", ctypes.c_double),
+                ("max_latency", ctypes.c_double), ("hbm_budget", ctypes.c_uint64)]
+
+
+def serve_lib():
+    global _serve
+    if _serve is None:
+        if not os.path.exists(SERVE_LIB_PATH):
+            raise RuntimeError("librin_serve.so is not built (make -C %s)" % HERE)
+        lib()
+        L = ctypes.CDLL(SERVE_LIB_PATH)
+        L.rin_server_create.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_uint64, ctypes.POINTER(_vp)]
+        L.rin_server_destroy.argtypes = [_vp]
+        L.rin_server_last_error.argtypes = [_vp]
+        L.rin_server_last_error.restype = ctypes.c_char_p
+        L.rin_server_submit.argtypes = [_vp, ctypes.POINTER(JobDesc), _u64p]
+        L.rin_server_wait.argtypes = [_vp, ctypes.c_uint64, ctypes.c_double, ctypes.POINTER(JobInfo)]
+        L.rin_server_job_verts.argtypes = [_vp, ctypes.c_uint64, _u64p, _f64p]
+        L.rin_server_job_faces.argtypes = [_vp, ctypes.c_uint64, _u64p, _u64p, _u64p, _u64p, _u64p]
+        L.rin_server_job_release.argtypes = [_vp, ctypes.c_uint64]
+        L.rin_server_stats_get.argtypes = [_vp, ctypes.POINTER(ServerStats)]
+        L.rin_server_now.argtypes = [_vp]
+        L.rin_server_now.restype = ctypes.c_double
+        _serve = L
+    return _serve
+
+
+class Server:
+    """Continuous-batching server on one GPU: n_slots concurrent jobs, FIFO
+    admission by free slot and HBM budget (bytes; 0 = 80% of free memory)."""
+
+    def __init__(self, device=0, slots=4, hbm_budget=0):
+        self.h = _vp()
+        rc = serve_lib().rin_server_create(device, slots, int(hbm_budget), ctypes.byref(self.h))
+        if rc != 0:
+            raise RinError(rc, "rin_server_create failed (no GPU?)")
+
+    def check(self, rc):
+        if rc != 0:
+            raise RinError(rc, serve_lib().rin_server_last_error(self.h).decode())
+
+    def now(self):
+        return serve_lib().rin_server_now(self.h)
+
+    def submit(self, mi, res, types, params, flags=0):
+        arr = descs(types, params)
+        d = JobDesc(1 if mi else 0, int(res), len(types), int(flags), arr)
+        jid = ctypes.c_uint64(0)
+        self.check(serve_lib().rin_server_submit(self.h, ctypes.byref(d), ctypes.byref(jid)))
+        return jid.value
+
+    def wait(self, jid, timeout=120.0):
+        info = JobInfo()
+        self.check(serve_lib().rin_server_wait(self.h, jid, float(timeout), ctypes.byref(info)))
+        return info
+
+    def mesh(self, jid, info):
+        s = info.sizes
+        rec = np.zeros((s.n_verts, 11), dtype=np.uint64)
+        xyz = np.zeros((s.n_verts, 3), dtype=np.float64)
+        self.check(serve_lib().rin_server_job_verts(self.h, jid, _p(rec, _u64p), _p(xyz, _f64p)))
+        vs = np.zeros(s.n_faces + 1, dtype=np.uint64)
+        v = np.zeros(s.n_face_verts, dtype=np.uint64)
+        ts = np.zeros(s.n_faces + 1, dtype=np.uint64)
+        tf = np.zeros((s.n_tet_faces, 2), dtype=np.uint64)
+        fn = np.zeros((s.n_faces, 2), dtype=np.uint64)
+        self.check(serve_lib().rin_server_job_faces(self.h, jid, _p(vs, _u64p), _p(v, _u64p), _p(ts, _u64p),
+                                                    _p(tf, _u64p), _p(fn, _u64p)))
+        return rec, xyz, vs, v, ts, tf, fn
+
+    def release(self, jid):
+        self.check(serve_lib().rin_server_job_release(self.h, jid))
+
+    def stats(self):
+        st = ServerStats()
+        self.check(serve_lib().rin_server_stats_get(self.h, ctypes.byref(st)))
+        return st
+
+    def close(self):
+        if self.h:
+            serve_lib().rin_server_destroy(self.h)
+            self.h = _vp()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

@@ -36,6 +36,15 @@ def test_dropin_exports_reference_entry_points():
 
 def test_no_cpu_path_in_the_product():
     """The product libraries never link the oracle."""
-    for so in ("librin.so", "librin_host.so", "librin_dropin.so"):
+    for so in ("librin.so", "librin_host.so", "librin_dropin.so", "librin_serve.so"):
         deps = subprocess.run(["ldd", os.path.join(PKG, so)], capture_output=True, text=True).stdout
         assert "oracle" not in deps, (so, deps)
+
+
+def test_serving_library_exports_declared_symbols():
+    text = open(os.path.join(S.ROOT, "include", "rin", "serve.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = sorted(set(re.findall(r"\b(rin_server_[a-z_0-9]+)\s*\(", text)))
+    assert len(names) >= 9
+    lib = ctypes.CDLL(os.path.join(PKG, "librin_serve.so"))
+    assert not [n for n in names if not hasattr(lib, n)]

@@ -174,3 +174,33 @@ def test_batched_matches_oracle_and_replays(ctx, which, form, monkeypatch):
     b.release()
     vals.release()
     g.release()
+
+
+@pytest.mark.parametrize("mi", [0, 1])
+@pytest.mark.parametrize("batched", [False, True])
+def test_large_complexes_are_cut_again(ctx, mi, batched):
+    """Many functions crossing inside a few large tets: complexes outgrow the
+    bucket kernels' capacities, are cut again with the large-capacity kernel
+    (the batched form hands the run to the plain form) and still match the
+    oracle bit for bit."""
+    import rin
+
+    rng = np.random.default_rng(31 + mi)
+    fs = W.FuncSet()
+    for _ in range(8):
+        if mi:
+            fs.sphere(rng.uniform(-0.3, 0.3, 3), rng.uniform(0.0, 0.1))
+        else:
+            fs.plane(rng.uniform(-0.2, 0.2, 3), rng.standard_normal(3))
+    types, params = fs.arrays()
+    res = 2
+    g = ctx.grid_regular(res)
+    vals = g.eval(types, params)
+    flags = (rin.FLAG_MI if mi else 0) | (rin.FLAG_BATCHED if batched and not mi else 0)
+    dev = vals.run(flags)
+    pts, tets = S.grid(res)
+    ref = S.OracleRun(mi, pts, tets, S.evaluate(types, params, pts))
+    _compare(dev, ref)
+    dev.release()
+    vals.release()
+    g.release()
