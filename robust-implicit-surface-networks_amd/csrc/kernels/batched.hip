@@ -37,7 +37,7 @@ constexpr uint32_t ERR_OVERFLOW = 8;
 // counters in the workspace (device)
 enum Ctr : int {
     C_ACTIVE = 0, C_FIT, C_VO, C_FO, C_FV, C_VERTS, C_FACES, C_FACEVERTS, C_B1, C_B2, C_B3, C_B5, C_BX, C_DEGEN,
-    C_Q2, C_Q3, C_Q5, C_ABORT, C_TETFACES, C_RECW, C_N
+    C_Q2, C_Q3, C_Q5, C_ABORT, C_TETFACES, C_RECW, C_DFAIL, C_N
 };
 
 struct K1Entry {
@@ -280,6 +280,7 @@ struct FillArgs {
     const uint64_t *pos, *neg;
     uint64_t full;
     const uint32_t *csr_off, *rank;
+    uint32_t *csr_off_w, *rank_w;  // written by the cube form
     uint8_t* fit;
     uint32_t *act, *rec_loc, *cnt;
     uint8_t* act_k;
@@ -362,6 +363,229 @@ __global__ void k_fill_batched(FillArgs A) {
             }
         }
     }
+}
+
+// per vertex: sign masks; the count of exactly-zero values goes to ctr
+__global__ void k_signs_count(const double* __restrict__ vals, uint64_t V, uint32_t F, uint64_t* __restrict__ pos,
+                              uint64_t* __restrict__ neg, uint32_t* __restrict__ ctr) {
+    uint32_t z = 0;
+    GRID_STRIDE(v, V) {
+        uint64_t p = 0, n = 0;
+        for (uint32_t f = 0; f < F; ++f) {
+            const double x = vals[(uint64_t)f * V + v];
+            if (x > 0)
+                p |= 1ull << f;
+            else if (x < 0)
+                n |= 1ull << f;
+            else
+                ++z;
+        }
+        pos[v] = p;
+        neg[v] = n;
+    }
+    for (int d = 32; d > 0; d >>= 1) z += __shfl_xor(z, d, 64);
+    if ((threadIdx.x & 63) == 0 && z) atomicAdd(&ctr[C_DEGEN], z);
+}
+
+// ---- regular grids: one cube (five tets) per thread
+//
+// The count pass and the fill pass each load a cube's eight corner sign
+// masks once for its five tets and need no per-tet intermediate arrays: the
+// count pass leaves one (active functions, active tets) pair per tile of
+// kCubeTile cubes, and each fill block sums the pairs of the tiles before
+// its own to place its tets in the CSR and the active list.
+
+constexpr int kCubeTile = 256;
+
+__device__ inline void cube_masks(const GridView& g, uint32_t cube, const uint64_t* __restrict__ pos,
+                                  const uint64_t* __restrict__ neg, uint64_t full, uint32_t c[8], uint64_t m[5],
+                                  bool& even) {
+    const uint32_t R = (uint32_t)g.res, N = R + 1;
+    const uint32_t i = cube / (R * R), rem = cube - i * (R * R), j = rem / R, k = rem - j * R;
+    const uint32_t b = i * N * N + j * N + k;
+    c[0] = b;
+    c[1] = b + N * N;
+    c[2] = b + N * N + N;
+    c[3] = b + N;
+    for (int a = 0; a < 4; ++a) c[4 + a] = c[a] + 1;
+    even = ((i + j + k) & 1u) == 0;
+    uint64_t P[8], Q[8];
+    for (int a = 0; a < 8; ++a) {
+        P[a] = pos[c[a]];
+        Q[a] = neg[c[a]];
+    }
+    for (int q = 0; q < 5; ++q) {
+        const uint8_t* t = even ? dev::kCubeTetEven[q] : dev::kCubeTetOdd[q];
+        const uint64_t ap = P[t[0]] & P[t[1]] & P[t[2]] & P[t[3]];
+        const uint64_t an = Q[t[0]] & Q[t[1]] & Q[t[2]] & Q[t[3]];
+        m[q] = full & ~(ap | an);
+    }
+}
+
+// per tile: active-function and active-tet sums; bucket histogram and the
+// totals by atomics
+__global__ void __launch_bounds__(kCubeTile) k_cube_count(GridView g, const uint64_t* __restrict__ pos,
+                                                          const uint64_t* __restrict__ neg, uint64_t full,
+                                                          uint2* __restrict__ tile_sums, uint32_t* __restrict__ ctr) {
+    const uint32_t ncubes = (uint32_t)(g.T / 5);
+    const uint32_t cube = blockIdx.x * kCubeTile + threadIdx.x;
+    uint32_t ks = 0, as = 0, hb[5] = {0, 0, 0, 0, 0};
+    if (cube < ncubes) {
+        uint32_t c[8];
+        uint64_t m[5];
+        bool even;
+        cube_masks(g, cube, pos, neg, full, c, m, even);
+        for (int q = 0; q < 5; ++q) {
+            const int k = __popcll(m[q]);
+            ks += (uint32_t)k;
+            as += k != 0;
+            if (k) hb[bucket_of(k)]++;
+        }
+    }
+    __shared__ uint32_t red[kCubeTile / 64][7];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t v[7] = {ks, as, hb[0], hb[1], hb[2], hb[3], hb[4]};
+    for (int x = 0; x < 7; ++x) {
+        for (int d = 32; d > 0; d >>= 1) v[x] += __shfl_xor(v[x], d, 64);
+        if (lane == 0) red[wave][x] = v[x];
+    }
+    __syncthreads();
+    if (threadIdx.x < 7) {
+        uint32_t t = 0;
+        for (int w = 0; w < kCubeTile / 64; ++w) t += red[w][threadIdx.x];
+        const int x = threadIdx.x;
+        if (x == 0) tile_sums[blockIdx.x].x = t;
+        if (x == 1) tile_sums[blockIdx.x].y = t;
+        if (t) atomicAdd(&ctr[x == 0 ? C_FIT : (x == 1 ? C_ACTIVE : C_B1 + (x - 2))], t);
+    }
+}
+
+__global__ void __launch_bounds__(kCubeTile) k_cube_fill(FillArgs A, const uint2* __restrict__ tile_sums) {
+    __shared__ K1Entry tab[81];
+    __shared__ uint32_t red[kCubeTile / 64][2];
+    __shared__ uint32_t wsum[kCubeTile / 64];
+    __shared__ uint32_t s_off[5 * kCubeTile], s_rank[5 * kCubeTile];  // per-tet CSR entries of the tile
+    for (int i = threadIdx.x; i < 81; i += blockDim.x) tab[i] = A.k1[i];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // prefix of this tile: the sums of the tiles before it
+    uint32_t pk = 0, pa = 0;
+    for (uint32_t i = threadIdx.x; i < blockIdx.x; i += kCubeTile) {
+        const uint2 t = tile_sums[i];
+        pk += t.x;
+        pa += t.y;
+    }
+    for (int d = 32; d > 0; d >>= 1) {
+        pk += __shfl_xor(pk, d, 64);
+        pa += __shfl_xor(pa, d, 64);
+    }
+    if (lane == 0) {
+        red[wave][0] = pk;
+        red[wave][1] = pa;
+    }
+    const uint32_t ncubes = (uint32_t)(A.g.T / 5);
+    const uint32_t cube = blockIdx.x * kCubeTile + threadIdx.x;
+    uint32_t c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    uint64_t m[5] = {0, 0, 0, 0, 0};
+    bool even = true;
+    if (cube < ncubes) cube_masks(A.g, cube, A.pos, A.neg, A.full, c, m, even);
+    // per lane: active functions, active tets, and tets per bucket list
+    // (k = 2, 3..4, 5..8), packed ks << 11 | as and 10 bits per bucket
+    uint32_t ks = 0, as = 0, bp = 0;
+    for (int q = 0; q < 5; ++q) {
+        const int k = __popcll(m[q]);
+        ks += (uint32_t)k;
+        as += k != 0;
+        if (k >= 2 && k <= 8) bp += 1u << (10 * (k == 2 ? 0 : (k <= 4 ? 1 : 2)));
+    }
+    // bucket slots: one atomic per list per wave
+    uint32_t binc = bp;
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t o = __shfl_up(binc, d, 64);
+        if (lane >= d) binc += o;
+    }
+    const uint32_t btot = __shfl(binc, 63, 64);
+    uint32_t bstart = 0;
+    if (lane < 3) {
+        const uint32_t n = (btot >> (10 * lane)) & 1023u;
+        if (n) bstart = atomicAdd(&A.ctr[C_Q2 + lane], n);
+    }
+    uint32_t bnext[3];
+    for (int qq = 0; qq < 3; ++qq)
+        bnext[qq] = __shfl(bstart, qq, 64) + (((binc - bp) >> (10 * qq)) & 1023u);
+    // block exclusive scan of (ks, as) packed as ks << 11 | as (a tile holds
+    // at most 1280 active tets and 81920 active functions)
+    const uint32_t packed = ks << 11 | as;
+    uint32_t inc = packed;
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t o = __shfl_up(inc, d, 64);
+        if (lane >= d) inc += o;
+    }
+    if (lane == 63) wsum[wave] = inc;
+    __syncthreads();
+    uint32_t ex = inc - packed;
+    for (int w = 0; w < wave; ++w) ex += wsum[w];
+    uint32_t o = 0, r = 0;
+    for (int w = 0; w < kCubeTile / 64; ++w) {
+        o += red[w][0];
+        r += red[w][1];
+    }
+    o += ex >> 11;
+    r += ex & 2047u;
+    for (int q = 0; q < 5; ++q) {
+        const uint64_t t = (uint64_t)cube * 5 + q;
+        const uint64_t mq = m[q];
+        const int k = __popcll(mq);
+        s_off[5 * threadIdx.x + q] = o;
+        s_rank[5 * threadIdx.x + q] = r;
+        if (k) {
+            if (r >= A.cap_active || o + (uint32_t)k > A.cap_fit) {
+                A.errs[lane] |= ERR_OVERFLOW;
+                A.ctr[C_ABORT] = 1;
+            } else {
+                A.act[r] = (uint32_t)t;
+                A.act_k[r] = (uint8_t)k;
+                uint32_t oo = o;
+                for (uint64_t mm = mq; mm; mm &= mm - 1) A.fit[oo++] = (uint8_t)(__ffsll((unsigned long long)mm) - 1);
+                if (k == 1) {
+                    const uint32_t f = __ffsll((unsigned long long)mq) - 1;
+                    const uint8_t* tv = even ? dev::kCubeTetEven[q] : dev::kCubeTetOdd[q];
+                    int pat = 0, w = 1;
+                    for (int a = 0; a < 4; ++a, w *= 3) {
+                        const uint32_t v = c[tv[a]];
+                        const int sg = ((A.pos[v] >> f) & 1) ? 2 : (((A.neg[v] >> f) & 1) ? 0 : 1);
+                        pat += sg * w;
+                    }
+                    const K1Entry& E = tab[pat];
+                    A.cnt[3 * r] = E.niv;
+                    A.cnt[3 * r + 1] = E.nif;
+                    A.cnt[3 * r + 2] = E.nfv;
+                    A.rec_loc[r] = K1_FLAG | (uint32_t)pat;
+                } else if (k <= 8) {
+                    const int qq = k == 2 ? 0 : (k <= 4 ? 1 : 2);
+                    const uint32_t slot = bnext[qq]++;
+                    if (slot < A.cap[qq]) {
+                        A.list[qq][slot] = r;
+                        A.rec_loc[r] = A.rec_base[qq] + slot * A.slot[qq];
+                    } else {
+                        A.errs[lane] |= ERR_OVERFLOW;
+                        A.ctr[C_ABORT] = 1;
+                    }
+                } else {
+                    A.errs[lane] |= 1;  // more active functions than the device cut supports
+                }
+            }
+        }
+        o += (uint32_t)k;
+        r += k != 0;
+    }
+    // the tile's per-tet CSR entries, coalesced
+    __syncthreads();
+    const uint64_t t0 = (uint64_t)blockIdx.x * (5 * kCubeTile), T = A.g.T;
+    for (int i = threadIdx.x; i < 5 * kCubeTile; i += kCubeTile)
+        if (t0 + i < T) {
+            A.csr_off_w[t0 + i] = s_off[i];
+            A.rank_w[t0 + i] = s_rank[i];
+        }
 }
 
 struct CutArgs {
@@ -693,11 +917,6 @@ __global__ void k_cap_check(const uint32_t* ctr, uint32_t cap_vo, uint32_t cap_f
     if (lane == 0 && (ctr[C_VO] > cap_vo || ctr[C_FO] > cap_fo || ctr[C_FV] > cap_fv)) errs[lane] |= ERR_OVERFLOW;
 }
 
-int bitsf(uint64_t x) {
-    int b = 0;
-    while (b < 64 && (x >> b)) ++b;
-    return b;
-}
 
 }  // namespace
 
@@ -720,7 +939,8 @@ struct Workspace {
     // (header, vertices, faces with their cycles, coincidence groups)
     uint32_t slot[3] = {rec_words<CK2>(), rec_words<CMed>(), rec_words<CBig>()};
     DBuf<uint64_t> pos, neg;
-    DBuf<uint32_t> zeros, kcnt, isact, csr_off, rank, act, rec_loc, cnt, list[3];
+    DBuf<uint32_t> kcnt, isact, csr_off, rank, act, rec_loc, cnt, list[3];
+    DBuf<uint2> tile_sums;  // regular grids: per tile of cubes
     DBuf<uint8_t> fit, act_k;
     DBuf<uint16_t> rec;
     DBuf<uint32_t> cv, cf, cfv;
@@ -730,7 +950,14 @@ struct Workspace {
     DBuf<uint32_t> fq_a, fq_vs, fve;
     DBuf<uint16_t> fq_l;
     DBuf<uint8_t> fq_fn, fq_bnd;
-    DBuf<uint32_t> sidx, first, head, hexcl, seg_first, vid, vrank, first_occ;
+    DBuf<uint32_t> first, vid, vrank, first_occ, mo, fail;
+    DBuf<uint32_t> star_head;  // star tables of the regular grid (k_dedup_star)
+    DBuf<uint8_t> star_ent;
+    // vertex dedup hash table (zero between runs) and each occurrence's slot
+    DBuf<unsigned long long> hkey;
+    DBuf<uint32_t> hmx, hslot;
+    int hbits = 0;
+    bool hash_dirty = false;  // a run stopped early: clear the table before the next
     DBuf<uint32_t> ffirst, fmatch, ftfc, fvc, frank, ftpos, fvpos;
     // outputs
     DBuf<uint32_t> v_tet, v_sv, f_vstart, f_verts, f_tfstart, f_tf_tet;
@@ -771,15 +998,86 @@ struct Plan {
 
 namespace {
 
+// Stars of the simplices of a regular grid, from the cube decomposition
+// itself: every tet of a small grid contributes, for each of its corners,
+// edges and faces whose lowest vertex is an interior anchor of either parity,
+// its cube (offset from the anchor's cube, each coordinate -1 or 0) and its
+// index in the cube.  The decomposition repeats with the parity of i+j+k, so
+// the two anchors cover every simplex of every grid.  Entries are in tet order.
+struct StarTables {
+    std::vector<uint32_t> head;  // [2][kStarShapes]: byte offset << 8 | byte count
+    std::vector<uint8_t> ent;    // pairs (packed cube offset, tet in cube)
+};
+
+const StarTables& star_tables() {
+    static const StarTables T = [] {
+        const int res = 4, N = res + 1;
+        const uint32_t anchor[2] = {(uint32_t)((1 * N + 1) * N + 2), (uint32_t)((1 * N + 1) * N + 1)};  // parity 0, 1
+        std::vector<std::vector<std::pair<uint32_t, uint8_t>>> lists(2 * kStarShapes);
+        for (uint64_t t = 0; t < 5ull * res * res * res; ++t) {
+            uint32_t v[4];
+            dev::regular_tet(res, t, v);
+            const uint64_t cube = t / 5;
+            const int ci = (int)(cube / (res * res)), cj = (int)((cube / res) % res), ck = (int)(cube % res);
+            for (int sub = 1; sub < 16; ++sub) {
+                const int nvs = __builtin_popcount(sub);
+                if (nvs == 4) continue;
+                uint32_t w[3];
+                int n = 0;
+                for (int c = 0; c < 4; ++c)
+                    if (sub >> c & 1) w[n++] = v[c];
+                std::sort(w, w + n);
+                const int p = w[0] == anchor[0] ? 0 : (w[0] == anchor[1] ? 1 : -1);
+                if (p < 0) continue;
+                uint32_t shape = 0;
+                if (n == 2) shape = 1 + dev::lattice_dir(res, w[0], w[1]);
+                if (n == 3) shape = 28 + dev::lattice_dir(res, w[0], w[1]) * 27 + dev::lattice_dir(res, w[0], w[2]);
+                const int ai = (int)(w[0] / (N * N)), aj = (int)((w[0] / N) % N), ak = (int)(w[0] % N);
+                const int di = ci - ai, dj = cj - aj, dk = ck - ak;
+                if (di < -1 || di > 0 || dj < -1 || dj > 0 || dk < -1 || dk > 0)
+                    throw std::logic_error("star table: cube offset out of range");
+                const uint32_t order = (uint32_t)(((di + 1) * 4 + (dj + 1)) * 4 + (dk + 1)) * 8 + (uint32_t)(t % 5);
+                const uint8_t packed = (uint8_t)((di + 1) | (dj + 1) << 2 | (dk + 1) << 4);
+                auto& L = lists[p * kStarShapes + shape];
+                bool seen = false;
+                for (auto& e : L) seen = seen || (e.first == order);
+                if (!seen) L.push_back({order, packed});
+            }
+        }
+        StarTables S;
+        S.head.assign(2 * kStarShapes, 0);
+        for (size_t h = 0; h < lists.size(); ++h) {
+            auto& L = lists[h];
+            std::sort(L.begin(), L.end());
+            if (2 * L.size() > 255) throw std::logic_error("star table: star too large");
+            S.head[h] = (uint32_t)S.ent.size() << 8 | (uint32_t)(2 * L.size());
+            for (auto& e : L) {
+                S.ent.push_back(e.second);
+                S.ent.push_back((uint8_t)(e.first & 7u));
+            }
+        }
+        return S;
+    }();
+    return T;
+}
+
+unsigned cube_tiles(const Grid& g) {
+    const uint64_t cubes = g.view.T / 5;
+    return (unsigned)std::max<uint64_t>(1, (cubes + kCubeTile - 1) / kCubeTile);
+}
+
 void alloc_ws(Workspace& W, const Grid& g, const Caps_& c) {
     W.cap = c;
     DevMem& m = W.mem;
     const uint64_t V = g.view.V, T = g.view.T;
     W.pos.alloc(m, V);
     W.neg.alloc(m, V);
-    W.zeros.alloc(m, V + 1);
-    W.kcnt.alloc(m, T + 1);
-    W.isact.alloc(m, T + 1);
+    if (g.view.regular) {
+        W.tile_sums.alloc(m, cube_tiles(g) + 1);
+    } else {
+        W.kcnt.alloc(m, T + 1);
+        W.isact.alloc(m, T + 1);
+    }
     W.csr_off.alloc(m, T + 1);
     W.rank.alloc(m, T + 1);
     W.act.alloc(m, c.active + 1);
@@ -801,8 +1099,21 @@ void alloc_ws(Workspace& W, const Grid& g, const Caps_& c) {
     W.okey.alloc(m, c.vo + 1);
     W.oa.alloc(m, c.vo + 1);
     W.oj.alloc(m, c.vo + 1);
-    for (auto* b : {&W.sidx, &W.first, &W.head, &W.hexcl, &W.seg_first, &W.vid, &W.vrank, &W.first_occ})
-        b->alloc(m, c.vo + 1);
+    for (auto* b : {&W.first, &W.vid, &W.vrank, &W.first_occ, &W.hslot, &W.mo, &W.fail}) b->alloc(m, c.vo + 1);
+    {
+        const StarTables& ST = star_tables();
+        W.star_head.alloc(m, ST.head.size());
+        W.star_ent.alloc(m, ST.ent.size());
+        RIN_HIP(hipMemcpy(W.star_head.p, ST.head.data(), 4 * ST.head.size(), hipMemcpyHostToDevice));
+        RIN_HIP(hipMemcpy(W.star_ent.p, ST.ent.data(), ST.ent.size(), hipMemcpyHostToDevice));
+    }
+    W.hbits = 10;
+    while (W.hbits < 31 && (1ull << W.hbits) < 2ull * c.vo) ++W.hbits;
+    W.hkey.alloc(m, 1ull << W.hbits);
+    W.hmx.alloc(m, 1ull << W.hbits);
+    RIN_HIP(hipMemset(W.hkey.p, 0, 8ull << W.hbits));
+    RIN_HIP(hipMemset(W.hmx.p, 0, 4ull << W.hbits));
+    W.hash_dirty = false;
     W.fq_a.alloc(m, c.fo + 1);
     W.fq_vs.alloc(m, c.fo + 1);
     W.fq_l.alloc(m, c.fo + 1);
@@ -844,17 +1155,22 @@ void alloc_ws(Workspace& W, const Grid& g, const Caps_& c) {
     RIN_HIP(hipMemcpy(W.k2cnt.p, K2.cnt.data(), 4 * K2.cnt.size(), hipMemcpyHostToDevice));
 }
 
-// stage 1 (also run eagerly to size the workspace): signs, filter, CSR scan
+// stage 1 (also run eagerly to size the workspace): signs, filter, and
+// either the per-tile sums (regular grids) or the per-tet CSR scans
 void stage_count(Workspace& W, const Grid& g, const Values& val, hipStream_t s) {
     const uint64_t V = g.view.V, T = g.view.T;
     const uint32_t F = val.F;
     hipLaunchKernelGGL(k_zero_ctr, dim3(1), dim3(64), 0, s, W.ctr.p, (int)C_N, W.errs.p);
-    hipLaunchKernelGGL(k_signs_ia, dim3(nblk(V)), dim3(256), 0, s, (const double*)val.vals.p, V, F, W.pos.p, W.neg.p,
-                       W.zeros.p);
+    hipLaunchKernelGGL(k_signs_count, dim3(nblk(V)), dim3(256), 0, s, (const double*)val.vals.p, V, F, W.pos.p,
+                       W.neg.p, W.ctr.p);
     const uint64_t full = (F == 64) ? ~0ull : ((1ull << F) - 1);
+    if (g.view.regular) {
+        hipLaunchKernelGGL(k_cube_count, dim3(cube_tiles(g)), dim3(kCubeTile), 0, s, g.view, (const uint64_t*)W.pos.p,
+                           (const uint64_t*)W.neg.p, full, W.tile_sums.p, W.ctr.p);
+        return;
+    }
     hipLaunchKernelGGL(k_filter_count, dim3(nblk(T)), dim3(256), 0, s, g.view, (const uint64_t*)W.pos.p,
                        (const uint64_t*)W.neg.p, full, W.kcnt.p, W.isact.p, W.ctr.p);
-    scan_u32(W.mem, W.zeros.p, W.zeros.p, Count(V), W.ctr.p + C_DEGEN, s);
     scan_u32(W.mem, W.kcnt.p, W.csr_off.p, Count(T), W.ctr.p + C_FIT, s);
     scan_u32(W.mem, W.isact.p, W.rank.p, Count(T), W.ctr.p + C_ACTIVE, s);
 }
@@ -941,6 +1257,12 @@ void stage_fill(Workspace& W, const Grid& g, const Values& val, hipStream_t s) {
     A.errs = W.errs.p;
     A.cap_active = W.cap.active;
     A.cap_fit = W.cap.fit;
+    A.csr_off_w = W.csr_off.p;
+    A.rank_w = W.rank.p;
+    if (g.view.regular) {
+        hipLaunchKernelGGL(k_cube_fill, dim3(cube_tiles(g)), dim3(kCubeTile), 0, s, A, (const uint2*)W.tile_sums.p);
+        return;
+    }
     hipLaunchKernelGGL(k_fill_batched, dim3(nblk(T)), dim3(256), 0, s, A);
 }
 
@@ -998,22 +1320,39 @@ void stage_rest(Workspace& W, const Grid& g, const Values& val, hipStream_t s, i
 
     const Count NVO(ctr + C_VO, c.vo, abort), NFO(ctr + C_FO, c.fo, abort), NV(ctr + C_VERTS, c.vo, abort),
         NF(ctr + C_FACES, c.fo, abort);
-    const int kbits = 2 + std::max({bitsf(g.view.V), bitsf(g.view.n_edge_ids) + 6, bitsf(g.view.n_face_ids) + 12,
-                                    bitsf(c.active) + 10});
     if (part & REST_DEDUP) {
-        iota_u32(W.sidx.p, NVO, s);
-        sort_pairs(m, W.okey.p, W.sidx.p, NVO, kbits, s);
-        hipLaunchKernelGGL(k_heads, dim3(nblk(c.vo)), dim3(256), 0, s, NVO, (const uint64_t*)W.okey.p,
-                           (const uint32_t*)W.sidx.p, W.first.p, W.head.p);
-        scan_u32(m, W.head.p, W.hexcl.p, NVO, nullptr, s);
-        hipLaunchKernelGGL(k_assign, dim3(nblk(c.vo)), dim3(256), 0, s, NVO, (const uint32_t*)W.sidx.p,
-                           (const uint32_t*)W.head.p, (const uint32_t*)W.hexcl.p, W.seg_first.p);
+        // regular grids: star tables, then the hash path for what they leave;
+        // explicit meshes: the hash path for every occurrence
+        const bool star = g.view.regular != 0;
+        const Count NH = star ? Count(ctr + C_DFAIL, c.vo, abort) : NVO;
+        const uint32_t* list = star ? W.fail.p : nullptr;
+        const unsigned hgrid = star ? std::min(nblk(c.vo), 1024u) : nblk(c.vo);
+        if (star) {
+            StarArgs S;
+            S.res = g.view.res;
+            S.okey = W.okey.p;
+            S.oa = W.oa.p;
+            S.act = W.act.p;
+            S.rank = W.rank.p;
+            S.voff = W.cv.p;
+            S.star_head = W.star_head.p;
+            S.star_ent = W.star_ent.p;
+            S.first = W.first.p;
+            S.mo = W.mo.p;
+            S.fail = W.fail.p;
+            S.n_fail = ctr + C_DFAIL;
+            S.cap_fail = c.vo;
+            S.n_active = ctr + C_ACTIVE;
+            hipLaunchKernelGGL(k_dedup_star, dim3(nblk(c.vo)), dim3(256), 0, s, NVO, S);
+        }
+        hipLaunchKernelGGL(k_hash_insert, dim3(hgrid), dim3(256), 0, s, NH, list, (const uint64_t*)W.okey.p, W.hkey.p,
+                           W.hmx.p, W.hbits, W.hslot.p, W.errs.p);
+        hipLaunchKernelGGL(k_hash_first, dim3(hgrid), dim3(256), 0, s, NH, list, (const uint32_t*)W.hslot.p,
+                           (const uint32_t*)W.hmx.p, W.first.p, W.mo.p);
         scan_u32(m, W.first.p, W.vrank.p, NVO, ctr + C_VERTS, s);
-        hipLaunchKernelGGL(k_assign2, dim3(nblk(c.vo)), dim3(256), 0, s, NVO, (const uint32_t*)W.sidx.p,
-                           (const uint32_t*)W.head.p, (const uint32_t*)W.hexcl.p, (const uint32_t*)W.seg_first.p,
-                           (const uint32_t*)W.vrank.p, W.vid.p);
-        hipLaunchKernelGGL(k_first_list, dim3(nblk(c.vo)), dim3(256), 0, s, NVO, (const uint32_t*)W.first.p,
-                           (const uint32_t*)W.vrank.p, W.first_occ.p);
+        hipLaunchKernelGGL(k_dedup_assign, dim3(nblk(c.vo)), dim3(256), 0, s, NVO, (const uint32_t*)W.mo.p,
+                           (const uint32_t*)W.first.p, (const uint32_t*)W.vrank.p, W.vid.p, W.first_occ.p, NH, list,
+                           (const uint32_t*)W.hslot.p, W.hkey.p, W.hmx.p);
     }
     if (part & REST_XYZ) {
         hipLaunchKernelGGL(k_vert_fields_ia, dim3(nblk(c.vo)), dim3(256), 0, s, g.view, (const double*)val.vals.p,
@@ -1246,7 +1585,7 @@ void run_batched(DevMem& ctx_mem, const Grid& g, Values& val, uint32_t flags, Ru
             if (w.cap.b[2]) capture(cut_label("cut k=5..8", 2), [&] { launch_cut<CBig>(w, g, val, 2, grid[2], s); });
             capture("extract: per-tet output scans", [&] { stage_rest(w, g, val, s, REST_SCANS); });
             capture("extract occurrences (k_extract_batched)", [&] { stage_rest(w, g, val, s, REST_EXTRACT); });
-            capture("dedup verts (sort by key)", [&] { stage_rest(w, g, val, s, REST_DEDUP); });
+            capture("dedup verts (hash by key)", [&] { stage_rest(w, g, val, s, REST_DEDUP); });
             capture("compute xyz (k_vert_fields_ia)", [&] { stage_rest(w, g, val, s, REST_XYZ); });
             capture("faces", [&] { stage_rest(w, g, val, s, REST_FACES); });
             while (w.ev.size() < w.parts.size() + 1) {
@@ -1256,6 +1595,11 @@ void run_batched(DevMem& ctx_mem, const Grid& g, Values& val, uint32_t flags, Ru
             }
             w.exec = true;
         }
+        if (w.hash_dirty) {
+            RIN_HIP(hipMemsetAsync(w.hkey.p, 0, 8ull << w.hbits, s));
+            RIN_HIP(hipMemsetAsync(w.hmx.p, 0, 4ull << w.hbits, s));
+            w.hash_dirty = false;
+        }
         RIN_HIP(hipEventRecord(w.ev[0], s));
         for (size_t i = 0; i < w.parts.size(); ++i) {
             RIN_HIP(hipGraphLaunch(w.parts[i].second, s));
@@ -1263,6 +1607,7 @@ void run_batched(DevMem& ctx_mem, const Grid& g, Values& val, uint32_t flags, Ru
         }
         read_back();
         const uint32_t e = errors();
+        if (e) w.hash_dirty = true;
         if (e & ERR_OVERFLOW) {
             w.drop_graphs();
             w.exec = false;

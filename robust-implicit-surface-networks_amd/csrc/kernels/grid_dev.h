@@ -27,6 +27,11 @@ struct GridView {
 
 namespace dev {
 
+// corners (0..7: (i,j,k), (i+1,j,k), (i+1,j+1,k), (i,j+1,k), then the same
+// at k+1) of the five tets of cube (i,j,k), for even and odd i+j+k
+constexpr uint8_t kCubeTetEven[5][4] = {{4, 6, 1, 3}, {6, 3, 4, 7}, {1, 3, 0, 4}, {3, 1, 2, 6}, {4, 1, 6, 5}};
+constexpr uint8_t kCubeTetOdd[5][4] = {{7, 0, 2, 5}, {2, 3, 0, 7}, {5, 7, 0, 4}, {7, 2, 6, 5}, {0, 1, 2, 5}};
+
 __host__ __device__ inline void regular_tet(int res, uint64_t t, uint32_t v[4]) {
     const uint64_t R = (uint64_t)res, N = R + 1;
     const uint64_t cube = t / 5;
@@ -40,10 +45,8 @@ __host__ __device__ inline void regular_tet(int res, uint64_t t, uint32_t v[4]) 
                            (i + 1) * N * N + j * N + k + 1,
                            (i + 1) * N * N + (j + 1) * N + k + 1,
                            i * N * N + (j + 1) * N + k + 1};
-    const uint8_t even[5][4] = {{4, 6, 1, 3}, {6, 3, 4, 7}, {1, 3, 0, 4}, {3, 1, 2, 6}, {4, 1, 6, 5}};
-    const uint8_t odd[5][4] = {{7, 0, 2, 5}, {2, 3, 0, 7}, {5, 7, 0, 4}, {7, 2, 6, 5}, {0, 1, 2, 5}};
     const bool ev = ((i + j + k) % 2) == 0;
-    for (int a = 0; a < 4; ++a) v[a] = (uint32_t)c[ev ? even[q][a] : odd[q][a]];
+    for (int a = 0; a < 4; ++a) v[a] = (uint32_t)c[ev ? kCubeTetEven[q][a] : kCubeTetOdd[q][a]];
 }
 
 __host__ __device__ inline void tet_corners(const GridView& g, uint64_t t, uint32_t v[4]) {

@@ -513,6 +513,163 @@ __global__ void k_first_list(Count n, const uint32_t* __restrict__ first, const 
     }
 }
 
+// First-occurrence dedup.  Vertex ids are the ranks of first occurrences in
+// occurrence order, as a stable sort by key gives; every occurrence gets the
+// index `mo` of its key's first occurrence.
+//
+// Regular grids resolve almost every key without atomics: an edge, face or
+// corner key names a simplex of the grid, every tet of that simplex's star
+// holds the key (its function is active there), and occurrences run in tet
+// order, so the first occurrence lies in the lowest tet of the star.  A
+// host-built table lists the star of each simplex shape (the lattice
+// directions from its lowest vertex, by that vertex's parity) as cube offsets
+// and tets, in tet order; the first in-grid entry is that lowest tet, whose
+// occurrences are searched for the key.  Interior keys are unique.  A key
+// whose lowest star tet does not hold it (only degenerate inputs) goes, with
+// every other occurrence of it, to the hash path below.
+//
+// Hash path: the table holds key + 1 (0: empty) and ~(smallest occurrence
+// index) (0: none) per slot, linear probing; it is zero between runs (the
+// assign kernel clears the slots a run used, one per distinct key).
+__device__ inline uint32_t hash_slot(uint64_t k, int bits) {
+    return (uint32_t)((k * 0x9E3779B97F4A7C15ull) >> (64 - bits));
+}
+
+constexpr int kStarShapes = 1 + 27 + 27 * 27;  // corner, edge by direction, face by two directions
+
+struct StarArgs {
+    int res;
+    const uint64_t* okey;
+    const uint32_t *oa, *act, *rank, *voff;
+    const uint32_t* star_head;  // [2][kStarShapes]: first entry << 8 | entry count
+    const uint8_t* star_ent;    // (di+1) | (dj+1) << 2 | (dk+1) << 4 with di.. in {-1, 0}, then q
+    uint32_t *first, *mo, *fail, *n_fail;
+    uint32_t cap_fail;
+    const uint32_t* n_active;
+};
+
+__global__ void k_dedup_star(Count n, StarArgs A) {
+    const uint32_t R = (uint32_t)A.res, N = R + 1;
+    const uint32_t NA = *A.n_active;
+    const uint32_t NO = (uint32_t)count_of(n);
+    GRID_STRIDE(o, n) {
+        const uint64_t K = A.okey[o];
+        const uint32_t type = (uint32_t)(K & 3);
+        if (type == 3) {
+            A.first[o] = 1;
+            A.mo[o] = (uint32_t)o;
+            continue;
+        }
+        uint32_t a, shape;
+        if (type == 0) {
+            a = (uint32_t)(K >> 2);
+            shape = 0;
+        } else if (type == 1) {
+            const uint64_t e = K >> 8;
+            a = (uint32_t)(e / 27);
+            shape = 1 + (uint32_t)(e % 27);
+        } else {
+            const uint64_t fc = K >> 14;
+            a = (uint32_t)(fc / 729);
+            shape = 28 + (uint32_t)(fc % 729);
+        }
+        const uint32_t ai = a / (N * N), ar = a - ai * N * N, aj = ar / N, ak = ar - aj * N;
+        const uint32_t head = A.star_head[((ai + aj + ak) & 1u) * kStarShapes + shape];
+        uint64_t ts = ~0ull;  // lowest tet of the star inside the grid
+        for (uint32_t e = head >> 8, end = (head >> 8) + (head & 255u); e < end; e += 2) {
+            const uint32_t d = A.star_ent[e];
+            const int ci = (int)ai + (int)(d & 3u) - 1, cj = (int)aj + (int)((d >> 2) & 3u) - 1,
+                      ck = (int)ak + (int)((d >> 4) & 3u) - 1;
+            if (ci < 0 || cj < 0 || ck < 0 || ci >= (int)R || cj >= (int)R || ck >= (int)R) continue;
+            ts = (((uint64_t)ci * R + cj) * R + ck) * 5 + A.star_ent[e + 1];
+            break;
+        }
+        const uint32_t t = A.act[A.oa[o]];
+        uint32_t found = ~0u;
+        if (ts == t) {
+            found = (uint32_t)o;
+        } else if (ts != ~0ull) {
+            const uint32_t as = A.rank[ts];
+            if (as < NA && A.act[as] == (uint32_t)ts) {
+                const uint32_t lo = A.voff[as], hi = as + 1 < NA ? A.voff[as + 1] : NO;
+                for (uint32_t p = lo; p < hi && p < NO; ++p)
+                    if (A.okey[p] == K) {
+                        found = p;
+                        break;
+                    }
+            }
+        }
+        if (found != ~0u) {
+            A.first[o] = found == (uint32_t)o;
+            A.mo[o] = found;
+        } else {
+            A.first[o] = 0;
+            const uint32_t slot = atomicAdd(A.n_fail, 1u);
+            if (slot < A.cap_fail) A.fail[slot] = (uint32_t)o;
+        }
+    }
+}
+
+// hash path over occurrences list[0..n) (all occurrences when list is null)
+__global__ void k_hash_insert(Count n, const uint32_t* __restrict__ list, const uint64_t* __restrict__ okey,
+                              unsigned long long* __restrict__ hkey, uint32_t* __restrict__ hmx, int bits,
+                              uint32_t* __restrict__ hslot, uint32_t* __restrict__ errs) {
+    const uint32_t mask = (uint32_t)((1ull << bits) - 1);
+    GRID_STRIDE(i, n) {
+        const uint32_t o = list ? list[i] : (uint32_t)i;
+        const unsigned long long k = okey[o] + 1;
+        uint32_t h = hash_slot(k, bits);
+        bool ok = false;
+        for (uint32_t probe = 0; probe <= mask; ++probe) {
+            const unsigned long long prev = atomicCAS(&hkey[h], 0ull, k);
+            if (prev == 0 || prev == k) {
+                ok = true;
+                break;
+            }
+            h = (h + 1) & mask;
+        }
+        if (!ok) {  // full table (sized at twice the occurrence capacity: not reached)
+            flag_error(errs, ERR_CAPACITY);
+            hslot[o] = 0;
+            continue;
+        }
+        atomicMax(&hmx[h], ~o);
+        hslot[o] = h;
+    }
+}
+
+__global__ void k_hash_first(Count n, const uint32_t* __restrict__ list, const uint32_t* __restrict__ hslot,
+                             const uint32_t* __restrict__ hmx, uint32_t* __restrict__ first, uint32_t* __restrict__ mo) {
+    GRID_STRIDE(i, n) {
+        const uint32_t o = list ? list[i] : (uint32_t)i;
+        const uint32_t m = ~hmx[hslot[o]];
+        first[o] = m == o;
+        mo[o] = m;
+    }
+}
+
+// vrank: exclusive scan of `first`.  The hash slots the run used (those of
+// list[0..n_hash), or of every occurrence when list is null) are cleared here.
+__global__ void k_dedup_assign(Count n, const uint32_t* __restrict__ mo, const uint32_t* __restrict__ first,
+                               const uint32_t* __restrict__ vrank, uint32_t* __restrict__ id_of_occ,
+                               uint32_t* __restrict__ first_occ, Count n_hash, const uint32_t* __restrict__ list,
+                               const uint32_t* __restrict__ hslot, unsigned long long* __restrict__ hkey,
+                               uint32_t* __restrict__ hmx) {
+    GRID_STRIDE(o, n) {
+        const uint32_t id = vrank[mo[o]];
+        id_of_occ[o] = id;
+        if (first[o]) first_occ[id] = (uint32_t)o;
+    }
+    GRID_STRIDE(i, n_hash) {
+        const uint32_t o = list ? list[i] : (uint32_t)i;
+        if (first[o]) {
+            const uint32_t h = hslot[o];
+            hkey[h] = 0;
+            hmx[h] = 0;
+        }
+    }
+}
+
 // vertex records and coordinates (src/extract_mesh.cpp:95-228, 1446-1538)
 __global__ void k_vert_fields_ia(GridView g, const double* __restrict__ vals, const uint32_t* __restrict__ act,
                                  const uint32_t* __restrict__ csr_off, const uint8_t* __restrict__ fit,
