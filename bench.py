@@ -58,7 +58,7 @@ def extract_bytes(s):
 
 # single-kernel stages with an algorithmic byte count: stage label -> (kernel, bytes(sizes))
 KERNEL_STAGES = {
-    "extract occurrences (k_extract_batched)": ("k_extract_batched", extract_bytes),
+    "extract record occurrences (k_extract_records)": ("k_extract_records", extract_bytes),
 }
 
 
@@ -215,20 +215,28 @@ def main():
             vals.run(flags).release()
         t0 = time.perf_counter()
         last = None
-        stage_acc = {}
         for i in range(args.steps):
             # rin_run returns after the device work and the size read-back
             r = vals.run(flags)
-            for k, ms in r.stage_times():
-                stage_acc[k] = stage_acc.get(k, 0.0) + ms
             if i + 1 < args.steps:
                 r.release()
             else:
                 last = r
         dt = time.perf_counter() - t0
-        results[name] = {"s_per_step": dt / args.steps, "sizes": last.sizes,
-                         "stages_ms": {k: v / args.steps for k, v in stage_acc.items()}}
+        sizes = last.sizes
         last.release()
+        # per-stage device times from separate, untimed runs: the batched form
+        # then replays one captured graph per stage with events between them
+        stage_acc = {}
+        n_stage = max(1, min(args.steps, 5))
+        sflags = flags | (rin.FLAG_STAGE_TIMES if name == "batched" else 0)
+        for i in range(n_stage):
+            r = vals.run(sflags)
+            for k, ms in r.stage_times():
+                stage_acc[k] = stage_acc.get(k, 0.0) + ms
+            r.release()
+        results[name] = {"s_per_step": dt / args.steps, "sizes": sizes,
+                         "stages_ms": {k: v / n_stage for k, v in stage_acc.items()}}
 
     if args.copy_probe:
         nbytes = args.copy_probe << 20

@@ -58,6 +58,7 @@ typedef struct rin_func_desc {
 #define RIN_FLAG_BATCHED 8u      /* bucketed fused form */
 #define RIN_FLAG_KEEP_RECORDS 16u /* keep per-tet cut records for rin_result_records */
 #define RIN_FLAG_CUT_ONLY 32u    /* stop after the per-tet cut (the -R robustness check) */
+#define RIN_FLAG_STAGE_TIMES 64u /* with BATCHED: per-stage device times (one graph per stage; slower) */
 
 typedef struct rin_sizes {
     uint64_t n_tets, n_funcs;

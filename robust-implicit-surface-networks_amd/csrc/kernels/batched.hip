@@ -31,6 +31,13 @@ namespace {
 using CK2 = dev::Caps<16, 24, 6, 6>;
 // CMed (k 3..4) and CBig (k 5..8) come from kernels_ia.h
 
+// words of a record slot: the largest record a bucket's capacity allows
+// (header, vertices, faces with their cycles, coincidence groups)
+template <class C>
+constexpr uint32_t rec_words() {
+    return 8 + 4 * C::MV + C::MF * (4 + C::MFV) + 2 * C::MP;
+}
+
 constexpr uint32_t K1_FLAG = 0x80000000u;
 constexpr uint32_t ERR_OVERFLOW = 8;
 
@@ -397,9 +404,11 @@ __global__ void k_signs_count(const double* __restrict__ vals, uint64_t V, uint3
 
 constexpr int kCubeTile = 256;
 
+// pat (optional): per tet with one active function, its corner sign pattern
+// (sum of s_c 3^c, s_c = 0 negative, 1 zero, 2 positive), else 0
 __device__ inline void cube_masks(const GridView& g, uint32_t cube, const uint64_t* __restrict__ pos,
                                   const uint64_t* __restrict__ neg, uint64_t full, uint32_t c[8], uint64_t m[5],
-                                  bool& even) {
+                                  bool& even, uint8_t* pat = nullptr) {
     const uint32_t R = (uint32_t)g.res, N = R + 1;
     const uint32_t i = cube / (R * R), rem = cube - i * (R * R), j = rem / R, k = rem - j * R;
     const uint32_t b = i * N * N + j * N + k;
@@ -419,6 +428,16 @@ __device__ inline void cube_masks(const GridView& g, uint32_t cube, const uint64
         const uint64_t ap = P[t[0]] & P[t[1]] & P[t[2]] & P[t[3]];
         const uint64_t an = Q[t[0]] & Q[t[1]] & Q[t[2]] & Q[t[3]];
         m[q] = full & ~(ap | an);
+        if (pat) {
+            pat[q] = 0;
+            if (__popcll(m[q]) == 1) {
+                const uint32_t f = __ffsll((unsigned long long)m[q]) - 1;
+                int p = 0, w = 1;
+                for (int a = 0; a < 4; ++a, w *= 3)
+                    p += (((P[t[a]] >> f) & 1) ? 2 : (((Q[t[a]] >> f) & 1) ? 0 : 1)) * w;
+                pat[q] = (uint8_t)p;
+            }
+        }
     }
 }
 
@@ -460,34 +479,67 @@ __global__ void __launch_bounds__(kCubeTile) k_cube_count(GridView g, const uint
     }
 }
 
+// exclusive prefix of the tile sums, in place (one workgroup, 2048 tiles
+// per pass)
+__global__ void __launch_bounds__(256) k_tile_scan(uint2* __restrict__ ts, uint32_t n) {
+    __shared__ uint32_t wx[4], wy[4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t cx = 0, cy = 0;
+    for (uint32_t base = 0; base < n; base += 2048) {
+        const uint32_t i0 = base + threadIdx.x * 8;
+        uint2 v[8];
+        uint32_t sx = 0, sy = 0;
+        for (int i = 0; i < 8; ++i) {
+            v[i] = i0 + i < n ? ts[i0 + i] : make_uint2(0, 0);
+            sx += v[i].x;
+            sy += v[i].y;
+        }
+        uint32_t ix = sx, iy = sy;
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t ox = __shfl_up(ix, d, 64), oy = __shfl_up(iy, d, 64);
+            if (lane >= d) {
+                ix += ox;
+                iy += oy;
+            }
+        }
+        if (lane == 63) {
+            wx[wave] = ix;
+            wy[wave] = iy;
+        }
+        __syncthreads();
+        uint32_t rx = cx + ix - sx, ry = cy + iy - sy;
+        for (int w = 0; w < 4; ++w) {
+            if (w < wave) {
+                rx += wx[w];
+                ry += wy[w];
+            }
+            cx += wx[w];
+            cy += wy[w];
+        }
+        for (int i = 0; i < 8; ++i) {
+            if (i0 + i < n) ts[i0 + i] = make_uint2(rx, ry);
+            rx += v[i].x;
+            ry += v[i].y;
+        }
+        __syncthreads();
+    }
+}
+
+// tile_sums: exclusive prefixes (k_tile_scan)
 __global__ void __launch_bounds__(kCubeTile) k_cube_fill(FillArgs A, const uint2* __restrict__ tile_sums) {
     __shared__ K1Entry tab[81];
-    __shared__ uint32_t red[kCubeTile / 64][2];
     __shared__ uint32_t wsum[kCubeTile / 64];
     __shared__ uint32_t s_off[5 * kCubeTile], s_rank[5 * kCubeTile];  // per-tet CSR entries of the tile
     for (int i = threadIdx.x; i < 81; i += blockDim.x) tab[i] = A.k1[i];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    // prefix of this tile: the sums of the tiles before it
-    uint32_t pk = 0, pa = 0;
-    for (uint32_t i = threadIdx.x; i < blockIdx.x; i += kCubeTile) {
-        const uint2 t = tile_sums[i];
-        pk += t.x;
-        pa += t.y;
-    }
-    for (int d = 32; d > 0; d >>= 1) {
-        pk += __shfl_xor(pk, d, 64);
-        pa += __shfl_xor(pa, d, 64);
-    }
-    if (lane == 0) {
-        red[wave][0] = pk;
-        red[wave][1] = pa;
-    }
+    const uint2 pre = tile_sums[blockIdx.x];
     const uint32_t ncubes = (uint32_t)(A.g.T / 5);
     const uint32_t cube = blockIdx.x * kCubeTile + threadIdx.x;
     uint32_t c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     uint64_t m[5] = {0, 0, 0, 0, 0};
     bool even = true;
-    if (cube < ncubes) cube_masks(A.g, cube, A.pos, A.neg, A.full, c, m, even);
+    uint8_t pat[5] = {0, 0, 0, 0, 0};
+    if (cube < ncubes) cube_masks(A.g, cube, A.pos, A.neg, A.full, c, m, even, pat);
     // per lane: active functions, active tets, and tets per bucket list
     // (k = 2, 3..4, 5..8), packed ks << 11 | as and 10 bits per bucket
     uint32_t ks = 0, as = 0, bp = 0;
@@ -524,13 +576,7 @@ __global__ void __launch_bounds__(kCubeTile) k_cube_fill(FillArgs A, const uint2
     __syncthreads();
     uint32_t ex = inc - packed;
     for (int w = 0; w < wave; ++w) ex += wsum[w];
-    uint32_t o = 0, r = 0;
-    for (int w = 0; w < kCubeTile / 64; ++w) {
-        o += red[w][0];
-        r += red[w][1];
-    }
-    o += ex >> 11;
-    r += ex & 2047u;
+    uint32_t o = pre.x + (ex >> 11), r = pre.y + (ex & 2047u);
     for (int q = 0; q < 5; ++q) {
         const uint64_t t = (uint64_t)cube * 5 + q;
         const uint64_t mq = m[q];
@@ -547,19 +593,11 @@ __global__ void __launch_bounds__(kCubeTile) k_cube_fill(FillArgs A, const uint2
                 uint32_t oo = o;
                 for (uint64_t mm = mq; mm; mm &= mm - 1) A.fit[oo++] = (uint8_t)(__ffsll((unsigned long long)mm) - 1);
                 if (k == 1) {
-                    const uint32_t f = __ffsll((unsigned long long)mq) - 1;
-                    const uint8_t* tv = even ? dev::kCubeTetEven[q] : dev::kCubeTetOdd[q];
-                    int pat = 0, w = 1;
-                    for (int a = 0; a < 4; ++a, w *= 3) {
-                        const uint32_t v = c[tv[a]];
-                        const int sg = ((A.pos[v] >> f) & 1) ? 2 : (((A.neg[v] >> f) & 1) ? 0 : 1);
-                        pat += sg * w;
-                    }
-                    const K1Entry& E = tab[pat];
+                    const K1Entry& E = tab[pat[q]];
                     A.cnt[3 * r] = E.niv;
                     A.cnt[3 * r + 1] = E.nif;
                     A.cnt[3 * r + 2] = E.nfv;
-                    A.rec_loc[r] = K1_FLAG | (uint32_t)pat;
+                    A.rec_loc[r] = K1_FLAG | (uint32_t)pat[q];
                 } else if (k <= 8) {
                     const int qq = k == 2 ? 0 : (k <= 4 ? 1 : 2);
                     const uint32_t slot = bnext[qq]++;
@@ -769,52 +807,75 @@ struct ExtractArgs {
     uint16_t* fq_l;
     uint8_t *fq_fn, *fq_bnd;
     uint32_t *fq_vs, *fve, *errs;
+    const uint32_t* list[3];  // bucket lists (k = 2, 3..4, 5..8) of active ranks
+    uint32_t list_cap[3];
 };
 
-__global__ void k_extract_batched(ExtractArgs A) {
-    __shared__ K1Entry tab[81];
-    for (int i = threadIdx.x; i < 81; i += blockDim.x) tab[i] = A.k1[i];
-    __syncthreads();
-    if (A.ctr[C_ABORT]) return;
+// Extraction in two passes so that waves stay uniform: the table pass
+// writes the occurrences of every k = 1 tet (81 sign patterns in LDS), the
+// record pass those of the tets on the bucket lists (k = 2..8), whose records
+// vary in size.
+__device__ inline bool extract_guard(const ExtractArgs& A) {
+    if (A.ctr[C_ABORT]) return false;
     if (A.tot[C_VO] > A.cap_vo || A.tot[C_FO] > A.cap_fo || A.tot[C_FV] > A.cap_fv) {
         if (blockIdx.x == 0) {
             A.errs[threadIdx.x & 63] |= ERR_OVERFLOW;
             A.ctr[C_ABORT] = 1;
         }
-        return;
+        return false;
     }
+    return true;
+}
+
+constexpr uint8_t kLocalEdge[6][2] = {{0, 1}, {0, 2}, {0, 3}, {1, 2}, {1, 3}, {2, 3}};
+
+__global__ void k_extract_k1(ExtractArgs A) {
+    __shared__ K1Entry tab[81];
+    for (int i = threadIdx.x; i < 81; i += blockDim.x) tab[i] = A.k1[i];
+    __syncthreads();
+    if (!extract_guard(A)) return;
     const uint64_t NA = min((uint64_t)*A.n_active, (uint64_t)A.cap_active);
-    const uint8_t le[6][2] = {{0, 1}, {0, 2}, {0, 3}, {1, 2}, {1, 3}, {2, 3}};
-    uint32_t recw = 0;  // record words read (throughput accounting)
     GRID_STRIDE(a, NA) {
+        const uint32_t loc = A.rec_loc[a];
+        if (!(loc & K1_FLAG) || (loc & ~K1_FLAG) >= 81) continue;
+        const uint32_t t = A.act[a];
+        uint32_t v[4];
+        dev::tet_corners(A.g, t, v);
+        const K1Entry& E = tab[loc & 0xFF];
+        const uint32_t f = A.fit[A.csr_off[t]];
+        uint32_t o = A.voff[a];
+        for (int i = 0; i < E.niv; ++i) {
+            const int vt = E.vt[i];
+            A.okey[o] = vt < 4 ? key_corner(v[vt])
+                               : key_edge(dev::edge_id(A.g, t, v, kLocalEdge[vt - 4][0], kLocalEdge[vt - 4][1]), f);
+            A.oa[o] = (uint32_t)a;
+            A.oj[o] = E.vj[i];
+            ++o;
+        }
+        if (E.nif) {
+            const uint32_t q = A.foff[a], e = A.fvoff[a], vb = A.voff[a];
+            A.fq_a[q] = (uint32_t)a;
+            A.fq_l[q] = E.fl;
+            A.fq_fn[q] = (uint8_t)f;
+            A.fq_bnd[q] = E.fb;
+            A.fq_vs[q] = e;
+            for (int i = 0; i < E.fn; ++i) A.fve[e + i] = vb + E.fv[i];
+        }
+    }
+}
+
+// k = 2 records (mostly sign-vector table records): one lane per tet
+__global__ void k_extract_records(ExtractArgs A) {
+    if (!extract_guard(A)) return;
+    const uint32_t n0 = min(A.ctr[C_Q2], A.list_cap[0]);
+    uint32_t recw = 0;  // record words read (throughput accounting)
+    GRID_STRIDE(i, n0) {
+        const uint32_t a = A.list[0][i];
         const uint32_t loc = A.rec_loc[a];
         const uint32_t t = A.act[a];
         uint32_t v[4];
         dev::tet_corners(A.g, t, v);
         const uint32_t s0 = A.csr_off[t];
-        if ((loc & K1_FLAG) && (loc & ~K1_FLAG) < 81) {
-            const K1Entry& E = tab[loc & 0xFF];
-            const uint32_t f = A.fit[s0];
-            uint32_t o = A.voff[a];
-            for (int i = 0; i < E.niv; ++i) {
-                const int vt = E.vt[i];
-                A.okey[o] = vt < 4 ? key_corner(v[vt])
-                                   : key_edge(dev::edge_id(A.g, t, v, le[vt - 4][0], le[vt - 4][1]), f);
-                A.oa[o] = (uint32_t)a;
-                A.oj[o] = E.vj[i];
-                ++o;
-            }
-            if (E.nif) {
-                const uint32_t q = A.foff[a], e = A.fvoff[a], vb = A.voff[a];
-                A.fq_a[q] = (uint32_t)a;
-                A.fq_l[q] = E.fl;
-                A.fq_fn[q] = (uint8_t)f;
-                A.fq_bnd[q] = E.fb;
-                A.fq_vs[q] = e;
-                for (int i = 0; i < E.fn; ++i) A.fve[e + i] = vb + E.fv[i];
-            }
-            continue;
-        }
         // general record (same layout and rules as the plain extraction)
         const Rec R((loc & K1_FLAG) ? A.tab + A.tab_off[loc & ~K1_FLAG] : A.rec + loc);
         if (R.r[0] != 1) continue;
@@ -896,6 +957,138 @@ __global__ void k_extract_batched(ExtractArgs A) {
     if ((threadIdx.x & 63) == 0 && recw) atomicAdd(A.ctr + C_RECW, recw);
 }
 
+// k = 3..8 records: one wave per tet.  The record is copied to LDS in
+// parallel, one lane walks the variable-length face list there for the face
+// offsets, then the lanes take faces and vertices in parallel; occurrences
+// and face rows come out in the same order as one lane per tet writes them.
+constexpr uint32_t kWaveRecWords = rec_words<CBig>();
+
+__global__ void __launch_bounds__(64) k_extract_records_wave(ExtractArgs A) {
+    __shared__ uint16_t W[kWaveRecWords];
+    __shared__ uint16_t fo[CBig::MF];
+    __shared__ uint32_t isov[(CBig::MV + 31) / 32];
+    __shared__ uint8_t isof[CBig::MF];
+    if (!extract_guard(A)) return;
+    const uint32_t n2 = min(A.ctr[C_Q5], A.list_cap[2]), n1 = min(A.ctr[C_Q3], A.list_cap[1]);
+    const uint32_t i = blockIdx.x;
+    if (i >= n1 + n2) return;
+    // the larger complexes first
+    const uint32_t a = i < n2 ? A.list[2][i] : A.list[1][i - n2];
+    const int lane = threadIdx.x;
+    const uint32_t loc = A.rec_loc[a];
+    const uint16_t* src = (loc & K1_FLAG) ? A.tab + A.tab_off[loc & ~K1_FLAG] : A.rec + loc;
+    if (src[0] != 1) return;
+    const uint32_t len = src[6];
+    const int nv = src[2], nf = src[3], L = src[1];
+    if (len > kWaveRecWords || nv > CBig::MV || nf > CBig::MF || len < 8u + 4u * nv) {
+        if (lane == 0) A.errs[0] |= ERR_RECORD;
+        return;
+    }
+    for (uint32_t x = lane; x < len; x += 64) W[x] = src[x];
+    for (int x = lane; x < (nv + 31) / 32; x += 64) isov[x] = 0;
+    if (lane == 0) atomicAdd(A.ctr + C_RECW, len);
+    __syncthreads();
+    const bool dup = W[5] & 1;
+    if (lane == 0) {
+        uint32_t off = 8 + 4 * nv;
+        for (int f = 0; f < nf; ++f) {
+            fo[f] = (uint16_t)off;
+            off += off + 3 < len ? 4 + W[off + 3] : 4;
+        }
+    }
+    __syncthreads();
+    const uint16_t* grp = W + len - 2 * L;
+    for (int f = lane; f < nf; f += 64) {
+        const uint16_t* F = W + fo[f];
+        bool iso = F[0] > 3;
+        if (!iso && dup) {
+            const uint16_t rep = grp[F[0]];
+            for (int q = 4; q < L && !iso; ++q) iso = grp[q] == rep;
+        }
+        isof[f] = iso;
+        if (iso)
+            for (int q = 0; q < F[3]; ++q) atomicOr(&isov[F[4 + q] >> 5], 1u << (F[4 + q] & 31));
+    }
+    __syncthreads();
+    auto vrank = [&](int j) {
+        uint32_t r = 0;
+        for (int w = 0; w < (j >> 5); ++w) r += __popc(isov[w]);
+        return r + __popc(isov[j >> 5] & ((1u << (j & 31)) - 1u));
+    };
+    const uint32_t t = A.act[a];
+    uint32_t v[4];
+    dev::tet_corners(A.g, t, v);
+    const uint32_t s0 = A.csr_off[t];
+    const uint32_t vb = A.voff[a];
+    for (int j = lane; j < nv; j += 64) {
+        if (!((isov[j >> 5] >> (j & 31)) & 1u)) continue;
+        const uint16_t* T3 = W + 8 + 4 * j;
+        int bnd[3], nb = 0;
+        uint32_t fn[3];
+        int ni = 0;
+        for (int q = 0; q < 3; ++q) {
+            if (T3[q] > 3)
+                fn[ni++] = A.fit[s0 + T3[q] - 4];
+            else
+                bnd[nb++] = T3[q];
+        }
+        uint64_t key;
+        if (nb == 3) {
+            int c = 0;
+            while (c == bnd[0] || c == bnd[1] || c == bnd[2]) ++c;
+            key = key_corner(v[c]);
+        } else if (nb == 2) {
+            int l[2], nl = 0;
+            for (int c = 0; c < 4; ++c)
+                if (c != bnd[0] && c != bnd[1]) l[nl++] = c;
+            key = key_edge(dev::edge_id(A.g, t, v, l[0], l[1]), fn[0]);
+        } else if (nb == 1) {
+            key = key_face(dev::face_id(A.g, t, v, bnd[0]), fn[0], fn[1]);
+        } else {
+            key = key_inner(a, j);
+        }
+        const uint32_t o = vb + vrank(j);
+        A.okey[o] = key;
+        A.oa[o] = a;
+        A.oj[o] = (uint16_t)j;
+    }
+    // iso faces in face order: 64 at a time, ordinals and corner offsets by
+    // wave prefix sums
+    const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+    uint32_t q = A.foff[a], e = A.fvoff[a];
+    for (int base = 0; base < nf; base += 64) {
+        const int f = base + lane;
+        const bool iso = f < nf && isof[f];
+        const uint16_t* F = W + fo[f < nf ? f : 0];
+        const uint32_t n = iso ? F[3] : 0u;
+        const uint64_t m = __ballot(iso);
+        uint32_t inc = n;
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t x = __shfl_up(inc, d, 64);
+            if (lane >= d) inc += x;
+        }
+        if (iso) {
+            const uint32_t qi = q + (uint32_t)__popcll(m & lt);
+            uint32_t ei = e + inc - n;
+            int sp = F[0];
+            if (sp <= 3)
+                for (int p = 4; p < L; ++p)
+                    if (grp[p] == grp[sp]) {
+                        sp = p;
+                        break;
+                    }
+            A.fq_a[qi] = a;
+            A.fq_l[qi] = (uint16_t)f;
+            A.fq_fn[qi] = A.fit[s0 + sp - 4];
+            A.fq_bnd[qi] = F[2] == 0xFFFF ? (uint8_t)(1 + F[0]) : 0;
+            A.fq_vs[qi] = ei;
+            for (uint32_t c = 0; c < n; ++c) A.fve[ei++] = vb + vrank(F[4 + c]);
+        }
+        q += (uint32_t)__popcll(m);
+        e += __shfl(inc, 63, 64);
+    }
+}
+
 __global__ void k_write_totals(const uint32_t* ctr, uint64_t T, uint32_t* csr_off, uint32_t* f_vstart,
                                uint32_t* f_tfstart, uint32_t cap_fo) {
     if (threadIdx.x == 0) {
@@ -921,11 +1114,6 @@ __global__ void k_cap_check(const uint32_t* ctr, uint32_t cap_vo, uint32_t cap_f
 }  // namespace
 
 // ------------------------------------------------------------------ plan
-
-template <class C>
-constexpr uint32_t rec_words() {
-    return 8 + 4 * C::MV + C::MF * (4 + C::MFV) + 2 * C::MP;
-}
 
 struct Caps_ {
     uint32_t active = 0, fit = 0, b[3] = {0, 0, 0}, vo = 0, fo = 0, fv = 0;
@@ -975,6 +1163,13 @@ struct Workspace {
     std::vector<std::pair<std::string, hipGraph_t>> graphs;
     std::vector<std::pair<std::string, hipGraphExec_t>> parts;
     std::vector<hipEvent_t> ev;
+    // the same path as one graph: replays without stage times
+    hipGraph_t whole_graph = nullptr;
+    hipGraphExec_t whole = nullptr;
+    // side branch of the whole graph (the k = 5..8 and k = 2 cuts run beside
+    // the k = 3..4 cut): a stream to capture it on and the fork / join events
+    hipStream_t side = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
     bool exec = false;  // graphs captured for the current capacities
     bool in_use = false;
     void drop_graphs() {
@@ -982,10 +1177,17 @@ struct Workspace {
         for (auto& g : graphs) (void)hipGraphDestroy(g.second);
         parts.clear();
         graphs.clear();
+        if (whole) (void)hipGraphExecDestroy(whole);
+        if (whole_graph) (void)hipGraphDestroy(whole_graph);
+        whole = nullptr;
+        whole_graph = nullptr;
     }
     ~Workspace() {
         drop_graphs();
         for (auto e : ev) (void)hipEventDestroy(e);
+        if (fork) (void)hipEventDestroy(fork);
+        if (join) (void)hipEventDestroy(join);
+        if (side) (void)hipStreamDestroy(side);
     }
 };
 
@@ -1260,6 +1462,7 @@ void stage_fill(Workspace& W, const Grid& g, const Values& val, hipStream_t s) {
     A.csr_off_w = W.csr_off.p;
     A.rank_w = W.rank.p;
     if (g.view.regular) {
+        hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(256), 0, s, W.tile_sums.p, cube_tiles(g));
         hipLaunchKernelGGL(k_cube_fill, dim3(cube_tiles(g)), dim3(kCubeTile), 0, s, A, (const uint2*)W.tile_sums.p);
         return;
     }
@@ -1269,7 +1472,19 @@ void stage_fill(Workspace& W, const Grid& g, const Values& val, hipStream_t s) {
 // everything after the cut
 // the stages after the cut, each captured as its own graph so that the
 // stream events between graphs time them separately
-enum RestPart { REST_SCANS = 1, REST_EXTRACT = 2, REST_DEDUP = 4, REST_XYZ = 8, REST_FACES = 16, REST_ALL = 31 };
+enum RestPart {
+    REST_SCANS = 1,
+    REST_EXTRACT_K1 = 2,
+    REST_DEDUP = 4,
+    REST_XYZ = 8,
+    REST_FACES = 16,
+    REST_EXTRACT_REC = 32,
+    REST_ALL = 63
+};
+
+// grid of a grid-stride kernel whose element count lives on the device:
+// bounded, so that capacity-sized launches do not pay for idle workgroups
+unsigned cgrid(uint64_t cap) { return std::min(nblk(cap), 2048u); }
 
 void stage_rest(Workspace& W, const Grid& g, const Values& val, hipStream_t s, int part) {
     DevMem& m = W.mem;
@@ -1278,13 +1493,12 @@ void stage_rest(Workspace& W, const Grid& g, const Values& val, hipStream_t s, i
     const uint32_t* abort = ctr + C_ABORT;
     const Count NA(ctr + C_ACTIVE, c.active, abort);
     if (part & REST_SCANS) {
-        hipLaunchKernelGGL(k_split3, dim3(nblk(c.active)), dim3(256), 0, s, NA, (const uint32_t*)W.cnt.p, W.cv.p, W.cf.p,
-                           W.cfv.p);
-        scan_u32(m, W.cv.p, W.cv.p, NA, ctr + C_VO, s);
-        scan_u32(m, W.cf.p, W.cf.p, NA, ctr + C_FO, s);
-        scan_u32(m, W.cfv.p, W.cfv.p, NA, ctr + C_FV, s);
+        const uint32_t* in[3] = {W.cnt.p, W.cnt.p + 1, W.cnt.p + 2};
+        uint32_t* out[3] = {W.cv.p, W.cf.p, W.cfv.p};
+        uint32_t* tot[3] = {ctr + C_VO, ctr + C_FO, ctr + C_FV};
+        scan3_u32(m, in, 3, out, NA, tot, s);
     }
-    if (part & REST_EXTRACT) {
+    if (part & (REST_EXTRACT_K1 | REST_EXTRACT_REC)) {
         ExtractArgs E;
         E.g = g.view;
         E.act = W.act.p;
@@ -1315,7 +1529,15 @@ void stage_rest(Workspace& W, const Grid& g, const Values& val, hipStream_t s, i
         E.fq_vs = W.fq_vs.p;
         E.fve = W.fve.p;
         E.errs = W.errs.p;
-        hipLaunchKernelGGL(k_extract_batched, dim3(nblk(c.active)), dim3(256), 0, s, E);
+        for (int q = 0; q < 3; ++q) {
+            E.list[q] = W.list[q].p;
+            E.list_cap[q] = c.b[q];
+        }
+        if (part & REST_EXTRACT_K1) hipLaunchKernelGGL(k_extract_k1, dim3(cgrid(c.active)), dim3(256), 0, s, E);
+        if (part & REST_EXTRACT_REC) {
+            hipLaunchKernelGGL(k_extract_records_wave, dim3(std::max(1u, c.b[1] + c.b[2])), dim3(64), 0, s, E);
+            hipLaunchKernelGGL(k_extract_records, dim3(cgrid(c.b[0])), dim3(256), 0, s, E);
+        }
     }
 
     const Count NVO(ctr + C_VO, c.vo, abort), NFO(ctr + C_FO, c.fo, abort), NV(ctr + C_VERTS, c.vo, abort),
@@ -1343,19 +1565,19 @@ void stage_rest(Workspace& W, const Grid& g, const Values& val, hipStream_t s, i
             S.n_fail = ctr + C_DFAIL;
             S.cap_fail = c.vo;
             S.n_active = ctr + C_ACTIVE;
-            hipLaunchKernelGGL(k_dedup_star, dim3(nblk(c.vo)), dim3(256), 0, s, NVO, S);
+            hipLaunchKernelGGL(k_dedup_star, dim3(cgrid(c.vo)), dim3(256), 0, s, NVO, S);
         }
         hipLaunchKernelGGL(k_hash_insert, dim3(hgrid), dim3(256), 0, s, NH, list, (const uint64_t*)W.okey.p, W.hkey.p,
                            W.hmx.p, W.hbits, W.hslot.p, W.errs.p);
         hipLaunchKernelGGL(k_hash_first, dim3(hgrid), dim3(256), 0, s, NH, list, (const uint32_t*)W.hslot.p,
                            (const uint32_t*)W.hmx.p, W.first.p, W.mo.p);
         scan_u32(m, W.first.p, W.vrank.p, NVO, ctr + C_VERTS, s);
-        hipLaunchKernelGGL(k_dedup_assign, dim3(nblk(c.vo)), dim3(256), 0, s, NVO, (const uint32_t*)W.mo.p,
+        hipLaunchKernelGGL(k_dedup_assign, dim3(cgrid(c.vo)), dim3(256), 0, s, NVO, (const uint32_t*)W.mo.p,
                            (const uint32_t*)W.first.p, (const uint32_t*)W.vrank.p, W.vid.p, W.first_occ.p, NH, list,
                            (const uint32_t*)W.hslot.p, W.hkey.p, W.hmx.p);
     }
     if (part & REST_XYZ) {
-        hipLaunchKernelGGL(k_vert_fields_ia, dim3(nblk(c.vo)), dim3(256), 0, s, g.view, (const double*)val.vals.p,
+        hipLaunchKernelGGL(k_vert_fields_ia, dim3(cgrid(c.vo)), dim3(256), 0, s, g.view, (const double*)val.vals.p,
                            (const uint32_t*)W.act.p, (const uint32_t*)W.csr_off.p, (const uint8_t*)W.fit.p,
                            (const uint32_t*)W.rec_loc.p, (const uint16_t*)W.rec.p, (const uint16_t*)W.k1_words.p,
                            (const uint32_t*)W.k1_off.p, (const uint32_t*)W.first_occ.p,
@@ -1364,7 +1586,7 @@ void stage_rest(Workspace& W, const Grid& g, const Values& val, hipStream_t s, i
     }
     if (part & REST_FACES) {
         // faces: a boundary face may repeat in the tet across its tet face
-        hipLaunchKernelGGL(k_face_map, dim3(nblk(c.fv)), dim3(256), 0, s, Count(ctr + C_FV, c.fv, abort), W.fve.p,
+        hipLaunchKernelGGL(k_face_map, dim3(cgrid(c.fv)), dim3(256), 0, s, Count(ctr + C_FV, c.fv, abort), W.fve.p,
                            (const uint32_t*)W.vid.p);
         FaceArgs FA;
         FA.fq_a = W.fq_a.p;
@@ -1386,10 +1608,13 @@ void stage_rest(Workspace& W, const Grid& g, const Values& val, hipStream_t s, i
         FA.match = W.fmatch.p;
         FA.tfc = W.ftfc.p;
         FA.vcnt = W.fvc.p;
-        hipLaunchKernelGGL(k_face_first, dim3(nblk(c.fo)), dim3(256), 0, s, FA);
-        scan_u32(m, W.ffirst.p, W.frank.p, NFO, ctr + C_FACES, s);
-        scan_u32(m, W.ftfc.p, W.ftpos.p, NFO, ctr + C_TETFACES, s);
-        scan_u32(m, W.fvc.p, W.fvpos.p, NFO, ctr + C_FACEVERTS, s);
+        hipLaunchKernelGGL(k_face_first, dim3(cgrid(c.fo)), dim3(256), 0, s, FA);
+        {
+            const uint32_t* in[3] = {W.ffirst.p, W.ftfc.p, W.fvc.p};
+            uint32_t* out[3] = {W.frank.p, W.ftpos.p, W.fvpos.p};
+            uint32_t* tot[3] = {ctr + C_FACES, ctr + C_TETFACES, ctr + C_FACEVERTS};
+            scan3_u32(m, in, 1, out, NFO, tot, s);
+        }
         FaceOutArgs FO;
         FO.fq_a = W.fq_a.p;
         FO.fq_vs = W.fq_vs.p;
@@ -1411,7 +1636,7 @@ void stage_rest(Workspace& W, const Grid& g, const Values& val, hipStream_t s, i
         FO.f_tf_tet = W.f_tf_tet.p;
         FO.f_tf_local = W.f_tf_local.p;
         FO.f_func = W.f_func.p;
-        hipLaunchKernelGGL(k_face_emit, dim3(nblk(c.fo)), dim3(256), 0, s, FO);
+        hipLaunchKernelGGL(k_face_emit, dim3(cgrid(c.fo)), dim3(256), 0, s, FO);
         hipLaunchKernelGGL(k_write_totals, dim3(1), dim3(64), 0, s, (const uint32_t*)ctr, g.view.T, W.csr_off.p,
                            W.f_vstart.p, W.f_tfstart.p, c.fo);
     }
@@ -1460,6 +1685,7 @@ float time_ms(hipStream_t s, const std::function<void()>& f) {
 void run_batched(DevMem& ctx_mem, const Grid& g, Values& val, uint32_t flags, RunOut& out, hipStream_t s, int n_cu) {
     if (flags & (RUN_MI | RUN_CUT_ONLY | RUN_REVERSE))
         throw std::runtime_error("batched form is not available for this mode (use the plain form)");
+    const bool staged = (flags & RUN_STAGE_TIMES) != 0;
     const uint64_t V = g.view.V, T = g.view.T;
     if (val.V != V) throw std::invalid_argument("values do not match the grid");
     ensure_nbr(ctx_mem, g, s);
@@ -1572,22 +1798,51 @@ void run_batched(DevMem& ctx_mem, const Grid& g, Values& val, uint32_t flags, Ru
                 w.graphs.push_back({label, gr});
                 w.parts.push_back({label, ex});
             };
-            capture("signs + filter + CSR scans", [&] { stage_count(w, g, val, s); });
-            capture("CSR fill + buckets + k=1 table", [&] { stage_fill(w, g, val, s); });
+            std::vector<std::pair<std::string, std::function<void()>>> stages;
+            stages.push_back({"signs + filter + CSR scans", [&] { stage_count(w, g, val, s); }});
+            stages.push_back({"CSR fill + buckets + k=1 table", [&] { stage_fill(w, g, val, s); }});
             auto cut_label = [&](const char* base, int q) {
                 return std::string(base) + (grid[q] == -2 ? " (sign-vector table)"
                                             : grid[q] < 0 ? " (wave per tet)"
                                                           : " (thread per tet, " + std::to_string(P.blocks_per_cu[q]) +
                                                                 " blocks/CU)");
             };
-            if (w.cap.b[0]) capture(cut_label("cut k=2", 0), [&] { launch_cut<CK2>(w, g, val, 0, grid[0], s); });
-            if (w.cap.b[1]) capture(cut_label("cut k=3..4", 1), [&] { launch_cut<CMed>(w, g, val, 1, grid[1], s); });
-            if (w.cap.b[2]) capture(cut_label("cut k=5..8", 2), [&] { launch_cut<CBig>(w, g, val, 2, grid[2], s); });
-            capture("extract: per-tet output scans", [&] { stage_rest(w, g, val, s, REST_SCANS); });
-            capture("extract occurrences (k_extract_batched)", [&] { stage_rest(w, g, val, s, REST_EXTRACT); });
-            capture("dedup verts (hash by key)", [&] { stage_rest(w, g, val, s, REST_DEDUP); });
-            capture("compute xyz (k_vert_fields_ia)", [&] { stage_rest(w, g, val, s, REST_XYZ); });
-            capture("faces", [&] { stage_rest(w, g, val, s, REST_FACES); });
+            if (w.cap.b[0]) stages.push_back({cut_label("cut k=2", 0), [&] { launch_cut<CK2>(w, g, val, 0, grid[0], s); }});
+            if (w.cap.b[1])
+                stages.push_back({cut_label("cut k=3..4", 1), [&] { launch_cut<CMed>(w, g, val, 1, grid[1], s); }});
+            if (w.cap.b[2])
+                stages.push_back({cut_label("cut k=5..8", 2), [&] { launch_cut<CBig>(w, g, val, 2, grid[2], s); }});
+            stages.push_back({"extract: per-tet output scans", [&] { stage_rest(w, g, val, s, REST_SCANS); }});
+            stages.push_back(
+                {"extract k=1 occurrences (k_extract_k1)", [&] { stage_rest(w, g, val, s, REST_EXTRACT_K1); }});
+            stages.push_back({"extract record occurrences (k_extract_records)",
+                              [&] { stage_rest(w, g, val, s, REST_EXTRACT_REC); }});
+            stages.push_back({"dedup verts (star tables)", [&] { stage_rest(w, g, val, s, REST_DEDUP); }});
+            stages.push_back({"compute xyz (k_vert_fields_ia)", [&] { stage_rest(w, g, val, s, REST_XYZ); }});
+            stages.push_back({"faces", [&] { stage_rest(w, g, val, s, REST_FACES); }});
+            for (auto& st : stages) capture(st.first, st.second);
+            // the whole path: the k = 5..8 cut (few tets, long serial
+            // insertions) and the k = 2 table lookups on a branch beside the
+            // k = 3..4 cut; the cut kernels share no outputs
+            if (!w.side) {
+                RIN_HIP(hipStreamCreateWithFlags(&w.side, hipStreamNonBlocking));
+                RIN_HIP(hipEventCreateWithFlags(&w.fork, hipEventDisableTiming));
+                RIN_HIP(hipEventCreateWithFlags(&w.join, hipEventDisableTiming));
+            }
+            RIN_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+            stage_count(w, g, val, s);
+            stage_fill(w, g, val, s);
+            RIN_HIP(hipEventRecord(w.fork, s));
+            RIN_HIP(hipStreamWaitEvent(w.side, w.fork, 0));
+            if (w.cap.b[2]) launch_cut<CBig>(w, g, val, 2, grid[2], w.side);
+            if (w.cap.b[0]) launch_cut<CK2>(w, g, val, 0, grid[0], w.side);
+            RIN_HIP(hipEventRecord(w.join, w.side));
+            if (w.cap.b[1]) launch_cut<CMed>(w, g, val, 1, grid[1], s);
+            RIN_HIP(hipStreamWaitEvent(s, w.join, 0));
+            for (int part : {REST_SCANS, REST_EXTRACT_K1, REST_EXTRACT_REC, REST_DEDUP, REST_XYZ, REST_FACES})
+                stage_rest(w, g, val, s, part);
+            RIN_HIP(hipStreamEndCapture(s, &w.whole_graph));
+            RIN_HIP(hipGraphInstantiate(&w.whole, w.whole_graph, nullptr, nullptr, 0));
             while (w.ev.size() < w.parts.size() + 1) {
                 hipEvent_t e;
                 RIN_HIP(hipEventCreate(&e));
@@ -1601,9 +1856,14 @@ void run_batched(DevMem& ctx_mem, const Grid& g, Values& val, uint32_t flags, Ru
             w.hash_dirty = false;
         }
         RIN_HIP(hipEventRecord(w.ev[0], s));
-        for (size_t i = 0; i < w.parts.size(); ++i) {
-            RIN_HIP(hipGraphLaunch(w.parts[i].second, s));
-            RIN_HIP(hipEventRecord(w.ev[i + 1], s));
+        if (staged) {
+            for (size_t i = 0; i < w.parts.size(); ++i) {
+                RIN_HIP(hipGraphLaunch(w.parts[i].second, s));
+                RIN_HIP(hipEventRecord(w.ev[i + 1], s));
+            }
+        } else {
+            RIN_HIP(hipGraphLaunch(w.whole, s));
+            RIN_HIP(hipEventRecord(w.ev[1], s));
         }
         read_back();
         const uint32_t e = errors();
@@ -1621,10 +1881,16 @@ void run_batched(DevMem& ctx_mem, const Grid& g, Values& val, uint32_t flags, Ru
     }
     if (!ran) throw std::runtime_error("batched workspace did not converge");
     w.in_use = true;
-    for (size_t i = 0; i < w.parts.size(); ++i) {
+    if (staged) {
+        for (size_t i = 0; i < w.parts.size(); ++i) {
+            float ms = 0;
+            RIN_HIP(hipEventElapsedTime(&ms, w.ev[i], w.ev[i + 1]));
+            out.times.ms.push_back({w.parts[i].first, ms});
+        }
+    } else {
         float ms = 0;
-        RIN_HIP(hipEventElapsedTime(&ms, w.ev[i], w.ev[i + 1]));
-        out.times.ms.push_back({w.parts[i].first, ms});
+        RIN_HIP(hipEventElapsedTime(&ms, w.ev[0], w.ev[1]));
+        out.times.ms.push_back({"batched path (one graph)", ms});
     }
     // result views into the workspace
     out.mi = false;

@@ -188,16 +188,23 @@ __host__ __device__ inline int det_exp(const double (&M)[N][N], const int* cols,
     }
 }
 
+// The exact path is out of line and takes its matrix by value: the caller
+// builds the copy only when the filter has failed, so on the common path the
+// matrix stays in registers (a by-reference argument would give it an
+// address, and every classification would store it to scratch memory).
 template <int N>
-__host__ __device__ __noinline__ int det_exact(const double (&M)[N][N], bool& err) {
+struct ExactMat {
+    double m[N][N];
+};
+
+// sign of the determinant, or 2 when an expansion outgrows its capacity
+template <int N>
+__host__ __device__ __noinline__ int det_exact(ExactMat<N> M) {
     int cols[N];
     for (int c = 0; c < N; ++c) cols[c] = c;
     double out[kCap];
-    const int n = det_exp<N, N>(M, cols, out);
-    if (n < 0) {
-        err = true;
-        return 0;
-    }
+    const int n = det_exp<N, N>(M.m, cols, out);
+    if (n < 0) return 2;
     return exp_sign(out, n);
 }
 
@@ -214,10 +221,18 @@ __host__ __device__ inline int det_sign(double (&M)[N][N], bool& err) {
     // with every non-zero entry of the normalised rows at least 2^-100, every
     // partial product of the expansion has its lowest bit above 2^-760 and no
     // error-free transformation can underflow; smaller entries are reported
+    ExactMat<N> E;
     for (int r = 0; r < N; ++r)
-        for (int c = 0; c < N; ++c)
+        for (int c = 0; c < N; ++c) {
             if (M[r][c] != 0.0 && fabs(M[r][c]) < 0x1p-100) err = true;
-    return det_exact<N>(M, err);
+            E.m[r][c] = M[r][c];
+        }
+    const int sgn = det_exact<N>(E);
+    if (sgn == 2) {
+        err = true;
+        return 0;
+    }
+    return sgn;
 }
 
 // ---------------------------------------------------------------- IA
@@ -237,43 +252,45 @@ __host__ __device__ inline int orient_ia_edge(double f1, double f2, double q1, d
 }
 
 __host__ __device__ inline int orient_ia(const double (*P)[4], int t0, int t1, int t2, const double* q, bool& err) {
-    const int t[3] = {t0, t1, t2};
-    bool pinned[4] = {false, false, false, false};
-    int impl[3], ni = 0;
-    for (int i = 0; i < 3; ++i) {
-        if (t[i] < 4)
-            pinned[t[i]] = true;
-        else
-            impl[ni++] = t[i] - 4;
-    }
-    int fr[4], nf = 0;
-    for (int c = 0; c < 4; ++c)
-        if (!pinned[c]) fr[nf++] = c;
-    // nf == ni + 1
+    // pinned corners and implicit rows without indexed local arrays (which
+    // would live in scratch memory): the free corners come from a bit mask,
+    // and the row order of the implicit planes does not matter (it flips the
+    // signs of both determinants alike)
+    unsigned pm = 0;
+    if (t0 < 4) pm |= 1u << t0;
+    if (t1 < 4) pm |= 1u << t1;
+    if (t2 < 4) pm |= 1u << t2;
+    const int ni = (t0 >= 4) + (t1 >= 4) + (t2 >= 4);
+    // implicit ids in the order given
+    const int i0 = t0 >= 4 ? t0 - 4 : (t1 >= 4 ? t1 - 4 : t2 - 4);
+    const int i1 = (t0 >= 4 && t1 >= 4) ? t1 - 4 : t2 - 4;
+    const int i2 = t2 - 4;
+    unsigned fm = ~pm & 15u;
+    const int c0 = __builtin_ctz(fm | 16u);
+    fm &= fm - 1;
+    const int c1 = __builtin_ctz(fm | 16u);
+    fm &= fm - 1;
+    const int c2 = __builtin_ctz(fm | 16u);
     if (ni == 0) {
-        double v = q[fr[0]];
+        const double v = q[c0];
         return v > 0 ? 1 : (v < 0 ? -1 : 0);
     }
-    if (ni == 1) return orient_ia_edge(P[impl[0]][fr[0]], P[impl[0]][fr[1]], q[fr[0]], q[fr[1]], err);
+    if (ni == 1) return orient_ia_edge(P[i0][c0], P[i0][c1], q[c0], q[c1], err);
     if (ni == 2) {
-        double A[3][3], B[3][3];
-        for (int r = 0; r < 2; ++r)
-            for (int c = 0; c < 3; ++c) A[r][c] = B[r][c] = P[impl[r]][fr[c]];
-        for (int c = 0; c < 3; ++c) {
-            A[2][c] = q[fr[c]];
-            B[2][c] = 1.0;
-        }
+        double A[3][3] = {{P[i0][c0], P[i0][c1], P[i0][c2]}, {P[i1][c0], P[i1][c1], P[i1][c2]}, {q[c0], q[c1], q[c2]}};
+        double B[3][3] = {{P[i0][c0], P[i0][c1], P[i0][c2]}, {P[i1][c0], P[i1][c1], P[i1][c2]}, {1.0, 1.0, 1.0}};
         int a = det_sign<3>(A, err), b = det_sign<3>(B, err);
         if (b == 0) err = true;
         return a * b;
     }
-    double A[4][4], B[4][4];
-    for (int r = 0; r < 3; ++r)
-        for (int c = 0; c < 4; ++c) A[r][c] = B[r][c] = P[impl[r]][c];
-    for (int c = 0; c < 4; ++c) {
-        A[3][c] = q[c];
-        B[3][c] = 1.0;
-    }
+    double A[4][4] = {{P[i0][0], P[i0][1], P[i0][2], P[i0][3]},
+                      {P[i1][0], P[i1][1], P[i1][2], P[i1][3]},
+                      {P[i2][0], P[i2][1], P[i2][2], P[i2][3]},
+                      {q[0], q[1], q[2], q[3]}};
+    double B[4][4] = {{P[i0][0], P[i0][1], P[i0][2], P[i0][3]},
+                      {P[i1][0], P[i1][1], P[i1][2], P[i1][3]},
+                      {P[i2][0], P[i2][1], P[i2][2], P[i2][3]},
+                      {1.0, 1.0, 1.0, 1.0}};
     int a = det_sign<4>(A, err), b = det_sign<4>(B, err);
     if (b == 0) err = true;
     return a * b;

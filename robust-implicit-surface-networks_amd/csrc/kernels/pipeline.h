@@ -43,6 +43,7 @@ enum RunFlags : uint32_t {
     RUN_BATCHED = 8,     // bucketed fused form
     RUN_KEEP_RECORDS = 16,
     RUN_CUT_ONLY = 32,   // stop after the per-tet cut (robustness check)
+    RUN_STAGE_TIMES = 64,  // batched form: time every stage (one captured graph per stage)
 };
 
 struct StageTimes {

@@ -33,6 +33,11 @@ struct Count {
 // `in` and `out` may alias.  Works in place on up to 2^32-1 total.
 void scan_u32(DevMem& mem, const uint32_t* in, uint32_t* out, Count n, uint32_t* total, hipStream_t s);
 
+// Three exclusive scans at once: out[f][i] = sum(in[f][j * stride], j < i),
+// *total[f] (optional) = the sum.  Inputs and outputs must not alias.
+void scan3_u32(DevMem& mem, const uint32_t* const in[3], int stride, uint32_t* const out[3], Count n,
+               uint32_t* const total[3], hipStream_t s);
+
 // Stable sort of (key, val) pairs by key bits [0, bits).  Results are left in
 // keys/vals.  Uses scratch from `mem`.
 void sort_pairs(DevMem& mem, uint64_t* keys, uint32_t* vals, Count n, int bits, hipStream_t s);

@@ -79,6 +79,69 @@ __global__ void k_scan_apply(const uint32_t* in, uint32_t* out, Count cn, const 
     if (total && blockIdx.x == last_blk && threadIdx.x == kThreads - 1) *total = run;
 }
 
+// Three exclusive scans in one pass over in[f][i * stride], f = 0..2.
+struct Scan3 {
+    const uint32_t* in[3];
+    uint32_t* out[3];
+    uint32_t* total[3];
+    int stride;
+};
+
+__global__ void k_scan3_reduce(Scan3 S, Count cn, uint32_t* __restrict__ sums, uint32_t nb) {
+    const size_t n = cn.get();
+    const size_t base = (size_t)blockIdx.x * kTile;
+    uint32_t acc[3] = {0, 0, 0};
+    for (int i = 0; i < kItems; ++i) {
+        const size_t idx = base + (size_t)i * kThreads + threadIdx.x;
+        if (idx < n)
+            for (int f = 0; f < 3; ++f) acc[f] += S.in[f][idx * S.stride];
+    }
+    for (int f = 0; f < 3; ++f) {
+        uint32_t tot;
+        (void)block_excl_scan(acc[f], tot);
+        if (threadIdx.x == 0) sums[f * nb + blockIdx.x] = tot;
+    }
+}
+
+// one block per field: exclusive scan of that field's nb <= kTile tile sums
+__global__ void k_scan3_top(uint32_t* __restrict__ sums, uint32_t nb) {
+    uint32_t* p = sums + (size_t)blockIdx.x * nb;
+    const uint32_t base = threadIdx.x * kItems;
+    uint32_t v[kItems], acc = 0;
+    for (int i = 0; i < kItems; ++i) {
+        v[i] = base + i < nb ? p[base + i] : 0u;
+        acc += v[i];
+    }
+    uint32_t tot;
+    uint32_t run = block_excl_scan(acc, tot);
+    for (int i = 0; i < kItems; ++i) {
+        if (base + i < nb) p[base + i] = run;
+        run += v[i];
+    }
+}
+
+__global__ void k_scan3_apply(Scan3 S, Count cn, const uint32_t* __restrict__ sums, uint32_t nb) {
+    const size_t n = cn.get();
+    const size_t base = (size_t)blockIdx.x * kTile + (size_t)threadIdx.x * kItems;
+    const size_t last_blk = n ? (n - 1) / kTile : 0;
+    for (int f = 0; f < 3; ++f) {
+        uint32_t v[kItems], acc = 0;
+        for (int i = 0; i < kItems; ++i) {
+            const size_t idx = base + i;
+            v[i] = idx < n ? S.in[f][idx * S.stride] : 0u;
+            acc += v[i];
+        }
+        uint32_t tot;
+        uint32_t run = block_excl_scan(acc, tot) + sums[f * nb + blockIdx.x];
+        for (int i = 0; i < kItems; ++i) {
+            const size_t idx = base + i;
+            if (idx < n) S.out[f][idx] = run;
+            run += v[i];
+        }
+        if (S.total[f] && blockIdx.x == last_blk && threadIdx.x == kThreads - 1) *S.total[f] = run;
+    }
+}
+
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t n) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
 }
@@ -204,6 +267,35 @@ void scan_u32(DevMem& mem, const uint32_t* in, uint32_t* out, Count n, uint32_t*
     scan_u32(mem, sums.p, sums.p, Count(nb), nullptr, s);
     hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nb), dim3(kThreads), 0, s, in, out, n, (const uint32_t*)sums.p,
                        total);
+}
+
+void scan3_u32(DevMem& mem, const uint32_t* const in[3], int stride, uint32_t* const out[3], Count n,
+               uint32_t* const total[3], hipStream_t s) {
+    const size_t nb = (n.cap + kTile - 1) / kTile;
+    if (nb > (size_t)kTile) {  // beyond one top-level block: three separate scans
+        for (int f = 0; f < 3; ++f) {
+            if (stride != 1) throw std::runtime_error("strided scan larger than one top-level tile");
+            scan_u32(mem, in[f], out[f], n, total[f], s);
+        }
+        return;
+    }
+    Scan3 S;
+    for (int f = 0; f < 3; ++f) {
+        S.in[f] = in[f];
+        S.out[f] = out[f];
+        S.total[f] = total[f];
+    }
+    S.stride = stride;
+    if (n.cap == 0) {
+        for (int f = 0; f < 3; ++f)
+            if (total[f]) fill_u32(total[f], 0, 1, s);
+        return;
+    }
+    DBuf<uint32_t> sums(mem, 3 * nb);
+    hipLaunchKernelGGL(k_scan3_reduce, dim3((unsigned)nb), dim3(kThreads), 0, s, S, n, sums.p, (uint32_t)nb);
+    hipLaunchKernelGGL(k_scan3_top, dim3(3), dim3(kThreads), 0, s, sums.p, (uint32_t)nb);
+    hipLaunchKernelGGL(k_scan3_apply, dim3((unsigned)nb), dim3(kThreads), 0, s, S, n, (const uint32_t*)sums.p,
+                       (uint32_t)nb);
 }
 
 void sort_pairs(DevMem& mem, uint64_t* keys, uint32_t* vals, Count n, int bits, hipStream_t s) {
