@@ -46,7 +46,9 @@ struct WaveIA {
     uint16_t cface[CAND];
     uint8_t cedge[CAND], cfirst[CAND], cvid[CAND];
     uint8_t isov[MV];
-    uint8_t tv[64][MFV], te[64][MFV], tb[64][MFV];  // per-lane rows
+    uint8_t tv[64][MFV], te[64][MFV], tb[64][MFV];  // per-lane (or per split cell) rows
+    uint16_t tk[64][MFV];                            // collection order of a split cell's edges
+    uint32_t ccnt[64];                               // edges collected per split cell
     int nv, nf, nc, dup, err;
 };
 
@@ -355,52 +357,77 @@ __device__ void wave_ia_cut(WaveIA<C>& X, int k) {
             if (c != NIL && X.crank[c] != 0xFFFF) X.fc[g] = (uint8_t)(nc0 + X.crank[c]);
         }
         __syncthreads();
+        // the new face of each split cell, bounded by the edges on the plane
+        // of the faces on its positive side (fb == c: edge taken b -> a) and
+        // negative side (fc == c: a -> b).  Lanes take faces and file each
+        // such edge under its cell's row (the cell's split rank) with its
+        // (face, position) order; a lane per split cell then chains its row
+        // from the earliest edge, always to the earliest edge that continues
+        // it: the order of a scan of the faces one by one.
+        if (nsplit > 64) return fail();
+        for (int c = lane; c < (int)nsplit; c += 64) X.ccnt[c] = 0;
+        __syncthreads();
         bool cerr = false;
-        for (int c = lane; c < nc0 && !cerr; c += 64) {
-            if (X.crank[c] == 0xFFFF) continue;
-            uint8_t* ea = X.tv[lane];
-            uint8_t* eb = X.tb[lane];
-            uint8_t* el = X.te[lane];
-            int ne = 0;
-            for (int g = 0; g < nf1 && !cerr; ++g) {
-                const bool pos_side = X.fb[g] == c;
-                if (!pos_side && X.fc[g] != c) continue;
-                const int n = X.fn[g];
-                for (int i = 0; i < n; ++i) {
-                    const uint8_t a = X.fv[g][i], b = X.fv[g][i + 1 == n ? 0 : i + 1];
-                    if (X.s[a] != 0 || X.s[b] != 0) continue;
-                    if (ne >= C::MFV) {
+        for (int g = lane; g < nf1; g += 64) {
+            const uint8_t b = X.fb[g], cc = X.fc[g];
+            const int rb = (b < nc0 && X.crank[b] != 0xFFFF) ? (int)X.crank[b] : -1;
+            const int rc = (cc != NIL && cc < nc0 && X.crank[cc] != 0xFFFF) ? (int)X.crank[cc] : -1;
+            if (rb < 0 && rc < 0) continue;
+            const int n = X.fn[g];
+            for (int i = 0; i < n; ++i) {
+                const uint8_t a = X.fv[g][i], w = X.fv[g][i + 1 == n ? 0 : i + 1];
+                if (X.s[a] != 0 || X.s[w] != 0) continue;
+                for (int side = 0; side < 2; ++side) {
+                    const int row = side ? rc : rb;
+                    if (row < 0) continue;
+                    const uint32_t slot = atomicAdd(&X.ccnt[row], 1u);
+                    if (slot >= (uint32_t)C::MFV) {
                         cerr = true;
-                        break;
+                        continue;
                     }
-                    ea[ne] = pos_side ? b : a;
-                    eb[ne] = pos_side ? a : b;
-                    el[ne] = X.fa[g];
-                    ++ne;
+                    X.tv[row][slot] = side ? a : w;
+                    X.tb[row][slot] = side ? w : a;
+                    X.te[row][slot] = X.fa[g];
+                    X.tk[row][slot] = (uint16_t)(g << 4 | i);
                 }
             }
-            if (cerr || ne < 3) {
+        }
+        if (__ballot(cerr)) return fail();
+        __syncthreads();
+        for (int c = lane; c < nc0 && !cerr; c += 64) {
+            if (X.crank[c] == 0xFFFF) continue;
+            const int row = X.crank[c];
+            const int ne = (int)X.ccnt[row];
+            if (ne < 3) {
                 cerr = true;
                 break;
             }
-            const int F = nf1 + X.crank[c];
-            int cur = 0, cnt = 0;
+            const uint8_t* ea = X.tv[row];
+            const uint8_t* eb = X.tb[row];
+            const uint8_t* el = X.te[row];
+            const uint16_t* ek = X.tk[row];
+            int start = 0;
+            for (int q = 1; q < ne; ++q)
+                if (ek[q] < ek[start]) start = q;
+            const int F = nf1 + row;
+            int cur = start, cnt = 0;
             for (;;) {
+                if (cnt >= ne) {  // the chain does not close within the row
+                    cerr = true;
+                    break;
+                }
                 X.fv[F][cnt] = ea[cur];
                 X.fe[F][cnt] = el[cur];
                 ++cnt;
                 int nxt = -1;
                 for (int q = 0; q < ne; ++q)
-                    if (ea[q] == eb[cur]) {
-                        nxt = q;
-                        break;
-                    }
-                if (nxt < 0 || cnt > ne) {
+                    if (ea[q] == eb[cur] && (nxt < 0 || ek[q] < ek[nxt])) nxt = q;
+                if (nxt < 0) {
                     cerr = true;
                     break;
                 }
                 cur = nxt;
-                if (cur == 0) break;
+                if (cur == start) break;
             }
             if (cerr || cnt != ne) {
                 cerr = true;
@@ -409,7 +436,7 @@ __device__ void wave_ia_cut(WaveIA<C>& X, int k) {
             X.fn[F] = (uint8_t)cnt;
             X.fa[F] = pid;
             X.fb[F] = (uint8_t)c;
-            X.fc[F] = (uint8_t)(nc0 + X.crank[c]);
+            X.fc[F] = (uint8_t)(nc0 + row);
         }
         if (__ballot(cerr)) return fail();
         __syncthreads();
