@@ -41,7 +41,10 @@ constexpr uint32_t rec_words() {
 constexpr uint32_t K1_FLAG = 0x80000000u;
 constexpr uint32_t ERR_OVERFLOW = 8;
 
-// counters in the workspace (device)
+// Counters in the workspace (device).  Sums that many workgroups add to
+// go to kSpread words 16 apart (separate cache lines, so the adds do not all
+// queue on one address) and are folded into their counter at the end.
+constexpr int kSpreadBase = 64, kSpread = 64, kSpreadStride = 16;
 enum Ctr : int {
     C_ACTIVE = 0, C_FIT, C_VO, C_FO, C_FV, C_VERTS, C_FACES, C_FACEVERTS, C_B1, C_B2, C_B3, C_B5, C_BX, C_DEGEN,
     C_Q2, C_Q3, C_Q5, C_ABORT, C_TETFACES, C_RECW, C_DFAIL, C_N
@@ -398,11 +401,12 @@ __global__ void k_signs_count(const double* __restrict__ vals, uint64_t V, uint3
 //
 // The count pass and the fill pass each load a cube's eight corner sign
 // masks once for its five tets and need no per-tet intermediate arrays: the
-// count pass leaves one (active functions, active tets) pair per tile of
-// kCubeTile cubes, and each fill block sums the pairs of the tiles before
-// its own to place its tets in the CSR and the active list.
+// count pass leaves per tile of kCubeTile cubes its active functions, active
+// tets and bucket sizes, one workgroup scans those, and each fill workgroup
+// places its tets in the CSR, the active list and the bucket lists from its
+// tile's prefixes, in tet order and without atomics.
 
-constexpr int kCubeTile = 256;
+constexpr int kCubeTile = 256;  // cubes per tile (one workgroup)
 
 // pat (optional): per tet with one active function, its corner sign pattern
 // (sum of s_c 3^c, s_c = 0 negative, 1 zero, 2 positive), else 0
@@ -441,14 +445,22 @@ __device__ inline void cube_masks(const GridView& g, uint32_t cube, const uint64
     }
 }
 
-// per tile: active-function and active-tet sums; bucket histogram and the
-// totals by atomics
+// Per tile of cubes: active functions, active tets and tets per bucket
+// (k = 1, 2, 3..4, 5..8, > 8).  No atomics: the tile scan below turns the
+// tile sums into prefixes (the CSR offsets and the bucket-list slots of each
+// tile) and writes the totals, so the bucket lists come out in tet order.
+constexpr int kTileFields = 8;
+enum TileField { TF_FIT = 0, TF_ACTIVE, TF_B1, TF_B2, TF_B3, TF_B5, TF_BX };
+struct TileSums {
+    uint32_t v[kTileFields];
+};
+
 __global__ void __launch_bounds__(kCubeTile) k_cube_count(GridView g, const uint64_t* __restrict__ pos,
                                                           const uint64_t* __restrict__ neg, uint64_t full,
-                                                          uint2* __restrict__ tile_sums, uint32_t* __restrict__ ctr) {
+                                                          TileSums* __restrict__ tile_sums) {
     const uint32_t ncubes = (uint32_t)(g.T / 5);
     const uint32_t cube = blockIdx.x * kCubeTile + threadIdx.x;
-    uint32_t ks = 0, as = 0, hb[5] = {0, 0, 0, 0, 0};
+    uint32_t v[kTileFields] = {0, 0, 0, 0, 0, 0, 0, 0};
     if (cube < ncubes) {
         uint32_t c[8];
         uint64_t m[5];
@@ -456,83 +468,95 @@ __global__ void __launch_bounds__(kCubeTile) k_cube_count(GridView g, const uint
         cube_masks(g, cube, pos, neg, full, c, m, even);
         for (int q = 0; q < 5; ++q) {
             const int k = __popcll(m[q]);
-            ks += (uint32_t)k;
-            as += k != 0;
-            if (k) hb[bucket_of(k)]++;
+            v[TF_FIT] += (uint32_t)k;
+            v[TF_ACTIVE] += k != 0;
+            if (k) v[TF_B1 + bucket_of(k)]++;
         }
     }
-    __shared__ uint32_t red[kCubeTile / 64][7];
+    __shared__ uint32_t red[kCubeTile / 64][kTileFields];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    uint32_t v[7] = {ks, as, hb[0], hb[1], hb[2], hb[3], hb[4]};
-    for (int x = 0; x < 7; ++x) {
+    for (int x = 0; x < kTileFields - 1; ++x) {
         for (int d = 32; d > 0; d >>= 1) v[x] += __shfl_xor(v[x], d, 64);
         if (lane == 0) red[wave][x] = v[x];
     }
     __syncthreads();
-    if (threadIdx.x < 7) {
+    if (threadIdx.x < kTileFields) {
         uint32_t t = 0;
-        for (int w = 0; w < kCubeTile / 64; ++w) t += red[w][threadIdx.x];
-        const int x = threadIdx.x;
-        if (x == 0) tile_sums[blockIdx.x].x = t;
-        if (x == 1) tile_sums[blockIdx.x].y = t;
-        if (t) atomicAdd(&ctr[x == 0 ? C_FIT : (x == 1 ? C_ACTIVE : C_B1 + (x - 2))], t);
+        if (threadIdx.x < kTileFields - 1)
+            for (int w = 0; w < kCubeTile / 64; ++w) t += red[w][threadIdx.x];
+        tile_sums[blockIdx.x].v[threadIdx.x] = t;
     }
 }
 
-// exclusive prefix of the tile sums, in place (one workgroup, 2048 tiles
-// per pass)
-__global__ void __launch_bounds__(256) k_tile_scan(uint2* __restrict__ ts, uint32_t n) {
-    __shared__ uint32_t wx[4], wy[4];
+// Exclusive prefixes of the tile sums, in place, and the totals: CSR size,
+// active tets, the bucket histogram and the bucket-list sizes.  One
+// workgroup, 1024 tiles per pass, transposed through LDS: wave f scans field
+// f, each lane over 16 consecutive tiles.
+constexpr int kTileScanThreads = 1024;
+
+__global__ void __launch_bounds__(kTileScanThreads) k_tile_scan(TileSums* __restrict__ ts, uint32_t n,
+                                                                 uint32_t* __restrict__ ctr) {
+    constexpr int P = kTileScanThreads + kTileScanThreads / 16;  // a pad word every 16 against bank conflicts
+    __shared__ uint32_t sm[kTileFields][P];
+    __shared__ uint32_t carry[kTileFields];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    uint32_t cx = 0, cy = 0;
-    for (uint32_t base = 0; base < n; base += 2048) {
-        const uint32_t i0 = base + threadIdx.x * 8;
-        uint2 v[8];
-        uint32_t sx = 0, sy = 0;
-        for (int i = 0; i < 8; ++i) {
-            v[i] = i0 + i < n ? ts[i0 + i] : make_uint2(0, 0);
-            sx += v[i].x;
-            sy += v[i].y;
-        }
-        uint32_t ix = sx, iy = sy;
-        for (int d = 1; d < 64; d <<= 1) {
-            const uint32_t ox = __shfl_up(ix, d, 64), oy = __shfl_up(iy, d, 64);
-            if (lane >= d) {
-                ix += ox;
-                iy += oy;
+    if (threadIdx.x < kTileFields) carry[threadIdx.x] = 0;
+    for (uint32_t base = 0; base < n; base += kTileScanThreads) {
+        const uint32_t i = base + threadIdx.x;
+        const int pi = threadIdx.x + threadIdx.x / 16;
+        TileSums x = {};
+        if (i < n) x = ts[i];
+        for (int f = 0; f < kTileFields; ++f) sm[f][pi] = x.v[f];
+        __syncthreads();
+        if (wave < kTileFields) {
+            const int f = wave, e0 = 16 * lane, p0 = e0 + e0 / 16;
+            uint32_t sum = 0;
+            for (int e = 0; e < 16; ++e) sum += sm[f][p0 + e];
+            uint32_t inc = sum;
+            for (int d = 1; d < 64; d <<= 1) {
+                const uint32_t o = __shfl_up(inc, d, 64);
+                if (lane >= d) inc += o;
             }
-        }
-        if (lane == 63) {
-            wx[wave] = ix;
-            wy[wave] = iy;
+            uint32_t run = carry[f] + inc - sum;
+            for (int e = 0; e < 16; ++e) {
+                const uint32_t v = sm[f][p0 + e];
+                sm[f][p0 + e] = run;
+                run += v;
+            }
+            const uint32_t total = __shfl(inc, 63, 64);
+            if (lane == 0) carry[f] += total;
         }
         __syncthreads();
-        uint32_t rx = cx + ix - sx, ry = cy + iy - sy;
-        for (int w = 0; w < 4; ++w) {
-            if (w < wave) {
-                rx += wx[w];
-                ry += wy[w];
-            }
-            cx += wx[w];
-            cy += wy[w];
-        }
-        for (int i = 0; i < 8; ++i) {
-            if (i0 + i < n) ts[i0 + i] = make_uint2(rx, ry);
-            rx += v[i].x;
-            ry += v[i].y;
+        if (i < n) {
+            TileSums y;
+            for (int f = 0; f < kTileFields; ++f) y.v[f] = sm[f][pi];
+            ts[i] = y;
         }
         __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        ctr[C_FIT] = carry[TF_FIT];
+        ctr[C_ACTIVE] = carry[TF_ACTIVE];
+        ctr[C_B1] = carry[TF_B1];
+        ctr[C_B2] = carry[TF_B2];
+        ctr[C_B3] = carry[TF_B3];
+        ctr[C_B5] = carry[TF_B5];
+        ctr[C_BX] = carry[TF_BX];
+        ctr[C_Q2] = carry[TF_B2];
+        ctr[C_Q3] = carry[TF_B3];
+        ctr[C_Q5] = carry[TF_B5];
     }
 }
 
 // tile_sums: exclusive prefixes (k_tile_scan)
-__global__ void __launch_bounds__(kCubeTile) k_cube_fill(FillArgs A, const uint2* __restrict__ tile_sums) {
+__global__ void __launch_bounds__(kCubeTile) k_cube_fill(FillArgs A, const TileSums* __restrict__ tile_sums) {
     __shared__ K1Entry tab[81];
     __shared__ uint32_t wsum[kCubeTile / 64];
+    __shared__ uint32_t wbk[kCubeTile / 64];  // per-wave bucket counts, 10 bits per list
     __shared__ uint32_t s_off[5 * kCubeTile], s_rank[5 * kCubeTile];  // per-tet CSR entries of the tile
     for (int i = threadIdx.x; i < 81; i += blockDim.x) tab[i] = A.k1[i];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const uint2 pre = tile_sums[blockIdx.x];
+    const TileSums pre = tile_sums[blockIdx.x];
     const uint32_t ncubes = (uint32_t)(A.g.T / 5);
     const uint32_t cube = blockIdx.x * kCubeTile + threadIdx.x;
     uint32_t c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -549,24 +573,19 @@ __global__ void __launch_bounds__(kCubeTile) k_cube_fill(FillArgs A, const uint2
         as += k != 0;
         if (k >= 2 && k <= 8) bp += 1u << (10 * (k == 2 ? 0 : (k <= 4 ? 1 : 2)));
     }
-    // bucket slots: one atomic per list per wave
+    // bucket slots: the tile's prefix, the waves before this one, the lanes
+    // before this one (a wave holds at most 320 tets of a list)
     uint32_t binc = bp;
     for (int d = 1; d < 64; d <<= 1) {
         const uint32_t o = __shfl_up(binc, d, 64);
         if (lane >= d) binc += o;
     }
-    const uint32_t btot = __shfl(binc, 63, 64);
-    uint32_t bstart = 0;
-    if (lane < 3) {
-        const uint32_t n = (btot >> (10 * lane)) & 1023u;
-        if (n) bstart = atomicAdd(&A.ctr[C_Q2 + lane], n);
-    }
-    uint32_t bnext[3];
-    for (int qq = 0; qq < 3; ++qq)
-        bnext[qq] = __shfl(bstart, qq, 64) + (((binc - bp) >> (10 * qq)) & 1023u);
-    // block exclusive scan of (ks, as) packed as ks << 11 | as (a tile holds
-    // at most 1280 active tets and 81920 active functions)
-    const uint32_t packed = ks << 11 | as;
+    if (lane == 63) wbk[wave] = binc;
+    // block exclusive scan of (ks, as) packed as ks << kAsBits | as (a tile
+    // holds at most 5 kCubeTile active tets and 64 times as many functions)
+    constexpr int kAsBits = 13;
+    static_assert(5 * kCubeTile < (1 << kAsBits) && 5 * 64 * kCubeTile < (1 << (32 - kAsBits)), "tile packing");
+    const uint32_t packed = ks << kAsBits | as;
     uint32_t inc = packed;
     for (int d = 1; d < 64; d <<= 1) {
         const uint32_t o = __shfl_up(inc, d, 64);
@@ -576,7 +595,13 @@ __global__ void __launch_bounds__(kCubeTile) k_cube_fill(FillArgs A, const uint2
     __syncthreads();
     uint32_t ex = inc - packed;
     for (int w = 0; w < wave; ++w) ex += wsum[w];
-    uint32_t o = pre.x + (ex >> 11), r = pre.y + (ex & 2047u);
+    uint32_t o = pre.v[TF_FIT] + (ex >> kAsBits), r = pre.v[TF_ACTIVE] + (ex & ((1u << kAsBits) - 1u));
+    uint32_t bnext[3];
+    for (int qq = 0; qq < 3; ++qq) {
+        uint32_t b = pre.v[TF_B2 + qq] + (((binc - bp) >> (10 * qq)) & 1023u);
+        for (int w = 0; w < wave; ++w) b += (wbk[w] >> (10 * qq)) & 1023u;
+        bnext[qq] = b;
+    }
     for (int q = 0; q < 5; ++q) {
         const uint64_t t = (uint64_t)cube * 5 + q;
         const uint64_t mq = m[q];
@@ -809,6 +834,8 @@ struct ExtractArgs {
     uint32_t *fq_vs, *fve, *errs;
     const uint32_t* list[3];  // bucket lists (k = 2, 3..4, 5..8) of active ranks
     uint32_t list_cap[3];
+    uint32_t slot_words[3];   // words per record slot of each bucket
+    uint32_t fo_table[3];     // 1: the bucket's cut leaves face offsets after a record that fits
 };
 
 // Extraction in two passes so that waves stay uniform: the table pass
@@ -954,17 +981,17 @@ __global__ void k_extract_records(ExtractArgs A) {
         }
     }
     for (int d = 32; d > 0; d >>= 1) recw += __shfl_xor(recw, d, 64);
-    if ((threadIdx.x & 63) == 0 && recw) atomicAdd(A.ctr + C_RECW, recw);
+    if ((threadIdx.x & 63) == 0 && recw)
+        atomicAdd(A.ctr + kSpreadBase + kSpreadStride * ((blockIdx.x * 4 + (threadIdx.x >> 6)) % kSpread), recw);
 }
 
-// k = 3..8 records: one wave per tet.  The record is copied to LDS in
-// parallel, one lane walks the variable-length face list there for the face
-// offsets, then the lanes take faces and vertices in parallel; occurrences
-// and face rows come out in the same order as one lane per tet writes them.
-constexpr uint32_t kWaveRecWords = rec_words<CBig>();
-
+// k = 3..8 records: one wave per tet, lanes over faces and vertices.  The
+// wave cut leaves each record's face offsets after it, so lanes read their
+// faces in place; a record without them (thread-form cut) gets its offsets
+// from one lane's walk of the face list.  LDS holds only the offsets and
+// the iso flags, so many waves stay resident to cover the loads.
+// Occurrences and face rows come out in the order one lane per tet gives.
 __global__ void __launch_bounds__(64) k_extract_records_wave(ExtractArgs A) {
-    __shared__ uint16_t W[kWaveRecWords];
     __shared__ uint16_t fo[CBig::MF];
     __shared__ uint32_t isov[(CBig::MV + 31) / 32];
     __shared__ uint8_t isof[CBig::MF];
@@ -973,23 +1000,24 @@ __global__ void __launch_bounds__(64) k_extract_records_wave(ExtractArgs A) {
     const uint32_t i = blockIdx.x;
     if (i >= n1 + n2) return;
     // the larger complexes first
-    const uint32_t a = i < n2 ? A.list[2][i] : A.list[1][i - n2];
+    const int bq = i < n2 ? 2 : 1;
+    const uint32_t a = bq == 2 ? A.list[2][i] : A.list[1][i - n2];
     const int lane = threadIdx.x;
     const uint32_t loc = A.rec_loc[a];
-    const uint16_t* src = (loc & K1_FLAG) ? A.tab + A.tab_off[loc & ~K1_FLAG] : A.rec + loc;
-    if (src[0] != 1) return;
-    const uint32_t len = src[6];
-    const int nv = src[2], nf = src[3], L = src[1];
-    if (len > kWaveRecWords || nv > CBig::MV || nf > CBig::MF || len < 8u + 4u * nv) {
+    const uint16_t* W = (loc & K1_FLAG) ? A.tab + A.tab_off[loc & ~K1_FLAG] : A.rec + loc;
+    if (W[0] != 1) return;
+    const uint32_t len = W[6];
+    const int nv = W[2], nf = W[3], L = W[1];
+    if (nv > CBig::MV || nf > CBig::MF || len < 8u + 4u * nv) {
         if (lane == 0) A.errs[0] |= ERR_RECORD;
         return;
     }
-    for (uint32_t x = lane; x < len; x += 64) W[x] = src[x];
     for (int x = lane; x < (nv + 31) / 32; x += 64) isov[x] = 0;
-    if (lane == 0) atomicAdd(A.ctr + C_RECW, len);
-    __syncthreads();
+    if (lane == 0) atomicAdd(A.ctr + kSpreadBase + kSpreadStride * (blockIdx.x % kSpread), len);
     const bool dup = W[5] & 1;
-    if (lane == 0) {
+    if (!(loc & K1_FLAG) && A.fo_table[bq] && len + nf <= A.slot_words[bq]) {
+        for (int f = lane; f < nf; f += 64) fo[f] = W[len + f];
+    } else if (lane == 0) {
         uint32_t off = 8 + 4 * nv;
         for (int f = 0; f < nf; ++f) {
             fo[f] = (uint16_t)off;
@@ -1089,9 +1117,12 @@ __global__ void __launch_bounds__(64) k_extract_records_wave(ExtractArgs A) {
     }
 }
 
-__global__ void k_write_totals(const uint32_t* ctr, uint64_t T, uint32_t* csr_off, uint32_t* f_vstart,
+__global__ void k_write_totals(uint32_t* ctr, uint64_t T, uint32_t* csr_off, uint32_t* f_vstart,
                                uint32_t* f_tfstart, uint32_t cap_fo) {
+    uint32_t recw = threadIdx.x < kSpread ? ctr[kSpreadBase + kSpreadStride * threadIdx.x] : 0u;
+    for (int d = 32; d > 0; d >>= 1) recw += __shfl_xor(recw, d, 64);
     if (threadIdx.x == 0) {
+        ctr[C_RECW] = recw;
         csr_off[T] = ctr[C_FIT];
         const uint32_t nf = ctr[C_ABORT] ? 0u : min(ctr[C_FACES], cap_fo);
         f_vstart[nf] = ctr[C_FACEVERTS];
@@ -1102,6 +1133,7 @@ __global__ void k_write_totals(const uint32_t* ctr, uint64_t T, uint32_t* csr_of
 __global__ void k_zero_ctr(uint32_t* ctr, int n, uint32_t* errs) {
     if (threadIdx.x < (unsigned)n) ctr[threadIdx.x] = 0;
     if (threadIdx.x < 64) errs[threadIdx.x] = 0;
+    if (threadIdx.x < kSpread) ctr[kSpreadBase + kSpreadStride * threadIdx.x] = 0;
 }
 
 // the counters the host reads back after a replay
@@ -1126,9 +1158,10 @@ struct Workspace {
     // words per record slot: the largest record a bucket's capacity allows
     // (header, vertices, faces with their cycles, coincidence groups)
     uint32_t slot[3] = {rec_words<CK2>(), rec_words<CMed>(), rec_words<CBig>()};
+    bool wave_cut[3] = {false, false, false};  // bucket cut by the wave form (records carry face offsets)
     DBuf<uint64_t> pos, neg;
     DBuf<uint32_t> kcnt, isact, csr_off, rank, act, rec_loc, cnt, list[3];
-    DBuf<uint2> tile_sums;  // regular grids: per tile of cubes
+    DBuf<TileSums> tile_sums;  // regular grids: per tile of cubes
     DBuf<uint8_t> fit, act_k;
     DBuf<uint16_t> rec;
     DBuf<uint32_t> cv, cf, cfv;
@@ -1336,7 +1369,7 @@ void alloc_ws(Workspace& W, const Grid& g, const Caps_& c) {
     W.f_tf_tet.alloc(m, c.fo + 1);
     W.f_tf_local.alloc(m, c.fo + 1);
     W.f_func.alloc(m, 2 * (uint64_t)c.fo + 2);
-    W.ctr.alloc(m, 32);
+    W.ctr.alloc(m, kSpreadBase + kSpread * kSpreadStride);
     W.errs.alloc(m, 64);
     const K1Table& K = k1_table();
     const K2Table& K2 = k2_table();
@@ -1368,7 +1401,8 @@ void stage_count(Workspace& W, const Grid& g, const Values& val, hipStream_t s) 
     const uint64_t full = (F == 64) ? ~0ull : ((1ull << F) - 1);
     if (g.view.regular) {
         hipLaunchKernelGGL(k_cube_count, dim3(cube_tiles(g)), dim3(kCubeTile), 0, s, g.view, (const uint64_t*)W.pos.p,
-                           (const uint64_t*)W.neg.p, full, W.tile_sums.p, W.ctr.p);
+                           (const uint64_t*)W.neg.p, full, W.tile_sums.p);
+        hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(kTileScanThreads), 0, s, W.tile_sums.p, cube_tiles(g), W.ctr.p);
         return;
     }
     hipLaunchKernelGGL(k_filter_count, dim3(nblk(T)), dim3(256), 0, s, g.view, (const uint64_t*)W.pos.p,
@@ -1462,8 +1496,8 @@ void stage_fill(Workspace& W, const Grid& g, const Values& val, hipStream_t s) {
     A.csr_off_w = W.csr_off.p;
     A.rank_w = W.rank.p;
     if (g.view.regular) {
-        hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(256), 0, s, W.tile_sums.p, cube_tiles(g));
-        hipLaunchKernelGGL(k_cube_fill, dim3(cube_tiles(g)), dim3(kCubeTile), 0, s, A, (const uint2*)W.tile_sums.p);
+        hipLaunchKernelGGL(k_cube_fill, dim3(cube_tiles(g)), dim3(kCubeTile), 0, s, A,
+                           (const TileSums*)W.tile_sums.p);
         return;
     }
     hipLaunchKernelGGL(k_fill_batched, dim3(nblk(T)), dim3(256), 0, s, A);
@@ -1486,7 +1520,8 @@ enum RestPart {
 // bounded, so that capacity-sized launches do not pay for idle workgroups
 unsigned cgrid(uint64_t cap) { return std::min(nblk(cap), 2048u); }
 
-void stage_rest(Workspace& W, const Grid& g, const Values& val, hipStream_t s, int part) {
+// side: a second stream for the k = 2 record extraction (null: s)
+void stage_rest(Workspace& W, const Grid& g, const Values& val, hipStream_t s, int part, hipStream_t side = nullptr) {
     DevMem& m = W.mem;
     const Caps_& c = W.cap;
     uint32_t* ctr = W.ctr.p;
@@ -1532,11 +1567,13 @@ void stage_rest(Workspace& W, const Grid& g, const Values& val, hipStream_t s, i
         for (int q = 0; q < 3; ++q) {
             E.list[q] = W.list[q].p;
             E.list_cap[q] = c.b[q];
+            E.slot_words[q] = W.slot[q];
+            E.fo_table[q] = W.wave_cut[q] ? 1u : 0u;
         }
         if (part & REST_EXTRACT_K1) hipLaunchKernelGGL(k_extract_k1, dim3(cgrid(c.active)), dim3(256), 0, s, E);
         if (part & REST_EXTRACT_REC) {
             hipLaunchKernelGGL(k_extract_records_wave, dim3(std::max(1u, c.b[1] + c.b[2])), dim3(64), 0, s, E);
-            hipLaunchKernelGGL(k_extract_records, dim3(cgrid(c.b[0])), dim3(256), 0, s, E);
+            hipLaunchKernelGGL(k_extract_records, dim3(cgrid(c.b[0])), dim3(256), 0, side ? side : s, E);
         }
     }
 
@@ -1637,7 +1674,7 @@ void stage_rest(Workspace& W, const Grid& g, const Values& val, hipStream_t s, i
         FO.f_tf_local = W.f_tf_local.p;
         FO.f_func = W.f_func.p;
         hipLaunchKernelGGL(k_face_emit, dim3(cgrid(c.fo)), dim3(256), 0, s, FO);
-        hipLaunchKernelGGL(k_write_totals, dim3(1), dim3(64), 0, s, (const uint32_t*)ctr, g.view.T, W.csr_off.p,
+        hipLaunchKernelGGL(k_write_totals, dim3(1), dim3(64), 0, s, ctr, g.view.T, W.csr_off.p,
                            W.f_vstart.p, W.f_tfstart.p, c.fo);
     }
 }
@@ -1784,8 +1821,10 @@ void run_batched(DevMem& ctx_mem, const Grid& g, Values& val, uint32_t flags, Ru
                 }
             }
             int grid[3];
-            for (int q = 0; q < 3; ++q)
+            for (int q = 0; q < 3; ++q) {
                 grid[q] = P.blocks_per_cu[q] < 0 ? P.blocks_per_cu[q] : std::max(1, n_cu * P.blocks_per_cu[q]);
+                w.wave_cut[q] = grid[q] == -1;
+            }
             // capture the whole path
             RIN_HIP(hipStreamSynchronize(s));
             auto capture = [&](const std::string& label, const std::function<void()>& body) {
@@ -1839,8 +1878,15 @@ void run_batched(DevMem& ctx_mem, const Grid& g, Values& val, uint32_t flags, Ru
             RIN_HIP(hipEventRecord(w.join, w.side));
             if (w.cap.b[1]) launch_cut<CMed>(w, g, val, 1, grid[1], s);
             RIN_HIP(hipStreamWaitEvent(s, w.join, 0));
-            for (int part : {REST_SCANS, REST_EXTRACT_K1, REST_EXTRACT_REC, REST_DEDUP, REST_XYZ, REST_FACES})
-                stage_rest(w, g, val, s, part);
+            // extraction: the three passes write disjoint rows, so the k = 2
+            // record pass runs on the branch beside the other two
+            stage_rest(w, g, val, s, REST_SCANS);
+            RIN_HIP(hipEventRecord(w.fork, s));
+            RIN_HIP(hipStreamWaitEvent(w.side, w.fork, 0));
+            stage_rest(w, g, val, s, REST_EXTRACT_K1 | REST_EXTRACT_REC, w.side);
+            RIN_HIP(hipEventRecord(w.join, w.side));
+            RIN_HIP(hipStreamWaitEvent(s, w.join, 0));
+            for (int part : {REST_DEDUP, REST_XYZ, REST_FACES}) stage_rest(w, g, val, s, part);
             RIN_HIP(hipStreamEndCapture(s, &w.whole_graph));
             RIN_HIP(hipGraphInstantiate(&w.whole, w.whole_graph, nullptr, nullptr, 0));
             while (w.ev.size() < w.parts.size() + 1) {

@@ -451,7 +451,10 @@ __device__ void wave_ia_cut(WaveIA<C>& X, int k) {
 
 // Writes the record (include/rin/cut_record.h layout) with the whole wave and
 // returns its length, or -1 when it exceeds cap; niv / nif / nfv are the
-// tet's iso-vertex, iso-face and iso-face-corner counts (uniform).
+// tet's iso-vertex, iso-face and iso-face-corner counts (uniform).  When the
+// slot has room, the word offset of every face within the record follows the
+// record (nf words past its length), so that readers need not walk the
+// variable-length face list.
 template <class C>
 __device__ int wave_ia_record(WaveIA<C>& X, int k, uint16_t* out, int cap, uint32_t& niv, uint32_t& nif,
                               uint32_t& nfv) {
@@ -502,6 +505,8 @@ __device__ int wave_ia_record(WaveIA<C>& X, int k, uint16_t* out, int cap, uint3
             out[fb0 + fwords + l] = X.grp[l];
             out[fb0 + fwords + L + l] = X.ori[l];
         }
+    if (total + nf <= cap)
+        for (int f = lane; f < nf; f += 64) out[total + f] = (uint16_t)(fb0 + X.foff[f]);
     if (lane == 0) {
         out[0] = 1;
         out[1] = (uint16_t)L;

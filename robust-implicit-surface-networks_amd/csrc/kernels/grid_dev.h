@@ -33,20 +33,16 @@ constexpr uint8_t kCubeTetEven[5][4] = {{4, 6, 1, 3}, {6, 3, 4, 7}, {1, 3, 0, 4}
 constexpr uint8_t kCubeTetOdd[5][4] = {{7, 0, 2, 5}, {2, 3, 0, 7}, {5, 7, 0, 4}, {7, 2, 6, 5}, {0, 1, 2, 5}};
 
 __host__ __device__ inline void regular_tet(int res, uint64_t t, uint32_t v[4]) {
-    const uint64_t R = (uint64_t)res, N = R + 1;
-    const uint64_t cube = t / 5;
-    const int q = (int)(t % 5);
-    const uint64_t i = cube / (R * R), j = (cube / R) % R, k = cube % R;
-    const uint64_t c[8] = {i * N * N + j * N + k,
-                           (i + 1) * N * N + j * N + k,
-                           (i + 1) * N * N + (j + 1) * N + k,
-                           i * N * N + (j + 1) * N + k,
-                           i * N * N + j * N + k + 1,
-                           (i + 1) * N * N + j * N + k + 1,
-                           (i + 1) * N * N + (j + 1) * N + k + 1,
-                           i * N * N + (j + 1) * N + k + 1};
-    const bool ev = ((i + j + k) % 2) == 0;
-    for (int a = 0; a < 4; ++a) v[a] = (uint32_t)c[ev ? kCubeTetEven[q][a] : kCubeTetOdd[q][a]];
+    // 32-bit index arithmetic: a regular grid has fewer than 2^32 / 27
+    // vertices, so its cube and vertex indices fit
+    const uint32_t R = (uint32_t)res, N = R + 1;
+    const uint32_t t32 = (uint32_t)t, cube = t32 / 5u;
+    const int q = (int)(t32 - cube * 5u);
+    const uint32_t i = cube / (R * R), rem = cube - i * (R * R), j = rem / R, k = rem - j * R;
+    const uint32_t b = i * N * N + j * N + k;
+    const uint32_t c[8] = {b, b + N * N, b + N * N + N, b + N, b + 1, b + N * N + 1, b + N * N + N + 1, b + N + 1};
+    const bool ev = ((i + j + k) & 1u) == 0;
+    for (int a = 0; a < 4; ++a) v[a] = c[ev ? kCubeTetEven[q][a] : kCubeTetOdd[q][a]];
 }
 
 __host__ __device__ inline void tet_corners(const GridView& g, uint64_t t, uint32_t v[4]) {
@@ -57,12 +53,23 @@ __host__ __device__ inline void tet_corners(const GridView& g, uint64_t t, uint3
     }
 }
 
-// lattice direction code of b - a in {-1,0,1}^3, 0..26
+// lattice direction code of b - a in {-1,0,1}^3, 0..26 (a and b corners of
+// one cube).  The offsets come from the index difference alone: with N >= 3,
+// b - a = di N^2 + dj N + dk splits at N^2 / 2 and N / 2.
 __host__ __device__ inline uint32_t lattice_dir(int res, uint32_t a, uint32_t b) {
     const int64_t N = res + 1;
-    const int64_t ai = a / (N * N), aj = (a / N) % N, ak = a % N;
-    const int64_t bi = b / (N * N), bj = (b / N) % N, bk = b % N;
-    return (uint32_t)((bi - ai + 1) * 9 + (bj - aj + 1) * 3 + (bk - ak + 1));
+    if (N < 3) {
+        const int64_t ai = a / (N * N), aj = (a / N) % N, ak = a % N;
+        const int64_t bi = b / (N * N), bj = (b / N) % N, bk = b % N;
+        return (uint32_t)((bi - ai + 1) * 9 + (bj - aj + 1) * 3 + (bk - ak + 1));
+    }
+    const int64_t NN = N * N;
+    int64_t d = (int64_t)b - (int64_t)a;
+    const int64_t di = 2 * d > NN ? 1 : (2 * d < -NN ? -1 : 0);
+    d -= di * NN;
+    const int64_t dj = 2 * d > N ? 1 : (2 * d < -N ? -1 : 0);
+    d -= dj * N;
+    return (uint32_t)((di + 1) * 9 + (dj + 1) * 3 + (d + 1));
 }
 
 __host__ __device__ inline int local_edge(int a, int b) {
