@@ -46,20 +46,39 @@ def path_bytes(V, F, s):
     return b
 
 
-def extract_bytes(s):
-    """k_extract_batched, per launch: per active tet 24 B of indices (tet,
-    record location, CSR offset, three output offsets) and its function ids;
-    the cut records it reads (2 B a word; k = 1 tets read an LDS table); per
-    vertex occurrence a 14 B (key, active rank, local index) row; per face
-    occurrence a 12 B row and 4 B per face corner."""
-    return (24 * s.num_intersecting + s.n_func_in_tet + 2 * s.n_rec_words + 14 * s.n_vert_occ +
-            12 * s.n_face_occ + 4 * s.n_face_occ_verts)
+def cut_bytes(bucket):
+    """The per-bucket cut kernel (k = 3..4 or 5..8), per launch: per tet of the
+    bucket 17 B of indices read (list entry, active count, tet, CSR offset,
+    record location), its k function ids (1 B) and 4k corner values (8 B), and
+    12 B of counts written; the record words it writes (2 B a word, face
+    offsets included)."""
+
+    def f(s):
+        n = s.n_cut34 if bucket == 0 else s.n_cut58
+        k = s.cut_k34 if bucket == 0 else s.cut_k58
+        w = s.cut_words34 if bucket == 0 else s.cut_words58
+        return 29 * n + 33 * k + 2 * w
+    return f
 
 
-# single-kernel stages with an algorithmic byte count: stage label -> (kernel, bytes(sizes))
+# single-kernel stages with an algorithmic byte count:
+# stage label prefix -> {form marker in the label: (kernel, file tag, bytes(sizes))}
 KERNEL_STAGES = {
-    "extract record occurrences (k_extract_records)": ("k_extract_records", extract_bytes),
+    "cut k=3..4": {"wave per tet": ("k_cut_wave<Caps<40,64,8,8>>", "k_cut_wave_k3to4", cut_bytes(0)),
+                   "thread per tet": ("k_cut_bucket<Caps<40,64,8,8>>", "k_cut_bucket_k3to4", cut_bytes(0))},
+    "cut k=5..8": {"wave per tet": ("k_cut_wave<Caps<132,284,10,12>>", "k_cut_wave_k5to8", cut_bytes(1)),
+                   "thread per tet": ("k_cut_bucket<Caps<132,284,10,12>>", "k_cut_bucket_k5to8", cut_bytes(1))},
 }
+
+
+def kernel_of_stage(label):
+    """(kernel, file tag, bytes fn) of a single-kernel stage label, or None."""
+    for prefix, forms in KERNEL_STAGES.items():
+        if label.startswith(prefix):
+            for marker, v in forms.items():
+                if marker in label:
+                    return v
+    return None
 
 
 def cpu_baseline(res, fs):
@@ -80,9 +99,9 @@ def cpu_baseline(res, fs):
             "stage_s": {k: round(v, 4) for k, v in r.timings.items()}}
 
 
-def traffic_for(kernel):
+def traffic_for(tag):
     """HBM bytes per launch from the committed counter summary, or None."""
-    p = os.path.join(ROOT, "profiles", "traffic_%s_%s.json" % (kernel, WORKLOAD))
+    p = os.path.join(ROOT, "profiles", "traffic_%s_%s.json" % (tag, WORKLOAD))
     if not os.path.exists(p):
         return None, None
     d = json.load(open(p))
@@ -90,9 +109,13 @@ def traffic_for(kernel):
 
 
 def _short(name):
+    """Kernel name without namespaces, argument list and cap types' extra
+    parameters: 'k_cut_wave<Caps<40,64,8,8>>'."""
     name = re.sub(r"rin::\(anonymous namespace\)::", "", name)
-    name = re.sub(r"rin::dev::Caps<(\d+), (\d+), (\d+), (\d+) >", r"Caps<\1,\2,\3,\4>", name)
-    return re.sub(r"\(.*\)$", "", name).strip()
+    name = re.sub(r"^void ", "", name.strip())
+    name = re.sub(r"rin::dev::Caps<(\d+), (\d+), (\d+), (\d+)[^>]*>", r"Caps<\1,\2,\3,\4>", name)
+    name = re.sub(r"\(.*\)$", "", name).strip()
+    return re.sub(r"\s*>\s*>", ">>", name)
 
 
 def pmc_summary(fetch_csv, write_csv, trace_csv, copy_bytes, command):
@@ -134,7 +157,7 @@ def pmc_summary(fetch_csv, write_csv, trace_csv, copy_bytes, command):
         lines.append("| %s | %d | %.0f | %.0f | %.2f | %.1f |" % (k, n, f, w, mb * 1024 / 1e6, us))
     md = os.path.join(ROOT, "profiles", "hbm_traffic_%s.md" % WORKLOAD)
     open(md, "w").write("\n".join(lines) + "\n")
-    for kernel in {v[0] for v in KERNEL_STAGES.values()}:
+    for kernel, tag in {(v[0], v[1]) for forms in KERNEL_STAGES.values() for v in forms.values()}:
         if kernel not in fetch:
             continue
         f = sum(fetch[kernel]) / len(fetch[kernel]) * 1024 * sf
@@ -142,7 +165,7 @@ def pmc_summary(fetch_csv, write_csv, trace_csv, copy_bytes, command):
         json.dump({"kernel": kernel, "workload": WORKLOAD, "fetch_bytes_per_launch": f, "write_bytes_per_launch": w,
                    "hbm_bytes_per_launch": f + w, "copy_scale_fetch": sf, "copy_scale_write": sw,
                    "source": os.path.relpath(md, ROOT)},
-                  open(os.path.join(ROOT, "profiles", "traffic_%s_%s.json" % (kernel, WORKLOAD)), "w"), indent=1)
+                  open(os.path.join(ROOT, "profiles", "traffic_%s_%s.json" % (tag, WORKLOAD)), "w"), indent=1)
     print(open(md).read())
 
 
@@ -253,18 +276,21 @@ def main():
             print(name, json.dumps({k: round(v, 4) for k, v in r["stages_ms"].items()}), file=sys.stderr)
     dom_stage = max(stages, key=stages.get) if stages else None
     # dominant single-kernel stage with an algorithmic byte count
-    kstages = [k for k in stages if k in KERNEL_STAGES]
+    kstages = [k for k in stages if kernel_of_stage(k)]
     roof = None
     if kstages:
         ks = max(kstages, key=stages.get)
-        kernel, fbytes = KERNEL_STAGES[ks]
+        kernel, tag, fbytes = kernel_of_stage(ks)
         B = fbytes(sizes)
         achieved = B / (stages[ks] * 1e-3) / 1e9
-        traffic, tsrc = traffic_for(kernel)
+        traffic, tsrc = traffic_for(tag)
         roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": kernel,
                 "kernel_ms": stages[ks], "algorithmic_bytes": B, "traffic_source": tsrc,
-                "timing": "stream events around the kernel's own captured graph, every timed step"}
+                "timing": "stream events around the kernel's own captured graph, in per-stage replays after "
+                          "the timed steps",
+                "note": "the cut runs exact-predicate insertions in LDS, one wavefront per tet; it is bound by "
+                        "instruction issue and latency, not HBM, so frac is small by nature"}
     pb = path_bytes(V, F, sizes)
     out = {
         "metric": METRIC,

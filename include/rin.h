@@ -73,6 +73,11 @@ typedef struct rin_sizes {
     uint64_t n_face_occ_verts;
     uint64_t n_cut[2];        /* active tets with 3..4 and 5..8 functions */
     uint64_t n_rec_words;     /* cut-record words read by the extraction (batched form; 0 otherwise) */
+    /* batched form, cut buckets 3..4 and 5..8 (0 otherwise): active functions
+     * summed over the bucket's tets, and the words its cut wrote (records and
+     * their face offsets) */
+    uint64_t cut_k_sum[2];
+    uint64_t cut_words[2];
 } rin_sizes;
 
 /* Context: one HIP stream and a device-memory cache on `device`.

@@ -43,11 +43,18 @@ constexpr uint32_t ERR_OVERFLOW = 8;
 
 // Counters in the workspace (device).  Sums that many workgroups add to
 // go to kSpread words 16 apart (separate cache lines, so the adds do not all
-// queue on one address) and are folded into their counter at the end.
+// queue on one address) and are folded into their counter at the end:
+// record words read by the extraction, and per cut bucket (k = 3..4, 5..8)
+// the active functions and the record words written.
 constexpr int kSpreadBase = 64, kSpread = 64, kSpreadStride = 16;
+enum SpreadCounter { SP_RECW = 0, SP_CUTK3, SP_CUTW3, SP_CUTK5, SP_CUTW5, SP_N };
+constexpr int kQueueBase = 40;  // work counters of the thread-form cuts
+__host__ __device__ inline uint32_t spread_at(int counter, uint32_t slot) {
+    return kSpreadBase + ((uint32_t)counter * kSpread + slot % kSpread) * kSpreadStride;
+}
 enum Ctr : int {
     C_ACTIVE = 0, C_FIT, C_VO, C_FO, C_FV, C_VERTS, C_FACES, C_FACEVERTS, C_B1, C_B2, C_B3, C_B5, C_BX, C_DEGEN,
-    C_Q2, C_Q3, C_Q5, C_ABORT, C_TETFACES, C_RECW, C_DFAIL, C_N
+    C_Q2, C_Q3, C_Q5, C_ABORT, C_TETFACES, C_RECW, C_DFAIL, C_CUTK3, C_CUTW3, C_CUTK5, C_CUTW5, C_N
 };
 
 struct K1Entry {
@@ -662,6 +669,8 @@ struct CutArgs {
     uint32_t cap, slot;
     uint16_t* rec;
     uint32_t *cnt, *errs;
+    uint32_t* ctr;       // for the work statistics (spread counters)
+    int stat_k, stat_w;  // spread counters of this bucket's functions and words (-1: none)
 };
 
 // persistent per-bucket cut: each wave takes 64 bucket entries at a time
@@ -697,7 +706,12 @@ __global__ void __launch_bounds__(64) k_cut_bucket(CutArgs A) {
             A.cnt[3 * r] = A.cnt[3 * r + 1] = A.cnt[3 * r + 2] = 0;
             continue;
         }
-        if (dev::write_record_ia<C>(X, k, out, (int)A.slot) < 0) A.errs[lane] |= 4;
+        const int len = dev::write_record_ia<C>(X, k, out, (int)A.slot);
+        if (len < 0) A.errs[lane] |= 4;
+        if (A.stat_k >= 0 && len > 0) {
+            atomicAdd(A.ctr + spread_at(A.stat_k, blockIdx.x * 64 + lane), (uint32_t)k);
+            atomicAdd(A.ctr + spread_at(A.stat_w, blockIdx.x * 64 + lane), (uint32_t)len);
+        }
         uint32_t a0, a1, a2;
         ia_counts<C>(X, a0, a1, a2);
         A.cnt[3 * r] = a0;
@@ -738,6 +752,12 @@ __global__ void __launch_bounds__(64) k_cut_wave(CutArgs A) {
     const int len = dev::wave_ia_record<C>(X, k, out, (int)A.slot, a0, a1, a2);
     if (lane == 0) {
         if (len < 0) atomicOr(A.errs, 4u);
+        if (A.stat_k >= 0 && len > 0) {
+            // words written: the record, and the face offsets when they fit
+            const uint32_t w = (uint32_t)len + ((uint32_t)(len + X.nf) <= A.slot ? (uint32_t)X.nf : 0u);
+            atomicAdd(A.ctr + spread_at(A.stat_k, blockIdx.x), (uint32_t)k);
+            atomicAdd(A.ctr + spread_at(A.stat_w, blockIdx.x), w);
+        }
         A.cnt[3 * r] = a0;
         A.cnt[3 * r + 1] = a1;
         A.cnt[3 * r + 2] = a2;
@@ -982,7 +1002,7 @@ __global__ void k_extract_records(ExtractArgs A) {
     }
     for (int d = 32; d > 0; d >>= 1) recw += __shfl_xor(recw, d, 64);
     if ((threadIdx.x & 63) == 0 && recw)
-        atomicAdd(A.ctr + kSpreadBase + kSpreadStride * ((blockIdx.x * 4 + (threadIdx.x >> 6)) % kSpread), recw);
+        atomicAdd(A.ctr + spread_at(SP_RECW, blockIdx.x * 4 + (threadIdx.x >> 6)), recw);
 }
 
 // k = 3..8 records: one wave per tet, lanes over faces and vertices.  The
@@ -1013,7 +1033,7 @@ __global__ void __launch_bounds__(64) k_extract_records_wave(ExtractArgs A) {
         return;
     }
     for (int x = lane; x < (nv + 31) / 32; x += 64) isov[x] = 0;
-    if (lane == 0) atomicAdd(A.ctr + kSpreadBase + kSpreadStride * (blockIdx.x % kSpread), len);
+    if (lane == 0) atomicAdd(A.ctr + spread_at(SP_RECW, blockIdx.x), len);
     const bool dup = W[5] & 1;
     if (!(loc & K1_FLAG) && A.fo_table[bq] && len + nf <= A.slot_words[bq]) {
         for (int f = lane; f < nf; f += 64) fo[f] = W[len + f];
@@ -1119,10 +1139,13 @@ __global__ void __launch_bounds__(64) k_extract_records_wave(ExtractArgs A) {
 
 __global__ void k_write_totals(uint32_t* ctr, uint64_t T, uint32_t* csr_off, uint32_t* f_vstart,
                                uint32_t* f_tfstart, uint32_t cap_fo) {
-    uint32_t recw = threadIdx.x < kSpread ? ctr[kSpreadBase + kSpreadStride * threadIdx.x] : 0u;
-    for (int d = 32; d > 0; d >>= 1) recw += __shfl_xor(recw, d, 64);
+    const int fold_to[SP_N] = {C_RECW, C_CUTK3, C_CUTW3, C_CUTK5, C_CUTW5};
+    for (int c = 0; c < SP_N; ++c) {
+        uint32_t v = threadIdx.x < kSpread ? ctr[spread_at(c, threadIdx.x)] : 0u;
+        for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
+        if (threadIdx.x == 0) ctr[fold_to[c]] = v;
+    }
     if (threadIdx.x == 0) {
-        ctr[C_RECW] = recw;
         csr_off[T] = ctr[C_FIT];
         const uint32_t nf = ctr[C_ABORT] ? 0u : min(ctr[C_FACES], cap_fo);
         f_vstart[nf] = ctr[C_FACEVERTS];
@@ -1133,7 +1156,8 @@ __global__ void k_write_totals(uint32_t* ctr, uint64_t T, uint32_t* csr_off, uin
 __global__ void k_zero_ctr(uint32_t* ctr, int n, uint32_t* errs) {
     if (threadIdx.x < (unsigned)n) ctr[threadIdx.x] = 0;
     if (threadIdx.x < 64) errs[threadIdx.x] = 0;
-    if (threadIdx.x < kSpread) ctr[kSpreadBase + kSpreadStride * threadIdx.x] = 0;
+    for (int c = 0; c < SP_N; ++c)
+        if (threadIdx.x < kSpread) ctr[spread_at(c, threadIdx.x)] = 0;
 }
 
 // the counters the host reads back after a replay
@@ -1369,7 +1393,7 @@ void alloc_ws(Workspace& W, const Grid& g, const Caps_& c) {
     W.f_tf_tet.alloc(m, c.fo + 1);
     W.f_tf_local.alloc(m, c.fo + 1);
     W.f_func.alloc(m, 2 * (uint64_t)c.fo + 2);
-    W.ctr.alloc(m, kSpreadBase + kSpread * kSpreadStride);
+    W.ctr.alloc(m, kSpreadBase + SP_N * kSpread * kSpreadStride);
     W.errs.alloc(m, 64);
     const K1Table& K = k1_table();
     const K2Table& K2 = k2_table();
@@ -1424,12 +1448,15 @@ void launch_cut(Workspace& W, const Grid& g, const Values& val, int q, int grid,
     A.fit = W.fit.p;
     A.n = W.ctr.p + C_Q2 + q;
     A.abort = W.ctr.p + C_ABORT;
-    A.queue = W.ctr.p + 24 + q;
+    A.queue = W.ctr.p + kQueueBase + q;
     A.cap = W.cap.b[q];
     A.slot = W.slot[q];
     A.rec = W.rec.p;
     A.cnt = W.cnt.p;
     A.errs = W.errs.p;
+    A.ctr = W.ctr.p;
+    A.stat_k = q == 1 ? SP_CUTK3 : (q == 2 ? SP_CUTK5 : -1);
+    A.stat_w = q == 1 ? SP_CUTW3 : (q == 2 ? SP_CUTW5 : -1);
     if (grid == -2) {  // k = 2 table
         K2Args K;
         K.g = g.view;
@@ -1457,7 +1484,7 @@ void launch_cut(Workspace& W, const Grid& g, const Values& val, int q, int grid,
         hipLaunchKernelGGL(k_cut_wave<C>, dim3(W.cap.b[q]), dim3(64), 0, s, A);
         return;
     }
-    fill_u32(W.ctr.p + 24 + q, 0, 1, s);
+    fill_u32(W.ctr.p + kQueueBase + q, 0, 1, s);
     hipLaunchKernelGGL(k_cut_bucket<C>, dim3(grid), dim3(64), 0, s, A);
 }
 
@@ -1959,6 +1986,10 @@ void run_batched(DevMem& ctx_mem, const Grid& g, Values& val, uint32_t flags, Ru
     out.n_cut34 = hctr[C_B3];
     out.n_cut58 = hctr[C_B5];
     out.n_rec_words = hctr[C_RECW];
+    out.cut_k_sum[0] = hctr[C_CUTK3];
+    out.cut_k_sum[1] = hctr[C_CUTK5];
+    out.cut_words[0] = hctr[C_CUTW3];
+    out.cut_words[1] = hctr[C_CUTW5];
     auto& vw = out.view;
     vw.fit = w.fit.p;
     vw.csr_off = w.csr_off.p;

@@ -70,6 +70,7 @@ struct RunOut {
     // mesh
     uint64_t n_verts = 0, n_faces = 0, n_face_verts = 0, n_tet_faces = 0;
     uint64_t n_vert_occ = 0, n_face_occ = 0, n_face_occ_verts = 0, n_cut34 = 0, n_cut58 = 0, n_rec_words = 0;
+    uint64_t cut_k_sum[2] = {0, 0}, cut_words[2] = {0, 0};  // batched form, cut buckets 3..4 and 5..8
     DBuf<uint32_t> v_tet;
     DBuf<uint16_t> v_j;
     DBuf<uint8_t> v_ss;

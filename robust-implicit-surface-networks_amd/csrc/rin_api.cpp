@@ -233,6 +233,10 @@ int rin_result_sizes(rin_ctx* c, const rin_result* res, rin_sizes* s) {
     s->n_cut[0] = r.n_cut34;
     s->n_cut[1] = r.n_cut58;
     s->n_rec_words = r.n_rec_words;
+    for (int i = 0; i < 2; ++i) {
+        s->cut_k_sum[i] = r.cut_k_sum[i];
+        s->cut_words[i] = r.cut_words[i];
+    }
     return RIN_OK;
 }
 

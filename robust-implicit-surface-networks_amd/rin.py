@@ -38,7 +38,8 @@ class Sizes(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint64) for n in
                 ("n_tets", "n_funcs", "num_degenerate", "num_intersecting", "b0", "b1", "b2", "n_func_in_tet",
                  "n_verts", "n_faces", "n_face_verts", "n_tet_faces", "n_vert_occ", "n_face_occ",
-                 "n_face_occ_verts", "n_cut34", "n_cut58", "n_rec_words")]
+                 "n_face_occ_verts", "n_cut34", "n_cut58", "n_rec_words", "cut_k34", "cut_k58", "cut_words34",
+                 "cut_words58")]
 
 
 class RinError(RuntimeError):
