@@ -723,7 +723,7 @@ __global__ void __launch_bounds__(64) k_cut_bucket(CutArgs A) {
 // wave-cooperative cut: one block (a single wavefront) per bucket entry, the
 // complex in LDS; lanes share the predicates and the splits of each insertion
 template <class C>
-__global__ void __launch_bounds__(64) k_cut_wave(CutArgs A) {
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_cut_wave(CutArgs A) {
     __shared__ dev::WaveIA<C> X;
     const int lane = threadIdx.x;
     const uint32_t n = *A.abort ? 0u : min(*A.n, A.cap);

@@ -45,8 +45,11 @@ struct WaveIA {
     uint16_t ckey[CAND];
     uint16_t cface[CAND];
     uint8_t cedge[CAND], cfirst[CAND], cvid[CAND];
+    // candidate edges by key: key << 16 | smallest candidate index
+    static constexpr int HT = CAND <= 64 ? 128 : (CAND <= 128 ? 256 : (CAND <= 512 ? 1024 : 2048));
+    uint32_t ht[HT];
     uint8_t isov[MV];
-    uint8_t tv[64][MFV], te[64][MFV], tb[64][MFV];  // per-lane (or per split cell) rows
+    uint8_t tv[64][MFV], te[64][MFV], tb[64][MFV];  // per split cell rows (new-face edges)
     uint16_t tk[64][MFV];                            // collection order of a split cell's edges
     uint32_t ccnt[64];                               // edges collected per split cell
     int nv, nf, nc, dup, err;
@@ -90,21 +93,23 @@ __device__ inline uint32_t wave_prefix(int n, F count, G store) {
     return base;
 }
 
-// one side (sgn = +1 / -1) of a face split, streamed over the face's cycle
+// One side (sgn = +1 / -1) of a face split, streamed over the face's cycle
 // with the crossing vertices inserted; the cycle starts at the first kept
 // position and an edge between two kept positions that were not adjacent
-// lies on the new plane.  Returns the vertex count, or -1 past MFV.
-template <class C>
-__device__ inline int wave_split_side(const WaveIA<C>& X, int f, int sgn, uint8_t pid, uint8_t* dv, uint8_t* de) {
-    const int n = X.fn[f];
+// lies on the new plane.  The face comes in registers: its vertex ids, edge
+// labels and vertex signs (fvr, fer, sr), and the crossing vertex after each
+// position (cr, valid where the edge changes sign strictly).  Returns the
+// vertex count, or -1 past MFV.
+template <int MFV>
+__device__ inline int split_side_regs(const uint8_t (&fvr)[MFV], const uint8_t (&fer)[MFV], const int8_t (&sr)[MFV],
+                                      const uint8_t (&cr)[MFV], int n, int sgn, uint8_t pid, uint8_t* dv, uint8_t* de) {
     int cnt = 0, p = 0, first = -1, prev = -1;
     uint8_t pv = 0, pe = 0;
-    uint32_t ci = X.fcand[f];
     bool over = false;
-    auto visit = [&](uint8_t v, uint8_t e) {
-        if (X.s[v] * sgn >= 0) {
+    auto visit = [&](uint8_t v, int sv, uint8_t e) {
+        if (sv * sgn >= 0) {
             if (prev >= 0) {
-                if (cnt >= C::MFV)
+                if (cnt >= MFV)
                     over = true;
                 else {
                     dv[cnt] = pv;
@@ -120,13 +125,19 @@ __device__ inline int wave_split_side(const WaveIA<C>& X, int f, int sgn, uint8_
         }
         ++p;
     };
-    for (int i = 0; i < n; ++i) {
-        const uint8_t u = X.fv[f][i], w = X.fv[f][i + 1 == n ? 0 : i + 1], e = X.fe[f][i];
-        visit(u, e);
-        if (X.s[u] * X.s[w] < 0) visit(X.cvid[ci++], e);
+#pragma unroll
+    for (int i = 0; i < MFV; ++i) {
+        if (i < n) {
+            const int w = i + 1 == n ? 0 : i + 1;
+            int sw = 0;
+#pragma unroll
+            for (int q = 0; q < MFV; ++q) sw = q == w ? sr[q] : sw;
+            visit(fvr[i], sr[i], fer[i]);
+            if (sr[i] * sw < 0) visit(cr[i], 0, fer[i]);
+        }
     }
     if (prev >= 0) {
-        if (cnt >= C::MFV)
+        if (cnt >= MFV)
             over = true;
         else {
             dv[cnt] = pv;
@@ -227,40 +238,83 @@ __device__ void wave_ia_cut(WaveIA<C>& X, int k) {
             continue;
         }
 
-        // strict sign-change edges in (face, edge) order
-        const uint32_t ncand = wave_prefix(
-            nf0,
-            [&](int f) {
-                uint32_t c = 0;
-                const int n = X.fn[f];
-                for (int i = 0; i < n; ++i) c += X.s[X.fv[f][i]] * X.s[X.fv[f][i + 1 == n ? 0 : i + 1]] < 0;
-                return c;
-            },
-            [&](int f, uint32_t o) { X.fcand[f] = (uint16_t)o; });
-        if (ncand > (uint32_t)WaveIA<C>::CAND) return fail();
-        for (int f = lane; f < nf0; f += 64) {
-            uint32_t o = X.fcand[f];
-            const int n = X.fn[f];
-            for (int i = 0; i < n; ++i) {
-                const uint8_t u = X.fv[f][i], w = X.fv[f][i + 1 == n ? 0 : i + 1];
-                if (X.s[u] * X.s[w] >= 0) continue;
-                X.ckey[o] = (uint16_t)((u < w ? u : w) << 8 | (u < w ? w : u));
-                X.cface[o] = (uint16_t)f;
-                X.cedge[o] = (uint8_t)i;
-                ++o;
+        // strict sign-change edges in (face, edge) order: each lane loads its
+        // face's vertex ids and signs in parallel, counts and files them
+        constexpr int MFV = C::MFV, CAND = WaveIA<C>::CAND, HT = WaveIA<C>::HT;
+        for (int h = lane; h < HT; h += 64) X.ht[h] = 0xFFFFFFFFu;
+        uint32_t ncand = 0;
+        for (int base = 0; base < nf0; base += 64) {
+            const int f = base + lane;
+            uint8_t fvr[MFV];
+            int8_t sr[MFV];
+            int n = 0;
+            uint32_t c = 0;
+            if (f < nf0) {
+                n = X.fn[f];
+#pragma unroll
+                for (int i = 0; i < MFV; ++i) fvr[i] = X.fv[f][i];
+#pragma unroll
+                for (int i = 0; i < MFV; ++i) sr[i] = i < n ? X.s[fvr[i]] : (int8_t)0;
+#pragma unroll
+                for (int i = 0; i < MFV; ++i) {
+                    int sw = 0;
+#pragma unroll
+                    for (int q = 0; q < MFV; ++q) sw = q == (i + 1 == n ? 0 : i + 1) ? sr[q] : sw;
+                    c += i < n && sr[i] * sw < 0;
+                }
+            }
+            uint32_t o = ncand + wave_excl(c);
+            if (f < nf0) {
+                X.fcand[f] = (uint16_t)o;
+#pragma unroll
+                for (int i = 0; i < MFV; ++i) {
+                    if (i >= n) continue;
+                    const int w = i + 1 == n ? 0 : i + 1;
+                    int sw = 0;
+                    uint8_t vw = 0;
+#pragma unroll
+                    for (int q = 0; q < MFV; ++q) {
+                        sw = q == w ? sr[q] : sw;
+                        vw = q == w ? fvr[q] : vw;
+                    }
+                    if (sr[i] * sw >= 0) continue;
+                    if (o < (uint32_t)CAND) {
+                        const uint8_t u = fvr[i];
+                        X.ckey[o] = (uint16_t)((u < vw ? u : vw) << 8 | (u < vw ? vw : u));
+                        X.cface[o] = (uint16_t)f;
+                        X.cedge[o] = (uint8_t)i;
+                    }
+                    ++o;
+                }
+            }
+            ncand += wave_sum(c);
+        }
+        if (ncand > (uint32_t)CAND) return fail();
+        __syncthreads();
+        // the first candidate of each edge: an LDS hash from key to the
+        // smallest candidate index (keys are 16 bits, indices below CAND)
+        auto home = [&](uint32_t key) { return (key * 2654435761u) >> 16 & (uint32_t)(HT - 1); };
+        // smallest candidate index of a key already inserted
+        auto first_of = [&](uint32_t key) {
+            uint32_t h = home(key);
+            for (int probe = 0; probe < HT && X.ht[h] >> 16 != key; ++probe) h = (h + 1) & (uint32_t)(HT - 1);
+            return X.ht[h] & 0xFFFFu;
+        };
+        for (int c = lane; c < (int)ncand; c += 64) {
+            const uint32_t key = X.ckey[c], packed = key << 16 | (uint32_t)c;
+            uint32_t h = home(key);
+            for (int probe = 0; probe < HT; ++probe) {
+                const uint32_t cur = atomicCAS(&X.ht[h], 0xFFFFFFFFu, packed);
+                if (cur == 0xFFFFFFFFu) break;
+                if (cur >> 16 == key) {
+                    atomicMin(&X.ht[h], packed);
+                    break;
+                }
+                h = (h + 1) & (uint32_t)(HT - 1);
             }
         }
         __syncthreads();
-        for (int c = lane; c < (int)ncand; c += 64) {
-            const uint16_t key = X.ckey[c];
-            uint8_t first = 1;
-            for (int d = 0; d < c; ++d)
-                if (X.ckey[d] == key) {
-                    first = 0;
-                    break;
-                }
-            X.cfirst[c] = first;
-        }
+        for (int c = lane; c < (int)ncand; c += 64) X.cfirst[c] = first_of(X.ckey[c]) == (uint32_t)c;
         __syncthreads();
         const uint32_t nnew = wave_prefix(
             (int)ncand, [&](int c) { return (uint32_t)X.cfirst[c]; },
@@ -280,12 +334,7 @@ __device__ void wave_ia_cut(WaveIA<C>& X, int k) {
                 X.vl[id][2] = t2;
                 X.s[id] = 0;
             } else {
-                const uint16_t key = X.ckey[c];
-                for (int d = 0; d < c; ++d)
-                    if (X.ckey[d] == key) {
-                        X.cvid[c] = X.cvid[d];
-                        break;
-                    }
+                X.cvid[c] = X.cvid[first_of(X.ckey[c])];
             }
         }
         __syncthreads();
@@ -308,9 +357,35 @@ __device__ void wave_ia_cut(WaveIA<C>& X, int k) {
         bool ferr = false;
         for (int f = lane; f < nf0; f += 64) {
             if (X.fflag[f] != 3) continue;
+            // the face in registers: ids, labels and signs load in parallel,
+            // then both sides are streamed without further LDS reads but
+            // the crossing vertices; the positive side overwrites the row
+            constexpr int MFV = C::MFV;
+            const int n = X.fn[f];
+            uint8_t fvr[MFV], fer[MFV], cr[MFV];
+            int8_t sr[MFV];
+#pragma unroll
+            for (int i = 0; i < MFV; ++i) {
+                fvr[i] = X.fv[f][i];
+                fer[i] = X.fe[f][i];
+            }
+#pragma unroll
+            for (int i = 0; i < MFV; ++i) sr[i] = i < n ? X.s[fvr[i]] : (int8_t)0;
+            uint32_t ci = X.fcand[f];
+#pragma unroll
+            for (int i = 0; i < MFV; ++i) {
+                cr[i] = 0;
+                if (i < n) {
+                    const int w = i + 1 == n ? 0 : i + 1;
+                    int sw = 0;
+#pragma unroll
+                    for (int q = 0; q < MFV; ++q) sw = q == w ? sr[q] : sw;
+                    if (sr[i] * sw < 0) cr[i] = X.cvid[ci++];
+                }
+            }
             const int dn = nf0 + X.frank[f];
-            const int nneg = wave_split_side<C>(X, f, -1, pid, X.fv[dn], X.fe[dn]);
-            const int npos = wave_split_side<C>(X, f, +1, pid, X.tv[lane], X.te[lane]);
+            const int nneg = split_side_regs<MFV>(fvr, fer, sr, cr, n, -1, pid, X.fv[dn], X.fe[dn]);
+            const int npos = split_side_regs<MFV>(fvr, fer, sr, cr, n, +1, pid, X.fv[f], X.fe[f]);
             if (nneg < 0 || npos < 0) {
                 ferr = true;
                 continue;
@@ -319,10 +394,6 @@ __device__ void wave_ia_cut(WaveIA<C>& X, int k) {
             X.fa[dn] = X.fa[f];
             X.fb[dn] = X.fb[f];
             X.fc[dn] = X.fc[f];
-            for (int q = 0; q < npos; ++q) {
-                X.fv[f][q] = X.tv[lane][q];
-                X.fe[f][q] = X.te[lane][q];
-            }
             X.fn[f] = (uint8_t)npos;
         }
         if (__ballot(ferr)) return fail();
