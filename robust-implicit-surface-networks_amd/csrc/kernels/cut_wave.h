@@ -93,6 +93,28 @@ __device__ inline uint32_t wave_prefix(int n, F count, G store) {
     return base;
 }
 
+// A face's vertex ids and their signs in registers: the row and then the
+// signs load in parallel (two LDS round trips instead of two per vertex).
+// Entries past the vertex count hold sign 0.  Returns the vertex count.
+template <class C>
+__device__ inline int load_face(const WaveIA<C>& X, int f, uint8_t (&fvr)[C::MFV], int8_t (&sr)[C::MFV]) {
+    const int n = X.fn[f];
+#pragma unroll
+    for (int i = 0; i < C::MFV; ++i) fvr[i] = X.fv[f][i];
+#pragma unroll
+    for (int i = 0; i < C::MFV; ++i) sr[i] = i < n ? X.s[fvr[i]] : (int8_t)0;
+    return n;
+}
+
+// bit 0: some vertex with s > 0, bit 1: some with s < 0
+template <int MFV>
+__device__ inline uint8_t sign_flags(const int8_t (&sr)[MFV]) {
+    uint8_t fl = 0;
+#pragma unroll
+    for (int i = 0; i < MFV; ++i) fl |= sr[i] > 0 ? 1 : (sr[i] < 0 ? 2 : 0);
+    return fl;
+}
+
 // One side (sgn = +1 / -1) of a face split, streamed over the face's cycle
 // with the crossing vertices inserted; the cycle starts at the first kept
 // position and an edge between two kept positions that were not adjacent
@@ -204,9 +226,10 @@ __device__ void wave_ia_cut(WaveIA<C>& X, int k) {
         // a plane containing an existing face joins that face's group
         int cop = INT_MAX;
         for (int f = lane; f < nf0; f += 64) {
-            bool allz = true;
-            for (int q = 0; q < X.fn[f]; ++q) allz &= X.s[X.fv[f][q]] == 0;
-            if (allz && f < cop) cop = f;
+            uint8_t fvr[C::MFV];
+            int8_t sr[C::MFV];
+            (void)load_face<C>(X, f, fvr, sr);
+            if (sign_flags<C::MFV>(sr) == 0 && f < cop) cop = f;
         }
         cop = wave_min(cop);
         if (cop != INT_MAX) {
@@ -344,11 +367,10 @@ __device__ void wave_ia_cut(WaveIA<C>& X, int k) {
         const uint32_t nmix = wave_prefix(
             nf0,
             [&](int f) {
-                uint8_t fl = 0;
-                for (int i = 0; i < X.fn[f]; ++i) {
-                    const int sv = X.s[X.fv[f][i]];
-                    fl |= sv > 0 ? 1 : (sv < 0 ? 2 : 0);
-                }
+                uint8_t fvr[C::MFV];
+                int8_t sr[C::MFV];
+                (void)load_face<C>(X, f, fvr, sr);
+                const uint8_t fl = sign_flags<C::MFV>(sr);
                 X.fflag[f] = fl;
                 return (uint32_t)(fl == 3);
             },
@@ -404,11 +426,10 @@ __device__ void wave_ia_cut(WaveIA<C>& X, int k) {
         // cell splits: the positive side keeps the id, the negative side and
         // one face on the new plane are appended in cell order
         for (int g = lane; g < nf1; g += 64) {
-            uint8_t fl = 0;
-            for (int i = 0; i < X.fn[g]; ++i) {
-                const int sv = X.s[X.fv[g][i]];
-                fl |= sv > 0 ? 1 : (sv < 0 ? 2 : 0);
-            }
+            uint8_t fvr[C::MFV];
+            int8_t sr[C::MFV];
+            (void)load_face<C>(X, g, fvr, sr);
+            const uint8_t fl = sign_flags<C::MFV>(sr);
             X.fflag[g] = fl;
             if (fl) {
                 atomicOr(&X.cflag[X.fb[g]], (uint32_t)fl);
@@ -444,10 +465,22 @@ __device__ void wave_ia_cut(WaveIA<C>& X, int k) {
             const int rb = (b < nc0 && X.crank[b] != 0xFFFF) ? (int)X.crank[b] : -1;
             const int rc = (cc != NIL && cc < nc0 && X.crank[cc] != 0xFFFF) ? (int)X.crank[cc] : -1;
             if (rb < 0 && rc < 0) continue;
-            const int n = X.fn[g];
-            for (int i = 0; i < n; ++i) {
-                const uint8_t a = X.fv[g][i], w = X.fv[g][i + 1 == n ? 0 : i + 1];
-                if (X.s[a] != 0 || X.s[w] != 0) continue;
+            uint8_t fvr[C::MFV];
+            int8_t sr[C::MFV];
+            const int n = load_face<C>(X, g, fvr, sr);
+#pragma unroll
+            for (int i = 0; i < C::MFV; ++i) {
+                if (i >= n) continue;
+                const int wi = i + 1 == n ? 0 : i + 1;
+                uint8_t w = 0;
+                int sw = 0;
+#pragma unroll
+                for (int q = 0; q < C::MFV; ++q) {
+                    w = q == wi ? fvr[q] : w;
+                    sw = q == wi ? sr[q] : sw;
+                }
+                const uint8_t a = fvr[i];
+                if (sr[i] != 0 || sw != 0) continue;
                 for (int side = 0; side < 2; ++side) {
                     const int row = side ? rc : rb;
                     if (row < 0) continue;
@@ -473,13 +506,25 @@ __device__ void wave_ia_cut(WaveIA<C>& X, int k) {
                 cerr = true;
                 break;
             }
-            const uint8_t* ea = X.tv[row];
-            const uint8_t* eb = X.tb[row];
-            const uint8_t* el = X.te[row];
-            const uint16_t* ek = X.tk[row];
+            // the row in registers; chaining then needs no LDS round trips
+            constexpr int MFV = C::MFV;
+            uint8_t ea[MFV], eb[MFV], el[MFV];
+            uint16_t ek[MFV];
+#pragma unroll
+            for (int q = 0; q < MFV; ++q) {
+                ea[q] = X.tv[row][q];
+                eb[q] = X.tb[row][q];
+                el[q] = X.te[row][q];
+                ek[q] = q < ne ? X.tk[row][q] : (uint16_t)0xFFFF;
+            }
             int start = 0;
-            for (int q = 1; q < ne; ++q)
-                if (ek[q] < ek[start]) start = q;
+            uint16_t kmin = ek[0];
+#pragma unroll
+            for (int q = 1; q < MFV; ++q)
+                if (ek[q] < kmin) {
+                    kmin = ek[q];
+                    start = q;
+                }
             const int F = nf1 + row;
             int cur = start, cnt = 0;
             for (;;) {
@@ -487,12 +532,25 @@ __device__ void wave_ia_cut(WaveIA<C>& X, int k) {
                     cerr = true;
                     break;
                 }
-                X.fv[F][cnt] = ea[cur];
-                X.fe[F][cnt] = el[cur];
+                uint8_t a = 0, b = 0, l = 0;
+#pragma unroll
+                for (int q = 0; q < MFV; ++q)
+                    if (q == cur) {
+                        a = ea[q];
+                        b = eb[q];
+                        l = el[q];
+                    }
+                X.fv[F][cnt] = a;
+                X.fe[F][cnt] = l;
                 ++cnt;
                 int nxt = -1;
-                for (int q = 0; q < ne; ++q)
-                    if (ea[q] == eb[cur] && (nxt < 0 || ek[q] < ek[nxt])) nxt = q;
+                uint16_t knxt = 0xFFFF;
+#pragma unroll
+                for (int q = 0; q < MFV; ++q)
+                    if (q < ne && ea[q] == b && (nxt < 0 || ek[q] < knxt)) {
+                        nxt = q;
+                        knxt = ek[q];
+                    }
                 if (nxt < 0) {
                     cerr = true;
                     break;
