@@ -535,6 +535,25 @@ __device__ inline uint32_t hash_slot(uint64_t k, int bits) {
     return (uint32_t)((k * 0x9E3779B97F4A7C15ull) >> (64 - bits));
 }
 
+__device__ inline void hash_insert_one(uint32_t o, const uint64_t* __restrict__ okey, unsigned long long* hkey,
+                                       uint32_t* hmx, int bits, uint32_t* hslot, uint32_t* errs) {
+    const uint32_t mask = (uint32_t)((1ull << bits) - 1);
+    const unsigned long long k = okey[o] + 1;
+    uint32_t h = hash_slot(k, bits);
+    for (uint32_t probe = 0; probe <= mask; ++probe) {
+        const unsigned long long prev = atomicCAS(&hkey[h], 0ull, k);
+        if (prev == 0 || prev == k) {
+            atomicMax(&hmx[h], ~o);
+            hslot[o] = h;
+            return;
+        }
+        h = (h + 1) & mask;
+    }
+    // full table (sized at twice the occurrence capacity: not reached)
+    flag_error(errs, ERR_CAPACITY);
+    hslot[o] = 0;
+}
+
 constexpr int kStarShapes = 1 + 27 + 27 * 27;  // corner, edge by direction, face by two directions
 
 struct StarArgs {
@@ -614,28 +633,7 @@ __global__ void k_dedup_star(Count n, StarArgs A) {
 __global__ void k_hash_insert(Count n, const uint32_t* __restrict__ list, const uint64_t* __restrict__ okey,
                               unsigned long long* __restrict__ hkey, uint32_t* __restrict__ hmx, int bits,
                               uint32_t* __restrict__ hslot, uint32_t* __restrict__ errs) {
-    const uint32_t mask = (uint32_t)((1ull << bits) - 1);
-    GRID_STRIDE(i, n) {
-        const uint32_t o = list ? list[i] : (uint32_t)i;
-        const unsigned long long k = okey[o] + 1;
-        uint32_t h = hash_slot(k, bits);
-        bool ok = false;
-        for (uint32_t probe = 0; probe <= mask; ++probe) {
-            const unsigned long long prev = atomicCAS(&hkey[h], 0ull, k);
-            if (prev == 0 || prev == k) {
-                ok = true;
-                break;
-            }
-            h = (h + 1) & mask;
-        }
-        if (!ok) {  // full table (sized at twice the occurrence capacity: not reached)
-            flag_error(errs, ERR_CAPACITY);
-            hslot[o] = 0;
-            continue;
-        }
-        atomicMax(&hmx[h], ~o);
-        hslot[o] = h;
-    }
+    GRID_STRIDE(i, n) hash_insert_one(list ? list[i] : (uint32_t)i, okey, hkey, hmx, bits, hslot, errs);
 }
 
 __global__ void k_hash_first(Count n, const uint32_t* __restrict__ list, const uint32_t* __restrict__ hslot,

@@ -103,20 +103,25 @@ __global__ void k_scan3_reduce(Scan3 S, Count cn, uint32_t* __restrict__ sums, u
     }
 }
 
-// one block per field: exclusive scan of that field's nb <= kTile tile sums
+// one block per field: exclusive scan of that field's nb tile sums, kTile
+// at a time
 __global__ void k_scan3_top(uint32_t* __restrict__ sums, uint32_t nb) {
     uint32_t* p = sums + (size_t)blockIdx.x * nb;
-    const uint32_t base = threadIdx.x * kItems;
-    uint32_t v[kItems], acc = 0;
-    for (int i = 0; i < kItems; ++i) {
-        v[i] = base + i < nb ? p[base + i] : 0u;
-        acc += v[i];
-    }
-    uint32_t tot;
-    uint32_t run = block_excl_scan(acc, tot);
-    for (int i = 0; i < kItems; ++i) {
-        if (base + i < nb) p[base + i] = run;
-        run += v[i];
+    uint32_t carry = 0;
+    for (uint32_t b0 = 0; b0 < nb; b0 += kTile) {
+        const uint32_t base = b0 + threadIdx.x * kItems;
+        uint32_t v[kItems], acc = 0;
+        for (int i = 0; i < kItems; ++i) {
+            v[i] = base + i < nb ? p[base + i] : 0u;
+            acc += v[i];
+        }
+        uint32_t tot;
+        uint32_t run = carry + block_excl_scan(acc, tot);
+        for (int i = 0; i < kItems; ++i) {
+            if (base + i < nb) p[base + i] = run;
+            run += v[i];
+        }
+        carry += tot;
     }
 }
 
@@ -264,7 +269,10 @@ void scan_u32(DevMem& mem, const uint32_t* in, uint32_t* out, Count n, uint32_t*
     }
     DBuf<uint32_t> sums(mem, nb);
     hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb), dim3(kThreads), 0, s, in, n, sums.p);
-    scan_u32(mem, sums.p, sums.p, Count(nb), nullptr, s);
+    if (nb <= (size_t)16 * kTile)  // one workgroup scans the tile sums
+        hipLaunchKernelGGL(k_scan3_top, dim3(1), dim3(kThreads), 0, s, sums.p, (uint32_t)nb);
+    else
+        scan_u32(mem, sums.p, sums.p, Count(nb), nullptr, s);
     hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nb), dim3(kThreads), 0, s, in, out, n, (const uint32_t*)sums.p,
                        total);
 }
@@ -272,7 +280,7 @@ void scan_u32(DevMem& mem, const uint32_t* in, uint32_t* out, Count n, uint32_t*
 void scan3_u32(DevMem& mem, const uint32_t* const in[3], int stride, uint32_t* const out[3], Count n,
                uint32_t* const total[3], hipStream_t s) {
     const size_t nb = (n.cap + kTile - 1) / kTile;
-    if (nb > (size_t)kTile) {  // beyond one top-level block: three separate scans
+    if (nb > (size_t)16 * kTile) {  // beyond one top-level workgroup: three separate scans
         for (int f = 0; f < 3; ++f) {
             if (stride != 1) throw std::runtime_error("strided scan larger than one top-level tile");
             scan_u32(mem, in[f], out[f], n, total[f], s);
