@@ -720,14 +720,18 @@ __global__ void __launch_bounds__(64) k_cut_bucket(CutArgs A) {
     }
 }
 
-// wave-cooperative cut: one block (a single wavefront) per bucket entry, the
-// complex in LDS; lanes share the predicates and the splits of each insertion
-template <class C>
+// wave-cooperative cut: one lane group of G lanes per bucket entry (a block
+// is a single wavefront carrying 64 / G entries), the complex in LDS; the
+// group's lanes share the predicates and the splits of each insertion
+template <class C, int G>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_cut_wave(CutArgs A) {
-    __shared__ dev::WaveIA<C> X;
-    const int lane = threadIdx.x;
+    // 64 / G tets per wavefront, one lane group each
+    constexpr int NG = 64 / G;
+    __shared__ dev::WaveIA<C> Xs[NG];
+    const int gid = threadIdx.x / G, lane = threadIdx.x & (G - 1);
+    dev::WaveIA<C>& X = Xs[gid];
     const uint32_t n = *A.abort ? 0u : min(*A.n, A.cap);
-    const uint32_t i = blockIdx.x;
+    const uint32_t i = blockIdx.x * NG + gid;
     if (i >= n) return;
     const uint32_t r = A.list[i];
     const int k = A.act_k[r];
@@ -738,7 +742,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
         const uint32_t f = A.fit[A.csr_off[t] + (lane >> 2)];
         X.P[lane >> 2][lane & 3] = A.vals[(uint64_t)f * A.g.V + v[lane & 3]];
     }
-    dev::wave_ia_cut<C>(X, k);
+    dev::wave_ia_cut<C, G>(X, k);
     uint16_t* out = A.rec + A.rec_loc[r];
     if (X.err) {
         if (lane == 0) {
@@ -749,14 +753,14 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
         return;
     }
     uint32_t a0, a1, a2;
-    const int len = dev::wave_ia_record<C>(X, k, out, (int)A.slot, a0, a1, a2);
+    const int len = dev::wave_ia_record<C, G>(X, k, out, (int)A.slot, a0, a1, a2);
     if (lane == 0) {
         if (len < 0) atomicOr(A.errs, 4u);
         if (A.stat_k >= 0 && len > 0) {
             // words written: the record, and the face offsets when they fit
             const uint32_t w = (uint32_t)len + ((uint32_t)(len + X.nf) <= A.slot ? (uint32_t)X.nf : 0u);
-            atomicAdd(A.ctr + spread_at(A.stat_k, blockIdx.x), (uint32_t)k);
-            atomicAdd(A.ctr + spread_at(A.stat_w, blockIdx.x), w);
+            atomicAdd(A.ctr + spread_at(A.stat_k, i), (uint32_t)k);
+            atomicAdd(A.ctr + spread_at(A.stat_w, i), w);
         }
         A.cnt[3 * r] = a0;
         A.cnt[3 * r + 1] = a1;
@@ -1481,7 +1485,10 @@ void launch_cut(Workspace& W, const Grid& g, const Values& val, int q, int grid,
         return;
     }
     if (grid < 0) {  // wave form: one block per bucket entry
-        hipLaunchKernelGGL(k_cut_wave<C>, dim3(W.cap.b[q]), dim3(64), 0, s, A);
+        // two tets per wavefront up to 4 active functions (their complexes
+        // rarely fill 32 lanes); the whole wavefront for k = 5..8
+        constexpr int G = C::MV > 64 ? 64 : 32;
+        hipLaunchKernelGGL((k_cut_wave<C, G>), dim3((W.cap.b[q] + 64 / G - 1) / (64 / G)), dim3(64), 0, s, A);
         return;
     }
     fill_u32(W.ctr.p + kQueueBase + q, 0, 1, s);

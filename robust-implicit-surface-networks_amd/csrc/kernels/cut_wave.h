@@ -55,40 +55,51 @@ struct WaveIA {
     int nv, nf, nc, dup, err;
 };
 
-__device__ inline int wave_lane() { return threadIdx.x & 63; }
-
-__device__ inline uint32_t wave_excl(uint32_t x) {
-    const int lane = wave_lane();
-    uint32_t v = x;
-    for (int d = 1; d < 64; d <<= 1) {
-        const uint32_t o = __shfl_up(v, d, 64);
-        if (lane >= d) v += o;
+// Lane-group primitives: a group of G lanes (16, 32 or the whole 64-lane
+// wavefront) works on one tet, so a wavefront can carry 64 / G tets.
+template <int G>
+struct Grp {
+    static_assert(G == 16 || G == 32 || G == 64, "group size");
+    __device__ static int lane() { return threadIdx.x & (G - 1); }
+    __device__ static bool any(bool p) {
+        const uint64_t b = __ballot(p);
+        if (G == 64) return b != 0;
+        return ((b >> ((threadIdx.x & 63) & ~(G - 1))) & ((1ull << (G & 63)) - 1)) != 0;
     }
-    return v - x;
-}
-__device__ inline uint32_t wave_sum(uint32_t x) {
-    for (int d = 32; d > 0; d >>= 1) x += __shfl_xor(x, d, 64);
-    return x;
-}
-__device__ inline int wave_min(int x) {
-    for (int d = 32; d > 0; d >>= 1) {
-        const int o = __shfl_xor(x, d, 64);
-        x = o < x ? o : x;
+    __device__ static uint32_t excl(uint32_t x) {
+        const int l = lane();
+        uint32_t v = x;
+        for (int d = 1; d < G; d <<= 1) {
+            const uint32_t o = __shfl_up(v, d, G);
+            if (l >= d) v += o;
+        }
+        return v - x;
     }
-    return x;
-}
+    __device__ static uint32_t sum(uint32_t x) {
+        for (int d = G / 2; d > 0; d >>= 1) x += __shfl_xor(x, d, G);
+        return x;
+    }
+    __device__ static int min(int x) {
+        for (int d = G / 2; d > 0; d >>= 1) {
+            const int o = __shfl_xor(x, d, G);
+            x = o < x ? o : x;
+        }
+        return x;
+    }
+};
 
-// exclusive prefix of count(i) over items 0..n-1, item i on lane i % 64;
-// store(i, offset) runs on that same lane; returns the total (uniform)
-template <class F, class G>
-__device__ inline uint32_t wave_prefix(int n, F count, G store) {
+// exclusive prefix of count(i) over items 0..n-1, item i on group lane
+// i % G; store(i, offset) runs on that same lane; returns the total (uniform
+// over the group)
+template <int G, class F, class H>
+__device__ inline uint32_t wave_prefix(int n, F count, H store) {
     uint32_t base = 0;
-    for (int r0 = 0; r0 < n; r0 += 64) {
-        const int i = r0 + wave_lane();
+    for (int r0 = 0; r0 < n; r0 += G) {
+        const int i = r0 + Grp<G>::lane();
         const uint32_t c = i < n ? count(i) : 0u;
-        const uint32_t pre = wave_excl(c);
+        const uint32_t pre = Grp<G>::excl(c);
         if (i < n) store(i, base + pre);
-        base += wave_sum(c);
+        base += Grp<G>::sum(c);
     }
     return base;
 }
@@ -173,9 +184,10 @@ __device__ inline int split_side_regs(const uint8_t (&fvr)[MFV], const uint8_t (
 // Runs the insertion of k planes (X.P[0..k-1], loaded by the caller) into
 // the tet; on return X.err is set when a capacity or the predicates'
 // exponent range was exceeded.
-template <class C>
+template <class C, int G = 64>
 __device__ void wave_ia_cut(WaveIA<C>& X, int k) {
-    const int lane = wave_lane();
+    using W = Grp<G>;
+    const int lane = W::lane();
     if (lane < 4) {
         const int i = lane;
         for (int q = 0; q < 3; ++q) X.vl[i][q] = (uint8_t)(q < i ? q : q + 1);  // corner i: planes != i
@@ -213,25 +225,25 @@ __device__ void wave_ia_cut(WaveIA<C>& X, int k) {
 
         // classify every vertex against plane j
         bool e = false, ap = false, an = false;
-        for (int v = lane; v < nv0; v += 64) {
+        for (int v = lane; v < nv0; v += G) {
             const int sv = orient_ia(X.P, X.vl[v][0], X.vl[v][1], X.vl[v][2], X.P[j], e);
             X.s[v] = (int8_t)sv;
             ap |= sv > 0;
             an |= sv < 0;
         }
-        const bool any_err = __ballot(e) != 0, any_pos = __ballot(ap) != 0, any_neg = __ballot(an) != 0;
+        const bool any_err = W::any(e), any_pos = W::any(ap), any_neg = W::any(an);
         if (any_err) return fail();
         __syncthreads();
 
         // a plane containing an existing face joins that face's group
         int cop = INT_MAX;
-        for (int f = lane; f < nf0; f += 64) {
+        for (int f = lane; f < nf0; f += G) {
             uint8_t fvr[C::MFV];
             int8_t sr[C::MFV];
             (void)load_face<C>(X, f, fvr, sr);
             if (sign_flags<C::MFV>(sr) == 0 && f < cop) cop = f;
         }
-        cop = wave_min(cop);
+        cop = W::min(cop);
         if (cop != INT_MAX) {
             if (lane == 0) {
                 const uint8_t cell = X.fb[cop];
@@ -264,9 +276,9 @@ __device__ void wave_ia_cut(WaveIA<C>& X, int k) {
         // strict sign-change edges in (face, edge) order: each lane loads its
         // face's vertex ids and signs in parallel, counts and files them
         constexpr int MFV = C::MFV, CAND = WaveIA<C>::CAND, HT = WaveIA<C>::HT;
-        for (int h = lane; h < HT; h += 64) X.ht[h] = 0xFFFFFFFFu;
+        for (int h = lane; h < HT; h += G) X.ht[h] = 0xFFFFFFFFu;
         uint32_t ncand = 0;
-        for (int base = 0; base < nf0; base += 64) {
+        for (int base = 0; base < nf0; base += G) {
             const int f = base + lane;
             uint8_t fvr[MFV];
             int8_t sr[MFV];
@@ -286,7 +298,7 @@ __device__ void wave_ia_cut(WaveIA<C>& X, int k) {
                     c += i < n && sr[i] * sw < 0;
                 }
             }
-            uint32_t o = ncand + wave_excl(c);
+            uint32_t o = ncand + W::excl(c);
             if (f < nf0) {
                 X.fcand[f] = (uint16_t)o;
 #pragma unroll
@@ -310,7 +322,7 @@ __device__ void wave_ia_cut(WaveIA<C>& X, int k) {
                     ++o;
                 }
             }
-            ncand += wave_sum(c);
+            ncand += W::sum(c);
         }
         if (ncand > (uint32_t)CAND) return fail();
         __syncthreads();
@@ -323,7 +335,7 @@ __device__ void wave_ia_cut(WaveIA<C>& X, int k) {
             for (int probe = 0; probe < HT && X.ht[h] >> 16 != key; ++probe) h = (h + 1) & (uint32_t)(HT - 1);
             return X.ht[h] & 0xFFFFu;
         };
-        for (int c = lane; c < (int)ncand; c += 64) {
+        for (int c = lane; c < (int)ncand; c += G) {
             const uint32_t key = X.ckey[c], packed = key << 16 | (uint32_t)c;
             uint32_t h = home(key);
             for (int probe = 0; probe < HT; ++probe) {
@@ -337,16 +349,16 @@ __device__ void wave_ia_cut(WaveIA<C>& X, int k) {
             }
         }
         __syncthreads();
-        for (int c = lane; c < (int)ncand; c += 64) X.cfirst[c] = first_of(X.ckey[c]) == (uint32_t)c;
+        for (int c = lane; c < (int)ncand; c += G) X.cfirst[c] = first_of(X.ckey[c]) == (uint32_t)c;
         __syncthreads();
-        const uint32_t nnew = wave_prefix(
+        const uint32_t nnew = wave_prefix<G>(
             (int)ncand, [&](int c) { return (uint32_t)X.cfirst[c]; },
             [&](int c, uint32_t o) {
                 if (X.cfirst[c]) X.cvid[c] = (uint8_t)(nv0 + o);
             });
         if (nv0 + (int)nnew > C::MV) return fail();
         __syncthreads();
-        for (int c = lane; c < (int)ncand; c += 64) {
+        for (int c = lane; c < (int)ncand; c += G) {
             if (X.cfirst[c]) {
                 const int f = X.cface[c];
                 uint8_t t0 = X.fa[f], t1 = X.fe[f][X.cedge[c]], t2 = pid;
@@ -364,7 +376,7 @@ __device__ void wave_ia_cut(WaveIA<C>& X, int k) {
 
         // face splits: the positive part keeps the id, the negative part is
         // appended in face order
-        const uint32_t nmix = wave_prefix(
+        const uint32_t nmix = wave_prefix<G>(
             nf0,
             [&](int f) {
                 uint8_t fvr[C::MFV];
@@ -377,7 +389,7 @@ __device__ void wave_ia_cut(WaveIA<C>& X, int k) {
             [&](int f, uint32_t o) { X.frank[f] = (uint16_t)o; });
         if (nf0 + (int)nmix > C::MF) return fail();
         bool ferr = false;
-        for (int f = lane; f < nf0; f += 64) {
+        for (int f = lane; f < nf0; f += G) {
             if (X.fflag[f] != 3) continue;
             // the face in registers: ids, labels and signs load in parallel,
             // then both sides are streamed without further LDS reads but
@@ -418,14 +430,14 @@ __device__ void wave_ia_cut(WaveIA<C>& X, int k) {
             X.fc[dn] = X.fc[f];
             X.fn[f] = (uint8_t)npos;
         }
-        if (__ballot(ferr)) return fail();
+        if (W::any(ferr)) return fail();
         const int nf1 = nf0 + (int)nmix;
-        for (int c = lane; c < nc0; c += 64) X.cflag[c] = 0;
+        for (int c = lane; c < nc0; c += G) X.cflag[c] = 0;
         __syncthreads();
 
         // cell splits: the positive side keeps the id, the negative side and
         // one face on the new plane are appended in cell order
-        for (int g = lane; g < nf1; g += 64) {
+        for (int g = lane; g < nf1; g += G) {
             uint8_t fvr[C::MFV];
             int8_t sr[C::MFV];
             (void)load_face<C>(X, g, fvr, sr);
@@ -437,12 +449,12 @@ __device__ void wave_ia_cut(WaveIA<C>& X, int k) {
             }
         }
         __syncthreads();
-        const uint32_t nsplit = wave_prefix(
+        const uint32_t nsplit = wave_prefix<G>(
             nc0, [&](int c) { return (uint32_t)(X.cflag[c] == 3); },
             [&](int c, uint32_t o) { X.crank[c] = X.cflag[c] == 3 ? (uint16_t)o : (uint16_t)0xFFFF; });
         if (nf1 + (int)nsplit > C::MF || nc0 + (int)nsplit > WaveIA<C>::MC) return fail();
         __syncthreads();
-        for (int g = lane; g < nf1; g += 64) {
+        for (int g = lane; g < nf1; g += G) {
             if (!(X.fflag[g] & 2)) continue;
             const uint8_t b = X.fb[g], c = X.fc[g];
             if (X.crank[b] != 0xFFFF) X.fb[g] = (uint8_t)(nc0 + X.crank[b]);
@@ -457,10 +469,10 @@ __device__ void wave_ia_cut(WaveIA<C>& X, int k) {
         // from the earliest edge, always to the earliest edge that continues
         // it: the order of a scan of the faces one by one.
         if (nsplit > 64) return fail();
-        for (int c = lane; c < (int)nsplit; c += 64) X.ccnt[c] = 0;
+        for (int c = lane; c < (int)nsplit; c += G) X.ccnt[c] = 0;
         __syncthreads();
         bool cerr = false;
-        for (int g = lane; g < nf1; g += 64) {
+        for (int g = lane; g < nf1; g += G) {
             const uint8_t b = X.fb[g], cc = X.fc[g];
             const int rb = (b < nc0 && X.crank[b] != 0xFFFF) ? (int)X.crank[b] : -1;
             const int rc = (cc != NIL && cc < nc0 && X.crank[cc] != 0xFFFF) ? (int)X.crank[cc] : -1;
@@ -496,9 +508,9 @@ __device__ void wave_ia_cut(WaveIA<C>& X, int k) {
                 }
             }
         }
-        if (__ballot(cerr)) return fail();
+        if (W::any(cerr)) return fail();
         __syncthreads();
-        for (int c = lane; c < nc0 && !cerr; c += 64) {
+        for (int c = lane; c < nc0 && !cerr; c += G) {
             if (X.crank[c] == 0xFFFF) continue;
             const int row = X.crank[c];
             const int ne = (int)X.ccnt[row];
@@ -567,7 +579,7 @@ __device__ void wave_ia_cut(WaveIA<C>& X, int k) {
             X.fb[F] = (uint8_t)c;
             X.fc[F] = (uint8_t)(nc0 + row);
         }
-        if (__ballot(cerr)) return fail();
+        if (W::any(cerr)) return fail();
         __syncthreads();
         if (lane == 0) {
             X.nv = nv0 + (int)nnew;
@@ -584,18 +596,19 @@ __device__ void wave_ia_cut(WaveIA<C>& X, int k) {
 // slot has room, the word offset of every face within the record follows the
 // record (nf words past its length), so that readers need not walk the
 // variable-length face list.
-template <class C>
+template <class C, int G = 64>
 __device__ int wave_ia_record(WaveIA<C>& X, int k, uint16_t* out, int cap, uint32_t& niv, uint32_t& nif,
                               uint32_t& nfv) {
-    const int lane = wave_lane();
+    using W = Grp<G>;
+    const int lane = W::lane();
     const int L = 4 + k, nv = X.nv, nf = X.nf;
-    const uint32_t fwords = wave_prefix(
+    const uint32_t fwords = wave_prefix<G>(
         nf, [&](int f) { return (uint32_t)(4 + X.fn[f]); }, [&](int f, uint32_t o) { X.foff[f] = (uint16_t)o; });
     const int total = 8 + 4 * nv + (int)fwords + (X.dup ? 2 * L : 0);
-    for (int v = lane; v < nv; v += 64) X.isov[v] = 0;
+    for (int v = lane; v < nv; v += G) X.isov[v] = 0;
     __syncthreads();
     uint32_t cf = 0, cfv = 0;
-    for (int f = lane; f < nf; f += 64) {
+    for (int f = lane; f < nf; f += G) {
         bool is = X.fa[f] > 3;
         if (!is && X.dup) {
             const uint8_t rep = X.grp[X.fa[f]];
@@ -606,14 +619,14 @@ __device__ int wave_ia_record(WaveIA<C>& X, int k, uint16_t* out, int cap, uint3
         cfv += X.fn[f];
         for (int q = 0; q < X.fn[f]; ++q) X.isov[X.fv[f][q]] = 1;
     }
-    nif = wave_sum(cf);
-    nfv = wave_sum(cfv);
+    nif = W::sum(cf);
+    nfv = W::sum(cfv);
     __syncthreads();
     uint32_t cv = 0;
-    for (int v = lane; v < nv; v += 64) cv += X.isov[v];
-    niv = wave_sum(cv);
+    for (int v = lane; v < nv; v += G) cv += X.isov[v];
+    niv = W::sum(cv);
     if (total > cap) return -1;
-    for (int v = lane; v < nv; v += 64) {
+    for (int v = lane; v < nv; v += G) {
         uint16_t* o = out + 8 + 4 * v;
         o[0] = X.vl[v][0];
         o[1] = X.vl[v][1];
@@ -621,7 +634,7 @@ __device__ int wave_ia_record(WaveIA<C>& X, int k, uint16_t* out, int cap, uint3
         o[3] = 0xFFFF;
     }
     const int fb0 = 8 + 4 * nv;
-    for (int f = lane; f < nf; f += 64) {
+    for (int f = lane; f < nf; f += G) {
         uint16_t* o = out + fb0 + X.foff[f];
         o[0] = X.fa[f];
         o[1] = X.fb[f];
@@ -630,12 +643,12 @@ __device__ int wave_ia_record(WaveIA<C>& X, int k, uint16_t* out, int cap, uint3
         for (int q = 0; q < X.fn[f]; ++q) o[4 + q] = X.fv[f][q];
     }
     if (X.dup)
-        for (int l = lane; l < L; l += 64) {
+        for (int l = lane; l < L; l += G) {
             out[fb0 + fwords + l] = X.grp[l];
             out[fb0 + fwords + L + l] = X.ori[l];
         }
     if (total + nf <= cap)
-        for (int f = lane; f < nf; f += 64) out[total + f] = (uint16_t)(fb0 + X.foff[f]);
+        for (int f = lane; f < nf; f += G) out[total + f] = (uint16_t)(fb0 + X.foff[f]);
     if (lane == 0) {
         out[0] = 1;
         out[1] = (uint16_t)L;
