@@ -268,6 +268,83 @@ const K2Table& k2_table() {
     return T;
 }
 
+// Extraction plans of the k = 2 table records: what the record pass would
+// derive from each record, precomputed once.  A plan is bytes:
+//   [niv] [nif], then per iso vertex in vertex order [j] [plane 0..2] (the
+//   vertex's plane triple), then per iso face in face order
+//   [face] [function slot] [boundary code] [n] [n iso-vertex ranks],
+// where the function slot is the supporting plane (or, for a tet face, the
+// implicit plane of its coincidence group) minus 4 and the boundary code is
+// 1 + the tet face for faces on the tet boundary, else 0.
+struct K2Plans {
+    std::vector<uint32_t> off;  // per k = 2 slot: byte offset, or K2_PLAN_NONE
+    std::vector<uint8_t> bytes;
+};
+constexpr uint32_t K2_PLAN_NONE = 0xFFFFFFFFu;
+
+const K2Plans& k2_plans() {
+    static const K2Plans P = [] {
+        const K2Table& K2 = k2_table();
+        K2Plans Q;
+        Q.off.assign(K2.nslots, K2_PLAN_NONE);
+        for (uint32_t slot = 0; slot < K2.nslots; ++slot) {
+            if (K2.off[slot] == K2_NONE) continue;
+            const uint16_t* r = K2.words.data() + K2.off[slot];
+            if (r[0] != 1) continue;
+            const int L = r[1], nv = r[2], nf = r[3];
+            const bool dup = r[5] & 1;
+            const uint32_t len = r[6];
+            const uint16_t* grp = r + len - 2 * L;
+            std::vector<uint8_t> isov(nv, 0);
+            std::vector<const uint16_t*> iso_faces;
+            std::vector<int> iso_f;
+            const uint16_t* F = r + 8 + 4 * nv;
+            for (int f = 0; f < nf; ++f) {
+                const int n = F[3];
+                bool iso = F[0] > 3;
+                if (!iso && dup)
+                    for (int q = 4; q < L && !iso; ++q) iso = grp[q] == grp[F[0]];
+                if (iso) {
+                    for (int q = 0; q < n; ++q) isov[F[4 + q]] = 1;
+                    iso_faces.push_back(F);
+                    iso_f.push_back(f);
+                }
+                F += 4 + n;
+            }
+            std::vector<int> rank(nv, -1);
+            int niv = 0;
+            for (int j = 0; j < nv; ++j)
+                if (isov[j]) rank[j] = niv++;
+            if (niv > 255 || iso_faces.size() > 255) throw std::logic_error("k2 plan: record too large");
+            Q.off[slot] = (uint32_t)Q.bytes.size();
+            Q.bytes.push_back((uint8_t)niv);
+            Q.bytes.push_back((uint8_t)iso_faces.size());
+            for (int j = 0; j < nv; ++j)
+                if (isov[j]) {
+                    Q.bytes.push_back((uint8_t)j);
+                    for (int q = 0; q < 3; ++q) Q.bytes.push_back((uint8_t)r[8 + 4 * j + q]);
+                }
+            for (size_t i = 0; i < iso_faces.size(); ++i) {
+                const uint16_t* G = iso_faces[i];
+                int sp = G[0];
+                if (sp <= 3)
+                    for (int q = 4; q < L; ++q)
+                        if (grp[q] == grp[sp]) {
+                            sp = q;
+                            break;
+                        }
+                Q.bytes.push_back((uint8_t)iso_f[i]);
+                Q.bytes.push_back((uint8_t)(sp - 4));
+                Q.bytes.push_back(G[2] == 0xFFFF ? (uint8_t)(1 + G[0]) : (uint8_t)0);
+                Q.bytes.push_back((uint8_t)G[3]);
+                for (int q = 0; q < G[3]; ++q) Q.bytes.push_back((uint8_t)rank[G[4 + q]]);
+            }
+        }
+        return Q;
+    }();
+    return P;
+}
+
 __device__ inline int bucket_of(int k) { return k <= 1 ? 0 : (k == 2 ? 1 : (k <= 4 ? 2 : (k <= 8 ? 3 : 4))); }
 
 // per tet: active count and bucket histogram
@@ -860,6 +937,9 @@ struct ExtractArgs {
     uint32_t list_cap[3];
     uint32_t slot_words[3];   // words per record slot of each bucket
     uint32_t fo_table[3];     // 1: the bucket's cut leaves face offsets after a record that fits
+    const uint32_t* plan_off;  // k = 2 table records: extraction plans (k2_plans)
+    const uint8_t* plan;
+    uint32_t n_plans;
 };
 
 // Extraction in two passes so that waves stay uniform: the table pass
@@ -927,46 +1007,61 @@ __global__ void k_extract_records(ExtractArgs A) {
         uint32_t v[4];
         dev::tet_corners(A.g, t, v);
         const uint32_t s0 = A.csr_off[t];
-        // general record (same layout and rules as the plain extraction)
+        const uint32_t ti = loc & ~K1_FLAG;
+        const uint32_t po = ((loc & K1_FLAG) && ti >= 81 && ti - 81 < A.n_plans) ? A.plan_off[ti - 81] : K2_PLAN_NONE;
+        if (po != K2_PLAN_NONE) {
+            // a table record: stream its plan
+            const uint8_t* P = A.plan + po;
+            const int niv = P[0], nif = P[1];
+            const uint8_t* pv = P + 2;
+            const uint32_t vb = A.voff[a];
+            for (int x = 0; x < niv; ++x) {
+                const int j = pv[4 * x];
+                const uint64_t key =
+                    ia_vertex_key(A.g, t, v, a, j, pv[4 * x + 1], pv[4 * x + 2], pv[4 * x + 3], A.fit, s0);
+                A.okey[vb + x] = key;
+                A.oa[vb + x] = a;
+                A.oj[vb + x] = (uint16_t)j;
+            }
+            const uint8_t* pf = pv + 4 * niv;
+            uint32_t q = A.foff[a], e = A.fvoff[a];
+            for (int x = 0; x < nif; ++x) {
+                const int n = pf[3];
+                A.fq_a[q] = a;
+                A.fq_l[q] = pf[0];
+                A.fq_fn[q] = A.fit[s0 + pf[1]];
+                A.fq_bnd[q] = pf[2];
+                A.fq_vs[q] = e;
+                for (int c = 0; c < n; ++c) A.fve[e++] = vb + pf[4 + c];
+                ++q;
+                pf += 4 + n;
+            }
+            continue;
+        }
+        // general record (same layout and rules as the plain extraction); a
+        // k = 2 complex has at most 16 vertices, so one 64-bit mask holds
+        // its iso vertices
         const Rec R((loc & K1_FLAG) ? A.tab + A.tab_off[loc & ~K1_FLAG] : A.rec + loc);
         if (R.r[0] != 1) continue;
+        if (R.nv > 64) {
+            A.errs[threadIdx.x & 63] |= ERR_RECORD;
+            continue;
+        }
         recw += R.r[6];
-        uint64_t iso[4] = {0, 0, 0, 0};
+        uint64_t iso = 0;
         const uint16_t* F = R.faces();
         for (int f = 0; f < R.nf; ++f) {
             const int n = F[3];
             if (ia_face_is_iso(R, F))
-                for (int q = 0; q < n; ++q) iso[F[4 + q] >> 6] |= 1ull << (F[4 + q] & 63);
+                for (int q = 0; q < n; ++q) iso |= 1ull << (F[4 + q] & 63);
             F += 4 + n;
         }
         uint32_t o = A.voff[a];
         for (int j = 0; j < R.nv; ++j) {
-            if (!((iso[j >> 6] >> (j & 63)) & 1)) continue;
+            if (!((iso >> j) & 1)) continue;
             const uint16_t* T3 = R.vert(j);
-            int bnd[3], nb = 0;
-            uint32_t fn[3];
-            int ni = 0;
-            for (int q = 0; q < 3; ++q) {
-                if (T3[q] > 3)
-                    fn[ni++] = A.fit[s0 + T3[q] - 4];
-                else
-                    bnd[nb++] = T3[q];
-            }
-            uint64_t key;
-            if (nb == 3) {
-                int c = 0;
-                while (c == bnd[0] || c == bnd[1] || c == bnd[2]) ++c;
-                key = key_corner(v[c]);
-            } else if (nb == 2) {
-                int l[2], nl = 0;
-                for (int c = 0; c < 4; ++c)
-                    if (c != bnd[0] && c != bnd[1]) l[nl++] = c;
-                key = key_edge(dev::edge_id(A.g, t, v, l[0], l[1]), fn[0]);
-            } else if (nb == 1) {
-                key = key_face(dev::face_id(A.g, t, v, bnd[0]), fn[0], fn[1]);
-            } else {
-                key = key_inner(a, j);
-            }
+            const uint64_t key =
+                ia_vertex_key(A.g, t, v, a, j, T3[0], T3[1], T3[2], A.fit, s0);
             A.okey[o] = key;
             A.oa[o] = (uint32_t)a;
             A.oj[o] = (uint16_t)j;
@@ -993,11 +1088,8 @@ __global__ void k_extract_records(ExtractArgs A) {
                 A.fq_bnd[q] = F[2] == 0xFFFF ? (uint8_t)(1 + F[0]) : 0;
                 A.fq_vs[q] = e;
                 for (int i = 0; i < n; ++i) {
-                    const int j = F[4 + i];
-                    uint32_t rank = 0;
-                    for (int w = 0; w < (j >> 6); ++w) rank += __popcll(iso[w]);
-                    rank += __popcll(iso[j >> 6] & ((1ull << (j & 63)) - 1));
-                    A.fve[e++] = vb + rank;
+                    const int j = F[4 + i] & 63;
+                    A.fve[e++] = vb + (uint32_t)__popcll(iso & ((1ull << j) - 1));
                 }
                 ++q;
             }
@@ -1075,30 +1167,8 @@ __global__ void __launch_bounds__(64) k_extract_records_wave(ExtractArgs A) {
     for (int j = lane; j < nv; j += 64) {
         if (!((isov[j >> 5] >> (j & 31)) & 1u)) continue;
         const uint16_t* T3 = W + 8 + 4 * j;
-        int bnd[3], nb = 0;
-        uint32_t fn[3];
-        int ni = 0;
-        for (int q = 0; q < 3; ++q) {
-            if (T3[q] > 3)
-                fn[ni++] = A.fit[s0 + T3[q] - 4];
-            else
-                bnd[nb++] = T3[q];
-        }
-        uint64_t key;
-        if (nb == 3) {
-            int c = 0;
-            while (c == bnd[0] || c == bnd[1] || c == bnd[2]) ++c;
-            key = key_corner(v[c]);
-        } else if (nb == 2) {
-            int l[2], nl = 0;
-            for (int c = 0; c < 4; ++c)
-                if (c != bnd[0] && c != bnd[1]) l[nl++] = c;
-            key = key_edge(dev::edge_id(A.g, t, v, l[0], l[1]), fn[0]);
-        } else if (nb == 1) {
-            key = key_face(dev::face_id(A.g, t, v, bnd[0]), fn[0], fn[1]);
-        } else {
-            key = key_inner(a, j);
-        }
+        const uint64_t key =
+            ia_vertex_key(A.g, t, v, a, j, T3[0], T3[1], T3[2], A.fit, s0);
         const uint32_t o = vb + vrank(j);
         A.okey[o] = key;
         A.oa[o] = a;
@@ -1218,6 +1288,8 @@ struct Workspace {
     DBuf<uint16_t> k1_words;  // table records: k = 1 patterns, then k = 2 slots
     DBuf<uint32_t> k1_off;    // word offset per table index (81 + k = 2 slots)
     DBuf<K2Pat> k2pat;
+    DBuf<uint32_t> k2plan_off;
+    DBuf<uint8_t> k2plan;
     DBuf<uint32_t> k2cnt;
     // the path as a few captured graphs launched back to back; stream events
     // between them time each part on every replay
@@ -1416,6 +1488,11 @@ void alloc_ws(Workspace& W, const Grid& g, const Caps_& c) {
     RIN_HIP(hipMemcpy(W.k1_off.p, off.data(), 4 * off.size(), hipMemcpyHostToDevice));
     RIN_HIP(hipMemcpy(W.k2pat.p, K2.pat, sizeof(K2.pat), hipMemcpyHostToDevice));
     RIN_HIP(hipMemcpy(W.k2cnt.p, K2.cnt.data(), 4 * K2.cnt.size(), hipMemcpyHostToDevice));
+    const K2Plans& KP = k2_plans();
+    W.k2plan_off.alloc(m, KP.off.size() + 1);
+    W.k2plan.alloc(m, KP.bytes.size() + 1);
+    RIN_HIP(hipMemcpy(W.k2plan_off.p, KP.off.data(), 4 * KP.off.size(), hipMemcpyHostToDevice));
+    RIN_HIP(hipMemcpy(W.k2plan.p, KP.bytes.data(), KP.bytes.size(), hipMemcpyHostToDevice));
 }
 
 // stage 1 (also run eagerly to size the workspace): signs, filter, and
@@ -1604,6 +1681,9 @@ void stage_rest(Workspace& W, const Grid& g, const Values& val, hipStream_t s, i
             E.slot_words[q] = W.slot[q];
             E.fo_table[q] = W.wave_cut[q] ? 1u : 0u;
         }
+        E.plan_off = W.k2plan_off.p;
+        E.plan = W.k2plan.p;
+        E.n_plans = (uint32_t)k2_plans().off.size();
         if (part & REST_EXTRACT_K1) hipLaunchKernelGGL(k_extract_k1, dim3(cgrid(c.active)), dim3(256), 0, s, E);
         if (part & REST_EXTRACT_REC) {
             hipLaunchKernelGGL(k_extract_records_wave, dim3(std::max(1u, c.b[1] + c.b[2])), dim3(64), 0, s, E);

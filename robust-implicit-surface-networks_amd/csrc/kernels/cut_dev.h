@@ -80,7 +80,6 @@ __host__ __device__ inline void sort4(T* v) {
 // planes exceed the label capacity
 template <class C>
 __host__ __device__ void ia_init(Complex<C>& X, int k) {
-    using I = typename C::Idx;
     X.err = false;
     X.dup = false;
     X.nv = 4;
@@ -345,7 +344,6 @@ __host__ __device__ void ia_cut(const double (*P)[4], int k, Complex<C>& X) {
 // record layout: include/rin/cut_record.h; returns words written or -1
 template <class C>
 __host__ __device__ int write_record_ia(const Complex<C>& X, int k, uint16_t* out, int cap) {
-    using I = typename C::Idx;
     int p = 8;
     const int L = 4 + k;
     auto put = [&](uint16_t w) {
@@ -761,7 +759,6 @@ __host__ __device__ void mi_cut(const double (*P)[4], int k, Complex<C>& X) {
 
 template <class C>
 __host__ __device__ int write_record_mi(const Complex<C>& X, int k, uint16_t* out, int cap) {
-    using I = typename C::Idx;
     int p = 8;
     const int L = 4 + k;
     auto put = [&](uint16_t w) {

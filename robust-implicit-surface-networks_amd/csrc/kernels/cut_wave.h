@@ -49,9 +49,12 @@ struct WaveIA {
     static constexpr int HT = CAND <= 64 ? 128 : (CAND <= 128 ? 256 : (CAND <= 512 ? 1024 : 2048));
     uint32_t ht[HT];
     uint8_t isov[MV];
-    uint8_t tv[64][MFV], te[64][MFV], tb[64][MFV];  // per split cell rows (new-face edges)
-    uint16_t tk[64][MFV];                            // collection order of a split cell's edges
-    uint32_t ccnt[64];                               // edges collected per split cell
+    // rows per split cell (new-face edges): an insertion splits at most the
+    // cells already there, and up to 4 planes cut a tet into at most 15
+    static constexpr int ROWS = MP <= 8 ? 16 : 64;
+    uint8_t tv[ROWS][MFV], te[ROWS][MFV], tb[ROWS][MFV];
+    uint16_t tk[ROWS][MFV];  // collection order of a split cell's edges
+    uint32_t ccnt[ROWS];     // edges collected per split cell
     int nv, nf, nc, dup, err;
 };
 
@@ -468,7 +471,7 @@ __device__ void wave_ia_cut(WaveIA<C>& X, int k) {
         // (face, position) order; a lane per split cell then chains its row
         // from the earliest edge, always to the earliest edge that continues
         // it: the order of a scan of the faces one by one.
-        if (nsplit > 64) return fail();
+        if (nsplit > (uint32_t)WaveIA<C>::ROWS) return fail();
         for (int c = lane; c < (int)nsplit; c += G) X.ccnt[c] = 0;
         __syncthreads();
         bool cerr = false;

@@ -366,6 +366,31 @@ __device__ inline uint64_t key_face(uint64_t fc, uint32_t f0, uint32_t f1) {
 }
 __device__ inline uint64_t key_inner(uint64_t a, uint32_t j) { return ((a << 10 | j) << 2) | 3; }
 
+// Key of the iso vertex j of active tet a (tet t, corners v) given its plane
+// triple (ids 0..3 tet faces, 4.. implicit planes in CSR order from s0):
+// three tet faces meet at a corner, two at an edge, one at a face, none
+// inside.  Bit logic instead of indexed local arrays keeps everything in
+// registers.
+__device__ inline uint64_t ia_vertex_key(const GridView& g, uint64_t t, const uint32_t v[4], uint64_t a, int j,
+                                         int t0, int t1, int t2, const uint8_t* __restrict__ fit, uint32_t s0) {
+    unsigned bm = 0;
+    if (t0 < 4) bm |= 1u << t0;
+    if (t1 < 4) bm |= 1u << t1;
+    if (t2 < 4) bm |= 1u << t2;
+    const int nb = __popc(bm);
+    // implicit planes in triple order
+    const int i0 = t0 >= 4 ? t0 : (t1 >= 4 ? t1 : t2);
+    const int i1 = (t0 >= 4 && t1 >= 4) ? t1 : t2;
+    if (nb == 3) return key_corner(v[__builtin_ctz(~bm & 15u)]);
+    if (nb == 2) {
+        const unsigned m = ~bm & 15u;
+        const int l0 = __builtin_ctz(m), l1 = __builtin_ctz(m & (m - 1));
+        return key_edge(dev::edge_id(g, t, v, l0, l1), fit[s0 + i0 - 4]);
+    }
+    if (nb == 1) return key_face(dev::face_id(g, t, v, __builtin_ctz(bm)), fit[s0 + i0 - 4], fit[s0 + i1 - 4]);
+    return key_inner(a, j);
+}
+
 struct Rec {
     const uint16_t* r;
     int L, nv, nf;
@@ -414,30 +439,8 @@ __global__ void k_extract_ia(GridView g, const uint32_t* __restrict__ act, const
         for (int j = 0; j < R.nv; ++j) {
             if (!((iso[j >> 6] >> (j & 63)) & 1)) continue;
             const uint16_t* T3 = R.vert(j);
-            int bnd[3], nb = 0;
-            uint32_t fn[3];
-            int ni = 0;
-            for (int q = 0; q < 3; ++q) {
-                if (T3[q] > 3)
-                    fn[ni++] = fit[s0 + T3[q] - 4];
-                else
-                    bnd[nb++] = T3[q];
-            }
-            uint64_t key;
-            if (nb == 3) {
-                int c = 0;
-                while (c == bnd[0] || c == bnd[1] || c == bnd[2]) ++c;
-                key = key_corner(v[c]);
-            } else if (nb == 2) {
-                int l[2], nl = 0;
-                for (int c = 0; c < 4; ++c)
-                    if (c != bnd[0] && c != bnd[1]) l[nl++] = c;
-                key = key_edge(dev::edge_id(g, t, v, l[0], l[1]), fn[0]);
-            } else if (nb == 1) {
-                key = key_face(dev::face_id(g, t, v, bnd[0]), fn[0], fn[1]);
-            } else {
-                key = key_inner(a, j);
-            }
+            const uint64_t key =
+                ia_vertex_key(g, t, v, a, j, T3[0], T3[1], T3[2], fit, s0);
             okey[o] = key;
             oa[o] = (uint32_t)a;
             oj[o] = (uint16_t)j;
