@@ -289,8 +289,8 @@ def main():
                 "kernel_ms": stages[ks], "algorithmic_bytes": B, "traffic_source": tsrc,
                 "timing": "stream events around the kernel's own captured graph, in per-stage replays after "
                           "the timed steps",
-                "note": "the cut runs exact-predicate insertions in LDS, one wavefront per tet; it is bound by "
-                        "instruction issue and latency, not HBM, so frac is small by nature"}
+                "note": "the cut runs exact-predicate plane insertions on a complex in LDS, a lane group per tet; "
+                        "it is bound by instruction issue and LDS latency, not HBM, so frac is small by nature"}
     pb = path_bytes(V, F, sizes)
     out = {
         "metric": METRIC,
