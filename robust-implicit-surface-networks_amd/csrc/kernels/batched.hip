@@ -1211,8 +1211,26 @@ __global__ void __launch_bounds__(64) k_extract_records_wave(ExtractArgs A) {
     }
 }
 
+// the host-visible summary of a run: the counters, then the union of the
+// error flags (one 64-lane block)
+constexpr int kPubWords = 33;
+static_assert(C_N <= 32, "the published summary holds 32 counters");
+__device__ void publish_counters(const uint32_t* ctr, const uint32_t* errs, uint32_t* pub) {
+    if (threadIdx.x < C_N) pub[threadIdx.x] = ctr[threadIdx.x];
+    uint32_t e = errs[threadIdx.x];
+    for (int d = 32; d > 0; d >>= 1) e |= __shfl_xor(e, d, 64);
+    if (threadIdx.x == 0) pub[32] = e;
+    __threadfence_system();
+}
+
+__global__ void k_publish(const uint32_t* ctr, const uint32_t* errs, uint32_t* pub) {
+    publish_counters(ctr, errs, pub);
+}
+
+// fold the spread counters, close the CSR and face offset arrays, and publish
+// the counters (the last kernel of a run)
 __global__ void k_write_totals(uint32_t* ctr, uint64_t T, uint32_t* csr_off, uint32_t* f_vstart,
-                               uint32_t* f_tfstart, uint32_t cap_fo) {
+                               uint32_t* f_tfstart, uint32_t cap_fo, const uint32_t* errs, uint32_t* pub) {
     const int fold_to[SP_N] = {C_RECW, C_CUTK3, C_CUTW3, C_CUTK5, C_CUTW5};
     for (int c = 0; c < SP_N; ++c) {
         uint32_t v = threadIdx.x < kSpread ? ctr[spread_at(c, threadIdx.x)] : 0u;
@@ -1225,6 +1243,8 @@ __global__ void k_write_totals(uint32_t* ctr, uint64_t T, uint32_t* csr_off, uin
         f_vstart[nf] = ctr[C_FACEVERTS];
         f_tfstart[nf] = ctr[C_ABORT] ? 0u : min(ctr[C_TETFACES], cap_fo);
     }
+    __syncthreads();
+    publish_counters(ctr, errs, pub);
 }
 
 __global__ void k_zero_ctr(uint32_t* ctr, int n, uint32_t* errs) {
@@ -1303,6 +1323,10 @@ struct Workspace {
     // the k = 3..4 cut): a stream to capture it on and the fork / join events
     hipStream_t side = nullptr;
     hipEvent_t fork = nullptr, join = nullptr;
+    // pinned, device-mapped summary the last kernel of a run writes (counters
+    // and error flags), read by the host after the stream synchronises
+    uint32_t* pub_h = nullptr;
+    uint32_t* pub_d = nullptr;
     bool exec = false;  // graphs captured for the current capacities
     bool in_use = false;
     void drop_graphs() {
@@ -1321,6 +1345,7 @@ struct Workspace {
         if (fork) (void)hipEventDestroy(fork);
         if (join) (void)hipEventDestroy(join);
         if (side) (void)hipStreamDestroy(side);
+        if (pub_h) (void)hipHostFree(pub_h);
     }
 };
 
@@ -1789,7 +1814,7 @@ void stage_rest(Workspace& W, const Grid& g, const Values& val, hipStream_t s, i
         FO.f_func = W.f_func.p;
         hipLaunchKernelGGL(k_face_emit, dim3(cgrid(c.fo)), dim3(256), 0, s, FO);
         hipLaunchKernelGGL(k_write_totals, dim3(1), dim3(64), 0, s, ctr, g.view.T, W.csr_off.p,
-                           W.f_vstart.p, W.f_tfstart.p, c.fo);
+                           W.f_vstart.p, W.f_tfstart.p, c.fo, W.errs.p, W.pub_d);
     }
 }
 
@@ -1856,18 +1881,22 @@ void run_batched(DevMem& ctx_mem, const Grid& g, Values& val, uint32_t flags, Ru
     }
     Workspace& w = *W;
 
-    uint32_t hctr[32];
-    uint32_t herr[64];
-    auto read_back = [&] {
-        RIN_HIP(hipMemcpyAsync(hctr, w.ctr.p, sizeof(hctr), hipMemcpyDeviceToHost, s));
-        RIN_HIP(hipMemcpyAsync(herr, w.errs.p, sizeof(herr), hipMemcpyDeviceToHost, s));
+    if (!w.pub_h) {
+        RIN_HIP(hipHostMalloc((void**)&w.pub_h, kPubWords * sizeof(uint32_t), hipHostMallocMapped | hipHostMallocCoherent));
+        RIN_HIP(hipHostGetDevicePointer((void**)&w.pub_d, w.pub_h, 0));
+    }
+    uint32_t hctr[32] = {};
+    uint32_t herr = 0;
+    // the run's last kernel has written the summary; after an eager pass that
+    // ends before it, `publish` launches the summary kernel itself
+    auto read_back = [&](bool publish) {
+        if (publish) hipLaunchKernelGGL(k_publish, dim3(1), dim3(64), 0, s, w.ctr.p, w.errs.p, w.pub_d);
+        RIN_HIP(hipGetLastError());
         RIN_HIP(hipStreamSynchronize(s));
+        for (int i = 0; i < C_N; ++i) hctr[i] = w.pub_h[i];
+        herr = w.pub_h[32];
     };
-    auto errors = [&] {
-        uint32_t e = 0;
-        for (uint32_t x : herr) e |= x;
-        return e;
-    };
+    auto errors = [&] { return herr; };
     bool ran = false;
     for (int attempt = 0; attempt < 3 && !ran; ++attempt) {
         if (!w.exec) {
@@ -1880,7 +1909,7 @@ void run_batched(DevMem& ctx_mem, const Grid& g, Values& val, uint32_t flags, Ru
                 alloc_ws(w, g, tiny);
             }
             stage_count(w, g, val, s);
-            read_back();
+            read_back(true);
             if (hctr[C_BX]) throw NeedsPlainForm("a tet has more than 8 active functions");
             Caps_ need;
             caps_from_counts(hctr, need);
@@ -2025,7 +2054,7 @@ void run_batched(DevMem& ctx_mem, const Grid& g, Values& val, uint32_t flags, Ru
             RIN_HIP(hipGraphLaunch(w.whole, s));
             RIN_HIP(hipEventRecord(w.ev[1], s));
         }
-        read_back();
+        read_back(false);
         const uint32_t e = errors();
         if (e) w.hash_dirty = true;
         if (e & ERR_OVERFLOW) {
