@@ -99,6 +99,51 @@ def cpu_baseline(res, fs):
             "stage_s": {k: round(v, 4) for k, v in r.timings.items()}}
 
 
+def mi_leg(ctx, steps, warmup, with_cpu):
+    """Material interface on the 65^3-vertex grid (res 64, 1,310,720 tets)
+    with 8 seeded materials: device tets/s of the plain form (the batched form
+    covers implicit arrangements) and the oracle's MI stages on the same input."""
+    import rin
+    import workloads as W
+
+    res, fs = W.config3()
+    types, params = fs.arrays()
+    grid = ctx.grid_regular(res)
+    vals = grid.eval(types, params)
+    T = grid.T
+    for _ in range(warmup):
+        vals.run(rin.FLAG_MI).release()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        r = vals.run(rin.FLAG_MI)
+        if i + 1 < steps:
+            r.release()
+    dt = (time.perf_counter() - t0) / steps
+    sizes = r.sizes
+    r.release()
+    out = {"workload": "material interface, %d^3 vertices (res %d, %d tets), 8 seeded materials "
+                       "-|x - p_i| + w_i, values resident in HBM" % (res + 1, res, T),
+           "form": "plain", "tets_per_s": T / dt, "ms_per_step": dt * 1e3, "steps": steps,
+           "active_tets": int(sizes.num_intersecting), "mi_verts": int(sizes.n_verts),
+           "mi_faces": int(sizes.n_faces)}
+    if with_cpu:
+        import support as S
+
+        pts, tets = S.grid(res)
+        hv = S.evaluate(types, params, pts)
+        orc = S.OracleRun(1, pts, tets, hv)
+        stage = (orc.timings["highest func"] + orc.timings["filter"] + orc.timings["MI(wall)"] +
+                 orc.timings["extract mesh"] + orc.timings["compute xyz"])
+        cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        out["cpu_baseline"] = {"value": T / stage, "unit": "tets/s", "cores": cores, "kind": "port",
+                               "sample": "the full MI workload once: oracle stages highest func, filter, "
+                                         "per-tet cut (OpenMP, %d threads), extraction and xyz; %.2f s" % (cores, stage)}
+        out["vs_cpu"] = out["tets_per_s"] / out["cpu_baseline"]["value"]
+    vals.release()
+    grid.release()
+    return out
+
+
 def traffic_for(tag):
     """HBM bytes per launch from the committed counter summary, or None."""
     p = os.path.join(ROOT, "profiles", "traffic_%s_%s.json" % (tag, WORKLOAD))
@@ -198,6 +243,8 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=20.0, help="serving arrivals per second (Poisson)")
     ap.add_argument("--serve-slots", type=int, default=1,
                     help="server slots (the library currently runs one job at a time)")
+    ap.add_argument("--mi-steps", type=int, default=10,
+                    help="timed steps of the material-interface leg (0: skip)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also run plain device copies of MIB MiB (reference kernel for counter scaling)")
     ap.add_argument("--pmc-summary", nargs=3, metavar=("FETCH_CSV", "WRITE_CSV", "TRACE_CSV"),
@@ -330,6 +377,8 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(res, fs)
         out["vs_cpu"] = out["value"] / out["cpu_baseline"]["value"]
+    if args.mi_steps:
+        out["material_interface"] = mi_leg(ctx, args.mi_steps, 2, not args.no_cpu)
     print(json.dumps(out))
     vals.release()
     grid.release()

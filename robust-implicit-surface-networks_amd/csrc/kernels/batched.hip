@@ -465,14 +465,18 @@ __global__ void k_signs_count(const double* __restrict__ vals, uint64_t V, uint3
     uint32_t z = 0;
     GRID_STRIDE(v, V) {
         uint64_t p = 0, n = 0;
-        for (uint32_t f = 0; f < F; ++f) {
-            const double x = vals[(uint64_t)f * V + v];
-            if (x > 0)
-                p |= 1ull << f;
-            else if (x < 0)
-                n |= 1ull << f;
-            else
-                ++z;
+        // eight function rows in flight per lane: the pass is bound by the
+        // number of loads outstanding, not by their issue
+        for (uint32_t f0 = 0; f0 < F; f0 += 8) {
+            double x[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) x[u] = f0 + u < F ? vals[(uint64_t)(f0 + u) * V + v] : 1.0;
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                p |= (uint64_t)(x[u] > 0) << (f0 + u);
+                n |= (uint64_t)(x[u] < 0) << (f0 + u);
+                z += x[u] == 0;
+            }
         }
         pos[v] = p;
         neg[v] = n;

@@ -213,7 +213,7 @@ __device__ inline void ia_counts(const Complex<C>& X, uint32_t& niv, uint32_t& n
 }
 
 template <class C>
-__device__ inline void mi_counts(const Complex<C>& X, uint32_t& niv, uint32_t& nif, uint32_t& nfv) {
+__host__ __device__ inline void mi_counts(const Complex<C>& X, uint32_t& niv, uint32_t& nif, uint32_t& nfv) {
     // MI faces: positive label > 3; boundary faces are resolved across tets
     constexpr int W = (C::MV + 63) / 64;
     uint64_t iso[W];
@@ -226,7 +226,60 @@ __device__ inline void mi_counts(const Complex<C>& X, uint32_t& niv, uint32_t& n
         for (int q = 0; q < X.fn[f]; ++q) iso[X.fv[f][q] >> 6] |= 1ull << (X.fv[f][q] & 63);
     }
     niv = 0;
-    for (int w = 0; w < W; ++w) niv += (uint32_t)__popcll(iso[w]);
+    for (int w = 0; w < W; ++w) niv += (uint32_t)__builtin_popcountll(iso[w]);
+}
+
+// One active tet's cut into its record slot.
+template <class C, bool MI>
+__device__ inline void cut_tet(const GridView& g, const double* __restrict__ vals, const uint32_t* __restrict__ act,
+                               const uint32_t* __restrict__ csr_off, const uint8_t* __restrict__ fit, uint64_t a, int k,
+                               uint32_t flags, const uint32_t* __restrict__ rec_off, uint16_t* __restrict__ rec,
+                               uint32_t* __restrict__ cnt, uint32_t* __restrict__ errs, Complex<C>& X) {
+    const uint64_t V = g.V;
+    const uint32_t t = act[a];
+    uint32_t v[4];
+    dev::tet_corners(g, t, v);
+    const uint32_t s0 = csr_off[t];
+    double P[C::MP - 4][4];
+    for (int q = 0; q < k; ++q) {
+        const uint32_t f = fit[s0 + ((flags & RUN_REVERSE) ? k - 1 - q : q)];
+        for (int c = 0; c < 4; ++c) P[q][c] = vals[(uint64_t)f * V + v[c]];
+    }
+    if (MI)
+        dev::mi_cut<C>(P, k, X);
+    else
+        dev::ia_cut<C>(P, k, X);
+    if (X.err) {
+        // capacities or predicate range: cut again with the large kernel
+        flag_error(errs, ERR_RETRY);
+        cnt[3 * a] = CNT_RETRY;
+        cnt[3 * a + 1] = cnt[3 * a + 2] = 0;
+        rec[rec_off[a]] = 0;
+        return;
+    }
+    const int cap = (int)(rec_off[a + 1] - rec_off[a]);
+    const int w = MI ? dev::write_record_mi<C>(X, k, rec + rec_off[a], cap)
+                     : dev::write_record_ia<C>(X, k, rec + rec_off[a], cap);
+    if (w < 0) {
+        flag_error(errs, ERR_RETRY);
+        cnt[3 * a] = CNT_RETRY;
+        cnt[3 * a + 1] = cnt[3 * a + 2] = 0;
+        rec[rec_off[a]] = 0;
+        return;
+    }
+    uint32_t a0, a1, a2;
+    if (flags & RUN_CUT_ONLY) {
+        a0 = X.nv;
+        a1 = X.nf;
+        a2 = X.nc;
+    } else if (MI) {
+        mi_counts<C>(X, a0, a1, a2);
+    } else {
+        ia_counts<C>(X, a0, a1, a2);
+    }
+    cnt[3 * a] = a0;
+    cnt[3 * a + 1] = a1;
+    cnt[3 * a + 2] = a2;
 }
 
 template <class C, bool MI>
@@ -236,56 +289,12 @@ __global__ void __launch_bounds__(64) k_cut(GridView g, const double* __restrict
                                             uint32_t flags, const uint32_t* __restrict__ rec_off,
                                             uint16_t* __restrict__ rec, uint32_t* __restrict__ cnt,
                                             uint32_t* __restrict__ errs, const uint32_t* __restrict__ list) {
-    const uint64_t V = g.V;
     GRID_STRIDE(i, n_active) {
         const uint64_t a = list ? list[i] : i;
         const int k = act_k[a];
         if (k < kmin || k > kmax) continue;
-        const uint32_t t = act[a];
-        uint32_t v[4];
-        dev::tet_corners(g, t, v);
-        const uint32_t s0 = csr_off[t];
-        double P[C::MP - 4][4];
-        for (int q = 0; q < k; ++q) {
-            const uint32_t f = fit[s0 + ((flags & RUN_REVERSE) ? k - 1 - q : q)];
-            for (int c = 0; c < 4; ++c) P[q][c] = vals[(uint64_t)f * V + v[c]];
-        }
         Complex<C> X;
-        if (MI)
-            dev::mi_cut<C>(P, k, X);
-        else
-            dev::ia_cut<C>(P, k, X);
-        if (X.err) {
-            // capacities or predicate range: cut again with the large kernel
-            flag_error(errs, ERR_RETRY);
-            cnt[3 * a] = CNT_RETRY;
-            cnt[3 * a + 1] = cnt[3 * a + 2] = 0;
-            rec[rec_off[a]] = 0;
-            continue;
-        }
-        const int cap = (int)(rec_off[a + 1] - rec_off[a]);
-        const int w = MI ? dev::write_record_mi<C>(X, k, rec + rec_off[a], cap)
-                         : dev::write_record_ia<C>(X, k, rec + rec_off[a], cap);
-        if (w < 0) {
-            flag_error(errs, ERR_RETRY);
-            cnt[3 * a] = CNT_RETRY;
-            cnt[3 * a + 1] = cnt[3 * a + 2] = 0;
-            rec[rec_off[a]] = 0;
-            continue;
-        }
-        uint32_t a0, a1, a2;
-        if (flags & RUN_CUT_ONLY) {
-            a0 = X.nv;
-            a1 = X.nf;
-            a2 = X.nc;
-        } else if (MI) {
-            mi_counts<C>(X, a0, a1, a2);
-        } else {
-            ia_counts<C>(X, a0, a1, a2);
-        }
-        cnt[3 * a] = a0;
-        cnt[3 * a + 1] = a1;
-        cnt[3 * a + 2] = a2;
+        cut_tet<C, MI>(g, vals, act, csr_off, fit, a, k, flags, rec_off, rec, cnt, errs, X);
     }
 }
 

@@ -59,6 +59,57 @@ __global__ void k_pair_faces(uint64_t n, const uint64_t* __restrict__ key, const
     }
 }
 
+// ------------------------------------------------------------------ 2-material table
+//
+// With two active materials a and b (a < b, CSR order) the only
+// classification the cut makes is material b against material a at the four
+// tet corners (exact comparisons of the corner values), so the complex and
+// its record depend on those four comparisons alone: one record per pattern
+// sum_c s_c 3^c (s_c = 0: b < a, 1: tie, 2: b > a), built once by the cut
+// itself (the reference's 2-material lookup table plays the same part,
+// src/material_interface.cpp:320-328).
+
+struct MI2Entry {
+    uint16_t off, len;   // record words in the table's word array
+    uint32_t cnt[3];     // interface vertices, faces, face corners
+    uint32_t cut[3];     // vertices, faces, cells of the complex
+};
+
+__global__ void k_mi_table2(GridView g, const double* __restrict__ vals, const uint32_t* __restrict__ act,
+                            const uint32_t* __restrict__ csr_off, const uint8_t* __restrict__ fit,
+                            const uint32_t* __restrict__ list, uint64_t n, uint32_t flags,
+                            const uint32_t* __restrict__ rec_off, uint16_t* __restrict__ rec,
+                            uint32_t* __restrict__ cnt, uint32_t* __restrict__ errs, const MI2Entry* __restrict__ tab,
+                            const uint16_t* __restrict__ words) {
+    const uint64_t V = g.V;
+    GRID_STRIDE(i, n) {
+        const uint32_t a = list[i];
+        const uint32_t t = act[a];
+        uint32_t v[4];
+        dev::tet_corners(g, t, v);
+        const uint32_t s0 = csr_off[t];
+        const double* fa = vals + (uint64_t)fit[s0] * V;
+        const double* fb = vals + (uint64_t)fit[s0 + 1] * V;
+        int pat = 0;
+        for (int c = 0, w = 1; c < 4; ++c, w *= 3) {
+            const double x = fa[v[c]], y = fb[v[c]];
+            pat += (y > x ? 2 : (y < x ? 0 : 1)) * w;
+        }
+        const MI2Entry E = tab[pat];
+        const uint32_t at = rec_off[a];
+        if (E.len > rec_off[a + 1] - at) {
+            flag_error(errs, ERR_RETRY);
+            cnt[3 * (uint64_t)a] = CNT_RETRY;
+            cnt[3 * (uint64_t)a + 1] = cnt[3 * (uint64_t)a + 2] = 0;
+            rec[at] = 0;
+            continue;
+        }
+        for (int w = 0; w < E.len; ++w) rec[at + w] = words[E.off + w];
+        const uint32_t* c3 = (flags & RUN_CUT_ONLY) ? E.cut : E.cnt;
+        for (int q = 0; q < 3; ++q) cnt[3 * (uint64_t)a + q] = c3[q];
+    }
+}
+
 // ------------------------------------------------------------------ per-tet helpers
 
 struct MITet {
@@ -101,22 +152,23 @@ __device__ inline uint64_t mi_vkey(const GridView& g, const MITet& M, const uint
     return key_inner(a, j);
 }
 
-// material set on the inner side of a tet-boundary face: its negative
-// label's material, or every material of that label's coincidence group
-__device__ inline int mi_side_set(const MITet& M, const uint8_t* __restrict__ fit, int neg, uint8_t* S) {
-    if (!M.R.dup) {
-        S[0] = fit[M.s0 + neg - 4];
-        return 1;
+// Material set on the inner side of a tet-boundary face: its negative
+// label's material, or every material of that label's coincidence group (in
+// label order).  Enumerated, not stored: an indexed local array would live in
+// scratch memory.
+template <class Fn>
+__device__ inline void mi_side_each(const MITet& M, const uint8_t* __restrict__ fit, int neg, Fn&& fn) {
+    if (M.R.dup) {
+        const uint16_t* grp = M.groups();
+        int n = 0;
+        for (int l = 4; l < M.R.L; ++l) n += grp[l] == grp[neg];
+        if (n > 1) {
+            for (int l = 4; l < M.R.L; ++l)
+                if (grp[l] == grp[neg]) fn((uint8_t)fit[M.s0 + l - 4]);
+            return;
+        }
     }
-    const uint16_t* grp = M.groups();
-    int n = 0;
-    for (int l = 4; l < M.R.L; ++l)
-        if (grp[l] == grp[neg]) S[n++] = fit[M.s0 + l - 4];
-    if (n <= 1) {
-        S[0] = fit[M.s0 + neg - 4];
-        n = 1;
-    }
-    return n;
+    fn((uint8_t)fit[M.s0 + neg - 4]);
 }
 
 // (min, second-min, max) of a face's vertex keys
@@ -169,13 +221,13 @@ __device__ inline bool mi_bface_emits(const MIArgs& A, const MITet& M, uint64_t 
             uint64_t q3[3];
             mi_face_triple(A.g, N, A.fit, r2, G, q3);
             if (q3[0] == k3[0] && q3[1] == k3[1] && q3[2] == k3[2]) {
-                uint8_t S1[CHuge::MP], S2[CHuge::MP];
-                const int n1 = mi_side_set(N, A.fit, G[1], S1), n2 = mi_side_set(M, A.fit, F[1], S2);
-                for (int x = 0; x < n1; ++x)
-                    for (int y = 0; y < n2; ++y)
-                        if (S1[x] == S2[y]) return false;
-                other = S1[0];
-                return true;
+                bool share = false, first = true;
+                mi_side_each(N, A.fit, G[1], [&](uint8_t x) {
+                    if (first) other = x;
+                    first = false;
+                    mi_side_each(M, A.fit, F[1], [&](uint8_t y) { share |= x == y; });
+                });
+                return !share;
             }
         }
         G += 4 + n;
@@ -211,24 +263,71 @@ __device__ inline int corner_of(const Rec& R, int j) {
     return c;
 }
 
+// Which tet-boundary faces of each active tet become interface faces here
+// (mi_bface_emits, the neighbour-record scan): bit f for face f < 63; bit 63
+// marks a tet with more faces, whose faces from 63 on are tested where they
+// are used.  kBfaceLanes lanes per tet (adjacent lanes of one wave) take faces
+// f = lane, lane + kBfaceLanes, ... and OR their bits together.
+constexpr int kBfaceLanes = 8;
+constexpr int kBfaceBits = 63;
+
+__global__ void k_mi_bface(MIArgs A, uint64_t* __restrict__ emask) {
+    const uint64_t n = A.n_active * kBfaceLanes;
+    GRID_STRIDE(idx, n) {
+        const uint64_t a = idx / kBfaceLanes;
+        const int l0 = (int)(idx % kBfaceLanes);
+        const uint32_t t = A.act[a];
+        const MITet M(A.g, A.rec + A.rec_loc[a], t, A.csr_off[t]);
+        uint64_t bits = 0;
+        if (M.R.r[0] == 2) {
+            const uint16_t* F = M.R.faces();
+            const int nf = M.R.nf;
+            for (int f = 0; f < nf && f < kBfaceBits; ++f) {
+                const int nfv = F[3];
+                uint8_t other;
+                if (f % kBfaceLanes == l0 && F[0] <= 3 && mi_bface_emits(A, M, a, F, other)) bits |= 1ull << f;
+                F += 4 + nfv;
+            }
+            if (nf > kBfaceBits) bits |= 1ull << kBfaceBits;
+        }
+        // the lanes of one tet are adjacent and leave the loop together
+        for (int d = 1; d < kBfaceLanes; d <<= 1) bits |= __shfl_xor(bits, d, 64);
+        if (l0 == 0) emask[a] = bits;
+    }
+}
+
+// whether tet-boundary face f (record words F) emits here, from the mask
+__device__ inline bool mi_emits(const MIArgs& A, const MITet& M, uint64_t a, const uint16_t* F, int f, uint64_t em,
+                                uint8_t& other) {
+    if (f < kBfaceBits) {
+        if (!((em >> f) & 1)) return false;
+        // rare: the scan again for the first copy's material
+        return mi_bface_emits(A, M, a, F, other);
+    }
+    return mi_bface_emits(A, M, a, F, other);
+}
+
 // per active tet: vertex occurrences (phase 0 + phase 1), faces, face corners
-__global__ void k_mi_count(MIArgs A, uint32_t* __restrict__ cnt) {
+__global__ void k_mi_count(MIArgs A, const uint64_t* __restrict__ emask, uint32_t* __restrict__ cnt) {
     GRID_STRIDE(a, A.n_active) {
         const uint32_t t = A.act[a];
         const MITet M(A.g, A.rec + A.rec_loc[a], t, A.csr_off[t]);
         uint32_t nv = 0, nf = 0, nfv = 0;
         if (M.R.r[0] == 2) {
+            const uint64_t em = emask[a];
             uint64_t m[kMaskWords];
             mi_iface_mask(M.R, m);
             for (int w = 0; w < kMaskWords; ++w) nv += __popcll(m[w]);
             const uint16_t* F = M.R.faces();
             for (int f = 0; f < M.R.nf; ++f) {
                 const int n = F[3];
-                uint8_t other;
                 if (F[0] > 3) {
                     ++nf;
                     nfv += n;
-                } else if (mi_bface_emits(A, M, a, F, other)) {
+                } else if (f < kBfaceBits ? ((em >> f) & 1) != 0 : [&] {
+                               uint8_t other;
+                               return mi_bface_emits(A, M, a, F, other);
+                           }()) {
                     ++nf;
                     nfv += n;
                     for (int q = 0; q < n; ++q) nv += is_corner(M.R, F[4 + q]);
@@ -242,8 +341,9 @@ __global__ void k_mi_count(MIArgs A, uint32_t* __restrict__ cnt) {
     }
 }
 
-__global__ void k_mi_extract(MIArgs A, const uint32_t* __restrict__ voff, const uint32_t* __restrict__ foff,
-                             const uint32_t* __restrict__ fvoff, uint64_t* __restrict__ okey,
+__global__ void k_mi_extract(MIArgs A, const uint64_t* __restrict__ emask, const uint32_t* __restrict__ voff,
+                             const uint32_t* __restrict__ foff, const uint32_t* __restrict__ fvoff,
+                             uint64_t* __restrict__ okey,
                              uint32_t* __restrict__ oa, uint16_t* __restrict__ oj, uint32_t* __restrict__ fq_a,
                              uint16_t* __restrict__ fq_l, uint8_t* __restrict__ fq_m, uint32_t* __restrict__ fq_vs,
                              uint32_t* __restrict__ fve) {
@@ -264,12 +364,13 @@ __global__ void k_mi_extract(MIArgs A, const uint32_t* __restrict__ voff, const 
             ++o;
         }
         uint32_t q = foff[a], e = fvoff[a];
+        const uint64_t em = emask[a];
         const uint16_t* F = M.R.faces();
         for (int f = 0; f < M.R.nf; ++f) {
             const int n = F[3];
             uint8_t other = 0xFF;
             const bool iface = F[0] > 3;
-            if (iface || mi_bface_emits(A, M, a, F, other)) {
+            if (iface || mi_emits(A, M, a, F, f, em, other)) {
                 fq_a[q] = (uint32_t)a;
                 fq_l[q] = (uint16_t)f;
                 fq_m[2 * q] = iface ? A.fit[M.s0 + F[0] - 4] : other;

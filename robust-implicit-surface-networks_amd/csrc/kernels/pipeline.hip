@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <stdexcept>
 
 #include "cut_wave.h"
@@ -63,6 +64,43 @@ inline int bits_for(uint64_t x) {
 
 // vertex dedup shared by both modes: stable sort of the occurrence keys, then
 // each key's id is the rank of its first occurrence in occurrence order
+// the 2-material records, one per corner comparison pattern, from the cut
+// itself (host build of the device algorithm)
+struct MI2Table {
+    MI2Entry e[81];
+    std::vector<uint16_t> words;
+};
+
+const MI2Table& mi2_table() {
+    static MI2Table T;
+    static std::once_flag once;
+    std::call_once(once, [] {
+        static dev::Complex<CMed> X;
+        for (int pat = 0; pat < 81; ++pat) {
+            // material a is 0 at every corner; material b is -1, 0 or 1
+            double P[2][4];
+            for (int c = 0, q = pat; c < 4; ++c, q /= 3) {
+                P[0][c] = 0.0;
+                P[1][c] = (double)(q % 3 - 1);
+            }
+            dev::mi_cut<CMed>(P, 2, X);
+            if (X.err) throw std::runtime_error("2-material table: cut failed");
+            uint16_t rec[1024];
+            const int len = dev::write_record_mi<CMed>(X, 2, rec, 1024);
+            if (len < 0 || T.words.size() + len > 0xFFFF) throw std::runtime_error("2-material table: record");
+            MI2Entry& E = T.e[pat];
+            E.off = (uint16_t)T.words.size();
+            E.len = (uint16_t)len;
+            T.words.insert(T.words.end(), rec, rec + len);
+            mi_counts<CMed>(X, E.cnt[0], E.cnt[1], E.cnt[2]);
+            E.cut[0] = (uint32_t)X.nv;
+            E.cut[1] = (uint32_t)X.nf;
+            E.cut[2] = (uint32_t)X.nc;
+        }
+    });
+    return T;
+}
+
 struct Dedup {
     DBuf<uint32_t> vid;        // id of every occurrence
     DBuf<uint32_t> first_occ;  // first occurrence of every id
@@ -128,7 +166,11 @@ void extract_mi(DevMem& mem, const Grid& g, const Values& val, RunOut& out, Time
     A.rec = out.rec.p;
     A.n_active = NA;
     DBuf<uint32_t> cnt(mem, 3 * NA + 3), cv(mem, NA + 1), cf(mem, NA + 1), cfv(mem, NA + 1);
-    if (NA) hipLaunchKernelGGL(k_mi_count, dim3(nblk(NA)), dim3(256), 0, s, A, cnt.p);
+    DBuf<uint64_t> emask(mem, NA + 1);
+    if (NA) {
+        hipLaunchKernelGGL(k_mi_bface, dim3(nblk(NA * kBfaceLanes)), dim3(256), 0, s, A, emask.p);
+        hipLaunchKernelGGL(k_mi_count, dim3(nblk(NA)), dim3(256), 0, s, A, (const uint64_t*)emask.p, cnt.p);
+    }
     hipLaunchKernelGGL(k_split3, dim3(nblk(NA)), dim3(256), 0, s, Count(NA), (const uint32_t*)cnt.p, cv.p, cf.p,
                        cfv.p);
     scan_u32(mem, cv.p, cv.p, NA, cv.p + NA, s);
@@ -143,7 +185,8 @@ void extract_mi(DevMem& mem, const Grid& g, const Values& val, RunOut& out, Time
     DBuf<uint16_t> oj(mem, NVO + 1), fq_l(mem, NF + 1);
     DBuf<uint8_t> fq_m(mem, 2 * NF + 2);
     if (NA)
-        hipLaunchKernelGGL(k_mi_extract, dim3(nblk(NA)), dim3(256), 0, s, A, (const uint32_t*)cv.p,
+        hipLaunchKernelGGL(k_mi_extract, dim3(nblk(NA)), dim3(256), 0, s, A, (const uint64_t*)emask.p,
+                           (const uint32_t*)cv.p,
                            (const uint32_t*)cf.p, (const uint32_t*)cfv.p, okey.p, oa.p, oj.p, fq_a.p, fq_l.p, fq_m.p,
                            fq_vs.p, fve.p);
     RIN_HIP(hipGetLastError());
@@ -375,6 +418,9 @@ void run_pipeline(DevMem& mem, const Grid& g, const Values& val, uint32_t flags,
 
     // bucket counts (reference labels: IA 1/2/>=3, MI 2/3/>=4)
     std::vector<uint32_t> huge, big, med;  // active ranks with > 8, 5..8 and 1..4 functions
+    // MI with lookups on: two-material tets take their record from the table
+    const bool mi_table = MI && !(flags & (RUN_REVERSE | RUN_NO_LOOKUP));
+    std::vector<uint32_t> two;
     {
         std::vector<uint8_t> hk(NA);
         if (NA) RIN_HIP(hipMemcpyAsync(hk.data(), out.act_k.p, NA, hipMemcpyDeviceToHost, s));
@@ -387,6 +433,7 @@ void run_pipeline(DevMem& mem, const Grid& g, const Values& val, uint32_t flags,
             kmax = std::max<int>(kmax, k);
             if (k > 8) huge.push_back((uint32_t)a);
             else if (k > 4) big.push_back((uint32_t)a);
+            else if (mi_table && k == 2) two.push_back((uint32_t)a);
             else med.push_back((uint32_t)a);
             out.n_cut34 += k >= 3 && k <= 4;
             out.n_cut58 += k >= 5 && k <= 8;
@@ -410,6 +457,21 @@ void run_pipeline(DevMem& mem, const Grid& g, const Values& val, uint32_t flags,
     const uint64_t nwords = read1(out.rec_off.p + NA, s);
     out.rec.alloc(mem, nwords + 1);
     DBuf<uint32_t> cnt(mem, 3 * NA + 3);
+    if (!two.empty()) {
+        const MI2Table& TT = mi2_table();
+        DBuf<MI2Entry> dtab(mem, 81);
+        DBuf<uint16_t> dw(mem, TT.words.size());
+        DBuf<uint32_t> dl(mem, two.size());
+        RIN_HIP(hipMemcpyAsync(dtab.p, TT.e, sizeof(TT.e), hipMemcpyHostToDevice, s));
+        RIN_HIP(hipMemcpyAsync(dw.p, TT.words.data(), 2 * TT.words.size(), hipMemcpyHostToDevice, s));
+        RIN_HIP(hipMemcpyAsync(dl.p, two.data(), 4 * two.size(), hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_mi_table2, dim3(nblk(two.size())), dim3(256), 0, s, g.view, (const double*)val.vals.p,
+                           (const uint32_t*)out.act.p, (const uint32_t*)out.csr_off.p, (const uint8_t*)out.fit.p,
+                           (const uint32_t*)dl.p, (uint64_t)two.size(), flags, (const uint32_t*)out.rec_off.p,
+                           out.rec.p, cnt.p, errs.p, (const MI2Entry*)dtab.p, (const uint16_t*)dw.p);
+        RIN_HIP(hipGetLastError());
+        RIN_HIP(hipStreamSynchronize(s));  // the table and list buffers return to the pool
+    }
     // each thread-per-tet cut kernel runs over its own k range only
     for (int q = 0; q < 2 && NA; ++q) {
         const std::vector<uint32_t>& L = q == 0 ? med : big;
