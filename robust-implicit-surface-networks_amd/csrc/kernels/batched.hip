@@ -1294,6 +1294,7 @@ struct Workspace {
     DBuf<uint16_t> fq_l;
     DBuf<uint8_t> fq_fn, fq_bnd;
     DBuf<uint32_t> first, vid, vrank, first_occ, mo, fail;
+    DBuf<uint32_t> tile_first;  // star path: first occurrences per tile of occurrences
     DBuf<uint32_t> star_head;  // star tables of the regular grid (k_dedup_star)
     DBuf<uint8_t> star_ent;
     // vertex dedup hash table (zero between runs) and each occurrence's slot
@@ -1301,7 +1302,8 @@ struct Workspace {
     DBuf<uint32_t> hmx, hslot;
     int hbits = 0;
     bool hash_dirty = false;  // a run stopped early: clear the table before the next
-    DBuf<uint32_t> ffirst, fmatch, ftfc, fvc, frank, ftpos, fvpos;
+    // faces: per occurrence flags and partner, corner ids, per-tile counts
+    DBuf<uint32_t> ffirst, fmatch, fvm, ftiles;
     // outputs
     DBuf<uint32_t> v_tet, v_sv, f_vstart, f_verts, f_tfstart, f_tf_tet;
     DBuf<uint16_t> v_j, f_tf_local;
@@ -1464,6 +1466,7 @@ void alloc_ws(Workspace& W, const Grid& g, const Caps_& c) {
     W.oa.alloc(m, c.vo + 1);
     W.oj.alloc(m, c.vo + 1);
     for (auto* b : {&W.first, &W.vid, &W.vrank, &W.first_occ, &W.hslot, &W.mo, &W.fail}) b->alloc(m, c.vo + 1);
+    W.tile_first.alloc(m, (c.vo + kDedupTile - 1) / kDedupTile + 1);
     {
         const StarTables& ST = star_tables();
         W.star_head.alloc(m, ST.head.size());
@@ -1484,8 +1487,9 @@ void alloc_ws(Workspace& W, const Grid& g, const Caps_& c) {
     W.fq_fn.alloc(m, c.fo + 1);
     W.fq_bnd.alloc(m, c.fo + 1);
     W.fve.alloc(m, c.fv + 1);
-    for (auto* b : {&W.ffirst, &W.fmatch, &W.ftfc, &W.fvc, &W.frank, &W.ftpos, &W.fvpos})
-        b->alloc(m, c.fo + 1);
+    for (auto* b : {&W.ffirst, &W.fmatch}) b->alloc(m, c.fo + 1);
+    W.fvm.alloc(m, c.fv + 1);
+    W.ftiles.alloc(m, 4 * ((c.fo + kFaceTile - 1) / kFaceTile) + 4);
     W.v_tet.alloc(m, c.vo + 1);
     W.v_j.alloc(m, c.vo + 1);
     W.v_ss.alloc(m, c.vo + 1);
@@ -1729,6 +1733,7 @@ void stage_rest(Workspace& W, const Grid& g, const Values& val, hipStream_t s, i
         const Count NH = star ? Count(ctr + C_DFAIL, c.vo, abort) : NVO;
         const uint32_t* list = star ? W.fail.p : nullptr;
         const unsigned hgrid = star ? std::min(nblk(c.vo), 1024u) : nblk(c.vo);
+        const unsigned tgrid = (unsigned)((c.vo + kDedupTile - 1) / kDedupTile);
         if (star) {
             StarArgs S;
             S.res = g.view.res;
@@ -1745,16 +1750,25 @@ void stage_rest(Workspace& W, const Grid& g, const Values& val, hipStream_t s, i
             S.n_fail = ctr + C_DFAIL;
             S.cap_fail = c.vo;
             S.n_active = ctr + C_ACTIVE;
-            hipLaunchKernelGGL(k_dedup_star, dim3(cgrid(c.vo)), dim3(256), 0, s, NVO, S);
+            S.tile_first = W.tile_first.p;
+            hipLaunchKernelGGL(k_dedup_star, dim3(tgrid), dim3(256), 0, s, NVO, S);
         }
         hipLaunchKernelGGL(k_hash_insert, dim3(hgrid), dim3(256), 0, s, NH, list, (const uint64_t*)W.okey.p, W.hkey.p,
                            W.hmx.p, W.hbits, W.hslot.p, W.errs.p);
         hipLaunchKernelGGL(k_hash_first, dim3(hgrid), dim3(256), 0, s, NH, list, (const uint32_t*)W.hslot.p,
-                           (const uint32_t*)W.hmx.p, W.first.p, W.mo.p);
-        scan_u32(m, W.first.p, W.vrank.p, NVO, ctr + C_VERTS, s);
+                           (const uint32_t*)W.hmx.p, W.first.p, W.mo.p, star ? W.tile_first.p : nullptr);
+        if (star) {
+            // ranks from the per-tile counts of the star pass
+            TileTotals tt = {{ctr + C_VERTS, nullptr, nullptr}};
+            hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, s, NVO, kDedupTile, W.tile_first.p, 1, 1, tt);
+            hipLaunchKernelGGL(k_dedup_rank, dim3(tgrid), dim3(256), 0, s, NVO, (const uint32_t*)W.first.p,
+                               (const uint32_t*)W.tile_first.p, W.vrank.p, W.first_occ.p);
+        } else {
+            scan_u32(m, W.first.p, W.vrank.p, NVO, ctr + C_VERTS, s);
+        }
         hipLaunchKernelGGL(k_dedup_assign, dim3(cgrid(c.vo)), dim3(256), 0, s, NVO, (const uint32_t*)W.mo.p,
                            (const uint32_t*)W.first.p, (const uint32_t*)W.vrank.p, W.vid.p, W.first_occ.p, NH, list,
-                           (const uint32_t*)W.hslot.p, W.hkey.p, W.hmx.p);
+                           (const uint32_t*)W.hslot.p, W.hkey.p, W.hmx.p, !star);
     }
     if (part & REST_XYZ) {
         hipLaunchKernelGGL(k_vert_fields_ia, dim3(cgrid(c.vo)), dim3(256), 0, s, g.view, (const double*)val.vals.p,
@@ -1766,57 +1780,39 @@ void stage_rest(Workspace& W, const Grid& g, const Values& val, hipStream_t s, i
     }
     if (part & REST_FACES) {
         // faces: a boundary face may repeat in the tet across its tet face
-        hipLaunchKernelGGL(k_face_map, dim3(cgrid(c.fv)), dim3(256), 0, s, Count(ctr + C_FV, c.fv, abort), W.fve.p,
-                           (const uint32_t*)W.vid.p);
-        FaceArgs FA;
+        FaceTileArgs FA;
         FA.fq_a = W.fq_a.p;
         FA.fq_vs = W.fq_vs.p;
         FA.fve = W.fve.p;
+        FA.vid = W.vid.p;
         FA.act = W.act.p;
         FA.rank = W.rank.p;
         FA.foff = W.cf.p;
         FA.nbr = g.nbr.p;
         FA.fq_bnd = W.fq_bnd.p;
         FA.fq_l = W.fq_l.p;
+        FA.fq_fn = W.fq_fn.p;
         FA.nfv_total = ctr + C_FV;
         FA.nfo_total = ctr + C_FO;
         FA.nfv_cap = c.fv;
         FA.nfo_cap = c.fo;
         FA.nq = NFO;
         FA.n_active = NA;
-        FA.first = W.ffirst.p;
-        FA.match = W.fmatch.p;
-        FA.tfc = W.ftfc.p;
-        FA.vcnt = W.fvc.p;
-        hipLaunchKernelGGL(k_face_first, dim3(cgrid(c.fo)), dim3(256), 0, s, FA);
-        {
-            const uint32_t* in[3] = {W.ffirst.p, W.ftfc.p, W.fvc.p};
-            uint32_t* out[3] = {W.frank.p, W.ftpos.p, W.fvpos.p};
-            uint32_t* tot[3] = {ctr + C_FACES, ctr + C_TETFACES, ctr + C_FACEVERTS};
-            scan3_u32(m, in, 1, out, NFO, tot, s);
-        }
-        FaceOutArgs FO;
-        FO.fq_a = W.fq_a.p;
-        FO.fq_vs = W.fq_vs.p;
-        FO.fve = W.fve.p;
-        FO.act = W.act.p;
-        FO.first = W.ffirst.p;
-        FO.match = W.fmatch.p;
-        FO.frank = W.frank.p;
-        FO.tfpos = W.ftpos.p;
-        FO.vpos = W.fvpos.p;
-        FO.fq_l = W.fq_l.p;
-        FO.fq_fn = W.fq_fn.p;
-        FO.nfv_total = ctr + C_FV;
-        FO.nfv_cap = c.fv;
-        FO.nq = NFO;
-        FO.f_vstart = W.f_vstart.p;
-        FO.f_verts = W.f_verts.p;
-        FO.f_tfstart = W.f_tfstart.p;
-        FO.f_tf_tet = W.f_tf_tet.p;
-        FO.f_tf_local = W.f_tf_local.p;
-        FO.f_func = W.f_func.p;
-        hipLaunchKernelGGL(k_face_emit, dim3(cgrid(c.fo)), dim3(256), 0, s, FO);
+        FA.fvm = W.fvm.p;
+        FA.flags = W.ffirst.p;
+        FA.partner = W.fmatch.p;
+        FA.tiles = W.ftiles.p;
+        FA.f_vstart = W.f_vstart.p;
+        FA.f_verts = W.f_verts.p;
+        FA.f_tfstart = W.f_tfstart.p;
+        FA.f_tf_tet = W.f_tf_tet.p;
+        FA.f_tf_local = W.f_tf_local.p;
+        FA.f_func = W.f_func.p;
+        const unsigned fgrid = (unsigned)((c.fo + kFaceTile - 1) / kFaceTile);
+        hipLaunchKernelGGL(k_face_first_tiled, dim3(fgrid), dim3(256), 0, s, FA);
+        TileTotals tt = {{ctr + C_FACES, ctr + C_TETFACES, ctr + C_FACEVERTS}};
+        hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, s, NFO, kFaceTile, W.ftiles.p, 4, 3, tt);
+        hipLaunchKernelGGL(k_face_emit_tiled, dim3(fgrid), dim3(256), 0, s, FA);
         hipLaunchKernelGGL(k_write_totals, dim3(1), dim3(64), 0, s, ctr, g.view.T, W.csr_off.p,
                            W.f_vstart.p, W.f_tfstart.p, c.fo, W.errs.p, W.pub_d);
     }

@@ -577,67 +577,203 @@ struct StarArgs {
     uint32_t *first, *mo, *fail, *n_fail;
     uint32_t cap_fail;
     const uint32_t* n_active;
+    uint32_t* tile_first;  // first occurrences per tile of kDedupTile occurrences
 };
 
-__global__ void k_dedup_star(Count n, StarArgs A) {
+// Occurrences per lane of the star pass, handled side by side so that their
+// chains of dependent loads (key, star tet, its rank and offsets, its keys)
+// overlap; a workgroup takes one tile of consecutive occurrences, lane-major
+// within each of the kDedupU rows.
+constexpr int kDedupU = 4;
+constexpr uint32_t kDedupTile = 256u * kDedupU;
+
+// lowest tet of the star of key K's simplex inside the grid (~0: none)
+__device__ inline uint64_t star_lowest_tet(const StarArgs& A, uint32_t R, uint32_t N, uint64_t K) {
+    const uint32_t type = (uint32_t)(K & 3);
+    uint32_t a, shape;
+    if (type == 0) {
+        a = (uint32_t)(K >> 2);
+        shape = 0;
+    } else if (type == 1) {
+        const uint64_t e = K >> 8;
+        a = (uint32_t)(e / 27);
+        shape = 1 + (uint32_t)(e % 27);
+    } else {
+        const uint64_t fc = K >> 14;
+        a = (uint32_t)(fc / 729);
+        shape = 28 + (uint32_t)(fc % 729);
+    }
+    const uint32_t ai = a / (N * N), ar = a - ai * N * N, aj = ar / N, ak = ar - aj * N;
+    const uint32_t head = A.star_head[((ai + aj + ak) & 1u) * kStarShapes + shape];
+    for (uint32_t e = head >> 8, end = (head >> 8) + (head & 255u); e < end; e += 2) {
+        const uint32_t d = A.star_ent[e];
+        const int ci = (int)ai + (int)(d & 3u) - 1, cj = (int)aj + (int)((d >> 2) & 3u) - 1,
+                  ck = (int)ak + (int)((d >> 4) & 3u) - 1;
+        if (ci < 0 || cj < 0 || ck < 0 || ci >= (int)R || cj >= (int)R || ck >= (int)R) continue;
+        return (((uint64_t)ci * R + cj) * R + ck) * 5 + A.star_ent[e + 1];
+    }
+    return ~0ull;
+}
+
+__global__ void __launch_bounds__(256) k_dedup_star(Count n, StarArgs A) {
     const uint32_t R = (uint32_t)A.res, N = R + 1;
-    const uint32_t NA = *A.n_active;
     const uint32_t NO = (uint32_t)count_of(n);
-    GRID_STRIDE(o, n) {
-        const uint64_t K = A.okey[o];
-        const uint32_t type = (uint32_t)(K & 3);
-        if (type == 3) {
-            A.first[o] = 1;
-            A.mo[o] = (uint32_t)o;
-            continue;
+    const uint32_t base = blockIdx.x * kDedupTile;
+    if (base >= NO) return;
+    const uint32_t NA = *A.n_active;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint64_t K[kDedupU], ts[kDedupU];
+    uint32_t o[kDedupU], oa[kDedupU], t[kDedupU], as[kDedupU], lo[kDedupU], hi[kDedupU], found[kDedupU];
+#pragma unroll
+    for (int u = 0; u < kDedupU; ++u) {
+        o[u] = base + u * 256u + threadIdx.x;
+        const bool in = o[u] < NO;
+        K[u] = in ? A.okey[o[u]] : 3ull;  // out of range: an interior key, decided without loads
+        oa[u] = in ? A.oa[o[u]] : 0u;
+    }
+#pragma unroll
+    for (int u = 0; u < kDedupU; ++u) {
+        const bool inner = (K[u] & 3) == 3;
+        t[u] = inner ? 0u : A.act[oa[u]];
+        ts[u] = inner ? ~0ull : star_lowest_tet(A, R, N, K[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < kDedupU; ++u) {
+        // interior keys, and keys whose lowest star tet is this one, are first here
+        found[u] = ((K[u] & 3) == 3 || ts[u] == t[u]) ? o[u] : ~0u;
+        as[u] = (found[u] == ~0u && ts[u] != ~0ull) ? A.rank[ts[u]] : ~0u;
+    }
+#pragma unroll
+    for (int u = 0; u < kDedupU; ++u) {
+        const bool ok = as[u] < NA && A.act[as[u]] == (uint32_t)ts[u];
+        lo[u] = ok ? A.voff[as[u]] : 0u;
+        hi[u] = ok ? (as[u] + 1 < NA ? A.voff[as[u] + 1] : NO) : 0u;
+        hi[u] = hi[u] < NO ? hi[u] : NO;
+    }
+    // the lowest tet's keys, eight at a time (a tet rarely has more)
+#pragma unroll
+    for (int u = 0; u < kDedupU; ++u) {
+        for (uint32_t p0 = lo[u]; p0 < hi[u] && found[u] == ~0u; p0 += 8) {
+            uint64_t kk[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) kk[j] = p0 + j < hi[u] ? A.okey[p0 + j] : ~K[u];
+#pragma unroll
+            for (int j = 7; j >= 0; --j)
+                if (kk[j] == K[u]) found[u] = p0 + j;
         }
-        uint32_t a, shape;
-        if (type == 0) {
-            a = (uint32_t)(K >> 2);
-            shape = 0;
-        } else if (type == 1) {
-            const uint64_t e = K >> 8;
-            a = (uint32_t)(e / 27);
-            shape = 1 + (uint32_t)(e % 27);
-        } else {
-            const uint64_t fc = K >> 14;
-            a = (uint32_t)(fc / 729);
-            shape = 28 + (uint32_t)(fc % 729);
+    }
+    uint32_t nfirst = 0;
+#pragma unroll
+    for (int u = 0; u < kDedupU; ++u) {
+        const bool in = o[u] < NO;
+        const bool fail = in && found[u] == ~0u;
+        if (in) {
+            A.first[o[u]] = found[u] == o[u];
+            if (!fail) A.mo[o[u]] = found[u];
         }
-        const uint32_t ai = a / (N * N), ar = a - ai * N * N, aj = ar / N, ak = ar - aj * N;
-        const uint32_t head = A.star_head[((ai + aj + ak) & 1u) * kStarShapes + shape];
-        uint64_t ts = ~0ull;  // lowest tet of the star inside the grid
-        for (uint32_t e = head >> 8, end = (head >> 8) + (head & 255u); e < end; e += 2) {
-            const uint32_t d = A.star_ent[e];
-            const int ci = (int)ai + (int)(d & 3u) - 1, cj = (int)aj + (int)((d >> 2) & 3u) - 1,
-                      ck = (int)ak + (int)((d >> 4) & 3u) - 1;
-            if (ci < 0 || cj < 0 || ck < 0 || ci >= (int)R || cj >= (int)R || ck >= (int)R) continue;
-            ts = (((uint64_t)ci * R + cj) * R + ck) * 5 + A.star_ent[e + 1];
-            break;
-        }
-        const uint32_t t = A.act[A.oa[o]];
-        uint32_t found = ~0u;
-        if (ts == t) {
-            found = (uint32_t)o;
-        } else if (ts != ~0ull) {
-            const uint32_t as = A.rank[ts];
-            if (as < NA && A.act[as] == (uint32_t)ts) {
-                const uint32_t lo = A.voff[as], hi = as + 1 < NA ? A.voff[as + 1] : NO;
-                for (uint32_t p = lo; p < hi && p < NO; ++p)
-                    if (A.okey[p] == K) {
-                        found = p;
-                        break;
-                    }
+        nfirst += in && found[u] == o[u];
+        // failures: one counter update per wave
+        const uint64_t m = __ballot(fail);
+        if (m) {
+            const int leader = __ffsll((unsigned long long)m) - 1;
+            uint32_t b0 = 0;
+            if (lane == leader) b0 = atomicAdd(A.n_fail, (uint32_t)__popcll(m));
+            b0 = __shfl(b0, leader, 64);
+            if (fail) {
+                const uint32_t slot = b0 + (uint32_t)__popcll(m & ((1ull << lane) - 1));
+                if (slot < A.cap_fail) A.fail[slot] = o[u];
             }
         }
-        if (found != ~0u) {
-            A.first[o] = found == (uint32_t)o;
-            A.mo[o] = found;
-        } else {
-            A.first[o] = 0;
-            const uint32_t slot = atomicAdd(A.n_fail, 1u);
-            if (slot < A.cap_fail) A.fail[slot] = (uint32_t)o;
+    }
+    __shared__ uint32_t red[256 / 64];
+    for (int d = 32; d > 0; d >>= 1) nfirst += __shfl_xor(nfirst, d, 64);
+    if (lane == 0) red[wave] = nfirst;
+    __syncthreads();
+    if (threadIdx.x == 0) A.tile_first[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+// Exclusive prefixes of per-tile counts in place, one workgroup; the tiles
+// are those of `tile` elements over n, field f of tile i at x[i * stride + f]
+// (f < nfield <= 3); *total[f] = the field's sum.
+struct TileTotals {
+    uint32_t* p[3];
+};
+
+__global__ void __launch_bounds__(1024) k_scan_tiles(Count n, uint32_t tile, uint32_t* __restrict__ x, int stride,
+                                                     int nfield, TileTotals total) {
+    const uint32_t nt = (uint32_t)((count_of(n) + tile - 1) / tile);
+    __shared__ uint32_t wsum[16];
+    __shared__ uint32_t carry;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int f = 0; f < nfield; ++f) {
+        if (threadIdx.x == 0) carry = 0;
+        __syncthreads();
+        for (uint32_t base = 0; base < nt; base += 4 * 1024) {
+            const uint32_t i0 = base + 4 * threadIdx.x;
+            uint32_t v[4];
+            for (int q = 0; q < 4; ++q) v[q] = i0 + q < nt ? x[(uint64_t)(i0 + q) * stride + f] : 0u;
+            const uint32_t sum = v[0] + v[1] + v[2] + v[3];
+            uint32_t inc = sum;
+            for (int d = 1; d < 64; d <<= 1) {
+                const uint32_t y = __shfl_up(inc, d, 64);
+                if (lane >= d) inc += y;
+            }
+            if (lane == 63) wsum[wave] = inc;
+            __syncthreads();
+            uint32_t before = carry;
+            for (int w = 0; w < 16; ++w) before += w < wave ? wsum[w] : 0u;
+            uint32_t run = before + inc - sum;
+            for (int q = 0; q < 4; ++q) {
+                if (i0 + q < nt) x[(uint64_t)(i0 + q) * stride + f] = run;
+                run += v[q];
+            }
+            __syncthreads();  // every lane has read carry and wsum
+            if (threadIdx.x == 1023) carry = run;
+            __syncthreads();
         }
+        if (threadIdx.x == 0) *total.p[f] = carry;
+    }
+}
+
+// Star path: each first occurrence's vertex id (its rank among first
+// occurrences, in occurrence order) from its tile's prefix, written where
+// the id gather reads it (vrank at first occurrences), and first_occ.
+__global__ void __launch_bounds__(256) k_dedup_rank(Count n, const uint32_t* __restrict__ first,
+                                                    const uint32_t* __restrict__ tile_first,
+                                                    uint32_t* __restrict__ vrank, uint32_t* __restrict__ first_occ) {
+    const uint32_t NO = (uint32_t)count_of(n);
+    const uint32_t base = blockIdx.x * kDedupTile;
+    if (base >= NO) return;
+    __shared__ uint32_t ws[kDedupU][256 / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t f[kDedupU];
+    uint64_t m[kDedupU];
+#pragma unroll
+    for (int u = 0; u < kDedupU; ++u) {
+        const uint32_t o = base + u * 256u + threadIdx.x;
+        f[u] = o < NO ? first[o] : 0u;
+    }
+#pragma unroll
+    for (int u = 0; u < kDedupU; ++u) {
+        m[u] = __ballot(f[u] != 0);
+        if (lane == 0) ws[u][wave] = (uint32_t)__popcll(m[u]);
+    }
+    __syncthreads();
+    uint32_t carry = tile_first[blockIdx.x];
+#pragma unroll
+    for (int u = 0; u < kDedupU; ++u) {
+        uint32_t before = 0, tot = 0;
+        for (int w = 0; w < 256 / 64; ++w) {
+            before += w < wave ? ws[u][w] : 0u;
+            tot += ws[u][w];
+        }
+        if (f[u]) {
+            const uint32_t o = base + u * 256u + threadIdx.x;
+            const uint32_t r = carry + before + (uint32_t)__popcll(m[u] & ((1ull << lane) - 1));
+            vrank[o] = r;
+            first_occ[r] = o;
+        }
+        carry += tot;
     }
 }
 
@@ -648,13 +784,17 @@ __global__ void k_hash_insert(Count n, const uint32_t* __restrict__ list, const 
     GRID_STRIDE(i, n) hash_insert_one(list ? list[i] : (uint32_t)i, okey, hkey, hmx, bits, hslot, errs);
 }
 
+// tile_first (star path, or null): the first occurrences found here join
+// their tile's count
 __global__ void k_hash_first(Count n, const uint32_t* __restrict__ list, const uint32_t* __restrict__ hslot,
-                             const uint32_t* __restrict__ hmx, uint32_t* __restrict__ first, uint32_t* __restrict__ mo) {
+                             const uint32_t* __restrict__ hmx, uint32_t* __restrict__ first, uint32_t* __restrict__ mo,
+                             uint32_t* __restrict__ tile_first) {
     GRID_STRIDE(i, n) {
         const uint32_t o = list ? list[i] : (uint32_t)i;
         const uint32_t m = ~hmx[hslot[o]];
         first[o] = m == o;
         mo[o] = m;
+        if (tile_first && m == o) atomicAdd(&tile_first[o / kDedupTile], 1u);
     }
 }
 
@@ -664,11 +804,11 @@ __global__ void k_dedup_assign(Count n, const uint32_t* __restrict__ mo, const u
                                const uint32_t* __restrict__ vrank, uint32_t* __restrict__ id_of_occ,
                                uint32_t* __restrict__ first_occ, Count n_hash, const uint32_t* __restrict__ list,
                                const uint32_t* __restrict__ hslot, unsigned long long* __restrict__ hkey,
-                               uint32_t* __restrict__ hmx) {
+                               uint32_t* __restrict__ hmx, bool set_first_occ) {
     GRID_STRIDE(o, n) {
         const uint32_t id = vrank[mo[o]];
         id_of_occ[o] = id;
-        if (first[o]) first_occ[id] = (uint32_t)o;
+        if (set_first_occ && first[o]) first_occ[id] = (uint32_t)o;
     }
     GRID_STRIDE(i, n_hash) {
         const uint32_t o = list ? list[i] : (uint32_t)i;
@@ -925,6 +1065,196 @@ __global__ void k_face_emit(FaceOutArgs A) {
             const uint32_t p = A.tfpos[A.match[q]] + 1;
             A.f_tf_tet[p] = t;
             A.f_tf_local[p] = A.fq_l[q];
+        }
+    }
+}
+
+// ---- faces in tiles (batched form)
+//
+// The same two passes over tiles of kFaceTile face occurrences (lane-major in
+// each of kFaceU rows), with no global scans: the first pass maps corner
+// occurrences to vertex ids (into fvm), decides each occurrence (flags: bit 0
+// first, bit 1 merged with a later copy; partner: the other copy) and leaves
+// per-tile counts of faces, tet-face entries and corners; one workgroup scans
+// the tiles; the second pass ranks its tile itself and writes the faces.  A
+// first occurrence writes its copy's tet-face entry too.
+constexpr int kFaceU = 4;
+constexpr uint32_t kFaceTile = 256u * kFaceU;
+
+struct FaceTileArgs {
+    const uint32_t *fq_a, *fq_vs, *fve, *vid, *act, *rank, *foff, *nbr;
+    const uint8_t* fq_bnd;
+    const uint16_t* fq_l;
+    const uint8_t* fq_fn;
+    const uint32_t* nfv_total;
+    const uint32_t* nfo_total;
+    uint64_t nfv_cap, nfo_cap;
+    Count nq, n_active;
+    uint32_t *fvm, *flags, *partner, *tiles;  // tiles: 4 words per tile (faces, tet-face entries, corners)
+    uint32_t *f_vstart, *f_verts, *f_tfstart, *f_tf_tet;
+    uint16_t* f_tf_local;
+    uint8_t* f_func;
+};
+
+// (min, second-min, max) of a face's vertex ids, its corners mapped through vid
+__device__ inline void face_triple_ids(const uint32_t* __restrict__ fve, uint32_t b, uint32_t e,
+                                       const uint32_t* __restrict__ vid, uint32_t k3[3]) {
+    uint32_t mn = 0xFFFFFFFFu, sm = 0xFFFFFFFFu, mx = 0;
+    for (uint32_t i = b; i < e; ++i) {
+        const uint32_t x = vid[fve[i]];
+        if (x < mn) {
+            sm = mn;
+            mn = x;
+        } else if (x < sm) {
+            sm = x;
+        }
+        if (x > mx) mx = x;
+    }
+    k3[0] = mn;
+    k3[1] = sm;
+    k3[2] = mx;
+}
+
+__global__ void __launch_bounds__(256) k_face_first_tiled(FaceTileArgs A) {
+    const uint32_t NQ = (uint32_t)A.nq.get();
+    const uint32_t base = blockIdx.x * kFaceTile;
+    if (base >= NQ) return;
+    const uint32_t NA = (uint32_t)A.n_active.get();
+    const uint32_t nfv = clamp_count(A.nfv_total, A.nfv_cap);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t sf = 0, st = 0, sv = 0;
+    for (int u = 0; u < kFaceU; ++u) {
+        const uint32_t q = base + u * 256u + threadIdx.x;
+        if (q >= NQ) break;
+        const uint32_t b = A.fq_vs[q], e = (q + 1 < NQ) ? A.fq_vs[q + 1] : nfv;
+        uint32_t mn = 0xFFFFFFFFu, sm = 0xFFFFFFFFu, mx = 0;
+        for (uint32_t i = b; i < e; ++i) {
+            const uint32_t x = A.vid[A.fve[i]];
+            A.fvm[i] = x;
+            if (x < mn) {
+                sm = mn;
+                mn = x;
+            } else if (x < sm) {
+                sm = x;
+            }
+            if (x > mx) mx = x;
+        }
+        uint32_t first = 1, partner = q, dup = 0;
+        if (A.fq_bnd[q]) {
+            const uint32_t a = A.fq_a[q], t = A.act[a], l = A.fq_bnd[q] - 1u;
+            const uint32_t nb = A.nbr[4 * (uint64_t)t + l];
+            if (nb != 0xFFFFFFFFu) {
+                const uint32_t t2 = nb >> 2, l2 = nb & 3u, r2 = A.rank[t2];
+                if (r2 < NA && A.act[r2] == t2) {
+                    const uint32_t q0 = A.foff[r2];
+                    const uint32_t q1 = (r2 + 1 < NA) ? A.foff[r2 + 1] : clamp_count(A.nfo_total, A.nfo_cap);
+                    for (uint32_t p = q0; p < q1; ++p) {
+                        if (A.fq_bnd[p] != 1 + l2) continue;
+                        const uint32_t pb = A.fq_vs[p], pe = (p + 1 < NQ) ? A.fq_vs[p + 1] : nfv;
+                        uint32_t p3[3];
+                        face_triple_ids(A.fve, pb, pe, A.vid, p3);
+                        if (p3[0] == mn && p3[1] == sm && p3[2] == mx) {
+                            partner = p;
+                            if (t2 < t)
+                                first = 0;
+                            else
+                                dup = 1;
+                            break;
+                        }
+                    }
+                }
+            }
+        }
+        A.flags[q] = first | dup << 1;
+        A.partner[q] = partner;
+        sf += first;
+        st += first ? 1 + dup : 0;
+        sv += first ? e - b : 0;
+    }
+    __shared__ uint32_t red[3][256 / 64];
+    for (int d = 32; d > 0; d >>= 1) {
+        sf += __shfl_xor(sf, d, 64);
+        st += __shfl_xor(st, d, 64);
+        sv += __shfl_xor(sv, d, 64);
+    }
+    if (lane == 0) {
+        red[0][wave] = sf;
+        red[1][wave] = st;
+        red[2][wave] = sv;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        const int f = threadIdx.x;
+        A.tiles[4 * (uint64_t)blockIdx.x + f] = red[f][0] + red[f][1] + red[f][2] + red[f][3];
+    }
+}
+
+// inclusive prefix over the lanes of a wave
+__device__ inline uint32_t wave_inclusive(uint32_t x, int lane) {
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    return x;
+}
+
+__global__ void __launch_bounds__(256) k_face_emit_tiled(FaceTileArgs A) {
+    const uint32_t NQ = (uint32_t)A.nq.get();
+    const uint32_t base = blockIdx.x * kFaceTile;
+    if (base >= NQ) return;
+    const uint32_t nfv = clamp_count(A.nfv_total, A.nfv_cap);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    __shared__ uint32_t ws[kFaceU][3][256 / 64];
+    uint32_t fl[kFaceU], vb[kFaceU], ve[kFaceU], inc[kFaceU][3];
+#pragma unroll
+    for (int u = 0; u < kFaceU; ++u) {
+        const uint32_t q = base + u * 256u + threadIdx.x;
+        const bool in = q < NQ;
+        fl[u] = in ? A.flags[q] : 0u;
+        vb[u] = in ? A.fq_vs[q] : 0u;
+        ve[u] = in ? ((q + 1 < NQ) ? A.fq_vs[q + 1] : nfv) : 0u;
+    }
+#pragma unroll
+    for (int u = 0; u < kFaceU; ++u) {
+        const uint32_t first = fl[u] & 1u;
+        const uint32_t c[3] = {first, first ? 1u + ((fl[u] >> 1) & 1u) : 0u, first ? ve[u] - vb[u] : 0u};
+        for (int f = 0; f < 3; ++f) {
+            inc[u][f] = wave_inclusive(c[f], lane);
+            if (lane == 63) ws[u][f][wave] = inc[u][f];
+            inc[u][f] -= c[f];  // exclusive within the wave
+        }
+    }
+    __syncthreads();
+    uint32_t carry[3] = {A.tiles[4 * (uint64_t)blockIdx.x], A.tiles[4 * (uint64_t)blockIdx.x + 1],
+                         A.tiles[4 * (uint64_t)blockIdx.x + 2]};
+#pragma unroll
+    for (int u = 0; u < kFaceU; ++u) {
+        uint32_t pos[3];
+        for (int f = 0; f < 3; ++f) {
+            uint32_t before = 0, tot = 0;
+            for (int w = 0; w < 256 / 64; ++w) {
+                before += w < wave ? ws[u][f][w] : 0u;
+                tot += ws[u][f][w];
+            }
+            pos[f] = carry[f] + before + inc[u][f];
+            carry[f] += tot;
+        }
+        const uint32_t q = base + u * 256u + threadIdx.x;
+        if (q >= NQ || !(fl[u] & 1u)) continue;
+        const uint32_t fid = pos[0], tp = pos[1];
+        uint32_t o = pos[2];
+        A.f_vstart[fid] = o;
+        for (uint32_t i = vb[u]; i < ve[u]; ++i) A.f_verts[o++] = A.fvm[i];
+        A.f_tfstart[fid] = tp;
+        A.f_tf_tet[tp] = A.act[A.fq_a[q]];
+        A.f_tf_local[tp] = A.fq_l[q];
+        A.f_func[2 * fid] = A.fq_fn[q];
+        A.f_func[2 * fid + 1] = 0xFF;
+        if (fl[u] & 2u) {
+            // the copy in the tet across, after this one
+            const uint32_t p = A.partner[q];
+            A.f_tf_tet[tp + 1] = A.act[A.fq_a[p]];
+            A.f_tf_local[tp + 1] = A.fq_l[p];
         }
     }
 }
