@@ -1231,10 +1231,10 @@ __global__ void k_publish(const uint32_t* ctr, const uint32_t* errs, uint32_t* p
     publish_counters(ctr, errs, pub);
 }
 
-// fold the spread counters, close the CSR and face offset arrays, and publish
-// the counters (the last kernel of a run)
+// fold the spread counters, close the CSR and face offset arrays, publish
+// the counters and clear them (the last kernel of a run)
 __global__ void k_write_totals(uint32_t* ctr, uint64_t T, uint32_t* csr_off, uint32_t* f_vstart,
-                               uint32_t* f_tfstart, uint32_t cap_fo, const uint32_t* errs, uint32_t* pub) {
+                               uint32_t* f_tfstart, uint32_t cap_fo, uint32_t* errs, uint32_t* pub) {
     const int fold_to[SP_N] = {C_RECW, C_CUTK3, C_CUTW3, C_CUTK5, C_CUTW5};
     for (int c = 0; c < SP_N; ++c) {
         uint32_t v = threadIdx.x < kSpread ? ctr[spread_at(c, threadIdx.x)] : 0u;
@@ -1249,6 +1249,11 @@ __global__ void k_write_totals(uint32_t* ctr, uint64_t T, uint32_t* csr_off, uin
     }
     __syncthreads();
     publish_counters(ctr, errs, pub);
+    // leave the counters clear for the next run
+    __syncthreads();
+    if (threadIdx.x < C_N) ctr[threadIdx.x] = 0;
+    errs[threadIdx.x] = 0;
+    for (int c = 0; c < SP_N; ++c) ctr[spread_at(c, threadIdx.x)] = 0;
 }
 
 __global__ void k_zero_ctr(uint32_t* ctr, int n, uint32_t* errs) {
@@ -1334,6 +1339,7 @@ struct Workspace {
     uint32_t* pub_h = nullptr;
     uint32_t* pub_d = nullptr;
     bool exec = false;  // graphs captured for the current capacities
+    bool ctr_clear = false;  // counters and error flags are zero (left so by the last run)
     bool in_use = false;
     void drop_graphs() {
         for (auto& p : parts) (void)hipGraphExecDestroy(p.second);
@@ -1529,11 +1535,12 @@ void alloc_ws(Workspace& W, const Grid& g, const Caps_& c) {
 }
 
 // stage 1 (also run eagerly to size the workspace): signs, filter, and
-// either the per-tile sums (regular grids) or the per-tet CSR scans
-void stage_count(Workspace& W, const Grid& g, const Values& val, hipStream_t s) {
+// either the per-tile sums (regular grids) or the per-tet CSR scans; zero:
+// clear the counters first (the whole-path graph leaves them clear instead)
+void stage_count(Workspace& W, const Grid& g, const Values& val, hipStream_t s, bool zero = true) {
     const uint64_t V = g.view.V, T = g.view.T;
     const uint32_t F = val.F;
-    hipLaunchKernelGGL(k_zero_ctr, dim3(1), dim3(64), 0, s, W.ctr.p, (int)C_N, W.errs.p);
+    if (zero) hipLaunchKernelGGL(k_zero_ctr, dim3(1), dim3(64), 0, s, W.ctr.p, (int)C_N, W.errs.p);
     hipLaunchKernelGGL(k_signs_count, dim3(nblk(V)), dim3(256), 0, s, (const double*)val.vals.p, V, F, W.pos.p,
                        W.neg.p, W.ctr.p);
     const uint64_t full = (F == 64) ? ~0ull : ((1ull << F) - 1);
@@ -1762,13 +1769,14 @@ void stage_rest(Workspace& W, const Grid& g, const Values& val, hipStream_t s, i
             TileTotals tt = {{ctr + C_VERTS, nullptr, nullptr}};
             hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, s, NVO, kDedupTile, W.tile_first.p, 1, 1, tt);
             hipLaunchKernelGGL(k_dedup_rank, dim3(tgrid), dim3(256), 0, s, NVO, (const uint32_t*)W.first.p,
-                               (const uint32_t*)W.tile_first.p, W.vrank.p, W.first_occ.p);
+                               (const uint32_t*)W.tile_first.p, W.vrank.p, W.first_occ.p, NH, list,
+                               (const uint32_t*)W.hslot.p, W.hkey.p, W.hmx.p);
         } else {
             scan_u32(m, W.first.p, W.vrank.p, NVO, ctr + C_VERTS, s);
+            hipLaunchKernelGGL(k_dedup_assign, dim3(cgrid(c.vo)), dim3(256), 0, s, NVO, (const uint32_t*)W.mo.p,
+                               (const uint32_t*)W.first.p, (const uint32_t*)W.vrank.p, W.vid.p, W.first_occ.p, NH,
+                               list, (const uint32_t*)W.hslot.p, W.hkey.p, W.hmx.p, true);
         }
-        hipLaunchKernelGGL(k_dedup_assign, dim3(cgrid(c.vo)), dim3(256), 0, s, NVO, (const uint32_t*)W.mo.p,
-                           (const uint32_t*)W.first.p, (const uint32_t*)W.vrank.p, W.vid.p, W.first_occ.p, NH, list,
-                           (const uint32_t*)W.hslot.p, W.hkey.p, W.hmx.p, !star);
     }
     if (part & REST_XYZ) {
         hipLaunchKernelGGL(k_vert_fields_ia, dim3(cgrid(c.vo)), dim3(256), 0, s, g.view, (const double*)val.vals.p,
@@ -1784,7 +1792,8 @@ void stage_rest(Workspace& W, const Grid& g, const Values& val, hipStream_t s, i
         FA.fq_a = W.fq_a.p;
         FA.fq_vs = W.fq_vs.p;
         FA.fve = W.fve.p;
-        FA.vid = W.vid.p;
+        FA.mo = W.mo.p;
+        FA.vrank = W.vrank.p;
         FA.act = W.act.p;
         FA.rank = W.rank.p;
         FA.foff = W.cf.p;
@@ -1900,6 +1909,7 @@ void run_batched(DevMem& ctx_mem, const Grid& g, Values& val, uint32_t flags, Ru
     bool ran = false;
     for (int attempt = 0; attempt < 3 && !ran; ++attempt) {
         if (!w.exec) {
+            w.ctr_clear = false;  // the eager passes below use the counters
             // size the workspace from an eager counting pass
             if (!w.ctr.p) {
                 Caps_ tiny;
@@ -2012,7 +2022,7 @@ void run_batched(DevMem& ctx_mem, const Grid& g, Values& val, uint32_t flags, Ru
                 RIN_HIP(hipEventCreateWithFlags(&w.join, hipEventDisableTiming));
             }
             RIN_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
-            stage_count(w, g, val, s);
+            stage_count(w, g, val, s, false);
             stage_fill(w, g, val, s);
             RIN_HIP(hipEventRecord(w.fork, s));
             RIN_HIP(hipStreamWaitEvent(w.side, w.fork, 0));
@@ -2044,6 +2054,8 @@ void run_batched(DevMem& ctx_mem, const Grid& g, Values& val, uint32_t flags, Ru
             RIN_HIP(hipMemsetAsync(w.hmx.p, 0, 4ull << w.hbits, s));
             w.hash_dirty = false;
         }
+        if (!w.ctr_clear) hipLaunchKernelGGL(k_zero_ctr, dim3(1), dim3(64), 0, s, w.ctr.p, (int)C_N, w.errs.p);
+        w.ctr_clear = false;
         RIN_HIP(hipEventRecord(w.ev[0], s));
         if (staged) {
             for (size_t i = 0; i < w.parts.size(); ++i) {
@@ -2055,6 +2067,7 @@ void run_batched(DevMem& ctx_mem, const Grid& g, Values& val, uint32_t flags, Ru
             RIN_HIP(hipEventRecord(w.ev[1], s));
         }
         read_back(false);
+        w.ctr_clear = true;  // the run's last kernel cleared them
         const uint32_t e = errors();
         if (e) w.hash_dirty = true;
         if (e & ERR_OVERFLOW) {

@@ -737,10 +737,24 @@ __global__ void __launch_bounds__(1024) k_scan_tiles(Count n, uint32_t tile, uin
 
 // Star path: each first occurrence's vertex id (its rank among first
 // occurrences, in occurrence order) from its tile's prefix, written where
-// the id gather reads it (vrank at first occurrences), and first_occ.
+// the id lookups read it (vrank at first occurrences: the id of occurrence o
+// is vrank[mo[o]]), and first_occ.
+// It also clears the hash slots the fallback used (one per distinct key:
+// that of its first occurrence), leaving the table zero for the next run.
 __global__ void __launch_bounds__(256) k_dedup_rank(Count n, const uint32_t* __restrict__ first,
                                                     const uint32_t* __restrict__ tile_first,
-                                                    uint32_t* __restrict__ vrank, uint32_t* __restrict__ first_occ) {
+                                                    uint32_t* __restrict__ vrank, uint32_t* __restrict__ first_occ,
+                                                    Count n_hash, const uint32_t* __restrict__ list,
+                                                    const uint32_t* __restrict__ hslot,
+                                                    unsigned long long* __restrict__ hkey, uint32_t* __restrict__ hmx) {
+    GRID_STRIDE(i, n_hash) {
+        const uint32_t o = list[i];
+        if (first[o]) {
+            const uint32_t h = hslot[o];
+            hkey[h] = 0;
+            hmx[h] = 0;
+        }
+    }
     const uint32_t NO = (uint32_t)count_of(n);
     const uint32_t base = blockIdx.x * kDedupTile;
     if (base >= NO) return;
@@ -1082,7 +1096,8 @@ constexpr int kFaceU = 4;
 constexpr uint32_t kFaceTile = 256u * kFaceU;
 
 struct FaceTileArgs {
-    const uint32_t *fq_a, *fq_vs, *fve, *vid, *act, *rank, *foff, *nbr;
+    const uint32_t *fq_a, *fq_vs, *fve, *act, *rank, *foff, *nbr;
+    const uint32_t *mo, *vrank;  // vertex id of occurrence o: vrank[mo[o]]
     const uint8_t* fq_bnd;
     const uint16_t* fq_l;
     const uint8_t* fq_fn;
@@ -1096,12 +1111,11 @@ struct FaceTileArgs {
     uint8_t* f_func;
 };
 
-// (min, second-min, max) of a face's vertex ids, its corners mapped through vid
-__device__ inline void face_triple_ids(const uint32_t* __restrict__ fve, uint32_t b, uint32_t e,
-                                       const uint32_t* __restrict__ vid, uint32_t k3[3]) {
+// (min, second-min, max) of a face's vertex ids
+__device__ inline void face_triple_ids(const FaceTileArgs& A, uint32_t b, uint32_t e, uint32_t k3[3]) {
     uint32_t mn = 0xFFFFFFFFu, sm = 0xFFFFFFFFu, mx = 0;
     for (uint32_t i = b; i < e; ++i) {
-        const uint32_t x = vid[fve[i]];
+        const uint32_t x = A.vrank[A.mo[A.fve[i]]];
         if (x < mn) {
             sm = mn;
             mn = x;
@@ -1129,7 +1143,7 @@ __global__ void __launch_bounds__(256) k_face_first_tiled(FaceTileArgs A) {
         const uint32_t b = A.fq_vs[q], e = (q + 1 < NQ) ? A.fq_vs[q + 1] : nfv;
         uint32_t mn = 0xFFFFFFFFu, sm = 0xFFFFFFFFu, mx = 0;
         for (uint32_t i = b; i < e; ++i) {
-            const uint32_t x = A.vid[A.fve[i]];
+            const uint32_t x = A.vrank[A.mo[A.fve[i]]];
             A.fvm[i] = x;
             if (x < mn) {
                 sm = mn;
@@ -1152,7 +1166,7 @@ __global__ void __launch_bounds__(256) k_face_first_tiled(FaceTileArgs A) {
                         if (A.fq_bnd[p] != 1 + l2) continue;
                         const uint32_t pb = A.fq_vs[p], pe = (p + 1 < NQ) ? A.fq_vs[p + 1] : nfv;
                         uint32_t p3[3];
-                        face_triple_ids(A.fve, pb, pe, A.vid, p3);
+                        face_triple_ids(A, pb, pe, p3);
                         if (p3[0] == mn && p3[1] == sm && p3[2] == mx) {
                             partner = p;
                             if (t2 < t)
