@@ -211,6 +211,25 @@ def test_batched_matches_oracle_and_replays(ctx, which, form, monkeypatch):
     g.release()
 
 
+def test_batched_explicit_mesh(ctx):
+    """The batched form on an unstructured mesh (no star tables: every
+    occurrence takes the hash path, ids by a global scan), across replays."""
+    import rin
+
+    m = np.load(S.REF_TESTS + "/mesh.npz")
+    pts, tets = m["pts"], m["tets"].astype(np.uint64)
+    types, params = S.load_ref_funcs("8-sphere.json")
+    g = ctx.grid_explicit(pts, tets)
+    vals = g.upload(S.evaluate(types, params, pts))
+    ref = S.OracleRun(0, pts, tets, S.evaluate(types, params, pts))
+    for rep in range(3):
+        dev = vals.run(rin.FLAG_BATCHED)
+        _compare(dev, ref)
+        dev.release()
+    vals.release()
+    g.release()
+
+
 @pytest.mark.parametrize("mi", [0, 1])
 @pytest.mark.parametrize("batched", [False, True])
 def test_large_complexes_are_cut_again(ctx, mi, batched):
