@@ -64,9 +64,9 @@ def cut_bytes(bucket):
 # single-kernel stages with an algorithmic byte count:
 # stage label prefix -> {form marker in the label: (kernel, file tag, bytes(sizes))}
 KERNEL_STAGES = {
-    "cut k=3..4": {"wave per tet": ("k_cut_wave<Caps<40,64,8,8>>", "k_cut_wave_k3to4", cut_bytes(0)),
+    "cut k=3..4": {"wave per tet": ("k_cut_wave<Caps<40,64,8,8>, 32>", "k_cut_wave_k3to4", cut_bytes(0)),
                    "thread per tet": ("k_cut_bucket<Caps<40,64,8,8>>", "k_cut_bucket_k3to4", cut_bytes(0))},
-    "cut k=5..8": {"wave per tet": ("k_cut_wave<Caps<132,284,10,12>>", "k_cut_wave_k5to8", cut_bytes(1)),
+    "cut k=5..8": {"wave per tet": ("k_cut_wave<Caps<132,284,10,12>, 64>", "k_cut_wave_k5to8", cut_bytes(1)),
                    "thread per tet": ("k_cut_bucket<Caps<132,284,10,12>>", "k_cut_bucket_k5to8", cut_bytes(1))},
 }
 
@@ -155,7 +155,7 @@ def traffic_for(tag):
 
 def _short(name):
     """Kernel name without namespaces, argument list and cap types' extra
-    parameters: 'k_cut_wave<Caps<40,64,8,8>>'."""
+    parameters: 'k_cut_wave<Caps<40,64,8,8>, 32>'."""
     name = re.sub(r"rin::\(anonymous namespace\)::", "", name)
     name = re.sub(r"^void ", "", name.strip())
     name = re.sub(r"rin::dev::Caps<(\d+), (\d+), (\d+), (\d+)[^>]*>", r"Caps<\1,\2,\3,\4>", name)
@@ -214,17 +214,61 @@ def pmc_summary(fetch_csv, write_csv, trace_csv, copy_bytes, command):
     print(open(md).read())
 
 
+def _replay_timeline(trace_csv):
+    """Kernels of the fastest full replay of the batched path in a kernel
+    trace (segments start at k_signs_count; the most common kernel count
+    marks a whole-path replay): (start us, duration us, queue, kernel)."""
+    rows = sorted(csv.DictReader(open(trace_csv)), key=lambda r: int(r["Start_Timestamp"]))
+    starts = [i for i, r in enumerate(rows) if _short(r["Kernel_Name"]) == "k_signs_count"]
+    segs = [rows[a:b] for a, b in zip(starts, starts[1:] + [len(rows)])]
+    if not segs:
+        return None
+    counts = [len(x) for x in segs]
+    mode = max(set(counts), key=counts.count)
+    best = None
+    for seg in segs:
+        if len(seg) != mode:
+            continue
+        t0 = int(seg[0]["Start_Timestamp"])
+        span = max(int(r["End_Timestamp"]) for r in seg) - t0
+        if best is None or span < best[0]:
+            best = (span, [((int(r["Start_Timestamp"]) - t0) / 1e3,
+                            (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3,
+                            r.get("Queue_Id", "?"), _short(r["Kernel_Name"])) for r in seg])
+    return best
+
+
 def trace_summary(stats_csv, command, title, out_name):
-    """Markdown table of a rocprofv3 --kernel-trace --stats run into profiles/."""
+    """Markdown table of a rocprofv3 --kernel-trace --stats run into profiles/:
+    per-kernel totals, means and medians (medians leave out the one-time work
+    of the first run), and for the batched path the timeline of its fastest
+    whole-path replay."""
     rows = list(csv.DictReader(open(stats_csv)))
+    trace_csv = stats_csv.replace("kernel_stats", "kernel_trace")
+    med = {}
+    if os.path.exists(trace_csv):
+        durs = {}
+        for r in csv.DictReader(open(trace_csv)):
+            durs.setdefault(_short(r["Kernel_Name"]), []).append(
+                int(r["End_Timestamp"]) - int(r["Start_Timestamp"]))
+        med = {k: sorted(v)[len(v) // 2] / 1e3 for k, v in durs.items()}
     lines = ["# Kernel trace: %s" % title, "", "Command (from the repository root on the GPU box):", "",
              "    " + command, "",
-             "Durations in microseconds over every dispatch of the run, one-time work included (first-run counting "
-             "pass, autotuning candidates, tet adjacency).", "",
-             "| kernel | calls | total us | avg us | % |", "|---|---|---|---|---|"]
+             "Durations in microseconds over every dispatch of the run; totals and means include one-time work "
+             "(first-run counting pass, autotuning candidates, tet adjacency), medians mostly do not.", "",
+             "| kernel | calls | total us | avg us | median us | % |", "|---|---|---|---|---|---|"]
     for r in rows[:30]:
-        lines.append("| %s | %s | %.1f | %.1f | %.1f |" % (_short(r["Name"]), r["Calls"], float(r["TotalDurationNs"]) / 1e3,
-                                                          float(r["AverageNs"]) / 1e3, float(r["Percentage"])))
+        k = _short(r["Name"])
+        lines.append("| %s | %s | %.1f | %.1f | %s | %.1f |" % (
+            k, r["Calls"], float(r["TotalDurationNs"]) / 1e3, float(r["AverageNs"]) / 1e3,
+            ("%.1f" % med[k]) if k in med else "-", float(r["Percentage"])))
+    tl = _replay_timeline(trace_csv) if os.path.exists(trace_csv) else None
+    if tl and len(tl[1]) > 5:
+        lines += ["", "Fastest whole-path replay: %.1f us from the first kernel's start to the last kernel's end "
+                      "(queues: the main stream and the side branch)." % (tl[0] / 1e3), "",
+                  "| start us | duration us | queue | kernel |", "|---|---|---|---|"]
+        for st, du, q, k in tl[1]:
+            lines.append("| %.1f | %.1f | %s | %s |" % (st, du, q, k))
     path = os.path.join(ROOT, "profiles", out_name)
     open(path, "w").write("\n".join(lines) + "\n")
     print(open(path).read())
@@ -245,6 +289,8 @@ def main():
                     help="server slots (the library currently runs one job at a time)")
     ap.add_argument("--mi-steps", type=int, default=10,
                     help="timed steps of the material-interface leg (0: skip)")
+    ap.add_argument("--mi-only", action="store_true",
+                    help="run only the material-interface leg (prints its JSON object)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also run plain device copies of MIB MiB (reference kernel for counter scaling)")
     ap.add_argument("--pmc-summary", nargs=3, metavar=("FETCH_CSV", "WRITE_CSV", "TRACE_CSV"),
@@ -266,6 +312,12 @@ def main():
 
     import rin
     import workloads as W
+
+    if args.mi_only:
+        ctx = rin.Context(0)
+        print(json.dumps(mi_leg(ctx, max(1, args.mi_steps), args.warmup, not args.no_cpu)))
+        ctx.close()
+        return
 
     res, fs = W.config2()
     res = args.res

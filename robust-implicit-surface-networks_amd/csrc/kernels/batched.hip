@@ -380,6 +380,7 @@ struct FillArgs {
     uint8_t* act_k;
     const K1Entry* k1;
     uint32_t* list[3];      // buckets k=2, 3..4, 5..8
+    uint64_t* lts[3];       // per slot: CSR offset << 32 | tet
     uint32_t cap[3];
     uint32_t rec_base[3];   // word offsets of each bucket's slots
     uint32_t slot[3];       // words per slot
@@ -408,10 +409,11 @@ __global__ void k_fill_batched(FillArgs A) {
         }
         const int k = __popcll(m);
         int b = -1;
-        uint32_t r = 0;
+        uint32_t r = 0, o0 = 0;
         if (k) {
             r = A.rank[t];
             uint32_t o = A.csr_off[t];
+            o0 = o;
             if (r >= A.cap_active || o + (uint32_t)k > A.cap_fit) {
                 A.errs[lane] |= ERR_OVERFLOW;
                 A.ctr[C_ABORT] = 1;
@@ -449,6 +451,7 @@ __global__ void k_fill_batched(FillArgs A) {
                 const uint32_t slot = start + (uint32_t)__popcll(mask & lt);
                 if (slot < A.cap[q]) {
                     A.list[q][slot] = r;
+                    A.lts[q][slot] = (uint64_t)o0 << 32 | (uint32_t)t;
                     A.rec_loc[r] = A.rec_base[q] + slot * A.slot[q];
                 } else {
                     A.errs[lane] |= ERR_OVERFLOW;
@@ -716,6 +719,7 @@ __global__ void __launch_bounds__(kCubeTile) k_cube_fill(FillArgs A, const TileS
                     const uint32_t slot = bnext[qq]++;
                     if (slot < A.cap[qq]) {
                         A.list[qq][slot] = r;
+                        A.lts[qq][slot] = (uint64_t)o << 32 | (uint32_t)t;
                         A.rec_loc[r] = A.rec_base[qq] + slot * A.slot[qq];
                     } else {
                         A.errs[lane] |= ERR_OVERFLOW;
@@ -743,11 +747,13 @@ struct CutArgs {
     GridView g;
     const double* vals;
     const uint32_t *act, *csr_off, *list, *rec_loc;
+    const uint64_t* lts;  // per slot: CSR offset << 32 | tet
     const uint8_t *act_k, *fit;
     const uint32_t* n;   // bucket size (device)
     const uint32_t* abort;
     uint32_t* queue;     // work counter
     uint32_t cap, slot;
+    uint32_t F;          // functions (bounds the speculative corner-value loads)
     uint16_t* rec;
     uint32_t *cnt, *errs;
     uint32_t* ctr;       // for the work statistics (spread counters)
@@ -815,13 +821,19 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
     const uint32_t i = blockIdx.x * NG + gid;
     if (i >= n) return;
     const uint32_t r = A.list[i];
+    const uint64_t ts = A.lts[i];
     const int k = A.act_k[r];
-    const uint32_t t = A.act[r];
-    if (lane < 4 * k && 4 + k <= C::MP) {
+    // corner values: the loads need not wait for k (rows past the tet's
+    // functions read the CSR's next entries, inside its head-room of 64,
+    // with the function id clamped into range; their values go unused)
+    if (lane < 4 * (C::MP - 4)) {
+        const uint32_t t = (uint32_t)ts, s0 = (uint32_t)(ts >> 32);
         uint32_t v[4];
         dev::tet_corners(A.g, t, v);
-        const uint32_t f = A.fit[A.csr_off[t] + (lane >> 2)];
-        X.P[lane >> 2][lane & 3] = A.vals[(uint64_t)f * A.g.V + v[lane & 3]];
+        const uint32_t fr = A.fit[s0 + (lane >> 2)];
+        const uint32_t f = fr < A.F ? fr : 0u;
+        const double x = A.vals[(uint64_t)f * A.g.V + v[lane & 3]];
+        if (lane < 4 * k && 4 + k <= C::MP) X.P[lane >> 2][lane & 3] = x;
     }
     dev::wave_ia_cut<C, G>(X, k);
     uint16_t* out = A.rec + A.rec_loc[r];
@@ -854,6 +866,7 @@ struct K2Args {
     const double* vals;
     const uint64_t *pos, *neg;
     const uint32_t *act, *csr_off, *list;
+    const uint64_t* lts;  // per slot: CSR offset << 32 | tet
     const uint8_t* fit;
     const uint32_t* n;  // bucket size (device)
     const uint32_t* abort;
@@ -874,8 +887,8 @@ __global__ void k_k2_lookup(K2Args A) {
     const uint64_t V = A.g.V;
     GRID_STRIDE(i, n) {
         const uint32_t r = A.list[i];
-        const uint32_t t = A.act[r];
-        const uint32_t s0 = A.csr_off[t];
+        const uint64_t ts = A.lts[i];
+        const uint32_t t = (uint32_t)ts, s0 = (uint32_t)(ts >> 32);
         const uint32_t fa = A.fit[s0], fb = A.fit[s0 + 1];
         uint32_t v[4];
         dev::tet_corners(A.g, t, v);
@@ -938,6 +951,7 @@ struct ExtractArgs {
     uint8_t *fq_fn, *fq_bnd;
     uint32_t *fq_vs, *fve, *errs;
     const uint32_t* list[3];  // bucket lists (k = 2, 3..4, 5..8) of active ranks
+    const uint64_t* lts[3];   // per slot: CSR offset << 32 | tet
     uint32_t list_cap[3];
     uint32_t slot_words[3];   // words per record slot of each bucket
     uint32_t fo_table[3];     // 1: the bucket's cut leaves face offsets after a record that fits
@@ -1006,11 +1020,11 @@ __global__ void k_extract_records(ExtractArgs A) {
     uint32_t recw = 0;  // record words read (throughput accounting)
     GRID_STRIDE(i, n0) {
         const uint32_t a = A.list[0][i];
+        const uint64_t ts = A.lts[0][i];
         const uint32_t loc = A.rec_loc[a];
-        const uint32_t t = A.act[a];
+        const uint32_t t = (uint32_t)ts, s0 = (uint32_t)(ts >> 32);
         uint32_t v[4];
         dev::tet_corners(A.g, t, v);
-        const uint32_t s0 = A.csr_off[t];
         const uint32_t ti = loc & ~K1_FLAG;
         const uint32_t po = ((loc & K1_FLAG) && ti >= 81 && ti - 81 < A.n_plans) ? A.plan_off[ti - 81] : K2_PLAN_NONE;
         if (po != K2_PLAN_NONE) {
@@ -1288,6 +1302,7 @@ struct Workspace {
     bool wave_cut[3] = {false, false, false};  // bucket cut by the wave form (records carry face offsets)
     DBuf<uint64_t> pos, neg;
     DBuf<uint32_t> kcnt, isact, csr_off, rank, act, rec_loc, cnt, list[3];
+    DBuf<uint64_t> lts[3];  // per bucket slot: CSR offset << 32 | tet
     DBuf<TileSums> tile_sums;  // regular grids: per tile of cubes
     DBuf<uint8_t> fit, act_k;
     DBuf<uint16_t> rec;
@@ -1460,6 +1475,7 @@ void alloc_ws(Workspace& W, const Grid& g, const Caps_& c) {
     uint64_t words = 0;
     for (int q = 0; q < 3; ++q) {
         W.list[q].alloc(m, c.b[q] + 1);
+        W.lts[q].alloc(m, c.b[q] + 1);
         W.rec_base[q] = (uint32_t)words;
         words += (uint64_t)c.b[q] * W.slot[q];
     }
@@ -1564,6 +1580,7 @@ void launch_cut(Workspace& W, const Grid& g, const Values& val, int q, int grid,
     A.act = W.act.p;
     A.csr_off = W.csr_off.p;
     A.list = W.list[q].p;
+    A.lts = W.lts[q].p;
     A.rec_loc = W.rec_loc.p;
     A.act_k = W.act_k.p;
     A.fit = W.fit.p;
@@ -1572,6 +1589,7 @@ void launch_cut(Workspace& W, const Grid& g, const Values& val, int q, int grid,
     A.queue = W.ctr.p + kQueueBase + q;
     A.cap = W.cap.b[q];
     A.slot = W.slot[q];
+    A.F = val.F;
     A.rec = W.rec.p;
     A.cnt = W.cnt.p;
     A.errs = W.errs.p;
@@ -1587,6 +1605,7 @@ void launch_cut(Workspace& W, const Grid& g, const Values& val, int q, int grid,
         K.act = W.act.p;
         K.csr_off = W.csr_off.p;
         K.list = W.list[q].p;
+        K.lts = W.lts[q].p;
         K.fit = W.fit.p;
         K.n = A.n;
         K.abort = A.abort;
@@ -1636,6 +1655,7 @@ void stage_fill(Workspace& W, const Grid& g, const Values& val, hipStream_t s) {
     A.k1 = W.k1.p;
     for (int q = 0; q < 3; ++q) {
         A.list[q] = W.list[q].p;
+        A.lts[q] = W.lts[q].p;
         A.cap[q] = W.cap.b[q];
         A.rec_base[q] = W.rec_base[q];
         A.slot[q] = W.slot[q];
@@ -1717,6 +1737,7 @@ void stage_rest(Workspace& W, const Grid& g, const Values& val, hipStream_t s, i
         E.errs = W.errs.p;
         for (int q = 0; q < 3; ++q) {
             E.list[q] = W.list[q].p;
+            E.lts[q] = W.lts[q].p;
             E.list_cap[q] = c.b[q];
             E.slot_words[q] = W.slot[q];
             E.fo_table[q] = W.wave_cut[q] ? 1u : 0u;
