@@ -198,7 +198,7 @@ def pmc_summary(fetch_csv, write_csv, trace_csv, copy_bytes, command):
         w = sum(write.get(k, [0])) / max(1, len(write.get(k, [0])))
         d = durs.get(k)
         rows.append((f * sf + w * sw, k, len(fetch[k]), f, w, (sum(d) / len(d) / 1e3) if d else float("nan")))
-    for mb, k, n, f, w, us in sorted(rows, reverse=True)[:25]:
+    for mb, k, n, f, w, us in sorted(rows, reverse=True)[:45]:
         lines.append("| %s | %d | %.0f | %.0f | %.2f | %.1f |" % (k, n, f, w, mb * 1024 / 1e6, us))
     md = os.path.join(ROOT, "profiles", "hbm_traffic_%s.md" % WORKLOAD)
     open(md, "w").write("\n".join(lines) + "\n")
