@@ -54,7 +54,8 @@ typedef struct rin_func_desc {
 /* rin_run flags */
 #define RIN_FLAG_MI 1u           /* material interface instead of implicit arrangement */
 #define RIN_FLAG_REVERSE 2u      /* insert active functions in descending order (with CUT_ONLY) */
-#define RIN_FLAG_NO_LOOKUP 4u    /* general cut for every tet */
+#define RIN_FLAG_NO_LOOKUP 4u    /* general cut for every tet (no k=1/k=2 or 2-material tables;
+                                    with BATCHED the run takes the plain form) */
 #define RIN_FLAG_BATCHED 8u      /* bucketed fused form */
 #define RIN_FLAG_KEEP_RECORDS 16u /* keep per-tet cut records for rin_result_records */
 #define RIN_FLAG_CUT_ONLY 32u    /* stop after the per-tet cut (the -R robustness check) */

@@ -1891,6 +1891,8 @@ float time_ms(hipStream_t s, const std::function<void()>& f) {
 void run_batched(DevMem& ctx_mem, const Grid& g, Values& val, uint32_t flags, RunOut& out, hipStream_t s, int n_cu) {
     if (flags & (RUN_MI | RUN_CUT_ONLY | RUN_REVERSE))
         throw std::runtime_error("batched form is not available for this mode (use the plain form)");
+    // the batched form is built on the k = 1 and k = 2 tables
+    if (flags & RUN_NO_LOOKUP) throw NeedsPlainForm("lookups off: the general cut runs for every tet");
     const bool staged = (flags & RUN_STAGE_TIMES) != 0;
     const uint64_t V = g.view.V, T = g.view.T;
     if (val.V != V) throw std::invalid_argument("values do not match the grid");
