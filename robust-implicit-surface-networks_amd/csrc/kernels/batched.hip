@@ -1781,18 +1781,20 @@ void stage_rest(Workspace& W, const Grid& g, const Values& val, hipStream_t s, i
             S.tile_first = W.tile_first.p;
             hipLaunchKernelGGL(k_dedup_star, dim3(tgrid), dim3(256), 0, s, NVO, S);
         }
-        hipLaunchKernelGGL(k_hash_insert, dim3(hgrid), dim3(256), 0, s, NH, list, (const uint64_t*)W.okey.p, W.hkey.p,
-                           W.hmx.p, W.hbits, W.hslot.p, W.errs.p);
-        hipLaunchKernelGGL(k_hash_first, dim3(hgrid), dim3(256), 0, s, NH, list, (const uint32_t*)W.hslot.p,
-                           (const uint32_t*)W.hmx.p, W.first.p, W.mo.p, star ? W.tile_first.p : nullptr);
         if (star) {
-            // ranks from the per-tile counts of the star pass
-            TileTotals tt = {{ctr + C_VERTS, nullptr, nullptr}};
-            hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, s, NVO, kDedupTile, W.tile_first.p, 1, 1, tt);
+            // the fallback and the scan of the star pass's tile counts in one
+            // workgroup, then ranks per tile
+            hipLaunchKernelGGL(k_dedup_fallback_scan, dim3(1), dim3(1024), 0, s, NH, list, (const uint64_t*)W.okey.p,
+                               W.hkey.p, W.hmx.p, W.hbits, W.hslot.p, W.errs.p, W.first.p, W.mo.p, NVO,
+                               W.tile_first.p, ctr + C_VERTS);
             hipLaunchKernelGGL(k_dedup_rank, dim3(tgrid), dim3(256), 0, s, NVO, (const uint32_t*)W.first.p,
                                (const uint32_t*)W.tile_first.p, W.vrank.p, W.first_occ.p, NH, list,
                                (const uint32_t*)W.hslot.p, W.hkey.p, W.hmx.p);
         } else {
+            hipLaunchKernelGGL(k_hash_insert, dim3(hgrid), dim3(256), 0, s, NH, list, (const uint64_t*)W.okey.p,
+                               W.hkey.p, W.hmx.p, W.hbits, W.hslot.p, W.errs.p);
+            hipLaunchKernelGGL(k_hash_first, dim3(hgrid), dim3(256), 0, s, NH, list, (const uint32_t*)W.hslot.p,
+                               (const uint32_t*)W.hmx.p, W.first.p, W.mo.p, nullptr);
             scan_u32(m, W.first.p, W.vrank.p, NVO, ctr + C_VERTS, s);
             hipLaunchKernelGGL(k_dedup_assign, dim3(cgrid(c.vo)), dim3(256), 0, s, NVO, (const uint32_t*)W.mo.p,
                                (const uint32_t*)W.first.p, (const uint32_t*)W.vrank.p, W.vid.p, W.first_occ.p, NH,

@@ -699,9 +699,10 @@ struct TileTotals {
     uint32_t* p[3];
 };
 
-__global__ void __launch_bounds__(1024) k_scan_tiles(Count n, uint32_t tile, uint32_t* __restrict__ x, int stride,
-                                                     int nfield, TileTotals total) {
-    const uint32_t nt = (uint32_t)((count_of(n) + tile - 1) / tile);
+// one workgroup of 1024 lanes; the counts are read with device-scope loads,
+// so that counts other workgroups of this kernel added atomically are seen
+__device__ inline void scan_tiles_block(uint32_t nt, uint32_t* __restrict__ x, int stride, int nfield,
+                                        const TileTotals& total) {
     __shared__ uint32_t wsum[16];
     __shared__ uint32_t carry;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -711,7 +712,10 @@ __global__ void __launch_bounds__(1024) k_scan_tiles(Count n, uint32_t tile, uin
         for (uint32_t base = 0; base < nt; base += 4 * 1024) {
             const uint32_t i0 = base + 4 * threadIdx.x;
             uint32_t v[4];
-            for (int q = 0; q < 4; ++q) v[q] = i0 + q < nt ? x[(uint64_t)(i0 + q) * stride + f] : 0u;
+            for (int q = 0; q < 4; ++q)
+                v[q] = i0 + q < nt ? __hip_atomic_load(&x[(uint64_t)(i0 + q) * stride + f], __ATOMIC_RELAXED,
+                                                       __HIP_MEMORY_SCOPE_AGENT)
+                                   : 0u;
             const uint32_t sum = v[0] + v[1] + v[2] + v[3];
             uint32_t inc = sum;
             for (int d = 1; d < 64; d <<= 1) {
@@ -733,6 +737,38 @@ __global__ void __launch_bounds__(1024) k_scan_tiles(Count n, uint32_t tile, uin
         }
         if (threadIdx.x == 0) *total.p[f] = carry;
     }
+}
+
+__global__ void __launch_bounds__(1024) k_scan_tiles(Count n, uint32_t tile, uint32_t* __restrict__ x, int stride,
+                                                     int nfield, TileTotals total) {
+    scan_tiles_block((uint32_t)((count_of(n) + tile - 1) / tile), x, stride, nfield, total);
+}
+
+// Star path, after the star pass, in one workgroup: the hash fallback for
+// the occurrences the star tables left (only degenerate inputs leave any; a
+// long list is taken in passes), their first occurrences joining the tile
+// counts, then the scan of the tile counts (*total: the vertex count).
+__global__ void __launch_bounds__(1024) k_dedup_fallback_scan(Count n_fail, const uint32_t* __restrict__ list,
+                                                              const uint64_t* __restrict__ okey,
+                                                              unsigned long long* __restrict__ hkey,
+                                                              uint32_t* __restrict__ hmx, int bits,
+                                                              uint32_t* __restrict__ hslot, uint32_t* __restrict__ errs,
+                                                              uint32_t* __restrict__ first, uint32_t* __restrict__ mo,
+                                                              Count n, uint32_t* __restrict__ tile_first,
+                                                              uint32_t* __restrict__ total) {
+    const uint32_t nf = (uint32_t)count_of(n_fail);
+    for (uint32_t i = threadIdx.x; i < nf; i += blockDim.x) hash_insert_one(list[i], okey, hkey, hmx, bits, hslot, errs);
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < nf; i += blockDim.x) {
+        const uint32_t o = list[i];
+        const uint32_t m = ~__hip_atomic_load(&hmx[hslot[o]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        first[o] = m == o;
+        mo[o] = m;
+        if (m == o) atomicAdd(&tile_first[o / kDedupTile], 1u);
+    }
+    __syncthreads();
+    const TileTotals tt = {{total, nullptr, nullptr}};
+    scan_tiles_block((uint32_t)((count_of(n) + kDedupTile - 1) / kDedupTile), tile_first, 1, 1, tt);
 }
 
 // Star path: each first occurrence's vertex id (its rank among first
