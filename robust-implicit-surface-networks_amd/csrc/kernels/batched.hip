@@ -4,15 +4,16 @@
 //    never runs the cut: its complex depends only on the four corner signs,
 //    so an 81-entry table built from the cut itself gives counts, iso
 //    vertices, iso faces and the record; each block stages the table in LDS.
-//  * k = 2, 3..4 and 5..8 run per-bucket cut kernels whose thread-private
-//    state is sized for that bucket, as persistent grids pulling 64-tet
-//    chunks from a work queue; grid sizes come from a small autotuner that
-//    times candidates on the run's own buckets.
+//  * k = 2 goes by a sign-vector table; 3..4 and 5..8 run per-bucket cut
+//    kernels sized for that bucket: lane groups per tet with the complex in
+//    LDS, or one lane per tet (persistent grids pulling 64-tet chunks from a
+//    work queue); a small autotuner times the candidates on the run's own
+//    buckets.
 //  * Every stage reads its element count from device memory and checks it
 //    against the workspace capacity, so the whole path is captured once as a
-//    hipGraph and replayed; one small read-back per run returns the sizes and
-//    the overflow flags (on overflow the workspace grows and the graph is
-//    re-captured).
+//    hipGraph and replayed; the run's last kernel publishes the sizes and the
+//    overflow flags into mapped host memory (on overflow the workspace grows
+//    and the graph is re-captured).
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
@@ -1277,12 +1278,6 @@ __global__ void k_zero_ctr(uint32_t* ctr, int n, uint32_t* errs) {
         if (threadIdx.x < kSpread) ctr[spread_at(c, threadIdx.x)] = 0;
 }
 
-// the counters the host reads back after a replay
-__global__ void k_cap_check(const uint32_t* ctr, uint32_t cap_vo, uint32_t cap_fo, uint32_t cap_fv, uint32_t* errs) {
-    const int lane = threadIdx.x & 63;
-    if (lane == 0 && (ctr[C_VO] > cap_vo || ctr[C_FO] > cap_fo || ctr[C_FV] > cap_fv)) errs[lane] |= ERR_OVERFLOW;
-}
-
 
 }  // namespace
 
@@ -1794,7 +1789,7 @@ void stage_rest(Workspace& W, const Grid& g, const Values& val, hipStream_t s, i
             hipLaunchKernelGGL(k_hash_insert, dim3(hgrid), dim3(256), 0, s, NH, list, (const uint64_t*)W.okey.p,
                                W.hkey.p, W.hmx.p, W.hbits, W.hslot.p, W.errs.p);
             hipLaunchKernelGGL(k_hash_first, dim3(hgrid), dim3(256), 0, s, NH, list, (const uint32_t*)W.hslot.p,
-                               (const uint32_t*)W.hmx.p, W.first.p, W.mo.p, nullptr);
+                               (const uint32_t*)W.hmx.p, W.first.p, W.mo.p);
             scan_u32(m, W.first.p, W.vrank.p, NVO, ctr + C_VERTS, s);
             hipLaunchKernelGGL(k_dedup_assign, dim3(cgrid(c.vo)), dim3(256), 0, s, NVO, (const uint32_t*)W.mo.p,
                                (const uint32_t*)W.first.p, (const uint32_t*)W.vrank.p, W.vid.p, W.first_occ.p, NH,

@@ -834,17 +834,13 @@ __global__ void k_hash_insert(Count n, const uint32_t* __restrict__ list, const 
     GRID_STRIDE(i, n) hash_insert_one(list ? list[i] : (uint32_t)i, okey, hkey, hmx, bits, hslot, errs);
 }
 
-// tile_first (star path, or null): the first occurrences found here join
-// their tile's count
 __global__ void k_hash_first(Count n, const uint32_t* __restrict__ list, const uint32_t* __restrict__ hslot,
-                             const uint32_t* __restrict__ hmx, uint32_t* __restrict__ first, uint32_t* __restrict__ mo,
-                             uint32_t* __restrict__ tile_first) {
+                             const uint32_t* __restrict__ hmx, uint32_t* __restrict__ first, uint32_t* __restrict__ mo) {
     GRID_STRIDE(i, n) {
         const uint32_t o = list ? list[i] : (uint32_t)i;
         const uint32_t m = ~hmx[hslot[o]];
         first[o] = m == o;
         mo[o] = m;
-        if (tile_first && m == o) atomicAdd(&tile_first[o / kDedupTile], 1u);
     }
 }
 

@@ -120,35 +120,50 @@ struct MITet {
         dev::tet_corners(g, t, v);
     }
     __device__ const uint16_t* groups() const { return R.r + R.r[6] - R.L; }
+    // corner c by selects (an indexed read would move v to scratch memory)
+    __device__ uint32_t vc(int c) const { return c == 0 ? v[0] : (c == 1 ? v[1] : (c == 2 ? v[2] : v[3])); }
 };
 
+// global ids of the tet edge between corners ca < cb and of the tet face
+// opposite corner c, from the corners in registers (as dev::edge_id and
+// dev::face_id)
+__device__ inline uint64_t mi_edge_id(const GridView& g, const MITet& M, int ca, int cb) {
+    if (!g.regular) return g.tedge[6 * (uint64_t)M.t + dev::local_edge(ca, cb)];
+    const uint32_t x = M.vc(ca), y = M.vc(cb);
+    const uint32_t a = x < y ? x : y, b = x < y ? y : x;
+    return (uint64_t)a * 27 + dev::lattice_dir(g.res, a, b);
+}
+__device__ inline uint64_t mi_face_id(const GridView& g, const MITet& M, int c) {
+    if (!g.regular) return g.tface[4 * (uint64_t)M.t + c];
+    uint32_t x = M.vc(c == 0 ? 1 : 0), y = M.vc(c <= 1 ? 2 : 1), z = M.vc(c <= 2 ? 3 : 2);
+    uint32_t u;
+    if (y < x) { u = x; x = y; y = u; }
+    if (z < y) { u = y; y = z; z = u; }
+    if (y < x) { u = x; x = y; y = u; }
+    return ((uint64_t)x * 27 + dev::lattice_dir(g.res, x, y)) * 27 + dev::lattice_dir(g.res, x, z);
+}
+
 // key of local vertex j (labels sorted ascending: tet faces first); nb = number
-// of tet-face labels
+// of tet-face labels.  Bit logic instead of indexed local arrays keeps the
+// labels and corners in registers.
 __device__ inline uint64_t mi_vkey(const GridView& g, const MITet& M, const uint8_t* __restrict__ fit, uint64_t a,
                                    int j, int& nb) {
     const uint16_t* L4 = M.R.vert(j);
-    int bnd[4];
-    uint32_t m[4];
-    int nm = 0;
-    nb = 0;
-    for (int q = 0; q < 4; ++q) {
-        if (L4[q] > 3)
-            m[nm++] = fit[M.s0 + L4[q] - 4];
-        else
-            bnd[nb++] = L4[q];
-    }
-    if (nb == 3) {
-        int c = 0;
-        while (c == bnd[0] || c == bnd[1] || c == bnd[2]) ++c;
-        return key_corner(M.v[c]);
-    }
+    const uint32_t l0 = L4[0], l1 = L4[1], l2 = L4[2], l3 = L4[3];
+    unsigned bm = 0;
+    if (l0 <= 3) bm |= 1u << l0;
+    if (l1 <= 3) bm |= 1u << l1;
+    if (l2 <= 3) bm |= 1u << l2;
+    if (l3 <= 3) bm |= 1u << l3;
+    nb = __popc(bm);
+    auto mat = [&](uint32_t l) { return (uint32_t)fit[M.s0 + l - 4]; };
+    if (nb == 3) return key_corner(M.vc(__builtin_ctz(~bm & 15u)));
     if (nb == 2) {
-        int l[2], nl = 0;
-        for (int c = 0; c < 4; ++c)
-            if (c != bnd[0] && c != bnd[1]) l[nl++] = c;
-        return key_mi_edge(dev::edge_id(g, M.t, M.v, l[0], l[1]), m[0], m[1]);
+        const unsigned m = ~bm & 15u;
+        const int c0 = __builtin_ctz(m), c1 = __builtin_ctz(m & (m - 1));
+        return key_mi_edge(mi_edge_id(g, M, c0, c1), mat(l2), mat(l3));
     }
-    if (nb == 1) return key_mi_face(dev::face_id(g, M.t, M.v, bnd[0]), m[0], m[1], m[2]);
+    if (nb == 1) return key_mi_face(mi_face_id(g, M, (int)l0), mat(l1), mat(l2), mat(l3));
     return key_inner(a, j);
 }
 
@@ -212,12 +227,17 @@ __device__ inline bool mi_bface_emits(const MIArgs& A, const MITet& M, uint64_t 
     const uint32_t r2 = A.rank[t2];
     if (!(r2 < A.n_active && A.act[r2] == t2)) return false;
     uint64_t k3[3];
-    mi_face_triple(A.g, M, A.fit, a, F, k3);
+    bool have_k3 = false;
     const MITet N(A.g, A.rec + A.rec_loc[r2], t2, A.csr_off[t2]);
     const uint16_t* G = N.R.faces();
     for (int f = 0; f < N.R.nf; ++f) {
         const int n = G[3];
-        if (G[0] == l2) {
+        // a copy lies on the shared tet face and has as many corners
+        if (G[0] == l2 && n == F[3]) {
+            if (!have_k3) {
+                mi_face_triple(A.g, M, A.fit, a, F, k3);
+                have_k3 = true;
+            }
             uint64_t q3[3];
             mi_face_triple(A.g, N, A.fit, r2, G, q3);
             if (q3[0] == k3[0] && q3[1] == k3[1] && q3[2] == k3[2]) {
@@ -264,29 +284,104 @@ __device__ inline int corner_of(const Rec& R, int j) {
 }
 
 // Which tet-boundary faces of each active tet become interface faces here
-// (mi_bface_emits, the neighbour-record scan): bit f for face f < 63; bit 63
-// marks a tet with more faces, whose faces from 63 on are tested where they
-// are used.  kBfaceLanes lanes per tet (adjacent lanes of one wave) take faces
-// f = lane, lane + kBfaceLanes, ... and OR their bits together.
+// (the reference's two-map pairing): bit f for face f < 63; bit 63 marks a
+// tet with more faces, whose faces from 63 on are tested where they are used.
+// The faces' keys are computed once: per active tet its face count (k_mi_nf),
+// a scan gives each tet's first face slot, k_mi_face_keys writes every face's
+// tet face, corner count, record word offset and (boundary faces) its
+// (min, second-min, max) vertex-key triple, and k_mi_bface compares a
+// boundary face's triple with those of the neighbour's faces on the shared
+// tet face.  kBfaceLanes lanes per tet (adjacent lanes of one wave) take
+// faces f = lane, lane + kBfaceLanes, ...
 constexpr int kBfaceLanes = 8;
 constexpr int kBfaceBits = 63;
 
-__global__ void k_mi_bface(MIArgs A, uint64_t* __restrict__ emask) {
+struct MIFaceKeys {
+    const uint32_t* fb;  // first face slot of each active tet (n_active + 1 entries)
+    uint16_t* fl;        // tet face (0..3, or 0xFF inside) | corners << 8
+    uint16_t* fw;        // word offset of the face in its record
+    uint64_t* ft;        // vertex-key triple (boundary faces)
+};
+
+__global__ void k_mi_nf(MIArgs A, uint32_t* __restrict__ nf) {
+    GRID_STRIDE(a, A.n_active) {
+        const uint16_t* r = A.rec + A.rec_loc[a];
+        nf[a] = r[0] == 2 ? r[3] : 0u;
+    }
+}
+
+__global__ void k_mi_face_keys(MIArgs A, MIFaceKeys K) {
     const uint64_t n = A.n_active * kBfaceLanes;
     GRID_STRIDE(idx, n) {
         const uint64_t a = idx / kBfaceLanes;
         const int l0 = (int)(idx % kBfaceLanes);
         const uint32_t t = A.act[a];
         const MITet M(A.g, A.rec + A.rec_loc[a], t, A.csr_off[t]);
+        if (M.R.r[0] != 2) continue;
+        const int nf = M.R.nf;
+        const uint32_t fb = K.fb[a];
+        const uint16_t* F = M.R.faces();
+        for (int f = 0; f < l0 && f < nf; ++f) F += 4 + F[3];
+        for (int f = l0; f < nf; f += kBfaceLanes) {
+            const uint32_t slot = fb + (uint32_t)f;
+            const int l = F[0];
+            K.fl[slot] = (uint16_t)((l <= 3 ? l : 0xFF) | F[3] << 8);
+            K.fw[slot] = (uint16_t)(F - M.R.r);
+            if (l <= 3) {
+                uint64_t k3[3];
+                mi_face_triple(A.g, M, A.fit, a, F, k3);
+                K.ft[3 * (uint64_t)slot] = k3[0];
+                K.ft[3 * (uint64_t)slot + 1] = k3[1];
+                K.ft[3 * (uint64_t)slot + 2] = k3[2];
+            }
+            for (int q = 0; q < kBfaceLanes && f + q + 1 < nf; ++q) F += 4 + F[3];
+        }
+    }
+}
+
+__global__ void k_mi_bface(MIArgs A, MIFaceKeys K, uint64_t* __restrict__ emask) {
+    const uint64_t n = A.n_active * kBfaceLanes;
+    GRID_STRIDE(idx, n) {
+        const uint64_t a = idx / kBfaceLanes;
+        const int l0 = (int)(idx % kBfaceLanes);
+        const uint32_t t = A.act[a];
+        const uint16_t* r = A.rec + A.rec_loc[a];
         uint64_t bits = 0;
-        if (M.R.r[0] == 2) {
-            const uint16_t* F = M.R.faces();
-            const int nf = M.R.nf;
-            for (int f = 0; f < nf && f < kBfaceBits; ++f) {
-                const int nfv = F[3];
-                uint8_t other;
-                if (f % kBfaceLanes == l0 && F[0] <= 3 && mi_bface_emits(A, M, a, F, other)) bits |= 1ull << f;
-                F += 4 + nfv;
+        if (r[0] == 2) {
+            const int nf = r[3], nt = nf < kBfaceBits ? nf : kBfaceBits;
+            const uint32_t fb = K.fb[a];
+            for (int f = l0; f < nt; f += kBfaceLanes) {
+                const uint32_t slot = fb + (uint32_t)f;
+                const uint16_t fl = K.fl[slot];
+                const uint32_t l = fl & 0xFFu;
+                if (l > 3) continue;
+                const uint32_t nb = A.nbr[4 * (uint64_t)t + l];
+                if (nb == NO_NBR) continue;
+                const uint32_t t2 = nb >> 2, l2 = nb & 3;
+                if (t2 >= t) continue;
+                const uint32_t r2 = A.rank[t2];
+                if (!(r2 < A.n_active && A.act[r2] == t2)) continue;
+                const uint64_t k0 = K.ft[3 * (uint64_t)slot], k1 = K.ft[3 * (uint64_t)slot + 1],
+                               k2 = K.ft[3 * (uint64_t)slot + 2];
+                // the copy: on the shared tet face, as many corners, same key triple
+                const uint16_t want = (uint16_t)(l2 | (fl & 0xFF00u));
+                const uint32_t g0 = K.fb[r2], g1 = K.fb[r2 + 1];
+                for (uint32_t gs = g0; gs < g1; ++gs) {
+                    if (K.fl[gs] != want) continue;
+                    if (K.ft[3 * (uint64_t)gs] != k0 || K.ft[3 * (uint64_t)gs + 1] != k1 ||
+                        K.ft[3 * (uint64_t)gs + 2] != k2)
+                        continue;
+                    const MITet M(A.g, r, t, A.csr_off[t]);
+                    const MITet N(A.g, A.rec + A.rec_loc[r2], t2, A.csr_off[t2]);
+                    const uint16_t* G = N.R.r + K.fw[gs];
+                    const uint16_t* F = M.R.r + K.fw[slot];
+                    bool share = false;
+                    mi_side_each(N, A.fit, G[1], [&](uint8_t x) {
+                        mi_side_each(M, A.fit, F[1], [&](uint8_t y) { share |= x == y; });
+                    });
+                    if (!share) bits |= 1ull << f;
+                    break;
+                }
             }
             if (nf > kBfaceBits) bits |= 1ull << kBfaceBits;
         }
@@ -380,7 +475,7 @@ __global__ void k_mi_extract(MIArgs A, const uint64_t* __restrict__ emask, const
                     const int j = F[4 + i];
                     if (!iface && is_corner(M.R, j)) {
                         // phase 1: the corner is created here if it is new
-                        okey[o] = key_corner(M.v[corner_of(M.R, j)]);
+                        okey[o] = key_corner(M.vc(corner_of(M.R, j)));
                         oa[o] = (uint32_t)a;
                         oj[o] = (uint16_t)j;
                         fve[e++] = o++;

@@ -168,7 +168,22 @@ void extract_mi(DevMem& mem, const Grid& g, const Values& val, RunOut& out, Time
     DBuf<uint32_t> cnt(mem, 3 * NA + 3), cv(mem, NA + 1), cf(mem, NA + 1), cfv(mem, NA + 1);
     DBuf<uint64_t> emask(mem, NA + 1);
     if (NA) {
-        hipLaunchKernelGGL(k_mi_bface, dim3(nblk(NA * kBfaceLanes)), dim3(256), 0, s, A, emask.p);
+        // every face's keys once, then the boundary-face pairing from them
+        DBuf<uint32_t> fb(mem, NA + 1);
+        hipLaunchKernelGGL(k_mi_nf, dim3(nblk(NA)), dim3(256), 0, s, A, fb.p);
+        scan_u32(mem, fb.p, fb.p, NA, fb.p + NA, s);
+        const uint64_t NFT = read1(fb.p + NA, s);
+        DBuf<uint16_t> fl(mem, NFT + 1), fw(mem, NFT + 1);
+        DBuf<uint64_t> ft(mem, 3 * NFT + 3);
+        MIFaceKeys K;
+        K.fb = fb.p;
+        K.fl = fl.p;
+        K.fw = fw.p;
+        K.ft = ft.p;
+        hipLaunchKernelGGL(k_mi_face_keys, dim3(nblk(NA * kBfaceLanes)), dim3(256), 0, s, A, K);
+        hipLaunchKernelGGL(k_mi_bface, dim3(nblk(NA * kBfaceLanes)), dim3(256), 0, s, A, K, emask.p);
+        RIN_HIP(hipGetLastError());
+        RIN_HIP(hipStreamSynchronize(s));  // the face-key buffers return to the pool
         hipLaunchKernelGGL(k_mi_count, dim3(nblk(NA)), dim3(256), 0, s, A, (const uint64_t*)emask.p, cnt.p);
     }
     hipLaunchKernelGGL(k_split3, dim3(nblk(NA)), dim3(256), 0, s, Count(NA), (const uint32_t*)cnt.p, cv.p, cf.p,
