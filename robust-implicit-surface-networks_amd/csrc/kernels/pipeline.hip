@@ -493,10 +493,16 @@ void run_pipeline(DevMem& mem, const Grid& g, const Values& val, uint32_t flags,
         if (L.empty()) continue;
         DBuf<uint32_t> dl(mem, L.size());
         RIN_HIP(hipMemcpyAsync(dl.p, L.data(), 4 * L.size(), hipMemcpyHostToDevice, s));
-        const unsigned blocks = (unsigned)((L.size() + 63) / 64);
+        // lanes per workgroup (one wavefront): a lane runs a whole cut, whose
+        // loops diverge between lanes, so a short list runs faster with few
+        // lanes per wavefront as long as its wavefronts all fit on the GPU
+        // at once (about 2048 of these)
+        unsigned lanes = 2;
+        while (lanes < 64 && L.size() > (size_t)lanes * 2048) lanes *= 2;
+        const unsigned blocks = (unsigned)((L.size() + lanes - 1) / lanes);
         const int kmin = q == 0 ? 1 : 5, kmax = q == 0 ? 4 : 8;
         auto go = [&](auto kernel) {
-            hipLaunchKernelGGL(kernel, dim3(blocks), dim3(64), 0, s, g.view, (const double*)val.vals.p,
+            hipLaunchKernelGGL(kernel, dim3(blocks), dim3(lanes), 0, s, g.view, (const double*)val.vals.p,
                                (const uint32_t*)out.act.p, (const uint8_t*)out.act_k.p,
                                (const uint32_t*)out.csr_off.p, (const uint8_t*)out.fit.p, (uint64_t)L.size(), kmin,
                                kmax, flags, (const uint32_t*)out.rec_off.p, out.rec.p, cnt.p, errs.p,
