@@ -378,6 +378,7 @@ struct FillArgs {
     uint32_t *csr_off_w, *rank_w;  // written by the cube form
     uint8_t* fit;
     uint32_t *act, *rec_loc, *cnt;
+    uint32_t* act_off;  // CSR offset by active rank
     uint8_t* act_k;
     const K1Entry* k1;
     uint32_t* list[3];      // buckets k=2, 3..4, 5..8
@@ -421,6 +422,7 @@ __global__ void k_fill_batched(FillArgs A) {
             } else {
                 A.act[r] = (uint32_t)t;
                 A.act_k[r] = (uint8_t)k;
+                A.act_off[r] = o;
                 for (uint64_t mm = m; mm; mm &= mm - 1) A.fit[o++] = (uint8_t)(__ffsll((unsigned long long)mm) - 1);
                 if (k == 1) {
                     const uint32_t f = __ffsll((unsigned long long)m) - 1;
@@ -707,6 +709,7 @@ __global__ void __launch_bounds__(kCubeTile) k_cube_fill(FillArgs A, const TileS
             } else {
                 A.act[r] = (uint32_t)t;
                 A.act_k[r] = (uint8_t)k;
+                A.act_off[r] = o;
                 uint32_t oo = o;
                 for (uint64_t mm = mq; mm; mm &= mm - 1) A.fit[oo++] = (uint8_t)(__ffsll((unsigned long long)mm) - 1);
                 if (k == 1) {
@@ -933,6 +936,7 @@ __global__ void k_k2_lookup(K2Args A) {
 struct ExtractArgs {
     GridView g;
     const uint32_t *act, *csr_off, *rec_loc;
+    const uint32_t* act_off;   // CSR offset by active rank
     const uint8_t* fit;
     const uint16_t* rec;
     const K1Entry* k1;
@@ -992,7 +996,7 @@ __global__ void k_extract_k1(ExtractArgs A) {
         uint32_t v[4];
         dev::tet_corners(A.g, t, v);
         const K1Entry& E = tab[loc & 0xFF];
-        const uint32_t f = A.fit[A.csr_off[t]];
+        const uint32_t f = A.fit[A.act_off[a]];
         uint32_t o = A.voff[a];
         for (int i = 0; i < E.niv; ++i) {
             const int vt = E.vt[i];
@@ -1181,7 +1185,7 @@ __global__ void __launch_bounds__(64) k_extract_records_wave(ExtractArgs A) {
     const uint32_t t = A.act[a];
     uint32_t v[4];
     dev::tet_corners(A.g, t, v);
-    const uint32_t s0 = A.csr_off[t];
+    const uint32_t s0 = A.act_off[a];
     const uint32_t vb = A.voff[a];
     for (int j = lane; j < nv; j += 64) {
         if (!((isov[j >> 5] >> (j & 31)) & 1u)) continue;
@@ -1298,6 +1302,7 @@ struct Workspace {
     DBuf<uint64_t> pos, neg;
     DBuf<uint32_t> kcnt, isact, csr_off, rank, act, rec_loc, cnt, list[3];
     DBuf<uint64_t> lts[3];  // per bucket slot: CSR offset << 32 | tet
+    DBuf<uint32_t> act_off;  // CSR offset of each active tet, by active rank
     DBuf<TileSums> tile_sums;  // regular grids: per tile of cubes
     DBuf<uint8_t> fit, act_k;
     DBuf<uint16_t> rec;
@@ -1464,6 +1469,7 @@ void alloc_ws(Workspace& W, const Grid& g, const Caps_& c) {
     W.rank.alloc(m, T + 1);
     W.act.alloc(m, c.active + 1);
     W.act_k.alloc(m, c.active + 1);
+    W.act_off.alloc(m, c.active + 1);
     W.rec_loc.alloc(m, c.active + 1);
     W.cnt.alloc(m, 3 * (uint64_t)c.active + 3);
     W.fit.alloc(m, c.fit + 1);
@@ -1644,6 +1650,7 @@ void stage_fill(Workspace& W, const Grid& g, const Values& val, hipStream_t s) {
     A.rank = W.rank.p;
     A.fit = W.fit.p;
     A.act = W.act.p;
+    A.act_off = W.act_off.p;
     A.rec_loc = W.rec_loc.p;
     A.cnt = W.cnt.p;
     A.act_k = W.act_k.p;
@@ -1703,6 +1710,7 @@ void stage_rest(Workspace& W, const Grid& g, const Values& val, hipStream_t s, i
         ExtractArgs E;
         E.g = g.view;
         E.act = W.act.p;
+        E.act_off = W.act_off.p;
         E.csr_off = W.csr_off.p;
         E.rec_loc = W.rec_loc.p;
         E.fit = W.fit.p;
@@ -1798,7 +1806,7 @@ void stage_rest(Workspace& W, const Grid& g, const Values& val, hipStream_t s, i
     }
     if (part & REST_XYZ) {
         hipLaunchKernelGGL(k_vert_fields_ia, dim3(cgrid(c.vo)), dim3(256), 0, s, g.view, (const double*)val.vals.p,
-                           (const uint32_t*)W.act.p, (const uint32_t*)W.csr_off.p, (const uint8_t*)W.fit.p,
+                           (const uint32_t*)W.act.p, (const uint32_t*)W.act_off.p, true, (const uint8_t*)W.fit.p,
                            (const uint32_t*)W.rec_loc.p, (const uint16_t*)W.rec.p, (const uint16_t*)W.k1_words.p,
                            (const uint32_t*)W.k1_off.p, (const uint32_t*)W.first_occ.p,
                            (const uint32_t*)W.oa.p, (const uint16_t*)W.oj.p, NV, W.v_tet.p, W.v_j.p, W.v_ss.p, W.v_sv.p,

@@ -867,8 +867,9 @@ __global__ void k_dedup_assign(Count n, const uint32_t* __restrict__ mo, const u
 }
 
 // vertex records and coordinates (src/extract_mesh.cpp:95-228, 1446-1538)
+// off: CSR offsets by tet (per_rank false) or by active rank (per_rank true)
 __global__ void k_vert_fields_ia(GridView g, const double* __restrict__ vals, const uint32_t* __restrict__ act,
-                                 const uint32_t* __restrict__ csr_off, const uint8_t* __restrict__ fit,
+                                 const uint32_t* __restrict__ off, bool per_rank, const uint8_t* __restrict__ fit,
                                  const uint32_t* __restrict__ rec_loc, const uint16_t* __restrict__ rec,
                                  const uint16_t* __restrict__ k1_words, const uint32_t* __restrict__ k1_off,
                                  const uint32_t* __restrict__ first_occ, const uint32_t* __restrict__ oa,
@@ -887,7 +888,7 @@ __global__ void k_vert_fields_ia(GridView g, const double* __restrict__ vals, co
         const Rec R((loc & 0x80000000u) ? k1_words + k1_off[loc & 0x7FFFFFFFu] : rec + loc);
         uint32_t v[4];
         dev::tet_corners(g, t, v);
-        const uint32_t s0 = csr_off[t];
+        const uint32_t s0 = per_rank ? off[a] : off[t];
         const uint16_t* T3 = R.vert(j);
         int bnd[3], nb = 0;
         uint32_t fn[3] = {0xFF, 0xFF, 0xFF};

@@ -634,7 +634,7 @@ void run_pipeline(DevMem& mem, const Grid& g, const Values& val, uint32_t flags,
     out.v_fn.alloc(mem, 4 * NV + 4);
     out.xyz.alloc(mem, 3 * NV + 3);
     hipLaunchKernelGGL(k_vert_fields_ia, dim3(nblk(NV)), dim3(256), 0, s, g.view, (const double*)val.vals.p,
-                       (const uint32_t*)out.act.p, (const uint32_t*)out.csr_off.p, (const uint8_t*)out.fit.p,
+                       (const uint32_t*)out.act.p, (const uint32_t*)out.csr_off.p, false, (const uint8_t*)out.fit.p,
                        (const uint32_t*)out.rec_loc.p, (const uint16_t*)out.rec.p, (const uint16_t*)nullptr,
                        (const uint32_t*)nullptr, (const uint32_t*)first_occ.p,
                        (const uint32_t*)oa.p, (const uint16_t*)oj.p, NV, out.v_tet.p, out.v_j.p, out.v_ss.p,
