@@ -229,6 +229,60 @@ __host__ __device__ inline void mi_counts(const Complex<C>& X, uint32_t& niv, ui
     for (int w = 0; w < W; ++w) niv += (uint32_t)__builtin_popcountll(iso[w]);
 }
 
+// Plain form: the active tets' cut lists by active-function count, built on
+// the device.  List q (0: MI two-material table, 1: up to 4 functions,
+// 2: 5..8, 3: more) gets its entries by wave-aggregated atomic appends, so
+// the order within a list is arbitrary (every cut writes its own tet's slot).
+// out: list sizes [0..4), the reference's bucket histogram [4..7) (IA: 1, 2,
+// >= 3 functions; MI: 2, 3, >= 4), tets with 3..4 and 5..8 functions [7],
+// [8], the largest count [9].  out starts zero.
+__global__ void __launch_bounds__(256) k_bucket_lists(uint64_t n, const uint8_t* __restrict__ act_k, int mi,
+                                                      int table2, uint32_t* __restrict__ l0, uint32_t* __restrict__ l1,
+                                                      uint32_t* __restrict__ l2, uint32_t* __restrict__ l3,
+                                                      uint32_t* __restrict__ out) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+    uint32_t* lists[4] = {l0, l1, l2, l3};
+    uint32_t h[3] = {0, 0, 0}, n34 = 0, n58 = 0, kmax = 0;
+    for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < n; base += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t a = base + threadIdx.x;
+        int b = -1;
+        if (a < n) {
+            const int k = act_k[a];
+            const int lo = mi ? 2 : 1;
+            h[0] += k <= lo;
+            h[1] += k == lo + 1;
+            h[2] += k > lo + 1;
+            n34 += k >= 3 && k <= 4;
+            n58 += k >= 5 && k <= 8;
+            kmax = max(kmax, (uint32_t)k);
+            b = k > 8 ? 3 : (k > 4 ? 2 : ((table2 && k == 2) ? 0 : 1));
+        }
+        for (int q = 0; q < 4; ++q) {
+            const uint64_t mask = __ballot(b == q);
+            if (!mask) continue;
+            const int leader = __ffsll((unsigned long long)mask) - 1;
+            uint32_t start = 0;
+            if (lane == leader) start = atomicAdd(&out[q], (uint32_t)__popcll(mask));
+            start = __shfl(start, leader, 64);
+            if (b == q) lists[q][start + (uint32_t)__popcll(mask & lt)] = (uint32_t)a;
+        }
+    }
+    for (int d = 32; d > 0; d >>= 1) {
+        for (int q = 0; q < 3; ++q) h[q] += __shfl_xor(h[q], d, 64);
+        n34 += __shfl_xor(n34, d, 64);
+        n58 += __shfl_xor(n58, d, 64);
+        kmax = max(kmax, (uint32_t)__shfl_xor(kmax, d, 64));
+    }
+    if (lane == 0) {
+        for (int q = 0; q < 3; ++q)
+            if (h[q]) atomicAdd(&out[4 + q], h[q]);
+        if (n34) atomicAdd(&out[7], n34);
+        if (n58) atomicAdd(&out[8], n58);
+        atomicMax(&out[9], kmax);
+    }
+}
+
 // One active tet's cut into its record slot.
 template <class C, bool MI>
 __device__ inline void cut_tet(const GridView& g, const double* __restrict__ vals, const uint32_t* __restrict__ act,

@@ -431,31 +431,31 @@ void run_pipeline(DevMem& mem, const Grid& g, const Values& val, uint32_t flags,
     isact.reset();
     vflag.reset();
 
-    // bucket counts (reference labels: IA 1/2/>=3, MI 2/3/>=4)
-    std::vector<uint32_t> huge, big, med;  // active ranks with > 8, 5..8 and 1..4 functions
+    // bucket counts (reference labels: IA 1/2/>=3, MI 2/3/>=4) and the cut
+    // lists, built on the device
     // MI with lookups on: two-material tets take their record from the table
     const bool mi_table = MI && !(flags & (RUN_REVERSE | RUN_NO_LOOKUP));
-    std::vector<uint32_t> two;
-    {
-        std::vector<uint8_t> hk(NA);
-        if (NA) RIN_HIP(hipMemcpyAsync(hk.data(), out.act_k.p, NA, hipMemcpyDeviceToHost, s));
+    DBuf<uint32_t> blists(mem, 4 * (NA + 1));
+    uint32_t* blist[4];
+    for (int q = 0; q < 4; ++q) blist[q] = blists.p + (uint64_t)q * (NA + 1);
+    uint32_t bcnt[4] = {0, 0, 0, 0};  // list sizes: two-material table, up to 4, 5..8, more
+    if (NA) {
+        DBuf<uint32_t> bo(mem, 16);
+        RIN_HIP(hipMemsetAsync(bo.p, 0, 16 * sizeof(uint32_t), s));
+        hipLaunchKernelGGL(k_bucket_lists, dim3(std::min(nblk(NA), 2048u)), dim3(256), 0, s, NA,
+                           (const uint8_t*)out.act_k.p, (int)MI, (int)mi_table, blist[0], blist[1], blist[2], blist[3],
+                           bo.p);
+        RIN_HIP(hipGetLastError());
+        uint32_t hb[16];
+        RIN_HIP(hipMemcpyAsync(hb, bo.p, sizeof(hb), hipMemcpyDeviceToHost, s));
         RIN_HIP(hipStreamSynchronize(s));
-        int kmax = 0;
-        for (uint64_t a = 0; a < NA; ++a) {
-            const int k = hk[a];
-            const int lo = MI ? 2 : 1;
-            out.nb[k <= lo ? 0 : (k == lo + 1 ? 1 : 2)]++;
-            kmax = std::max<int>(kmax, k);
-            if (k > 8) huge.push_back((uint32_t)a);
-            else if (k > 4) big.push_back((uint32_t)a);
-            else if (mi_table && k == 2) two.push_back((uint32_t)a);
-            else med.push_back((uint32_t)a);
-            out.n_cut34 += k >= 3 && k <= 4;
-            out.n_cut58 += k >= 5 && k <= 8;
-        }
-        if (kmax > kMaxHuge) {
+        for (int q = 0; q < 4; ++q) bcnt[q] = hb[q];
+        for (int q = 0; q < 3; ++q) out.nb[q] = hb[4 + q];
+        out.n_cut34 = hb[7];
+        out.n_cut58 = hb[8];
+        if ((int)hb[9] > kMaxHuge) {
             out.err |= ERR_CAPACITY;
-            throw std::runtime_error("a tet has " + std::to_string(kmax) +
+            throw std::runtime_error("a tet has " + std::to_string(hb[9]) +
                                      " active functions; the device cut supports at most " +
                                      std::to_string(kMaxHuge));
         }
@@ -472,68 +472,61 @@ void run_pipeline(DevMem& mem, const Grid& g, const Values& val, uint32_t flags,
     const uint64_t nwords = read1(out.rec_off.p + NA, s);
     out.rec.alloc(mem, nwords + 1);
     DBuf<uint32_t> cnt(mem, 3 * NA + 3);
-    if (!two.empty()) {
+    if (bcnt[0]) {
         const MI2Table& TT = mi2_table();
         DBuf<MI2Entry> dtab(mem, 81);
         DBuf<uint16_t> dw(mem, TT.words.size());
-        DBuf<uint32_t> dl(mem, two.size());
         RIN_HIP(hipMemcpyAsync(dtab.p, TT.e, sizeof(TT.e), hipMemcpyHostToDevice, s));
         RIN_HIP(hipMemcpyAsync(dw.p, TT.words.data(), 2 * TT.words.size(), hipMemcpyHostToDevice, s));
-        RIN_HIP(hipMemcpyAsync(dl.p, two.data(), 4 * two.size(), hipMemcpyHostToDevice, s));
-        hipLaunchKernelGGL(k_mi_table2, dim3(nblk(two.size())), dim3(256), 0, s, g.view, (const double*)val.vals.p,
+        hipLaunchKernelGGL(k_mi_table2, dim3(nblk(bcnt[0])), dim3(256), 0, s, g.view, (const double*)val.vals.p,
                            (const uint32_t*)out.act.p, (const uint32_t*)out.csr_off.p, (const uint8_t*)out.fit.p,
-                           (const uint32_t*)dl.p, (uint64_t)two.size(), flags, (const uint32_t*)out.rec_off.p,
+                           (const uint32_t*)blist[0], (uint64_t)bcnt[0], flags, (const uint32_t*)out.rec_off.p,
                            out.rec.p, cnt.p, errs.p, (const MI2Entry*)dtab.p, (const uint16_t*)dw.p);
         RIN_HIP(hipGetLastError());
-        RIN_HIP(hipStreamSynchronize(s));  // the table and list buffers return to the pool
+        RIN_HIP(hipStreamSynchronize(s));  // the table buffers return to the pool
     }
     // each thread-per-tet cut kernel runs over its own k range only
     for (int q = 0; q < 2 && NA; ++q) {
-        const std::vector<uint32_t>& L = q == 0 ? med : big;
-        if (L.empty()) continue;
-        DBuf<uint32_t> dl(mem, L.size());
-        RIN_HIP(hipMemcpyAsync(dl.p, L.data(), 4 * L.size(), hipMemcpyHostToDevice, s));
+        const uint64_t nl = bcnt[1 + q];
+        if (!nl) continue;
         // lanes per workgroup (one wavefront): a lane runs a whole cut, whose
         // loops diverge between lanes, so a short list runs faster with few
         // lanes per wavefront as long as its wavefronts all fit on the GPU
         // at once (about 2048 of these)
         unsigned lanes = 2;
-        while (lanes < 64 && L.size() > (size_t)lanes * 2048) lanes *= 2;
-        const unsigned blocks = (unsigned)((L.size() + lanes - 1) / lanes);
+        while (lanes < 64 && nl > (uint64_t)lanes * 2048) lanes *= 2;
+        const unsigned blocks = (unsigned)((nl + lanes - 1) / lanes);
         const int kmin = q == 0 ? 1 : 5, kmax = q == 0 ? 4 : 8;
         auto go = [&](auto kernel) {
             hipLaunchKernelGGL(kernel, dim3(blocks), dim3(lanes), 0, s, g.view, (const double*)val.vals.p,
                                (const uint32_t*)out.act.p, (const uint8_t*)out.act_k.p,
-                               (const uint32_t*)out.csr_off.p, (const uint8_t*)out.fit.p, (uint64_t)L.size(), kmin,
-                               kmax, flags, (const uint32_t*)out.rec_off.p, out.rec.p, cnt.p, errs.p,
-                               (const uint32_t*)dl.p);
+                               (const uint32_t*)out.csr_off.p, (const uint8_t*)out.fit.p, nl, kmin, kmax, flags,
+                               (const uint32_t*)out.rec_off.p, out.rec.p, cnt.p, errs.p,
+                               (const uint32_t*)blist[1 + q]);
         };
         if (q == 0)
             MI ? go(k_cut<CMed, true>) : go(k_cut<CMed, false>);
         else
             MI ? go(k_cut<CBig, true>) : go(k_cut<CBig, false>);
         RIN_HIP(hipGetLastError());
-        RIN_HIP(hipStreamSynchronize(s));  // the list buffer returns to the pool
     }
-    // rare tets: one lane each, complexes in device memory
-    auto cut_large = [&](const std::vector<uint32_t>& list, const uint32_t* loc, uint32_t fixed_cap) {
-        DBuf<uint32_t> hl(mem, list.size());
-        RIN_HIP(hipMemcpyAsync(hl.p, list.data(), 4 * list.size(), hipMemcpyHostToDevice, s));
+    // rare tets: one lane each, complexes in device memory; list: device
+    auto cut_large = [&](const uint32_t* list, uint64_t nl, const uint32_t* loc, uint32_t fixed_cap) {
         // ~200 KB of complex per lane: a few blocks
-        const unsigned blocks = (unsigned)std::min<size_t>((list.size() + 63) / 64, 4);
+        const unsigned blocks = (unsigned)std::min<uint64_t>((nl + 63) / 64, 4);
         DBuf<uint8_t> ws(mem, sizeof(Complex<CHuge>) * 64 * blocks);
         auto* wsp = reinterpret_cast<Complex<CHuge>*>(ws.p);
         auto go = [&](auto kernel) {
             hipLaunchKernelGGL(kernel, dim3(blocks), dim3(64), 0, s, g.view, (const double*)val.vals.p,
                                (const uint32_t*)out.act.p, (const uint8_t*)out.act_k.p, (const uint32_t*)out.csr_off.p,
-                               (const uint8_t*)out.fit.p, (const uint32_t*)hl.p, (uint64_t)list.size(), flags,
-                               (const uint32_t*)out.rec_off.p, out.rec.p, cnt.p, errs.p, wsp, loc, fixed_cap);
+                               (const uint8_t*)out.fit.p, list, nl, flags, (const uint32_t*)out.rec_off.p, out.rec.p,
+                               cnt.p, errs.p, wsp, loc, fixed_cap);
         };
         MI ? go(k_cut_huge<true>) : go(k_cut_huge<false>);
         RIN_HIP(hipGetLastError());
-        RIN_HIP(hipStreamSynchronize(s));
+        RIN_HIP(hipStreamSynchronize(s));  // the workspace returns to the pool
     };
-    if (!huge.empty()) cut_large(huge, nullptr, 0);
+    if (bcnt[3]) cut_large(blist[3], bcnt[3], nullptr, 0);
     // record location of every active tet: its slot, or (after a retry) a slot
     // appended for a large complex
     out.rec_loc.alloc(mem, NA + 1);
@@ -567,7 +560,9 @@ void run_pipeline(DevMem& mem, const Grid& g, const Values& val, uint32_t flags,
         RIN_HIP(hipStreamSynchronize(s));
         out.rec = std::move(rec2);
         fill_u32(errs.p, 0, 64, s);
-        cut_large(retry, out.rec_loc.p, slot);
+        DBuf<uint32_t> rl(mem, retry.size());
+        RIN_HIP(hipMemcpyAsync(rl.p, retry.data(), 4 * retry.size(), hipMemcpyHostToDevice, s));
+        cut_large(rl.p, retry.size(), out.rec_loc.p, slot);
         e = read_errs();
     }
     tm.mark(MI ? "MI cut" : "simp_arr");
