@@ -211,6 +211,42 @@ def test_batched_matches_oracle_and_replays(ctx, which, form, monkeypatch):
     g.release()
 
 
+@pytest.mark.parametrize("seed", [101, 102, 103, 104, 105, 106])
+def test_random_sets_all_forms(ctx, seed):
+    """Seeded random function sets (2..14 mixed functions, resolutions 9..27,
+    some with exactly repeated functions): the batched and plain implicit
+    arrangements and the material interface match the oracle bit for bit."""
+    import rin
+
+    rng = np.random.default_rng(seed)
+    n = [int(x) for x in rng.integers(0, 5, 4)]
+    n[0] += 1
+    fs = W.mixed(rng, *n)
+    if seed % 2:
+        # a repeated function: coincident planes and groups in the cuts
+        t0, p0 = fs.arrays()
+        fs.types.append(int(t0[0]))
+        fs.params.append(list(p0[0]))
+    res = int(rng.integers(9, 28))
+    types, params = fs.arrays()
+    g = ctx.grid_regular(res)
+    vals = g.eval(types, params)
+    pts, tets = S.grid(res)
+    hv = S.evaluate(types, params, pts)
+    ref = S.OracleRun(0, pts, tets, hv)
+    for flags in (rin.FLAG_BATCHED, 0):
+        dev = vals.run(flags)
+        _compare(dev, ref)
+        dev.release()
+    if len(types) <= 12:
+        ref_mi = S.OracleRun(1, pts, tets, hv)
+        dev = vals.run(rin.FLAG_MI)
+        _compare(dev, ref_mi)
+        dev.release()
+    vals.release()
+    g.release()
+
+
 def test_batched_explicit_mesh(ctx):
     """The batched form on an unstructured mesh (no star tables: every
     occurrence takes the hash path, ids by a global scan), across replays."""
