@@ -1845,7 +1845,7 @@ void stage_rest(Workspace& W, const Grid& g, const Values& val, hipStream_t s, i
         FA.f_func = W.f_func.p;
         const unsigned fgrid = (unsigned)((c.fo + kFaceTile - 1) / kFaceTile);
         hipLaunchKernelGGL(k_face_first_tiled, dim3(fgrid), dim3(256), 0, s, FA);
-        TileTotals tt = {{ctr + C_FACES, ctr + C_TETFACES, ctr + C_FACEVERTS}};
+        TileTotals tt = {{ctr + C_FACES, ctr + C_TETFACES, ctr + C_FACEVERTS, nullptr}};
         hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, s, NFO, kFaceTile, W.ftiles.p, 4, 3, tt);
         hipLaunchKernelGGL(k_face_emit_tiled, dim3(fgrid), dim3(256), 0, s, FA);
         hipLaunchKernelGGL(k_write_totals, dim3(1), dim3(64), 0, s, ctr, g.view.T, W.csr_off.p,

@@ -229,60 +229,6 @@ __host__ __device__ inline void mi_counts(const Complex<C>& X, uint32_t& niv, ui
     for (int w = 0; w < W; ++w) niv += (uint32_t)__builtin_popcountll(iso[w]);
 }
 
-// Plain form: the active tets' cut lists by active-function count, built on
-// the device.  List q (0: MI two-material table, 1: up to 4 functions,
-// 2: 5..8, 3: more) gets its entries by wave-aggregated atomic appends, so
-// the order within a list is arbitrary (every cut writes its own tet's slot).
-// out: list sizes [0..4), the reference's bucket histogram [4..7) (IA: 1, 2,
-// >= 3 functions; MI: 2, 3, >= 4), tets with 3..4 and 5..8 functions [7],
-// [8], the largest count [9].  out starts zero.
-__global__ void __launch_bounds__(256) k_bucket_lists(uint64_t n, const uint8_t* __restrict__ act_k, int mi,
-                                                      int table2, uint32_t* __restrict__ l0, uint32_t* __restrict__ l1,
-                                                      uint32_t* __restrict__ l2, uint32_t* __restrict__ l3,
-                                                      uint32_t* __restrict__ out) {
-    const int lane = threadIdx.x & 63;
-    const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
-    uint32_t* lists[4] = {l0, l1, l2, l3};
-    uint32_t h[3] = {0, 0, 0}, n34 = 0, n58 = 0, kmax = 0;
-    for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < n; base += (uint64_t)gridDim.x * blockDim.x) {
-        const uint64_t a = base + threadIdx.x;
-        int b = -1;
-        if (a < n) {
-            const int k = act_k[a];
-            const int lo = mi ? 2 : 1;
-            h[0] += k <= lo;
-            h[1] += k == lo + 1;
-            h[2] += k > lo + 1;
-            n34 += k >= 3 && k <= 4;
-            n58 += k >= 5 && k <= 8;
-            kmax = max(kmax, (uint32_t)k);
-            b = k > 8 ? 3 : (k > 4 ? 2 : ((table2 && k == 2) ? 0 : 1));
-        }
-        for (int q = 0; q < 4; ++q) {
-            const uint64_t mask = __ballot(b == q);
-            if (!mask) continue;
-            const int leader = __ffsll((unsigned long long)mask) - 1;
-            uint32_t start = 0;
-            if (lane == leader) start = atomicAdd(&out[q], (uint32_t)__popcll(mask));
-            start = __shfl(start, leader, 64);
-            if (b == q) lists[q][start + (uint32_t)__popcll(mask & lt)] = (uint32_t)a;
-        }
-    }
-    for (int d = 32; d > 0; d >>= 1) {
-        for (int q = 0; q < 3; ++q) h[q] += __shfl_xor(h[q], d, 64);
-        n34 += __shfl_xor(n34, d, 64);
-        n58 += __shfl_xor(n58, d, 64);
-        kmax = max(kmax, (uint32_t)__shfl_xor(kmax, d, 64));
-    }
-    if (lane == 0) {
-        for (int q = 0; q < 3; ++q)
-            if (h[q]) atomicAdd(&out[4 + q], h[q]);
-        if (n34) atomicAdd(&out[7], n34);
-        if (n58) atomicAdd(&out[8], n58);
-        atomicMax(&out[9], kmax);
-    }
-}
-
 // One active tet's cut into its record slot.
 template <class C, bool MI>
 __device__ inline void cut_tet(const GridView& g, const double* __restrict__ vals, const uint32_t* __restrict__ act,
@@ -748,9 +694,9 @@ __global__ void __launch_bounds__(256) k_dedup_star(Count n, StarArgs A) {
 
 // Exclusive prefixes of per-tile counts in place, one workgroup; the tiles
 // are those of `tile` elements over n, field f of tile i at x[i * stride + f]
-// (f < nfield <= 3); *total[f] = the field's sum.
+// (f < nfield <= 4); *total[f] = the field's sum.
 struct TileTotals {
-    uint32_t* p[3];
+    uint32_t* p[4];
 };
 
 // one workgroup of 1024 lanes; the counts are read with device-scope loads,
@@ -821,8 +767,139 @@ __global__ void __launch_bounds__(1024) k_dedup_fallback_scan(Count n_fail, cons
         if (m == o) atomicAdd(&tile_first[o / kDedupTile], 1u);
     }
     __syncthreads();
-    const TileTotals tt = {{total, nullptr, nullptr}};
+    const TileTotals tt = {{total, nullptr, nullptr, nullptr}};
     scan_tiles_block((uint32_t)((count_of(n) + kDedupTile - 1) / kDedupTile), tile_first, 1, 1, tt);
+}
+
+// Plain form: the active tets' cut lists by active-function count, built on
+// the device in active-rank order.  List 0: MI two-material table, 1: up to
+// 4 functions, 2: 5..8, 3: more.  Per chunk of kBucketChunk active tets,
+// kBucketWords words: list sizes [0..4), the reference's bucket histogram
+// [4..7) (IA: 1, 2, >= 3 functions; MI: 2, 3, >= 4), tets with 3..4 and
+// 5..8 functions [7], [8], the largest count [9] (k_bucket_count).  One
+// workgroup turns the list sizes into each chunk's list offsets and writes
+// the totals (k_bucket_scan); each chunk then writes its entries
+// (k_bucket_fill).  No atomics.
+constexpr uint32_t kBucketChunk = 2048;
+constexpr int kBucketWords = 10;
+
+__device__ inline int bucket_of_k(int k, int table2) {
+    return k > 8 ? 3 : (k > 4 ? 2 : ((table2 && k == 2) ? 0 : 1));
+}
+
+__global__ void __launch_bounds__(256) k_bucket_count(uint64_t n, const uint8_t* __restrict__ act_k, int mi,
+                                                      int table2, uint32_t* __restrict__ bc) {
+    const uint64_t base = (uint64_t)blockIdx.x * kBucketChunk;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lo = mi ? 2 : 1;
+    uint32_t w[kBucketWords] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (uint32_t i = threadIdx.x; i < kBucketChunk; i += 256) {
+        const uint64_t a = base + i;
+        if (a >= n) break;
+        const int k = act_k[a];
+        const int b = bucket_of_k(k, table2);
+        w[0] += b == 0;
+        w[1] += b == 1;
+        w[2] += b == 2;
+        w[3] += b == 3;
+        w[4] += k <= lo;
+        w[5] += k == lo + 1;
+        w[6] += k > lo + 1;
+        w[7] += k >= 3 && k <= 4;
+        w[8] += k >= 5 && k <= 8;
+        w[9] = max(w[9], (uint32_t)k);
+    }
+    __shared__ uint32_t red[256 / 64][kBucketWords];
+#pragma unroll
+    for (int f = 0; f < kBucketWords; ++f) {
+        uint32_t x = w[f];
+        for (int d = 32; d > 0; d >>= 1) {
+            const uint32_t y = __shfl_xor(x, d, 64);
+            x = f == 9 ? max(x, y) : x + y;
+        }
+        if (lane == 0) red[wave][f] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x < kBucketWords) {
+        const int f = threadIdx.x;
+        uint32_t x = 0;
+        for (int v = 0; v < 256 / 64; ++v) x = f == 9 ? max(x, red[v][f]) : x + red[v][f];
+        bc[(uint64_t)blockIdx.x * kBucketWords + f] = x;
+    }
+}
+
+// one workgroup of 1024 lanes; tot: the 10 totals
+__global__ void __launch_bounds__(1024) k_bucket_scan(uint32_t nchunks, uint32_t* __restrict__ bc,
+                                                      uint32_t* __restrict__ tot) {
+    // statistics first (the scan overwrites the list sizes only)
+    __shared__ uint32_t red[16][kBucketWords];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t w[kBucketWords - 4] = {0, 0, 0, 0, 0, 0};
+    for (uint32_t c = threadIdx.x; c < nchunks; c += 1024)
+        for (int f = 0; f < kBucketWords - 4; ++f) {
+            const uint32_t x = bc[(uint64_t)c * kBucketWords + 4 + f];
+            w[f] = f == 5 ? max(w[f], x) : w[f] + x;
+        }
+#pragma unroll
+    for (int f = 0; f < kBucketWords - 4; ++f) {
+        uint32_t x = w[f];
+        for (int d = 32; d > 0; d >>= 1) {
+            const uint32_t y = __shfl_xor(x, d, 64);
+            x = f == 5 ? max(x, y) : x + y;
+        }
+        if (lane == 0) red[wave][f] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x < kBucketWords - 4) {
+        const int f = threadIdx.x;
+        uint32_t x = 0;
+        for (int v = 0; v < 16; ++v) x = f == 5 ? max(x, red[v][f]) : x + red[v][f];
+        tot[4 + f] = x;
+    }
+    __syncthreads();
+    const TileTotals tt = {{tot, tot + 1, tot + 2, tot + 3}};
+    scan_tiles_block(nchunks, bc, kBucketWords, 4, tt);
+}
+
+__global__ void __launch_bounds__(256) k_bucket_fill(uint64_t n, const uint8_t* __restrict__ act_k, int table2,
+                                                     const uint32_t* __restrict__ bc, uint32_t* __restrict__ l0,
+                                                     uint32_t* __restrict__ l1, uint32_t* __restrict__ l2,
+                                                     uint32_t* __restrict__ l3) {
+    const uint64_t base = (uint64_t)blockIdx.x * kBucketChunk;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+    __shared__ uint32_t wc[256 / 64][4];
+    uint32_t c0 = bc[(uint64_t)blockIdx.x * kBucketWords], c1 = bc[(uint64_t)blockIdx.x * kBucketWords + 1],
+             c2 = bc[(uint64_t)blockIdx.x * kBucketWords + 2], c3 = bc[(uint64_t)blockIdx.x * kBucketWords + 3];
+    for (uint32_t i0 = 0; i0 < kBucketChunk && base + i0 < n; i0 += 256) {
+        const uint64_t a = base + i0 + threadIdx.x;
+        const int b = a < n ? bucket_of_k(act_k[a], table2) : -1;
+        const uint64_t m0 = __ballot(b == 0), m1 = __ballot(b == 1), m2 = __ballot(b == 2), m3 = __ballot(b == 3);
+        if (lane == 0) {
+            wc[wave][0] = (uint32_t)__popcll(m0);
+            wc[wave][1] = (uint32_t)__popcll(m1);
+            wc[wave][2] = (uint32_t)__popcll(m2);
+            wc[wave][3] = (uint32_t)__popcll(m3);
+        }
+        __syncthreads();
+        uint32_t p[4] = {c0, c1, c2, c3};
+        uint32_t t[4] = {0, 0, 0, 0};
+        for (int v = 0; v < 256 / 64; ++v)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                p[q] += v < wave ? wc[v][q] : 0u;
+                t[q] += wc[v][q];
+            }
+        if (b == 0) l0[p[0] + (uint32_t)__popcll(m0 & lt)] = (uint32_t)a;
+        if (b == 1) l1[p[1] + (uint32_t)__popcll(m1 & lt)] = (uint32_t)a;
+        if (b == 2) l2[p[2] + (uint32_t)__popcll(m2 & lt)] = (uint32_t)a;
+        if (b == 3) l3[p[3] + (uint32_t)__popcll(m3 & lt)] = (uint32_t)a;
+        c0 += t[0];
+        c1 += t[1];
+        c2 += t[2];
+        c3 += t[3];
+        __syncthreads();  // before wc is reused
+    }
 }
 
 // Star path: each first occurrence's vertex id (its rank among first

@@ -440,11 +440,13 @@ void run_pipeline(DevMem& mem, const Grid& g, const Values& val, uint32_t flags,
     for (int q = 0; q < 4; ++q) blist[q] = blists.p + (uint64_t)q * (NA + 1);
     uint32_t bcnt[4] = {0, 0, 0, 0};  // list sizes: two-material table, up to 4, 5..8, more
     if (NA) {
-        DBuf<uint32_t> bo(mem, 16);
-        RIN_HIP(hipMemsetAsync(bo.p, 0, 16 * sizeof(uint32_t), s));
-        hipLaunchKernelGGL(k_bucket_lists, dim3(std::min(nblk(NA), 2048u)), dim3(256), 0, s, NA,
-                           (const uint8_t*)out.act_k.p, (int)MI, (int)mi_table, blist[0], blist[1], blist[2], blist[3],
-                           bo.p);
+        const uint32_t nch = (uint32_t)((NA + kBucketChunk - 1) / kBucketChunk);
+        DBuf<uint32_t> bc(mem, (uint64_t)kBucketWords * nch + kBucketWords), bo(mem, 16);
+        hipLaunchKernelGGL(k_bucket_count, dim3(nch), dim3(256), 0, s, NA, (const uint8_t*)out.act_k.p, (int)MI,
+                           (int)mi_table, bc.p);
+        hipLaunchKernelGGL(k_bucket_scan, dim3(1), dim3(1024), 0, s, nch, bc.p, bo.p);
+        hipLaunchKernelGGL(k_bucket_fill, dim3(nch), dim3(256), 0, s, NA, (const uint8_t*)out.act_k.p, (int)mi_table,
+                           (const uint32_t*)bc.p, blist[0], blist[1], blist[2], blist[3]);
         RIN_HIP(hipGetLastError());
         uint32_t hb[16];
         RIN_HIP(hipMemcpyAsync(hb, bo.p, sizeof(hb), hipMemcpyDeviceToHost, s));
