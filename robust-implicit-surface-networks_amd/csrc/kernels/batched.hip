@@ -983,13 +983,14 @@ __device__ inline bool extract_guard(const ExtractArgs& A) {
 
 constexpr uint8_t kLocalEdge[6][2] = {{0, 1}, {0, 2}, {0, 3}, {1, 2}, {1, 3}, {2, 3}};
 
-__global__ void k_extract_k1(ExtractArgs A) {
-    __shared__ K1Entry tab[81];
+// The extraction passes as roles of a workgroup (blk: its index among the
+// nblk workgroups of the role), so that one launch can run all three.
+__device__ inline void extract_k1_role(const ExtractArgs& A, K1Entry* tab, uint32_t blk, uint32_t nblk) {
     for (int i = threadIdx.x; i < 81; i += blockDim.x) tab[i] = A.k1[i];
     __syncthreads();
     if (!extract_guard(A)) return;
     const uint64_t NA = min((uint64_t)*A.n_active, (uint64_t)A.cap_active);
-    GRID_STRIDE(a, NA) {
+    for (uint64_t a = (uint64_t)blk * blockDim.x + threadIdx.x; a < NA; a += (uint64_t)nblk * blockDim.x) {
         const uint32_t loc = A.rec_loc[a];
         if (!(loc & K1_FLAG) || (loc & ~K1_FLAG) >= 81) continue;
         const uint32_t t = A.act[a];
@@ -1018,12 +1019,17 @@ __global__ void k_extract_k1(ExtractArgs A) {
     }
 }
 
+__global__ void k_extract_k1(ExtractArgs A) {
+    __shared__ K1Entry tab[81];
+    extract_k1_role(A, tab, blockIdx.x, gridDim.x);
+}
+
 // k = 2 records (mostly sign-vector table records): one lane per tet
-__global__ void k_extract_records(ExtractArgs A) {
+__device__ inline void extract_rec_role(const ExtractArgs& A, uint32_t blk, uint32_t nblk) {
     if (!extract_guard(A)) return;
     const uint32_t n0 = min(A.ctr[C_Q2], A.list_cap[0]);
     uint32_t recw = 0;  // record words read (throughput accounting)
-    GRID_STRIDE(i, n0) {
+    for (uint64_t i = (uint64_t)blk * blockDim.x + threadIdx.x; i < n0; i += (uint64_t)nblk * blockDim.x) {
         const uint32_t a = A.list[0][i];
         const uint64_t ts = A.lts[0][i];
         const uint32_t loc = A.rec_loc[a];
@@ -1121,8 +1127,10 @@ __global__ void k_extract_records(ExtractArgs A) {
     }
     for (int d = 32; d > 0; d >>= 1) recw += __shfl_xor(recw, d, 64);
     if ((threadIdx.x & 63) == 0 && recw)
-        atomicAdd(A.ctr + spread_at(SP_RECW, blockIdx.x * 4 + (threadIdx.x >> 6)), recw);
+        atomicAdd(A.ctr + spread_at(SP_RECW, blk * 4 + (threadIdx.x >> 6)), recw);
 }
+
+__global__ void k_extract_records(ExtractArgs A) { extract_rec_role(A, blockIdx.x, gridDim.x); }
 
 // k = 3..8 records: one wave per tet, lanes over faces and vertices.  The
 // wave cut leaves each record's face offsets after it, so lanes read their
@@ -1130,13 +1138,19 @@ __global__ void k_extract_records(ExtractArgs A) {
 // from one lane's walk of the face list.  LDS holds only the offsets and
 // the iso flags, so many waves stay resident to cover the loads.
 // Occurrences and face rows come out in the order one lane per tet gives.
-__global__ void __launch_bounds__(64) k_extract_records_wave(ExtractArgs A) {
-    __shared__ uint16_t fo[CBig::MF];
-    __shared__ uint32_t isov[(CBig::MV + 31) / 32];
-    __shared__ uint8_t isof[CBig::MF];
+struct WaveExtractLds {
+    uint16_t fo[CBig::MF];
+    uint32_t isov[(CBig::MV + 31) / 32];
+    uint8_t isof[CBig::MF];
+};
+
+__device__ inline void extract_wave_role(const ExtractArgs& A, WaveExtractLds& S, uint32_t blk) {
+    uint16_t* fo = S.fo;
+    uint32_t* isov = S.isov;
+    uint8_t* isof = S.isof;
     if (!extract_guard(A)) return;
     const uint32_t n2 = min(A.ctr[C_Q5], A.list_cap[2]), n1 = min(A.ctr[C_Q3], A.list_cap[1]);
-    const uint32_t i = blockIdx.x;
+    const uint32_t i = blk;
     if (i >= n1 + n2) return;
     // the larger complexes first
     const int bq = i < n2 ? 2 : 1;
@@ -1152,7 +1166,7 @@ __global__ void __launch_bounds__(64) k_extract_records_wave(ExtractArgs A) {
         return;
     }
     for (int x = lane; x < (nv + 31) / 32; x += 64) isov[x] = 0;
-    if (lane == 0) atomicAdd(A.ctr + spread_at(SP_RECW, blockIdx.x), len);
+    if (lane == 0) atomicAdd(A.ctr + spread_at(SP_RECW, blk), len);
     const bool dup = W[5] & 1;
     if (!(loc & K1_FLAG) && A.fo_table[bq] && len + nf <= A.slot_words[bq]) {
         for (int f = lane; f < nf; f += 64) fo[f] = W[len + f];
@@ -1232,6 +1246,26 @@ __global__ void __launch_bounds__(64) k_extract_records_wave(ExtractArgs A) {
         q += (uint32_t)__popcll(m);
         e += __shfl(inc, 63, 64);
     }
+}
+
+__global__ void __launch_bounds__(64) k_extract_records_wave(ExtractArgs A) {
+    __shared__ WaveExtractLds S;
+    extract_wave_role(A, S, blockIdx.x);
+}
+
+// all three extraction passes in one launch of 64-lane workgroups: the
+// lane-per-tet k = 2 records first, the wave-per-tet records of 3..8
+// functions next, the k = 1 tets last
+__global__ void __launch_bounds__(64) k_extract_all(ExtractArgs A, uint32_t nb_rec, uint32_t nb_wave, uint32_t nb_k1) {
+    __shared__ WaveExtractLds S;
+    __shared__ K1Entry tab[81];
+    const uint32_t b = blockIdx.x;
+    if (b < nb_rec)
+        extract_rec_role(A, b, nb_rec);
+    else if (b < nb_rec + nb_wave)
+        extract_wave_role(A, S, b - nb_rec);
+    else
+        extract_k1_role(A, tab, b - nb_rec - nb_wave, nb_k1);
 }
 
 // the host-visible summary of a run: the counters, then the union of the
@@ -1748,10 +1782,18 @@ void stage_rest(Workspace& W, const Grid& g, const Values& val, hipStream_t s, i
         E.plan_off = W.k2plan_off.p;
         E.plan = W.k2plan.p;
         E.n_plans = (uint32_t)k2_plans().off.size();
-        if (part & REST_EXTRACT_K1) hipLaunchKernelGGL(k_extract_k1, dim3(cgrid(c.active)), dim3(256), 0, s, E);
-        if (part & REST_EXTRACT_REC) {
-            hipLaunchKernelGGL(k_extract_records_wave, dim3(std::max(1u, c.b[1] + c.b[2])), dim3(64), 0, s, E);
-            hipLaunchKernelGGL(k_extract_records, dim3(cgrid(c.b[0])), dim3(256), 0, side ? side : s, E);
+        if ((part & REST_EXTRACT_K1) && (part & REST_EXTRACT_REC) && !side) {
+            const uint32_t nb_rec = std::min<uint32_t>((c.b[0] + 63) / 64, 8192u) + 1;
+            const uint32_t nb_wave = std::max(1u, c.b[1] + c.b[2]);
+            const uint32_t nb_k1 = std::min<uint32_t>((c.active + 63) / 64, 8192u) + 1;
+            hipLaunchKernelGGL(k_extract_all, dim3(nb_rec + nb_wave + nb_k1), dim3(64), 0, s, E, nb_rec, nb_wave,
+                               nb_k1);
+        } else {
+            if (part & REST_EXTRACT_K1) hipLaunchKernelGGL(k_extract_k1, dim3(cgrid(c.active)), dim3(256), 0, s, E);
+            if (part & REST_EXTRACT_REC) {
+                hipLaunchKernelGGL(k_extract_records_wave, dim3(std::max(1u, c.b[1] + c.b[2])), dim3(64), 0, s, E);
+                hipLaunchKernelGGL(k_extract_records, dim3(cgrid(c.b[0])), dim3(256), 0, side ? side : s, E);
+            }
         }
     }
 
@@ -2062,11 +2104,9 @@ void run_batched(DevMem& ctx_mem, const Grid& g, Values& val, uint32_t flags, Ru
             // extraction: the three passes write disjoint rows, so the k = 2
             // record pass runs on the branch beside the other two
             stage_rest(w, g, val, s, REST_SCANS);
-            RIN_HIP(hipEventRecord(w.fork, s));
-            RIN_HIP(hipStreamWaitEvent(w.side, w.fork, 0));
-            stage_rest(w, g, val, s, REST_EXTRACT_K1 | REST_EXTRACT_REC, w.side);
-            RIN_HIP(hipEventRecord(w.join, w.side));
-            RIN_HIP(hipStreamWaitEvent(s, w.join, 0));
+            // extraction: the three passes write disjoint rows; one launch
+            // runs them side by side
+            stage_rest(w, g, val, s, REST_EXTRACT_K1 | REST_EXTRACT_REC);
             for (int part : {REST_DEDUP, REST_XYZ, REST_FACES}) stage_rest(w, g, val, s, part);
             RIN_HIP(hipStreamEndCapture(s, &w.whole_graph));
             RIN_HIP(hipGraphInstantiate(&w.whole, w.whole_graph, nullptr, nullptr, 0));
