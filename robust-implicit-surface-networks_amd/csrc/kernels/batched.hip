@@ -1846,14 +1846,31 @@ void stage_rest(Workspace& W, const Grid& g, const Values& val, hipStream_t s, i
                                list, (const uint32_t*)W.hslot.p, W.hkey.p, W.hmx.p, true);
         }
     }
-    if (part & REST_XYZ) {
-        hipLaunchKernelGGL(k_vert_fields_ia, dim3(cgrid(c.vo)), dim3(256), 0, s, g.view, (const double*)val.vals.p,
-                           (const uint32_t*)W.act.p, (const uint32_t*)W.act_off.p, true, (const uint8_t*)W.fit.p,
-                           (const uint32_t*)W.rec_loc.p, (const uint16_t*)W.rec.p, (const uint16_t*)W.k1_words.p,
-                           (const uint32_t*)W.k1_off.p, (const uint32_t*)W.first_occ.p,
-                           (const uint32_t*)W.oa.p, (const uint16_t*)W.oj.p, NV, W.v_tet.p, W.v_j.p, W.v_ss.p, W.v_sv.p,
-                           W.v_fn.p, W.xyz.p);
-    }
+    VertFieldArgs VF;
+    VF.g = g.view;
+    VF.vals = (const double*)val.vals.p;
+    VF.act = W.act.p;
+    VF.off = W.act_off.p;
+    VF.per_rank = true;
+    VF.fit = W.fit.p;
+    VF.rec_loc = W.rec_loc.p;
+    VF.rec = W.rec.p;
+    VF.k1_words = W.k1_words.p;
+    VF.k1_off = W.k1_off.p;
+    VF.first_occ = W.first_occ.p;
+    VF.oa = W.oa.p;
+    VF.oj = W.oj.p;
+    VF.nverts = NV;
+    VF.v_tet = W.v_tet.p;
+    VF.v_j = W.v_j.p;
+    VF.v_ss = W.v_ss.p;
+    VF.v_sv = W.v_sv.p;
+    VF.v_fn = W.v_fn.p;
+    VF.xyz = W.xyz.p;
+    // coordinates and faces together: one launch for both first passes
+    const bool xyz_with_faces = (part & REST_XYZ) && (part & REST_FACES);
+    if ((part & REST_XYZ) && !xyz_with_faces)
+        hipLaunchKernelGGL(k_vert_fields_ia, dim3(cgrid(c.vo)), dim3(256), 0, s, VF);
     if (part & REST_FACES) {
         // faces: a boundary face may repeat in the tet across its tet face
         FaceTileArgs FA;
@@ -1886,7 +1903,11 @@ void stage_rest(Workspace& W, const Grid& g, const Values& val, hipStream_t s, i
         FA.f_tf_local = W.f_tf_local.p;
         FA.f_func = W.f_func.p;
         const unsigned fgrid = (unsigned)((c.fo + kFaceTile - 1) / kFaceTile);
-        hipLaunchKernelGGL(k_face_first_tiled, dim3(fgrid), dim3(256), 0, s, FA);
+        if (xyz_with_faces)
+            hipLaunchKernelGGL(k_faces_and_xyz, dim3(fgrid + cgrid(c.vo)), dim3(256), 0, s, FA, VF, fgrid,
+                               cgrid(c.vo));
+        else
+            hipLaunchKernelGGL(k_face_first_tiled, dim3(fgrid), dim3(256), 0, s, FA);
         TileTotals tt = {{ctr + C_FACES, ctr + C_TETFACES, ctr + C_FACEVERTS, nullptr}};
         hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, s, NFO, kFaceTile, W.ftiles.p, 4, 3, tt);
         hipLaunchKernelGGL(k_face_emit_tiled, dim3(fgrid), dim3(256), 0, s, FA);
@@ -2107,7 +2128,8 @@ void run_batched(DevMem& ctx_mem, const Grid& g, Values& val, uint32_t flags, Ru
             // extraction: the three passes write disjoint rows; one launch
             // runs them side by side
             stage_rest(w, g, val, s, REST_EXTRACT_K1 | REST_EXTRACT_REC);
-            for (int part : {REST_DEDUP, REST_XYZ, REST_FACES}) stage_rest(w, g, val, s, part);
+            stage_rest(w, g, val, s, REST_DEDUP);
+            stage_rest(w, g, val, s, REST_XYZ | REST_FACES);
             RIN_HIP(hipStreamEndCapture(s, &w.whole_graph));
             RIN_HIP(hipGraphInstantiate(&w.whole, w.whole_graph, nullptr, nullptr, 0));
             while (w.ev.size() < w.parts.size() + 1) {

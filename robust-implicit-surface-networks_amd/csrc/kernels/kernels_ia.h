@@ -999,16 +999,52 @@ __global__ void k_dedup_assign(Count n, const uint32_t* __restrict__ mo, const u
 
 // vertex records and coordinates (src/extract_mesh.cpp:95-228, 1446-1538)
 // off: CSR offsets by tet (per_rank false) or by active rank (per_rank true)
-__global__ void k_vert_fields_ia(GridView g, const double* __restrict__ vals, const uint32_t* __restrict__ act,
-                                 const uint32_t* __restrict__ off, bool per_rank, const uint8_t* __restrict__ fit,
-                                 const uint32_t* __restrict__ rec_loc, const uint16_t* __restrict__ rec,
-                                 const uint16_t* __restrict__ k1_words, const uint32_t* __restrict__ k1_off,
-                                 const uint32_t* __restrict__ first_occ, const uint32_t* __restrict__ oa,
-                                 const uint16_t* __restrict__ oj, Count nverts, uint32_t* __restrict__ v_tet,
-                                 uint16_t* __restrict__ v_j, uint8_t* __restrict__ v_ss, uint32_t* __restrict__ v_sv,
-                                 uint8_t* __restrict__ v_fn, double* __restrict__ xyz) {
+struct VertFieldArgs {
+    GridView g;
+    const double* vals;
+    const uint32_t* act;
+    const uint32_t* off;  // CSR offsets by tet (per_rank false) or by active rank (per_rank true)
+    bool per_rank;
+    const uint8_t* fit;
+    const uint32_t* rec_loc;
+    const uint16_t* rec;
+    const uint16_t *k1_words;
+    const uint32_t* k1_off;
+    const uint32_t *first_occ, *oa;
+    const uint16_t* oj;
+    Count nverts;
+    uint32_t* v_tet;
+    uint16_t* v_j;
+    uint8_t* v_ss;
+    uint32_t* v_sv;
+    uint8_t* v_fn;
+    double* xyz;
+};
+
+// as a workgroup role: blk of nblk workgroups
+__device__ inline void vert_fields_ia_role(const VertFieldArgs& P, uint32_t blk, uint32_t nblk) {
+    const GridView& g = P.g;
+    const double* __restrict__ vals = P.vals;
+    const uint32_t* __restrict__ act = P.act;
+    const uint32_t* __restrict__ off = P.off;
+    const bool per_rank = P.per_rank;
+    const uint8_t* __restrict__ fit = P.fit;
+    const uint32_t* __restrict__ rec_loc = P.rec_loc;
+    const uint16_t* __restrict__ rec = P.rec;
+    const uint16_t* __restrict__ k1_words = P.k1_words;
+    const uint32_t* __restrict__ k1_off = P.k1_off;
+    const uint32_t* __restrict__ first_occ = P.first_occ;
+    const uint32_t* __restrict__ oa = P.oa;
+    const uint16_t* __restrict__ oj = P.oj;
+    uint32_t* __restrict__ v_tet = P.v_tet;
+    uint16_t* __restrict__ v_j = P.v_j;
+    uint8_t* __restrict__ v_ss = P.v_ss;
+    uint32_t* __restrict__ v_sv = P.v_sv;
+    uint8_t* __restrict__ v_fn = P.v_fn;
+    double* __restrict__ xyz = P.xyz;
     const uint64_t V = g.V;
-    GRID_STRIDE(i, nverts) {
+    const uint64_t nv = P.nverts.get();
+    for (uint64_t i = (uint64_t)blk * blockDim.x + threadIdx.x; i < nv; i += (uint64_t)nblk * blockDim.x) {
         const uint32_t o = first_occ[i];
         const uint32_t a = oa[o];
         const int j = oj[o];
@@ -1118,6 +1154,8 @@ __global__ void k_vert_fields_ia(GridView g, const double* __restrict__ vals, co
         xyz[3 * i + 2] = X2;
     }
 }
+
+__global__ void k_vert_fields_ia(VertFieldArgs P) { vert_fields_ia_role(P, blockIdx.x, gridDim.x); }
 
 // ------------------------------------------------------------------ faces
 //
@@ -1293,9 +1331,9 @@ __device__ inline void face_triple_ids(const FaceTileArgs& A, uint32_t b, uint32
     k3[2] = mx;
 }
 
-__global__ void __launch_bounds__(256) k_face_first_tiled(FaceTileArgs A) {
+__device__ inline void face_first_role(const FaceTileArgs& A, uint32_t blk) {
     const uint32_t NQ = (uint32_t)A.nq.get();
-    const uint32_t base = blockIdx.x * kFaceTile;
+    const uint32_t base = blk * kFaceTile;
     if (base >= NQ) return;
     const uint32_t NA = (uint32_t)A.n_active.get();
     const uint32_t nfv = clamp_count(A.nfv_total, A.nfv_cap);
@@ -1363,8 +1401,20 @@ __global__ void __launch_bounds__(256) k_face_first_tiled(FaceTileArgs A) {
     __syncthreads();
     if (threadIdx.x < 3) {
         const int f = threadIdx.x;
-        A.tiles[4 * (uint64_t)blockIdx.x + f] = red[f][0] + red[f][1] + red[f][2] + red[f][3];
+        A.tiles[4 * (uint64_t)blk + f] = red[f][0] + red[f][1] + red[f][2] + red[f][3];
     }
+}
+
+__global__ void __launch_bounds__(256) k_face_first_tiled(FaceTileArgs A) { face_first_role(A, blockIdx.x); }
+
+// the first face pass and the vertex coordinates in one launch (both read
+// only the dedup results): face tiles first, then vertices
+__global__ void __launch_bounds__(256) k_faces_and_xyz(FaceTileArgs A, VertFieldArgs P, uint32_t nb_face,
+                                                       uint32_t nb_xyz) {
+    if (blockIdx.x < nb_face)
+        face_first_role(A, blockIdx.x);
+    else
+        vert_fields_ia_role(P, blockIdx.x - nb_face, nb_xyz);
 }
 
 // inclusive prefix over the lanes of a wave

@@ -630,13 +630,30 @@ void run_pipeline(DevMem& mem, const Grid& g, const Values& val, uint32_t flags,
     out.v_sv.alloc(mem, 4 * NV + 4);
     out.v_fn.alloc(mem, 4 * NV + 4);
     out.xyz.alloc(mem, 3 * NV + 3);
-    hipLaunchKernelGGL(k_vert_fields_ia, dim3(nblk(NV)), dim3(256), 0, s, g.view, (const double*)val.vals.p,
-                       (const uint32_t*)out.act.p, (const uint32_t*)out.csr_off.p, false, (const uint8_t*)out.fit.p,
-                       (const uint32_t*)out.rec_loc.p, (const uint16_t*)out.rec.p, (const uint16_t*)nullptr,
-                       (const uint32_t*)nullptr, (const uint32_t*)first_occ.p,
-                       (const uint32_t*)oa.p, (const uint16_t*)oj.p, NV, out.v_tet.p, out.v_j.p, out.v_ss.p,
-                       out.v_sv.p, out.v_fn.p, out.xyz.p);
-    RIN_HIP(hipGetLastError());
+    {
+        VertFieldArgs VF;
+        VF.g = g.view;
+        VF.vals = (const double*)val.vals.p;
+        VF.act = out.act.p;
+        VF.off = out.csr_off.p;
+        VF.per_rank = false;
+        VF.fit = out.fit.p;
+        VF.rec_loc = out.rec_loc.p;
+        VF.rec = out.rec.p;
+        VF.k1_words = nullptr;
+        VF.k1_off = nullptr;
+        VF.first_occ = first_occ.p;
+        VF.oa = oa.p;
+        VF.oj = oj.p;
+        VF.nverts = Count(NV);
+        VF.v_tet = out.v_tet.p;
+        VF.v_j = out.v_j.p;
+        VF.v_ss = out.v_ss.p;
+        VF.v_sv = out.v_sv.p;
+        VF.v_fn = out.v_fn.p;
+        VF.xyz = out.xyz.p;
+        hipLaunchKernelGGL(k_vert_fields_ia, dim3(nblk(NV)), dim3(256), 0, s, VF);
+    }    RIN_HIP(hipGetLastError());
     tm.mark("compute xyz");
 
     // ---- faces: a boundary face may repeat in the tet across its tet face
