@@ -1727,8 +1727,7 @@ enum RestPart {
 // bounded, so that capacity-sized launches do not pay for idle workgroups
 unsigned cgrid(uint64_t cap) { return std::min(nblk(cap), 2048u); }
 
-// side: a second stream for the k = 2 record extraction (null: s)
-void stage_rest(Workspace& W, const Grid& g, const Values& val, hipStream_t s, int part, hipStream_t side = nullptr) {
+void stage_rest(Workspace& W, const Grid& g, const Values& val, hipStream_t s, int part) {
     DevMem& m = W.mem;
     const Caps_& c = W.cap;
     uint32_t* ctr = W.ctr.p;
@@ -1782,7 +1781,7 @@ void stage_rest(Workspace& W, const Grid& g, const Values& val, hipStream_t s, i
         E.plan_off = W.k2plan_off.p;
         E.plan = W.k2plan.p;
         E.n_plans = (uint32_t)k2_plans().off.size();
-        if ((part & REST_EXTRACT_K1) && (part & REST_EXTRACT_REC) && !side) {
+        if ((part & REST_EXTRACT_K1) && (part & REST_EXTRACT_REC)) {
             const uint32_t nb_rec = std::min<uint32_t>((c.b[0] + 63) / 64, 8192u) + 1;
             const uint32_t nb_wave = std::max(1u, c.b[1] + c.b[2]);
             const uint32_t nb_k1 = std::min<uint32_t>((c.active + 63) / 64, 8192u) + 1;
@@ -1792,7 +1791,7 @@ void stage_rest(Workspace& W, const Grid& g, const Values& val, hipStream_t s, i
             if (part & REST_EXTRACT_K1) hipLaunchKernelGGL(k_extract_k1, dim3(cgrid(c.active)), dim3(256), 0, s, E);
             if (part & REST_EXTRACT_REC) {
                 hipLaunchKernelGGL(k_extract_records_wave, dim3(std::max(1u, c.b[1] + c.b[2])), dim3(64), 0, s, E);
-                hipLaunchKernelGGL(k_extract_records, dim3(cgrid(c.b[0])), dim3(256), 0, side ? side : s, E);
+                hipLaunchKernelGGL(k_extract_records, dim3(cgrid(c.b[0])), dim3(256), 0, s, E);
             }
         }
     }
