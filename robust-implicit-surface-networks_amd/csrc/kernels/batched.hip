@@ -11,7 +11,7 @@
 //    buckets.
 //  * Every stage reads its element count from device memory and checks it
 //    against the workspace capacity, so the whole path is captured once as a
-//    hipGraph and replayed; the run's last kernel publishes the sizes and the
+//    hipGraph and replayed; the run's closing kernel publishes the sizes and the
 //    overflow flags into mapped host memory (on overflow the workspace grows
 //    and the graph is re-captured).
 #include <algorithm>
@@ -1284,29 +1284,45 @@ __global__ void k_publish(const uint32_t* ctr, const uint32_t* errs, uint32_t* p
     publish_counters(ctr, errs, pub);
 }
 
-// fold the spread counters, close the CSR and face offset arrays, publish
-// the counters and clear them (the last kernel of a run)
-__global__ void k_write_totals(uint32_t* ctr, uint64_t T, uint32_t* csr_off, uint32_t* f_vstart,
-                               uint32_t* f_tfstart, uint32_t cap_fo, uint32_t* errs, uint32_t* pub) {
-    const int fold_to[SP_N] = {C_RECW, C_CUTK3, C_CUTW3, C_CUTK5, C_CUTW5};
-    for (int c = 0; c < SP_N; ++c) {
-        uint32_t v = threadIdx.x < kSpread ? ctr[spread_at(c, threadIdx.x)] : 0u;
-        for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
-        if (threadIdx.x == 0) ctr[fold_to[c]] = v;
-    }
-    if (threadIdx.x == 0) {
-        csr_off[T] = ctr[C_FIT];
-        const uint32_t nf = ctr[C_ABORT] ? 0u : min(ctr[C_FACES], cap_fo);
-        f_vstart[nf] = ctr[C_FACEVERTS];
-        f_tfstart[nf] = ctr[C_ABORT] ? 0u : min(ctr[C_TETFACES], cap_fo);
+// The faces' tile scan and the run's closing work in one workgroup (the
+// emit pass then reads its sizes from `sizes`, not from the counters): scan
+// the per-tile face counts (totals into the counters), keep the emit pass's
+// sizes, fold the spread counters, close the CSR and face offset arrays,
+// publish the counters and clear them for the next run.
+__global__ void __launch_bounds__(1024) k_face_scan_close(uint32_t* ftiles, uint32_t* ctr, uint64_t T,
+                                                          uint32_t* csr_off, uint32_t* f_vstart, uint32_t* f_tfstart,
+                                                          uint32_t cap_fo, uint32_t cap_fv, uint32_t* errs,
+                                                          uint32_t* pub, uint32_t* sizes) {
+    const uint32_t nq = ctr[C_ABORT] ? 0u : min(ctr[C_FO], cap_fo);
+    const TileTotals tt = {{ctr + C_FACES, ctr + C_TETFACES, ctr + C_FACEVERTS, nullptr}};
+    scan_tiles_block((nq + kFaceTile - 1) / kFaceTile, ftiles, 4, 3, tt);
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        if (threadIdx.x == 0) {
+            sizes[0] = nq;
+            sizes[1] = min(ctr[C_FV], cap_fv);
+        }
+        const int fold_to[SP_N] = {C_RECW, C_CUTK3, C_CUTW3, C_CUTK5, C_CUTW5};
+        for (int c = 0; c < SP_N; ++c) {
+            uint32_t v = ctr[spread_at(c, threadIdx.x)];
+            for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
+            if (threadIdx.x == 0) ctr[fold_to[c]] = v;
+        }
+        if (threadIdx.x == 0) {
+            csr_off[T] = ctr[C_FIT];
+            const uint32_t nf = ctr[C_ABORT] ? 0u : min(ctr[C_FACES], cap_fo);
+            f_vstart[nf] = ctr[C_FACEVERTS];
+            f_tfstart[nf] = ctr[C_ABORT] ? 0u : min(ctr[C_TETFACES], cap_fo);
+        }
     }
     __syncthreads();
-    publish_counters(ctr, errs, pub);
-    // leave the counters clear for the next run
+    if (threadIdx.x < 64) publish_counters(ctr, errs, pub);
     __syncthreads();
-    if (threadIdx.x < C_N) ctr[threadIdx.x] = 0;
-    errs[threadIdx.x] = 0;
-    for (int c = 0; c < SP_N; ++c) ctr[spread_at(c, threadIdx.x)] = 0;
+    if (threadIdx.x < 64) {
+        if (threadIdx.x < C_N) ctr[threadIdx.x] = 0;
+        errs[threadIdx.x] = 0;
+        for (int c = 0; c < SP_N; ++c) ctr[spread_at(c, threadIdx.x)] = 0;
+    }
 }
 
 __global__ void k_zero_ctr(uint32_t* ctr, int n, uint32_t* errs) {
@@ -1358,6 +1374,7 @@ struct Workspace {
     bool hash_dirty = false;  // a run stopped early: clear the table before the next
     // faces: per occurrence flags and partner, corner ids, per-tile counts
     DBuf<uint32_t> ffirst, fmatch, fvm, ftiles;
+    DBuf<uint32_t> fsizes;  // face occurrences and corner entries, for the emit pass
     // outputs
     DBuf<uint32_t> v_tet, v_sv, f_vstart, f_verts, f_tfstart, f_tf_tet;
     DBuf<uint16_t> v_j, f_tf_local;
@@ -1383,7 +1400,7 @@ struct Workspace {
     // the k = 3..4 cut): a stream to capture it on and the fork / join events
     hipStream_t side = nullptr;
     hipEvent_t fork = nullptr, join = nullptr;
-    // pinned, device-mapped summary the last kernel of a run writes (counters
+    // pinned, device-mapped summary the closing kernel of a run writes (counters
     // and error flags), read by the host after the stream synchronises
     uint32_t* pub_h = nullptr;
     uint32_t* pub_d = nullptr;
@@ -1547,6 +1564,7 @@ void alloc_ws(Workspace& W, const Grid& g, const Caps_& c) {
     for (auto* b : {&W.ffirst, &W.fmatch}) b->alloc(m, c.fo + 1);
     W.fvm.alloc(m, c.fv + 1);
     W.ftiles.alloc(m, 4 * ((c.fo + kFaceTile - 1) / kFaceTile) + 4);
+    W.fsizes.alloc(m, 4);
     W.v_tet.alloc(m, c.vo + 1);
     W.v_j.alloc(m, c.vo + 1);
     W.v_ss.alloc(m, c.vo + 1);
@@ -1901,17 +1919,19 @@ void stage_rest(Workspace& W, const Grid& g, const Values& val, hipStream_t s, i
         FA.f_tf_tet = W.f_tf_tet.p;
         FA.f_tf_local = W.f_tf_local.p;
         FA.f_func = W.f_func.p;
+        FA.sizes = nullptr;
         const unsigned fgrid = (unsigned)((c.fo + kFaceTile - 1) / kFaceTile);
         if (xyz_with_faces)
             hipLaunchKernelGGL(k_faces_and_xyz, dim3(fgrid + cgrid(c.vo)), dim3(256), 0, s, FA, VF, fgrid,
                                cgrid(c.vo));
         else
             hipLaunchKernelGGL(k_face_first_tiled, dim3(fgrid), dim3(256), 0, s, FA);
-        TileTotals tt = {{ctr + C_FACES, ctr + C_TETFACES, ctr + C_FACEVERTS, nullptr}};
-        hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, s, NFO, kFaceTile, W.ftiles.p, 4, 3, tt);
+        // the tile scan closes the run (counters published and cleared); the
+        // emit pass reads its sizes from fsizes
+        hipLaunchKernelGGL(k_face_scan_close, dim3(1), dim3(1024), 0, s, W.ftiles.p, ctr, g.view.T, W.csr_off.p,
+                           W.f_vstart.p, W.f_tfstart.p, c.fo, c.fv, W.errs.p, W.pub_d, W.fsizes.p);
+        FA.sizes = W.fsizes.p;
         hipLaunchKernelGGL(k_face_emit_tiled, dim3(fgrid), dim3(256), 0, s, FA);
-        hipLaunchKernelGGL(k_write_totals, dim3(1), dim3(64), 0, s, ctr, g.view.T, W.csr_off.p,
-                           W.f_vstart.p, W.f_tfstart.p, c.fo, W.errs.p, W.pub_d);
     }
 }
 
@@ -1986,8 +2006,9 @@ void run_batched(DevMem& ctx_mem, const Grid& g, Values& val, uint32_t flags, Ru
     }
     uint32_t hctr[32] = {};
     uint32_t herr = 0;
-    // the run's last kernel has written the summary; after an eager pass that
-    // ends before it, `publish` launches the summary kernel itself
+    // the run's closing kernel (the faces' tile scan) has written the summary;
+    // after an eager pass that ends before it, `publish` launches the summary
+    // kernel itself
     auto read_back = [&](bool publish) {
         if (publish) hipLaunchKernelGGL(k_publish, dim3(1), dim3(64), 0, s, w.ctr.p, w.errs.p, w.pub_d);
         RIN_HIP(hipGetLastError());
@@ -2156,7 +2177,7 @@ void run_batched(DevMem& ctx_mem, const Grid& g, Values& val, uint32_t flags, Ru
             RIN_HIP(hipEventRecord(w.ev[1], s));
         }
         read_back(false);
-        w.ctr_clear = true;  // the run's last kernel cleared them
+        w.ctr_clear = true;  // the run's closing kernel cleared them
         const uint32_t e = errors();
         if (e) w.hash_dirty = true;
         if (e & ERR_OVERFLOW) {

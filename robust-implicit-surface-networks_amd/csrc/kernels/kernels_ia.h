@@ -1308,6 +1308,7 @@ struct FaceTileArgs {
     uint64_t nfv_cap, nfo_cap;
     Count nq, n_active;
     uint32_t *fvm, *flags, *partner, *tiles;  // tiles: 4 words per tile (faces, tet-face entries, corners)
+    const uint32_t* sizes;  // for the emit pass (or null): face occurrences and corner entries
     uint32_t *f_vstart, *f_verts, *f_tfstart, *f_tf_tet;
     uint16_t* f_tf_local;
     uint8_t* f_func;
@@ -1427,10 +1428,10 @@ __device__ inline uint32_t wave_inclusive(uint32_t x, int lane) {
 }
 
 __global__ void __launch_bounds__(256) k_face_emit_tiled(FaceTileArgs A) {
-    const uint32_t NQ = (uint32_t)A.nq.get();
+    const uint32_t NQ = A.sizes ? A.sizes[0] : (uint32_t)A.nq.get();
     const uint32_t base = blockIdx.x * kFaceTile;
     if (base >= NQ) return;
-    const uint32_t nfv = clamp_count(A.nfv_total, A.nfv_cap);
+    const uint32_t nfv = A.sizes ? A.sizes[1] : clamp_count(A.nfv_total, A.nfv_cap);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     __shared__ uint32_t ws[kFaceU][3][256 / 64];
     uint32_t fl[kFaceU], vb[kFaceU], ve[kFaceU], inc[kFaceU][3];
