@@ -147,6 +147,38 @@ __global__ void k_scan3_apply(Scan3 S, Count cn, const uint32_t* __restrict__ su
     }
 }
 
+// The same without the top-level scan: each workgroup adds up the tile sums
+// before its own (a few hundred tiles: at most a few loads per lane)
+__global__ void k_scan3_apply_direct(Scan3 S, Count cn, const uint32_t* __restrict__ sums, uint32_t nb) {
+    const size_t n = cn.get();
+    uint32_t pre[3] = {0, 0, 0};
+    for (uint32_t b = threadIdx.x; b < blockIdx.x; b += kThreads)
+        for (int f = 0; f < 3; ++f) pre[f] += sums[f * nb + b];
+    for (int f = 0; f < 3; ++f) {
+        uint32_t tot;
+        (void)block_excl_scan(pre[f], tot);
+        pre[f] = tot;
+    }
+    const size_t base = (size_t)blockIdx.x * kTile + (size_t)threadIdx.x * kItems;
+    const size_t last_blk = n ? (n - 1) / kTile : 0;
+    for (int f = 0; f < 3; ++f) {
+        uint32_t v[kItems], acc = 0;
+        for (int i = 0; i < kItems; ++i) {
+            const size_t idx = base + i;
+            v[i] = idx < n ? S.in[f][idx * S.stride] : 0u;
+            acc += v[i];
+        }
+        uint32_t tot;
+        uint32_t run = block_excl_scan(acc, tot) + pre[f];
+        for (int i = 0; i < kItems; ++i) {
+            const size_t idx = base + i;
+            if (idx < n) S.out[f][idx] = run;
+            run += v[i];
+        }
+        if (S.total[f] && blockIdx.x == last_blk && threadIdx.x == kThreads - 1) *S.total[f] = run;
+    }
+}
+
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t n) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
 }
@@ -301,6 +333,12 @@ void scan3_u32(DevMem& mem, const uint32_t* const in[3], int stride, uint32_t* c
     }
     DBuf<uint32_t> sums(mem, 3 * nb);
     hipLaunchKernelGGL(k_scan3_reduce, dim3((unsigned)nb), dim3(kThreads), 0, s, S, n, sums.p, (uint32_t)nb);
+    if (nb <= 4096) {
+        // few tiles: each applying workgroup adds up the sums before it
+        hipLaunchKernelGGL(k_scan3_apply_direct, dim3((unsigned)nb), dim3(kThreads), 0, s, S, n,
+                           (const uint32_t*)sums.p, (uint32_t)nb);
+        return;
+    }
     hipLaunchKernelGGL(k_scan3_top, dim3(3), dim3(kThreads), 0, s, sums.p, (uint32_t)nb);
     hipLaunchKernelGGL(k_scan3_apply, dim3((unsigned)nb), dim3(kThreads), 0, s, S, n, (const uint32_t*)sums.p,
                        (uint32_t)nb);
