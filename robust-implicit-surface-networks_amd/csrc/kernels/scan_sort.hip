@@ -56,9 +56,21 @@ __global__ void k_scan_reduce(const uint32_t* __restrict__ in, Count cn, uint32_
     if (threadIdx.x == 0) sums[blockIdx.x] = tot;
 }
 
+// offs: the tiles' scanned sums, or (direct) their raw sums, which each
+// workgroup then adds up itself for the tiles before it
 __global__ void k_scan_apply(const uint32_t* in, uint32_t* out, Count cn, const uint32_t* __restrict__ offs,
-                             uint32_t* total) {
+                             uint32_t* total, bool direct = false) {
     const size_t n = cn.get();
+    uint32_t pre = 0;
+    if (offs && direct) {
+        uint32_t x = 0;
+        for (uint32_t b = threadIdx.x; b < blockIdx.x; b += kThreads) x += offs[b];
+        uint32_t tot;
+        (void)block_excl_scan(x, tot);
+        pre = tot;
+    } else if (offs) {
+        pre = offs[blockIdx.x];
+    }
     const size_t base = (size_t)blockIdx.x * kTile + (size_t)threadIdx.x * kItems;
     uint32_t v[kItems];
     uint32_t acc = 0;
@@ -68,7 +80,7 @@ __global__ void k_scan_apply(const uint32_t* in, uint32_t* out, Count cn, const 
         acc += v[i];
     }
     uint32_t tot;
-    uint32_t run = block_excl_scan(acc, tot) + (offs ? offs[blockIdx.x] : 0u);
+    uint32_t run = block_excl_scan(acc, tot) + pre;
     for (int i = 0; i < kItems; ++i) {
         size_t idx = base + i;
         if (idx < n) out[idx] = run;
@@ -301,6 +313,12 @@ void scan_u32(DevMem& mem, const uint32_t* in, uint32_t* out, Count n, uint32_t*
     }
     DBuf<uint32_t> sums(mem, nb);
     hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb), dim3(kThreads), 0, s, in, n, sums.p);
+    if (nb <= 4096) {
+        // few tiles: each applying workgroup adds up the sums before it
+        hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nb), dim3(kThreads), 0, s, in, out, n,
+                           (const uint32_t*)sums.p, total, true);
+        return;
+    }
     if (nb <= (size_t)16 * kTile)  // one workgroup scans the tile sums
         hipLaunchKernelGGL(k_scan3_top, dim3(1), dim3(kThreads), 0, s, sums.p, (uint32_t)nb);
     else
