@@ -1284,35 +1284,49 @@ __global__ void k_publish(const uint32_t* ctr, const uint32_t* errs, uint32_t* p
     publish_counters(ctr, errs, pub);
 }
 
-// The faces' tile scan and the run's closing work in one workgroup (the
-// emit pass then reads its sizes from `sizes`, not from the counters): scan
-// the per-tile face counts (totals into the counters), keep the emit pass's
-// sizes, fold the spread counters, close the CSR and face offset arrays,
-// publish the counters and clear them for the next run.
-__global__ void __launch_bounds__(1024) k_face_scan_close(uint32_t* ftiles, uint32_t* ctr, uint64_t T,
-                                                          uint32_t* csr_off, uint32_t* f_vstart, uint32_t* f_tfstart,
-                                                          uint32_t cap_fo, uint32_t cap_fv, uint32_t* errs,
-                                                          uint32_t* pub, uint32_t* sizes) {
-    const uint32_t nq = ctr[C_ABORT] ? 0u : min(ctr[C_FO], cap_fo);
-    const TileTotals tt = {{ctr + C_FACES, ctr + C_TETFACES, ctr + C_FACEVERTS, nullptr}};
-    scan_tiles_block((nq + kFaceTile - 1) / kFaceTile, ftiles, 4, 3, tt);
+// The faces' emit pass and the run's closing work in one launch. Workgroups
+// 1.. emit (each adds up the raw tile counts before its tile; the sizes come
+// from the first pass, so no emitting workgroup reads a counter); workgroup 0
+// adds up all the tiles (totals into the counters), folds the spread
+// counters, closes the CSR and face offset arrays, publishes the counters
+// and clears them for the next run.
+__global__ void __launch_bounds__(256) k_face_emit_close(FaceTileArgs A, uint32_t* ctr, uint64_t T, uint32_t* csr_off,
+                                                         uint32_t* errs, uint32_t* pub) {
+    if (blockIdx.x > 0) {
+        face_emit_role(A, blockIdx.x - 1);
+        return;
+    }
+    __shared__ uint32_t part[3][4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t nq = A.sizes[0];
+    const uint32_t nt = (nq + kFaceTile - 1) / kFaceTile;
+    uint32_t x[3] = {0, 0, 0};
+    for (uint32_t b = threadIdx.x; b < nt; b += 256)
+        for (int f = 0; f < 3; ++f) x[f] += A.tiles[4 * (uint64_t)b + f];
+    for (int f = 0; f < 3; ++f) {
+        for (int d = 32; d > 0; d >>= 1) x[f] += __shfl_xor(x[f], d, 64);
+        if (lane == 0) part[f][wave] = x[f];
+    }
     __syncthreads();
     if (threadIdx.x < 64) {
         if (threadIdx.x == 0) {
-            sizes[0] = nq;
-            sizes[1] = min(ctr[C_FV], cap_fv);
+            const uint32_t tot[3] = {part[0][0] + part[0][1] + part[0][2] + part[0][3],
+                                     part[1][0] + part[1][1] + part[1][2] + part[1][3],
+                                     part[2][0] + part[2][1] + part[2][2] + part[2][3]};
+            ctr[C_FACES] = tot[0];
+            ctr[C_TETFACES] = tot[1];
+            ctr[C_FACEVERTS] = tot[2];
+            csr_off[T] = ctr[C_FIT];
+            // the totals are at most nq (faces, tet-face entries) and the
+            // corner capacity, as in the first pass
+            A.f_vstart[tot[0]] = tot[2];
+            A.f_tfstart[tot[0]] = tot[1];
         }
         const int fold_to[SP_N] = {C_RECW, C_CUTK3, C_CUTW3, C_CUTK5, C_CUTW5};
         for (int c = 0; c < SP_N; ++c) {
             uint32_t v = ctr[spread_at(c, threadIdx.x)];
             for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
             if (threadIdx.x == 0) ctr[fold_to[c]] = v;
-        }
-        if (threadIdx.x == 0) {
-            csr_off[T] = ctr[C_FIT];
-            const uint32_t nf = ctr[C_ABORT] ? 0u : min(ctr[C_FACES], cap_fo);
-            f_vstart[nf] = ctr[C_FACEVERTS];
-            f_tfstart[nf] = ctr[C_ABORT] ? 0u : min(ctr[C_TETFACES], cap_fo);
         }
     }
     __syncthreads();
@@ -1920,18 +1934,18 @@ void stage_rest(Workspace& W, const Grid& g, const Values& val, hipStream_t s, i
         FA.f_tf_local = W.f_tf_local.p;
         FA.f_func = W.f_func.p;
         FA.sizes = nullptr;
+        FA.sizes_out = W.fsizes.p;
         const unsigned fgrid = (unsigned)((c.fo + kFaceTile - 1) / kFaceTile);
         if (xyz_with_faces)
             hipLaunchKernelGGL(k_faces_and_xyz, dim3(fgrid + cgrid(c.vo)), dim3(256), 0, s, FA, VF, fgrid,
                                cgrid(c.vo));
         else
             hipLaunchKernelGGL(k_face_first_tiled, dim3(fgrid), dim3(256), 0, s, FA);
-        // the tile scan closes the run (counters published and cleared); the
-        // emit pass reads its sizes from fsizes
-        hipLaunchKernelGGL(k_face_scan_close, dim3(1), dim3(1024), 0, s, W.ftiles.p, ctr, g.view.T, W.csr_off.p,
-                           W.f_vstart.p, W.f_tfstart.p, c.fo, c.fv, W.errs.p, W.pub_d, W.fsizes.p);
+        // the emit launch closes the run (counters published and cleared); it
+        // reads its sizes from fsizes, kept by the first pass
         FA.sizes = W.fsizes.p;
-        hipLaunchKernelGGL(k_face_emit_tiled, dim3(fgrid), dim3(256), 0, s, FA);
+        hipLaunchKernelGGL(k_face_emit_close, dim3(fgrid + 1), dim3(256), 0, s, FA, ctr, g.view.T, W.csr_off.p,
+                           W.errs.p, W.pub_d);
     }
 }
 

@@ -1309,6 +1309,7 @@ struct FaceTileArgs {
     Count nq, n_active;
     uint32_t *fvm, *flags, *partner, *tiles;  // tiles: 4 words per tile (faces, tet-face entries, corners)
     const uint32_t* sizes;  // for the emit pass (or null): face occurrences and corner entries
+    uint32_t* sizes_out = nullptr;  // the first pass keeps those two sizes here (or not, if null)
     uint32_t *f_vstart, *f_verts, *f_tfstart, *f_tf_tet;
     uint16_t* f_tf_local;
     uint8_t* f_func;
@@ -1334,10 +1335,14 @@ __device__ inline void face_triple_ids(const FaceTileArgs& A, uint32_t b, uint32
 
 __device__ inline void face_first_role(const FaceTileArgs& A, uint32_t blk) {
     const uint32_t NQ = (uint32_t)A.nq.get();
+    const uint32_t nfv = clamp_count(A.nfv_total, A.nfv_cap);
+    if (A.sizes_out && blk == 0 && threadIdx.x == 0) {
+        A.sizes_out[0] = NQ;
+        A.sizes_out[1] = nfv;
+    }
     const uint32_t base = blk * kFaceTile;
     if (base >= NQ) return;
     const uint32_t NA = (uint32_t)A.n_active.get();
-    const uint32_t nfv = clamp_count(A.nfv_total, A.nfv_cap);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     uint32_t sf = 0, st = 0, sv = 0;
     for (int u = 0; u < kFaceU; ++u) {
@@ -1427,9 +1432,11 @@ __device__ inline uint32_t wave_inclusive(uint32_t x, int lane) {
     return x;
 }
 
-__global__ void __launch_bounds__(256) k_face_emit_tiled(FaceTileArgs A) {
+// A.tiles holds the raw per-tile counts (no scan): the workgroup adds up
+// those of the tiles before its own
+__device__ inline void face_emit_role(const FaceTileArgs& A, uint32_t blk) {
     const uint32_t NQ = A.sizes ? A.sizes[0] : (uint32_t)A.nq.get();
-    const uint32_t base = blockIdx.x * kFaceTile;
+    const uint32_t base = blk * kFaceTile;
     if (base >= NQ) return;
     const uint32_t nfv = A.sizes ? A.sizes[1] : clamp_count(A.nfv_total, A.nfv_cap);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -1454,8 +1461,19 @@ __global__ void __launch_bounds__(256) k_face_emit_tiled(FaceTileArgs A) {
         }
     }
     __syncthreads();
-    uint32_t carry[3] = {A.tiles[4 * (uint64_t)blockIdx.x], A.tiles[4 * (uint64_t)blockIdx.x + 1],
-                         A.tiles[4 * (uint64_t)blockIdx.x + 2]};
+    __shared__ uint32_t pred[3][256 / 64];
+    {
+        uint32_t x[3] = {0, 0, 0};
+        for (uint32_t b = threadIdx.x; b < blk; b += 256)
+            for (int f = 0; f < 3; ++f) x[f] += A.tiles[4 * (uint64_t)b + f];
+        for (int f = 0; f < 3; ++f) {
+            for (int d = 32; d > 0; d >>= 1) x[f] += __shfl_xor(x[f], d, 64);
+            if (lane == 0) pred[f][wave] = x[f];
+        }
+    }
+    __syncthreads();
+    uint32_t carry[3];
+    for (int f = 0; f < 3; ++f) carry[f] = pred[f][0] + pred[f][1] + pred[f][2] + pred[f][3];
 #pragma unroll
     for (int u = 0; u < kFaceU; ++u) {
         uint32_t pos[3];
