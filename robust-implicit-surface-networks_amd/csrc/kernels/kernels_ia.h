@@ -739,11 +739,6 @@ __device__ inline void scan_tiles_block(uint32_t nt, uint32_t* __restrict__ x, i
     }
 }
 
-__global__ void __launch_bounds__(1024) k_scan_tiles(Count n, uint32_t tile, uint32_t* __restrict__ x, int stride,
-                                                     int nfield, TileTotals total) {
-    scan_tiles_block((uint32_t)((count_of(n) + tile - 1) / tile), x, stride, nfield, total);
-}
-
 // Star path, after the star pass, in one workgroup: the hash fallback for
 // the occurrences the star tables left (only degenerate inputs leave any; a
 // long list is taken in passes), their first occurrences joining the tile
