@@ -476,7 +476,8 @@ __global__ void k_signs_count(const double* __restrict__ vals, uint64_t V, uint3
         for (uint32_t f0 = 0; f0 < F; f0 += 8) {
             double x[8];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) x[u] = f0 + u < F ? vals[(uint64_t)(f0 + u) * V + v] : 1.0;
+            for (int u = 0; u < 8; ++u)
+                x[u] = f0 + u < F ? __builtin_nontemporal_load(&vals[(uint64_t)(f0 + u) * V + v]) : 1.0;
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
                 p |= (uint64_t)(x[u] > 0) << (f0 + u);
