@@ -501,7 +501,7 @@ __global__ void k_signs_count(const double* __restrict__ vals, uint64_t V, uint3
 // places its tets in the CSR, the active list and the bucket lists from its
 // tile's prefixes, in tet order and without atomics.
 
-constexpr int kCubeTile = 256;  // cubes per tile (one workgroup)
+constexpr int kCubeTile = 512;  // cubes per tile (one workgroup)
 
 // pat (optional): per tet with one active function, its corner sign pattern
 // (sum of s_c 3^c, s_c = 0 negative, 1 zero, 2 positive), else 0
