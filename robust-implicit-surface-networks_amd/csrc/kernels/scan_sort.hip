@@ -14,6 +14,10 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kItems = 8;
 constexpr int kTile = kThreads * kItems;
+// the three-field scan: smaller tiles (its inputs are a few hundred
+// thousand per-tet counts: twice the workgroups keep more loads in flight)
+constexpr int kItems3 = 4;
+constexpr int kTile3 = kThreads * kItems3;
 constexpr int kWaves = kThreads / 64;
 
 __device__ inline uint32_t wave_incl_scan(uint32_t v) {
@@ -101,9 +105,9 @@ struct Scan3 {
 
 __global__ void k_scan3_reduce(Scan3 S, Count cn, uint32_t* __restrict__ sums, uint32_t nb) {
     const size_t n = cn.get();
-    const size_t base = (size_t)blockIdx.x * kTile;
+    const size_t base = (size_t)blockIdx.x * kTile3;
     uint32_t acc[3] = {0, 0, 0};
-    for (int i = 0; i < kItems; ++i) {
+    for (int i = 0; i < kItems3; ++i) {
         const size_t idx = base + (size_t)i * kThreads + threadIdx.x;
         if (idx < n)
             for (int f = 0; f < 3; ++f) acc[f] += S.in[f][idx * S.stride];
@@ -139,18 +143,18 @@ __global__ void k_scan3_top(uint32_t* __restrict__ sums, uint32_t nb) {
 
 __global__ void k_scan3_apply(Scan3 S, Count cn, const uint32_t* __restrict__ sums, uint32_t nb) {
     const size_t n = cn.get();
-    const size_t base = (size_t)blockIdx.x * kTile + (size_t)threadIdx.x * kItems;
-    const size_t last_blk = n ? (n - 1) / kTile : 0;
+    const size_t base = (size_t)blockIdx.x * kTile3 + (size_t)threadIdx.x * kItems3;
+    const size_t last_blk = n ? (n - 1) / kTile3 : 0;
     for (int f = 0; f < 3; ++f) {
-        uint32_t v[kItems], acc = 0;
-        for (int i = 0; i < kItems; ++i) {
+        uint32_t v[kItems3], acc = 0;
+        for (int i = 0; i < kItems3; ++i) {
             const size_t idx = base + i;
             v[i] = idx < n ? S.in[f][idx * S.stride] : 0u;
             acc += v[i];
         }
         uint32_t tot;
         uint32_t run = block_excl_scan(acc, tot) + sums[f * nb + blockIdx.x];
-        for (int i = 0; i < kItems; ++i) {
+        for (int i = 0; i < kItems3; ++i) {
             const size_t idx = base + i;
             if (idx < n) S.out[f][idx] = run;
             run += v[i];
@@ -171,18 +175,18 @@ __global__ void k_scan3_apply_direct(Scan3 S, Count cn, const uint32_t* __restri
         (void)block_excl_scan(pre[f], tot);
         pre[f] = tot;
     }
-    const size_t base = (size_t)blockIdx.x * kTile + (size_t)threadIdx.x * kItems;
-    const size_t last_blk = n ? (n - 1) / kTile : 0;
+    const size_t base = (size_t)blockIdx.x * kTile3 + (size_t)threadIdx.x * kItems3;
+    const size_t last_blk = n ? (n - 1) / kTile3 : 0;
     for (int f = 0; f < 3; ++f) {
-        uint32_t v[kItems], acc = 0;
-        for (int i = 0; i < kItems; ++i) {
+        uint32_t v[kItems3], acc = 0;
+        for (int i = 0; i < kItems3; ++i) {
             const size_t idx = base + i;
             v[i] = idx < n ? S.in[f][idx * S.stride] : 0u;
             acc += v[i];
         }
         uint32_t tot;
         uint32_t run = block_excl_scan(acc, tot) + pre[f];
-        for (int i = 0; i < kItems; ++i) {
+        for (int i = 0; i < kItems3; ++i) {
             const size_t idx = base + i;
             if (idx < n) S.out[f][idx] = run;
             run += v[i];
@@ -329,7 +333,7 @@ void scan_u32(DevMem& mem, const uint32_t* in, uint32_t* out, Count n, uint32_t*
 
 void scan3_u32(DevMem& mem, const uint32_t* const in[3], int stride, uint32_t* const out[3], Count n,
                uint32_t* const total[3], hipStream_t s) {
-    const size_t nb = (n.cap + kTile - 1) / kTile;
+    const size_t nb = (n.cap + kTile3 - 1) / kTile3;
     if (nb > (size_t)16 * kTile) {  // beyond one top-level workgroup: three separate scans
         for (int f = 0; f < 3; ++f) {
             if (stride != 1) throw std::runtime_error("strided scan larger than one top-level tile");
