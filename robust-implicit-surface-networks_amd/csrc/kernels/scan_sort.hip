@@ -15,8 +15,9 @@ constexpr int kThreads = 256;
 constexpr int kItems = 8;
 constexpr int kTile = kThreads * kItems;
 // the three-field scan: smaller tiles (its inputs are a few hundred
-// thousand per-tet counts: twice the workgroups keep more loads in flight)
-constexpr int kItems3 = 4;
+// thousand per-tet counts: four times the workgroups keep more loads in
+// flight)
+constexpr int kItems3 = 2;
 constexpr int kTile3 = kThreads * kItems3;
 constexpr int kWaves = kThreads / 64;
 
