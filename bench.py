@@ -81,22 +81,124 @@ def kernel_of_stage(label):
     return None
 
 
-def cpu_baseline(res, fs):
+ORACLE_LOOKUP = 4  # the oracle's one-function lookup table (the reference's default lookup path)
+
+
+def _oracle_threads(n):
+    import ctypes
+
+    import support as S
+
+    lib = S.oracle()
+    lib.orc_set_threads.argtypes = [ctypes.c_int]
+    return lib.orc_set_threads(n)
+
+
+def _stage_s(r, mi):
+    t = r.timings
+    if mi:
+        return t["highest func"] + t["filter"] + t["MI(wall)"] + t["extract mesh"] + t["compute xyz"]
+    return t["func signs"] + t["filter"] + t["simp_arr(wall)"] + t["extract mesh"] + t["compute xyz"]
+
+
+def cpu_baseline(res, fs, mi=False):
+    """The oracle's stages on this host: once on every core (OpenMP over tets
+    in the cut), once on one core; both take the one-function (IA) or
+    two-material (MI) lookup table, as the reference does by default.
+    Returns (baseline dict, the multi-core oracle run for the parity gate)."""
     import support as S
 
     types, params = fs.arrays()
     pts, tets = S.grid(res)
     vals = S.evaluate(types, params, pts)
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    r = S.OracleRun(0, pts, tets, vals)
-    # stage wall time: signs + filter + cut (wall) + extract + xyz
-    stage = (r.timings["func signs"] + r.timings["filter"] + r.timings["simp_arr(wall)"] +
-             r.timings["extract mesh"] + r.timings["compute xyz"])
+    _oracle_threads(cores)
+    r = S.OracleRun(int(mi), pts, tets, vals, flags=ORACLE_LOOKUP)
+    stage = _stage_s(r, mi)
+    _oracle_threads(1)
+    r1 = S.OracleRun(int(mi), pts, tets, vals, flags=ORACLE_LOOKUP)
+    stage1 = _stage_s(r1, mi)
+    del r1
+    _oracle_threads(cores)
     T = tets.shape[0]
-    return {"value": T / stage, "unit": "tets/s", "cores": cores, "kind": "port",
-            "sample": "the full workload once (%d tets): oracle stages signs, filter, per-tet cut (OpenMP over "
-                      "tets, %d threads), extraction and xyz (one thread); %.2f s of stage time" % (T, cores, stage),
-            "stage_s": {k: round(v, 4) for k, v in r.timings.items()}}
+    out = {"value": T / stage, "unit": "tets/s", "cores": cores, "kind": "port",
+           "sample": "the full workload (%d tets), twice: oracle stages %s, filter, per-tet cut with the lookup "
+                     "table (OpenMP over tets: %d threads, then 1), extraction and xyz (one thread); %.2f s and "
+                     "%.2f s of stage time" % (T, "highest func" if mi else "signs", cores, stage, stage1),
+           "one_core": {"value": T / stage1, "unit": "tets/s", "cores": 1},
+           "stage_s": {k: round(v, 4) for k, v in r.timings.items()}}
+    return out, r
+
+
+def parity_gate(dev, ref):
+    """The device result against the oracle on the same input: topology bit for
+    bit (CSR, vertex records, face cycles, tet-face pairs, labels), the largest
+    coordinate error relative to the coordinate's magnitude, and the reported
+    counters."""
+    import numpy as np
+
+    fit, st = dev.csr()
+    rec, xyz = dev.verts()
+    vs, v, ts, tf, fn = dev.faces()
+    topo = bool(np.array_equal(st, ref.start_index) and np.array_equal(fit, ref.func_in_tet) and
+                rec.shape == ref.verts.shape and np.array_equal(rec, ref.verts) and
+                np.array_equal(vs, ref.face_start) and np.array_equal(v, ref.face_verts) and
+                np.array_equal(ts, ref.tf_start) and np.array_equal(tf, ref.tf) and np.array_equal(fn, ref.face_func))
+    err = None
+    if xyz.shape == ref.xyz.shape:
+        err = float(np.max(np.abs(xyz - ref.xyz) / np.maximum(np.abs(ref.xyz), 1e-300))) if len(xyz) else 0.0
+    s = dev.sizes
+    stats = bool(dev.num_degenerate == ref.num_degenerate and dev.num_intersecting == ref.num_intersecting and
+                 dev.buckets == ref.buckets and s.n_verts == len(ref.verts) and s.n_faces == len(ref.face_func))
+    return {"topology_bit_exact": topo, "max_rel_xyz_err": err, "xyz_bit_exact": bool(
+        xyz.shape == ref.xyz.shape and np.array_equal(xyz.view(np.uint64), ref.xyz.view(np.uint64))),
+        "stats_equal": stats}
+
+
+def d2h_ms(res_obj, reps=3):
+    """Milliseconds to bring a result to host memory through the C ABI (CSR,
+    vertex records and coordinates, faces, in the reference's size_t layout)."""
+    best = None
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        res_obj.csr()
+        res_obj.verts()
+        res_obj.faces()
+        dt = (time.perf_counter() - t0) * 1e3
+        best = dt if best is None else min(best, dt)
+    return best
+
+
+def fresh_values_leg(ctx, res, n_sets, steps):
+    """Batched runs that switch value sets every step: n_sets seeded function
+    sets of the headline shape (6 planes, 6 spheres, 4 cylinders, 4 tori) on
+    one grid, values resident; each switch re-records the captured graph and
+    updates it in place.  Also the first run on a new grid (counting pass,
+    tuning, capture)."""
+    import numpy as np
+
+    import rin
+    import workloads as W
+
+    grid = ctx.grid_regular(res)
+    sets = []
+    for i in range(n_sets):
+        types, params = W.mixed(np.random.default_rng(1000 + i), 6, 6, 4, 4).arrays()
+        sets.append(grid.eval(types, params))
+    t0 = time.perf_counter()
+    sets[0].run(rin.FLAG_BATCHED).release()
+    first = time.perf_counter() - t0
+    for v in sets[1:]:
+        v.run(rin.FLAG_BATCHED).release()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        sets[i % n_sets].run(rin.FLAG_BATCHED).release()
+    dt = (time.perf_counter() - t0) / steps
+    for v in sets:
+        v.release()
+    grid.release()
+    return {"ms_per_step": dt * 1e3, "tets_per_s": 5 * res ** 3 / dt, "value_sets": n_sets, "steps": steps,
+            "first_run_ms": first * 1e3}
 
 
 def mi_leg(ctx, steps, warmup, with_cpu):
@@ -120,25 +222,16 @@ def mi_leg(ctx, steps, warmup, with_cpu):
             r.release()
     dt = (time.perf_counter() - t0) / steps
     sizes = r.sizes
-    r.release()
     out = {"workload": "material interface, %d^3 vertices (res %d, %d tets), 8 seeded materials "
                        "-|x - p_i| + w_i, values resident in HBM" % (res + 1, res, T),
            "form": "plain", "tets_per_s": T / dt, "ms_per_step": dt * 1e3, "steps": steps,
            "active_tets": int(sizes.num_intersecting), "mi_verts": int(sizes.n_verts),
            "mi_faces": int(sizes.n_faces)}
     if with_cpu:
-        import support as S
-
-        pts, tets = S.grid(res)
-        hv = S.evaluate(types, params, pts)
-        orc = S.OracleRun(1, pts, tets, hv)
-        stage = (orc.timings["highest func"] + orc.timings["filter"] + orc.timings["MI(wall)"] +
-                 orc.timings["extract mesh"] + orc.timings["compute xyz"])
-        cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        out["cpu_baseline"] = {"value": T / stage, "unit": "tets/s", "cores": cores, "kind": "port",
-                               "sample": "the full MI workload once: oracle stages highest func, filter, "
-                                         "per-tet cut (OpenMP, %d threads), extraction and xyz; %.2f s" % (cores, stage)}
+        out["cpu_baseline"], orc = cpu_baseline(res, fs, mi=True)
         out["vs_cpu"] = out["tets_per_s"] / out["cpu_baseline"]["value"]
+        out["parity"] = parity_gate(r, orc)
+    r.release()
     vals.release()
     grid.release()
     return out
@@ -291,6 +384,8 @@ def main():
                     help="timed steps of the material-interface leg (0: skip)")
     ap.add_argument("--mi-only", action="store_true",
                     help="run only the material-interface leg (prints its JSON object)")
+    ap.add_argument("--fresh-steps", type=int, default=12,
+                    help="timed steps of the leg that switches value sets every step (0: skip)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also run plain device copies of MIB MiB (reference kernel for counter scaling)")
     ap.add_argument("--pmc-summary", nargs=3, metavar=("FETCH_CSV", "WRITE_CSV", "TRACE_CSV"),
@@ -346,7 +441,6 @@ def main():
                 last = r
         dt = time.perf_counter() - t0
         sizes = last.sizes
-        last.release()
         # per-stage device times from separate, untimed runs: the batched form
         # then replays one captured graph per stage with events between them
         stage_acc = {}
@@ -357,7 +451,7 @@ def main():
             for k, ms in r.stage_times():
                 stage_acc[k] = stage_acc.get(k, 0.0) + ms
             r.release()
-        results[name] = {"s_per_step": dt / args.steps, "sizes": sizes,
+        results[name] = {"s_per_step": dt / args.steps, "sizes": sizes, "last": last,
                          "stages_ms": {k: v / n_stage for k, v in stage_acc.items()}}
 
     if args.copy_probe:
@@ -426,9 +520,19 @@ def main():
         summary["workload"] = ("Poisson arrivals; each job a %s grid, 2..32 mixed functions, 70%% implicit "
                                "arrangement / 30%% material interface (seeded)" % "{17^3, 33^3, 65^3}")
         out["serving"] = summary
+    out["d2h_ms"] = d2h_ms(results[best]["last"])
+    out["pcie_inclusive_tets_per_s"] = T / (sps + out["d2h_ms"] * 1e-3)
+    if args.fresh_steps:
+        out["fresh_values"] = fresh_values_leg(ctx, res, 4, args.fresh_steps)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(res, fs)
+        out["cpu_baseline"], orc = cpu_baseline(res, fs)
         out["vs_cpu"] = out["value"] / out["cpu_baseline"]["value"]
+        out["vs_cpu_one_core"] = out["value"] / out["cpu_baseline"]["one_core"]["value"]
+        # correctness gate of every timed form: its last step's output against the oracle
+        out["parity"] = {k: parity_gate(v["last"], orc) for k, v in results.items()}
+        del orc
+    for v in results.values():
+        v["last"].release()
     if args.mi_steps:
         out["material_interface"] = mi_leg(ctx, args.mi_steps, 2, not args.no_cpu)
     print(json.dumps(out))

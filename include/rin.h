@@ -109,6 +109,12 @@ int rin_values_eval(rin_ctx* ctx, const rin_grid* grid, const rin_func_desc* fun
 int rin_values_upload(rin_ctx* ctx, const rin_grid* grid, const double* vals_rowmajor, uint32_t n_funcs,
                       rin_values** out);
 int rin_values_download(rin_ctx* ctx, const rin_values* vals, double* vals_rowmajor);
+/* Function values at arbitrary points, evaluated on the GPU into host memory:
+ * n x 3 points in, n x n_funcs row-major values out.  This is the evaluation
+ * inside load_functions (external implicit_functions, called at
+ * app/implicit_arrangement.cpp:59 and tests/test_implicit_networks.cpp:62). */
+int rin_eval_points(rin_ctx* ctx, const double* pts, uint64_t n, const rin_func_desc* funcs, uint32_t n_funcs,
+                    double* vals_rowmajor);
 int rin_values_release(rin_ctx* ctx, rin_values* vals);
 
 /* The hot path: src/implicit_arrangement.cpp:53-402 (IA) or
@@ -151,6 +157,15 @@ int rin_result_release(rin_ctx* ctx, rin_result* res);
  * words + i*cap and its length (or a negative error) to lens[i]. */
 int rin_debug_cuts(rin_ctx* ctx, int32_t mi, const double* rows, const int32_t* ks, int32_t n_cases, int32_t cap,
                    uint16_t* words, int32_t* lens);
+
+/* Test hook with no counterpart in the reference, host only (no context, no
+ * device): the per-tet record the lookup tables give for independent tets.
+ * mode 0: IA one-function table (1 plane row per case), 1: IA two-function
+ * table (2 rows), 2: MI two-material table (2 material rows).  Each case
+ * writes at most cap words at words + i*cap and its length to lens[i]
+ * (-1: the table has no entry for it, -2: longer than cap). */
+int rin_debug_table_cuts(int32_t mode, const double* rows, int32_t n_cases, int32_t cap, uint16_t* words,
+                         int32_t* lens);
 
 /* Test hook with no counterpart in the reference: `reps` plain device copies
  * of `bytes` (a reference kernel of known traffic for hardware-counter

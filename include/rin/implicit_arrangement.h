@@ -15,6 +15,7 @@
 #pragma once
 #include <array>
 #include <cstddef>
+#include <cstdint>
 #include <string>
 #include <type_traits>
 #include <utility>
@@ -42,6 +43,15 @@ struct RowMajorValues {
 
 // Device used by the drop-in entry points on the calling thread (default 0).
 void set_device(int device);
+
+// The -R verdict of src/implicit_arrangement.cpp:329-343 (and
+// src/material_interface.cpp:374-386), which the reference computes inline:
+// type 2 when the cut failed in the normal order, type 3 when it failed in the
+// reverse order, type 1 when a tet's (vertex, face, cell) counts differ
+// between the orders, success otherwise.  Prints the reference's message and
+// returns whether the check passed.
+bool robust_verdict(bool forward_ok, bool reverse_ok, const std::vector<uint64_t>& forward_counts,
+                    const std::vector<uint64_t>& reverse_counts);
 
 }  // namespace rin
 

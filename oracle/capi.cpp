@@ -1,5 +1,7 @@
 // ORACLE — test infrastructure only.  C entry points for the test suite
 // (loaded with ctypes from tests/ and from bench.py's CPU-baseline leg).
+#include <omp.h>
+
 #include <cstring>
 #include <string>
 
@@ -29,6 +31,12 @@ std::vector<Func> make_funcs(const int* types, const double* params, int F) {
 extern "C" {
 
 const char* orc_last_error() { return g_err.c_str(); }
+// OpenMP threads of the per-tet cut and the evaluation (0: leave as is);
+// returns the number in effect
+int orc_set_threads(int n) {
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+}
 uint64_t orc_exact_fallbacks() { return g_exact_fallbacks; }
 
 int orc_grid(int res, const double* bmin, const double* bmax, double* pts, uint64_t* tets) {

@@ -90,6 +90,7 @@ struct Result {
     bool mi = false;
     uint64_t nv = 0, nt = 0, nf = 0;
     // filter
+    // IA: zero (vertex, function) entries; MI: vertices with tied maxima
     uint64_t num_degenerate = 0, num_intersecting = 0;
     std::vector<uint64_t> func_in_tet, start_index;
     // MI highest material
@@ -108,7 +109,18 @@ struct Result {
     std::vector<std::pair<const char*, double>> timings;
 };
 
-enum Flags { F_KEEP_CUTS = 1, F_REVERSE = 2 };
+// F_LOOKUP: one-function IA tets and two-material MI tets take their complex
+// from a table indexed by corner signs (the reference's default lookup-table
+// path, app/implicit_arrangement.cpp:29-34, src/implicit_arrangement.cpp:276-283;
+// src/material_interface.cpp:318-328).  Used by the CPU baseline; the tests
+// check it against the general cut.
+enum Flags { F_KEEP_CUTS = 1, F_REVERSE = 2, F_LOOKUP = 4 };
+
+// The table: the complex of one plane (IA) or two materials (MI) depends only
+// on the corner signs of the plane (of material 1 minus material 0), so the
+// 81 patterns are cut once with rows of -1/0/+1.
+const IACut& ia_table1(int pattern);
+const MICut& mi_table2(int pattern);
 
 void run_ia(const std::vector<std::array<double, 3>>& pts,
             const std::vector<std::array<uint64_t, 4>>& tets,

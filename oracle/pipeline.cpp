@@ -106,6 +106,33 @@ void eval(const std::vector<Func>& funcs, const std::vector<std::array<double, 3
         for (size_t j = 0; j < F; ++j) vals[v * F + j] = eval_one(funcs[j], pts[v]);
 }
 
+// ------------------------------------------------------------------ tables
+namespace {
+template <class Cut, class Fn>
+std::vector<Cut> build_table(Fn cut, bool mi) {
+    std::vector<Cut> tab(81);
+    for (int pat = 0; pat < 81; ++pat) {
+        std::array<double, 4> row;
+        int p = pat;
+        for (int c = 0; c < 4; ++c, p /= 3) row[c] = double(p % 3 - 1);
+        if (mi)
+            tab[pat] = cut(std::vector<std::array<double, 4>>{{0, 0, 0, 0}, row});
+        else
+            tab[pat] = cut(std::vector<std::array<double, 4>>{row});
+    }
+    return tab;
+}
+}  // namespace
+
+const IACut& ia_table1(int pattern) {
+    static const std::vector<IACut> tab = build_table<IACut>(ia_cut, false);
+    return tab[pattern];
+}
+const MICut& mi_table2(int pattern) {
+    static const std::vector<MICut> tab = build_table<MICut>(mi_cut, true);
+    return tab[pattern];
+}
+
 // ------------------------------------------------------------------ IA
 void run_ia(const std::vector<std::array<double, 3>>& pts, const std::vector<std::array<uint64_t, 4>>& tets,
             const std::vector<double>& vals, uint64_t F, int flags, Result& r) {
@@ -159,12 +186,21 @@ void run_ia(const std::vector<std::array<double, 3>>& pts, const std::vector<std
             act.push_back(t);
         }
     r.ia_cuts.resize(act.size());
+    if (flags & F_LOOKUP) ia_table1(0);
     double bt[3] = {0, 0, 0};
 #pragma omp parallel for schedule(dynamic, 256) reduction(+ : bt[:3])
     for (long long a = 0; a < (long long)act.size(); ++a) {
         uint64_t t = act[a];
         double c0 = now();
         uint64_t s0 = r.start_index[t], k = r.start_index[t + 1] - s0;
+        if ((flags & F_LOOKUP) && k == 1) {
+            const uint64_t f = r.func_in_tet[s0];
+            int pat = 0;
+            for (int c = 3; c >= 0; --c) pat = pat * 3 + (sg[tets[t][c] * F + f] + 1);
+            r.ia_cuts[a] = ia_table1(pat);
+            bt[0] += now() - c0;
+            continue;
+        }
         std::vector<std::array<double, 4>> planes(k);
         for (uint64_t q = 0; q < k; ++q) {
             uint64_t f = r.func_in_tet[s0 + ((flags & F_REVERSE) ? k - 1 - q : q)];

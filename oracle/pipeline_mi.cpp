@@ -48,6 +48,9 @@ void run_mi(const std::vector<std::array<double, 3>>& pts, const std::vector<std
         }
         r.highest[v] = id;
         if (cnt > 1) {
+            // vertices with tied maxima (the reference's is_degenerate_vertex,
+            // src/material_interface.cpp:79-81)
+            ++r.num_degenerate;
             r.degenerate_vertex[v] = 1;
             auto& L = ties[v];
             for (uint64_t j = 0; j < F; ++j)
@@ -99,12 +102,24 @@ void run_mi(const std::vector<std::array<double, 3>>& pts, const std::vector<std
             act.push_back(t);
         }
     r.mi_cuts.resize(act.size());
+    if (flags & F_LOOKUP) mi_table2(0);
     double bt[3] = {0, 0, 0};
 #pragma omp parallel for schedule(dynamic, 256) reduction(+ : bt[:3])
     for (long long a = 0; a < (long long)act.size(); ++a) {
         uint64_t t = act[a];
         double c0 = now();
         uint64_t s0 = r.start_index[t], k = r.start_index[t + 1] - s0;
+        if ((flags & F_LOOKUP) && k == 2 && !(flags & F_REVERSE)) {
+            const uint64_t m0 = r.func_in_tet[s0], m1 = r.func_in_tet[s0 + 1];
+            int pat = 0;
+            for (int c = 3; c >= 0; --c) {
+                const double a0 = fv(tets[t][c], m0), a1 = fv(tets[t][c], m1);
+                pat = pat * 3 + (a1 > a0 ? 2 : (a1 < a0 ? 0 : 1));
+            }
+            r.mi_cuts[a] = mi_table2(pat);
+            bt[0] += now() - c0;
+            continue;
+        }
         std::vector<std::array<double, 4>> mats(k);
         for (uint64_t q = 0; q < k; ++q) {
             uint64_t f = r.func_in_tet[s0 + ((flags & F_REVERSE) ? k - 1 - q : q)];

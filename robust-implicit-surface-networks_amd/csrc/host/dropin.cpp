@@ -5,6 +5,7 @@
 // Driver order and reported labels follow src/implicit_arrangement.cpp:13-647
 // and src/material_interface.cpp:13-694.
 #include <chrono>
+#include <cmath>
 #include <cstring>
 #include <iostream>
 #include <memory>
@@ -17,22 +18,43 @@
 #include "graph.h"
 #include "rin.h"
 #include "rin/implicit_arrangement.h"
+#include "simplicial_arrangement/lookup_table.h"
+#include "thread_ctx.h"
 
 namespace {
 
 thread_local int t_device = 0;
 
+// the grid of the last call, kept on the device: callers that pass the same
+// mesh again (the same generate_tet_mesh grid, or the same explicit mesh) reuse
+// its device tables
+struct GridKey {
+    bool regular = false;
+    int res = 0;
+    double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    uint64_t V = 0, T = 0, hash = 0;
+    bool operator==(const GridKey& o) const {
+        return regular == o.regular && res == o.res && V == o.V && T == o.T && hash == o.hash &&
+               std::memcmp(lo, o.lo, sizeof(lo)) == 0 && std::memcmp(hi, o.hi, sizeof(hi)) == 0;
+    }
+};
+
 // one context per host thread (contexts are independent)
 struct Ctx {
     rin_ctx* c = nullptr;
     int device = -1;
-    ~Ctx() {
+    rin_grid* grid = nullptr;
+    GridKey key;
+    ~Ctx() { reset(); }
+    void reset() {
+        if (grid) rin_grid_release(c, grid);
+        grid = nullptr;
         if (c) rin_destroy(c);
+        c = nullptr;
     }
     rin_ctx* get() {
         if (!c || device != t_device) {
-            if (c) rin_destroy(c);
-            c = nullptr;
+            reset();
             if (rin_create(&c, t_device) != RIN_OK) throw std::runtime_error("rin_create failed");
             device = t_device;
         }
@@ -40,6 +62,70 @@ struct Ctx {
     }
 };
 thread_local Ctx t_ctx;
+
+// Is (pts, tets) exactly what generate_tet_mesh (src/io.cpp:95-152) makes for
+// some resolution and box?  Then the device builds the grid itself (implicit
+// tets, closed-form edge and face ids, star tables for the dedup).
+bool regular_grid_of(const std::vector<std::array<double, 3>>& pts, const std::vector<std::array<size_t, 4>>& tets,
+                     GridKey& k) {
+    const size_t T = tets.size();
+    if (T == 0 || T % 5) return false;
+    size_t r = (size_t)std::llround(std::cbrt((double)(T / 5)));
+    while (r * r * r * 5 > T) --r;
+    while ((r + 1) * (r + 1) * (r + 1) * 5 <= T) ++r;
+    if (r == 0 || r * r * r * 5 != T || r > 1600) return false;
+    const size_t N = r + 1;
+    if (pts.size() != N * N * N) return false;
+    const double* lo = pts.front().data();
+    const double* hi = pts.back().data();
+    auto coord = [&](size_t i, int a) { return (double(i) / double(N - 1)) * (hi[a] - lo[a]) + lo[a]; };
+    for (size_t i = 0; i < N; ++i) {
+        const double x = coord(i, 0);
+        for (size_t j = 0; j < N; ++j) {
+            const double y = coord(j, 1);
+            for (size_t q = 0; q < N; ++q) {
+                const auto& p = pts[(i * N + j) * N + q];
+                if (p[0] != x || p[1] != y || p[2] != coord(q, 2)) return false;
+            }
+        }
+    }
+    static const int even[5][4] = {{4, 6, 1, 3}, {6, 3, 4, 7}, {1, 3, 0, 4}, {3, 1, 2, 6}, {4, 1, 6, 5}};
+    static const int odd[5][4] = {{7, 0, 2, 5}, {2, 3, 0, 7}, {5, 7, 0, 4}, {7, 2, 6, 5}, {0, 1, 2, 5}};
+    for (size_t i = 0; i < r; ++i)
+        for (size_t j = 0; j < r; ++j)
+            for (size_t q = 0; q < r; ++q) {
+                const size_t b = (i * N + j) * N + q;
+                const size_t v[8] = {b, b + N * N, b + N * N + N, b + N, b + 1, b + N * N + 1, b + N * N + N + 1,
+                                     b + N + 1};
+                const int(*tab)[4] = ((i + j + q) % 2 == 0) ? even : odd;
+                const size_t t = ((i * r + j) * r + q) * 5;
+                for (int e = 0; e < 5; ++e)
+                    for (int c = 0; c < 4; ++c)
+                        if (tets[t + e][c] != v[tab[e][c]]) return false;
+            }
+    k.regular = true;
+    k.res = (int)r;
+    for (int a = 0; a < 3; ++a) {
+        k.lo[a] = lo[a];
+        k.hi[a] = hi[a];
+    }
+    k.V = pts.size();
+    k.T = T;
+    return true;
+}
+
+uint64_t mix_words(const void* p, size_t bytes, uint64_t h) {
+    const unsigned char* b = static_cast<const unsigned char*>(p);
+    size_t i = 0;
+    for (; i + 8 <= bytes; i += 8) {
+        uint64_t w;
+        std::memcpy(&w, b + i, 8);
+        h = (h ^ w) * 0x9E3779B97F4A7C15ull;
+        h ^= h >> 29;
+    }
+    for (; i < bytes; ++i) h = (h ^ b[i]) * 0x100000001B3ull;
+    return h;
+}
 
 struct Failure : std::runtime_error {
     using std::runtime_error::runtime_error;
@@ -49,7 +135,7 @@ void check(rin_ctx* c, int st, const char* what) {
     if (st != RIN_OK) throw Failure(std::string(what) + ": " + rin_last_error(c));
 }
 
-// RAII owners of the C ABI handles
+// RAII owners of the C ABI handles (the grid stays with the thread's context)
 struct Handles {
     rin_ctx* c;
     rin_grid* g = nullptr;
@@ -59,7 +145,6 @@ struct Handles {
     ~Handles() {
         for (auto* r : res) rin_result_release(c, r);
         if (v) rin_values_release(c, v);
-        if (g) rin_grid_release(c, g);
     }
     rin_result* run(uint32_t flags) {
         rin_result* r = nullptr;
@@ -119,35 +204,55 @@ void append_stage_times(rin_ctx* c, const rin_result* r, std::vector<std::string
     }
 }
 
+// The device grid for (pts, tets) — the thread's cached one when the mesh is
+// the last call's — and the function values uploaded onto it.
+void setup(Handles& H, const std::vector<std::array<double, 3>>& pts, const std::vector<std::array<size_t, 4>>& tets,
+           const rin::RowMajorValues& funcVals) {
+    rin_ctx* c = H.c;
+    GridKey k;
+    if (!regular_grid_of(pts, tets, k)) {
+        k = GridKey{};
+        k.V = pts.size();
+        k.T = tets.size();
+        k.hash = mix_words(pts.data(), pts.size() * sizeof(pts[0]), 0x12345);
+        k.hash = mix_words(tets.data(), tets.size() * sizeof(tets[0]), k.hash);
+    }
+    if (!(t_ctx.grid && t_ctx.key == k)) {
+        if (t_ctx.grid) rin_grid_release(c, t_ctx.grid);
+        t_ctx.grid = nullptr;
+        rin_grid* g = nullptr;
+        if (k.regular) {
+            check(c, rin_grid_regular(c, k.res, k.lo, k.hi, &g), "rin_grid_regular");
+        } else {
+            std::vector<uint64_t> t64(4 * tets.size());
+            for (size_t t = 0; t < tets.size(); ++t)
+                for (int q = 0; q < 4; ++q) t64[4 * t + q] = tets[t][q];
+            check(c, rin_grid_explicit(c, &pts[0][0], pts.size(), t64.data(), tets.size(), &g), "rin_grid_explicit");
+        }
+        t_ctx.grid = g;
+        t_ctx.key = k;
+    }
+    H.g = t_ctx.grid;
+    check(c, rin_values_upload(c, H.g, funcVals.data, (uint32_t)funcVals.cols, &H.v), "rin_values_upload");
+}
+
 // -R check: per-tet (vertex, face, cell) counts forward vs reversed
 // (src/implicit_arrangement.cpp:136-243, src/material_interface.cpp:174-287)
 bool robust_check(Handles& H, uint32_t mode, std::vector<std::string>& tl, std::vector<double>& tv) {
     rin_result* fwd = nullptr;
-    int st = rin_run(H.c, H.g, H.v, mode | RIN_FLAG_CUT_ONLY, &fwd);
-    if (st != RIN_OK) {
-        std::cout << "type 2 failure (crash in the normal order)." << std::endl;
-        return false;
-    }
+    if (rin_run(H.c, H.g, H.v, mode | RIN_FLAG_CUT_ONLY, &fwd) != RIN_OK) return rin::robust_verdict(false, false, {}, {});
     H.res.push_back(fwd);
     append_stage_times(H.c, fwd, tl, tv);
     rin_result* rev = nullptr;
-    st = rin_run(H.c, H.g, H.v, mode | RIN_FLAG_CUT_ONLY | RIN_FLAG_REVERSE, &rev);
-    if (st != RIN_OK) {
-        std::cout << "type 3 failure (crash in the reverse order)." << std::endl;
-        return false;
-    }
+    if (rin_run(H.c, H.g, H.v, mode | RIN_FLAG_CUT_ONLY | RIN_FLAG_REVERSE, &rev) != RIN_OK)
+        return rin::robust_verdict(true, false, {}, {});
     H.res.push_back(rev);
     rin_sizes s{};
     check(H.c, rin_result_sizes(H.c, fwd, &s), "rin_result_sizes");
     std::vector<uint64_t> a(3 * s.num_intersecting), b(3 * s.num_intersecting);
     check(H.c, rin_result_cut_counts(H.c, fwd, a.data()), "rin_result_cut_counts");
     check(H.c, rin_result_cut_counts(H.c, rev, b.data()), "rin_result_cut_counts");
-    if (a != b) {
-        std::cout << "type 1 failure (inconsistency)." << std::endl;
-        return false;
-    }
-    std::cout << "success." << std::endl;
-    return true;
+    return rin::robust_verdict(true, true, a, b);
 }
 
 struct DeviceMesh {
@@ -194,6 +299,27 @@ void push_stat(std::vector<std::string>& sl, std::vector<size_t>& sv, const char
 
 namespace rin {
 void set_device(int device) { t_device = device; }
+
+bool robust_verdict(bool forward_ok, bool reverse_ok, const std::vector<uint64_t>& forward_counts,
+                    const std::vector<uint64_t>& reverse_counts) {
+    if (!forward_ok) {
+        std::cout << "type 2 failure (crash in the normal order)." << std::endl;
+        return false;
+    }
+    if (!reverse_ok) {
+        std::cout << "type 3 failure (crash in the reverse order)." << std::endl;
+        return false;
+    }
+    if (forward_counts != reverse_counts) {
+        std::cout << "type 1 failure (inconsistency)." << std::endl;
+        return false;
+    }
+    std::cout << "success." << std::endl;
+    return true;
+}
+namespace detail {
+rin_ctx* thread_context() { return t_ctx.get(); }
+}  // namespace detail
 }  // namespace rin
 
 bool implicit_arrangement(bool robust_test, bool use_lookup, bool use_secondary_lookup, bool use_topo_ray_shooting,
@@ -222,12 +348,9 @@ bool implicit_arrangement(bool robust_test, bool use_lookup, bool use_secondary_
     try {
         rin_ctx* c = t_ctx.get();
         Handles H(c);
-        std::vector<uint64_t> t64(4 * n_tets);
-        for (size_t t = 0; t < n_tets; ++t)
-            for (int q = 0; q < 4; ++q) t64[4 * t + q] = tets[t][q];
-        check(c, rin_grid_explicit(c, &pts[0][0], n_pts, t64.data(), n_tets, &H.g), "rin_grid_explicit");
-        check(c, rin_values_upload(c, H.g, funcVals.data, (uint32_t)F, &H.v), "rin_values_upload");
-        const uint32_t lookup = use_lookup ? 0u : RIN_FLAG_NO_LOOKUP;
+        setup(H, pts, tets, funcVals);
+        const uint32_t lookup =
+            (use_lookup && simplicial_arrangement::lookup_table_enabled()) ? 0u : RIN_FLAG_NO_LOOKUP;
         if (robust_test) {
             const bool ok = robust_check(H, lookup, timing_labels, timings);
             rin_sizes s{};
@@ -348,12 +471,9 @@ bool material_interface(bool robust_test, bool use_lookup, bool use_secondary_lo
     try {
         rin_ctx* c = t_ctx.get();
         Handles H(c);
-        std::vector<uint64_t> t64(4 * n_tets);
-        for (size_t t = 0; t < n_tets; ++t)
-            for (int q = 0; q < 4; ++q) t64[4 * t + q] = tets[t][q];
-        check(c, rin_grid_explicit(c, &pts[0][0], n_pts, t64.data(), n_tets, &H.g), "rin_grid_explicit");
-        check(c, rin_values_upload(c, H.g, funcVals.data, (uint32_t)F, &H.v), "rin_values_upload");
-        const uint32_t mode = RIN_FLAG_MI | (use_lookup ? 0u : RIN_FLAG_NO_LOOKUP);
+        setup(H, pts, tets, funcVals);
+        const uint32_t mode =
+            RIN_FLAG_MI | ((use_lookup && simplicial_arrangement::lookup_table_enabled()) ? 0u : RIN_FLAG_NO_LOOKUP);
         if (robust_test) {
             const bool ok = robust_check(H, mode, timing_labels, timings);
             rin_sizes s{};

@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <functional>
 #include <mutex>
 #include <stdexcept>
@@ -1420,17 +1421,27 @@ struct Workspace {
     uint32_t* pub_h = nullptr;
     uint32_t* pub_d = nullptr;
     bool exec = false;  // graphs captured for the current capacities
+    int grid[3] = {0, 0, 0};  // cut kernels' grids the graphs were captured with
+    // the values the whole graph and the per-stage graphs read (a run with
+    // other values of the same size re-records the graph and updates it in place)
+    const double* bound_whole = nullptr;
+    const double* bound_parts = nullptr;
     bool ctr_clear = false;  // counters and error flags are zero (left so by the last run)
     bool in_use = false;
-    void drop_graphs() {
+    void drop_parts() {
         for (auto& p : parts) (void)hipGraphExecDestroy(p.second);
         for (auto& g : graphs) (void)hipGraphDestroy(g.second);
         parts.clear();
         graphs.clear();
+        bound_parts = nullptr;
+    }
+    void drop_graphs() {
+        drop_parts();
         if (whole) (void)hipGraphExecDestroy(whole);
         if (whole_graph) (void)hipGraphDestroy(whole_graph);
         whole = nullptr;
         whole_graph = nullptr;
+        bound_whole = nullptr;
     }
     ~Workspace() {
         drop_graphs();
@@ -1448,6 +1459,20 @@ struct Plan {
     int blocks_per_cu[3] = {0, 0, 0};  // autotuned grid of each bucket's cut kernel
     uint32_t hint_b[5] = {0, 0, 0, 0, 0};
 };
+
+// one plan per (mode, function count) of a grid
+struct PlanCache {
+    std::mutex mu;
+    std::map<uint64_t, std::shared_ptr<Plan>> plans;
+    std::shared_ptr<Plan> get(uint64_t key) {
+        std::lock_guard<std::mutex> l(mu);
+        auto& p = plans[key];
+        if (!p) p = std::make_shared<Plan>();
+        return p;
+    }
+};
+
+std::shared_ptr<PlanCache> make_plan_cache() { return std::make_shared<PlanCache>(); }
 
 namespace {
 
@@ -1990,7 +2015,141 @@ float time_ms(hipStream_t s, const std::function<void()>& f) {
 
 }  // namespace
 
-void run_batched(DevMem& ctx_mem, const Grid& g, Values& val, uint32_t flags, RunOut& out, hipStream_t s, int n_cu) {
+int debug_table_record_ia(int k, const double* rows, uint16_t* out, int cap) {
+    auto sgn = [](double x) { return x > 0 ? 1 : (x < 0 ? -1 : 0); };
+    if (k == 1) {
+        const K1Table& K = k1_table();
+        int pat = 0;
+        for (int c = 0, w = 1; c < 4; ++c, w *= 3) pat += (sgn(rows[c]) + 1) * w;
+        const K1Entry& E = K.e[pat];
+        if (!E.valid) return -1;
+        if (E.rec_len > cap) return -2;
+        std::memcpy(out, K.words.data() + E.rec_off, 2 * (size_t)E.rec_len);
+        return E.rec_len;
+    }
+    if (k != 2) return -3;
+    // the slot arithmetic of k_k2_lookup
+    const K2Table& K2 = k2_table();
+    int pa = 0, sB[4];
+    for (int c = 0, w = 1; c < 4; ++c, w *= 3) {
+        pa += (sgn(rows[c]) + 1) * w;
+        sB[c] = sgn(rows[4 + c]);
+    }
+    const K2Pat& K = K2.pat[pa];
+    if (!K.valid) return -1;
+    uint32_t slot = K.base, w = 81;
+    for (int c = 0, w3 = 1; c < 4; ++c, w3 *= 3) slot += (uint32_t)(sB[c] + 1) * w3;
+    bool err = false;
+    for (int q = 0; q + 4 < K.nv; ++q, w *= 3) {
+        const int c1 = K.e[q][0], c2 = K.e[q][1];
+        int sg;
+        if (sB[c1] * sB[c2] < 0)
+            sg = dev::orient_ia_edge(rows[c1], rows[c2], rows[4 + c1], rows[4 + c2], err);
+        else
+            sg = sB[c1] == sB[c2] ? sB[c1] : (sB[c1] != 0 ? sB[c1] : sB[c2]);
+        slot += (uint32_t)(sg + 1) * w;
+    }
+    if (err || K2.off[slot] == K2_NONE) return -1;
+    const uint16_t* r = K2.words.data() + K2.off[slot];
+    const int len = r[6];
+    if (len > cap) return -2;
+    std::memcpy(out, r, 2 * (size_t)len);
+    return len;
+}
+
+namespace {
+
+std::string cut_label(const char* base, const Workspace& w, const Plan& P, int q) {
+    return std::string(base) + (w.grid[q] == -2 ? " (sign-vector table)"
+                                : w.grid[q] < 0 ? " (wave per tet)"
+                                                : " (thread per tet, " + std::to_string(P.blocks_per_cu[q]) +
+                                                      " blocks/CU)");
+}
+
+// the whole path as one graph: the k = 5..8 cut (few tets, long serial
+// insertions) and the k = 2 table lookups on a branch beside the k = 3..4 cut
+// (the cut kernels share no outputs); the three extraction passes write
+// disjoint rows and run as one launch, and so do coordinates and the first
+// face pass
+void record_whole(Workspace& w, const Grid& g, const Values& val, hipStream_t s) {
+    if (!w.side) {
+        RIN_HIP(hipStreamCreateWithFlags(&w.side, hipStreamNonBlocking));
+        RIN_HIP(hipEventCreateWithFlags(&w.fork, hipEventDisableTiming));
+        RIN_HIP(hipEventCreateWithFlags(&w.join, hipEventDisableTiming));
+    }
+    hipGraph_t gr = nullptr;
+    RIN_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+    stage_count(w, g, val, s, false);
+    stage_fill(w, g, val, s);
+    RIN_HIP(hipEventRecord(w.fork, s));
+    RIN_HIP(hipStreamWaitEvent(w.side, w.fork, 0));
+    if (w.cap.b[2]) launch_cut<CBig>(w, g, val, 2, w.grid[2], w.side);
+    if (w.cap.b[0]) launch_cut<CK2>(w, g, val, 0, w.grid[0], w.side);
+    RIN_HIP(hipEventRecord(w.join, w.side));
+    if (w.cap.b[1]) launch_cut<CMed>(w, g, val, 1, w.grid[1], s);
+    RIN_HIP(hipStreamWaitEvent(s, w.join, 0));
+    stage_rest(w, g, val, s, REST_SCANS);
+    stage_rest(w, g, val, s, REST_EXTRACT_K1 | REST_EXTRACT_REC);
+    stage_rest(w, g, val, s, REST_DEDUP);
+    stage_rest(w, g, val, s, REST_XYZ | REST_FACES);
+    RIN_HIP(hipStreamEndCapture(s, &gr));
+    bool updated = false;
+    if (w.whole) {
+        hipGraphNode_t bad = nullptr;
+        hipGraphExecUpdateResult res;
+        updated = hipGraphExecUpdate(w.whole, gr, &bad, &res) == hipSuccess && res == hipGraphExecUpdateSuccess;
+        if (!updated) {
+            (void)hipGetLastError();
+            (void)hipGraphExecDestroy(w.whole);
+            w.whole = nullptr;
+        }
+    }
+    if (!updated) RIN_HIP(hipGraphInstantiate(&w.whole, gr, nullptr, nullptr, 0));
+    if (w.whole_graph) (void)hipGraphDestroy(w.whole_graph);
+    w.whole_graph = gr;
+    while (w.ev.size() < 2) {
+        hipEvent_t e;
+        RIN_HIP(hipEventCreate(&e));
+        w.ev.push_back(e);
+    }
+    w.bound_whole = val.vals.p;
+}
+
+// the same path as one graph per stage, for per-stage device times
+void record_parts(Workspace& w, const Grid& g, const Values& val, const Plan& P, hipStream_t s) {
+    w.drop_parts();
+    auto capture = [&](const std::string& label, const std::function<void()>& body) {
+        hipGraph_t gr = nullptr;
+        hipGraphExec_t ex = nullptr;
+        RIN_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+        body();
+        RIN_HIP(hipStreamEndCapture(s, &gr));
+        RIN_HIP(hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0));
+        w.graphs.push_back({label, gr});
+        w.parts.push_back({label, ex});
+    };
+    capture("signs + filter + CSR scans", [&] { stage_count(w, g, val, s); });
+    capture("CSR fill + buckets + k=1 table", [&] { stage_fill(w, g, val, s); });
+    if (w.cap.b[0]) capture(cut_label("cut k=2", w, P, 0), [&] { launch_cut<CK2>(w, g, val, 0, w.grid[0], s); });
+    if (w.cap.b[1]) capture(cut_label("cut k=3..4", w, P, 1), [&] { launch_cut<CMed>(w, g, val, 1, w.grid[1], s); });
+    if (w.cap.b[2]) capture(cut_label("cut k=5..8", w, P, 2), [&] { launch_cut<CBig>(w, g, val, 2, w.grid[2], s); });
+    capture("extract: per-tet output scans", [&] { stage_rest(w, g, val, s, REST_SCANS); });
+    capture("extract k=1 occurrences (k_extract_k1)", [&] { stage_rest(w, g, val, s, REST_EXTRACT_K1); });
+    capture("extract record occurrences (k_extract_records)", [&] { stage_rest(w, g, val, s, REST_EXTRACT_REC); });
+    capture("dedup verts (star tables)", [&] { stage_rest(w, g, val, s, REST_DEDUP); });
+    capture("compute xyz (k_vert_fields_ia)", [&] { stage_rest(w, g, val, s, REST_XYZ); });
+    capture("faces", [&] { stage_rest(w, g, val, s, REST_FACES); });
+    while (w.ev.size() < w.parts.size() + 1) {
+        hipEvent_t e;
+        RIN_HIP(hipEventCreate(&e));
+        w.ev.push_back(e);
+    }
+    w.bound_parts = val.vals.p;
+}
+
+}  // namespace
+
+void run_batched(DevMem& ctx_mem, const Grid& g, const Values& val, uint32_t flags, RunOut& out, hipStream_t s, int n_cu) {
     if (flags & (RUN_MI | RUN_CUT_ONLY | RUN_REVERSE))
         throw std::runtime_error("batched form is not available for this mode (use the plain form)");
     // the batched form is built on the k = 1 and k = 2 tables
@@ -1999,8 +2158,8 @@ void run_batched(DevMem& ctx_mem, const Grid& g, Values& val, uint32_t flags, Ru
     const uint64_t V = g.view.V, T = g.view.T;
     if (val.V != V) throw std::invalid_argument("values do not match the grid");
     ensure_nbr(ctx_mem, g, s);
-    auto& planp = val.plan[0];
-    if (!planp) planp = std::make_shared<Plan>();
+    if (val.grid_id != g.id) throw std::invalid_argument("values were made on another grid");
+    std::shared_ptr<Plan> planp = g.plans->get((uint64_t)val.F);
     Plan& P = *planp;
     std::lock_guard<std::mutex> lock(P.mu);
     std::shared_ptr<Workspace> W;
@@ -2099,81 +2258,18 @@ void run_batched(DevMem& ctx_mem, const Grid& g, Values& val, uint32_t flags, Ru
                     P.blocks_per_cu[q] = bestc;
                 }
             }
-            int grid[3];
             for (int q = 0; q < 3; ++q) {
-                grid[q] = P.blocks_per_cu[q] < 0 ? P.blocks_per_cu[q] : std::max(1, n_cu * P.blocks_per_cu[q]);
-                w.wave_cut[q] = grid[q] == -1;
+                w.grid[q] = P.blocks_per_cu[q] < 0 ? P.blocks_per_cu[q] : std::max(1, n_cu * P.blocks_per_cu[q]);
+                w.wave_cut[q] = w.grid[q] == -1;
             }
-            // capture the whole path
             RIN_HIP(hipStreamSynchronize(s));
-            auto capture = [&](const std::string& label, const std::function<void()>& body) {
-                hipGraph_t gr = nullptr;
-                hipGraphExec_t ex = nullptr;
-                RIN_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
-                body();
-                RIN_HIP(hipStreamEndCapture(s, &gr));
-                RIN_HIP(hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0));
-                w.graphs.push_back({label, gr});
-                w.parts.push_back({label, ex});
-            };
-            std::vector<std::pair<std::string, std::function<void()>>> stages;
-            stages.push_back({"signs + filter + CSR scans", [&] { stage_count(w, g, val, s); }});
-            stages.push_back({"CSR fill + buckets + k=1 table", [&] { stage_fill(w, g, val, s); }});
-            auto cut_label = [&](const char* base, int q) {
-                return std::string(base) + (grid[q] == -2 ? " (sign-vector table)"
-                                            : grid[q] < 0 ? " (wave per tet)"
-                                                          : " (thread per tet, " + std::to_string(P.blocks_per_cu[q]) +
-                                                                " blocks/CU)");
-            };
-            if (w.cap.b[0]) stages.push_back({cut_label("cut k=2", 0), [&] { launch_cut<CK2>(w, g, val, 0, grid[0], s); }});
-            if (w.cap.b[1])
-                stages.push_back({cut_label("cut k=3..4", 1), [&] { launch_cut<CMed>(w, g, val, 1, grid[1], s); }});
-            if (w.cap.b[2])
-                stages.push_back({cut_label("cut k=5..8", 2), [&] { launch_cut<CBig>(w, g, val, 2, grid[2], s); }});
-            stages.push_back({"extract: per-tet output scans", [&] { stage_rest(w, g, val, s, REST_SCANS); }});
-            stages.push_back(
-                {"extract k=1 occurrences (k_extract_k1)", [&] { stage_rest(w, g, val, s, REST_EXTRACT_K1); }});
-            stages.push_back({"extract record occurrences (k_extract_records)",
-                              [&] { stage_rest(w, g, val, s, REST_EXTRACT_REC); }});
-            stages.push_back({"dedup verts (star tables)", [&] { stage_rest(w, g, val, s, REST_DEDUP); }});
-            stages.push_back({"compute xyz (k_vert_fields_ia)", [&] { stage_rest(w, g, val, s, REST_XYZ); }});
-            stages.push_back({"faces", [&] { stage_rest(w, g, val, s, REST_FACES); }});
-            for (auto& st : stages) capture(st.first, st.second);
-            // the whole path: the k = 5..8 cut (few tets, long serial
-            // insertions) and the k = 2 table lookups on a branch beside the
-            // k = 3..4 cut; the cut kernels share no outputs
-            if (!w.side) {
-                RIN_HIP(hipStreamCreateWithFlags(&w.side, hipStreamNonBlocking));
-                RIN_HIP(hipEventCreateWithFlags(&w.fork, hipEventDisableTiming));
-                RIN_HIP(hipEventCreateWithFlags(&w.join, hipEventDisableTiming));
-            }
-            RIN_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
-            stage_count(w, g, val, s, false);
-            stage_fill(w, g, val, s);
-            RIN_HIP(hipEventRecord(w.fork, s));
-            RIN_HIP(hipStreamWaitEvent(w.side, w.fork, 0));
-            if (w.cap.b[2]) launch_cut<CBig>(w, g, val, 2, grid[2], w.side);
-            if (w.cap.b[0]) launch_cut<CK2>(w, g, val, 0, grid[0], w.side);
-            RIN_HIP(hipEventRecord(w.join, w.side));
-            if (w.cap.b[1]) launch_cut<CMed>(w, g, val, 1, grid[1], s);
-            RIN_HIP(hipStreamWaitEvent(s, w.join, 0));
-            // extraction: the three passes write disjoint rows, so the k = 2
-            // record pass runs on the branch beside the other two
-            stage_rest(w, g, val, s, REST_SCANS);
-            // extraction: the three passes write disjoint rows; one launch
-            // runs them side by side
-            stage_rest(w, g, val, s, REST_EXTRACT_K1 | REST_EXTRACT_REC);
-            stage_rest(w, g, val, s, REST_DEDUP);
-            stage_rest(w, g, val, s, REST_XYZ | REST_FACES);
-            RIN_HIP(hipStreamEndCapture(s, &w.whole_graph));
-            RIN_HIP(hipGraphInstantiate(&w.whole, w.whole_graph, nullptr, nullptr, 0));
-            while (w.ev.size() < w.parts.size() + 1) {
-                hipEvent_t e;
-                RIN_HIP(hipEventCreate(&e));
-                w.ev.push_back(e);
-            }
             w.exec = true;
         }
+        // (re-)record the graphs for these values: a new workspace capacity
+        // drops them; new values of the same size re-record the whole graph
+        // and update the instantiated one in place
+        if (w.bound_whole != val.vals.p) record_whole(w, g, val, s);
+        if (staged && w.bound_parts != val.vals.p) record_parts(w, g, val, P, s);
         if (w.hash_dirty) {
             RIN_HIP(hipMemsetAsync(w.hkey.p, 0, 8ull << w.hbits, s));
             RIN_HIP(hipMemsetAsync(w.hmx.p, 0, 4ull << w.hbits, s));

@@ -66,6 +66,14 @@ __global__ void k_transpose_in(const double* __restrict__ rm, uint64_t V, uint32
     }
 }
 
+// function-major F x V -> row-major V x F (the reference's funcVals layout)
+__global__ void k_transpose_out(const double* __restrict__ vals, uint64_t V, uint32_t F, double* __restrict__ rm) {
+    GRID_STRIDE(i, V * F) {
+        const uint64_t v = i / F, f = i % F;
+        rm[i] = vals[f * V + v];
+    }
+}
+
 // ------------------------------------------------------------------ signs / filter
 
 __global__ void k_signs_ia(const double* __restrict__ vals, uint64_t V, uint32_t F, uint64_t* __restrict__ pos,

@@ -18,22 +18,27 @@ namespace rin {
 constexpr uint32_t kMaxFuncs = 64;   // function ids are packed in 6 bits
 constexpr uint32_t kMaxActive = 10;  // active functions per tet on the device path
 
+struct PlanCache;  // batched form: workspaces and captured graphs per function count (batched.hip)
+std::shared_ptr<PlanCache> make_plan_cache();
+
 struct Grid {
+    uint64_t id = 0;  // unique per grid made in this process (values record it)
     GridView view{};
     DBuf<double> pts;
     DBuf<uint32_t> tets, tedge, tface;
     // tet across each tet face (4 t + local face, or all ones on the mesh
     // boundary); built on the first material-interface run
     mutable DBuf<uint32_t> nbr;
+    // the batched form's plans live with the grid, so value sets of the same
+    // size share workspaces, tuning and captured graphs
+    std::shared_ptr<PlanCache> plans;
 };
 
-struct Plan;  // batched form: workspaces and captured graphs (batched.hip)
-
 struct Values {
+    uint64_t grid_id = 0;  // the grid the values were made on
     uint32_t F = 0;
     uint64_t V = 0;
     DBuf<double> vals;  // F x V, function-major
-    std::shared_ptr<Plan> plan[2];  // IA, MI
 };
 
 enum RunFlags : uint32_t {
@@ -109,6 +114,9 @@ void grid_explicit(DevMem& mem, const double* pts, uint64_t V, const uint64_t* t
                    hipStream_t s);
 void values_eval(DevMem& mem, const Grid& g, const FuncDesc* funcs, uint32_t F, Values& v, hipStream_t s);
 void values_upload(DevMem& mem, const Grid& g, const double* rowmajor, uint32_t F, Values& v, hipStream_t s);
+// function values at arbitrary points, row-major n x F, into host memory
+void eval_points(DevMem& mem, const double* pts, uint64_t n, const FuncDesc* funcs, uint32_t F, double* rowmajor,
+                 hipStream_t s);
 void run_pipeline(DevMem& mem, const Grid& g, const Values& v, uint32_t flags, RunOut& out, hipStream_t s);
 
 // thrown by the batched form when a tet needs more than its buckets hold
@@ -120,7 +128,7 @@ struct NeedsPlainForm : std::runtime_error {
 
 // batched form: buckets by active-function count, k=1 lookup table in LDS,
 // persistent per-bucket cut kernels, whole path captured in a hipGraph
-void run_batched(DevMem& ctx_mem, const Grid& g, Values& v, uint32_t flags, RunOut& out, hipStream_t s,
+void run_batched(DevMem& ctx_mem, const Grid& g, const Values& v, uint32_t flags, RunOut& out, hipStream_t s,
                  int n_cu);
 
 // the tet across every tet face (Grid::nbr), built on first use; synchronises
@@ -139,5 +147,12 @@ float debug_copy(DevMem& mem, uint64_t bytes, int reps, hipStream_t s);
 // diagnostics: run the device cut on independent plane/material sets
 void debug_cuts(DevMem& mem, int mi, const double* rows, const int* ks, int ncase, int cap, uint16_t* words,
                 int* lens, hipStream_t s);
+
+// diagnostics (host only, no device): the record a lookup table gives for one
+// tet — IA k = 1 or k = 2 plane rows (k_cube_fill's and k_k2_lookup's
+// indexing), or two material rows (k_mi_table2's); returns the record length
+// or a negative code (-1: no table entry, -2: record longer than cap)
+int debug_table_record_ia(int k, const double* rows, uint16_t* out, int cap);
+int debug_table_record_mi2(const double* rows, uint16_t* out, int cap);
 
 }  // namespace rin

@@ -12,6 +12,7 @@
 //      src/extract_mesh.cpp:10-265
 //   -> vertex records + xyz, src/extract_mesh.cpp:1446-1538
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -250,8 +251,14 @@ void extract_mi(DevMem& mem, const Grid& g, const Values& val, RunOut& out, Time
 
 // ------------------------------------------------------------------ API
 
+namespace {
+std::atomic<uint64_t> g_grid_ids{0};
+}
+
 void grid_regular(DevMem& mem, int res, const double bmin[3], const double bmax[3], Grid& g, hipStream_t s) {
     if (res < 1) throw std::invalid_argument("grid resolution must be >= 1");
+    g.id = ++g_grid_ids;
+    g.plans = make_plan_cache();
     const uint64_t N = (uint64_t)res + 1;
     g.view = GridView{};
     g.view.regular = 1;
@@ -274,6 +281,8 @@ void grid_regular(DevMem& mem, int res, const double bmin[3], const double bmax[
 void grid_explicit(DevMem& mem, const double* pts, uint64_t V, const uint64_t* tets, uint64_t T, Grid& g,
                    hipStream_t s) {
     if (V >= (1ull << 32)) throw std::invalid_argument("too many vertices");
+    g.id = ++g_grid_ids;
+    g.plans = make_plan_cache();
     g.view = GridView{};
     g.view.regular = 0;
     g.view.V = V;
@@ -347,6 +356,7 @@ void grid_explicit(DevMem& mem, const double* pts, uint64_t V, const uint64_t* t
 
 void values_eval(DevMem& mem, const Grid& g, const FuncDesc* funcs, uint32_t F, Values& v, hipStream_t s) {
     if (F == 0 || F > kMaxFuncs) throw std::invalid_argument("function count must be 1..64");
+    v.grid_id = g.id;
     v.F = F;
     v.V = g.view.V;
     v.vals.alloc(mem, (size_t)F * v.V);
@@ -358,8 +368,24 @@ void values_eval(DevMem& mem, const Grid& g, const FuncDesc* funcs, uint32_t F, 
     RIN_HIP(hipStreamSynchronize(s));
 }
 
+void eval_points(DevMem& mem, const double* pts, uint64_t n, const FuncDesc* funcs, uint32_t F, double* rowmajor,
+                 hipStream_t s) {
+    if (F == 0) throw std::invalid_argument("no functions");
+    if (n == 0) return;
+    DBuf<double> dp(mem, 3 * n), dv(mem, (size_t)F * n), drm(mem, (size_t)F * n);
+    DBuf<FuncDesc> fd(mem, F);
+    RIN_HIP(hipMemcpyAsync(dp.p, pts, 24 * n, hipMemcpyHostToDevice, s));
+    RIN_HIP(hipMemcpyAsync(fd.p, funcs, sizeof(FuncDesc) * F, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_eval, dim3(nblk(n)), dim3(256), 0, s, (const double*)dp.p, n, (const FuncDesc*)fd.p, F, dv.p);
+    hipLaunchKernelGGL(k_transpose_out, dim3(nblk(F * n)), dim3(256), 0, s, (const double*)dv.p, n, F, drm.p);
+    RIN_HIP(hipGetLastError());
+    RIN_HIP(hipMemcpyAsync(rowmajor, drm.p, 8 * (size_t)F * n, hipMemcpyDeviceToHost, s));
+    RIN_HIP(hipStreamSynchronize(s));
+}
+
 void values_upload(DevMem& mem, const Grid& g, const double* rowmajor, uint32_t F, Values& v, hipStream_t s) {
     if (F == 0 || F > kMaxFuncs) throw std::invalid_argument("function count must be 1..64");
+    v.grid_id = g.id;
     v.F = F;
     v.V = g.view.V;
     v.vals.alloc(mem, (size_t)F * v.V);
@@ -906,6 +932,19 @@ float debug_copy(DevMem& mem, uint64_t bytes, int reps, hipStream_t s) {
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     return ms / reps;
+}
+
+int debug_table_record_mi2(const double* rows, uint16_t* out, int cap) {
+    const MI2Table& T = mi2_table();
+    int pat = 0;
+    for (int c = 0, w = 1; c < 4; ++c, w *= 3) {
+        const double x = rows[c], y = rows[4 + c];
+        pat += (y > x ? 2 : (y < x ? 0 : 1)) * w;
+    }
+    const MI2Entry& E = T.e[pat];
+    if (E.len > cap) return -2;
+    std::memcpy(out, T.words.data() + E.off, 2 * (size_t)E.len);
+    return E.len;
 }
 
 void debug_cuts(DevMem& mem, int mi, const double* rows, const int* ks, int ncase, int cap, uint16_t* words, int* lens,

@@ -174,6 +174,13 @@ int rin_values_upload(rin_ctx* c, const rin_grid* g, const double* rm, uint32_t 
     });
 }
 
+int rin_eval_points(rin_ctx* c, const double* pts, uint64_t n, const rin_func_desc* f, uint32_t F, double* rm) {
+    if ((!pts && n) || !f || (!rm && n)) return fail(c, RIN_ERR_ARG, "null argument");
+    return guarded(c, [&] {
+        rin::eval_points(c->mem, pts, n, reinterpret_cast<const rin::FuncDesc*>(f), F, rm, c->stream);
+    });
+}
+
 int rin_values_download(rin_ctx* c, const rin_values* v, double* rm) {
     if (!v || !rm) return fail(c, RIN_ERR_ARG, "null argument");
     return guarded(c, [&] {
@@ -194,11 +201,12 @@ int rin_values_release(rin_ctx* c, rin_values* v) {
 int rin_run(rin_ctx* c, const rin_grid* g, const rin_values* v, uint32_t flags, rin_result** out) {
     if (!g || !v || !out) return fail(c, RIN_ERR_ARG, "null argument");
     *out = nullptr;
+    if (v->v.grid_id != g->g.id) return fail(c, RIN_ERR_ARG, "the values were made on another grid");
     return guarded(c, [&] {
         auto r = std::make_unique<rin_result>();
         if (flags & RIN_FLAG_BATCHED) {
             try {
-                rin::run_batched(c->mem, g->g, const_cast<rin_values*>(v)->v, flags, r->r, c->stream, c->n_cu);
+                rin::run_batched(c->mem, g->g, v->v, flags, r->r, c->stream, c->n_cu);
             } catch (rin::NeedsPlainForm& e) {
                 // the plain form handles every tet the buckets cannot
                 RIN_HIP(hipStreamSynchronize(c->stream));
@@ -345,6 +353,21 @@ int rin_debug_cuts(rin_ctx* c, int32_t mi, const double* rows, const int32_t* ks
                    uint16_t* words, int32_t* lens) {
     if (!rows || !ks || !words || !lens || n < 0 || cap <= 8) return fail(c, RIN_ERR_ARG, "bad argument");
     return guarded(c, [&] { rin::debug_cuts(c->mem, mi, rows, ks, n, cap, words, lens, c->stream); });
+}
+
+int rin_debug_table_cuts(int32_t mode, const double* rows, int32_t n, int32_t cap, uint16_t* words, int32_t* lens) {
+    if (!rows || !words || !lens || n < 0 || cap <= 8 || mode < 0 || mode > 2) return RIN_ERR_ARG;
+    try {
+        const int nr = mode == 0 ? 1 : 2;
+        for (int32_t i = 0; i < n; ++i) {
+            const double* r = rows + (size_t)4 * nr * i;
+            uint16_t* w = words + (size_t)cap * i;
+            lens[i] = mode == 2 ? rin::debug_table_record_mi2(r, w, cap) : rin::debug_table_record_ia(nr, r, w, cap);
+        }
+        return RIN_OK;
+    } catch (std::exception&) {
+        return RIN_ERR_INTERNAL;
+    }
 }
 
 int rin_debug_copy(rin_ctx* c, uint64_t bytes, int32_t reps, double* ms) {

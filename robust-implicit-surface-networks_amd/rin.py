@@ -83,6 +83,7 @@ def lib():
         L.rin_debug_cuts.argtypes = [_vp, ctypes.c_int32, _f64p, _i32p, ctypes.c_int32, ctypes.c_int32, _u16p,
                                      _i32p]
         L.rin_debug_copy.argtypes = [_vp, ctypes.c_uint64, ctypes.c_int32, ctypes.POINTER(ctypes.c_double)]
+        L.rin_debug_table_cuts.argtypes = [ctypes.c_int32, _f64p, ctypes.c_int32, ctypes.c_int32, _u16p, _i32p]
         _lib = L
     return _lib
 
@@ -99,6 +100,22 @@ def descs(types, params):
         for q in range(12):
             arr[j].params[q] = float(params[j][q])
     return arr
+
+
+def debug_table_cuts(mode, rows, cap=512):
+    """Host-only lookup of the tables (no device): mode 0 IA one-function
+    table, rows (n, 4); mode 1 IA two-function table, rows (n, 2, 4); mode 2
+    MI two-material table, rows (n, 2, 4).  Returns a list of uint16 records,
+    or negative codes where the table has no entry."""
+    nr = 1 if mode == 0 else 2
+    rows = np.ascontiguousarray(np.asarray(rows, dtype=np.float64).reshape(-1, nr * 4))
+    n = rows.shape[0]
+    words = np.zeros(n * cap, dtype=np.uint16)
+    lens = np.zeros(n, dtype=np.int32)
+    rc = lib().rin_debug_table_cuts(mode, _p(rows, _f64p), n, cap, _p(words, _u16p), _p(lens, _i32p))
+    if rc != 0:
+        raise RinError(rc, "rin_debug_table_cuts failed")
+    return [words[i * cap:i * cap + int(lens[i])].copy() if lens[i] > 0 else int(lens[i]) for i in range(n)]
 
 
 class Context:

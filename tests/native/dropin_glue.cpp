@@ -96,3 +96,9 @@ extern "C" const char* dropin_run(int mi, int robust, const double* pts, uint64_
     g_out = o.str();
     return g_out.c_str();
 }
+
+// the -R verdict with given per-tet counts (CPU; prints the reference's message)
+extern "C" int verdict_probe(int forward_ok, int reverse_ok, const uint64_t* a, const uint64_t* b, uint64_t n) {
+    std::vector<uint64_t> x(a, a + n), y(b, b + n);
+    return rin::robust_verdict(forward_ok != 0, reverse_ok != 0, x, y) ? 1 : 0;
+}

@@ -13,6 +13,7 @@ pytestmark = pytest.mark.gpu
 
 def _compare(dev, ref):
     s = dev.sizes
+    assert dev.num_degenerate == ref.num_degenerate
     assert dev.num_intersecting == ref.num_intersecting
     assert dev.buckets == ref.buckets
     fit, st = dev.csr()
@@ -293,4 +294,148 @@ def test_large_complexes_are_cut_again(ctx, mi, batched):
     _compare(dev, ref)
     dev.release()
     vals.release()
+    g.release()
+
+
+@pytest.mark.parametrize("form", ["batched", "plain"])
+def test_headline_workload_full_size(ctx, form):
+    """The benchmark workload itself: 101^3 vertices (5,000,000 tets), 20
+    functions (6 planes, 6 spheres, 4 cylinders, 4 tori), in both forms, bit
+    for bit against the oracle (counts, CSR, vertices and coordinates, faces)."""
+    import rin
+
+    res, fs = W.config2()
+    types, params = fs.arrays()
+    g = ctx.grid_regular(res)
+    vals = g.eval(types, params)
+    pts, tets = S.grid(res)
+    ref = S.OracleRun(0, pts, tets, S.evaluate(types, params, pts))
+    flags = rin.FLAG_BATCHED if form == "batched" else 0
+    for rep in range(2 if form == "batched" else 1):  # the capturing run, then a replay
+        dev = vals.run(flags)
+        _compare(dev, ref)
+        dev.release()
+    vals.release()
+    g.release()
+
+
+@pytest.mark.parametrize("mi", [0, 1])
+def test_failing_reference_mesh_hot_path(ctx, mi):
+    """tests/test_implicit_networks.cpp:734-812 (mesh_fail.json, 8-sphere.json):
+    the reference's graph stage fails there, the hot path's outputs must still
+    equal the oracle's."""
+    import rin
+
+    m = np.load(S.REF_TESTS + "/mesh_fail.npz")
+    pts, tets = m["pts"], m["tets"].astype(np.uint64)
+    types, params = S.load_ref_funcs("8-sphere.json")
+    hv = S.evaluate(types, params, pts)
+    g = ctx.grid_explicit(pts, tets)
+    vals = g.upload(hv)
+    ref = S.OracleRun(mi, pts, tets, hv)
+    for flags in ((rin.FLAG_MI,) if mi else (0, rin.FLAG_BATCHED)):
+        dev = vals.run(flags)
+        _compare(dev, ref)
+        dev.release()
+    vals.release()
+    g.release()
+
+
+def _record_counts(ref):
+    """(vertices, faces, cells) of every active tet from the oracle's records."""
+    out = np.zeros((len(ref.active), 3), dtype=np.uint64)
+    for a in range(len(ref.active)):
+        w = ref.words[ref.rec_start[a]:ref.rec_start[a + 1]]
+        out[a] = (w[2], w[3], w[4])
+    return out
+
+
+@pytest.mark.parametrize("mi,which", [(0, "config1"), (0, "config4"), (1, "config3_small"), (1, "config4")])
+@pytest.mark.parametrize("lookup", [True, False])
+def test_robust_check_counts_match_oracle(ctx, mi, which, lookup):
+    """-R (src/implicit_arrangement.cpp:136-243, src/material_interface.cpp:174-287):
+    the per-tet (vertex, face, cell) counts of the forward and the reversed
+    insertion order equal the oracle's cuts in the same orders, and the run
+    stops after the cut."""
+    import rin
+
+    if which == "config1":
+        res, fs = W.config1()
+    elif which == "config4":
+        res, fs = W.config4(shuffle_seed=44)
+        res = 16
+    else:
+        _, fs = W.config3()
+        res = 24
+    types, params = fs.arrays()
+    g = ctx.grid_regular(res)
+    vals = g.eval(types, params)
+    pts, tets = S.grid(res)
+    hv = S.evaluate(types, params, pts)
+    base = (rin.FLAG_MI if mi else 0) | (0 if lookup else rin.FLAG_NO_LOOKUP) | rin.FLAG_CUT_ONLY
+    for rev in (False, True):
+        ref = S.OracleRun(mi, pts, tets, hv, flags=2 if rev else 0)
+        dev = vals.run(base | (rin.FLAG_REVERSE if rev else 0))
+        assert dev.num_intersecting == ref.num_intersecting and dev.buckets == ref.buckets
+        assert dev.num_degenerate == ref.num_degenerate
+        assert dev.sizes.n_verts == 0 and dev.sizes.n_faces == 0
+        np.testing.assert_array_equal(dev.cut_counts(), _record_counts(ref))
+        dev.release()
+    vals.release()
+    g.release()
+
+
+def test_values_from_another_grid_are_rejected(ctx):
+    """Values carry the grid they were made on; a run on another grid of the
+    same vertex count is an argument error, never a replay of the first
+    grid's graph."""
+    import rin
+
+    m = np.load(S.REF_TESTS + "/mesh.npz")
+    pts, tets = m["pts"], m["tets"].astype(np.uint64)
+    types, params = S.load_ref_funcs("8-sphere.json")
+    hv = S.evaluate(types, params, pts)
+    g1 = ctx.grid_explicit(pts, tets)
+    g2 = ctx.grid_explicit(pts, tets[::-1].copy())  # same V, other tets
+    v1 = g1.upload(hv)
+    v1.run(rin.FLAG_BATCHED).release()  # captures g1's graph
+    with pytest.raises(rin.RinError) as e:
+        v1.grid = g2
+        v1.run(rin.FLAG_BATCHED)
+    assert e.value.code == -1
+    v1.grid = g1
+    v2 = g2.upload(hv)
+    ref2 = S.OracleRun(0, pts, tets[::-1].copy(), hv)
+    dev = v2.run(rin.FLAG_BATCHED)
+    _compare(dev, ref2)
+    dev.release()
+    for o in (v1, v2, g1, g2):
+        o.release()
+
+
+def test_value_sets_share_the_grid_plan(ctx):
+    """Several value sets of the same size on one grid share the batched
+    form's workspace and graph: switching sets re-records the graph and
+    updates it in place; every run matches the oracle for its own values."""
+    import rin
+
+    res = 40
+    g = ctx.grid_regular(res)
+    pts, tets = S.grid(res)
+    sets = []
+    for seed in (21, 22, 23):
+        fs = W.mixed(np.random.default_rng(seed), 4, 4, 2, 2)
+        types, params = fs.arrays()
+        vals = g.eval(types, params)
+        sets.append((vals, S.OracleRun(0, pts, tets, S.evaluate(types, params, pts))))
+    for i in (0, 1, 2, 1, 0, 2):
+        vals, ref = sets[i]
+        dev = vals.run(rin.FLAG_BATCHED)
+        _compare(dev, ref)
+        dev.release()
+    st = sets[2][0].run(rin.FLAG_BATCHED | rin.FLAG_STAGE_TIMES)
+    _compare(st, sets[2][1])
+    st.release()
+    for vals, _ in sets:
+        vals.release()
     g.release()

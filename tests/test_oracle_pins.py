@@ -1,0 +1,454 @@
+"""Pins of the oracle's stages that the reference's known answers leave open,
+each by a property that a plausible mistake (a dropped term, a wrong sign, a
+swapped index, a wrong dedup key) would break, computed independently of how
+the oracle computes it:
+
+* function values: closed forms at points where the value is exact;
+* generate_tet_mesh (src/io.cpp:95-152): positive volumes, five tets per cube
+  filling it, every interior face shared by exactly two tets;
+* the IA filter (src/implicit_arrangement.cpp:82-116): a function is listed
+  iff its linear interpolant has a zero on the closed tet (min <= 0 <= max),
+  on a dyadic grid with exact zeros;
+* the MI filter (src/material_interface.cpp:94-152): every material that is
+  the maximum anywhere in the tet is listed (dense barycentric sampling), and
+  an unlisted tet has one maximum material throughout;
+* iso-vertex coordinates (src/extract_mesh.cpp:1446-1637): every vertex lies
+  in its simplex and on the zero sets of its functions' interpolants (MI:
+  where its materials tie and are maximal);
+* dedup (src/extract_mesh.cpp:95-226): vertex keys are unique, and a vertex on
+  a tet edge or face is used by the faces of every tet around that simplex;
+* order invariance: the canonical topology does not depend on the function
+  order (the -R property of src/implicit_arrangement.cpp:136-243);
+* the lookup tables (the device's and the oracle's) give the general cut's
+  record for every sign case;
+* the reference's [!shouldfail] material-interface case
+  (tests/test_implicit_networks.cpp:734-812) fails in the graph stage with
+  the reference's message.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import support as S
+import workloads as W
+
+NONE = np.iinfo(np.uint64).max
+
+
+# ------------------------------------------------------------------ eval
+
+
+def test_eval_closed_forms():
+    c = np.array([0.25, -0.5, 0.125])
+    fs = W.FuncSet()
+    fs.sphere(c, 0.375)                              # 0
+    fs.sphere(c, 0.375, squared=True)                # 1
+    fs.plane([0.5, 0.25, -0.75], [0, 0, 1])          # 2
+    fs.plane([0.5, 0.25, -0.75], [0.6, 0.8, 0.0])    # 3
+    fs.cylinder(c, [0, 0, 1], 0.3)                   # 4
+    fs.cylinder(c, [0, 0, 1], 0.3, flipped=True)     # 5
+    fs.torus(c, [0, 0, 1], 0.5, 0.125)               # 6
+    fs.torus(c, [0, 1, 0], 0.5, 0.125, flipped=True)  # 7
+    types, params = fs.arrays()
+    pts = np.array([
+        c,                                   # 0 centre
+        c + [0.5, 0, 0],                     # 1
+        c + [0, 0.375, 0],                   # 2 on the sphere
+        [0.5, 0.25, -0.75],                  # 3 the planes' point
+        [3.0, -7.0, -0.75 + 0.3125],         # 4 plane 2 at height 0.3125
+        [0.5 + 0.6 * 2, 0.25 + 0.8 * 2, 9.0],  # 5 plane 3 at distance 2
+        c + [0, 0, 7.0],                     # 6 on the cylinder axis
+        c + [0.5, 0, 0.125],                 # 7 on the torus surface (axis z)
+        c + [0.625, 0, 0],                   # 8 on the torus surface (axis z)
+        c + [0, 0, -0.5],                    # 9 on the y-axis torus core circle
+    ])
+    v = S.evaluate(types, params, pts)
+    # spheres are positive inside: r - |x - c| and r^2 - |x - c|^2
+    assert v[0, 0] == 0.375 and v[1, 0] == -0.125 and v[2, 0] == 0.0
+    assert v[0, 1] == 0.140625 and v[2, 1] == 0.0 and v[1, 1] == 0.140625 - 0.25
+    # planes are (x - p) . n
+    assert v[3, 2] == 0.0 and v[4, 2] == 0.3125 and v[3, 3] == 0.0
+    assert abs(v[5, 3] - 2.0) < 1e-15
+    assert v[0, 2] == c[2] + 0.75
+    # cylinders: r - distance to the axis, negated when flipped
+    assert v[6, 4] == 0.3 and v[0, 4] == 0.3 and v[6, 5] == -0.3
+    assert abs(v[1, 4] - (0.3 - 0.5)) < 1e-15 and abs(v[1, 5] - (0.5 - 0.3)) < 1e-15
+    # tori: minor radius on the core circle, r - R at the centre, 0 on the surface
+    assert v[1, 6] == 0.125 and v[0, 6] == 0.125 - 0.5
+    assert v[7, 6] == 0.0 and v[8, 6] == 0.0
+    assert v[9, 7] == -0.125 and v[0, 7] == -(0.125 - 0.5)
+
+
+def test_eval_matches_geometry_at_random_points():
+    """Distances computed by numpy's own routines (norm, cross products)
+    agree with the oracle's formulas to rounding."""
+    rng = np.random.default_rng(7)
+    fs = W.mixed(rng, 3, 3, 3, 3)
+    types, params = fs.arrays()
+    pts = rng.uniform(-1, 1, (500, 3))
+    v = S.evaluate(types, params, pts)
+    for j, (t, p) in enumerate(zip(types, params)):
+        d = pts - p[0:3]
+        if t == S.PLANE:
+            want = d @ p[3:6]
+        elif t == S.SPHERE:
+            want = p[3] - np.linalg.norm(d, axis=1)
+        elif t == S.CYLINDER:
+            want = p[6] - np.linalg.norm(np.cross(d, p[3:6]), axis=1)
+        else:
+            h = d @ p[3:6]
+            rho = np.linalg.norm(np.cross(d, p[3:6]), axis=1)
+            want = p[7] - np.hypot(rho - p[6], h)
+        np.testing.assert_allclose(v[:, j], want, rtol=0, atol=1e-14)
+
+
+# ------------------------------------------------------------------ grid
+
+
+@pytest.mark.parametrize("res,bmin,bmax", [(1, (-1, -1, -1), (1, 1, 1)), (2, (-1, -1, -1), (1, 1, 1)),
+                                           (3, (0.0, -2.0, 0.5), (1.0, 2.0, 0.75)), (4, (-1, -1, -1), (1, 1, 1))])
+def test_grid_invariants(res, bmin, bmax):
+    pts, tets = S.grid(res, bmin, bmax)
+    N = res + 1
+    assert pts.shape == (N ** 3, 3) and tets.shape == (5 * res ** 3, 4)
+    np.testing.assert_array_equal(pts[0], bmin)
+    np.testing.assert_array_equal(pts[-1], bmax)
+    # x-major: the last index (z) varies fastest
+    assert np.all(np.diff(pts[:N, 2]) > 0) and np.all(pts[:N, :2] == pts[0, :2])
+    t = tets.astype(np.int64)
+    P = pts[t]
+    vol = np.linalg.det(P[:, 1:] - P[:, :1]) / 6.0
+    assert np.all(vol > 0)
+    cube = np.prod(np.asarray(bmax, float) - np.asarray(bmin, float)) / res ** 3
+    # five tets per cube, inside that cube, filling it
+    lo = P.min(axis=1).reshape(-1, 5, 3)
+    hi = P.max(axis=1).reshape(-1, 5, 3)
+    assert np.all(lo == lo[:, :1]) and np.all(hi == hi[:, :1])
+    np.testing.assert_allclose(vol.reshape(-1, 5).sum(axis=1), cube, rtol=1e-12)
+    # conformity: interior faces shared by exactly two tets, boundary faces on the box
+    faces = {}
+    for i, tt in enumerate(t):
+        for c in range(4):
+            key = tuple(sorted(np.delete(tt, c)))
+            faces.setdefault(key, []).append(i)
+    for key, owners in faces.items():
+        assert len(owners) <= 2
+        if len(owners) == 1:
+            F = pts[list(key)]
+            on_box = [np.all(F[:, a] == bmin[a]) or np.all(F[:, a] == bmax[a]) for a in range(3)]
+            assert any(on_box), key
+    n_bnd = sum(len(o) == 1 for o in faces.values())
+    assert n_bnd == 6 * 2 * res * res
+
+
+# ------------------------------------------------------------------ filters
+
+
+def _dyadic_funcs():
+    fs = W.FuncSet()
+    fs.plane([0.0, 0, 0], [1, 0, 0])            # zeros on a grid plane
+    fs.plane([0.5, 0, 0], [1, 1, 0])            # x + y = 0.5 through grid vertices
+    fs.plane([0.0, 0.0, -0.5], [1, -2, 4])      # integer normal, zeros at some vertices
+    fs.sphere([0.0, 0.5, 0.0], 0.5, squared=True)  # r^2 - |x - c|^2, exact at dyadic points
+    fs.sphere([0.25, 0.25, 0.25], 0.75)
+    fs.copy_of(0)
+    return fs
+
+
+def test_ia_filter_brute_force_with_exact_zeros():
+    res = 4  # spacing 1/2 on [-1, 1]^3: every coordinate and value above is exact
+    pts, tets = S.grid(res)
+    types, params = _dyadic_funcs().arrays()
+    vals = S.evaluate(types, params, pts)
+    assert np.count_nonzero(vals == 0) > 20
+    r = S.OracleRun(0, pts, tets, vals)
+    V = vals[tets.astype(np.int64)]          # T x 4 x F
+    active = (V.min(axis=1) <= 0) & (V.max(axis=1) >= 0)
+    want_start = np.concatenate([[0], np.cumsum(active.sum(axis=1))]).astype(np.uint64)
+    want_fit = np.nonzero(active)[1].astype(np.uint64)
+    np.testing.assert_array_equal(r.start_index, want_start)
+    np.testing.assert_array_equal(r.func_in_tet, want_fit)
+    assert r.num_degenerate == np.count_nonzero(vals == 0)
+    assert r.num_intersecting == np.count_nonzero(active.any(axis=1))
+    k = active.sum(axis=1)
+    assert r.buckets == [np.count_nonzero(k == 1), np.count_nonzero(k == 2), np.count_nonzero(k >= 3)]
+
+
+def test_mi_filter_lists_every_maximum_material():
+    res = 3
+    pts, tets = S.grid(res)
+    rng = np.random.default_rng(12)
+    fs = W.FuncSet()
+    for _ in range(6):
+        fs.sphere(rng.uniform(-0.8, 0.8, 3), rng.uniform(0.0, 0.2))
+    types, params = fs.arrays()
+    vals = S.evaluate(types, params, pts)
+    vals[::7, 2] = vals[::7, 4]  # ties at some vertices
+    r = S.OracleRun(1, pts, tets, vals)
+    mx = vals.max(axis=1, keepdims=True)
+    assert r.num_degenerate == np.count_nonzero((vals == mx).sum(axis=1) > 1)
+    # barycentric lattice of step 1/8 on each tet
+    lam = np.array([(a, b, c, 8 - a - b - c) for a in range(9) for b in range(9 - a) for c in range(9 - a - b)],
+                   dtype=np.float64) / 8.0
+    for t, tt in enumerate(tets.astype(np.int64)):
+        listed = set(int(x) for x in r.func_in_tet[r.start_index[t]:r.start_index[t + 1]])
+        pl = lam @ vals[tt]                       # samples x materials
+        m = pl.max(axis=1, keepdims=True)
+        maxima = set(np.nonzero((pl >= m - 1e-12).any(axis=0))[0].tolist())
+        if listed:
+            assert len(listed) >= 2 and maxima <= listed, (t, maxima, listed)
+        else:
+            assert len(maxima) == 1, (t, maxima)
+
+
+# ------------------------------------------------------------------ coordinates and dedup
+
+
+def _bary(pts, sv, x):
+    """Barycentric coordinates of x over the simplex sv (least squares; exact
+    for affinely independent vertices)."""
+    A = np.vstack([pts[sv].T, np.ones(len(sv))])
+    lam, *_ = np.linalg.lstsq(A, np.append(x, 1.0), rcond=None)
+    return lam
+
+
+def _check_xyz(mi, pts, vals, r):
+    scale = np.abs(vals).max()
+    for i, rec in enumerate(r.verts):
+        ss = int(rec[2])
+        sv = [int(x) for x in rec[3:3 + ss]]
+        fn = [int(x) for x in rec[7:11] if x != NONE]
+        x = r.xyz[i]
+        if ss == 1:
+            assert np.array_equal(x, pts[sv[0]])
+        lam = _bary(pts, sv, x)
+        assert np.all(lam >= -1e-12) and abs(lam.sum() - 1) < 1e-12, (i, lam)
+        np.testing.assert_allclose(pts[sv].T @ lam, x, rtol=0, atol=1e-13)
+        pl = lam @ vals[sv]                       # interpolated values at x
+        if not mi:
+            assert np.all(np.abs(pl[fn]) <= 1e-12 * scale), (i, pl[fn])
+        else:
+            assert np.ptp(pl[fn]) <= 1e-12 * scale, (i, pl[fn])
+            assert pl.max() <= pl[fn].max() + 1e-12 * scale
+
+
+@pytest.mark.parametrize("which", ["config1", "config4", "config3_small"])
+def test_vertices_lie_on_their_zero_sets(which):
+    if which == "config1":
+        res, fs = W.config1()
+        mi = 0
+    elif which == "config4":
+        res, fs = W.config4()
+        res = 12
+        mi = 0
+    else:
+        _, fs = W.config3()
+        res, mi = 16, 1
+    types, params = fs.arrays()
+    pts, tets = S.grid(res)
+    vals = S.evaluate(types, params, pts)
+    r = S.OracleRun(mi, pts, tets, vals)
+    assert len(r.verts) > 100
+    _check_xyz(mi, pts, vals, r)
+
+
+def _vertex_tets(r):
+    """Tets whose iso-faces use each vertex."""
+    use = [set() for _ in range(len(r.verts))]
+    for f in range(len(r.face_start) - 1):
+        owners = {int(t) for t in r.tf[r.tf_start[f]:r.tf_start[f + 1], 0]}
+        for v in r.face_verts[r.face_start[f]:r.face_start[f + 1]]:
+            use[int(v)] |= owners
+    return use
+
+
+@pytest.mark.parametrize("mi", [0, 1])
+def test_dedup_shares_vertices_across_each_star(mi):
+    if mi:
+        _, fs = W.config3()
+        res = 12
+    else:
+        res, fs = W.config1()
+    types, params = fs.arrays()
+    pts, tets = S.grid(res)
+    vals = S.evaluate(types, params, pts)
+    r = S.OracleRun(mi, pts, tets, vals)
+    keys = {(int(v[2]), tuple(sorted(int(x) for x in v[3:3 + int(v[2])])), tuple(sorted(int(x) for x in v[7:11])))
+            for v in r.verts}
+    assert len(keys) == len(r.verts)  # no two vertices share a key
+    tsets = [set(map(int, tt)) for tt in tets]
+    by_vertex = {}
+    for t, ts in enumerate(tsets):
+        for v in ts:
+            by_vertex.setdefault(v, []).append(t)
+    use = _vertex_tets(r)
+    n_checked = 0
+    for i, v in enumerate(r.verts):
+        ss = int(v[2])
+        if ss not in (2, 3):
+            continue
+        sv = [int(x) for x in v[3:3 + ss]]
+        star = {t for t in by_vertex[sv[0]] if all(x in tsets[t] for x in sv)}
+        assert use[i] == star, (i, ss, sv, use[i], star)
+        n_checked += 1
+    assert n_checked > 100
+
+
+# ------------------------------------------------------------------ order invariance
+
+
+def _groups(vals):
+    """Smallest function id with the same zero set (a proportional column of
+    values), per function."""
+    F = vals.shape[1]
+    rep = list(range(F))
+    for a in range(F):
+        for b in range(a):
+            za, zb = vals[:, a] == 0, vals[:, b] == 0
+            if not np.array_equal(za, zb) or za.all():
+                continue
+            q = vals[~za, a] / vals[~za, b]
+            if np.ptp(q) <= 1e-12 * abs(q[0]):
+                rep[a] = rep[b]
+                break
+    return rep
+
+
+def _canonical(mi, r, order, rep, vals, pts):
+    """Vertex keys, coordinates by key, and faces as (rotated vertex-key
+    cycle, function labels), all in original function ids.  A vertex's key is
+    its simplex and the set of functions vanishing there (MI: the materials
+    tied for the maximum there): which of several functions through one point
+    names the vertex follows the insertion order, the point does not."""
+    scale = np.abs(vals).max()
+    vk = []
+    xyz = {}
+    for i, v in enumerate(r.verts):
+        ss = int(v[2])
+        sv = [int(x) for x in v[3:3 + ss]]
+        pl = _bary(pts, sv, r.xyz[i]) @ vals[sv]
+        on = np.abs(pl) <= 1e-12 * scale if not mi else pl >= pl.max() - 1e-12 * scale
+        k = (ss, tuple(sorted(sv)), tuple(sorted({rep[f] for f in np.nonzero(on)[0]})))
+        vk.append(k)
+        xyz[k] = r.xyz[i]
+    faces = set()
+    for f in range(len(r.face_start) - 1):
+        cyc = [vk[int(x)] for x in r.face_verts[r.face_start[f]:r.face_start[f + 1]]]
+        lab = tuple(rep[order[int(x)]] if x != NONE else -1 for x in r.face_func[f])
+        if mi and lab[0] > lab[1]:
+            # an MI face faces its positive material; which of the two is the
+            # positive one follows the local order, so orient by the smaller
+            cyc, lab = cyc[::-1], lab[::-1]
+        m = cyc.index(min(cyc))
+        cyc = tuple(cyc[m:] + cyc[:m])
+        faces.add((cyc, lab if mi else lab[:1]))
+    return set(vk), xyz, faces, len(vk)
+
+
+@pytest.mark.parametrize("mi", [0, 1])
+@pytest.mark.parametrize("which", ["config1", "config4"])
+def test_topology_is_independent_of_function_order(mi, which):
+    if which == "config1":
+        res, fs = W.config1()
+    else:
+        res, fs = W.config4()
+        res = 16
+    if mi and which == "config1":
+        _, fs = W.config3()
+        res = 14
+    types, params = fs.arrays()
+    pts, tets = S.grid(res)
+    vals = S.evaluate(types, params, pts)
+    rep = _groups(vals)
+    F = vals.shape[1]
+    base = _canonical(mi, S.OracleRun(mi, pts, tets, vals), list(range(F)), rep, vals, pts)
+    assert len(base[0]) == base[3]  # keys stay unique
+    for seed in (1, 2):
+        order = list(np.random.default_rng(seed).permutation(F))
+        got = _canonical(mi, S.OracleRun(mi, pts, tets, vals[:, order]), order, rep, vals, pts)
+        assert got[3] == base[3]
+        assert got[0] == base[0]
+        assert got[2] == base[2]
+        for k, x in base[1].items():
+            np.testing.assert_allclose(got[1][k], x, rtol=0, atol=1e-12)
+
+
+# ------------------------------------------------------------------ lookup tables
+
+
+def _active(row):
+    return (row > 0).sum() < 4 and (row < 0).sum() < 4
+
+
+def _table_cases(rng, n, nrows):
+    cats = [
+        rng.standard_normal((n, nrows, 4)),                                    # generic
+        rng.integers(-2, 3, (n, nrows, 4)).astype(np.float64),                 # small integers, zeros
+        rng.choice([-1.0, -0.5, -0.25, 0.0, 0.25, 0.5, 1.0], (n, nrows, 4)),   # dyadic, many ties
+        rng.standard_normal((n, nrows, 4)) * np.exp2(rng.integers(-60, 61, (n, nrows, 1))),  # scales
+    ]
+    if nrows == 2:
+        base = rng.standard_normal((n, 1, 4))
+        k = rng.choice([1.0, -1.0, 2.0, 0.5], (n, 1, 1))
+        cats.append(np.concatenate([base, base * k], axis=1))                  # coincident planes
+    return np.concatenate(cats)
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_lookup_tables_equal_the_general_cut(mode):
+    """The device's tables (read here on the host with the lookup kernels'
+    indexing) give, for every sign case, the oracle's general cut record."""
+    import rin
+
+    rng = np.random.default_rng(40 + mode)
+    nrows = 1 if mode == 0 else 2
+    rows = _table_cases(rng, 4000, nrows)
+    if mode < 2:
+        rows = np.array([r for r in rows if all(_active(x) for x in r)])
+    out = rin.debug_table_cuts(mode, rows)
+    n_exact = 0
+    for r, got in zip(rows, out):
+        want = S.oracle_cut(1 if mode == 2 else 0, r)
+        assert not isinstance(got, int), (r, got)
+        np.testing.assert_array_equal(got, want, err_msg=str(r))
+        n_exact += 1
+    assert n_exact > 10000
+
+
+@pytest.mark.parametrize("mi,which", [(0, "config1"), (0, "config4"), (1, "config3_small"), (1, "config4")])
+def test_oracle_lookup_path_equals_general_cut(mi, which):
+    if which == "config1":
+        res, fs = W.config1()
+    elif which == "config4":
+        res, fs = W.config4()
+        res = 16
+    else:
+        _, fs = W.config3()
+        res = 20
+    types, params = fs.arrays()
+    pts, tets = S.grid(res)
+    vals = S.evaluate(types, params, pts)
+    a = S.OracleRun(mi, pts, tets, vals)
+    b = S.OracleRun(mi, pts, tets, vals, flags=4)  # F_LOOKUP
+    np.testing.assert_array_equal(a.words, b.words)
+    np.testing.assert_array_equal(a.verts, b.verts)
+    np.testing.assert_array_equal(a.face_verts, b.face_verts)
+
+
+# ------------------------------------------------------------------ the reference's failing case
+
+
+def test_mesh_fail_fails_in_the_graph_stage():
+    """tests/test_implicit_networks.cpp:734-812 expects material_interface on
+    mesh_fail.json with 8-sphere.json to fail; the reference's compute_face_order
+    throws at src/pair_faces.cpp:131-134 and material_interface returns false
+    with "order patches failed: ..." (src/material_interface.cpp:562-564)."""
+    m = np.load(os.path.join(S.REF_TESTS, "mesh_fail.npz"))
+    pts, tets = m["pts"], m["tets"].astype(np.uint64)
+    types, params = S.load_ref_funcs("8-sphere.json")
+    vals = S.evaluate(types, params, pts)
+    out = S.known_answer_run(1, pts, tets, vals)
+    assert not out["ok"]
+    assert out["error"] == ("order patches failed: More than one tets containing the iso-faces incident to "
+                            "the iso-edge.")
