@@ -161,7 +161,8 @@ int rin_debug_cuts(rin_ctx* ctx, int32_t mi, const double* rows, const int32_t* 
 /* Test hook with no counterpart in the reference, host only (no context, no
  * device): the per-tet record the lookup tables give for independent tets.
  * mode 0: IA one-function table (1 plane row per case), 1: IA two-function
- * table (2 rows), 2: MI two-material table (2 material rows).  Each case
+ * table (2 rows), 2: MI two-material table (2 material rows), 3: MI
+ * three-material table (3 material rows).  Each case
  * writes at most cap words at words + i*cap and its length to lens[i]
  * (-1: the table has no entry for it, -2: longer than cap). */
 int rin_debug_table_cuts(int32_t mode, const double* rows, int32_t n_cases, int32_t cap, uint16_t* words,

@@ -764,6 +764,7 @@ struct CutArgs {
     uint32_t *cnt, *errs;
     uint32_t* ctr;       // for the work statistics (spread counters)
     int stat_k, stat_w;  // spread counters of this bucket's functions and words (-1: none)
+    uint8_t* ws;         // thread form: one LaneCut slot per lane of the grid
 };
 
 // persistent per-bucket cut: each wave takes 64 bucket entries at a time
@@ -790,8 +791,9 @@ __global__ void __launch_bounds__(64) k_cut_bucket(CutArgs A) {
             const uint32_t f = A.fit[s0 + q];
             for (int c = 0; c < 4; ++c) P[q][c] = A.vals[(uint64_t)f * V + v[c]];
         }
-        dev::Complex<C> X;
-        dev::ia_cut<C>(P, k, X);
+        const uint64_t slot = (uint64_t)blockIdx.x * 64 + lane;
+        dev::Complex<C>& X = LaneCut<C>::X(A.ws, slot);
+        dev::ia_cut<C>(P, k, X, LaneCut<C>::T(A.ws, slot));
         uint16_t* out = A.rec + A.rec_loc[r];
         if (X.err) {
             A.errs[lane] |= 1;
@@ -1404,6 +1406,7 @@ struct Workspace {
     DBuf<uint32_t> k2plan_off;
     DBuf<uint8_t> k2plan;
     DBuf<uint32_t> k2cnt;
+    DBuf<uint8_t> ws_thread[3];  // thread-form cut state per bucket (LaneCut slots)
     // the path as a few captured graphs launched back to back; stream events
     // between them time each part on every replay
     std::vector<std::pair<std::string, hipGraph_t>> graphs;
@@ -1721,6 +1724,11 @@ void launch_cut(Workspace& W, const Grid& g, const Values& val, int q, int grid,
         return;
     }
     fill_u32(W.ctr.p + kQueueBase + q, 0, 1, s);
+    // the complexes of the thread form live in the workspace (grown outside
+    // capture: the autotuner launches every candidate eagerly first)
+    const size_t need = LaneCut<C>::stride * 64 * (size_t)grid;
+    if (W.ws_thread[q].n < need) W.ws_thread[q].alloc(W.mem, need);
+    A.ws = W.ws_thread[q].p;
     hipLaunchKernelGGL(k_cut_bucket<C>, dim3(grid), dim3(64), 0, s, A);
 }
 
@@ -2234,16 +2242,16 @@ void run_batched(DevMem& ctx_mem, const Grid& g, const Values& val, uint32_t fla
                 const bool only_table = form && !std::strcmp(form, "table");
                 for (int q = 0; q < 3; ++q) {
                     float best = 1e30f;
-                    int bestc = only_wave ? -1 : (only_table && q == 0 ? -2 : 4);
+                    int bestc = only_wave ? -1 : (only_table && q == 0 ? -2 : (q == 2 ? 1 : 4));
                     for (int ci = 0; ci < 7; ++ci) {
                         const int c = cand[ci];
                         if (c == -2 && (q != 0 || only_wave || only_thread)) continue;
                         if (only_table && q == 0 && c != -2) continue;
                         if ((only_wave && c >= 0) || (only_thread && c < 0)) continue;
-                        // thread forms keep the complex in private memory: the
-                        // runtime sizes scratch by the dispatch's waves, so the
-                        // larger complexes get smaller grids
-                        if (c > (q == 2 ? 2 : (q == 1 ? 4 : 16))) continue;
+                        // thread forms keep one complex per lane in the
+                        // workspace: the larger complexes get smaller grids,
+                        // and k = 5..8 (about 9 KB a lane) only the wave form
+                        if (c > (q == 2 ? 0 : (q == 1 ? 4 : 8))) continue;
                         const int grid = c < 0 ? c : std::max(1, n_cu * c);
                         float ms = time_ms(s, [&] {
                             if (q == 0 && w.cap.b[0]) launch_cut<CK2>(w, g, val, 0, grid, s);

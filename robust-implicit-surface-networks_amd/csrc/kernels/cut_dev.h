@@ -57,6 +57,21 @@ struct Complex {
     I grp[C::MP], ori[C::MP];
 };
 
+// The insertion's working arrays that scale with the capacities (the rest
+// are a few dozen bytes).  Kernels keep them, like the complex, in LDS or in
+// device memory, so the per-lane private segment stays small.
+template <class C>
+struct InsertTemps {
+    using I = typename C::Idx;
+    static constexpr int PE = 4 * C::MFV * 4;  // MI: tet-face edges won by the new material
+    int8_t s[C::MV];                           // sign of the new plane / material at each vertex
+    I cu[C::MV], cw[C::MV], cx[C::MV];         // crossing edges (u < w) and their new vertex
+    I pp[PE], pa[PE], pb[PE], pl[PE];          // (tet face, from, to, edge label)
+    I remap[C::MV];                            // vertex renumbering after an MI insertion
+    bool used[C::MV];                          // vertices still referenced
+    bool cpos[C::MP], cneg[C::MP], consumed[C::MP], split[C::MP];
+};
+
 template <class T>
 __host__ __device__ inline void sort3(T& a, T& b, T& c) {
     T t;
@@ -117,7 +132,7 @@ __host__ __device__ void ia_classify(const double (*P)[4], int j, Complex<C>& X,
 // inserts plane Pid given its sign s[v] at every vertex; the result depends
 // on the complex and the signs only
 template <class C>
-__host__ __device__ void ia_insert(Complex<C>& X, typename C::Idx Pid, int8_t* s) {
+__host__ __device__ void ia_insert(Complex<C>& X, typename C::Idx Pid, int8_t* s, InsertTemps<C>& T) {
     using I = typename C::Idx;
     {
         bool any_pos = false, any_neg = false;
@@ -153,7 +168,9 @@ __host__ __device__ void ia_insert(Complex<C>& X, typename C::Idx Pid, int8_t* s
         if (!any_pos || !any_neg) return;
     }
     // one new vertex per edge with a strict sign change
-    I cu[C::MV], cw[C::MV], cx[C::MV];
+    I* cu = T.cu;
+    I* cw = T.cw;
+    I* cx = T.cx;
     int ncross = 0;
     for (int f = 0; f < X.nf; ++f) {
         const int n = X.fn[f];
@@ -329,16 +346,27 @@ __host__ __device__ void ia_insert(Complex<C>& X, typename C::Idx Pid, int8_t* s
 }
 
 template <class C>
-__host__ __device__ void ia_cut(const double (*P)[4], int k, Complex<C>& X) {
+__host__ __device__ void ia_cut(const double (*P)[4], int k, Complex<C>& X, InsertTemps<C>& T) {
     ia_init(X, k);
     if (X.err) return;
-    int8_t s[C::MV];
     for (int j = 0; j < k; ++j) {
-        ia_classify(P, j, X, s);
+        ia_classify(P, j, X, T.s);
         if (X.err) return;
-        ia_insert(X, (typename C::Idx)(4 + j), s);
+        ia_insert(X, (typename C::Idx)(4 + j), T.s, T);
         if (X.err) return;
     }
+}
+
+// host forms (the table builders): per-thread working arrays
+template <class C>
+void ia_insert(Complex<C>& X, typename C::Idx Pid, int8_t* s) {
+    static thread_local InsertTemps<C> T;
+    ia_insert(X, Pid, s, T);
+}
+template <class C>
+void ia_cut(const double (*P)[4], int k, Complex<C>& X) {
+    static thread_local InsertTemps<C> T;
+    ia_cut(P, k, X, T);
 }
 
 // record layout: include/rin/cut_record.h; returns words written or -1
@@ -403,9 +431,10 @@ __host__ __device__ inline int chain_edges(const T* ea, const T* eb, const T* el
     return cnt == ne ? cnt : -1;
 }
 
+// the tet filled with material 4 (the first of k materials); err when k
+// exceeds the label capacity
 template <class C>
-__host__ __device__ void mi_cut(const double (*P)[4], int k, Complex<C>& X) {
-    using I = typename C::Idx;
+__host__ __device__ void mi_init(Complex<C>& X, int k) {
     X.err = false;
     X.dup = false;
     for (int p = 0; p < C::MP; ++p) X.grp[p] = p < 4 ? p : C::NIL;
@@ -436,325 +465,361 @@ __host__ __device__ void mi_cut(const double (*P)[4], int k, Complex<C>& X) {
     X.nc = 1;
     X.cells[0] = 4;
     X.grp[4] = 4;
-    int8_t s[C::MV];
-    constexpr int PE = 4 * C::MFV * 4;
-    for (int j = 1; j < k; ++j) {
-        const I m = 4 + j;
-        const int nv0 = X.nv;
-        bool any_pos = false;
-        for (int v = 0; v < nv0; ++v) {
-            int lab[4] = {X.vl[v][0], X.vl[v][1], X.vl[v][2], X.vl[v][3]};
-            int sv = orient_mi(P, lab, P[j], X.err);
-            s[v] = (int8_t)sv;
-            any_pos |= sv > 0;
-        }
-        if (X.err) return;
-        bool cpos[C::MP], cneg[C::MP];
-        for (int l = 0; l < C::MP; ++l) cpos[l] = cneg[l] = false;
-        for (int f = 0; f < X.nf; ++f)
-            for (int side = 0; side < 2; ++side) {
-                I lab = side ? X.fb[f] : X.fa[f];
-                if (lab < 4) continue;
-                for (int q = 0; q < X.fn[f]; ++q) {
-                    cpos[lab] |= s[X.fv[f][q]] > 0;
-                    cneg[lab] |= s[X.fv[f][q]] < 0;
-                }
+}
+
+// sign of material j against each vertex's materials (exact): > 0 where j wins
+template <class C>
+__host__ __device__ void mi_classify(const double (*P)[4], int j, Complex<C>& X, int8_t* s) {
+    for (int v = 0; v < X.nv; ++v) {
+        int lab[4] = {X.vl[v][0], X.vl[v][1], X.vl[v][2], X.vl[v][3]};
+        s[v] = (int8_t)orient_mi(P, lab, P[j], X.err);
+    }
+}
+
+// inserts material m given its sign s[v] at every vertex; the result depends
+// on the complex and the signs only
+template <class C>
+__host__ __device__ void mi_insert(Complex<C>& X, typename C::Idx m, int8_t* s, InsertTemps<C>& T) {
+    using I = typename C::Idx;
+    constexpr int PE = InsertTemps<C>::PE;
+    const int nv0 = X.nv;
+    bool any_pos = false;
+    for (int v = 0; v < nv0; ++v) any_pos |= s[v] > 0;
+    bool* cpos = T.cpos;
+    bool* cneg = T.cneg;
+    for (int l = 0; l < C::MP; ++l) cpos[l] = cneg[l] = false;
+    for (int f = 0; f < X.nf; ++f)
+        for (int side = 0; side < 2; ++side) {
+            I lab = side ? X.fb[f] : X.fa[f];
+            if (lab < 4) continue;
+            for (int q = 0; q < X.fn[f]; ++q) {
+                cpos[lab] |= s[X.fv[f][q]] > 0;
+                cneg[lab] |= s[X.fv[f][q]] < 0;
             }
-        if (!any_pos) {
-            X.grp[m] = m;
-            for (int c = 0; c < X.nc; ++c) {
-                I L = X.cells[c];
-                if (!cpos[L] && !cneg[L]) {
-                    X.grp[m] = X.grp[L];
-                    X.dup = true;
-                    break;
-                }
-            }
-            continue;
         }
+    if (!any_pos) {
         X.grp[m] = m;
-        bool consumed[C::MP], split[C::MP];
-        for (int l = 0; l < C::MP; ++l) {
-            consumed[l] = cpos[l] && !cneg[l];
-            split[l] = cpos[l] && cneg[l];
-        }
-        I cu[C::MV], cw[C::MV], cx[C::MV];
-        int ncross = 0;
-        for (int f = 0; f < X.nf; ++f) {
-            const int n = X.fn[f];
-            for (int i = 0; i < n; ++i) {
-                I u = X.fv[f][i], w = X.fv[f][(i + 1) % n];
-                if (s[u] * s[w] >= 0) continue;
-                I a = u < w ? u : w, b = u < w ? w : u;
-                bool have = false;
-                for (int q = 0; q < ncross; ++q) have |= (cu[q] == a && cw[q] == b);
-                if (have) continue;
-                if (X.nv >= C::MV) {
-                    X.err = true;
-                    return;
-                }
-                I t[4] = {X.fa[f], X.fb[f], X.fe[f][i], m};
-                sort4(t);
-                for (int q = 0; q < 4; ++q) X.vl[X.nv][q] = t[q];
-                s[X.nv] = 0;
-                cu[ncross] = a;
-                cw[ncross] = b;
-                cx[ncross] = (I)X.nv;
-                ++ncross;
-                ++X.nv;
+        for (int c = 0; c < X.nc; ++c) {
+            I L = X.cells[c];
+            if (!cpos[L] && !cneg[L]) {
+                X.grp[m] = X.grp[L];
+                X.dup = true;
+                break;
             }
         }
-        auto cross_of = [&](I u, I w) -> I {
+        return;
+    }
+    X.grp[m] = m;
+    bool* consumed = T.consumed;
+    bool* split = T.split;
+    for (int l = 0; l < C::MP; ++l) {
+        consumed[l] = cpos[l] && !cneg[l];
+        split[l] = cpos[l] && cneg[l];
+    }
+    I* cu = T.cu;
+    I* cw = T.cw;
+    I* cx = T.cx;
+    int ncross = 0;
+    for (int f = 0; f < X.nf; ++f) {
+        const int n = X.fn[f];
+        for (int i = 0; i < n; ++i) {
+            I u = X.fv[f][i], w = X.fv[f][(i + 1) % n];
+            if (s[u] * s[w] >= 0) continue;
             I a = u < w ? u : w, b = u < w ? w : u;
-            for (int q = 0; q < ncross; ++q)
-                if (cu[q] == a && cw[q] == b) return cx[q];
-            return C::NIL;
-        };
-        // tet-face edges won by m, per tet face: (plane, from, to, label)
-        I pp[PE], pa[PE], pb[PE], pl[PE];
-        int npe = 0;
-        auto add_plus = [&](I plane, I a, I b, I l) {
-            if (npe >= PE) {
+            bool have = false;
+            for (int q = 0; q < ncross; ++q) have |= (cu[q] == a && cw[q] == b);
+            if (have) continue;
+            if (X.nv >= C::MV) {
                 X.err = true;
                 return;
             }
-            pp[npe] = plane;
-            pa[npe] = a;
-            pb[npe] = b;
-            pl[npe] = l;
-            ++npe;
-        };
-        // clip faces in place, keeping order
-        int w = 0;
-        const int nf0 = X.nf;
-        for (int f = 0; f < nf0; ++f) {
-            const int n = X.fn[f];
-            bool fp = false, fq = false;
+            I t[4] = {X.fa[f], X.fb[f], X.fe[f][i], m};
+            sort4(t);
+            for (int q = 0; q < 4; ++q) X.vl[X.nv][q] = t[q];
+            s[X.nv] = 0;
+            cu[ncross] = a;
+            cw[ncross] = b;
+            cx[ncross] = (I)X.nv;
+            ++ncross;
+            ++X.nv;
+        }
+    }
+    auto cross_of = [&](I u, I w) -> I {
+        I a = u < w ? u : w, b = u < w ? w : u;
+        for (int q = 0; q < ncross; ++q)
+            if (cu[q] == a && cw[q] == b) return cx[q];
+        return C::NIL;
+    };
+    // tet-face edges won by m, per tet face: (plane, from, to, label)
+    I* pp = T.pp;
+    I* pa = T.pa;
+    I* pb = T.pb;
+    I* pl = T.pl;
+    int npe = 0;
+    auto add_plus = [&](I plane, I a, I b, I l) {
+        if (npe >= PE) {
+            X.err = true;
+            return;
+        }
+        pp[npe] = plane;
+        pa[npe] = a;
+        pb[npe] = b;
+        pl[npe] = l;
+        ++npe;
+    };
+    // clip faces in place, keeping order
+    int w = 0;
+    const int nf0 = X.nf;
+    for (int f = 0; f < nf0; ++f) {
+        const int n = X.fn[f];
+        bool fp = false, fq = false;
+        for (int q = 0; q < n; ++q) {
+            fp |= s[X.fv[f][q]] > 0;
+            fq |= s[X.fv[f][q]] < 0;
+        }
+        const I P0 = X.fa[f], N0 = X.fb[f];
+        I nv_[C::MFV], ne_[C::MFV];
+        int nn = -1;
+        I oP = P0, oN = N0;
+        if (fq && !fp) {
+            nn = n;
             for (int q = 0; q < n; ++q) {
-                fp |= s[X.fv[f][q]] > 0;
-                fq |= s[X.fv[f][q]] < 0;
+                nv_[q] = X.fv[f][q];
+                ne_[q] = X.fe[f][q];
             }
-            const I P0 = X.fa[f], N0 = X.fb[f];
-            I nv_[C::MFV], ne_[C::MFV];
-            int nn = -1;
-            I oP = P0, oN = N0;
-            if (fq && !fp) {
-                nn = n;
-                for (int q = 0; q < n; ++q) {
-                    nv_[q] = X.fv[f][q];
-                    ne_[q] = X.fe[f][q];
-                }
-            } else if (fq && fp) {
-                I av[2 * C::MFV], ae[2 * C::MFV];
-                int mm = 0;
-                for (int i = 0; i < n; ++i) {
-                    I u = X.fv[f][i], ww = X.fv[f][(i + 1) % n];
-                    av[mm] = u;
+        } else if (fq && fp) {
+            I av[2 * C::MFV], ae[2 * C::MFV];
+            int mm = 0;
+            for (int i = 0; i < n; ++i) {
+                I u = X.fv[f][i], ww = X.fv[f][(i + 1) % n];
+                av[mm] = u;
+                ae[mm] = X.fe[f][i];
+                ++mm;
+                if (s[u] * s[ww] < 0) {
+                    av[mm] = cross_of(u, ww);
                     ae[mm] = X.fe[f][i];
                     ++mm;
-                    if (s[u] * s[ww] < 0) {
-                        av[mm] = cross_of(u, ww);
-                        ae[mm] = X.fe[f][i];
-                        ++mm;
-                    }
                 }
-                for (int sgn = -1; sgn <= 1; sgn += 2) {
-                    if (sgn > 0 && P0 >= 4) break;
-                    int idx[2 * C::MFV], ni = 0;
-                    for (int i = 0; i < mm; ++i)
-                        if (s[av[i]] * sgn >= 0) idx[ni++] = i;
-                    if (ni > C::MFV) {
-                        X.err = true;
-                        return;
-                    }
-                    const I cutlab = sgn < 0 ? m : N0;
-                    for (int q = 0; q < ni; ++q) {
-                        int ia = idx[q], ib = idx[(q + 1) % ni];
-                        I lab = ((ia + 1) % mm == ib) ? ae[ia] : cutlab;
-                        if (sgn < 0) {
-                            nv_[q] = av[ia];
-                            ne_[q] = lab;
-                        } else {
-                            add_plus(P0, av[ia], av[ib], lab);
-                        }
-                    }
-                    if (sgn < 0) nn = ni;
+            }
+            for (int sgn = -1; sgn <= 1; sgn += 2) {
+                if (sgn > 0 && P0 >= 4) break;
+                int idx[2 * C::MFV], ni = 0;
+                for (int i = 0; i < mm; ++i)
+                    if (s[av[i]] * sgn >= 0) idx[ni++] = i;
+                if (ni > C::MFV) {
+                    X.err = true;
+                    return;
                 }
-            } else if (fp && !fq) {
-                if (P0 < 4)
-                    for (int i = 0; i < n; ++i) add_plus(P0, X.fv[f][i], X.fv[f][(i + 1) % n], X.fe[f][i]);
-            } else {
-                const bool pc = P0 >= 4 && consumed[P0];
-                const bool nc = consumed[N0];
-                if (P0 < 4) {
-                    if (nc) {
-                        for (int i = 0; i < n; ++i) add_plus(P0, X.fv[f][i], X.fv[f][(i + 1) % n], X.fe[f][i]);
+                const I cutlab = sgn < 0 ? m : N0;
+                for (int q = 0; q < ni; ++q) {
+                    int ia = idx[q], ib = idx[(q + 1) % ni];
+                    I lab = ((ia + 1) % mm == ib) ? ae[ia] : cutlab;
+                    if (sgn < 0) {
+                        nv_[q] = av[ia];
+                        ne_[q] = lab;
                     } else {
-                        nn = n;
-                        for (int q = 0; q < n; ++q) {
-                            nv_[q] = X.fv[f][q];
-                            ne_[q] = X.fe[f][q];
-                        }
+                        add_plus(P0, av[ia], av[ib], lab);
                     }
-                } else if (pc && nc) {
-                } else if (nc) {
-                    // m becomes the positive side: reverse the cycle
-                    nn = n;
-                    for (int q = 0; q < n; ++q) {
-                        nv_[q] = X.fv[f][(n - q) % n];
-                        ne_[q] = X.fe[f][(2 * n - 1 - q) % n];
-                    }
-                    oP = m;
-                    oN = P0;
+                }
+                if (sgn < 0) nn = ni;
+            }
+        } else if (fp && !fq) {
+            if (P0 < 4)
+                for (int i = 0; i < n; ++i) add_plus(P0, X.fv[f][i], X.fv[f][(i + 1) % n], X.fe[f][i]);
+        } else {
+            const bool pc = P0 >= 4 && consumed[P0];
+            const bool nc = consumed[N0];
+            if (P0 < 4) {
+                if (nc) {
+                    for (int i = 0; i < n; ++i) add_plus(P0, X.fv[f][i], X.fv[f][(i + 1) % n], X.fe[f][i]);
                 } else {
                     nn = n;
                     for (int q = 0; q < n; ++q) {
                         nv_[q] = X.fv[f][q];
                         ne_[q] = X.fe[f][q];
                     }
-                    if (pc) oP = m;
                 }
-            }
-            if (X.err) return;
-            if (nn >= 0) {
-                X.fa[w] = oP;
-                X.fb[w] = oN;
-                X.fc[w] = C::NIL;
-                X.fn[w] = (I)nn;
-                for (int q = 0; q < nn; ++q) {
-                    X.fv[w][q] = nv_[q];
-                    X.fe[w][q] = ne_[q];
+            } else if (pc && nc) {
+            } else if (nc) {
+                // m becomes the positive side: reverse the cycle
+                nn = n;
+                for (int q = 0; q < n; ++q) {
+                    nv_[q] = X.fv[f][(n - q) % n];
+                    ne_[q] = X.fe[f][(2 * n - 1 - q) % n];
                 }
-                ++w;
+                oP = m;
+                oN = P0;
+            } else {
+                nn = n;
+                for (int q = 0; q < n; ++q) {
+                    nv_[q] = X.fv[f][q];
+                    ne_[q] = X.fe[f][q];
+                }
+                if (pc) oP = m;
             }
         }
-        X.nf = w;
-        // one merged face per tet face won by m
-        for (int i = 0; i < 4; ++i) {
-            I ea[C::MFV * 2], eb[C::MFV * 2], el[C::MFV * 2];
-            int ne = 0, any = 0;
-            for (int q = 0; q < npe; ++q) {
-                if (pp[q] != i) continue;
-                ++any;
-                bool shared = false;
-                for (int r = 0; r < npe; ++r)
-                    if (pp[r] == i && pa[r] == pb[q] && pb[r] == pa[q]) {
-                        shared = true;
-                        break;
-                    }
-                if (shared) continue;
-                if (ne >= 2 * C::MFV) {
+        if (X.err) return;
+        if (nn >= 0) {
+            X.fa[w] = oP;
+            X.fb[w] = oN;
+            X.fc[w] = C::NIL;
+            X.fn[w] = (I)nn;
+            for (int q = 0; q < nn; ++q) {
+                X.fv[w][q] = nv_[q];
+                X.fe[w][q] = ne_[q];
+            }
+            ++w;
+        }
+    }
+    X.nf = w;
+    // one merged face per tet face won by m
+    for (int i = 0; i < 4; ++i) {
+        I ea[C::MFV * 2], eb[C::MFV * 2], el[C::MFV * 2];
+        int ne = 0, any = 0;
+        for (int q = 0; q < npe; ++q) {
+            if (pp[q] != i) continue;
+            ++any;
+            bool shared = false;
+            for (int r = 0; r < npe; ++r)
+                if (pp[r] == i && pa[r] == pb[q] && pb[r] == pa[q]) {
+                    shared = true;
+                    break;
+                }
+            if (shared) continue;
+            if (ne >= 2 * C::MFV) {
+                X.err = true;
+                return;
+            }
+            ea[ne] = pa[q];
+            eb[ne] = pb[q];
+            el[ne] = pl[q];
+            ++ne;
+        }
+        if (!any) continue;
+        I vs[C::MFV * 2], es[C::MFV * 2];
+        int n = chain_edges<2 * C::MFV>(ea, eb, el, ne, vs, es);
+        if (n < 0 || X.nf >= C::MF) {
+            X.err = true;
+            return;
+        }
+        const int F = X.nf++;
+        int cnt = 0;
+        for (int q = 0; q < n; ++q) {
+            int prev = (q + n - 1) % n;
+            if (vs[q] >= 4 && es[prev] == es[q]) continue;
+            if (cnt >= C::MFV) {
+                X.err = true;
+                return;
+            }
+            X.fv[F][cnt] = vs[q];
+            X.fe[F][cnt] = es[q];
+            ++cnt;
+        }
+        X.fa[F] = i;
+        X.fb[F] = m;
+        X.fc[F] = C::NIL;
+        X.fn[F] = (I)cnt;
+    }
+    // one face (m | X) per cell crossed by f_m = f_X
+    for (int c = 0; c < X.nc; ++c) {
+        const I L = X.cells[c];
+        if (!split[L]) continue;
+        I ea[C::MFV], eb[C::MFV], el[C::MFV];
+        int ne = 0;
+        for (int g = 0; g < X.nf; ++g) {
+            if (X.fa[g] != L && X.fb[g] != L) continue;
+            const int n = X.fn[g];
+            const I other = (X.fa[g] == L) ? X.fb[g] : X.fa[g];
+            for (int i = 0; i < n; ++i) {
+                I a = X.fv[g][i], b = X.fv[g][(i + 1) % n];
+                if (s[a] != 0 || s[b] != 0) continue;
+                if (ne >= C::MFV) {
                     X.err = true;
                     return;
                 }
-                ea[ne] = pa[q];
-                eb[ne] = pb[q];
-                el[ne] = pl[q];
+                if (X.fb[g] == L) {
+                    ea[ne] = b;
+                    eb[ne] = a;
+                } else {
+                    ea[ne] = a;
+                    eb[ne] = b;
+                }
+                el[ne] = other;
                 ++ne;
             }
-            if (!any) continue;
-            I vs[C::MFV * 2], es[C::MFV * 2];
-            int n = chain_edges<2 * C::MFV>(ea, eb, el, ne, vs, es);
-            if (n < 0 || X.nf >= C::MF) {
-                X.err = true;
-                return;
-            }
-            const int F = X.nf++;
-            int cnt = 0;
-            for (int q = 0; q < n; ++q) {
-                int prev = (q + n - 1) % n;
-                if (vs[q] >= 4 && es[prev] == es[q]) continue;
-                if (cnt >= C::MFV) {
-                    X.err = true;
-                    return;
-                }
-                X.fv[F][cnt] = vs[q];
-                X.fe[F][cnt] = es[q];
-                ++cnt;
-            }
-            X.fa[F] = i;
-            X.fb[F] = m;
-            X.fc[F] = C::NIL;
-            X.fn[F] = (I)cnt;
         }
-        // one face (m | X) per cell crossed by f_m = f_X
-        for (int c = 0; c < X.nc; ++c) {
-            const I L = X.cells[c];
-            if (!split[L]) continue;
-            I ea[C::MFV], eb[C::MFV], el[C::MFV];
-            int ne = 0;
-            for (int g = 0; g < X.nf; ++g) {
-                if (X.fa[g] != L && X.fb[g] != L) continue;
-                const int n = X.fn[g];
-                const I other = (X.fa[g] == L) ? X.fb[g] : X.fa[g];
-                for (int i = 0; i < n; ++i) {
-                    I a = X.fv[g][i], b = X.fv[g][(i + 1) % n];
-                    if (s[a] != 0 || s[b] != 0) continue;
-                    if (ne >= C::MFV) {
-                        X.err = true;
-                        return;
-                    }
-                    if (X.fb[g] == L) {
-                        ea[ne] = b;
-                        eb[ne] = a;
-                    } else {
-                        ea[ne] = a;
-                        eb[ne] = b;
-                    }
-                    el[ne] = other;
-                    ++ne;
-                }
-            }
-            if (X.nf >= C::MF) {
-                X.err = true;
-                return;
-            }
-            const int F = X.nf;
-            int n = chain_edges<C::MFV>(ea, eb, el, ne, X.fv[F], X.fe[F]);
-            if (n < 0) {
-                X.err = true;
-                return;
-            }
-            X.fa[F] = m;
-            X.fb[F] = L;
-            X.fc[F] = C::NIL;
-            X.fn[F] = (I)n;
-            ++X.nf;
+        if (X.nf >= C::MF) {
+            X.err = true;
+            return;
         }
-        // cells: survivors in order, then m
-        int nc = 0;
-        for (int c = 0; c < X.nc; ++c)
-            if (!consumed[X.cells[c]]) X.cells[nc++] = X.cells[c];
-        X.cells[nc++] = m;
-        X.nc = nc;
-        // corners won by m change label; unreferenced vertices disappear
-        for (int c = 0; c < 4; ++c)
-            if (s[c] > 0) {
-                int n = 0;
-                for (int q = 0; q < 4; ++q)
-                    if (q != c) X.vl[c][n++] = q;
-                X.vl[c][3] = m;
-            }
-        I remap[C::MV];
-        bool used[C::MV];
-        for (int v = 0; v < X.nv; ++v) used[v] = v < 4;
-        for (int f = 0; f < X.nf; ++f)
-            for (int q = 0; q < X.fn[f]; ++q) used[X.fv[f][q]] = true;
-        int nvn = 0;
-        for (int v = 0; v < X.nv; ++v) {
-            if (!used[v]) {
-                remap[v] = C::NIL;
-                continue;
-            }
-            remap[v] = (I)nvn;
-            if (nvn != v)
-                for (int q = 0; q < 4; ++q) X.vl[nvn][q] = X.vl[v][q];
-            ++nvn;
+        const int F = X.nf;
+        int n = chain_edges<C::MFV>(ea, eb, el, ne, X.fv[F], X.fe[F]);
+        if (n < 0) {
+            X.err = true;
+            return;
         }
-        X.nv = nvn;
-        for (int f = 0; f < X.nf; ++f)
-            for (int q = 0; q < X.fn[f]; ++q) X.fv[f][q] = remap[X.fv[f][q]];
+        X.fa[F] = m;
+        X.fb[F] = L;
+        X.fc[F] = C::NIL;
+        X.fn[F] = (I)n;
+        ++X.nf;
     }
+    // cells: survivors in order, then m
+    int nc = 0;
+    for (int c = 0; c < X.nc; ++c)
+        if (!consumed[X.cells[c]]) X.cells[nc++] = X.cells[c];
+    X.cells[nc++] = m;
+    X.nc = nc;
+    // corners won by m change label; unreferenced vertices disappear
+    for (int c = 0; c < 4; ++c)
+        if (s[c] > 0) {
+            int n = 0;
+            for (int q = 0; q < 4; ++q)
+                if (q != c) X.vl[c][n++] = q;
+            X.vl[c][3] = m;
+        }
+    I* remap = T.remap;
+    bool* used = T.used;
+    for (int v = 0; v < X.nv; ++v) used[v] = v < 4;
+    for (int f = 0; f < X.nf; ++f)
+        for (int q = 0; q < X.fn[f]; ++q) used[X.fv[f][q]] = true;
+    int nvn = 0;
+    for (int v = 0; v < X.nv; ++v) {
+        if (!used[v]) {
+            remap[v] = C::NIL;
+            continue;
+        }
+        remap[v] = (I)nvn;
+        if (nvn != v)
+            for (int q = 0; q < 4; ++q) X.vl[nvn][q] = X.vl[v][q];
+        ++nvn;
+    }
+    X.nv = nvn;
+    for (int f = 0; f < X.nf; ++f)
+        for (int q = 0; q < X.fn[f]; ++q) X.fv[f][q] = remap[X.fv[f][q]];
+}
+
+template <class C>
+__host__ __device__ void mi_cut(const double (*P)[4], int k, Complex<C>& X, InsertTemps<C>& T) {
+    mi_init(X, k);
+    if (X.err || k < 1) return;
+    for (int j = 1; j < k; ++j) {
+        mi_classify(P, j, X, T.s);
+        if (X.err) return;
+        mi_insert(X, (typename C::Idx)(4 + j), T.s, T);
+        if (X.err) return;
+    }
+}
+
+template <class C>
+void mi_insert(Complex<C>& X, typename C::Idx m, int8_t* s) {
+    static thread_local InsertTemps<C> T;
+    mi_insert(X, m, s, T);
+}
+template <class C>
+void mi_cut(const double (*P)[4], int k, Complex<C>& X) {
+    static thread_local InsertTemps<C> T;
+    mi_cut(P, k, X, T);
 }
 
 template <class C>

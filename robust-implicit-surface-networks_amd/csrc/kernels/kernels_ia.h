@@ -38,6 +38,26 @@ __device__ inline uint64_t count_of(const Count& n) { return n.get(); }
 
 __device__ inline void flag_error(uint32_t* errs, uint32_t bit) { errs[threadIdx.x & 63] |= bit; }
 
+// One lane's thread-cut state (the complex and the insertion's working
+// arrays) in a device-memory workspace: lane i of a launch owns bytes
+// [i * stride, (i + 1) * stride).  Kept out of the private segment, so the
+// runtime's per-queue scratch stays small and several queues can run cuts
+// at once.
+// lanes of a plain-form thread-cut launch (the grid strides beyond)
+constexpr uint64_t kCutLanes = 16384;
+
+template <class C>
+struct LaneCut {
+    static constexpr size_t stride =
+        (sizeof(dev::Complex<C>) + sizeof(dev::InsertTemps<C>) + 255) & ~size_t(255);
+    __device__ static dev::Complex<C>& X(uint8_t* ws, uint64_t lane) {
+        return *reinterpret_cast<dev::Complex<C>*>(ws + lane * stride);
+    }
+    __device__ static dev::InsertTemps<C>& T(uint8_t* ws, uint64_t lane) {
+        return *reinterpret_cast<dev::InsertTemps<C>*>(ws + lane * stride + sizeof(dev::Complex<C>));
+    }
+};
+
 // ------------------------------------------------------------------ setup
 
 __global__ void k_grid_pts(GridView g, double* pts) {
@@ -242,7 +262,8 @@ template <class C, bool MI>
 __device__ inline void cut_tet(const GridView& g, const double* __restrict__ vals, const uint32_t* __restrict__ act,
                                const uint32_t* __restrict__ csr_off, const uint8_t* __restrict__ fit, uint64_t a, int k,
                                uint32_t flags, const uint32_t* __restrict__ rec_off, uint16_t* __restrict__ rec,
-                               uint32_t* __restrict__ cnt, uint32_t* __restrict__ errs, Complex<C>& X) {
+                               uint32_t* __restrict__ cnt, uint32_t* __restrict__ errs, Complex<C>& X,
+                               dev::InsertTemps<C>& Tm) {
     const uint64_t V = g.V;
     const uint32_t t = act[a];
     uint32_t v[4];
@@ -254,9 +275,9 @@ __device__ inline void cut_tet(const GridView& g, const double* __restrict__ val
         for (int c = 0; c < 4; ++c) P[q][c] = vals[(uint64_t)f * V + v[c]];
     }
     if (MI)
-        dev::mi_cut<C>(P, k, X);
+        dev::mi_cut<C>(P, k, X, Tm);
     else
-        dev::ia_cut<C>(P, k, X);
+        dev::ia_cut<C>(P, k, X, Tm);
     if (X.err) {
         // capacities or predicate range: cut again with the large kernel
         flag_error(errs, ERR_RETRY);
@@ -296,13 +317,16 @@ __global__ void __launch_bounds__(64) k_cut(GridView g, const double* __restrict
                                             const uint8_t* __restrict__ fit, uint64_t n_active, int kmin, int kmax,
                                             uint32_t flags, const uint32_t* __restrict__ rec_off,
                                             uint16_t* __restrict__ rec, uint32_t* __restrict__ cnt,
-                                            uint32_t* __restrict__ errs, const uint32_t* __restrict__ list) {
+                                            uint32_t* __restrict__ errs, const uint32_t* __restrict__ list,
+                                            uint8_t* __restrict__ ws) {
+    const uint64_t lane = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    Complex<C>& X = LaneCut<C>::X(ws, lane);
+    dev::InsertTemps<C>& Tm = LaneCut<C>::T(ws, lane);
     GRID_STRIDE(i, n_active) {
         const uint64_t a = list ? list[i] : i;
         const int k = act_k[a];
         if (k < kmin || k > kmax) continue;
-        Complex<C> X;
-        cut_tet<C, MI>(g, vals, act, csr_off, fit, a, k, flags, rec_off, rec, cnt, errs, X);
+        cut_tet<C, MI>(g, vals, act, csr_off, fit, a, k, flags, rec_off, rec, cnt, errs, X, Tm);
     }
 }
 
@@ -316,11 +340,13 @@ __global__ void __launch_bounds__(64) k_cut_huge(GridView g, const double* __res
                                                  const uint32_t* __restrict__ list, uint64_t n, uint32_t flags,
                                                  const uint32_t* __restrict__ rec_off, uint16_t* __restrict__ rec,
                                                  uint32_t* __restrict__ cnt, uint32_t* __restrict__ errs,
-                                                 Complex<CHuge>* __restrict__ ws, const uint32_t* __restrict__ rec_loc,
+                                                 uint8_t* __restrict__ ws, const uint32_t* __restrict__ rec_loc,
                                                  uint32_t fixed_cap) {
     using C = CHuge;
     const uint64_t V = g.V;
-    Complex<C>& X = ws[(uint64_t)blockIdx.x * blockDim.x + threadIdx.x];
+    const uint64_t lane = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    Complex<C>& X = LaneCut<C>::X(ws, lane);
+    dev::InsertTemps<C>& Tm = LaneCut<C>::T(ws, lane);
     GRID_STRIDE(i, n) {
         const uint32_t a = list[i];
         const int k = act_k[a];
@@ -335,9 +361,9 @@ __global__ void __launch_bounds__(64) k_cut_huge(GridView g, const double* __res
             for (int c = 0; c < 4; ++c) P[q][c] = vals[(uint64_t)f * V + v[c]];
         }
         if (MI)
-            dev::mi_cut<C>(P, k, X);
+            dev::mi_cut<C>(P, k, X, Tm);
         else
-            dev::ia_cut<C>(P, k, X);
+            dev::ia_cut<C>(P, k, X, Tm);
         // the record goes to rec_loc[a] (retries) or to the tet's own slot
         const uint32_t at = rec_loc ? rec_loc[a] : rec_off[a];
         if (X.err) {
@@ -786,8 +812,9 @@ __global__ void __launch_bounds__(1024) k_dedup_fallback_scan(Count n_fail, cons
 constexpr uint32_t kBucketChunk = 2048;
 constexpr int kBucketWords = 10;
 
+// table2: the material-interface tables take two- and three-material tets
 __device__ inline int bucket_of_k(int k, int table2) {
-    return k > 8 ? 3 : (k > 4 ? 2 : ((table2 && k == 2) ? 0 : 1));
+    return k > 8 ? 3 : (k > 4 ? 2 : ((table2 && (k == 2 || k == 3)) ? 0 : 1));
 }
 
 __global__ void __launch_bounds__(256) k_bucket_count(uint64_t n, const uint8_t* __restrict__ act_k, int mi,

@@ -75,12 +75,49 @@ struct MI2Entry {
     uint32_t cut[3];     // vertices, faces, cells of the complex
 };
 
-__global__ void k_mi_table2(GridView g, const double* __restrict__ vals, const uint32_t* __restrict__ act,
-                            const uint32_t* __restrict__ csr_off, const uint8_t* __restrict__ fit,
-                            const uint32_t* __restrict__ list, uint64_t n, uint32_t flags,
-                            const uint32_t* __restrict__ rec_off, uint16_t* __restrict__ rec,
-                            uint32_t* __restrict__ cnt, uint32_t* __restrict__ errs, const MI2Entry* __restrict__ tab,
-                            const uint16_t* __restrict__ words) {
+// 3 materials a < b < c: the complex of a and b is one of the 81 patterns
+// above; inserting c depends only on that complex and c's sign against each
+// of its vertices (the only classifications the cut makes): at a corner an
+// exact comparison, at a vertex where a and b tie on a tet edge one exact
+// 3 x 3 predicate.  Every sign vector is cut once on the host.  Per pattern:
+// the first slot of its sign vectors and the labels of its vertices.
+struct MI3Pat {
+    uint32_t base;
+    uint8_t nv, valid, pad0, pad1;
+    uint8_t lab[8][4];
+};
+static_assert(sizeof(MI3Pat) == 40, "3-material pattern layout");
+constexpr uint32_t MI3_NONE = 0xFFFFFFFFu;
+
+// the tables on the device: 2-material records by pattern; 3-material
+// patterns (staged in LDS by the kernel), slots (word offset or MI3_NONE),
+// packed counts per slot (interface vertices | faces << 8 | corners << 16,
+// and vertices | faces << 8 | cells << 16 of the complex) and records
+struct MITablesDev {
+    const MI2Entry* e2 = nullptr;
+    const uint16_t* w2 = nullptr;
+    const MI3Pat* pat3 = nullptr;
+    const uint32_t *off3 = nullptr, *cnt3 = nullptr, *cut3 = nullptr;
+    const uint16_t* w3 = nullptr;
+};
+
+// Active tets with two or three materials by table (list: their active
+// ranks; the count at n_dev when given, else n).  A tet whose sign vector has
+// no table entry, or whose record outgrows its slot, is marked for the
+// general cut (CNT_RETRY).
+__global__ void __launch_bounds__(256) k_mi_table(GridView g, const double* __restrict__ vals,
+                                                  const uint32_t* __restrict__ act,
+                                                  const uint32_t* __restrict__ csr_off, const uint8_t* __restrict__ fit,
+                                                  const uint32_t* __restrict__ list, uint64_t n,
+                                                  const uint32_t* __restrict__ n_dev, uint32_t flags,
+                                                  const uint32_t* __restrict__ rec_off, uint16_t* __restrict__ rec,
+                                                  uint32_t* __restrict__ cnt, uint32_t* __restrict__ errs,
+                                                  MITablesDev T) {
+    __shared__ MI3Pat pats[81];
+    for (int i = threadIdx.x; i < 81 * (int)sizeof(MI3Pat) / 4; i += blockDim.x)
+        reinterpret_cast<uint32_t*>(pats)[i] = reinterpret_cast<const uint32_t*>(T.pat3)[i];
+    __syncthreads();
+    if (n_dev) n = *n_dev < n ? *n_dev : n;
     const uint64_t V = g.V;
     GRID_STRIDE(i, n) {
         const uint32_t a = list[i];
@@ -88,24 +125,51 @@ __global__ void k_mi_table2(GridView g, const double* __restrict__ vals, const u
         uint32_t v[4];
         dev::tet_corners(g, t, v);
         const uint32_t s0 = csr_off[t];
-        const double* fa = vals + (uint64_t)fit[s0] * V;
-        const double* fb = vals + (uint64_t)fit[s0 + 1] * V;
-        int pat = 0;
-        for (int c = 0, w = 1; c < 4; ++c, w *= 3) {
-            const double x = fa[v[c]], y = fb[v[c]];
-            pat += (y > x ? 2 : (y < x ? 0 : 1)) * w;
+        const int k = (int)(csr_off[t + 1] - s0);
+        double P[3][4];
+        for (int q = 0; q < k && q < 3; ++q) {
+            const double* f = vals + (uint64_t)fit[s0 + q] * V;
+            for (int c = 0; c < 4; ++c) P[q][c] = f[v[c]];
         }
-        const MI2Entry E = tab[pat];
+        int pat = 0;
+        for (int c = 0, w = 1; c < 4; ++c, w *= 3) pat += (P[1][c] > P[0][c] ? 2 : (P[1][c] < P[0][c] ? 0 : 1)) * w;
+        const uint16_t* src = nullptr;
+        uint32_t len = 0, c3[3];
+        if (k == 2) {
+            const MI2Entry E = T.e2[pat];
+            src = T.w2 + E.off;
+            len = E.len;
+            const uint32_t* cc = (flags & RUN_CUT_ONLY) ? E.cut : E.cnt;
+            c3[0] = cc[0];
+            c3[1] = cc[1];
+            c3[2] = cc[2];
+        } else {
+            const MI3Pat& K = pats[pat];
+            bool err = !K.valid || k != 3;
+            uint32_t slot = K.base, w = 1;
+            for (int q = 0; q < K.nv && !err; ++q, w *= 3) {
+                const int lab[4] = {K.lab[q][0], K.lab[q][1], K.lab[q][2], K.lab[q][3]};
+                slot += (uint32_t)(dev::orient_mi(P, lab, P[2], err) + 1) * w;
+            }
+            const uint32_t off = err ? MI3_NONE : T.off3[slot];
+            if (off != MI3_NONE) {
+                src = T.w3 + off;
+                len = src[6];
+                const uint32_t pk = (flags & RUN_CUT_ONLY) ? T.cut3[slot] : T.cnt3[slot];
+                c3[0] = pk & 0xFF;
+                c3[1] = (pk >> 8) & 0xFF;
+                c3[2] = pk >> 16;
+            }
+        }
         const uint32_t at = rec_off[a];
-        if (E.len > rec_off[a + 1] - at) {
+        if (!src || len > rec_off[a + 1] - at) {
             flag_error(errs, ERR_RETRY);
             cnt[3 * (uint64_t)a] = CNT_RETRY;
             cnt[3 * (uint64_t)a + 1] = cnt[3 * (uint64_t)a + 2] = 0;
             rec[at] = 0;
             continue;
         }
-        for (int w = 0; w < E.len; ++w) rec[at + w] = words[E.off + w];
-        const uint32_t* c3 = (flags & RUN_CUT_ONLY) ? E.cut : E.cnt;
+        for (uint32_t q = 0; q < len; ++q) rec[at + q] = src[q];
         for (int q = 0; q < 3; ++q) cnt[3 * (uint64_t)a + q] = c3[q];
     }
 }

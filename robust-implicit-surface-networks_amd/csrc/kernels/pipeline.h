@@ -150,9 +150,10 @@ void debug_cuts(DevMem& mem, int mi, const double* rows, const int* ks, int ncas
 
 // diagnostics (host only, no device): the record a lookup table gives for one
 // tet — IA k = 1 or k = 2 plane rows (k_cube_fill's and k_k2_lookup's
-// indexing), or two material rows (k_mi_table2's); returns the record length
+// indexing), or two or three material rows (k_mi_table's); returns the record length
 // or a negative code (-1: no table entry, -2: record longer than cap)
 int debug_table_record_ia(int k, const double* rows, uint16_t* out, int cap);
 int debug_table_record_mi2(const double* rows, uint16_t* out, int cap);
+int debug_table_record_mi3(const double* rows, uint16_t* out, int cap);
 
 }  // namespace rin

@@ -102,6 +102,117 @@ const MI2Table& mi2_table() {
     return T;
 }
 
+// the 3-material table: per 2-material pattern, every sign vector of the
+// third material over that complex's vertices, cut once by the insertion
+// itself; identical records are stored once
+struct MI3Table {
+    MI3Pat pat[81];
+    uint32_t nslots = 0;
+    std::vector<uint32_t> off, cnt, cut;
+    std::vector<uint16_t> words;
+};
+
+const MI3Table& mi3_table() {
+    static MI3Table T;
+    static std::once_flag once;
+    std::call_once(once, [] {
+        std::memset(T.pat, 0, sizeof(T.pat));
+        static dev::Complex<CMed> X1, X2;
+        auto two = [&](int pat) {
+            double P[2][4];
+            for (int c = 0, q = pat; c < 4; ++c, q /= 3) {
+                P[0][c] = 0.0;
+                P[1][c] = (double)(q % 3 - 1);
+            }
+            dev::mi_cut<CMed>(P, 2, X1);
+            if (X1.err) throw std::runtime_error("3-material table: 2-material cut failed");
+        };
+        uint32_t base = 0;
+        for (int pat = 0; pat < 81; ++pat) {
+            two(pat);
+            if (X1.nv > 8) throw std::runtime_error("3-material table: more than 8 vertices");
+            MI3Pat& K = T.pat[pat];
+            K.valid = 1;
+            K.nv = (uint8_t)X1.nv;
+            K.base = base;
+            for (int v = 0; v < X1.nv; ++v)
+                for (int q = 0; q < 4; ++q) K.lab[v][q] = X1.vl[v][q];
+            uint32_t p3 = 1;
+            for (int v = 0; v < X1.nv; ++v) p3 *= 3;
+            base += p3;
+        }
+        T.nslots = base;
+        T.off.assign(base, MI3_NONE);
+        T.cnt.assign(base, 0);
+        T.cut.assign(base, 0);
+        std::map<std::vector<uint16_t>, uint32_t> seen;
+        for (int pat = 0; pat < 81; ++pat) {
+            const MI3Pat& K = T.pat[pat];
+            two(pat);
+            uint32_t p3 = 1;
+            for (int v = 0; v < K.nv; ++v) p3 *= 3;
+            for (uint32_t sv = 0; sv < p3; ++sv) {
+                int8_t sg[CMed::MV];
+                for (int v = 0, r = (int)sv; v < K.nv; ++v, r /= 3) sg[v] = (int8_t)(r % 3 - 1);
+                X2 = X1;
+                dev::mi_insert<CMed>(X2, 6, sg);
+                if (X2.err) continue;  // no tet has this sign vector; the general cut would fail alike
+                uint16_t rec[1024];
+                const int len = dev::write_record_mi<CMed>(X2, 3, rec, 1024);
+                if (len < 0) throw std::runtime_error("3-material table: record overflow");
+                std::vector<uint16_t> key(rec, rec + len);
+                auto it = seen.find(key);
+                uint32_t off;
+                if (it == seen.end()) {
+                    off = (uint32_t)T.words.size();
+                    T.words.insert(T.words.end(), rec, rec + len);
+                    seen.emplace(std::move(key), off);
+                } else {
+                    off = it->second;
+                }
+                uint32_t a0, a1, a2;
+                mi_counts<CMed>(X2, a0, a1, a2);
+                if (a0 > 255 || a1 > 255 || X2.nv > 255 || X2.nf > 255) throw std::runtime_error("3-material table");
+                T.off[K.base + sv] = off;
+                T.cnt[K.base + sv] = a0 | a1 << 8 | a2 << 16;
+                T.cut[K.base + sv] = (uint32_t)X2.nv | (uint32_t)X2.nf << 8 | (uint32_t)X2.nc << 16;
+            }
+        }
+    });
+    return T;
+}
+
+// the material-interface tables on each device, uploaded once per process
+// and device (they stay until the process ends)
+MITablesDev mi_tables_dev(hipStream_t s) {
+    static std::mutex mu;
+    static std::map<int, MITablesDev> per_device;
+    int dev = 0;
+    RIN_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = per_device.find(dev);
+    if (it != per_device.end()) return it->second;
+    const MI2Table& T2 = mi2_table();
+    const MI3Table& T3 = mi3_table();
+    auto up = [&](const void* src, size_t bytes) {
+        void* p = nullptr;
+        RIN_HIP(hipMalloc(&p, bytes ? bytes : 4));
+        if (bytes) RIN_HIP(hipMemcpyAsync(p, src, bytes, hipMemcpyHostToDevice, s));
+        return p;
+    };
+    MITablesDev D;
+    D.e2 = (const MI2Entry*)up(T2.e, sizeof(T2.e));
+    D.w2 = (const uint16_t*)up(T2.words.data(), 2 * T2.words.size());
+    D.pat3 = (const MI3Pat*)up(T3.pat, sizeof(T3.pat));
+    D.off3 = (const uint32_t*)up(T3.off.data(), 4 * T3.off.size());
+    D.cnt3 = (const uint32_t*)up(T3.cnt.data(), 4 * T3.cnt.size());
+    D.cut3 = (const uint32_t*)up(T3.cut.data(), 4 * T3.cut.size());
+    D.w3 = (const uint16_t*)up(T3.words.data(), 2 * T3.words.size());
+    RIN_HIP(hipStreamSynchronize(s));
+    per_device[dev] = D;
+    return D;
+}
+
 struct Dedup {
     DBuf<uint32_t> vid;        // id of every occurrence
     DBuf<uint32_t> first_occ;  // first occurrence of every id
@@ -501,19 +612,16 @@ void run_pipeline(DevMem& mem, const Grid& g, const Values& val, uint32_t flags,
     out.rec.alloc(mem, nwords + 1);
     DBuf<uint32_t> cnt(mem, 3 * NA + 3);
     if (bcnt[0]) {
-        const MI2Table& TT = mi2_table();
-        DBuf<MI2Entry> dtab(mem, 81);
-        DBuf<uint16_t> dw(mem, TT.words.size());
-        RIN_HIP(hipMemcpyAsync(dtab.p, TT.e, sizeof(TT.e), hipMemcpyHostToDevice, s));
-        RIN_HIP(hipMemcpyAsync(dw.p, TT.words.data(), 2 * TT.words.size(), hipMemcpyHostToDevice, s));
-        hipLaunchKernelGGL(k_mi_table2, dim3(nblk(bcnt[0])), dim3(256), 0, s, g.view, (const double*)val.vals.p,
+        // two- and three-material tets by table
+        const MITablesDev TT = mi_tables_dev(s);
+        hipLaunchKernelGGL(k_mi_table, dim3(nblk(bcnt[0])), dim3(256), 0, s, g.view, (const double*)val.vals.p,
                            (const uint32_t*)out.act.p, (const uint32_t*)out.csr_off.p, (const uint8_t*)out.fit.p,
-                           (const uint32_t*)blist[0], (uint64_t)bcnt[0], flags, (const uint32_t*)out.rec_off.p,
-                           out.rec.p, cnt.p, errs.p, (const MI2Entry*)dtab.p, (const uint16_t*)dw.p);
+                           (const uint32_t*)blist[0], (uint64_t)bcnt[0], (const uint32_t*)nullptr, flags,
+                           (const uint32_t*)out.rec_off.p, out.rec.p, cnt.p, errs.p, TT);
         RIN_HIP(hipGetLastError());
-        RIN_HIP(hipStreamSynchronize(s));  // the table buffers return to the pool
     }
     // each thread-per-tet cut kernel runs over its own k range only
+    DBuf<uint8_t> ws_cut[2];
     for (int q = 0; q < 2 && NA; ++q) {
         const uint64_t nl = bcnt[1 + q];
         if (!nl) continue;
@@ -523,14 +631,19 @@ void run_pipeline(DevMem& mem, const Grid& g, const Values& val, uint32_t flags,
         // at once (about 2048 of these)
         unsigned lanes = 2;
         while (lanes < 64 && nl > (uint64_t)lanes * 2048) lanes *= 2;
-        const unsigned blocks = (unsigned)((nl + lanes - 1) / lanes);
+        // each lane's complex and working arrays in a device workspace; the
+        // grid strides over the list beyond kCutLanes lanes
+        const uint64_t nb = (nl + lanes - 1) / lanes;
+        const unsigned blocks = (unsigned)std::min<uint64_t>(nb, kCutLanes / lanes);
         const int kmin = q == 0 ? 1 : 5, kmax = q == 0 ? 4 : 8;
+        const size_t stride = q == 0 ? LaneCut<CMed>::stride : LaneCut<CBig>::stride;
+        ws_cut[q].alloc(mem, stride * blocks * lanes);
         auto go = [&](auto kernel) {
             hipLaunchKernelGGL(kernel, dim3(blocks), dim3(lanes), 0, s, g.view, (const double*)val.vals.p,
                                (const uint32_t*)out.act.p, (const uint8_t*)out.act_k.p,
                                (const uint32_t*)out.csr_off.p, (const uint8_t*)out.fit.p, nl, kmin, kmax, flags,
                                (const uint32_t*)out.rec_off.p, out.rec.p, cnt.p, errs.p,
-                               (const uint32_t*)blist[1 + q]);
+                               (const uint32_t*)blist[1 + q], ws_cut[q].p);
         };
         if (q == 0)
             MI ? go(k_cut<CMed, true>) : go(k_cut<CMed, false>);
@@ -542,8 +655,8 @@ void run_pipeline(DevMem& mem, const Grid& g, const Values& val, uint32_t flags,
     auto cut_large = [&](const uint32_t* list, uint64_t nl, const uint32_t* loc, uint32_t fixed_cap) {
         // ~200 KB of complex per lane: a few blocks
         const unsigned blocks = (unsigned)std::min<uint64_t>((nl + 63) / 64, 4);
-        DBuf<uint8_t> ws(mem, sizeof(Complex<CHuge>) * 64 * blocks);
-        auto* wsp = reinterpret_cast<Complex<CHuge>*>(ws.p);
+        DBuf<uint8_t> ws(mem, LaneCut<CHuge>::stride * 64 * blocks);
+        uint8_t* wsp = ws.p;
         auto go = [&](auto kernel) {
             hipLaunchKernelGGL(kernel, dim3(blocks), dim3(64), 0, s, g.view, (const double*)val.vals.p,
                                (const uint32_t*)out.act.p, (const uint8_t*)out.act_k.p, (const uint32_t*)out.csr_off.p,
@@ -792,22 +905,22 @@ void RunOut::set_views_from_buffers() {
 namespace {
 template <class C, bool MI>
 __global__ void k_debug_cut(const double* rows, const int* ks, const int* starts, int ncase, int cap, uint16_t* words,
-                            int* lens) {
+                            int* lens, uint8_t* ws) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ncase) return;
     const int k = ks[i];
     double P[C::MP - 4][4];
     for (int q = 0; q < k && q < C::MP - 4; ++q)
         for (int c = 0; c < 4; ++c) P[q][c] = rows[4 * (starts[i] + q) + c];
-    Complex<C> X;
+    Complex<C>& X = LaneCut<C>::X(ws, (uint64_t)i);
     if (k > C::MP - 4) {
         lens[i] = -2;
         return;
     }
     if (MI)
-        dev::mi_cut<C>(P, k, X);
+        dev::mi_cut<C>(P, k, X, LaneCut<C>::T(ws, (uint64_t)i));
     else
-        dev::ia_cut<C>(P, k, X);
+        dev::ia_cut<C>(P, k, X, LaneCut<C>::T(ws, (uint64_t)i));
     if (X.err) {
         lens[i] = -1;
         return;
@@ -934,6 +1047,28 @@ float debug_copy(DevMem& mem, uint64_t bytes, int reps, hipStream_t s) {
     return ms / reps;
 }
 
+int debug_table_record_mi3(const double* rows, uint16_t* out, int cap) {
+    const MI3Table& T = mi3_table();
+    double P[3][4];
+    for (int q = 0; q < 3; ++q)
+        for (int c = 0; c < 4; ++c) P[q][c] = rows[4 * q + c];
+    int pat = 0;
+    for (int c = 0, w = 1; c < 4; ++c, w *= 3) pat += (P[1][c] > P[0][c] ? 2 : (P[1][c] < P[0][c] ? 0 : 1)) * w;
+    const MI3Pat& K = T.pat[pat];
+    bool err = !K.valid;
+    uint32_t slot = K.base, w = 1;
+    for (int q = 0; q < K.nv && !err; ++q, w *= 3) {
+        const int lab[4] = {K.lab[q][0], K.lab[q][1], K.lab[q][2], K.lab[q][3]};
+        slot += (uint32_t)(dev::orient_mi(P, lab, P[2], err) + 1) * w;
+    }
+    if (err || T.off[slot] == MI3_NONE) return -1;
+    const uint16_t* r = T.words.data() + T.off[slot];
+    const int len = r[6];
+    if (len > cap) return -2;
+    std::memcpy(out, r, 2 * (size_t)len);
+    return len;
+}
+
 int debug_table_record_mi2(const double* rows, uint16_t* out, int cap) {
     const MI2Table& T = mi2_table();
     int pat = 0;
@@ -962,15 +1097,16 @@ void debug_cuts(DevMem& mem, int mi, const double* rows, const int* ks, int ncas
     RIN_HIP(hipMemcpyAsync(dks.p, ks, 4 * (size_t)ncase, hipMemcpyHostToDevice, s));
     RIN_HIP(hipMemcpyAsync(dst.p, starts.data(), 4 * (size_t)ncase, hipMemcpyHostToDevice, s));
     const unsigned nb = (unsigned)((ncase + 63) / 64);
+    DBuf<uint8_t> dws(mem, mi == 2 ? 1 : LaneCut<CBig>::stride * 64 * nb);
     if (mi == 2)
         hipLaunchKernelGGL((k_debug_cut_wave<CBig>), dim3((unsigned)ncase), dim3(64), 0, s, (const double*)drows.p,
                            (const int*)dks.p, (const int*)dst.p, ncase, cap, dw.p, dlens.p);
     else if (mi)
         hipLaunchKernelGGL((k_debug_cut<CBig, true>), dim3(nb), dim3(64), 0, s, (const double*)drows.p,
-                           (const int*)dks.p, (const int*)dst.p, ncase, cap, dw.p, dlens.p);
+                           (const int*)dks.p, (const int*)dst.p, ncase, cap, dw.p, dlens.p, dws.p);
     else
         hipLaunchKernelGGL((k_debug_cut<CBig, false>), dim3(nb), dim3(64), 0, s, (const double*)drows.p,
-                           (const int*)dks.p, (const int*)dst.p, ncase, cap, dw.p, dlens.p);
+                           (const int*)dks.p, (const int*)dst.p, ncase, cap, dw.p, dlens.p, dws.p);
     RIN_HIP(hipGetLastError());
     RIN_HIP(hipMemcpyAsync(words, dw.p, 2 * (size_t)ncase * cap, hipMemcpyDeviceToHost, s));
     RIN_HIP(hipMemcpyAsync(lens, dlens.p, 4 * (size_t)ncase, hipMemcpyDeviceToHost, s));

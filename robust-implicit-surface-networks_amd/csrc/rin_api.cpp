@@ -356,13 +356,15 @@ int rin_debug_cuts(rin_ctx* c, int32_t mi, const double* rows, const int32_t* ks
 }
 
 int rin_debug_table_cuts(int32_t mode, const double* rows, int32_t n, int32_t cap, uint16_t* words, int32_t* lens) {
-    if (!rows || !words || !lens || n < 0 || cap <= 8 || mode < 0 || mode > 2) return RIN_ERR_ARG;
+    if (!rows || !words || !lens || n < 0 || cap <= 8 || mode < 0 || mode > 3) return RIN_ERR_ARG;
     try {
-        const int nr = mode == 0 ? 1 : 2;
+        const int nr = mode == 0 ? 1 : (mode == 3 ? 3 : 2);
         for (int32_t i = 0; i < n; ++i) {
             const double* r = rows + (size_t)4 * nr * i;
             uint16_t* w = words + (size_t)cap * i;
-            lens[i] = mode == 2 ? rin::debug_table_record_mi2(r, w, cap) : rin::debug_table_record_ia(nr, r, w, cap);
+            lens[i] = mode == 3   ? rin::debug_table_record_mi3(r, w, cap)
+                      : mode == 2 ? rin::debug_table_record_mi2(r, w, cap)
+                                  : rin::debug_table_record_ia(nr, r, w, cap);
         }
         return RIN_OK;
     } catch (std::exception&) {

@@ -105,9 +105,10 @@ def descs(types, params):
 def debug_table_cuts(mode, rows, cap=512):
     """Host-only lookup of the tables (no device): mode 0 IA one-function
     table, rows (n, 4); mode 1 IA two-function table, rows (n, 2, 4); mode 2
-    MI two-material table, rows (n, 2, 4).  Returns a list of uint16 records,
-    or negative codes where the table has no entry."""
-    nr = 1 if mode == 0 else 2
+    MI two-material table, rows (n, 2, 4); mode 3 MI three-material table,
+    rows (n, 3, 4).  Returns a list of uint16 records, or negative codes where
+    the table has no entry."""
+    nr = 1 if mode == 0 else (3 if mode == 3 else 2)
     rows = np.ascontiguousarray(np.asarray(rows, dtype=np.float64).reshape(-1, nr * 4))
     n = rows.shape[0]
     words = np.zeros(n * cap, dtype=np.uint16)

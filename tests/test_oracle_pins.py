@@ -388,28 +388,36 @@ def _table_cases(rng, n, nrows):
         rng.choice([-1.0, -0.5, -0.25, 0.0, 0.25, 0.5, 1.0], (n, nrows, 4)),   # dyadic, many ties
         rng.standard_normal((n, nrows, 4)) * np.exp2(rng.integers(-60, 61, (n, nrows, 1))),  # scales
     ]
-    if nrows == 2:
+    if nrows >= 2:
         base = rng.standard_normal((n, 1, 4))
         k = rng.choice([1.0, -1.0, 2.0, 0.5], (n, 1, 1))
-        cats.append(np.concatenate([base, base * k], axis=1))                  # coincident planes
+        extra = [rng.standard_normal((n, 1, 4)) for _ in range(nrows - 2)]
+        cats.append(np.concatenate([base, base * k] + extra, axis=1))          # coincident rows
+    if nrows == 3:
+        # a third material tied with the first or second at corners and edges
+        a = rng.integers(-2, 3, (n, 2, 4)).astype(np.float64)
+        pick = rng.integers(0, 2, n)
+        c = a[np.arange(n), pick][:, None, :] + rng.choice([0.0, 0.0, 1.0, -1.0], (n, 1, 4))
+        cats.append(np.concatenate([a, c], axis=1))
     return np.concatenate(cats)
 
 
-@pytest.mark.parametrize("mode", [0, 1, 2])
+@pytest.mark.parametrize("mode", [0, 1, 2, 3])
 def test_lookup_tables_equal_the_general_cut(mode):
     """The device's tables (read here on the host with the lookup kernels'
-    indexing) give, for every sign case, the oracle's general cut record."""
+    indexing) give, for every sign case, the oracle's general cut record:
+    IA one and two functions, MI two and three materials."""
     import rin
 
     rng = np.random.default_rng(40 + mode)
-    nrows = 1 if mode == 0 else 2
+    nrows = {0: 1, 1: 2, 2: 2, 3: 3}[mode]
     rows = _table_cases(rng, 4000, nrows)
     if mode < 2:
         rows = np.array([r for r in rows if all(_active(x) for x in r)])
     out = rin.debug_table_cuts(mode, rows)
     n_exact = 0
     for r, got in zip(rows, out):
-        want = S.oracle_cut(1 if mode == 2 else 0, r)
+        want = S.oracle_cut(1 if mode >= 2 else 0, r)
         assert not isinstance(got, int), (r, got)
         np.testing.assert_array_equal(got, want, err_msg=str(r))
         n_exact += 1
