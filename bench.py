@@ -378,8 +378,7 @@ def main():
     ap.add_argument("--form", choices=["both", "plain", "batched"], default="both")
     ap.add_argument("--serve-jobs", type=int, default=48, help="serving stream length (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=20.0, help="serving arrivals per second (Poisson)")
-    ap.add_argument("--serve-slots", type=int, default=1,
-                    help="server slots (the library currently runs one job at a time)")
+    ap.add_argument("--serve-slots", type=int, default=4, help="server slots (concurrent jobs, one stream each)")
     ap.add_argument("--mi-steps", type=int, default=10,
                     help="timed steps of the material-interface leg (0: skip)")
     ap.add_argument("--mi-only", action="store_true",

@@ -60,6 +60,8 @@ typedef struct rin_func_desc {
 #define RIN_FLAG_KEEP_RECORDS 16u /* keep per-tet cut records for rin_result_records */
 #define RIN_FLAG_CUT_ONLY 32u    /* stop after the per-tet cut (the -R robustness check) */
 #define RIN_FLAG_STAGE_TIMES 64u /* with BATCHED: per-stage device times (one graph per stage; slower) */
+#define RIN_FLAG_FIXED_FORMS 128u /* with BATCHED: no tuning launches on a new plan; the k=2 table and the
+                                     wave-per-tet cuts (for one-off jobs, e.g. serving) */
 
 typedef struct rin_sizes {
     uint64_t n_tets, n_funcs;

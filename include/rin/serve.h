@@ -2,15 +2,16 @@
  * rin/serve.h — serving loop over the per-tet hot path (no counterpart in the
  * reference, whose drivers run one arrangement per process).
  *
- * A server owns one GPU and n_slots worker slots (currently at most one slot
- * runs: see rin_server_create in the implementation); each slot has its own
+ * A server owns one GPU and n_slots worker slots; each slot has its own
  * context (HIP stream and device-memory pool) and keeps the background grids
  * it has built resident, keyed by resolution.  Jobs (an implicit arrangement
  * or a material interface over a regular grid on [-1,1]^3 with the given
  * functions) queue in arrival order and start as soon as a slot is free and
- * their estimated device memory fits the server's HBM budget, so jobs that
- * arrive while others run are admitted as slots and memory free up, and their
- * work overlaps on the device.  Each finished job keeps its mesh (the
+ * their estimated device memory fits the server's HBM budget
+ * (rin_server_admits), so jobs that arrive while others run are admitted as
+ * slots and memory free up, and their work overlaps on the device (one
+ * stream per slot).  Implicit arrangements run the batched form, material
+ * interfaces the plain form.  Each finished job keeps its mesh (the
  * rin_result_verts / rin_result_faces arrays of include/rin.h) in host memory
  * until released.  Times are seconds since the server was created.
  */
@@ -67,6 +68,9 @@ int rin_server_job_release(rin_server* server, uint64_t job_id);
 int rin_server_stats_get(rin_server* server, rin_server_stats* stats);
 /* seconds since the server was created (the clock of rin_job_info) */
 double rin_server_now(const rin_server* server);
+/* The admission rule: the head of the queue starts when nothing runs, or when
+ * the memory in use plus its estimate fits the budget (1 = admit). */
+int rin_server_admits(uint64_t in_flight, uint64_t in_use, uint64_t estimate, uint64_t budget);
 
 #ifdef __cplusplus
 }

@@ -1,6 +1,9 @@
 // Serving loop (include/rin/serve.h): FIFO admission by free slot and HBM
-// budget, one context (stream + memory pool) per slot, grids kept resident
-// per slot and resolution, meshes handed back in host memory.
+// budget, one context (stream + memory pool) per slot, so admitted jobs run
+// on separate streams and overlap on the device; grids kept resident per slot
+// and resolution, meshes handed back in host memory.  The cut kernels keep
+// their per-tet state in LDS or device workspaces (at most a few KB of
+// private memory a lane), so several queues can run them at once.
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
@@ -74,7 +77,7 @@ struct rin_server {
                 std::unique_lock<std::mutex> lk(mu);
                 // FIFO: the head job starts when its memory fits (or nothing runs)
                 cv_work.wait(lk, [&] {
-                    return stop || (!queue.empty() && (in_flight == 0 || in_use + queue.front()->est <= budget));
+                    return stop || (!queue.empty() && rin_server_admits(in_flight, in_use, queue.front()->est, budget));
                 });
                 if (stop && queue.empty()) break;
                 if (queue.empty()) continue;
@@ -132,7 +135,10 @@ struct rin_server {
             }
         }
         if ((st = rin_values_eval(c, g, J.funcs.data(), (uint32_t)J.funcs.size(), &v)) != RIN_OK) return fail(st);
-        const uint32_t flags = (J.desc.mode == 1 ? RIN_FLAG_MI : 0u) | J.desc.flags;
+        // implicit arrangements on the batched form (its plan, graph and tables
+        // stay with the slot's grid for the next job of the same size)
+        const uint32_t flags =
+            (J.desc.mode == 1 ? RIN_FLAG_MI : (RIN_FLAG_BATCHED | RIN_FLAG_FIXED_FORMS)) | J.desc.flags;
         st = rin_run(c, g, v, flags, &r);
         if (st == RIN_OK) st = rin_result_sizes(c, r, &J.sizes);
         if (st == RIN_OK) {
@@ -178,16 +184,12 @@ std::shared_ptr<Job> find(rin_server* s, uint64_t id, bool finished_only = false
 
 extern "C" {
 
-// Concurrent slots are off: several queues each running the thread-per-tet
-// cut kernels (whose complexes live in private memory, up to 12.5 KB a lane)
-// exhausted the runtime's per-queue scratch allocation.  Until those kernels
-// keep their state in LDS or device memory, the server runs one job at a time
-// on one context; n_slots is accepted and recorded.
-constexpr int kMaxConcurrentSlots = 1;
+int rin_server_admits(uint64_t in_flight, uint64_t in_use, uint64_t estimate, uint64_t budget) {
+    return in_flight == 0 || in_use + estimate <= budget;
+}
 
 int rin_server_create(int32_t device, int32_t n_slots, uint64_t hbm_budget, rin_server** out) {
     if (!out || n_slots < 1 || n_slots > 16) return RIN_ERR_ARG;
-    n_slots = n_slots < kMaxConcurrentSlots ? n_slots : kMaxConcurrentSlots;
     auto* s = new rin_server;
     s->device = device;
     for (int i = 0; i < n_slots; ++i) {

@@ -2229,6 +2229,11 @@ void run_batched(DevMem& ctx_mem, const Grid& g, const Values& val, uint32_t fla
                 stage_count(w, g, val, s);
             }
             stage_fill(w, g, val, s);
+            // a one-off run takes the usual choice without timing candidates
+            if (P.blocks_per_cu[0] == 0 && (flags & RUN_FIXED_FORMS) && !std::getenv("RIN_CUT_FORM")) {
+                P.blocks_per_cu[0] = -2;
+                P.blocks_per_cu[1] = P.blocks_per_cu[2] = -1;
+            }
             // autotune the bucket cut grids once per plan
             if (P.blocks_per_cu[0] == 0) {
                 RIN_HIP(hipStreamSynchronize(s));

@@ -49,6 +49,7 @@ enum RunFlags : uint32_t {
     RUN_KEEP_RECORDS = 16,
     RUN_CUT_ONLY = 32,   // stop after the per-tet cut (robustness check)
     RUN_STAGE_TIMES = 64,  // batched form: time every stage (one captured graph per stage)
+    RUN_FIXED_FORMS = 128, // batched form: no tuning launches (table for k = 2, wave form beyond)
 };
 
 struct StageTimes {

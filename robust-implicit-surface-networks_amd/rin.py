@@ -19,6 +19,7 @@ FLAG_BATCHED = 8
 FLAG_KEEP_RECORDS = 16
 FLAG_CUT_ONLY = 32
 FLAG_STAGE_TIMES = 64
+FLAG_FIXED_FORMS = 128
 
 STATUS = {0: "ok", -1: "bad argument", -2: "device error", -3: "out of device memory", -4: "capacity exceeded",
           -5: "unsupported", -6: "internal error"}

@@ -48,3 +48,17 @@ def test_serving_library_exports_declared_symbols():
     assert len(names) >= 9
     lib = ctypes.CDLL(os.path.join(PKG, "librin_serve.so"))
     assert not [n for n in names if not hasattr(lib, n)]
+
+
+def test_serving_admission_rule():
+    """rin_server_admits (include/rin/serve.h): the head job starts when
+    nothing runs, or when its estimate fits beside the memory in use."""
+    lib = ctypes.CDLL(os.path.join(PKG, "librin_serve.so"))
+    f = lib.rin_server_admits
+    f.argtypes = [ctypes.c_uint64] * 4
+    f.restype = ctypes.c_int
+    assert f(0, 0, 10**12, 1) == 1          # nothing runs: admit whatever the budget
+    assert f(1, 60, 40, 100) == 1           # fits exactly
+    assert f(1, 61, 40, 100) == 0           # one byte over
+    assert f(3, 0, 100, 100) == 1
+    assert f(2, 90, 0, 80) == 0             # already over budget

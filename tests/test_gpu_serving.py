@@ -16,11 +16,10 @@ def _same(a, b):
                                       y.view(np.uint64) if y.dtype == np.float64 else y)
 
 
-@pytest.mark.skip(reason="concurrent slots are switched off: four queues running the thread-per-tet cut kernels "
-                         "(up to 12.5 KB of private memory a lane) exhausted the runtime's scratch allocation "
-                         "(HSA_STATUS_ERROR_OUT_OF_RESOURCES, even for a 3-wave dispatch); re-enable once those "
-                         "kernels keep their complexes in LDS or device memory")
 def test_results_same_alone_and_in_the_mix():
+    """Jobs submitted faster than they finish run on four slots at once (their
+    work overlaps on the device); every mesh equals, bit for bit, the same
+    job run alone on one slot."""
     import serving
 
     jobs = serving.make_jobs(16, base=100)
@@ -29,21 +28,27 @@ def test_results_same_alone_and_in_the_mix():
     assert mix["failed"] == 0 and alone["failed"] == 0
     assert mix["max_in_flight"] > 1, mix  # jobs overlapped on the device
     assert alone["max_in_flight"] == 1
+    # some pair of jobs ran at the same time on different slots
+    assert any(a.t_start < b.t_end and b.t_start < a.t_end and a.slot != b.slot
+               for i, a in enumerate(infos) for b in infos[i + 1:])
     for m, r in zip(meshes, ref):
         _same(m, r)
     assert mix["p99_ms"] >= mix["p50_ms"] > 0
 
 
-def test_admission_waits_for_memory():
+def test_admission_by_memory_budget():
+    """Four slots: with a budget below one job's estimate the server admits a
+    job only when nothing else runs (start after the previous end); with the
+    default budget the same stream overlaps."""
     import serving
 
-    jobs = serving.make_jobs(6, base=200)
-    # a budget below one job's estimate admits one job at a time
-    st, infos, _ = serving.run_stream(jobs, rate=0.0, slots=1, hbm_budget=1)
+    jobs = serving.make_jobs(8, base=200)
+    st, infos, _ = serving.run_stream(jobs, rate=0.0, slots=4, hbm_budget=1)
     assert st["failed"] == 0 and st["max_in_flight"] == 1
-    starts = sorted(i.t_start for i in infos)
-    ends = sorted(i.t_end for i in infos)
-    assert all(s >= e - 1e-6 for s, e in zip(starts[1:], ends[:-1]))
+    spans = sorted((i.t_start, i.t_end) for i in infos)
+    assert all(b[0] >= a[1] - 1e-6 for a, b in zip(spans, spans[1:]))
+    st2, _, _ = serving.run_stream(jobs, rate=0.0, slots=4)
+    assert st2["failed"] == 0 and st2["max_in_flight"] > 1
 
 
 @pytest.mark.parametrize("mi", [False, True])
