@@ -484,7 +484,7 @@ bool material_interface(bool robust_test, bool use_lookup, bool use_secondary_lo
             push_stat(stats_labels, stats, "num_more_func", s.bucket[2]);
             return ok;
         }
-        rin_result* r = H.run(RIN_FLAG_KEEP_RECORDS | mode);
+        rin_result* r = H.run(RIN_FLAG_BATCHED | RIN_FLAG_KEEP_RECORDS | mode);
         append_stage_times(c, r, timing_labels, timings);
         DeviceMesh m;
         fetch_mesh(c, r, m);

@@ -1547,7 +1547,7 @@ unsigned cube_tiles(const Grid& g) {
     return (unsigned)std::max<uint64_t>(1, (cubes + kCubeTile - 1) / kCubeTile);
 }
 
-void alloc_ws(Workspace& W, const Grid& g, const Caps_& c) {
+void alloc_ws(Workspace& W, const Grid& g, const Caps_& c, hipStream_t s) {
     W.cap = c;
     DevMem& m = W.mem;
     const uint64_t V = g.view.V, T = g.view.T;
@@ -1588,15 +1588,15 @@ void alloc_ws(Workspace& W, const Grid& g, const Caps_& c) {
         const StarTables& ST = star_tables();
         W.star_head.alloc(m, ST.head.size());
         W.star_ent.alloc(m, ST.ent.size());
-        RIN_HIP(hipMemcpy(W.star_head.p, ST.head.data(), 4 * ST.head.size(), hipMemcpyHostToDevice));
-        RIN_HIP(hipMemcpy(W.star_ent.p, ST.ent.data(), ST.ent.size(), hipMemcpyHostToDevice));
+        RIN_HIP(hipMemcpyAsync(W.star_head.p, ST.head.data(), 4 * ST.head.size(), hipMemcpyHostToDevice, s));
+        RIN_HIP(hipMemcpyAsync(W.star_ent.p, ST.ent.data(), ST.ent.size(), hipMemcpyHostToDevice, s));
     }
     W.hbits = 10;
     while (W.hbits < 31 && (1ull << W.hbits) < 2ull * c.vo) ++W.hbits;
     W.hkey.alloc(m, 1ull << W.hbits);
     W.hmx.alloc(m, 1ull << W.hbits);
-    RIN_HIP(hipMemset(W.hkey.p, 0, 8ull << W.hbits));
-    RIN_HIP(hipMemset(W.hmx.p, 0, 4ull << W.hbits));
+    RIN_HIP(hipMemsetAsync(W.hkey.p, 0, 8ull << W.hbits, s));
+    RIN_HIP(hipMemsetAsync(W.hmx.p, 0, 4ull << W.hbits, s));
     W.hash_dirty = false;
     W.fq_a.alloc(m, c.fo + 1);
     W.fq_vs.alloc(m, c.fo + 1);
@@ -1633,17 +1633,21 @@ void alloc_ws(Workspace& W, const Grid& g, const Caps_& c) {
     for (int i = 0; i < 81; ++i) off[i] = K.e[i].rec_off;
     for (uint32_t i = 0; i < K2.nslots; ++i)
         off[81 + i] = K2.off[i] == K2_NONE ? K2_NONE : (uint32_t)K.words.size() + K2.off[i];
-    RIN_HIP(hipMemcpy(W.k1.p, K.e, sizeof(K.e), hipMemcpyHostToDevice));
-    RIN_HIP(hipMemcpy(W.k1_words.p, K.words.data(), 2 * K.words.size(), hipMemcpyHostToDevice));
-    RIN_HIP(hipMemcpy(W.k1_words.p + K.words.size(), K2.words.data(), 2 * K2.words.size(), hipMemcpyHostToDevice));
-    RIN_HIP(hipMemcpy(W.k1_off.p, off.data(), 4 * off.size(), hipMemcpyHostToDevice));
-    RIN_HIP(hipMemcpy(W.k2pat.p, K2.pat, sizeof(K2.pat), hipMemcpyHostToDevice));
-    RIN_HIP(hipMemcpy(W.k2cnt.p, K2.cnt.data(), 4 * K2.cnt.size(), hipMemcpyHostToDevice));
+    RIN_HIP(hipMemcpyAsync(W.k1.p, K.e, sizeof(K.e), hipMemcpyHostToDevice, s));
+    RIN_HIP(hipMemcpyAsync(W.k1_words.p, K.words.data(), 2 * K.words.size(), hipMemcpyHostToDevice, s));
+    RIN_HIP(hipMemcpyAsync(W.k1_words.p + K.words.size(), K2.words.data(), 2 * K2.words.size(), hipMemcpyHostToDevice, s));
+    RIN_HIP(hipMemcpyAsync(W.k1_off.p, off.data(), 4 * off.size(), hipMemcpyHostToDevice, s));
+    RIN_HIP(hipMemcpyAsync(W.k2pat.p, K2.pat, sizeof(K2.pat), hipMemcpyHostToDevice, s));
+    RIN_HIP(hipMemcpyAsync(W.k2cnt.p, K2.cnt.data(), 4 * K2.cnt.size(), hipMemcpyHostToDevice, s));
     const K2Plans& KP = k2_plans();
     W.k2plan_off.alloc(m, KP.off.size() + 1);
     W.k2plan.alloc(m, KP.bytes.size() + 1);
-    RIN_HIP(hipMemcpy(W.k2plan_off.p, KP.off.data(), 4 * KP.off.size(), hipMemcpyHostToDevice));
-    RIN_HIP(hipMemcpy(W.k2plan.p, KP.bytes.data(), KP.bytes.size(), hipMemcpyHostToDevice));
+    RIN_HIP(hipMemcpyAsync(W.k2plan_off.p, KP.off.data(), 4 * KP.off.size(), hipMemcpyHostToDevice, s));
+    RIN_HIP(hipMemcpyAsync(W.k2plan.p, KP.bytes.data(), KP.bytes.size(), hipMemcpyHostToDevice, s));
+    // copies on the context's stream (a synchronous copy would go through the
+    // legacy stream, which another thread's graph capture forbids); `off`
+    // lives until they are done
+    RIN_HIP(hipStreamSynchronize(s));
 }
 
 // stage 1 (also run eagerly to size the workspace): signs, filter, and
@@ -2209,7 +2213,7 @@ void run_batched(DevMem& ctx_mem, const Grid& g, const Values& val, uint32_t fla
                 tiny.active = tiny.fit = 64;
                 for (int q = 0; q < 3; ++q) tiny.b[q] = 64;
                 tiny.vo = tiny.fo = tiny.fv = 64;
-                alloc_ws(w, g, tiny);
+                alloc_ws(w, g, tiny, s);
             }
             stage_count(w, g, val, s);
             read_back(true);
@@ -2225,7 +2229,7 @@ void run_batched(DevMem& ctx_mem, const Grid& g, const Values& val, uint32_t fla
                 grow.vo = need.vo + need.vo / 8;
                 grow.fo = need.fo + need.fo / 8;
                 grow.fv = need.fv + need.fv / 8;
-                alloc_ws(w, g, grow);
+                alloc_ws(w, g, grow, s);
                 stage_count(w, g, val, s);
             }
             stage_fill(w, g, val, s);

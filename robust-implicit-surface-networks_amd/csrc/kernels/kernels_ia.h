@@ -314,7 +314,7 @@ __device__ inline void cut_tet(const GridView& g, const double* __restrict__ val
 template <class C, bool MI>
 __global__ void __launch_bounds__(64) k_cut(GridView g, const double* __restrict__ vals, const uint32_t* __restrict__ act,
                                             const uint8_t* __restrict__ act_k, const uint32_t* __restrict__ csr_off,
-                                            const uint8_t* __restrict__ fit, uint64_t n_active, int kmin, int kmax,
+                                            const uint8_t* __restrict__ fit, Count n_active, int kmin, int kmax,
                                             uint32_t flags, const uint32_t* __restrict__ rec_off,
                                             uint16_t* __restrict__ rec, uint32_t* __restrict__ cnt,
                                             uint32_t* __restrict__ errs, const uint32_t* __restrict__ list,
@@ -817,8 +817,9 @@ __device__ inline int bucket_of_k(int k, int table2) {
     return k > 8 ? 3 : (k > 4 ? 2 : ((table2 && (k == 2 || k == 3)) ? 0 : 1));
 }
 
-__global__ void __launch_bounds__(256) k_bucket_count(uint64_t n, const uint8_t* __restrict__ act_k, int mi,
+__global__ void __launch_bounds__(256) k_bucket_count(Count nc, const uint8_t* __restrict__ act_k, int mi,
                                                       int table2, uint32_t* __restrict__ bc) {
+    const uint64_t n = count_of(nc);
     const uint64_t base = (uint64_t)blockIdx.x * kBucketChunk;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int lo = mi ? 2 : 1;
@@ -891,10 +892,11 @@ __global__ void __launch_bounds__(1024) k_bucket_scan(uint32_t nchunks, uint32_t
     scan_tiles_block(nchunks, bc, kBucketWords, 4, tt);
 }
 
-__global__ void __launch_bounds__(256) k_bucket_fill(uint64_t n, const uint8_t* __restrict__ act_k, int table2,
+__global__ void __launch_bounds__(256) k_bucket_fill(Count nc, const uint8_t* __restrict__ act_k, int table2,
                                                      const uint32_t* __restrict__ bc, uint32_t* __restrict__ l0,
                                                      uint32_t* __restrict__ l1, uint32_t* __restrict__ l2,
                                                      uint32_t* __restrict__ l3) {
+    const uint64_t n = count_of(nc);
     const uint64_t base = (uint64_t)blockIdx.x * kBucketChunk;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;

@@ -101,6 +101,18 @@ struct MITablesDev {
     const uint16_t* w3 = nullptr;
 };
 
+inline MITablesDev mi_tables_view(const MITablesRaw& r) {
+    MITablesDev d;
+    d.e2 = (const MI2Entry*)r.e2;
+    d.w2 = (const uint16_t*)r.w2;
+    d.pat3 = (const MI3Pat*)r.pat3;
+    d.off3 = (const uint32_t*)r.off3;
+    d.cnt3 = (const uint32_t*)r.cnt3;
+    d.cut3 = (const uint32_t*)r.cut3;
+    d.w3 = (const uint16_t*)r.w3;
+    return d;
+}
+
 // Active tets with two or three materials by table (list: their active
 // ranks; the count at n_dev when given, else n).  A tet whose sign vector has
 // no table entry, or whose record outgrows its slot, is marked for the
@@ -276,7 +288,7 @@ struct MIArgs {
     const uint32_t *act, *csr_off, *rank, *rec_loc, *nbr;
     const uint8_t* fit;
     const uint16_t* rec;
-    uint64_t n_active;
+    Count n_active;
 };
 
 // whether tet-boundary face F of active tet a becomes an interface face
@@ -289,7 +301,7 @@ __device__ inline bool mi_bface_emits(const MIArgs& A, const MITet& M, uint64_t 
     const uint32_t t2 = nb >> 2, l2 = nb & 3;
     if (t2 >= M.t) return false;
     const uint32_t r2 = A.rank[t2];
-    if (!(r2 < A.n_active && A.act[r2] == t2)) return false;
+    if (!(r2 < count_of(A.n_active) && A.act[r2] == t2)) return false;
     uint64_t k3[3];
     bool have_k3 = false;
     const MITet N(A.g, A.rec + A.rec_loc[r2], t2, A.csr_off[t2]);
@@ -375,7 +387,7 @@ __global__ void k_mi_nf(MIArgs A, uint32_t* __restrict__ nf) {
 }
 
 __global__ void k_mi_face_keys(MIArgs A, MIFaceKeys K) {
-    const uint64_t n = A.n_active * kBfaceLanes;
+    const uint64_t n = count_of(A.n_active) * kBfaceLanes;
     GRID_STRIDE(idx, n) {
         const uint64_t a = idx / kBfaceLanes;
         const int l0 = (int)(idx % kBfaceLanes);
@@ -404,7 +416,7 @@ __global__ void k_mi_face_keys(MIArgs A, MIFaceKeys K) {
 }
 
 __global__ void k_mi_bface(MIArgs A, MIFaceKeys K, uint64_t* __restrict__ emask) {
-    const uint64_t n = A.n_active * kBfaceLanes;
+    const uint64_t n = count_of(A.n_active) * kBfaceLanes;
     GRID_STRIDE(idx, n) {
         const uint64_t a = idx / kBfaceLanes;
         const int l0 = (int)(idx % kBfaceLanes);
@@ -424,7 +436,7 @@ __global__ void k_mi_bface(MIArgs A, MIFaceKeys K, uint64_t* __restrict__ emask)
                 const uint32_t t2 = nb >> 2, l2 = nb & 3;
                 if (t2 >= t) continue;
                 const uint32_t r2 = A.rank[t2];
-                if (!(r2 < A.n_active && A.act[r2] == t2)) continue;
+                if (!(r2 < count_of(A.n_active) && A.act[r2] == t2)) continue;
                 const uint64_t k0 = K.ft[3 * (uint64_t)slot], k1 = K.ft[3 * (uint64_t)slot + 1],
                                k2 = K.ft[3 * (uint64_t)slot + 2];
                 // the copy: on the shared tet face, as many corners, same key triple
@@ -677,11 +689,11 @@ __global__ void k_vert_fields_mi(GridView g, const double* __restrict__ vals, co
 __global__ void k_mi_face_out(Count nf, const uint32_t* __restrict__ act, const uint32_t* __restrict__ fq_a,
                               const uint16_t* __restrict__ fq_l, const uint8_t* __restrict__ fq_m,
                               const uint32_t* __restrict__ fq_vs, const uint32_t* __restrict__ fve,
-                              const uint32_t* __restrict__ vid_of_occ, uint64_t nfv, uint32_t* __restrict__ f_vstart,
+                              const uint32_t* __restrict__ vid_of_occ, Count nfv_c, uint32_t* __restrict__ f_vstart,
                               uint32_t* __restrict__ f_verts, uint32_t* __restrict__ f_tfstart,
                               uint32_t* __restrict__ f_tf_tet, uint16_t* __restrict__ f_tf_local,
                               uint8_t* __restrict__ f_func) {
-    const uint64_t n = count_of(nf);
+    const uint64_t n = count_of(nf), nfv = count_of(nfv_c);
     GRID_STRIDE(q, nf) {
         const uint32_t b = fq_vs[q], e = (q + 1 < n) ? fq_vs[q + 1] : (uint32_t)nfv;
         f_vstart[q] = b;

@@ -20,6 +20,7 @@ constexpr uint32_t kMaxActive = 10;  // active functions per tet on the device p
 
 struct PlanCache;  // batched form: workspaces and captured graphs per function count (batched.hip)
 std::shared_ptr<PlanCache> make_plan_cache();
+std::shared_ptr<void> make_mi_plan_cache();  // the same for material interfaces (batched_mi.hip)
 
 struct Grid {
     uint64_t id = 0;  // unique per grid made in this process (values record it)
@@ -32,6 +33,7 @@ struct Grid {
     // the batched form's plans live with the grid, so value sets of the same
     // size share workspaces, tuning and captured graphs
     std::shared_ptr<PlanCache> plans;
+    std::shared_ptr<void> mi_plans;
 };
 
 struct Values {
@@ -131,6 +133,19 @@ struct NeedsPlainForm : std::runtime_error {
 // persistent per-bucket cut kernels, whole path captured in a hipGraph
 void run_batched(DevMem& ctx_mem, const Grid& g, const Values& v, uint32_t flags, RunOut& out, hipStream_t s,
                  int n_cu);
+
+// the material-interface lookup tables (2- and 3-material) on the current
+// device, uploaded once per process and device; plain pointers (the kernels'
+// view of them is MITablesDev in kernels_mi.h)
+struct MITablesRaw {
+    const void *e2, *w2, *pat3, *off3, *cnt3, *cut3, *w3;
+};
+MITablesRaw mi_tables_device(hipStream_t s);
+
+// batched form of the material interface (batched_mi.hip): the plain form's
+// stages with device-side counts and capacities, captured as one graph per
+// workspace and replayed; throws NeedsPlainForm for what it does not handle
+void run_batched_mi(DevMem& ctx_mem, const Grid& g, const Values& v, uint32_t flags, RunOut& out, hipStream_t s);
 
 // the tet across every tet face (Grid::nbr), built on first use; synchronises
 // the stream, so call it outside graph capture

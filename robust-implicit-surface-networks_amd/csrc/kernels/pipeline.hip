@@ -182,11 +182,13 @@ const MI3Table& mi3_table() {
     return T;
 }
 
+}  // namespace
+
 // the material-interface tables on each device, uploaded once per process
 // and device (they stay until the process ends)
-MITablesDev mi_tables_dev(hipStream_t s) {
+MITablesRaw mi_tables_device(hipStream_t s) {
     static std::mutex mu;
-    static std::map<int, MITablesDev> per_device;
+    static std::map<int, MITablesRaw> per_device;
     int dev = 0;
     RIN_HIP(hipGetDevice(&dev));
     std::lock_guard<std::mutex> lock(mu);
@@ -200,18 +202,22 @@ MITablesDev mi_tables_dev(hipStream_t s) {
         if (bytes) RIN_HIP(hipMemcpyAsync(p, src, bytes, hipMemcpyHostToDevice, s));
         return p;
     };
-    MITablesDev D;
-    D.e2 = (const MI2Entry*)up(T2.e, sizeof(T2.e));
-    D.w2 = (const uint16_t*)up(T2.words.data(), 2 * T2.words.size());
-    D.pat3 = (const MI3Pat*)up(T3.pat, sizeof(T3.pat));
-    D.off3 = (const uint32_t*)up(T3.off.data(), 4 * T3.off.size());
-    D.cnt3 = (const uint32_t*)up(T3.cnt.data(), 4 * T3.cnt.size());
-    D.cut3 = (const uint32_t*)up(T3.cut.data(), 4 * T3.cut.size());
-    D.w3 = (const uint16_t*)up(T3.words.data(), 2 * T3.words.size());
+    MITablesRaw D;
+    D.e2 = up(T2.e, sizeof(T2.e));
+    D.w2 = up(T2.words.data(), 2 * T2.words.size());
+    D.pat3 = up(T3.pat, sizeof(T3.pat));
+    D.off3 = up(T3.off.data(), 4 * T3.off.size());
+    D.cnt3 = up(T3.cnt.data(), 4 * T3.cnt.size());
+    D.cut3 = up(T3.cut.data(), 4 * T3.cut.size());
+    D.w3 = up(T3.words.data(), 2 * T3.words.size());
     RIN_HIP(hipStreamSynchronize(s));
     per_device[dev] = D;
     return D;
 }
+
+namespace {
+
+MITablesDev mi_tables_dev(hipStream_t s) { return mi_tables_view(mi_tables_device(s)); }
 
 struct Dedup {
     DBuf<uint32_t> vid;        // id of every occurrence
@@ -276,7 +282,7 @@ void extract_mi(DevMem& mem, const Grid& g, const Values& val, RunOut& out, Time
     A.nbr = g.nbr.p;
     A.fit = out.fit.p;
     A.rec = out.rec.p;
-    A.n_active = NA;
+    A.n_active = Count(NA);
     DBuf<uint32_t> cnt(mem, 3 * NA + 3), cv(mem, NA + 1), cf(mem, NA + 1), cfv(mem, NA + 1);
     DBuf<uint64_t> emask(mem, NA + 1);
     if (NA) {
@@ -349,7 +355,7 @@ void extract_mi(DevMem& mem, const Grid& g, const Values& val, RunOut& out, Time
     out.f_func.alloc(mem, 2 * NF + 2);
     hipLaunchKernelGGL(k_mi_face_out, dim3(nblk(NF)), dim3(256), 0, s, Count(NF), (const uint32_t*)out.act.p,
                        (const uint32_t*)fq_a.p, (const uint16_t*)fq_l.p, (const uint8_t*)fq_m.p,
-                       (const uint32_t*)fq_vs.p, (const uint32_t*)fve.p, (const uint32_t*)d.vid.p, NFV,
+                       (const uint32_t*)fq_vs.p, (const uint32_t*)fve.p, (const uint32_t*)d.vid.p, Count(NFV),
                        out.f_vstart.p, out.f_verts.p, out.f_tfstart.p, out.f_tf_tet.p, out.f_tf_local.p,
                        out.f_func.p);
     fill_u32(out.f_vstart.p + NF, (uint32_t)NFV, 1, s);
@@ -370,6 +376,7 @@ void grid_regular(DevMem& mem, int res, const double bmin[3], const double bmax[
     if (res < 1) throw std::invalid_argument("grid resolution must be >= 1");
     g.id = ++g_grid_ids;
     g.plans = make_plan_cache();
+    g.mi_plans = make_mi_plan_cache();
     const uint64_t N = (uint64_t)res + 1;
     g.view = GridView{};
     g.view.regular = 1;
@@ -394,6 +401,7 @@ void grid_explicit(DevMem& mem, const double* pts, uint64_t V, const uint64_t* t
     if (V >= (1ull << 32)) throw std::invalid_argument("too many vertices");
     g.id = ++g_grid_ids;
     g.plans = make_plan_cache();
+    g.mi_plans = make_mi_plan_cache();
     g.view = GridView{};
     g.view.regular = 0;
     g.view.V = V;
@@ -579,10 +587,10 @@ void run_pipeline(DevMem& mem, const Grid& g, const Values& val, uint32_t flags,
     if (NA) {
         const uint32_t nch = (uint32_t)((NA + kBucketChunk - 1) / kBucketChunk);
         DBuf<uint32_t> bc(mem, (uint64_t)kBucketWords * nch + kBucketWords), bo(mem, 16);
-        hipLaunchKernelGGL(k_bucket_count, dim3(nch), dim3(256), 0, s, NA, (const uint8_t*)out.act_k.p, (int)MI,
+        hipLaunchKernelGGL(k_bucket_count, dim3(nch), dim3(256), 0, s, Count(NA), (const uint8_t*)out.act_k.p, (int)MI,
                            (int)mi_table, bc.p);
         hipLaunchKernelGGL(k_bucket_scan, dim3(1), dim3(1024), 0, s, nch, bc.p, bo.p);
-        hipLaunchKernelGGL(k_bucket_fill, dim3(nch), dim3(256), 0, s, NA, (const uint8_t*)out.act_k.p, (int)mi_table,
+        hipLaunchKernelGGL(k_bucket_fill, dim3(nch), dim3(256), 0, s, Count(NA), (const uint8_t*)out.act_k.p, (int)mi_table,
                            (const uint32_t*)bc.p, blist[0], blist[1], blist[2], blist[3]);
         RIN_HIP(hipGetLastError());
         uint32_t hb[16];
@@ -641,7 +649,7 @@ void run_pipeline(DevMem& mem, const Grid& g, const Values& val, uint32_t flags,
         auto go = [&](auto kernel) {
             hipLaunchKernelGGL(kernel, dim3(blocks), dim3(lanes), 0, s, g.view, (const double*)val.vals.p,
                                (const uint32_t*)out.act.p, (const uint8_t*)out.act_k.p,
-                               (const uint32_t*)out.csr_off.p, (const uint8_t*)out.fit.p, nl, kmin, kmax, flags,
+                               (const uint32_t*)out.csr_off.p, (const uint8_t*)out.fit.p, Count(nl), kmin, kmax, flags,
                                (const uint32_t*)out.rec_off.p, out.rec.p, cnt.p, errs.p,
                                (const uint32_t*)blist[1 + q], ws_cut[q].p);
         };

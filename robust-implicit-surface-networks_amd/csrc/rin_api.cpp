@@ -206,7 +206,10 @@ int rin_run(rin_ctx* c, const rin_grid* g, const rin_values* v, uint32_t flags, 
         auto r = std::make_unique<rin_result>();
         if (flags & RIN_FLAG_BATCHED) {
             try {
-                rin::run_batched(c->mem, g->g, v->v, flags, r->r, c->stream, c->n_cu);
+                if (flags & RIN_FLAG_MI)
+                    rin::run_batched_mi(c->mem, g->g, v->v, flags, r->r, c->stream);
+                else
+                    rin::run_batched(c->mem, g->g, v->v, flags, r->r, c->stream, c->n_cu);
             } catch (rin::NeedsPlainForm& e) {
                 // the plain form handles every tet the buckets cannot
                 RIN_HIP(hipStreamSynchronize(c->stream));

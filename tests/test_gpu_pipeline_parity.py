@@ -439,3 +439,73 @@ def test_value_sets_share_the_grid_plan(ctx):
     for vals, _ in sets:
         vals.release()
     g.release()
+
+
+@pytest.mark.parametrize("which", ["config3", "config3_small", "config4", "config4_shuffled", "config1", "mesh"])
+def test_batched_mi_matches_oracle_and_replays(ctx, which):
+    """The batched material interface (one captured graph per workspace:
+    tables for two and three materials, thread cuts beyond, radix-sort dedup)
+    gives the oracle's output bit for bit, on the capturing run and on
+    replays with plain-form runs between them."""
+    import rin
+
+    if which == "mesh":
+        m = np.load(S.REF_TESTS + "/mesh.npz")
+        pts, tets = m["pts"], m["tets"].astype(np.uint64)
+        types, params = S.load_ref_funcs("8-sphere.json")
+        g = ctx.grid_explicit(pts, tets)
+        hv = S.evaluate(types, params, pts)
+        vals = g.upload(hv)
+    else:
+        if which == "config3":
+            res, fs = W.config3()
+        elif which == "config3_small":
+            _, fs = W.config3()
+            res = 24
+        elif which == "config4":
+            res, fs = W.config4()
+        elif which == "config4_shuffled":
+            res, fs = W.config4(shuffle_seed=44)
+        else:
+            res, fs = W.config1()
+        types, params = fs.arrays()
+        g = ctx.grid_regular(res)
+        vals = g.eval(types, params)
+        pts, tets = S.grid(res)
+        hv = S.evaluate(types, params, pts)
+    ref = S.OracleRun(1, pts, tets, hv)
+    for rep in range(3):
+        dev = vals.run(rin.FLAG_MI | rin.FLAG_BATCHED)
+        label = dev.stage_times()[0][0]
+        assert label == "batched path (one graph)", label  # no fallback to the plain form
+        _compare(dev, ref)
+        dev.release()
+        vals.run(rin.FLAG_MI).release()
+    vals.release()
+    g.release()
+
+
+def test_batched_mi_value_sets_share_the_plan(ctx):
+    """Material sets of the same size on one grid share the batched workspace
+    and graph (re-recorded and updated in place on each switch)."""
+    import rin
+
+    res = 40
+    g = ctx.grid_regular(res)
+    pts, tets = S.grid(res)
+    sets = []
+    for seed in (61, 62, 63):
+        rng = np.random.default_rng(seed)
+        fs = W.FuncSet()
+        for _ in range(7):
+            fs.sphere(rng.uniform(-0.8, 0.8, 3), rng.uniform(0.0, 0.2))
+        types, params = fs.arrays()
+        sets.append((g.eval(types, params), S.OracleRun(1, pts, tets, S.evaluate(types, params, pts))))
+    for i in (0, 1, 2, 0, 2, 1):
+        vals, ref = sets[i]
+        dev = vals.run(rin.FLAG_MI | rin.FLAG_BATCHED)
+        _compare(dev, ref)
+        dev.release()
+    for vals, _ in sets:
+        vals.release()
+    g.release()
