@@ -137,6 +137,17 @@ int rin_result_verts(rin_ctx* ctx, const rin_result* res, uint64_t* rec11, doubl
  * func[2*n_faces] (func_index; UINT64_MAX when unused) (src/mesh.h:15-25). */
 int rin_result_faces(rin_ctx* ctx, const rin_result* res, uint64_t* vert_start, uint64_t* verts, uint64_t* tf_start,
                      uint64_t* tf, uint64_t* func);
+/* Iso-edges of the result's faces, computed on the device on first request
+ * (compute_mesh_edges, src/mesh_connectivity.cpp:10-56): edges in order of
+ * first occurrence over (face, j), where face edge j joins face vertices j
+ * and j + 1 (cyclically).  rin_result_edges fills v12[2 * n_edges] (each
+ * edge's vertices, smaller first), start[n_edges + 1] and occ[2 * n_face_verts]
+ * (edge e's (face, j) pairs are occ[2 * start[e] .. 2 * start[e + 1]), in
+ * (face, j) order: face_edge_indices), and edge_of[n_face_verts] (the edge
+ * of face f's edge j at vert_start[f] + j: edges_of_face). */
+int rin_result_edge_count(rin_ctx* ctx, const rin_result* res, uint64_t* n_edges);
+int rin_result_edges(rin_ctx* ctx, const rin_result* res, uint64_t* v12, uint64_t* start, uint64_t* occ,
+                     uint64_t* edge_of);
 /* Per-tet cut records (layout in include/rin/cut_record.h) of the given tets,
  * packed: rec_start[n+1] word offsets into words[cap_words].  Needs
  * RIN_FLAG_KEEP_RECORDS.  Empty tets get an empty record. */
@@ -174,6 +185,15 @@ int rin_debug_table_cuts(int32_t mode, const double* rows, int32_t n_cases, int3
  * of `bytes` (a reference kernel of known traffic for hardware-counter
  * scaling); ms receives the average duration of one copy. */
 int rin_debug_copy(rin_ctx* ctx, uint64_t bytes, int32_t reps, double* ms);
+
+/* Test hook with no counterpart in the reference: the device's exclusive
+ * scans on host data.  fields = 1: out[i] = sum(in[0..i)); fields = 3: three
+ * scans in one pass, field f of element i at in[f * n + i] (stride 1) or
+ * in[i * stride + f], out[f * n + i].  count < n passes the element count
+ * through device memory with capacity n (as the pipeline's stages do);
+ * totals[f] receives each field's sum. */
+int rin_debug_scan(rin_ctx* ctx, int32_t fields, int32_t stride, const uint32_t* in, uint64_t n, uint64_t count,
+                   uint32_t* out, uint32_t* totals);
 
 #ifdef __cplusplus
 }

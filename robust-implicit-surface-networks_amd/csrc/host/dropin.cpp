@@ -289,6 +289,28 @@ void to_faces(const DeviceMesh& m, std::vector<PolygonFace>& faces) {
     }
 }
 
+// iso-edges computed on the device (rin_result_edges), in compute_mesh_edges'
+// form and order (src/mesh_connectivity.cpp:10-56)
+void device_edges(rin_ctx* c, const rin_result* r, const DeviceMesh& m, std::vector<std::vector<size_t>>& eof,
+                  std::vector<Edge>& edges) {
+    uint64_t ne = 0;
+    check(c, rin_result_edge_count(c, r, &ne), "rin_result_edge_count");
+    const uint64_t no = m.s.n_face_verts;
+    std::vector<uint64_t> v12(2 * ne + 1), st(ne + 1), occ(2 * no + 1), eo(no + 1);
+    check(c, rin_result_edges(c, r, v12.data(), st.data(), occ.data(), eo.data()), "rin_result_edges");
+    edges.resize(ne);
+    for (uint64_t e = 0; e < ne; ++e) {
+        Edge& E = edges[e];
+        E.v1 = v12[2 * e];
+        E.v2 = v12[2 * e + 1];
+        E.face_edge_indices.resize(st[e + 1] - st[e]);
+        for (uint64_t q = st[e], k = 0; q < st[e + 1]; ++q, ++k)
+            E.face_edge_indices[k] = {occ[2 * q], occ[2 * q + 1]};
+    }
+    eof.resize(m.s.n_faces);
+    for (uint64_t f = 0; f < m.s.n_faces; ++f) eof[f].assign(eo.begin() + m.vstart[f], eo.begin() + m.vstart[f + 1]);
+}
+
 template <class T>
 void push_stat(std::vector<std::string>& sl, std::vector<size_t>& sv, const char* label, T v) {
     sl.emplace_back(label);
@@ -421,6 +443,7 @@ bool implicit_arrangement(bool robust_test, bool use_lookup, bool use_secondary_
         in.n_func = F;
         for (size_t f = 0; f < F; ++f) in.sample_sign.push_back(funcVals.data[f] > 0);
         in.cuts = &cuts;
+        in.edges_from = [&](std::vector<std::vector<size_t>>& eof, std::vector<Edge>& e) { device_edges(c, r, m, eof, e); };
         rin::GraphOut g;
         const bool ok = rin::run_graph(in, g);
         for (auto& t : g.timings) {
@@ -548,6 +571,7 @@ bool material_interface(bool robust_test, bool use_lookup, bool use_secondary_lo
         in.n_func = F;
         for (size_t f = 0; f < F; ++f) in.sample_value.push_back(funcVals.data[f]);
         in.cuts = &cuts;
+        in.edges_from = [&](std::vector<std::vector<size_t>>& eof, std::vector<Edge>& e) { device_edges(c, r, m, eof, e); };
         rin::GraphOut g;
         const bool ok = rin::run_graph(in, g);
         for (auto& t : g.timings) {

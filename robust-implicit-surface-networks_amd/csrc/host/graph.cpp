@@ -932,7 +932,10 @@ bool run_graph(const GraphIn& in, GraphOut& out) {
         t = t2;
     };
     std::vector<std::vector<size_t>> eof;
-    compute_mesh_edges(faces, eof, out.edges);
+    if (in.edges_from)
+        in.edges_from(eof, out.edges);
+    else
+        compute_mesh_edges(faces, eof, out.edges);
     lap(in.mi ? "edge-face connectivity" : "isoEdge-face connectivity");
     out.stats.emplace_back(in.mi ? "num_MI_edges" : "num_iso_edges", out.edges.size());
 

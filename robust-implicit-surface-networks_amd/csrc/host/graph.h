@@ -5,6 +5,7 @@
 // they need (through CutSource).
 #pragma once
 #include <array>
+#include <functional>
 #include <string>
 #include <utility>
 #include <vector>
@@ -65,6 +66,9 @@ struct GraphIn {
     std::vector<bool> sample_sign;       // IA: sign of each function at vertex 0
     std::vector<double> sample_value;    // MI: value of each function at vertex 0
     CutSource* cuts = nullptr;
+    // when set, the iso-edges come from here (e.g. computed on the device)
+    // instead of compute_mesh_edges over the faces
+    std::function<void(std::vector<std::vector<size_t>>&, std::vector<Edge>&)> edges_from;
 };
 
 struct GraphOut {

@@ -1722,8 +1722,15 @@ void launch_cut(Workspace& W, const Grid& g, const Values& val, int q, int grid,
     }
     if (grid < 0) {  // wave form: one block per bucket entry
         // two tets per wavefront up to 4 active functions (their complexes
-        // rarely fill 32 lanes); the whole wavefront for k = 5..8
+        // rarely fill 32 lanes); the whole wavefront for k = 5..8; -3: four
+        // tets per wavefront (half the waves, each group slower)
         constexpr int G = C::MV > 64 ? 64 : 32;
+        if constexpr (G == 32) {
+            if (grid == -3) {
+                hipLaunchKernelGGL((k_cut_wave<C, 16>), dim3((W.cap.b[q] + 3) / 4), dim3(64), 0, s, A);
+                return;
+            }
+        }
         hipLaunchKernelGGL((k_cut_wave<C, G>), dim3((W.cap.b[q] + 64 / G - 1) / (64 / G)), dim3(64), 0, s, A);
         return;
     }
@@ -2073,6 +2080,7 @@ namespace {
 
 std::string cut_label(const char* base, const Workspace& w, const Plan& P, int q) {
     return std::string(base) + (w.grid[q] == -2 ? " (sign-vector table)"
+                                : w.grid[q] == -3 ? " (lane group of 16 per tet)"
                                 : w.grid[q] < 0 ? " (wave per tet)"
                                                 : " (thread per tet, " + std::to_string(P.blocks_per_cu[q]) +
                                                       " blocks/CU)");
@@ -2244,17 +2252,24 @@ void run_batched(DevMem& ctx_mem, const Grid& g, const Values& val, uint32_t fla
                 // thread form at 1..16 blocks per CU, or the wave form (-1);
                 // RIN_CUT_FORM=wave|thread restricts the choice
                 // k = 2 also by table (-2); RIN_CUT_FORM=table forces that
-                const int cand[7] = {1, 2, 4, 8, 16, -1, -2};
+                const int cand[8] = {1, 2, 4, 8, 16, -1, -3, -2};
                 const char* form = std::getenv("RIN_CUT_FORM");
-                const bool only_wave = form && !std::strcmp(form, "wave");
+                // RIN_CUT_FORM=wave16: the wave form with four tets per
+                // wavefront for k = 3..4
+                const bool only_wave16 = form && !std::strcmp(form, "wave16");
+                const bool only_wave = form && (!std::strcmp(form, "wave") || only_wave16);
                 const bool only_thread = form && !std::strcmp(form, "thread");
                 const bool only_table = form && !std::strcmp(form, "table");
                 for (int q = 0; q < 3; ++q) {
                     float best = 1e30f;
-                    int bestc = only_wave ? -1 : (only_table && q == 0 ? -2 : (q == 2 ? 1 : 4));
-                    for (int ci = 0; ci < 7; ++ci) {
+                    int bestc = only_wave16 && q == 1 ? -3
+                                : only_wave           ? -1
+                                                      : (only_table && q == 0 ? -2 : (q == 2 ? 1 : 4));
+                    for (int ci = 0; ci < 8; ++ci) {
                         const int c = cand[ci];
                         if (c == -2 && (q != 0 || only_wave || only_thread)) continue;
+                        if (c == -3 && q != 1) continue;
+                        if (only_wave16 && q == 1 && c != -3) continue;
                         if (only_table && q == 0 && c != -2) continue;
                         if ((only_wave && c >= 0) || (only_thread && c < 0)) continue;
                         // thread forms keep one complex per lane in the
@@ -2277,7 +2292,7 @@ void run_batched(DevMem& ctx_mem, const Grid& g, const Values& val, uint32_t fla
             }
             for (int q = 0; q < 3; ++q) {
                 w.grid[q] = P.blocks_per_cu[q] < 0 ? P.blocks_per_cu[q] : std::max(1, n_cu * P.blocks_per_cu[q]);
-                w.wave_cut[q] = w.grid[q] == -1;
+                w.wave_cut[q] = w.grid[q] == -1 || w.grid[q] == -3;
             }
             RIN_HIP(hipStreamSynchronize(s));
             w.exec = true;

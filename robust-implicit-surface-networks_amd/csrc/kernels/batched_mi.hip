@@ -38,11 +38,13 @@ enum MICtr : uint32_t {
     M_NFT,         // faces of the active tets' complexes
     M_VO, M_FO, M_FV,  // vertex occurrences, faces, face corners
     M_NV,          // vertices after dedup
+    M_FIT,         // active (tet, material) entries: the CSR's total
     M_ABORT,       // a count exceeded its capacity: later stages process nothing
     M_N
 };
 constexpr uint32_t kMiOverflow = 1u << 8, kMiPlain = 1u << 9;
 constexpr int kMiPub = 32;
+static_assert(M_N <= kMiPub, "published counters");
 
 struct MICaps {
     uint32_t active = 0, fit = 0, words = 0, nft = 0, vo = 0, fo = 0, fv = 0;
@@ -58,6 +60,7 @@ __global__ void k_mi_zero(uint32_t* ctr, uint32_t* errs) {
 // zeroed then, so the cut kernels read nothing out of range
 __global__ void k_mi_check(uint32_t* ctr, const uint32_t* csr_off, uint64_t T, MICaps c, uint32_t* errs) {
     if (threadIdx.x != 0) return;
+    ctr[M_FIT] = csr_off[T];
     bool over = ctr[M_ACTIVE] > c.active || csr_off[T] > c.fit || ctr[M_WORDS] > c.words ||
                 ctr[M_NFT] > c.nft || ctr[M_VO] > c.vo || ctr[M_FO] > c.fo || ctr[M_FV] > c.fv;
     if (over) {
@@ -407,7 +410,7 @@ MICaps grow_caps(const MICaps& have, const uint32_t* pub, uint64_t T) {
     MICaps c;
     const bool ran = !pub[M_ABORT];
     c.active = g(have.active, pub[M_ACTIVE]);
-    c.fit = g(have.fit, 0);
+    c.fit = g(have.fit, pub[M_FIT]);
     c.words = g(have.words, pub[M_WORDS]);
     c.nft = g(have.nft, pub[M_NFT]);
     c.vo = g(have.vo, pub[M_VO]);

@@ -559,6 +559,57 @@ __global__ void k_first_list(Count n, const uint32_t* __restrict__ first, const 
     }
 }
 
+// ------------------------------------------------------------------ iso-edges
+// compute_mesh_edges (src/mesh_connectivity.cpp:10-56) on the device: face
+// edge j of face f joins face vertices j and j + 1 (cyclically); its key is
+// the sorted vertex pair; edge ids are the ranks of first occurrences in
+// (face, j) order, and each edge lists its occurrences in that order.
+
+// occurrence o (a face-vertex position) -> its face, and the edge key
+__global__ void k_edge_keys(Count nf, const uint32_t* __restrict__ vstart, const uint32_t* __restrict__ verts,
+                            uint64_t* __restrict__ key, uint32_t* __restrict__ face_of) {
+    GRID_STRIDE(f, nf) {
+        const uint32_t b = vstart[f], e = vstart[f + 1];
+        for (uint32_t o = b; o < e; ++o) {
+            const uint64_t v1 = verts[o], v2 = verts[o + 1 == e ? b : o + 1];
+            key[o] = v1 < v2 ? (v1 << 32 | v2) : (v2 << 32 | v1);
+            face_of[o] = (uint32_t)f;
+        }
+    }
+}
+
+// per segment of equal keys (sorted positions): where it starts, and for its
+// edge the endpoints and the occurrence count
+__global__ void k_edge_segments(Count n, const uint64_t* __restrict__ skey, const uint32_t* __restrict__ sidx,
+                                const uint32_t* __restrict__ head, const uint32_t* __restrict__ head_excl,
+                                const uint32_t* __restrict__ id_of_occ, uint32_t* __restrict__ seg_pos,
+                                uint32_t* __restrict__ v12, uint32_t* __restrict__ count) {
+    const uint64_t nn = count_of(n);
+    GRID_STRIDE(p, n) {
+        if (!head[p]) continue;
+        const uint32_t g = head_excl[p];
+        seg_pos[g] = (uint32_t)p;
+        uint64_t q = p + 1;
+        while (q < nn && !head[q]) ++q;
+        const uint32_t e = id_of_occ[sidx[p]];
+        v12[2 * (uint64_t)e] = (uint32_t)(skey[p] >> 32);
+        v12[2 * (uint64_t)e + 1] = (uint32_t)skey[p];
+        count[e] = (uint32_t)(q - p);
+    }
+}
+
+// each occurrence into its edge's list, in occurrence order (the sort is stable)
+__global__ void k_edge_fill(Count n, const uint32_t* __restrict__ sidx, const uint32_t* __restrict__ head,
+                            const uint32_t* __restrict__ head_excl, const uint32_t* __restrict__ seg_pos,
+                            const uint32_t* __restrict__ id_of_occ, const uint32_t* __restrict__ start,
+                            uint32_t* __restrict__ occ) {
+    GRID_STRIDE(p, n) {
+        const uint32_t g = head_excl[p] + head[p] - 1;
+        const uint32_t o = sidx[p];
+        occ[start[id_of_occ[o]] + (uint32_t)(p - seg_pos[g])] = o;
+    }
+}
+
 // First-occurrence dedup.  Vertex ids are the ranks of first occurrences in
 // occurrence order, as a stable sort by key gives; every occurrence gets the
 // index `mo` of its key's first occurrence.

@@ -110,6 +110,14 @@ struct RunOut {
     } view;
     std::shared_ptr<void> hold;  // keeps a batched workspace checked out
     void set_views_from_buffers();
+    // iso-edges of the faces (compute_edges), on first request
+    bool have_edges = false;
+    uint64_t n_edges = 0;
+    DBuf<uint32_t> e_v12;       // 2 per edge: the sorted endpoints
+    DBuf<uint32_t> e_start;     // n_edges + 1: each edge's occurrences in e_occ
+    DBuf<uint32_t> e_occ;       // face-vertex positions (face f, position j <-> f_vstart[f] + j)
+    DBuf<uint32_t> e_of_occ;    // edge of each face-vertex position
+    DBuf<uint32_t> e_face_of;   // face of each face-vertex position
 };
 
 void grid_regular(DevMem& mem, int res, const double bmin[3], const double bmax[3], Grid& g, hipStream_t s);
@@ -156,9 +164,16 @@ void ensure_nbr(DevMem& mem, const Grid& g, hipStream_t s);
 void gather_records(DevMem& mem, const RunOut& r, const uint64_t* tets, uint64_t n, uint64_t* rec_start,
                     uint16_t* words, uint64_t cap, hipStream_t s);
 
+// iso-edges of the result's faces, on the device (compute_mesh_edges,
+// src/mesh_connectivity.cpp:10-56); fills the e_* buffers of the result
+void compute_edges(DevMem& mem, RunOut& r, hipStream_t s);
+
 // diagnostics: a plain device copy of `bytes` (reference kernel for
 // hardware-counter scaling); returns the average milliseconds per copy
 float debug_copy(DevMem& mem, uint64_t bytes, int reps, hipStream_t s);
+// the device scans (scan_u32; scan3_u32 over three fields) on host data
+void debug_scan(DevMem& mem, int fields, int stride, const uint32_t* in, uint64_t n, uint64_t count, uint32_t* out,
+                uint32_t* totals, hipStream_t s);
 
 // diagnostics: run the device cut on independent plane/material sets
 void debug_cuts(DevMem& mem, int mi, const double* rows, const int* ks, int ncase, int cap, uint16_t* words,

@@ -1035,6 +1035,85 @@ void gather_records(DevMem& mem, const RunOut& r, const uint64_t* tets, uint64_t
     RIN_HIP(hipStreamSynchronize(s));
 }
 
+void compute_edges(DevMem& mem, RunOut& r, hipStream_t s) {
+    if (r.have_edges) return;
+    const uint64_t NF = r.n_faces, NO = r.n_face_verts, NV = r.n_verts;
+    r.e_of_occ.alloc(mem, NO + 1);
+    r.e_face_of.alloc(mem, NO + 1);
+    r.e_occ.alloc(mem, NO + 1);
+    if (NO == 0) {
+        r.n_edges = 0;
+        r.e_v12.alloc(mem, 2);
+        r.e_start.alloc(mem, 1);
+        fill_u32(r.e_start.p, 0, 1, s);
+        r.have_edges = true;
+        return;
+    }
+    DBuf<uint64_t> key(mem, NO);
+    DBuf<uint32_t> sidx(mem, NO), first(mem, NO), head(mem, NO), hexcl(mem, NO), seg_first(mem, NO),
+        rank(mem, NO), nt(mem, 2), seg_pos(mem, NO);
+    hipLaunchKernelGGL(k_edge_keys, dim3(nblk(NF)), dim3(256), 0, s, Count(NF), r.view.f_vstart, r.view.f_verts,
+                       key.p, r.e_face_of.p);
+    iota_u32(sidx.p, NO, s);
+    sort_pairs(mem, key.p, sidx.p, Count(NO), 32 + bits_for(NV), s);
+    hipLaunchKernelGGL(k_heads, dim3(nblk(NO)), dim3(256), 0, s, Count(NO), (const uint64_t*)key.p,
+                       (const uint32_t*)sidx.p, first.p, head.p);
+    scan_u32(mem, head.p, hexcl.p, NO, nullptr, s);
+    hipLaunchKernelGGL(k_assign, dim3(nblk(NO)), dim3(256), 0, s, Count(NO), (const uint32_t*)sidx.p,
+                       (const uint32_t*)head.p, (const uint32_t*)hexcl.p, seg_first.p);
+    scan_u32(mem, first.p, rank.p, NO, nt.p, s);
+    hipLaunchKernelGGL(k_assign2, dim3(nblk(NO)), dim3(256), 0, s, Count(NO), (const uint32_t*)sidx.p,
+                       (const uint32_t*)head.p, (const uint32_t*)hexcl.p, (const uint32_t*)seg_first.p,
+                       (const uint32_t*)rank.p, r.e_of_occ.p);
+    const uint64_t NE = read1(nt.p, s);
+    r.n_edges = NE;
+    r.e_v12.alloc(mem, 2 * NE + 2);
+    r.e_start.alloc(mem, NE + 1);
+    DBuf<uint32_t> cnt(mem, NE + 1);
+    hipLaunchKernelGGL(k_edge_segments, dim3(nblk(NO)), dim3(256), 0, s, Count(NO), (const uint64_t*)key.p,
+                       (const uint32_t*)sidx.p, (const uint32_t*)head.p, (const uint32_t*)hexcl.p,
+                       (const uint32_t*)r.e_of_occ.p, seg_pos.p, r.e_v12.p, cnt.p);
+    scan_u32(mem, cnt.p, r.e_start.p, NE, r.e_start.p + NE, s);
+    hipLaunchKernelGGL(k_edge_fill, dim3(nblk(NO)), dim3(256), 0, s, Count(NO), (const uint32_t*)sidx.p,
+                       (const uint32_t*)head.p, (const uint32_t*)hexcl.p, (const uint32_t*)seg_pos.p,
+                       (const uint32_t*)r.e_of_occ.p, (const uint32_t*)r.e_start.p, r.e_occ.p);
+    RIN_HIP(hipGetLastError());
+    RIN_HIP(hipStreamSynchronize(s));
+    r.have_edges = true;
+}
+
+void debug_scan(DevMem& mem, int fields, int stride, const uint32_t* in, uint64_t n, uint64_t count, uint32_t* out,
+                uint32_t* totals, hipStream_t s) {
+    if (fields != 1 && fields != 3) throw std::invalid_argument("fields must be 1 or 3");
+    if (stride < 1 || (fields == 1 && stride != 1)) throw std::invalid_argument("bad stride");
+    if (count > n) throw std::invalid_argument("count beyond the capacity");
+    const uint64_t in_words = fields == 1 ? n : (stride == 1 ? 3 * n : (uint64_t)stride * n);
+    DBuf<uint32_t> din(mem, in_words + 1), dout(mem, (uint64_t)fields * n + 1), dtot(mem, 4), dcnt(mem, 1);
+    RIN_HIP(hipMemcpyAsync(din.p, in, 4 * in_words, hipMemcpyHostToDevice, s));
+    fill_u32(dout.p, 0xdeadbeefu, (uint64_t)fields * n + 1, s);
+    fill_u32(dtot.p, 0xdeadbeefu, 4, s);
+    fill_u32(dcnt.p, (uint32_t)count, 1, s);
+    // count below the capacity: a device-side count, clamped as the pipeline's are
+    const Count c = count == n ? Count(n) : Count(dcnt.p, n);
+    if (fields == 1) {
+        scan_u32(mem, din.p, dout.p, c, dtot.p, s);
+    } else {
+        const uint32_t* ins[3];
+        uint32_t* outs[3];
+        uint32_t* tots[3];
+        for (int f = 0; f < 3; ++f) {
+            ins[f] = stride == 1 ? din.p + f * n : din.p + f;
+            outs[f] = dout.p + f * n;
+            tots[f] = dtot.p + f;
+        }
+        scan3_u32(mem, ins, stride, outs, c, tots, s);
+    }
+    RIN_HIP(hipGetLastError());
+    RIN_HIP(hipMemcpyAsync(out, dout.p, 4 * (uint64_t)fields * n, hipMemcpyDeviceToHost, s));
+    RIN_HIP(hipMemcpyAsync(totals, dtot.p, 4 * (uint64_t)fields, hipMemcpyDeviceToHost, s));
+    RIN_HIP(hipStreamSynchronize(s));
+}
+
 float debug_copy(DevMem& mem, uint64_t bytes, int reps, hipStream_t s) {
     const uint64_t n = bytes / 8;
     if (n == 0 || reps < 1) throw std::invalid_argument("copy size and repetitions must be positive");

@@ -310,6 +310,39 @@ int rin_result_faces(rin_ctx* c, const rin_result* res, uint64_t* vstart, uint64
     });
 }
 
+int rin_result_edge_count(rin_ctx* c, const rin_result* res, uint64_t* n_edges) {
+    if (!res || !n_edges) return fail(c, RIN_ERR_ARG, "null argument");
+    return guarded(c, [&] {
+        auto& r = const_cast<rin_result*>(res)->r;  // the edges are cached in the result
+        rin::compute_edges(c->mem, r, c->stream);
+        *n_edges = r.n_edges;
+    });
+}
+
+int rin_result_edges(rin_ctx* c, const rin_result* res, uint64_t* v12, uint64_t* start, uint64_t* occ,
+                     uint64_t* edge_of) {
+    if (!res || !v12 || !start || !occ || !edge_of) return fail(c, RIN_ERR_ARG, "null argument");
+    return guarded(c, [&] {
+        auto& r = const_cast<rin_result*>(res)->r;
+        rin::compute_edges(c->mem, r, c->stream);
+        const uint64_t NE = r.n_edges, NO = r.n_face_verts, NF = r.n_faces;
+        auto a = fetch(r.e_v12.p, 2 * NE, c->stream);
+        auto b = fetch(r.e_start.p, NE + 1, c->stream);
+        auto o = fetch(r.e_occ.p, NO, c->stream);
+        auto e = fetch(r.e_of_occ.p, NO, c->stream);
+        auto f = fetch(r.e_face_of.p, NO, c->stream);
+        auto vs = fetch(r.view.f_vstart, NF + 1, c->stream);
+        for (uint64_t i = 0; i < 2 * NE; ++i) v12[i] = a[i];
+        for (uint64_t i = 0; i <= NE; ++i) start[i] = b[i];
+        for (uint64_t i = 0; i < NO; ++i) {
+            const uint32_t x = o[i], face = f[x];
+            occ[2 * i] = face;
+            occ[2 * i + 1] = x - vs[face];
+            edge_of[i] = e[i];
+        }
+    });
+}
+
 int rin_result_records(rin_ctx* c, const rin_result* res, const uint64_t* tets, uint64_t n, uint64_t* rec_start,
                        uint16_t* words, uint64_t cap) {
     if (!res || (!tets && n) || !rec_start) return fail(c, RIN_ERR_ARG, "null argument");
@@ -378,6 +411,12 @@ int rin_debug_table_cuts(int32_t mode, const double* rows, int32_t n, int32_t ca
 int rin_debug_copy(rin_ctx* c, uint64_t bytes, int32_t reps, double* ms) {
     if (!ms) return fail(c, RIN_ERR_ARG, "null argument");
     return guarded(c, [&] { *ms = rin::debug_copy(c->mem, bytes, reps, c->stream); });
+}
+
+int rin_debug_scan(rin_ctx* c, int32_t fields, int32_t stride, const uint32_t* in, uint64_t n, uint64_t count,
+                   uint32_t* out, uint32_t* totals) {
+    if ((!in && n) || (!out && n) || !totals) return fail(c, RIN_ERR_ARG, "null argument");
+    return guarded(c, [&] { rin::debug_scan(c->mem, fields, stride, in, n, count, out, totals, c->stream); });
 }
 
 }  // extern "C"

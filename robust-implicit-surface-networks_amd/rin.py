@@ -28,6 +28,7 @@ _u64p = ctypes.POINTER(ctypes.c_uint64)
 _f64p = ctypes.POINTER(ctypes.c_double)
 _i32p = ctypes.POINTER(ctypes.c_int32)
 _u16p = ctypes.POINTER(ctypes.c_uint16)
+_u32p = ctypes.POINTER(ctypes.c_uint32)
 _vp = ctypes.c_void_p
 
 
@@ -77,6 +78,8 @@ def lib():
         L.rin_result_faces.argtypes = [_vp, _vp, _u64p, _u64p, _u64p, _u64p, _u64p]
         L.rin_result_records.argtypes = [_vp, _vp, _u64p, ctypes.c_uint64, _u64p, _u16p, ctypes.c_uint64]
         L.rin_result_cut_counts.argtypes = [_vp, _vp, _u64p]
+        L.rin_result_edge_count.argtypes = [_vp, _vp, _u64p]
+        L.rin_result_edges.argtypes = [_vp, _vp, _u64p, _u64p, _u64p, _u64p]
         L.rin_result_stage_times.argtypes = [_vp, _vp, _f64p, ctypes.c_int32]
         L.rin_result_stage_label.argtypes = [_vp, _vp, ctypes.c_int32]
         L.rin_result_stage_label.restype = ctypes.c_char_p
@@ -84,6 +87,8 @@ def lib():
         L.rin_debug_cuts.argtypes = [_vp, ctypes.c_int32, _f64p, _i32p, ctypes.c_int32, ctypes.c_int32, _u16p,
                                      _i32p]
         L.rin_debug_copy.argtypes = [_vp, ctypes.c_uint64, ctypes.c_int32, ctypes.POINTER(ctypes.c_double)]
+        L.rin_debug_scan.argtypes = [_vp, ctypes.c_int32, ctypes.c_int32, _u32p, ctypes.c_uint64, ctypes.c_uint64,
+                                     _u32p, _u32p]
         L.rin_debug_table_cuts.argtypes = [ctypes.c_int32, _f64p, ctypes.c_int32, ctypes.c_int32, _u16p, _i32p]
         _lib = L
     return _lib
@@ -158,6 +163,15 @@ class Context:
         self.check(lib().rin_grid_explicit(self.h, _p(pts, _f64p), pts.shape[0], _p(tets, _u64p), tets.shape[0],
                                            ctypes.byref(g)))
         return Grid(self, g, pts.shape[0], tets.shape[0])
+
+    def debug_scan(self, fields, stride, data, n, count=None):
+        """Device exclusive scans of `data` (uint32; layout as rin_debug_scan)."""
+        data = np.ascontiguousarray(data, dtype=np.uint32)
+        out = np.zeros(fields * n, dtype=np.uint32)
+        tot = np.zeros(fields, dtype=np.uint32)
+        self.check(lib().rin_debug_scan(self.h, fields, stride, _p(data, _u32p), n, n if count is None else count,
+                                        _p(out, _u32p), _p(tot, _u32p)))
+        return out.reshape(fields, n), tot
 
     def debug_copy(self, nbytes, reps=5):
         """Average milliseconds of one plain device copy of nbytes."""
@@ -272,6 +286,20 @@ class Result:
         self.ctx.check(lib().rin_result_records(self.ctx.h, self.h, _p(tets, _u64p), len(tets), _p(st, _u64p),
                                                 _p(w, _u16p), cap))
         return st, w[:int(st[-1])]
+
+    def edges(self):
+        """Iso-edges on the device: (v12 (NE, 2), start (NE + 1), occ (NFV, 2) of
+        (face, j), edge_of (NFV))."""
+        ne = ctypes.c_uint64(0)
+        self.ctx.check(lib().rin_result_edge_count(self.ctx.h, self.h, ctypes.byref(ne)))
+        nfv = self.sizes.n_face_verts
+        v12 = np.zeros((ne.value, 2), dtype=np.uint64)
+        st = np.zeros(ne.value + 1, dtype=np.uint64)
+        occ = np.zeros((nfv, 2), dtype=np.uint64)
+        eo = np.zeros(nfv, dtype=np.uint64)
+        self.ctx.check(lib().rin_result_edges(self.ctx.h, self.h, _p(v12, _u64p), _p(st, _u64p), _p(occ, _u64p),
+                                              _p(eo, _u64p)))
+        return v12, st, occ, eo
 
     def cut_counts(self):
         n = self.sizes.num_intersecting

@@ -1,0 +1,122 @@
+"""Iso-edges on the device (rin_result_edges) against compute_mesh_edges
+(src/mesh_connectivity.cpp:10-56) restated over the oracle's faces: edge ids
+in order of first occurrence over (face, j), each edge's endpoints (smaller
+first), its (face, j) list in that order, and each face's edge list."""
+import numpy as np
+import pytest
+
+import support as S
+import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _edges_of_faces(face_start, face_verts):
+    """The reference's loop: face edge j joins vertices j and j + 1
+    (cyclically); a new sorted pair gets the next id."""
+    ids, v12, occ, eof = {}, [], [], []
+    for f in range(len(face_start) - 1):
+        b, e = int(face_start[f]), int(face_start[f + 1])
+        n = e - b
+        for j in range(n):
+            a, c = int(face_verts[b + j]), int(face_verts[b + (j + 1) % n])
+            k = (a, c) if a < c else (c, a)
+            i = ids.setdefault(k, len(v12))
+            if i == len(v12):
+                v12.append(k)
+                occ.append([])
+            occ[i].append((f, j))
+            eof.append(i)
+    return v12, occ, eof
+
+
+def _check(dev, ref):
+    v12, st, occ, eo = dev.edges()
+    rv, rocc, reof = _edges_of_faces(ref.face_start, ref.face_verts)
+    assert len(v12) == len(rv)
+    np.testing.assert_array_equal(v12, np.array(rv, dtype=np.uint64).reshape(-1, 2))
+    np.testing.assert_array_equal(np.diff(st.astype(np.int64)), [len(o) for o in rocc])
+    flat = np.array([p for o in rocc for p in o], dtype=np.uint64).reshape(-1, 2)
+    np.testing.assert_array_equal(occ, flat)
+    np.testing.assert_array_equal(eo, np.array(reof, dtype=np.uint64))
+
+
+@pytest.mark.parametrize("mi,which", [(0, "config1"), (0, "config4"), (1, "config4"), (1, "config3_small")])
+def test_edges_match_reference_loop(ctx, mi, which):
+    import rin
+
+    if which == "config1":
+        res, fs = W.config1()
+    elif which == "config4":
+        res, fs = W.config4(shuffle_seed=44)
+    else:
+        _, fs = W.config3()
+        res = 24
+    types, params = fs.arrays()
+    g = ctx.grid_regular(res)
+    vals = g.eval(types, params)
+    dev = vals.run(rin.FLAG_MI if mi else 0)
+    pts, tets = S.grid(res)
+    ref = S.OracleRun(mi, pts, tets, S.evaluate(types, params, pts))
+    _check(dev, ref)
+    # cached: a second request returns the same lists
+    a = dev.edges()
+    b = dev.edges()
+    for x, y in zip(a, b):
+        np.testing.assert_array_equal(x, y)
+    dev.release()
+    vals.release()
+    g.release()
+
+
+def test_edges_empty(ctx):
+    """No iso-faces: no edges."""
+    g = ctx.grid_regular(8)
+    fs = W.mixed(np.random.default_rng(3), 0, 0, 0, 0)
+    fs.plane([0.0, 0.0, 5.0], [0.0, 0.0, 1.0])
+    types, params = fs.arrays()
+    vals = g.eval(types, params)
+    dev = vals.run(0)
+    assert dev.sizes.n_faces == 0
+    v12, st, occ, eo = dev.edges()
+    assert v12.shape == (0, 2) and list(st) == [0] and occ.shape == (0, 2) and eo.shape == (0,)
+
+
+def test_edges_headline_properties(ctx):
+    """At the 128-grid, 16-function size: every occurrence's face edge is its
+    edge's vertex pair, each edge's occurrences are listed in (face, j) order,
+    every face-edge position is listed once, and edge ids follow first
+    occurrence (the first occurrences increase with the id)."""
+    res, fs = W.config2()
+    types, params = fs.arrays()
+    g = ctx.grid_regular(res)
+    vals = g.eval(types, params)
+    dev = vals.run(0)
+    vs, v, _, _, _ = dev.faces()
+    v12, st, occ, eo = dev.edges()
+    ne, nfv = len(v12), len(v)
+    assert ne > 0 and st[-1] == nfv
+    f, j = occ[:, 0].astype(np.int64), occ[:, 1].astype(np.int64)
+    pos = vs[f].astype(np.int64) + j
+    size = (vs[f + 1] - vs[f]).astype(np.int64)
+    assert np.all(j < size)
+    nxt = vs[f].astype(np.int64) + (j + 1) % size
+    a, b = v[pos], v[nxt]
+    eid = np.repeat(np.arange(ne), np.diff(st.astype(np.int64)))
+    np.testing.assert_array_equal(np.minimum(a, b), v12[eid, 0])
+    np.testing.assert_array_equal(np.maximum(a, b), v12[eid, 1])
+    assert np.all(v12[:, 0] < v12[:, 1])
+    # every face-edge position once, and edge_of agrees with the lists
+    np.testing.assert_array_equal(np.sort(pos), np.arange(nfv))
+    np.testing.assert_array_equal(eo[pos], eid)
+    # occurrence order within an edge, and first-occurrence order of ids
+    same = eid[1:] == eid[:-1]
+    assert np.all(pos[1:][same] > pos[:-1][same])
+    firsts = pos[st[:-1].astype(np.int64)]
+    assert np.all(np.diff(firsts) > 0)
+    # distinct pairs: no two ids share a key
+    key = v12[:, 0] * (1 << 32) + v12[:, 1]
+    assert len(np.unique(key)) == ne
+    dev.release()
+    vals.release()
+    g.release()

@@ -170,7 +170,7 @@ def test_pipeline_mi_explicit_mesh(ctx):
     _compare(dev, ref)
 
 
-@pytest.mark.parametrize("form", ["auto", "wave", "thread", "table"])
+@pytest.mark.parametrize("form", ["auto", "wave", "wave16", "thread", "table"])
 @pytest.mark.parametrize("which", ["config1", "config2_small", "config4"])
 def test_batched_matches_oracle_and_replays(ctx, which, form, monkeypatch):
     """The batched form (k=1 table, bucket kernels, captured graph) gives the
