@@ -937,6 +937,105 @@ __global__ void k_k2_lookup(K2Args A) {
     }
 }
 
+// The plain form's k = 1 and k = 2 tets by the same tables: the table
+// record is copied into the tet's record slot (the plain form's extraction
+// reads records there), with the tet's output counts.
+struct PlainTab {
+    const K1Entry* k1;
+    const uint16_t* w1;  // k = 1 records (K1Entry::rec_off)
+    const K2Pat* pat;
+    const uint32_t *off2, *cnt2;
+    const uint16_t* w2;  // k = 2 records (off2 per slot; a record's word 6 is its length)
+};
+
+__global__ void k_table_plain(GridView g, const double* __restrict__ vals, const uint32_t* __restrict__ act,
+                              const uint8_t* __restrict__ act_k, const uint32_t* __restrict__ csr_off,
+                              const uint8_t* __restrict__ fit, const uint32_t* __restrict__ list, uint64_t n,
+                              const uint32_t* __restrict__ rec_off, uint16_t* __restrict__ rec,
+                              uint32_t* __restrict__ cnt, uint32_t* __restrict__ errs, PlainTab T) {
+    __shared__ K1Entry k1[81];
+    __shared__ K2Pat pats[81];
+    for (int i = threadIdx.x; i < 81; i += blockDim.x) {
+        k1[i] = T.k1[i];
+        pats[i] = T.pat[i];
+    }
+    __syncthreads();
+    const uint64_t V = g.V;
+    GRID_STRIDE(i, n) {
+        const uint32_t a = list[i];
+        const int k = act_k[a];
+        const uint32_t t = act[a], s0 = csr_off[t];
+        uint32_t v[4];
+        dev::tet_corners(g, t, v);
+        const uint32_t fa = fit[s0];
+        double xa[4];
+        int pa = 0;
+        for (int c = 0, w = 1; c < 4; ++c, w *= 3) {
+            xa[c] = vals[(uint64_t)fa * V + v[c]];
+            pa += (xa[c] > 0 ? 2 : (xa[c] < 0 ? 0 : 1)) * w;
+        }
+        const uint16_t* src = nullptr;
+        uint32_t len = 0, c0 = 0, c1 = 0, c2 = 0;
+        bool err = false;
+        if (k == 1) {
+            const K1Entry& E = k1[pa];
+            err = !E.valid;
+            src = T.w1 + E.rec_off;
+            len = E.rec_len;
+            c0 = E.niv;
+            c1 = E.nif;
+            c2 = E.nfv;
+        } else {
+            const uint32_t fb = fit[s0 + 1];
+            double xb[4];
+            int sB[4];
+            for (int c = 0; c < 4; ++c) {
+                xb[c] = vals[(uint64_t)fb * V + v[c]];
+                sB[c] = xb[c] > 0 ? 1 : (xb[c] < 0 ? -1 : 0);
+            }
+            const K2Pat& K = pats[pa];
+            err = !K.valid;
+            uint32_t slot = K.base, w = 81;
+            for (int c = 0, w3 = 1; c < 4; ++c, w3 *= 3) slot += (uint32_t)(sB[c] + 1) * w3;
+            for (int q = 0; q + 4 < K.nv && !err; ++q, w *= 3) {
+                const int e1 = K.e[q][0], e2 = K.e[q][1];
+                int sg;
+                if (sB[e1] * sB[e2] < 0)
+                    sg = dev::orient_ia_edge(xa[e1], xa[e2], xb[e1], xb[e2], err);
+                else
+                    sg = sB[e1] == sB[e2] ? sB[e1] : (sB[e1] != 0 ? sB[e1] : sB[e2]);
+                slot += (uint32_t)(sg + 1) * w;
+            }
+            const uint32_t off = err ? K2_NONE : T.off2[slot];
+            err = off == K2_NONE;
+            if (!err) {
+                src = T.w2 + off;
+                len = src[6];
+                const uint32_t c3 = T.cnt2[slot];
+                c0 = c3 & 0xFF;
+                c1 = (c3 >> 8) & 0xFF;
+                c2 = c3 >> 16;
+            }
+        }
+        uint16_t* dst = rec + rec_off[a];
+        if (!err && len > rec_off[a + 1] - rec_off[a]) {
+            flag_error(errs, ERR_RECORD);
+            err = true;
+        } else if (err) {
+            flag_error(errs, ERR_CAPACITY);
+        }
+        if (err) {
+            dst[0] = 0;
+            cnt[3 * a] = cnt[3 * a + 1] = cnt[3 * a + 2] = 0;
+            continue;
+        }
+        for (uint32_t q = 0; q < len; ++q) dst[q] = src[q];
+        cnt[3 * a] = c0;
+        cnt[3 * a + 1] = c1;
+        cnt[3 * a + 2] = c2;
+    }
+}
+
 struct ExtractArgs {
     GridView g;
     const uint32_t *act, *csr_off, *rec_loc;
@@ -1352,6 +1451,46 @@ __global__ void k_zero_ctr(uint32_t* ctr, int n, uint32_t* errs) {
 
 
 }  // namespace
+
+// the k = 1 and k = 2 tables on each device, uploaded once per process and
+// device (they stay until the process ends)
+PlainTab plain_tables(hipStream_t s) {
+    static std::mutex mu;
+    static std::map<int, PlainTab> per_device;
+    int dev = 0;
+    RIN_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = per_device.find(dev);
+    if (it != per_device.end()) return it->second;
+    const K1Table& K1 = k1_table();
+    const K2Table& K2 = k2_table();
+    auto up = [&](const void* src, size_t bytes) {
+        void* p = nullptr;
+        RIN_HIP(hipMalloc(&p, bytes ? bytes : 4));
+        if (bytes) RIN_HIP(hipMemcpyAsync(p, src, bytes, hipMemcpyHostToDevice, s));
+        return p;
+    };
+    PlainTab D;
+    D.k1 = static_cast<const K1Entry*>(up(K1.e, sizeof(K1.e)));
+    D.w1 = static_cast<const uint16_t*>(up(K1.words.data(), 2 * K1.words.size()));
+    D.pat = static_cast<const K2Pat*>(up(K2.pat, sizeof(K2.pat)));
+    D.off2 = static_cast<const uint32_t*>(up(K2.off.data(), 4 * K2.off.size()));
+    D.cnt2 = static_cast<const uint32_t*>(up(K2.cnt.data(), 4 * K2.cnt.size()));
+    D.w2 = static_cast<const uint16_t*>(up(K2.words.data(), 2 * K2.words.size()));
+    RIN_HIP(hipStreamSynchronize(s));
+    per_device[dev] = D;
+    return D;
+}
+
+void table_cut_plain(const GridView& g, const double* vals, const uint32_t* act, const uint8_t* act_k,
+                     const uint32_t* csr_off, const uint8_t* fit, const uint32_t* list, uint64_t n,
+                     const uint32_t* rec_off, uint16_t* rec, uint32_t* cnt, uint32_t* errs, hipStream_t s) {
+    if (!n) return;
+    const PlainTab T = plain_tables(s);
+    hipLaunchKernelGGL(k_table_plain, dim3(nblk(n)), dim3(256), 0, s, g, vals, act, act_k, csr_off, fit, list, n,
+                       rec_off, rec, cnt, errs, T);
+    RIN_HIP(hipGetLastError());
+}
 
 // ------------------------------------------------------------------ plan
 

@@ -42,7 +42,9 @@ enum MICtr : uint32_t {
     M_ABORT,       // a count exceeded its capacity: later stages process nothing
     M_N
 };
-constexpr uint32_t kMiOverflow = 1u << 8, kMiPlain = 1u << 9;
+// kMiHuge: a tet has more than 8 materials (the large-capacity cut takes it)
+constexpr uint32_t kMiOverflow = 1u << 8, kMiPlain = 1u << 9, kMiHuge = 1u << 10;
+constexpr unsigned kHugeBlocks = 4;  // lanes of the large-capacity cut: 4 x 64
 constexpr int kMiPub = 32;
 static_assert(M_N <= kMiPub, "published counters");
 
@@ -69,9 +71,10 @@ __global__ void k_mi_check(uint32_t* ctr, const uint32_t* csr_off, uint64_t T, M
     }
     if (ctr[M_ABORT])
         for (int q = 0; q < 4; ++q) ctr[M_LIST + q] = 0;
-    // tets beyond 8 materials, or beyond the table and the thread cuts, go
-    // to the plain form
-    if (ctr[M_LIST + 9] > 8) errs[0] |= kMiPlain;
+    // tets beyond 8 materials take the large-capacity cut; beyond its
+    // capacity the plain form reports them
+    if (ctr[M_LIST + 9] > 8) errs[0] |= kMiHuge;
+    if (ctr[M_LIST + 9] > (uint32_t)kMaxHuge) errs[0] |= kMiPlain;
 }
 
 // the CSR, active list and counts (k_fill with the capacity check)
@@ -163,6 +166,7 @@ struct MIWorkspace {
     DBuf<uint64_t> emask;
     DBuf<uint8_t> ws_cut[2];
     unsigned cut_blocks[2] = {0, 0};
+    DBuf<uint8_t> ws_huge;  // the large-capacity cut's complexes, once a run needs them
     // per complex face, occurrence, face
     DBuf<uint16_t> fl, fw, oj, fq_l;
     DBuf<uint64_t> ft, okey;
@@ -333,6 +337,12 @@ void mi_path(MIWorkspace& W, const Grid& g, const Values& val, const MITablesDev
                        (const uint8_t*)W.act_k.p, (const uint32_t*)W.csr_off.p, (const uint8_t*)W.fit.p,
                        Count(ctr + M_LIST + 2, c.active, abort), 5, 8, (uint32_t)0, (const uint32_t*)W.rec_off.p,
                        W.rec.p, W.cnt.p, W.errs.p, (const uint32_t*)bl[2], W.ws_cut[1].p);
+    if (W.ws_huge.p)
+        hipLaunchKernelGGL(k_cut_huge<true>, dim3(kHugeBlocks), dim3(64), 0, s, G, vals, (const uint32_t*)W.act.p,
+                           (const uint8_t*)W.act_k.p, (const uint32_t*)W.csr_off.p, (const uint8_t*)W.fit.p,
+                           (const uint32_t*)bl[3], Count(ctr + M_LIST + 3, c.active, abort), (uint32_t)0,
+                           (const uint32_t*)W.rec_off.p, W.rec.p, W.cnt.p, W.errs.p, W.ws_huge.p,
+                           (const uint32_t*)nullptr, (uint32_t)0);
     // extraction (src/extract_mesh.cpp:569-986): boundary-face pairing, then
     // occurrences in the reference's creation order
     MIArgs A;
@@ -456,7 +466,7 @@ void run_batched_mi(DevMem& ctx_mem, const Grid& g, const Values& val, uint32_t 
     const uint64_t T = g.view.T;
     uint32_t pub[kMiPub + 1];
     bool ok = false;
-    for (int attempt = 0; attempt < 5 && !ok; ++attempt) {
+    for (int attempt = 0; attempt < 8 && !ok; ++attempt) {
         if (!w.ctr.p) {
             MICaps c0;
             c0.active = (uint32_t)std::min<uint64_t>(T, 1024 + T / 16);
@@ -511,8 +521,16 @@ void run_batched_mi(DevMem& ctx_mem, const Grid& g, const Values& val, uint32_t 
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         std::memcpy(pub, w.pub_h, sizeof(pub));
-        const uint32_t e = pub[kMiPub];
-        if (e & kMiPlain) throw NeedsPlainForm("a tet has more than 8 materials");
+        uint32_t e = pub[kMiPub];
+        if (e & kMiPlain) throw NeedsPlainForm("a tet has more materials than the device cut holds");
+        if ((e & kMiHuge) && !w.ws_huge.p) {
+            // first run with such tets: room for the large-capacity cut, and
+            // the path captured again with its launch
+            w.ws_huge.alloc(w.mem, LaneCut<CHuge>::stride * 64 * kHugeBlocks);
+            w.drop_graph();
+            continue;
+        }
+        e &= ~kMiHuge;
         if (e & kMiOverflow) {
             alloc_mi(w, g, grow_caps(w.cap, pub, T), s);
             continue;

@@ -39,22 +39,37 @@ struct WaveIA {
     uint8_t fflag[MF];  // bit 0: a vertex with s > 0, bit 1: s < 0
     uint16_t fcand[MF];
     uint16_t frank[MF];
-    uint16_t foff[MF];
-    uint32_t cflag[MC];
-    uint16_t crank[MC];
-    uint16_t ckey[CAND];
-    uint16_t cface[CAND];
-    uint8_t cedge[CAND], cfirst[CAND], cvid[CAND];
     // candidate edges by key: key << 16 | smallest candidate index
     static constexpr int HT = CAND <= 64 ? 128 : (CAND <= 128 ? 256 : (CAND <= 512 ? 1024 : 2048));
-    uint32_t ht[HT];
-    uint8_t isov[MV];
     // rows per split cell (new-face edges): an insertion splits at most the
     // cells already there, and up to 4 planes cut a tet into at most 15
     static constexpr int ROWS = MP <= 8 ? 16 : 64;
-    uint8_t tv[ROWS][MFV], te[ROWS][MFV], tb[ROWS][MFV];
-    uint16_t tk[ROWS][MFV];  // collection order of a split cell's edges
-    uint32_t ccnt[ROWS];     // edges collected per split cell
+    // Working arrays of stages that never overlap share their storage: the
+    // crossing edges live from the candidate listing to the face splits, the
+    // cell splits after those, and the record pass after the last insertion
+    // (each stage starts after a group barrier that ends the one before).
+    struct Cand {
+        uint32_t ht[HT];
+        uint16_t ckey[CAND];
+        uint16_t cface[CAND];
+        uint8_t cedge[CAND], cfirst[CAND], cvid[CAND];
+    };
+    struct Cells {
+        uint32_t cflag[MC];
+        uint16_t crank[MC];
+        uint8_t tv[ROWS][MFV], te[ROWS][MFV], tb[ROWS][MFV];
+        uint16_t tk[ROWS][MFV];  // collection order of a split cell's edges
+        uint32_t ccnt[ROWS];     // edges collected per split cell
+    };
+    struct Rec {
+        uint16_t foff[MF];
+        uint8_t isov[MV];
+    };
+    union {
+        Cand c;
+        Cells e;
+        Rec r;
+    } u;
     int nv, nf, nc, dup, err;
 };
 
@@ -129,6 +144,14 @@ __device__ inline uint8_t sign_flags(const int8_t (&sr)[MFV]) {
     return fl;
 }
 
+// The entry after position i of a face's cycle of n entries held in
+// registers (i is an unrolled constant, so both candidates are fixed
+// registers and the choice is one select, not a scan of the row).
+template <int MFV, class T>
+__device__ inline T cyc_next(const T (&r)[MFV], int i, int n) {
+    return i + 1 == n ? r[0] : r[i + 1 < MFV ? i + 1 : 0];
+}
+
 // One side (sgn = +1 / -1) of a face split, streamed over the face's cycle
 // with the crossing vertices inserted; the cycle starts at the first kept
 // position and an edge between two kept positions that were not adjacent
@@ -164,10 +187,7 @@ __device__ inline int split_side_regs(const uint8_t (&fvr)[MFV], const uint8_t (
 #pragma unroll
     for (int i = 0; i < MFV; ++i) {
         if (i < n) {
-            const int w = i + 1 == n ? 0 : i + 1;
-            int sw = 0;
-#pragma unroll
-            for (int q = 0; q < MFV; ++q) sw = q == w ? sr[q] : sw;
+            const int sw = cyc_next<MFV>(sr, i, n);
             visit(fvr[i], sr[i], fer[i]);
             if (sr[i] * sw < 0) visit(cr[i], 0, fer[i]);
         }
@@ -279,7 +299,7 @@ __device__ void wave_ia_cut(WaveIA<C>& X, int k) {
         // strict sign-change edges in (face, edge) order: each lane loads its
         // face's vertex ids and signs in parallel, counts and files them
         constexpr int MFV = C::MFV, CAND = WaveIA<C>::CAND, HT = WaveIA<C>::HT;
-        for (int h = lane; h < HT; h += G) X.ht[h] = 0xFFFFFFFFu;
+        for (int h = lane; h < HT; h += G) X.u.c.ht[h] = 0xFFFFFFFFu;
         uint32_t ncand = 0;
         for (int base = 0; base < nf0; base += G) {
             const int f = base + lane;
@@ -295,9 +315,7 @@ __device__ void wave_ia_cut(WaveIA<C>& X, int k) {
                 for (int i = 0; i < MFV; ++i) sr[i] = i < n ? X.s[fvr[i]] : (int8_t)0;
 #pragma unroll
                 for (int i = 0; i < MFV; ++i) {
-                    int sw = 0;
-#pragma unroll
-                    for (int q = 0; q < MFV; ++q) sw = q == (i + 1 == n ? 0 : i + 1) ? sr[q] : sw;
+                    const int sw = cyc_next<MFV>(sr, i, n);
                     c += i < n && sr[i] * sw < 0;
                 }
             }
@@ -307,20 +325,14 @@ __device__ void wave_ia_cut(WaveIA<C>& X, int k) {
 #pragma unroll
                 for (int i = 0; i < MFV; ++i) {
                     if (i >= n) continue;
-                    const int w = i + 1 == n ? 0 : i + 1;
-                    int sw = 0;
-                    uint8_t vw = 0;
-#pragma unroll
-                    for (int q = 0; q < MFV; ++q) {
-                        sw = q == w ? sr[q] : sw;
-                        vw = q == w ? fvr[q] : vw;
-                    }
+                    const int sw = cyc_next<MFV>(sr, i, n);
+                    const uint8_t vw = cyc_next<MFV>(fvr, i, n);
                     if (sr[i] * sw >= 0) continue;
                     if (o < (uint32_t)CAND) {
                         const uint8_t u = fvr[i];
-                        X.ckey[o] = (uint16_t)((u < vw ? u : vw) << 8 | (u < vw ? vw : u));
-                        X.cface[o] = (uint16_t)f;
-                        X.cedge[o] = (uint8_t)i;
+                        X.u.c.ckey[o] = (uint16_t)((u < vw ? u : vw) << 8 | (u < vw ? vw : u));
+                        X.u.c.cface[o] = (uint16_t)f;
+                        X.u.c.cedge[o] = (uint8_t)i;
                     }
                     ++o;
                 }
@@ -335,44 +347,44 @@ __device__ void wave_ia_cut(WaveIA<C>& X, int k) {
         // smallest candidate index of a key already inserted
         auto first_of = [&](uint32_t key) {
             uint32_t h = home(key);
-            for (int probe = 0; probe < HT && X.ht[h] >> 16 != key; ++probe) h = (h + 1) & (uint32_t)(HT - 1);
-            return X.ht[h] & 0xFFFFu;
+            for (int probe = 0; probe < HT && X.u.c.ht[h] >> 16 != key; ++probe) h = (h + 1) & (uint32_t)(HT - 1);
+            return X.u.c.ht[h] & 0xFFFFu;
         };
         for (int c = lane; c < (int)ncand; c += G) {
-            const uint32_t key = X.ckey[c], packed = key << 16 | (uint32_t)c;
+            const uint32_t key = X.u.c.ckey[c], packed = key << 16 | (uint32_t)c;
             uint32_t h = home(key);
             for (int probe = 0; probe < HT; ++probe) {
-                const uint32_t cur = atomicCAS(&X.ht[h], 0xFFFFFFFFu, packed);
+                const uint32_t cur = atomicCAS(&X.u.c.ht[h], 0xFFFFFFFFu, packed);
                 if (cur == 0xFFFFFFFFu) break;
                 if (cur >> 16 == key) {
-                    atomicMin(&X.ht[h], packed);
+                    atomicMin(&X.u.c.ht[h], packed);
                     break;
                 }
                 h = (h + 1) & (uint32_t)(HT - 1);
             }
         }
         __syncthreads();
-        for (int c = lane; c < (int)ncand; c += G) X.cfirst[c] = first_of(X.ckey[c]) == (uint32_t)c;
+        for (int c = lane; c < (int)ncand; c += G) X.u.c.cfirst[c] = first_of(X.u.c.ckey[c]) == (uint32_t)c;
         __syncthreads();
         const uint32_t nnew = wave_prefix<G>(
-            (int)ncand, [&](int c) { return (uint32_t)X.cfirst[c]; },
+            (int)ncand, [&](int c) { return (uint32_t)X.u.c.cfirst[c]; },
             [&](int c, uint32_t o) {
-                if (X.cfirst[c]) X.cvid[c] = (uint8_t)(nv0 + o);
+                if (X.u.c.cfirst[c]) X.u.c.cvid[c] = (uint8_t)(nv0 + o);
             });
         if (nv0 + (int)nnew > C::MV) return fail();
         __syncthreads();
         for (int c = lane; c < (int)ncand; c += G) {
-            if (X.cfirst[c]) {
-                const int f = X.cface[c];
-                uint8_t t0 = X.fa[f], t1 = X.fe[f][X.cedge[c]], t2 = pid;
+            if (X.u.c.cfirst[c]) {
+                const int f = X.u.c.cface[c];
+                uint8_t t0 = X.fa[f], t1 = X.fe[f][X.u.c.cedge[c]], t2 = pid;
                 sort3(t0, t1, t2);
-                const int id = X.cvid[c];
+                const int id = X.u.c.cvid[c];
                 X.vl[id][0] = t0;
                 X.vl[id][1] = t1;
                 X.vl[id][2] = t2;
                 X.s[id] = 0;
             } else {
-                X.cvid[c] = X.cvid[first_of(X.ckey[c])];
+                X.u.c.cvid[c] = X.u.c.cvid[first_of(X.u.c.ckey[c])];
             }
         }
         __syncthreads();
@@ -413,11 +425,8 @@ __device__ void wave_ia_cut(WaveIA<C>& X, int k) {
             for (int i = 0; i < MFV; ++i) {
                 cr[i] = 0;
                 if (i < n) {
-                    const int w = i + 1 == n ? 0 : i + 1;
-                    int sw = 0;
-#pragma unroll
-                    for (int q = 0; q < MFV; ++q) sw = q == w ? sr[q] : sw;
-                    if (sr[i] * sw < 0) cr[i] = X.cvid[ci++];
+                    const int sw = cyc_next<MFV>(sr, i, n);
+                    if (sr[i] * sw < 0) cr[i] = X.u.c.cvid[ci++];
                 }
             }
             const int dn = nf0 + X.frank[f];
@@ -435,7 +444,7 @@ __device__ void wave_ia_cut(WaveIA<C>& X, int k) {
         }
         if (W::any(ferr)) return fail();
         const int nf1 = nf0 + (int)nmix;
-        for (int c = lane; c < nc0; c += G) X.cflag[c] = 0;
+        for (int c = lane; c < nc0; c += G) X.u.e.cflag[c] = 0;
         __syncthreads();
 
         // cell splits: the positive side keeps the id, the negative side and
@@ -447,21 +456,21 @@ __device__ void wave_ia_cut(WaveIA<C>& X, int k) {
             const uint8_t fl = sign_flags<C::MFV>(sr);
             X.fflag[g] = fl;
             if (fl) {
-                atomicOr(&X.cflag[X.fb[g]], (uint32_t)fl);
-                if (X.fc[g] != NIL) atomicOr(&X.cflag[X.fc[g]], (uint32_t)fl);
+                atomicOr(&X.u.e.cflag[X.fb[g]], (uint32_t)fl);
+                if (X.fc[g] != NIL) atomicOr(&X.u.e.cflag[X.fc[g]], (uint32_t)fl);
             }
         }
         __syncthreads();
         const uint32_t nsplit = wave_prefix<G>(
-            nc0, [&](int c) { return (uint32_t)(X.cflag[c] == 3); },
-            [&](int c, uint32_t o) { X.crank[c] = X.cflag[c] == 3 ? (uint16_t)o : (uint16_t)0xFFFF; });
+            nc0, [&](int c) { return (uint32_t)(X.u.e.cflag[c] == 3); },
+            [&](int c, uint32_t o) { X.u.e.crank[c] = X.u.e.cflag[c] == 3 ? (uint16_t)o : (uint16_t)0xFFFF; });
         if (nf1 + (int)nsplit > C::MF || nc0 + (int)nsplit > WaveIA<C>::MC) return fail();
         __syncthreads();
         for (int g = lane; g < nf1; g += G) {
             if (!(X.fflag[g] & 2)) continue;
             const uint8_t b = X.fb[g], c = X.fc[g];
-            if (X.crank[b] != 0xFFFF) X.fb[g] = (uint8_t)(nc0 + X.crank[b]);
-            if (c != NIL && X.crank[c] != 0xFFFF) X.fc[g] = (uint8_t)(nc0 + X.crank[c]);
+            if (X.u.e.crank[b] != 0xFFFF) X.fb[g] = (uint8_t)(nc0 + X.u.e.crank[b]);
+            if (c != NIL && X.u.e.crank[c] != 0xFFFF) X.fc[g] = (uint8_t)(nc0 + X.u.e.crank[c]);
         }
         __syncthreads();
         // the new face of each split cell, bounded by the edges on the plane
@@ -472,13 +481,13 @@ __device__ void wave_ia_cut(WaveIA<C>& X, int k) {
         // from the earliest edge, always to the earliest edge that continues
         // it: the order of a scan of the faces one by one.
         if (nsplit > (uint32_t)WaveIA<C>::ROWS) return fail();
-        for (int c = lane; c < (int)nsplit; c += G) X.ccnt[c] = 0;
+        for (int c = lane; c < (int)nsplit; c += G) X.u.e.ccnt[c] = 0;
         __syncthreads();
         bool cerr = false;
         for (int g = lane; g < nf1; g += G) {
             const uint8_t b = X.fb[g], cc = X.fc[g];
-            const int rb = (b < nc0 && X.crank[b] != 0xFFFF) ? (int)X.crank[b] : -1;
-            const int rc = (cc != NIL && cc < nc0 && X.crank[cc] != 0xFFFF) ? (int)X.crank[cc] : -1;
+            const int rb = (b < nc0 && X.u.e.crank[b] != 0xFFFF) ? (int)X.u.e.crank[b] : -1;
+            const int rc = (cc != NIL && cc < nc0 && X.u.e.crank[cc] != 0xFFFF) ? (int)X.u.e.crank[cc] : -1;
             if (rb < 0 && rc < 0) continue;
             uint8_t fvr[C::MFV];
             int8_t sr[C::MFV];
@@ -486,37 +495,31 @@ __device__ void wave_ia_cut(WaveIA<C>& X, int k) {
 #pragma unroll
             for (int i = 0; i < C::MFV; ++i) {
                 if (i >= n) continue;
-                const int wi = i + 1 == n ? 0 : i + 1;
-                uint8_t w = 0;
-                int sw = 0;
-#pragma unroll
-                for (int q = 0; q < C::MFV; ++q) {
-                    w = q == wi ? fvr[q] : w;
-                    sw = q == wi ? sr[q] : sw;
-                }
+                const uint8_t w = cyc_next<C::MFV>(fvr, i, n);
+                const int sw = cyc_next<C::MFV>(sr, i, n);
                 const uint8_t a = fvr[i];
                 if (sr[i] != 0 || sw != 0) continue;
                 for (int side = 0; side < 2; ++side) {
                     const int row = side ? rc : rb;
                     if (row < 0) continue;
-                    const uint32_t slot = atomicAdd(&X.ccnt[row], 1u);
+                    const uint32_t slot = atomicAdd(&X.u.e.ccnt[row], 1u);
                     if (slot >= (uint32_t)C::MFV) {
                         cerr = true;
                         continue;
                     }
-                    X.tv[row][slot] = side ? a : w;
-                    X.tb[row][slot] = side ? w : a;
-                    X.te[row][slot] = X.fa[g];
-                    X.tk[row][slot] = (uint16_t)(g << 4 | i);
+                    X.u.e.tv[row][slot] = side ? a : w;
+                    X.u.e.tb[row][slot] = side ? w : a;
+                    X.u.e.te[row][slot] = X.fa[g];
+                    X.u.e.tk[row][slot] = (uint16_t)(g << 4 | i);
                 }
             }
         }
         if (W::any(cerr)) return fail();
         __syncthreads();
         for (int c = lane; c < nc0 && !cerr; c += G) {
-            if (X.crank[c] == 0xFFFF) continue;
-            const int row = X.crank[c];
-            const int ne = (int)X.ccnt[row];
+            if (X.u.e.crank[c] == 0xFFFF) continue;
+            const int row = X.u.e.crank[c];
+            const int ne = (int)X.u.e.ccnt[row];
             if (ne < 3) {
                 cerr = true;
                 break;
@@ -527,10 +530,10 @@ __device__ void wave_ia_cut(WaveIA<C>& X, int k) {
             uint16_t ek[MFV];
 #pragma unroll
             for (int q = 0; q < MFV; ++q) {
-                ea[q] = X.tv[row][q];
-                eb[q] = X.tb[row][q];
-                el[q] = X.te[row][q];
-                ek[q] = q < ne ? X.tk[row][q] : (uint16_t)0xFFFF;
+                ea[q] = X.u.e.tv[row][q];
+                eb[q] = X.u.e.tb[row][q];
+                el[q] = X.u.e.te[row][q];
+                ek[q] = q < ne ? X.u.e.tk[row][q] : (uint16_t)0xFFFF;
             }
             int start = 0;
             uint16_t kmin = ek[0];
@@ -606,9 +609,9 @@ __device__ int wave_ia_record(WaveIA<C>& X, int k, uint16_t* out, int cap, uint3
     const int lane = W::lane();
     const int L = 4 + k, nv = X.nv, nf = X.nf;
     const uint32_t fwords = wave_prefix<G>(
-        nf, [&](int f) { return (uint32_t)(4 + X.fn[f]); }, [&](int f, uint32_t o) { X.foff[f] = (uint16_t)o; });
+        nf, [&](int f) { return (uint32_t)(4 + X.fn[f]); }, [&](int f, uint32_t o) { X.u.r.foff[f] = (uint16_t)o; });
     const int total = 8 + 4 * nv + (int)fwords + (X.dup ? 2 * L : 0);
-    for (int v = lane; v < nv; v += G) X.isov[v] = 0;
+    for (int v = lane; v < nv; v += G) X.u.r.isov[v] = 0;
     __syncthreads();
     uint32_t cf = 0, cfv = 0;
     for (int f = lane; f < nf; f += G) {
@@ -620,13 +623,13 @@ __device__ int wave_ia_record(WaveIA<C>& X, int k, uint16_t* out, int cap, uint3
         if (!is) continue;
         ++cf;
         cfv += X.fn[f];
-        for (int q = 0; q < X.fn[f]; ++q) X.isov[X.fv[f][q]] = 1;
+        for (int q = 0; q < X.fn[f]; ++q) X.u.r.isov[X.fv[f][q]] = 1;
     }
     nif = W::sum(cf);
     nfv = W::sum(cfv);
     __syncthreads();
     uint32_t cv = 0;
-    for (int v = lane; v < nv; v += G) cv += X.isov[v];
+    for (int v = lane; v < nv; v += G) cv += X.u.r.isov[v];
     niv = W::sum(cv);
     if (total > cap) return -1;
     for (int v = lane; v < nv; v += G) {
@@ -638,7 +641,7 @@ __device__ int wave_ia_record(WaveIA<C>& X, int k, uint16_t* out, int cap, uint3
     }
     const int fb0 = 8 + 4 * nv;
     for (int f = lane; f < nf; f += G) {
-        uint16_t* o = out + fb0 + X.foff[f];
+        uint16_t* o = out + fb0 + X.u.r.foff[f];
         o[0] = X.fa[f];
         o[1] = X.fb[f];
         o[2] = X.fc[f] == NIL ? 0xFFFF : X.fc[f];
@@ -651,7 +654,7 @@ __device__ int wave_ia_record(WaveIA<C>& X, int k, uint16_t* out, int cap, uint3
             out[fb0 + fwords + L + l] = X.ori[l];
         }
     if (total + nf <= cap)
-        for (int f = lane; f < nf; f += G) out[total + f] = (uint16_t)(fb0 + X.foff[f]);
+        for (int f = lane; f < nf; f += G) out[total + f] = (uint16_t)(fb0 + X.u.r.foff[f]);
     if (lane == 0) {
         out[0] = 1;
         out[1] = (uint16_t)L;

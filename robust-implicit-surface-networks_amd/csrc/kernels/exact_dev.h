@@ -251,17 +251,116 @@ __host__ __device__ inline int orient_ia_edge(double f1, double f2, double q1, d
     return a * b;
 }
 
+// the row of M scaled by a power of two so its largest entry lies in [1,2)
+// (normalise_rows for one row)
+template <int N>
+__host__ __device__ inline void normalise_row(double (&r)[N]) {
+    double mx = 0.0;
+    for (int c = 0; c < N; ++c) mx = fmax(mx, fabs(r[c]));
+    if (mx == 0.0) return;
+    int e;
+    frexp(mx, &e);
+    for (int c = 0; c < N; ++c) r[c] = ldexp(r[c], 1 - e);
+}
+
+// sign of det[top; R] for an N x N matrix whose rows 1..N-1 are R, given
+// the values and permanents of R's (N-1)-minors (mv[c], mp[c]: column c
+// left out): the filter of det_sign on that matrix, with the exact
+// expansion when the bound does not decide.  The rows must be normalised.
+template <int N>
+__host__ __device__ inline int det_sign_top(const double (&top)[N], const double (&R)[N - 1][N], const double (&mv)[N],
+                                            const double (&mp)[N], bool& err) {
+    double d = 0.0, p = 0.0;
+    for (int c = 0; c < N; ++c) {
+        const double t = top[c] * mv[c];
+        d += (c & 1) ? -t : t;
+        p += fabs(top[c]) * mp[c];
+    }
+    if (p > 1e-200) {
+        const double bound = 2.0 * N * N * kEps * p;
+        if (d > bound) return 1;
+        if (d < -bound) return -1;
+    }
+    ExactMat<N> E;
+    for (int c = 0; c < N; ++c) {
+        if (top[c] != 0.0 && fabs(top[c]) < 0x1p-100) err = true;
+        E.m[0][c] = top[c];
+    }
+    for (int r = 0; r < N - 1; ++r)
+        for (int c = 0; c < N; ++c) {
+            if (R[r][c] != 0.0 && fabs(R[r][c]) < 0x1p-100) err = true;
+            E.m[r + 1][c] = R[r][c];
+        }
+    const int sgn = det_exact<N>(E);
+    if (sgn == 2) {
+        err = true;
+        return 0;
+    }
+    return sgn;
+}
+
+// the (N-1)-minors of the N-1 rows R (column c left out), by det_fp's
+// recursion on those rows: values and permanents
+template <int N>
+__host__ __device__ inline void row_minors(const double (&R)[N - 1][N], double (&mv)[N], double (&mp)[N]) {
+    constexpr int M = N - 1;
+    double val[1 << N], per[1 << N];
+    for (int m = 0; m < (1 << N); ++m) {
+        val[m] = 0.0;
+        per[m] = 0.0;
+    }
+    for (int c = 0; c < N; ++c) {
+        val[1 << c] = R[M - 1][c];
+        per[1 << c] = fabs(R[M - 1][c]);
+    }
+    for (int s = 2; s <= M; ++s) {
+        const int r = M - s;
+        for (int m = 0; m < (1 << N); ++m) {
+            if (__builtin_popcount(m) != s) continue;
+            double acc = 0.0, pacc = 0.0;
+            int k = 0;
+            for (int c = 0; c < N; ++c) {
+                if (!(m & (1 << c))) continue;
+                const double a = R[r][c];
+                const int sub = m & ~(1 << c);
+                const double t = a * val[sub];
+                acc += (k & 1) ? -t : t;
+                pacc += fabs(a) * per[sub];
+                ++k;
+            }
+            val[m] = acc;
+            per[m] = pacc;
+        }
+    }
+    for (int c = 0; c < N; ++c) {
+        mv[c] = val[((1 << N) - 1) & ~(1 << c)];
+        mp[c] = per[((1 << N) - 1) & ~(1 << c)];
+    }
+}
+
+// sign(det[q; R]) * sign(det[n; R]) for the query row q and the
+// normalisation row n: both determinants share R's minors (the row order
+// differs from [R; q] and [R; n] alike in both, so the product is the same
+// sign).  n must be normalised already (its largest entry 1).
+template <int N>
+__host__ __device__ inline int orient_rows(double (&R)[N - 1][N], double (&q)[N], const double (&n)[N], bool& err) {
+    for (int r = 0; r < N - 1; ++r) normalise_row<N>(R[r]);
+    normalise_row<N>(q);
+    double mv[N], mp[N];
+    row_minors<N>(R, mv, mp);
+    const int a = det_sign_top<N>(q, R, mv, mp, err);
+    const int b = det_sign_top<N>(n, R, mv, mp, err);
+    if (b == 0) err = true;
+    return a * b;
+}
+
+// orient_ia with the plane rows' minors shared between the two determinants
 __host__ __device__ inline int orient_ia(const double (*P)[4], int t0, int t1, int t2, const double* q, bool& err) {
-    // pinned corners and implicit rows without indexed local arrays (which
-    // would live in scratch memory): the free corners come from a bit mask,
-    // and the row order of the implicit planes does not matter (it flips the
-    // signs of both determinants alike)
     unsigned pm = 0;
     if (t0 < 4) pm |= 1u << t0;
     if (t1 < 4) pm |= 1u << t1;
     if (t2 < 4) pm |= 1u << t2;
     const int ni = (t0 >= 4) + (t1 >= 4) + (t2 >= 4);
-    // implicit ids in the order given
     const int i0 = t0 >= 4 ? t0 - 4 : (t1 >= 4 ? t1 - 4 : t2 - 4);
     const int i1 = (t0 >= 4 && t1 >= 4) ? t1 - 4 : t2 - 4;
     const int i2 = t2 - 4;
@@ -277,23 +376,17 @@ __host__ __device__ inline int orient_ia(const double (*P)[4], int t0, int t1, i
     }
     if (ni == 1) return orient_ia_edge(P[i0][c0], P[i0][c1], q[c0], q[c1], err);
     if (ni == 2) {
-        double A[3][3] = {{P[i0][c0], P[i0][c1], P[i0][c2]}, {P[i1][c0], P[i1][c1], P[i1][c2]}, {q[c0], q[c1], q[c2]}};
-        double B[3][3] = {{P[i0][c0], P[i0][c1], P[i0][c2]}, {P[i1][c0], P[i1][c1], P[i1][c2]}, {1.0, 1.0, 1.0}};
-        int a = det_sign<3>(A, err), b = det_sign<3>(B, err);
-        if (b == 0) err = true;
-        return a * b;
+        double R[2][3] = {{P[i0][c0], P[i0][c1], P[i0][c2]}, {P[i1][c0], P[i1][c1], P[i1][c2]}};
+        double qq[3] = {q[c0], q[c1], q[c2]};
+        const double one[3] = {1.0, 1.0, 1.0};
+        return orient_rows<3>(R, qq, one, err);
     }
-    double A[4][4] = {{P[i0][0], P[i0][1], P[i0][2], P[i0][3]},
+    double R[3][4] = {{P[i0][0], P[i0][1], P[i0][2], P[i0][3]},
                       {P[i1][0], P[i1][1], P[i1][2], P[i1][3]},
-                      {P[i2][0], P[i2][1], P[i2][2], P[i2][3]},
-                      {q[0], q[1], q[2], q[3]}};
-    double B[4][4] = {{P[i0][0], P[i0][1], P[i0][2], P[i0][3]},
-                      {P[i1][0], P[i1][1], P[i1][2], P[i1][3]},
-                      {P[i2][0], P[i2][1], P[i2][2], P[i2][3]},
-                      {1.0, 1.0, 1.0, 1.0}};
-    int a = det_sign<4>(A, err), b = det_sign<4>(B, err);
-    if (b == 0) err = true;
-    return a * b;
+                      {P[i2][0], P[i2][1], P[i2][2], P[i2][3]}};
+    double qq[4] = {q[0], q[1], q[2], q[3]};
+    const double one[4] = {1.0, 1.0, 1.0, 1.0};
+    return orient_rows<4>(R, qq, one, err);
 }
 
 // ---------------------------------------------------------------- MI
@@ -305,20 +398,18 @@ template <int S>
 __host__ __device__ inline int orient_mi_n(const double (*P)[4], const int* mats, const int* fr, const double* q,
                                            bool& err) {
     constexpr int N = S + 1;
-    double A[N][N], B[N][N];
+    double R[S][N], qq[N], nn[N];
     for (int r = 0; r < S; ++r) {
-        for (int c = 0; c < S; ++c) A[r][c] = B[r][c] = P[mats[r]][fr[c]];
-        A[r][S] = B[r][S] = -1.0;
+        for (int c = 0; c < S; ++c) R[r][c] = P[mats[r]][fr[c]];
+        R[r][S] = -1.0;
     }
     for (int c = 0; c < S; ++c) {
-        A[S][c] = q[fr[c]];
-        B[S][c] = 1.0;
+        qq[c] = q[fr[c]];
+        nn[c] = 1.0;
     }
-    A[S][S] = -1.0;
-    B[S][S] = 0.0;
-    int a = det_sign<N>(A, err), b = det_sign<N>(B, err);
-    if (b == 0) err = true;
-    return a * b;
+    qq[S] = -1.0;
+    nn[S] = 0.0;
+    return orient_rows<N>(R, qq, nn, err);
 }
 
 __host__ __device__ inline int orient_mi(const double (*P)[4], const int* lab, const double* q, bool& err) {

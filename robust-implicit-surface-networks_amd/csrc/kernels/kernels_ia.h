@@ -337,7 +337,7 @@ template <bool MI>
 __global__ void __launch_bounds__(64) k_cut_huge(GridView g, const double* __restrict__ vals,
                                                  const uint32_t* __restrict__ act, const uint8_t* __restrict__ act_k,
                                                  const uint32_t* __restrict__ csr_off, const uint8_t* __restrict__ fit,
-                                                 const uint32_t* __restrict__ list, uint64_t n, uint32_t flags,
+                                                 const uint32_t* __restrict__ list, Count n, uint32_t flags,
                                                  const uint32_t* __restrict__ rec_off, uint16_t* __restrict__ rec,
                                                  uint32_t* __restrict__ cnt, uint32_t* __restrict__ errs,
                                                  uint8_t* __restrict__ ws, const uint32_t* __restrict__ rec_loc,
@@ -864,8 +864,11 @@ constexpr uint32_t kBucketChunk = 2048;
 constexpr int kBucketWords = 10;
 
 // table2: the material-interface tables take two- and three-material tets
+// table2: 1 material interfaces (two and three materials by table), 2
+// implicit arrangements (one and two functions by table), 0 no tables
 __device__ inline int bucket_of_k(int k, int table2) {
-    return k > 8 ? 3 : (k > 4 ? 2 : ((table2 && (k == 2 || k == 3)) ? 0 : 1));
+    const bool tab = (table2 == 1 && (k == 2 || k == 3)) || (table2 == 2 && k <= 2);
+    return k > 8 ? 3 : (k > 4 ? 2 : (tab ? 0 : 1));
 }
 
 __global__ void __launch_bounds__(256) k_bucket_count(Count nc, const uint8_t* __restrict__ act_k, int mi,

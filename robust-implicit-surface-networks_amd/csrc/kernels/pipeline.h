@@ -171,6 +171,12 @@ void compute_edges(DevMem& mem, RunOut& r, hipStream_t s);
 // diagnostics: a plain device copy of `bytes` (reference kernel for
 // hardware-counter scaling); returns the average milliseconds per copy
 float debug_copy(DevMem& mem, uint64_t bytes, int reps, hipStream_t s);
+// the plain form's k = 1 and k = 2 implicit-arrangement tets (list of active
+// ranks, n of them) by the lookup tables: each table record is copied into
+// the tet's record slot (batched.hip)
+void table_cut_plain(const GridView& g, const double* vals, const uint32_t* act, const uint8_t* act_k,
+                     const uint32_t* csr_off, const uint8_t* fit, const uint32_t* list, uint64_t n,
+                     const uint32_t* rec_off, uint16_t* rec, uint32_t* cnt, uint32_t* errs, hipStream_t s);
 // the device scans (scan_u32; scan3_u32 over three fields) on host data
 void debug_scan(DevMem& mem, int fields, int stride, const uint32_t* in, uint64_t n, uint64_t count, uint32_t* out,
                 uint32_t* totals, hipStream_t s);

@@ -579,7 +579,12 @@ void run_pipeline(DevMem& mem, const Grid& g, const Values& val, uint32_t flags,
     // bucket counts (reference labels: IA 1/2/>=3, MI 2/3/>=4) and the cut
     // lists, built on the device
     // MI with lookups on: two-material tets take their record from the table
-    const bool mi_table = MI && !(flags & (RUN_REVERSE | RUN_NO_LOOKUP));
+    // IA with lookups on: one- and two-function tets likewise (their records
+    // from the k = 1 and k = 2 tables)
+    // (the IA table kernel writes output counts, so the -R check's complex
+    // counts take the general cut)
+    const bool use_tables = !(flags & (RUN_REVERSE | RUN_NO_LOOKUP)) && (MI || !(flags & RUN_CUT_ONLY));
+    const int table_mode = use_tables ? (MI ? 1 : 2) : 0;
     DBuf<uint32_t> blists(mem, 4 * (NA + 1));
     uint32_t* blist[4];
     for (int q = 0; q < 4; ++q) blist[q] = blists.p + (uint64_t)q * (NA + 1);
@@ -588,9 +593,9 @@ void run_pipeline(DevMem& mem, const Grid& g, const Values& val, uint32_t flags,
         const uint32_t nch = (uint32_t)((NA + kBucketChunk - 1) / kBucketChunk);
         DBuf<uint32_t> bc(mem, (uint64_t)kBucketWords * nch + kBucketWords), bo(mem, 16);
         hipLaunchKernelGGL(k_bucket_count, dim3(nch), dim3(256), 0, s, Count(NA), (const uint8_t*)out.act_k.p, (int)MI,
-                           (int)mi_table, bc.p);
+                           table_mode, bc.p);
         hipLaunchKernelGGL(k_bucket_scan, dim3(1), dim3(1024), 0, s, nch, bc.p, bo.p);
-        hipLaunchKernelGGL(k_bucket_fill, dim3(nch), dim3(256), 0, s, Count(NA), (const uint8_t*)out.act_k.p, (int)mi_table,
+        hipLaunchKernelGGL(k_bucket_fill, dim3(nch), dim3(256), 0, s, Count(NA), (const uint8_t*)out.act_k.p, table_mode,
                            (const uint32_t*)bc.p, blist[0], blist[1], blist[2], blist[3]);
         RIN_HIP(hipGetLastError());
         uint32_t hb[16];
@@ -619,7 +624,11 @@ void run_pipeline(DevMem& mem, const Grid& g, const Values& val, uint32_t flags,
     const uint64_t nwords = read1(out.rec_off.p + NA, s);
     out.rec.alloc(mem, nwords + 1);
     DBuf<uint32_t> cnt(mem, 3 * NA + 3);
-    if (bcnt[0]) {
+    if (bcnt[0] && !MI) {
+        table_cut_plain(g.view, (const double*)val.vals.p, (const uint32_t*)out.act.p, (const uint8_t*)out.act_k.p,
+                        (const uint32_t*)out.csr_off.p, (const uint8_t*)out.fit.p, (const uint32_t*)blist[0],
+                        (uint64_t)bcnt[0], (const uint32_t*)out.rec_off.p, out.rec.p, cnt.p, errs.p, s);
+    } else if (bcnt[0]) {
         // two- and three-material tets by table
         const MITablesDev TT = mi_tables_dev(s);
         hipLaunchKernelGGL(k_mi_table, dim3(nblk(bcnt[0])), dim3(256), 0, s, g.view, (const double*)val.vals.p,
@@ -668,7 +677,7 @@ void run_pipeline(DevMem& mem, const Grid& g, const Values& val, uint32_t flags,
         auto go = [&](auto kernel) {
             hipLaunchKernelGGL(kernel, dim3(blocks), dim3(64), 0, s, g.view, (const double*)val.vals.p,
                                (const uint32_t*)out.act.p, (const uint8_t*)out.act_k.p, (const uint32_t*)out.csr_off.p,
-                               (const uint8_t*)out.fit.p, list, nl, flags, (const uint32_t*)out.rec_off.p, out.rec.p,
+                               (const uint8_t*)out.fit.p, list, Count(nl), flags, (const uint32_t*)out.rec_off.p, out.rec.p,
                                cnt.p, errs.p, wsp, loc, fixed_cap);
         };
         MI ? go(k_cut_huge<true>) : go(k_cut_huge<false>);

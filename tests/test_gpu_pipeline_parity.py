@@ -156,6 +156,41 @@ def test_mi_two_material_table(ctx, which):
     g.release()
 
 
+@pytest.mark.parametrize("which", ["config1", "config4_shuffled"])
+def test_plain_ia_tables(ctx, which):
+    """The plain form's one- and two-function tets take their records from
+    the k = 1 and k = 2 tables: every record equals the oracle's cut of the
+    same tet and the general cut's (lookups off), and both runs give the
+    oracle's outputs."""
+    import rin
+
+    if which == "config1":
+        res, fs = W.config1()
+    else:
+        res, fs = W.config4(shuffle_seed=44)
+    types, params = fs.arrays()
+    g = ctx.grid_regular(res)
+    vals = g.eval(types, params)
+    on = vals.run(rin.FLAG_KEEP_RECORDS)
+    off = vals.run(rin.FLAG_NO_LOOKUP | rin.FLAG_KEEP_RECORDS)
+    pts, tets = S.grid(res)
+    ref = S.OracleRun(0, pts, tets, S.evaluate(types, params, pts))
+    assert on.buckets[0] > 0 and on.buckets[1] > 0  # one- and two-function tets present
+    cap = len(ref.words) + 16
+    st, w = on.records(ref.active, cap)
+    np.testing.assert_array_equal(st, ref.rec_start)
+    np.testing.assert_array_equal(w, ref.words)
+    st2, w2 = off.records(ref.active, cap)
+    np.testing.assert_array_equal(st2, st)
+    np.testing.assert_array_equal(w2, w)
+    _compare(on, ref)
+    _compare(off, ref)
+    for r in (on, off):
+        r.release()
+    vals.release()
+    g.release()
+
+
 def test_pipeline_mi_explicit_mesh(ctx):
     """The reference's unstructured eight-sphere material-interface case."""
     import rin
