@@ -117,6 +117,18 @@ int rin_values_download(rin_ctx* ctx, const rin_values* vals, double* vals_rowma
  * app/implicit_arrangement.cpp:59 and tests/test_implicit_networks.cpp:62). */
 int rin_eval_points(rin_ctx* ctx, const double* pts, uint64_t n, const rin_func_desc* funcs, uint32_t n_funcs,
                     double* vals_rowmajor);
+/* No counterpart in the reference: rin_eval_points for callers that cannot
+ * allocate device memory (code under graph capture, tight serving loops).
+ * pts (n x 3), funcs (n_funcs descriptors) and vals_rowmajor (n x n_funcs)
+ * are DEVICE pointers on the context's GPU.  The call allocates nothing: its
+ * one intermediate (the function-major values) lives in a fixed device array
+ * of 2^21 doubles (16 MiB) per GPU, held until the process ends, so
+ * n * n_funcs may not exceed 2097152 (RIN_ERR_ARG beyond).  Runs on the
+ * context's stream and returns when the values are written.  Every call on a
+ * GPU shares that GPU's array: calls that overlap on one GPU (other threads,
+ * other contexts) race. */
+int rin_eval_points_noalloc(rin_ctx* ctx, const double* pts, uint64_t n, const rin_func_desc* funcs,
+                            uint32_t n_funcs, double* vals_rowmajor);
 int rin_values_release(rin_ctx* ctx, rin_values* vals);
 
 /* The hot path: src/implicit_arrangement.cpp:53-402 (IA) or

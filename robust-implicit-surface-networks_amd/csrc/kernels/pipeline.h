@@ -150,6 +150,14 @@ struct MITablesRaw {
 };
 MITablesRaw mi_tables_device(hipStream_t s);
 
+// allocation-free point evaluation (noalloc.hip): values that one call may
+// stage, the device address of the staging array on the current device
+// (cached per device), and the evaluation itself (caller's device buffers)
+constexpr uint64_t kNoallocValues = 2u << 20;
+void* noalloc_scratch_address();
+void eval_points_noalloc(void* scratch_dev, const double* pts, uint64_t n, const FuncDesc* funcs, uint32_t F,
+                         double* rowmajor, hipStream_t s);
+
 // batched form of the material interface (batched_mi.hip): the plain form's
 // stages with device-side counts and capacities, captured as one graph per
 // workspace and replayed; throws NeedsPlainForm for what it does not handle

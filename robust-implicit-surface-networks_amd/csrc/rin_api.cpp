@@ -14,6 +14,7 @@
 struct rin_ctx {
     int device = 0;
     int n_cu = 256;
+    void* scratch = nullptr;  // the allocation-free path's array on this device
     hipStream_t stream = nullptr;
     rin::DevMem mem;
     std::mutex mu;
@@ -95,6 +96,13 @@ int rin_create(rin_ctx** out, int32_t device) {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
         c->n_cu = prop.multiProcessorCount;
+    try {
+        c->scratch = rin::noalloc_scratch_address();
+    } catch (std::exception&) {
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return RIN_ERR_DEVICE;
+    }
     *out = c;
     return RIN_OK;
 }
@@ -178,6 +186,15 @@ int rin_eval_points(rin_ctx* c, const double* pts, uint64_t n, const rin_func_de
     if ((!pts && n) || !f || (!rm && n)) return fail(c, RIN_ERR_ARG, "null argument");
     return guarded(c, [&] {
         rin::eval_points(c->mem, pts, n, reinterpret_cast<const rin::FuncDesc*>(f), F, rm, c->stream);
+    });
+}
+
+int rin_eval_points_noalloc(rin_ctx* c, const double* pts, uint64_t n, const rin_func_desc* f, uint32_t F,
+                            double* rm) {
+    if ((!pts && n) || !f || (!rm && n)) return fail(c, RIN_ERR_ARG, "null argument");
+    return guarded(c, [&] {
+        rin::eval_points_noalloc(c->scratch, pts, n, reinterpret_cast<const rin::FuncDesc*>(f), F, rm, c->stream);
+        RIN_HIP(hipStreamSynchronize(c->stream));
     });
 }
 
