@@ -33,8 +33,8 @@ struct WaveIA {
     uint8_t vl[MV][3];
     int8_t s[MV];
     uint8_t fa[MF], fb[MF], fc[MF], fn[MF];
-    uint8_t fv[MF][MFV];
-    uint8_t fe[MF][MFV];
+    alignas(8) uint8_t fv[MF][MFV];  // rows of 8 entries load as one word
+    alignas(8) uint8_t fe[MF][MFV];
     uint8_t grp[MP], ori[MP];
     uint8_t fflag[MF];  // bit 0: a vertex with s > 0, bit 1: s < 0
     uint16_t fcand[MF];
@@ -84,6 +84,12 @@ struct Grp {
         if (G == 64) return b != 0;
         return ((b >> ((threadIdx.x & 63) & ~(G - 1))) & ((1ull << (G & 63)) - 1)) != 0;
     }
+    // the group's lanes' predicate bits, group lane i at bit i
+    __device__ static uint64_t ballot(bool p) {
+        const uint64_t b = __ballot(p);
+        if (G == 64) return b;
+        return (b >> ((threadIdx.x & 63) & ~(G - 1))) & ((1ull << (G & 63)) - 1);
+    }
     __device__ static uint32_t excl(uint32_t x) {
         const int l = lane();
         uint32_t v = x;
@@ -122,16 +128,47 @@ __device__ inline uint32_t wave_prefix(int n, F count, H store) {
     return base;
 }
 
-// A face's vertex ids and their signs in registers: the row and then the
-// signs load in parallel (two LDS round trips instead of two per vertex).
-// Entries past the vertex count hold sign 0.  Returns the vertex count.
+// The vertex signs of the insertion under way.  When vertex ids fit in 64
+// bits, every lane of the group holds them as two bit masks (positive,
+// negative) built by ballots after the classification, so a face's signs
+// need no LDS reads; otherwise they are read from X.s.
 template <class C>
-__device__ inline int load_face(const WaveIA<C>& X, int f, uint8_t (&fvr)[C::MFV], int8_t (&sr)[C::MFV]) {
+struct SignView {
+    static constexpr bool kMask = C::MV <= 64;
+    uint64_t pos = 0, neg = 0;
+    __device__ int8_t get(const WaveIA<C>& X, uint8_t v) const {
+        if constexpr (kMask) {
+            const unsigned b = v & 63u;
+            return (int8_t)((int)((pos >> b) & 1u) - (int)((neg >> b) & 1u));
+        } else {
+            return X.s[v];
+        }
+    }
+};
+
+// A face row into registers: one 8-byte LDS read when rows are 8 entries
+template <int MFV>
+__device__ inline void load_row(const uint8_t (&row)[MFV], uint8_t (&r)[MFV]) {
+    if constexpr (MFV == 8) {
+        uint64_t w;
+        __builtin_memcpy(&w, row, 8);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) r[i] = (uint8_t)(w >> (8 * i));
+    } else {
+#pragma unroll
+        for (int i = 0; i < MFV; ++i) r[i] = row[i];
+    }
+}
+
+// A face's vertex ids and their signs in registers.  Entries past the
+// vertex count hold sign 0.  Returns the vertex count.
+template <class C>
+__device__ inline int load_face(const WaveIA<C>& X, const SignView<C>& S, int f, uint8_t (&fvr)[C::MFV],
+                                int8_t (&sr)[C::MFV]) {
     const int n = X.fn[f];
+    load_row<C::MFV>(X.fv[f], fvr);
 #pragma unroll
-    for (int i = 0; i < C::MFV; ++i) fvr[i] = X.fv[f][i];
-#pragma unroll
-    for (int i = 0; i < C::MFV; ++i) sr[i] = i < n ? X.s[fvr[i]] : (int8_t)0;
+    for (int i = 0; i < C::MFV; ++i) sr[i] = i < n ? S.get(X, fvr[i]) : (int8_t)0;
     return n;
 }
 
@@ -248,23 +285,45 @@ __device__ void wave_ia_cut(WaveIA<C>& X, int k) {
 
         // classify every vertex against plane j
         bool e = false, ap = false, an = false;
-        for (int v = lane; v < nv0; v += G) {
-            const int sv = orient_ia(X.P, X.vl[v][0], X.vl[v][1], X.vl[v][2], X.P[j], e);
-            X.s[v] = (int8_t)sv;
-            ap |= sv > 0;
-            an |= sv < 0;
+        SignView<C> S;
+        if constexpr (SignView<C>::kMask) {
+            uint64_t pm = 0, nm = 0;
+            for (int r0 = 0; r0 < nv0; r0 += G) {
+                const int v = r0 + lane;
+                int sv = 0;
+                if (v < nv0) {
+                    sv = orient_ia(X.P, X.vl[v][0], X.vl[v][1], X.vl[v][2], X.P[j], e);
+                    X.s[v] = (int8_t)sv;
+                }
+                pm |= W::ballot(sv > 0) << r0;
+                nm |= W::ballot(sv < 0) << r0;
+            }
+            S.pos = pm;
+            S.neg = nm;
+            ap = pm != 0;
+            an = nm != 0;
+        } else {
+            for (int v = lane; v < nv0; v += G) {
+                const int sv = orient_ia(X.P, X.vl[v][0], X.vl[v][1], X.vl[v][2], X.P[j], e);
+                X.s[v] = (int8_t)sv;
+                ap |= sv > 0;
+                an |= sv < 0;
+            }
         }
         const bool any_err = W::any(e), any_pos = W::any(ap), any_neg = W::any(an);
         if (any_err) return fail();
         __syncthreads();
 
-        // a plane containing an existing face joins that face's group
+        // each face's sign flags (kept for the face and cell splits), and a
+        // plane containing an existing face joins that face's group
         int cop = INT_MAX;
         for (int f = lane; f < nf0; f += G) {
             uint8_t fvr[C::MFV];
             int8_t sr[C::MFV];
-            (void)load_face<C>(X, f, fvr, sr);
-            if (sign_flags<C::MFV>(sr) == 0 && f < cop) cop = f;
+            (void)load_face<C>(X, S, f, fvr, sr);
+            const uint8_t fl = sign_flags<C::MFV>(sr);
+            X.fflag[f] = fl;
+            if (fl == 0 && f < cop) cop = f;
         }
         cop = W::min(cop);
         if (cop != INT_MAX) {
@@ -300,19 +359,18 @@ __device__ void wave_ia_cut(WaveIA<C>& X, int k) {
         // face's vertex ids and signs in parallel, counts and files them
         constexpr int MFV = C::MFV, CAND = WaveIA<C>::CAND, HT = WaveIA<C>::HT;
         for (int h = lane; h < HT; h += G) X.u.c.ht[h] = 0xFFFFFFFFu;
-        uint32_t ncand = 0;
+        // only faces with both signs have strict sign changes; the same pass
+        // ranks them for the face splits
+        uint32_t ncand = 0, nmix = 0;
         for (int base = 0; base < nf0; base += G) {
             const int f = base + lane;
             uint8_t fvr[MFV];
             int8_t sr[MFV];
             int n = 0;
             uint32_t c = 0;
-            if (f < nf0) {
-                n = X.fn[f];
-#pragma unroll
-                for (int i = 0; i < MFV; ++i) fvr[i] = X.fv[f][i];
-#pragma unroll
-                for (int i = 0; i < MFV; ++i) sr[i] = i < n ? X.s[fvr[i]] : (int8_t)0;
+            const bool mixed = f < nf0 && X.fflag[f] == 3;
+            if (mixed) {
+                n = load_face<C>(X, S, f, fvr, sr);
 #pragma unroll
                 for (int i = 0; i < MFV; ++i) {
                     const int sw = cyc_next<MFV>(sr, i, n);
@@ -320,7 +378,9 @@ __device__ void wave_ia_cut(WaveIA<C>& X, int k) {
                 }
             }
             uint32_t o = ncand + W::excl(c);
-            if (f < nf0) {
+            const uint32_t m = nmix + W::excl(mixed ? 1u : 0u);
+            if (mixed) {
+                X.frank[f] = (uint16_t)m;
                 X.fcand[f] = (uint16_t)o;
 #pragma unroll
                 for (int i = 0; i < MFV; ++i) {
@@ -338,6 +398,7 @@ __device__ void wave_ia_cut(WaveIA<C>& X, int k) {
                 }
             }
             ncand += W::sum(c);
+            nmix += W::sum(mixed ? 1u : 0u);
         }
         if (ncand > (uint32_t)CAND) return fail();
         __syncthreads();
@@ -391,17 +452,6 @@ __device__ void wave_ia_cut(WaveIA<C>& X, int k) {
 
         // face splits: the positive part keeps the id, the negative part is
         // appended in face order
-        const uint32_t nmix = wave_prefix<G>(
-            nf0,
-            [&](int f) {
-                uint8_t fvr[C::MFV];
-                int8_t sr[C::MFV];
-                (void)load_face<C>(X, f, fvr, sr);
-                const uint8_t fl = sign_flags<C::MFV>(sr);
-                X.fflag[f] = fl;
-                return (uint32_t)(fl == 3);
-            },
-            [&](int f, uint32_t o) { X.frank[f] = (uint16_t)o; });
         if (nf0 + (int)nmix > C::MF) return fail();
         bool ferr = false;
         for (int f = lane; f < nf0; f += G) {
@@ -410,16 +460,10 @@ __device__ void wave_ia_cut(WaveIA<C>& X, int k) {
             // then both sides are streamed without further LDS reads but
             // the crossing vertices; the positive side overwrites the row
             constexpr int MFV = C::MFV;
-            const int n = X.fn[f];
             uint8_t fvr[MFV], fer[MFV], cr[MFV];
             int8_t sr[MFV];
-#pragma unroll
-            for (int i = 0; i < MFV; ++i) {
-                fvr[i] = X.fv[f][i];
-                fer[i] = X.fe[f][i];
-            }
-#pragma unroll
-            for (int i = 0; i < MFV; ++i) sr[i] = i < n ? X.s[fvr[i]] : (int8_t)0;
+            const int n = load_face<C>(X, S, f, fvr, sr);
+            load_row<MFV>(X.fe[f], fer);
             uint32_t ci = X.fcand[f];
 #pragma unroll
             for (int i = 0; i < MFV; ++i) {
@@ -449,11 +493,16 @@ __device__ void wave_ia_cut(WaveIA<C>& X, int k) {
 
         // cell splits: the positive side keeps the id, the negative side and
         // one face on the new plane are appended in cell order
+        // a face's flags after the splits follow from its flags before: a
+        // split face's positive part keeps its positive vertices and gains
+        // crossing vertices of sign 0 (flags 1), its negative part (appended
+        // past nf0) the negative ones (flags 2)
         for (int g = lane; g < nf1; g += G) {
-            uint8_t fvr[C::MFV];
-            int8_t sr[C::MFV];
-            (void)load_face<C>(X, g, fvr, sr);
-            const uint8_t fl = sign_flags<C::MFV>(sr);
+            uint8_t fl = 2;
+            if (g < nf0) {
+                fl = X.fflag[g];
+                if (fl == 3) fl = 1;
+            }
             X.fflag[g] = fl;
             if (fl) {
                 atomicOr(&X.u.e.cflag[X.fb[g]], (uint32_t)fl);
@@ -491,7 +540,7 @@ __device__ void wave_ia_cut(WaveIA<C>& X, int k) {
             if (rb < 0 && rc < 0) continue;
             uint8_t fvr[C::MFV];
             int8_t sr[C::MFV];
-            const int n = load_face<C>(X, g, fvr, sr);
+            const int n = load_face<C>(X, S, g, fvr, sr);
 #pragma unroll
             for (int i = 0; i < C::MFV; ++i) {
                 if (i >= n) continue;
