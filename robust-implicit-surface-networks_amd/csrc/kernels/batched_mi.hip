@@ -164,7 +164,6 @@ struct MIWorkspace {
     DBuf<uint32_t> act, blists, bc, rbound, rec_off, cnt, fb;
     DBuf<uint16_t> rec;
     DBuf<uint64_t> emask;
-    DBuf<uint8_t> ws_cut[2];
     unsigned cut_blocks[2] = {0, 0};
     DBuf<uint8_t> ws_huge;  // the large-capacity cut's complexes, once a run needs them
     // per complex face, occurrence, face
@@ -250,12 +249,11 @@ void alloc_mi(MIWorkspace& W, const Grid& g, const MICaps& c, hipStream_t s) {
     W.cv.alloc(m, A + 2);
     W.cf.alloc(m, A + 2);
     W.cfv.alloc(m, A + 2);
-    // thread cuts (four and five to eight materials): two lanes a wavefront,
-    // persistent grids striding over their lists, state in the workspace
-    W.cut_blocks[0] = (unsigned)std::min<uint64_t>((A + 1) / 2 + 1, kCutLanes / 2);
-    W.cut_blocks[1] = (unsigned)std::min<uint64_t>((A + 1) / 2 + 1, 2048);
-    W.ws_cut[0].alloc(m, LaneCut<CMed>::stride * 2 * W.cut_blocks[0]);
-    W.ws_cut[1].alloc(m, LaneCut<CBig>::stride * 2 * W.cut_blocks[1]);
+    // thread cuts (four and five to eight materials): a lane per tet with
+    // its complex in LDS, two lanes (one for the larger complexes) a
+    // wavefront, grids striding over their lists
+    W.cut_blocks[0] = (unsigned)std::min<uint64_t>((A + 1) / 2 + 1, 4096);
+    W.cut_blocks[1] = (unsigned)std::min<uint64_t>(A + 1, 2048);
     W.fl.alloc(m, c.nft + 1);
     W.fw.alloc(m, c.nft + 1);
     W.ft.alloc(m, 3 * (uint64_t)c.nft + 3);
@@ -329,14 +327,14 @@ void mi_path(MIWorkspace& W, const Grid& g, const Values& val, const MITablesDev
                        (const uint32_t*)W.csr_off.p, (const uint8_t*)W.fit.p, (const uint32_t*)bl[0],
                        (uint64_t)c.active, (const uint32_t*)(ctr + M_LIST), (uint32_t)0, (const uint32_t*)W.rec_off.p,
                        W.rec.p, W.cnt.p, W.errs.p, TT);
-    hipLaunchKernelGGL((k_cut<CMed, true>), dim3(W.cut_blocks[0]), dim3(2), 0, s, G, vals, (const uint32_t*)W.act.p,
-                       (const uint8_t*)W.act_k.p, (const uint32_t*)W.csr_off.p, (const uint8_t*)W.fit.p,
-                       Count(ctr + M_LIST + 1, c.active, abort), 1, 4, (uint32_t)0, (const uint32_t*)W.rec_off.p,
-                       W.rec.p, W.cnt.p, W.errs.p, (const uint32_t*)bl[1], W.ws_cut[0].p);
-    hipLaunchKernelGGL((k_cut<CBig, true>), dim3(W.cut_blocks[1]), dim3(2), 0, s, G, vals, (const uint32_t*)W.act.p,
-                       (const uint8_t*)W.act_k.p, (const uint32_t*)W.csr_off.p, (const uint8_t*)W.fit.p,
-                       Count(ctr + M_LIST + 2, c.active, abort), 5, 8, (uint32_t)0, (const uint32_t*)W.rec_off.p,
-                       W.rec.p, W.cnt.p, W.errs.p, (const uint32_t*)bl[2], W.ws_cut[1].p);
+    hipLaunchKernelGGL((k_cut_lds<CMed, true, 2>), dim3(W.cut_blocks[0]), dim3(2), 0, s, G, vals,
+                       (const uint32_t*)W.act.p, (const uint8_t*)W.act_k.p, (const uint32_t*)W.csr_off.p,
+                       (const uint8_t*)W.fit.p, Count(ctr + M_LIST + 1, c.active, abort), 1, 4, (uint32_t)0,
+                       (const uint32_t*)W.rec_off.p, W.rec.p, W.cnt.p, W.errs.p, (const uint32_t*)bl[1]);
+    hipLaunchKernelGGL((k_cut_lds<CBig, true, 1>), dim3(W.cut_blocks[1]), dim3(1), 0, s, G, vals,
+                       (const uint32_t*)W.act.p, (const uint8_t*)W.act_k.p, (const uint32_t*)W.csr_off.p,
+                       (const uint8_t*)W.fit.p, Count(ctr + M_LIST + 2, c.active, abort), 5, 8, (uint32_t)0,
+                       (const uint32_t*)W.rec_off.p, W.rec.p, W.cnt.p, W.errs.p, (const uint32_t*)bl[2]);
     if (W.ws_huge.p)
         hipLaunchKernelGGL(k_cut_huge<true>, dim3(kHugeBlocks), dim3(64), 0, s, G, vals, (const uint32_t*)W.act.p,
                            (const uint8_t*)W.act_k.p, (const uint32_t*)W.csr_off.p, (const uint8_t*)W.fit.p,

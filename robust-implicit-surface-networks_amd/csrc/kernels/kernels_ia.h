@@ -330,6 +330,33 @@ __global__ void __launch_bounds__(64) k_cut(GridView g, const double* __restrict
     }
 }
 
+// The same with each lane's complex and working arrays in LDS (LANES lanes a
+// block, blockDim.x == LANES): for short lists, where one lane's latency
+// through its cut is the kernel's time and LDS answers several times faster
+// than the device-memory workspace.
+template <class C, bool MI, int LANES>
+__global__ void __launch_bounds__(64) k_cut_lds(GridView g, const double* __restrict__ vals,
+                                                const uint32_t* __restrict__ act, const uint8_t* __restrict__ act_k,
+                                                const uint32_t* __restrict__ csr_off, const uint8_t* __restrict__ fit,
+                                                Count n_active, int kmin, int kmax, uint32_t flags,
+                                                const uint32_t* __restrict__ rec_off, uint16_t* __restrict__ rec,
+                                                uint32_t* __restrict__ cnt, uint32_t* __restrict__ errs,
+                                                const uint32_t* __restrict__ list) {
+    __shared__ Complex<C> Xs[LANES];
+    __shared__ dev::InsertTemps<C> Ts[LANES];
+    const int l = threadIdx.x < LANES ? threadIdx.x : 0;
+    if ((int)threadIdx.x >= LANES) return;
+    Complex<C>& X = Xs[l];
+    dev::InsertTemps<C>& Tm = Ts[l];
+    const uint64_t n = count_of(n_active);
+    for (uint64_t i = (uint64_t)blockIdx.x * LANES + l; i < n; i += (uint64_t)gridDim.x * LANES) {
+        const uint64_t a = list ? list[i] : i;
+        const int k = act_k[a];
+        if (k < kmin || k > kmax) continue;
+        cut_tet<C, MI>(g, vals, act, csr_off, fit, a, k, flags, rec_off, rec, cnt, errs, X, Tm);
+    }
+}
+
 // active tets with more than 8 functions, and tets a bucket kernel could not
 // hold, one lane each; ws holds one complex
 // per lane of the grid
