@@ -520,6 +520,50 @@ def test_batched_mi_matches_oracle_and_replays(ctx, which):
     g.release()
 
 
+@pytest.mark.parametrize("form", ["wave", "thread"])
+@pytest.mark.parametrize("which", ["config3_small", "config4", "config4_shuffled", "mesh"])
+def test_batched_mi_cut_forms(ctx, which, form, monkeypatch):
+    """Every material-interface tet (tables off) through the lane-group cut or
+    the thread cut gives the oracle's output bit for bit, and the records of
+    every active tet equal the oracle's."""
+    import rin
+
+    monkeypatch.setenv("RIN_CUT_FORM", form)
+    if which == "mesh":
+        m = np.load(S.REF_TESTS + "/mesh.npz")
+        pts, tets = m["pts"], m["tets"].astype(np.uint64)
+        types, params = S.load_ref_funcs("8-sphere.json")
+        g = ctx.grid_explicit(pts, tets)
+        hv = S.evaluate(types, params, pts)
+        vals = g.upload(hv)
+    else:
+        if which == "config3_small":
+            _, fs = W.config3()
+            res = 24
+        elif which == "config4":
+            res, fs = W.config4()
+        else:
+            res, fs = W.config4(shuffle_seed=44)
+        types, params = fs.arrays()
+        g = ctx.grid_regular(res)
+        vals = g.eval(types, params)
+        pts, tets = S.grid(res)
+        hv = S.evaluate(types, params, pts)
+    ref = S.OracleRun(1, pts, tets, hv)
+    for rep in range(2):
+        dev = vals.run(rin.FLAG_MI | rin.FLAG_BATCHED | rin.FLAG_KEEP_RECORDS)
+        label = dev.stage_times()[0][0]
+        assert label == "batched path (one graph)", label
+        _compare(dev, ref)
+        cap = len(ref.words) + 16
+        st, w = dev.records(ref.active, cap)
+        np.testing.assert_array_equal(st, ref.rec_start)
+        np.testing.assert_array_equal(w, ref.words)
+        dev.release()
+    vals.release()
+    g.release()
+
+
 def test_batched_mi_value_sets_share_the_plan(ctx):
     """Material sets of the same size on one grid share the batched workspace
     and graph (re-recorded and updated in place on each switch)."""
