@@ -20,7 +20,6 @@
 #include <mutex>
 #include <stdexcept>
 
-#include "cut_wave_mi.h"
 #include "kernels_ia.h"
 #include "kernels_mi.h"
 #include "pipeline.h"
@@ -76,53 +75,6 @@ __global__ void k_mi_check(uint32_t* ctr, const uint32_t* csr_off, uint64_t T, M
     // capacity the plain form reports them
     if (ctr[M_LIST + 9] > 8) errs[0] |= kMiHuge;
     if (ctr[M_LIST + 9] > (uint32_t)kMaxHuge) errs[0] |= kMiPlain;
-}
-
-// Lane-group material-interface cut: 64 / G tets per wavefront, a lane group
-// each, its complex in LDS (cut_wave_mi.h); the grid strides over the list.
-template <class C, int G>
-__global__ void __launch_bounds__(64) k_mi_cut_wave(GridView g, const double* __restrict__ vals,
-                                                    const uint32_t* __restrict__ act,
-                                                    const uint8_t* __restrict__ act_k,
-                                                    const uint32_t* __restrict__ csr_off,
-                                                    const uint8_t* __restrict__ fit,
-                                                    const uint32_t* __restrict__ list, Count nlist,
-                                                    const uint32_t* __restrict__ rec_off, uint16_t* __restrict__ rec,
-                                                    uint32_t* __restrict__ cnt, uint32_t* __restrict__ errs) {
-    constexpr int NG = 64 / G;
-    __shared__ dev::WaveMI<C> Xs[NG];
-    const int gid = threadIdx.x / G, lane = threadIdx.x & (G - 1);
-    dev::WaveMI<C>& X = Xs[gid];
-    const uint64_t n = count_of(nlist);
-    const uint64_t V = g.V;
-    for (uint64_t i = (uint64_t)blockIdx.x * NG + gid; i < n; i += (uint64_t)gridDim.x * NG) {
-        const uint32_t a = list[i];
-        const int k = act_k[a];
-        const uint32_t t = act[a], s0 = csr_off[t];
-        if (lane < 4 * k && 4 + k <= C::MP) {
-            uint32_t v[4];
-            dev::tet_corners(g, t, v);
-            const uint32_t f = fit[s0 + (lane >> 2)];
-            X.P[lane >> 2][lane & 3] = vals[(uint64_t)f * V + v[lane & 3]];
-        }
-        __syncthreads();
-        dev::wave_mi_cut<C, G>(X, k);
-        uint16_t* out = rec + rec_off[a];
-        uint32_t a0 = 0, a1 = 0, a2 = 0;
-        int len = -1;
-        if (!X.err) len = dev::wave_mi_record<C, G>(X, k, out, (int)(rec_off[a + 1] - rec_off[a]), a0, a1, a2);
-        if (lane == 0) {
-            if (X.err || len < 0) {
-                atomicOr(errs, (uint32_t)ERR_CAPACITY);
-                out[0] = 0;
-                a0 = a1 = a2 = 0;
-            }
-            cnt[3 * a] = a0;
-            cnt[3 * a + 1] = a1;
-            cnt[3 * a + 2] = a2;
-        }
-        __syncthreads();  // the group's complex is reused by its next tet
-    }
 }
 
 // the CSR, active list and counts (k_fill with the capacity check)
@@ -396,12 +348,14 @@ void mi_path(MIWorkspace& W, const Grid& g, const Values& val, const MITablesDev
         // two tets a wavefront up to four materials, one for five to eight
         const unsigned g1 = (unsigned)std::min<uint64_t>((c.active + 1) / 2 + 1, 2048);
         const unsigned g2 = (unsigned)std::min<uint64_t>(c.active + 1, 1024);
-        hipLaunchKernelGGL((k_mi_cut_wave<CMed, 32>), dim3(g1), dim3(64), 0, s, G, vals, (const uint32_t*)W.act.p,
-                           (const uint8_t*)W.act_k.p, (const uint32_t*)W.csr_off.p, (const uint8_t*)W.fit.p,
-                           (const uint32_t*)bl[1], n1, (const uint32_t*)W.rec_off.p, W.rec.p, W.cnt.p, W.errs.p);
-        hipLaunchKernelGGL((k_mi_cut_wave<CBig, 64>), dim3(g2), dim3(64), 0, s, G, vals, (const uint32_t*)W.act.p,
-                           (const uint8_t*)W.act_k.p, (const uint32_t*)W.csr_off.p, (const uint8_t*)W.fit.p,
-                           (const uint32_t*)bl[2], n2, (const uint32_t*)W.rec_off.p, W.rec.p, W.cnt.p, W.errs.p);
+        hipLaunchKernelGGL((k_cut_group<CMed, 32, true>), dim3(g1), dim3(64), 0, s, G, vals,
+                           (const uint32_t*)W.act.p, (const uint8_t*)W.act_k.p, (const uint32_t*)W.csr_off.p,
+                           (const uint8_t*)W.fit.p, (const uint32_t*)bl[1], n1, (const uint32_t*)W.rec_off.p, W.rec.p,
+                           W.cnt.p, W.errs.p, 0);
+        hipLaunchKernelGGL((k_cut_group<CBig, 64, true>), dim3(g2), dim3(64), 0, s, G, vals,
+                           (const uint32_t*)W.act.p, (const uint8_t*)W.act_k.p, (const uint32_t*)W.csr_off.p,
+                           (const uint8_t*)W.fit.p, (const uint32_t*)bl[2], n2, (const uint32_t*)W.rec_off.p, W.rec.p,
+                           W.cnt.p, W.errs.p, 0);
     }
     if (W.ws_huge.p)
         hipLaunchKernelGGL(k_cut_huge<true>, dim3(kHugeBlocks), dim3(64), 0, s, G, vals, (const uint32_t*)W.act.p,

@@ -1,7 +1,10 @@
 // Kernels of the device hot path shared by the plain and batched drivers.
 // Included once per translation unit (anonymous namespace).
 #pragma once
+#include <type_traits>
+
 #include "cut_dev.h"
+#include "cut_wave_mi.h"
 #include "pipeline.h"
 #include "scan_sort.h"
 
@@ -354,6 +357,71 @@ __global__ void __launch_bounds__(64) k_cut_lds(GridView g, const double* __rest
         const int k = act_k[a];
         if (k < kmin || k > kmax) continue;
         cut_tet<C, MI>(g, vals, act, csr_off, fit, a, k, flags, rec_off, rec, cnt, errs, X, Tm);
+    }
+}
+
+// Lane-group cuts over a list of active tets (64 / G tets a wavefront, the
+// complex in LDS; cut_wave.h, cut_wave_mi.h), records at rec_off[a] in slots
+// of rec_off[a + 1] - rec_off[a] words.  A tet whose complex outgrows the
+// capacities is marked for the large-capacity retry (retry != 0) or sets the
+// run's capacity error.  The grid strides over the list.
+template <class C, int G, bool MI>
+__global__ void __launch_bounds__(64) k_cut_group(GridView g, const double* __restrict__ vals,
+                                                  const uint32_t* __restrict__ act, const uint8_t* __restrict__ act_k,
+                                                  const uint32_t* __restrict__ csr_off,
+                                                  const uint8_t* __restrict__ fit, const uint32_t* __restrict__ list,
+                                                  Count nlist, const uint32_t* __restrict__ rec_off,
+                                                  uint16_t* __restrict__ rec, uint32_t* __restrict__ cnt,
+                                                  uint32_t* __restrict__ errs, int retry) {
+    constexpr int NG = 64 / G;
+    using Cx = typename std::conditional<MI, dev::WaveMI<C>, dev::WaveIA<C>>::type;
+    __shared__ Cx Xs[NG];
+    const int gid = threadIdx.x / G, lane = threadIdx.x & (G - 1);
+    Cx& X = Xs[gid];
+    const uint64_t n = count_of(nlist);
+    const uint64_t V = g.V;
+    for (uint64_t i = (uint64_t)blockIdx.x * NG + gid; i < n; i += (uint64_t)gridDim.x * NG) {
+        const uint32_t a = list[i];
+        const int k = act_k[a];
+        const uint32_t t = act[a], s0 = csr_off[t];
+        if (lane < 4 * k && 4 + k <= C::MP) {
+            uint32_t v[4];
+            dev::tet_corners(g, t, v);
+            const uint32_t f = fit[s0 + (lane >> 2)];
+            X.P[lane >> 2][lane & 3] = vals[(uint64_t)f * V + v[lane & 3]];
+        }
+        __syncthreads();
+        if constexpr (MI)
+            dev::wave_mi_cut<C, G>(X, k);
+        else
+            dev::wave_ia_cut<C, G>(X, k);
+        uint16_t* out = rec + rec_off[a];
+        const int cap = (int)(rec_off[a + 1] - rec_off[a]);
+        uint32_t a0 = 0, a1 = 0, a2 = 0;
+        int len = -1;
+        if (!X.err) {
+            if constexpr (MI)
+                len = dev::wave_mi_record<C, G>(X, k, out, cap, a0, a1, a2);
+            else
+                len = dev::wave_ia_record<C, G>(X, k, out, cap, a0, a1, a2);
+        }
+        if (lane == 0) {
+            if (X.err || len < 0) {
+                if (retry) {
+                    atomicOr(errs, (uint32_t)ERR_RETRY);
+                    a0 = CNT_RETRY;
+                } else {
+                    atomicOr(errs, (uint32_t)ERR_CAPACITY);
+                    a0 = 0;
+                }
+                out[0] = 0;
+                a1 = a2 = 0;
+            }
+            cnt[3 * a] = a0;
+            cnt[3 * a + 1] = a1;
+            cnt[3 * a + 2] = a2;
+        }
+        __syncthreads();  // the group's complex is reused by its next tet
     }
 }
 

@@ -637,9 +637,29 @@ void run_pipeline(DevMem& mem, const Grid& g, const Values& val, uint32_t flags,
                            (const uint32_t*)out.rec_off.p, out.rec.p, cnt.p, errs.p, TT);
         RIN_HIP(hipGetLastError());
     }
+    // the cuts beyond the tables: lane groups with the complex in LDS (the
+    // -R check's counts and reversed order take the thread cut)
+    const bool groups = !(flags & (RUN_REVERSE | RUN_CUT_ONLY));
+    for (int q = 0; q < 2 && NA && groups; ++q) {
+        const uint64_t nl = bcnt[1 + q];
+        if (!nl) continue;
+        auto go = [&](auto kernel, int per_wave) {
+            const unsigned blocks = (unsigned)std::min<uint64_t>((nl + per_wave - 1) / per_wave, 4096);
+            hipLaunchKernelGGL(kernel, dim3(blocks), dim3(64), 0, s, g.view, (const double*)val.vals.p,
+                               (const uint32_t*)out.act.p, (const uint8_t*)out.act_k.p,
+                               (const uint32_t*)out.csr_off.p, (const uint8_t*)out.fit.p,
+                               (const uint32_t*)blist[1 + q], Count(nl), (const uint32_t*)out.rec_off.p, out.rec.p,
+                               cnt.p, errs.p, 1);
+        };
+        if (q == 0)
+            MI ? go(k_cut_group<CMed, 32, true>, 2) : go(k_cut_group<CMed, 16, false>, 4);
+        else
+            MI ? go(k_cut_group<CBig, 64, true>, 1) : go(k_cut_group<CBig, 64, false>, 1);
+        RIN_HIP(hipGetLastError());
+    }
     // each thread-per-tet cut kernel runs over its own k range only
     DBuf<uint8_t> ws_cut[2];
-    for (int q = 0; q < 2 && NA; ++q) {
+    for (int q = 0; q < 2 && NA && !groups; ++q) {
         const uint64_t nl = bcnt[1 + q];
         if (!nl) continue;
         // lanes per workgroup (one wavefront): a lane runs a whole cut, whose
