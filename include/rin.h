@@ -160,6 +160,15 @@ int rin_result_faces(rin_ctx* ctx, const rin_result* res, uint64_t* vert_start, 
 int rin_result_edge_count(rin_ctx* ctx, const rin_result* res, uint64_t* n_edges);
 int rin_result_edges(rin_ctx* ctx, const rin_result* res, uint64_t* v12, uint64_t* start, uint64_t* occ,
                      uint64_t* edge_of);
+/* Patches of the result's faces, computed on the device on first request
+ * (compute_patches, src/mesh_connectivity.cpp:58-93): components of the faces
+ * joined across manifold edges (edges with exactly two face occurrences),
+ * numbered by their smallest face, each listed in breadth-first order from
+ * that face (neighbours in edge order).  rin_result_patches fills
+ * start[n_patches + 1] and faces[n_faces] (patch p is faces[start[p] ..
+ * start[p + 1])). */
+int rin_result_patch_count(rin_ctx* ctx, const rin_result* res, uint64_t* n_patches);
+int rin_result_patches(rin_ctx* ctx, const rin_result* res, uint64_t* start, uint64_t* faces);
 /* Per-tet cut records (layout in include/rin/cut_record.h) of the given tets,
  * packed: rec_start[n+1] word offsets into words[cap_words].  Needs
  * RIN_FLAG_KEEP_RECORDS.  Empty tets get an empty record. */

@@ -311,6 +311,17 @@ void device_edges(rin_ctx* c, const rin_result* r, const DeviceMesh& m, std::vec
     for (uint64_t f = 0; f < m.s.n_faces; ++f) eof[f].assign(eo.begin() + m.vstart[f], eo.begin() + m.vstart[f + 1]);
 }
 
+// patches computed on the device (rin_result_patches), in compute_patches'
+// form and order (src/mesh_connectivity.cpp:58-93)
+void device_patches(rin_ctx* c, const rin_result* r, const DeviceMesh& m, std::vector<std::vector<size_t>>& patches) {
+    uint64_t np = 0;
+    check(c, rin_result_patch_count(c, r, &np), "rin_result_patch_count");
+    std::vector<uint64_t> st(np + 1), fc(m.s.n_faces + 1);
+    check(c, rin_result_patches(c, r, st.data(), fc.data()), "rin_result_patches");
+    patches.resize(np);
+    for (uint64_t p = 0; p < np; ++p) patches[p].assign(fc.begin() + st[p], fc.begin() + st[p + 1]);
+}
+
 template <class T>
 void push_stat(std::vector<std::string>& sl, std::vector<size_t>& sv, const char* label, T v) {
     sl.emplace_back(label);
@@ -444,6 +455,7 @@ bool implicit_arrangement(bool robust_test, bool use_lookup, bool use_secondary_
         for (size_t f = 0; f < F; ++f) in.sample_sign.push_back(funcVals.data[f] > 0);
         in.cuts = &cuts;
         in.edges_from = [&](std::vector<std::vector<size_t>>& eof, std::vector<Edge>& e) { device_edges(c, r, m, eof, e); };
+        in.patches_from = [&](std::vector<std::vector<size_t>>& p) { device_patches(c, r, m, p); };
         rin::GraphOut g;
         const bool ok = rin::run_graph(in, g);
         for (auto& t : g.timings) {
@@ -572,6 +584,7 @@ bool material_interface(bool robust_test, bool use_lookup, bool use_secondary_lo
         for (size_t f = 0; f < F; ++f) in.sample_value.push_back(funcVals.data[f]);
         in.cuts = &cuts;
         in.edges_from = [&](std::vector<std::vector<size_t>>& eof, std::vector<Edge>& e) { device_edges(c, r, m, eof, e); };
+        in.patches_from = [&](std::vector<std::vector<size_t>>& p) { device_patches(c, r, m, p); };
         rin::GraphOut g;
         const bool ok = rin::run_graph(in, g);
         for (auto& t : g.timings) {

@@ -940,7 +940,12 @@ bool run_graph(const GraphIn& in, GraphOut& out) {
     out.stats.emplace_back(in.mi ? "num_MI_edges" : "num_iso_edges", out.edges.size());
 
     std::vector<std::pair<size_t, size_t>> plabel;
-    compute_patches(eof, out.edges, faces, out.patches, plabel);
+    if (in.patches_from) {
+        in.patches_from(out.patches);
+        for (const auto& p : out.patches) plabel.push_back(faces[p[0]].func_index);
+    } else {
+        compute_patches(eof, out.edges, faces, out.patches, plabel);
+    }
     if (in.mi)
         out.patch_label_mi = plabel;
     else

@@ -69,6 +69,8 @@ struct GraphIn {
     // when set, the iso-edges come from here (e.g. computed on the device)
     // instead of compute_mesh_edges over the faces
     std::function<void(std::vector<std::vector<size_t>>&, std::vector<Edge>&)> edges_from;
+    // when set, the patches come from here instead of compute_patches
+    std::function<void(std::vector<std::vector<size_t>>&)> patches_from;
 };
 
 struct GraphOut {

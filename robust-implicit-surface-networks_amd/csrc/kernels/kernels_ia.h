@@ -705,6 +705,142 @@ __global__ void k_edge_fill(Count n, const uint32_t* __restrict__ sidx, const ui
     }
 }
 
+// ------------------------------------------------------------------ patches
+// compute_patches (src/mesh_connectivity.cpp:58-93) on the device: a patch is
+// a component of the faces joined across manifold edges (edges with exactly
+// two occurrences), seeded at its smallest face and listed in breadth-first
+// order (queue order, neighbours in edge order).  Components come from
+// union-find (a root hooks under the smaller root, so each component's root
+// is its smallest face); the breadth-first order from level-synchronous
+// expansion where each newly reached face keeps the smallest (position of
+// its parent in the frontier, edge index) that reaches it, exactly the queue
+// order of one search per patch.
+
+// the face across face corner position o's edge, or ~0u when the edge is
+// not manifold
+__global__ void k_face_adj(Count no, const uint32_t* __restrict__ e_of_occ, const uint32_t* __restrict__ e_start,
+                           const uint32_t* __restrict__ e_occ, const uint32_t* __restrict__ face_of,
+                           uint32_t* __restrict__ adj) {
+    GRID_STRIDE(o, no) {
+        const uint32_t e = e_of_occ[o];
+        const uint32_t b = e_start[e];
+        uint32_t other = ~0u;
+        if (e_start[e + 1] - b == 2) {
+            const uint32_t f0 = face_of[e_occ[b]], f1 = face_of[e_occ[b + 1]];
+            other = f0 == face_of[o] ? f1 : f0;
+        }
+        adj[o] = other;
+    }
+}
+
+__device__ inline uint32_t uf_find(uint32_t* parent, uint32_t x) {
+    uint32_t p = parent[x];
+    while (p != x) {
+        const uint32_t g = parent[p];
+        if (g != p) atomicMin(&parent[x], g);  // path halving (parents only decrease)
+        x = p;
+        p = g;
+    }
+    return x;
+}
+
+__global__ void k_uf_hook(Count no, const uint32_t* __restrict__ face_of, const uint32_t* __restrict__ adj,
+                          uint32_t* parent) {
+    GRID_STRIDE(o, no) {
+        const uint32_t v = adj[o];
+        if (v == ~0u) continue;
+        uint32_t a = face_of[o], b = v;
+        for (;;) {
+            a = uf_find(parent, a);
+            b = uf_find(parent, b);
+            if (a == b) break;
+            const uint32_t hi = a > b ? a : b, lo = a > b ? b : a;
+            if (atomicCAS(&parent[hi], hi, lo) == hi) break;
+        }
+    }
+}
+
+// roots (after hooking): each face's component root, and seed flags
+__global__ void k_uf_roots(Count nf, uint32_t* parent, uint32_t* __restrict__ seed) {
+    GRID_STRIDE(f, nf) {
+        const uint32_t r = uf_find(parent, (uint32_t)f);
+        parent[f] = r;
+        seed[f] = r == f ? 1u : 0u;
+    }
+}
+
+// the first frontier (the seeds, in face order) and every face's patch id
+__global__ void k_patch_seeds(Count nf, const uint32_t* __restrict__ root, const uint32_t* __restrict__ seed,
+                              const uint32_t* __restrict__ seed_rank, uint32_t* __restrict__ pid,
+                              uint32_t* __restrict__ level, uint32_t* __restrict__ frontier,
+                              unsigned long long* __restrict__ claim) {
+    GRID_STRIDE(f, nf) {
+        pid[f] = seed_rank[root[f]];
+        level[f] = seed[f] ? 0u : ~0u;
+        claim[f] = ~0ull;
+        if (seed[f]) frontier[seed_rank[f]] = (uint32_t)f;
+    }
+}
+
+// one level: every frontier face (position i) offers its manifold
+// neighbours not reached yet the key (i, corner j); the first offer to a face
+// appends it to the next frontier
+__global__ void k_patch_claim(const uint32_t* __restrict__ nfront, const uint32_t* __restrict__ frontier,
+                              const uint32_t* __restrict__ vstart, const uint32_t* __restrict__ adj,
+                              const uint32_t* __restrict__ level, unsigned long long* __restrict__ claim,
+                              uint32_t* __restrict__ next, uint32_t* __restrict__ nnext) {
+    const uint32_t n = *nfront;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t p = frontier[i];
+        const uint32_t b = vstart[p], e = vstart[p + 1];
+        for (uint32_t o = b; o < e; ++o) {
+            const uint32_t c = adj[o];
+            if (c == ~0u || level[c] != ~0u) continue;
+            const unsigned long long key = (unsigned long long)i << 16 | (o - b);
+            if (atomicMin(&claim[c], key) == ~0ull) next[atomicAdd(nnext, 1u)] = c;
+        }
+    }
+}
+
+// the next frontier: (claim key, face) pairs to sort
+__global__ void k_patch_keys(const uint32_t* __restrict__ nnext, const uint32_t* __restrict__ next,
+                             const unsigned long long* __restrict__ claim, uint64_t* __restrict__ key,
+                             uint32_t* __restrict__ val) {
+    const uint32_t n = *nnext;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t c = next[i];
+        key[i] = claim[c];
+        val[i] = c;
+    }
+}
+
+// after the sort: the level of each reached face, and the frontier appended
+// to the breadth-first list
+__global__ void k_patch_level(const uint32_t* __restrict__ nnext, const uint32_t* __restrict__ sorted, uint32_t lv,
+                              uint32_t* __restrict__ level, uint32_t* __restrict__ order, uint32_t base) {
+    const uint32_t n = *nnext;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t c = sorted[i];
+        level[c] = lv;
+        order[base + i] = c;
+    }
+}
+
+// patch sizes from the patch ids, and the faces by patch (the breadth-first
+// list is level-major and patch-major within a level, so a stable placement
+// by patch keeps each patch's queue order)
+__global__ void k_patch_count(Count nf, const uint32_t* __restrict__ pid, uint32_t* __restrict__ cnt) {
+    GRID_STRIDE(f, nf) atomicAdd(&cnt[pid[f]], 1u);
+}
+__global__ void k_patch_keys_pid(Count nf, const uint32_t* __restrict__ order, const uint32_t* __restrict__ pid,
+                                 uint64_t* __restrict__ key, uint32_t* __restrict__ val) {
+    GRID_STRIDE(i, nf) {
+        const uint32_t f = order[i];
+        key[i] = pid[f];
+        val[i] = f;
+    }
+}
+
 // First-occurrence dedup.  Vertex ids are the ranks of first occurrences in
 // occurrence order, as a stable sort by key gives; every occurrence gets the
 // index `mo` of its key's first occurrence.

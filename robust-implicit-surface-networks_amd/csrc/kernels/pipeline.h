@@ -118,6 +118,11 @@ struct RunOut {
     DBuf<uint32_t> e_occ;       // face-vertex positions (face f, position j <-> f_vstart[f] + j)
     DBuf<uint32_t> e_of_occ;    // edge of each face-vertex position
     DBuf<uint32_t> e_face_of;   // face of each face-vertex position
+    // patches (compute_patches), on first request
+    bool have_patches = false;
+    uint64_t n_patches = 0;
+    DBuf<uint32_t> p_start;  // n_patches + 1
+    DBuf<uint32_t> p_faces;  // each patch's faces in breadth-first order
 };
 
 void grid_regular(DevMem& mem, int res, const double bmin[3], const double bmax[3], Grid& g, hipStream_t s);
@@ -175,6 +180,8 @@ void gather_records(DevMem& mem, const RunOut& r, const uint64_t* tets, uint64_t
 // iso-edges of the result's faces, on the device (compute_mesh_edges,
 // src/mesh_connectivity.cpp:10-56); fills the e_* buffers of the result
 void compute_edges(DevMem& mem, RunOut& r, hipStream_t s);
+// the result's patches on the device (src/mesh_connectivity.cpp:58-93), computing the edges first
+void compute_patches(DevMem& mem, RunOut& r, hipStream_t s);
 
 // diagnostics: a plain device copy of `bytes` (reference kernel for
 // hardware-counter scaling); returns the average milliseconds per copy

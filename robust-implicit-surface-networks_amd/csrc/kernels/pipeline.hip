@@ -1143,6 +1143,75 @@ void debug_scan(DevMem& mem, int fields, int stride, const uint32_t* in, uint64_
     RIN_HIP(hipStreamSynchronize(s));
 }
 
+void compute_patches(DevMem& mem, RunOut& r, hipStream_t s) {
+    if (r.have_patches) return;
+    compute_edges(mem, r, s);
+    const uint64_t NF = r.n_faces, NO = r.n_face_verts;
+    r.p_start.alloc(mem, NF + 2);
+    r.p_faces.alloc(mem, NF + 1);
+    if (NF == 0) {
+        r.n_patches = 0;
+        fill_u32(r.p_start.p, 0, 1, s);
+        RIN_HIP(hipStreamSynchronize(s));
+        r.have_patches = true;
+        return;
+    }
+    DBuf<uint32_t> adj(mem, NO + 1), parent(mem, NF), seed(mem, NF + 1), srank(mem, NF + 1), pid(mem, NF),
+        level(mem, NF), fa(mem, NF), fb(mem, NF), order(mem, NF), cnt(mem, 2);
+    DBuf<unsigned long long> claim(mem, NF);
+    DBuf<uint64_t> key(mem, NF);
+    hipLaunchKernelGGL(k_face_adj, dim3(nblk(NO)), dim3(256), 0, s, Count(NO), (const uint32_t*)r.e_of_occ.p,
+                       (const uint32_t*)r.e_start.p, (const uint32_t*)r.e_occ.p, (const uint32_t*)r.e_face_of.p,
+                       adj.p);
+    iota_u32(parent.p, Count(NF), s);
+    hipLaunchKernelGGL(k_uf_hook, dim3(nblk(NO)), dim3(256), 0, s, Count(NO), (const uint32_t*)r.e_face_of.p,
+                       (const uint32_t*)adj.p, parent.p);
+    hipLaunchKernelGGL(k_uf_roots, dim3(nblk(NF)), dim3(256), 0, s, Count(NF), parent.p, seed.p);
+    scan_u32(mem, seed.p, srank.p, Count(NF), srank.p + NF, s);
+    hipLaunchKernelGGL(k_patch_seeds, dim3(nblk(NF)), dim3(256), 0, s, Count(NF), (const uint32_t*)parent.p,
+                       (const uint32_t*)seed.p, (const uint32_t*)srank.p, pid.p, level.p, fa.p, claim.p);
+    const uint64_t NP = read1(srank.p + NF, s);
+    r.n_patches = NP;
+    copy_u32(order.p, fa.p, Count(NP), s);
+    // breadth-first levels over every patch at once
+    uint32_t* front = fa.p;
+    uint32_t* next = fb.p;
+    uint64_t nfront = NP, base = NP;
+    uint32_t lv = 1;
+    DBuf<uint32_t> sorted(mem, NF);
+    while (nfront) {
+        fill_u32(cnt.p, (uint32_t)nfront, 1, s);
+        fill_u32(cnt.p + 1, 0, 1, s);
+        hipLaunchKernelGGL(k_patch_claim, dim3(nblk(nfront)), dim3(256), 0, s, (const uint32_t*)cnt.p,
+                           (const uint32_t*)front, (const uint32_t*)r.view.f_vstart, (const uint32_t*)adj.p,
+                           (const uint32_t*)level.p, claim.p, next, cnt.p + 1);
+        const uint64_t nn = read1(cnt.p + 1, s);
+        if (nn == 0) break;
+        if (base + nn > NF) throw std::runtime_error("patch search listed a face twice");
+        hipLaunchKernelGGL(k_patch_keys, dim3(nblk(nn)), dim3(256), 0, s, (const uint32_t*)(cnt.p + 1),
+                           (const uint32_t*)next, (const unsigned long long*)claim.p, key.p, sorted.p);
+        sort_pairs(mem, key.p, sorted.p, Count(nn), 16 + bits_for(nfront), s);
+        hipLaunchKernelGGL(k_patch_level, dim3(nblk(nn)), dim3(256), 0, s, (const uint32_t*)(cnt.p + 1),
+                           (const uint32_t*)sorted.p, lv, level.p, order.p, (uint32_t)base);
+        copy_u32(front, sorted.p, Count(nn), s);
+        base += nn;
+        nfront = nn;
+        ++lv;  // front holds the new frontier in order; next takes the next offers
+    }
+    if (base != NF) throw std::runtime_error("patch search did not reach every face");
+    // faces by patch, each patch in its breadth-first order (stable placement)
+    DBuf<uint32_t> pc(mem, NP + 1);
+    fill_u32(pc.p, 0, NP + 1, s);
+    hipLaunchKernelGGL(k_patch_count, dim3(nblk(NF)), dim3(256), 0, s, Count(NF), (const uint32_t*)pid.p, pc.p);
+    scan_u32(mem, pc.p, r.p_start.p, Count(NP), r.p_start.p + NP, s);
+    hipLaunchKernelGGL(k_patch_keys_pid, dim3(nblk(NF)), dim3(256), 0, s, Count(NF), (const uint32_t*)order.p,
+                       (const uint32_t*)pid.p, key.p, r.p_faces.p);
+    sort_pairs(mem, key.p, r.p_faces.p, Count(NF), bits_for(NP), s);
+    RIN_HIP(hipGetLastError());
+    RIN_HIP(hipStreamSynchronize(s));
+    r.have_patches = true;
+}
+
 float debug_copy(DevMem& mem, uint64_t bytes, int reps, hipStream_t s) {
     const uint64_t n = bytes / 8;
     if (n == 0 || reps < 1) throw std::invalid_argument("copy size and repetitions must be positive");

@@ -360,6 +360,27 @@ int rin_result_edges(rin_ctx* c, const rin_result* res, uint64_t* v12, uint64_t*
     });
 }
 
+int rin_result_patch_count(rin_ctx* c, const rin_result* res, uint64_t* n_patches) {
+    if (!res || !n_patches) return fail(c, RIN_ERR_ARG, "null argument");
+    return guarded(c, [&] {
+        auto& r = const_cast<rin_result*>(res)->r;  // the patches are cached in the result
+        rin::compute_patches(c->mem, r, c->stream);
+        *n_patches = r.n_patches;
+    });
+}
+
+int rin_result_patches(rin_ctx* c, const rin_result* res, uint64_t* start, uint64_t* faces) {
+    if (!res || !start || (!faces && res->r.n_faces)) return fail(c, RIN_ERR_ARG, "null argument");
+    return guarded(c, [&] {
+        auto& r = const_cast<rin_result*>(res)->r;
+        rin::compute_patches(c->mem, r, c->stream);
+        auto a = fetch(r.p_start.p, r.n_patches + 1, c->stream);
+        auto b = fetch(r.p_faces.p, r.n_faces, c->stream);
+        for (uint64_t i = 0; i <= r.n_patches; ++i) start[i] = a[i];
+        for (uint64_t i = 0; i < r.n_faces; ++i) faces[i] = b[i];
+    });
+}
+
 int rin_result_records(rin_ctx* c, const rin_result* res, const uint64_t* tets, uint64_t n, uint64_t* rec_start,
                        uint16_t* words, uint64_t cap) {
     if (!res || (!tets && n) || !rec_start) return fail(c, RIN_ERR_ARG, "null argument");

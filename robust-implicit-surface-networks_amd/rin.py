@@ -80,6 +80,8 @@ def lib():
         L.rin_result_cut_counts.argtypes = [_vp, _vp, _u64p]
         L.rin_result_edge_count.argtypes = [_vp, _vp, _u64p]
         L.rin_result_edges.argtypes = [_vp, _vp, _u64p, _u64p, _u64p, _u64p]
+        L.rin_result_patch_count.argtypes = [_vp, _vp, _u64p]
+        L.rin_result_patches.argtypes = [_vp, _vp, _u64p, _u64p]
         L.rin_result_stage_times.argtypes = [_vp, _vp, _f64p, ctypes.c_int32]
         L.rin_result_stage_label.argtypes = [_vp, _vp, ctypes.c_int32]
         L.rin_result_stage_label.restype = ctypes.c_char_p
@@ -300,6 +302,15 @@ class Result:
         self.ctx.check(lib().rin_result_edges(self.ctx.h, self.h, _p(v12, _u64p), _p(st, _u64p), _p(occ, _u64p),
                                               _p(eo, _u64p)))
         return v12, st, occ, eo
+
+    def patches(self):
+        """Patches on the device: (start (NP + 1), faces (NF))."""
+        n = ctypes.c_uint64(0)
+        self.ctx.check(lib().rin_result_patch_count(self.ctx.h, self.h, ctypes.byref(n)))
+        st = np.zeros(n.value + 1, dtype=np.uint64)
+        fc = np.zeros(max(1, self.sizes.n_faces), dtype=np.uint64)
+        self.ctx.check(lib().rin_result_patches(self.ctx.h, self.h, _p(st, _u64p), _p(fc, _u64p)))
+        return st, fc[:self.sizes.n_faces]
 
     def cut_counts(self):
         n = self.sizes.num_intersecting

@@ -1,7 +1,11 @@
-"""Iso-edges on the device (rin_result_edges) against compute_mesh_edges
-(src/mesh_connectivity.cpp:10-56) restated over the oracle's faces: edge ids
-in order of first occurrence over (face, j), each edge's endpoints (smaller
-first), its (face, j) list in that order, and each face's edge list."""
+"""Iso-edges and patches on the device (rin_result_edges, rin_result_patches)
+against compute_mesh_edges and compute_patches (src/mesh_connectivity.cpp:
+10-93) restated over the oracle's faces: edge ids in order of first
+occurrence over (face, j), each edge's endpoints (smaller first), its (face,
+j) list in that order, and each face's edge list; patches seeded at their
+smallest unvisited face, faces in queue order across manifold edges."""
+from collections import deque
+
 import numpy as np
 import pytest
 
@@ -83,7 +87,7 @@ def test_edges_empty(ctx):
 
 
 def test_edges_headline_properties(ctx):
-    """At the 128-grid, 16-function size: every occurrence's face edge is its
+    """At the 101^3-vertex, 20-function size: every occurrence's face edge is its
     edge's vertex pair, each edge's occurrences are listed in (face, j) order,
     every face-edge position is listed once, and edge ids follow first
     occurrence (the first occurrences increase with the id)."""
@@ -120,3 +124,106 @@ def test_edges_headline_properties(ctx):
     dev.release()
     vals.release()
     g.release()
+
+
+def _patches_of(face_start, face_verts):
+    """The reference's loop: a queue from each unvisited face, neighbours
+    across edges with exactly two occurrences, in the face's edge order."""
+    v12, occ, eof = _edges_of_faces(face_start, face_verts)
+    nf = len(face_start) - 1
+    seen = [False] * nf
+    patches = []
+    k = 0
+    face_edges = []
+    for f in range(nf):
+        n = int(face_start[f + 1] - face_start[f])
+        face_edges.append(eof[k:k + n])
+        k += n
+    for i in range(nf):
+        if seen[i]:
+            continue
+        patch = [i]
+        seen[i] = True
+        q = deque([i])
+        while q:
+            f = q.popleft()
+            for e in face_edges[f]:
+                if len(occ[e]) != 2:
+                    continue
+                g = occ[e][1][0] if occ[e][0][0] == f else occ[e][0][0]
+                if not seen[g]:
+                    seen[g] = True
+                    q.append(g)
+                    patch.append(g)
+        patches.append(patch)
+    return patches
+
+
+@pytest.mark.parametrize("mi,which", [(0, "config1"), (0, "config4"), (1, "config4"), (1, "config3_small"),
+                                      (0, "config2_res48")])
+def test_patches_match_reference_loop(ctx, mi, which):
+    import rin
+
+    if which == "config1":
+        res, fs = W.config1()
+    elif which == "config4":
+        res, fs = W.config4(shuffle_seed=44)
+    elif which == "config2_res48":
+        _, fs = W.config2()
+        res = 48
+    else:
+        _, fs = W.config3()
+        res = 24
+    types, params = fs.arrays()
+    g = ctx.grid_regular(res)
+    vals = g.eval(types, params)
+    for flags in (0, rin.FLAG_BATCHED):
+        dev = vals.run((rin.FLAG_MI if mi else 0) | flags)
+        pts, tets = S.grid(res)
+        ref = S.OracleRun(mi, pts, tets, S.evaluate(types, params, pts))
+        want = _patches_of(ref.face_start, ref.face_verts)
+        st, fc = dev.patches()
+        assert len(st) - 1 == len(want)
+        got = [list(fc[int(st[p]):int(st[p + 1])]) for p in range(len(want))]
+        assert got == want
+        # cached: a second request returns the same lists
+        st2, fc2 = dev.patches()
+        np.testing.assert_array_equal(st2, st)
+        np.testing.assert_array_equal(fc2, fc)
+        dev.release()
+    vals.release()
+    g.release()
+
+
+def test_patches_empty(ctx):
+    g = ctx.grid_regular(8)
+    fs = W.mixed(np.random.default_rng(3), 0, 0, 0, 0)
+    fs.plane([0.0, 0.0, 5.0], [0.0, 0.0, 1.0])
+    types, params = fs.arrays()
+    vals = g.eval(types, params)
+    dev = vals.run(0)
+    st, fc = dev.patches()
+    assert list(st) == [0] and len(fc) == 0
+
+
+def test_patches_headline(ctx):
+    """At the 101^3-vertex, 20-function size, the device patches equal the
+    reference loop run over the device's own faces."""
+    import rin
+
+    res, fs = W.config2()
+    types, params = fs.arrays()
+    g = ctx.grid_regular(res)
+    vals = g.eval(types, params)
+    dev = vals.run(rin.FLAG_BATCHED)
+    vs, v, _, _, _ = dev.faces()
+    want = _patches_of(vs, v)
+    st, fc = dev.patches()
+    assert len(st) - 1 == len(want)
+    flat = np.array([f for p in want for f in p], dtype=np.uint64)
+    np.testing.assert_array_equal(fc, flat)
+    np.testing.assert_array_equal(np.diff(st.astype(np.int64)), [len(p) for p in want])
+    dev.release()
+    vals.release()
+    g.release()
+
