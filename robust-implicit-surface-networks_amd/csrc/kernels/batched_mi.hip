@@ -50,6 +50,7 @@ static_assert(M_N <= kMiPub, "published counters");
 
 struct MICaps {
     uint32_t active = 0, fit = 0, words = 0, nft = 0, vo = 0, fo = 0, fv = 0;
+    uint32_t huge = 0;  // the captured path holds the large-capacity cut
 };
 
 __global__ void k_mi_zero(uint32_t* ctr, uint32_t* errs) {
@@ -69,12 +70,16 @@ __global__ void k_mi_check(uint32_t* ctr, const uint32_t* csr_off, uint64_t T, M
         ctr[M_ABORT] = 1;
         errs[0] |= kMiOverflow;
     }
+    // tets beyond 8 materials take the large-capacity cut (a path captured
+    // without it stops here and is captured again with it); beyond its
+    // capacity the plain form reports them
+    if (ctr[M_LIST + 9] > 8) {
+        errs[0] |= kMiHuge;
+        if (!c.huge) ctr[M_ABORT] = 1;
+    }
+    if (ctr[M_LIST + 9] > (uint32_t)kMaxHuge) errs[0] |= kMiPlain;
     if (ctr[M_ABORT])
         for (int q = 0; q < 4; ++q) ctr[M_LIST + q] = 0;
-    // tets beyond 8 materials take the large-capacity cut; beyond its
-    // capacity the plain form reports them
-    if (ctr[M_LIST + 9] > 8) errs[0] |= kMiHuge;
-    if (ctr[M_LIST + 9] > (uint32_t)kMaxHuge) errs[0] |= kMiPlain;
 }
 
 // the CSR, active list and counts (k_fill with the capacity check)
@@ -172,7 +177,13 @@ struct MIWorkspace {
     DBuf<uint32_t> cv, cf, cfv, oa, fq_a, fq_vs, fve;
     DBuf<uint8_t> fq_m;
     // dedup
-    DBuf<uint32_t> sidx, first, head, hexcl, seg_first, vrank, vid, first_occ;
+    DBuf<uint32_t> first, mo, hslot, vrank, vid, first_occ;
+    // first-occurrence hash table (key + 1, ~smallest occurrence), zero
+    // between runs: the assign pass clears the slots a run used
+    DBuf<unsigned long long> hkey;
+    DBuf<uint32_t> hmx;
+    int hbits = 0;
+    bool hash_dirty = false;  // a run stopped with entries left: clear before the next
     // outputs
     DBuf<uint32_t> v_tet, v_sv, f_vstart, f_verts, f_tfstart, f_tf_tet;
     DBuf<uint16_t> v_j, f_tf_local;
@@ -265,8 +276,15 @@ void alloc_mi(MIWorkspace& W, const Grid& g, const MICaps& c, hipStream_t s) {
     W.fq_m.alloc(m, 2 * (uint64_t)c.fo + 2);
     W.fq_vs.alloc(m, c.fo + 1);
     W.fve.alloc(m, c.fv + 1);
-    for (auto* b : {&W.sidx, &W.first, &W.head, &W.hexcl, &W.seg_first, &W.vrank, &W.vid, &W.first_occ})
-        b->alloc(m, c.vo + 1);
+    for (auto* b : {&W.first, &W.mo, &W.hslot, &W.vrank, &W.vid, &W.first_occ}) b->alloc(m, c.vo + 1);
+    // a table of at least twice the occurrence capacity, cleared once here
+    W.hbits = 10;
+    while ((1ull << W.hbits) < 2ull * (c.vo + 1)) ++W.hbits;
+    W.hkey.alloc(m, 1ull << W.hbits);
+    W.hmx.alloc(m, 1ull << W.hbits);
+    RIN_HIP(hipMemsetAsync(W.hkey.p, 0, 8ull << W.hbits, s));
+    RIN_HIP(hipMemsetAsync(W.hmx.p, 0, 4ull << W.hbits, s));
+    W.hash_dirty = false;
     W.v_tet.alloc(m, c.vo + 1);
     W.v_j.alloc(m, c.vo + 1);
     W.v_ss.alloc(m, c.vo + 1);
@@ -292,7 +310,8 @@ void mi_path(MIWorkspace& W, const Grid& g, const Values& val, const MITablesDev
     const GridView& G = g.view;
     const uint64_t V = G.V, T = G.T;
     const uint32_t F = val.F;
-    const MICaps& c = W.cap;
+    MICaps c = W.cap;
+    c.huge = W.ws_huge.p ? 1u : 0u;
     uint32_t* ctr = W.ctr.p;
     const uint32_t* abort = ctr + M_ABORT;
     const double* vals = val.vals.p;
@@ -400,21 +419,18 @@ void mi_path(MIWorkspace& W, const Grid& g, const Values& val, const MITablesDev
     hipLaunchKernelGGL(k_mi_extract, dim3(blocks_for(c.active)), dim3(256), 0, s, A, (const uint64_t*)W.emask.p,
                        (const uint32_t*)W.cv.p, (const uint32_t*)W.cf.p, (const uint32_t*)W.cfv.p, W.okey.p, W.oa.p,
                        W.oj.p, W.fq_a.p, W.fq_l.p, W.fq_m.p, W.fq_vs.p, W.fve.p);
-    // vertex dedup: stable sort of the occurrence keys, ids by first occurrence
+    // vertex dedup: each key's smallest occurrence by an atomic hash table,
+    // ids by a scan of the first-occurrence flags (the table is left empty)
     const Count NVO(ctr + M_VO, c.vo, abort);
-    iota_u32(W.sidx.p, NVO, s);
-    sort_pairs(W.mem, W.okey.p, W.sidx.p, NVO, kbits, s);
-    hipLaunchKernelGGL(k_heads, dim3(blocks_for(c.vo)), dim3(256), 0, s, NVO, (const uint64_t*)W.okey.p,
-                       (const uint32_t*)W.sidx.p, W.first.p, W.head.p);
-    scan_u32(W.mem, W.head.p, W.hexcl.p, NVO, nullptr, s);
-    hipLaunchKernelGGL(k_assign, dim3(blocks_for(c.vo)), dim3(256), 0, s, NVO, (const uint32_t*)W.sidx.p,
-                       (const uint32_t*)W.head.p, (const uint32_t*)W.hexcl.p, W.seg_first.p);
+    (void)kbits;
+    hipLaunchKernelGGL(k_hash_insert, dim3(blocks_for(c.vo)), dim3(256), 0, s, NVO, (const uint32_t*)nullptr,
+                       (const uint64_t*)W.okey.p, W.hkey.p, W.hmx.p, W.hbits, W.hslot.p, W.errs.p);
+    hipLaunchKernelGGL(k_hash_first, dim3(blocks_for(c.vo)), dim3(256), 0, s, NVO, (const uint32_t*)nullptr,
+                       (const uint32_t*)W.hslot.p, (const uint32_t*)W.hmx.p, W.first.p, W.mo.p);
     scan_u32(W.mem, W.first.p, W.vrank.p, NVO, ctr + M_NV, s);
-    hipLaunchKernelGGL(k_assign2, dim3(blocks_for(c.vo)), dim3(256), 0, s, NVO, (const uint32_t*)W.sidx.p,
-                       (const uint32_t*)W.head.p, (const uint32_t*)W.hexcl.p, (const uint32_t*)W.seg_first.p,
-                       (const uint32_t*)W.vrank.p, W.vid.p);
-    hipLaunchKernelGGL(k_first_list, dim3(blocks_for(c.vo)), dim3(256), 0, s, NVO, (const uint32_t*)W.first.p,
-                       (const uint32_t*)W.vrank.p, W.first_occ.p);
+    hipLaunchKernelGGL(k_dedup_assign, dim3(blocks_for(c.vo)), dim3(256), 0, s, NVO, (const uint32_t*)W.mo.p,
+                       (const uint32_t*)W.first.p, (const uint32_t*)W.vrank.p, W.vid.p, W.first_occ.p, NVO,
+                       (const uint32_t*)nullptr, (const uint32_t*)W.hslot.p, W.hkey.p, W.hmx.p, true);
     // coordinates and faces
     const Count NV(ctr + M_NV, c.vo, abort), NF(ctr + M_FO, c.fo, abort), NFV(ctr + M_FV, c.fv, abort);
     hipLaunchKernelGGL(k_vert_fields_mi, dim3(blocks_for(c.vo)), dim3(256), 0, s, G, vals, (const uint32_t*)W.act.p,
@@ -502,7 +518,7 @@ void run_batched_mi(DevMem& ctx_mem, const Grid& g, const Values& val, uint32_t 
             // (re-)record the path for these values; update the graph in place
             const int kbits = 2 + std::max({bits_of(g.view.V), bits_of(g.view.n_edge_ids) + 12,
                                             bits_of(g.view.n_face_ids) + 18, bits_of(w.cap.active) + 10});
-            if (kbits > 64) throw NeedsPlainForm("simplex keys exceed 64 bits for this mesh");
+            if (kbits > 63) throw NeedsPlainForm("simplex keys exceed 63 bits for this mesh");
             hipGraph_t gr = nullptr;
             RIN_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
             try {
@@ -532,6 +548,11 @@ void run_batched_mi(DevMem& ctx_mem, const Grid& g, const Values& val, uint32_t 
         hipEvent_t e0, e1;
         RIN_HIP(hipEventCreate(&e0));
         RIN_HIP(hipEventCreate(&e1));
+        if (w.hash_dirty) {
+            RIN_HIP(hipMemsetAsync(w.hkey.p, 0, 8ull << w.hbits, s));
+            RIN_HIP(hipMemsetAsync(w.hmx.p, 0, 4ull << w.hbits, s));
+            w.hash_dirty = false;
+        }
         RIN_HIP(hipEventRecord(e0, s));
         RIN_HIP(hipGraphLaunch(w.exec, s));
         RIN_HIP(hipEventRecord(e1, s));
@@ -542,6 +563,8 @@ void run_batched_mi(DevMem& ctx_mem, const Grid& g, const Values& val, uint32_t 
         (void)hipEventDestroy(e1);
         std::memcpy(pub, w.pub_h, sizeof(pub));
         uint32_t e = pub[kMiPub];
+        // a stopped or failed run may leave table entries
+        if ((e & ~kMiHuge) || ((e & kMiHuge) && !w.ws_huge.p)) w.hash_dirty = true;
         if (e & kMiPlain) throw NeedsPlainForm("a tet has more materials than the device cut holds");
         if ((e & kMiHuge) && !w.ws_huge.p) {
             // first run with such tets: room for the large-capacity cut, and
