@@ -163,7 +163,7 @@ struct MIWorkspace {
     bool in_use = false;
     // per vertex / tet (fixed by the grid)
     DBuf<uint64_t> hm, amask;
-    DBuf<uint32_t> vflag, kcnt, isact, csr_off, rank;
+    DBuf<uint32_t> kcnt, isact, csr_off, rank, tscan;
     // per active tet
     DBuf<uint8_t> fit, act_k;
     DBuf<uint32_t> act, blists, bc, rbound, rec_off, cnt, fb;
@@ -174,7 +174,7 @@ struct MIWorkspace {
     // per complex face, occurrence, face
     DBuf<uint16_t> fl, fw, oj, fq_l;
     DBuf<uint64_t> ft, okey;
-    DBuf<uint32_t> cv, cf, cfv, oa, fq_a, fq_vs, fve;
+    DBuf<uint32_t> cv, cf, cfv, tv, tf, tfv, oa, fq_a, fq_vs, fve;
     DBuf<uint8_t> fq_m;
     // dedup
     DBuf<uint32_t> first, mo, hslot, vrank, vid, first_occ;
@@ -235,10 +235,10 @@ void alloc_mi(MIWorkspace& W, const Grid& g, const MICaps& c, hipStream_t s) {
     W.cap = c;
     if (!W.hm.p) {
         W.hm.alloc(m, V);
-        W.vflag.alloc(m, V);
         W.amask.alloc(m, T);
         W.kcnt.alloc(m, T);
         W.isact.alloc(m, T);
+        W.tscan.alloc(m, T + 1);
         W.csr_off.alloc(m, T + 1);
         W.rank.alloc(m, T + 1);
         W.ctr.alloc(m, 64);
@@ -260,6 +260,9 @@ void alloc_mi(MIWorkspace& W, const Grid& g, const MICaps& c, hipStream_t s) {
     W.cv.alloc(m, A + 2);
     W.cf.alloc(m, A + 2);
     W.cfv.alloc(m, A + 2);
+    W.tv.alloc(m, A + 2);
+    W.tf.alloc(m, A + 2);
+    W.tfv.alloc(m, A + 2);
     // thread cuts (four and five to eight materials): a lane per tet with
     // its complex in LDS, two lanes (one for the larger complexes) a
     // wavefront, grids striding over their lists
@@ -317,12 +320,17 @@ void mi_path(MIWorkspace& W, const Grid& g, const Values& val, const MITablesDev
     const double* vals = val.vals.p;
     hipLaunchKernelGGL(k_mi_zero, dim3(1), dim3(64), 0, s, ctr, W.errs.p);
     // highest material and filter (src/material_interface.cpp:53-152)
-    hipLaunchKernelGGL(k_highest_mi, dim3(blocks_for(V)), dim3(256), 0, s, vals, V, F, W.hm.p, W.vflag.p);
-    scan_u32(W.mem, W.vflag.p, W.vflag.p, Count(V), ctr + M_DEGEN, s);
+    hipLaunchKernelGGL(k_highest_mi, dim3(blocks_for(V)), dim3(256), 0, s, vals, V, F, W.hm.p, (uint32_t*)nullptr,
+                       ctr + M_DEGEN);
     hipLaunchKernelGGL(k_filter_mi, dim3(blocks_for(T)), dim3(256), 0, s, G, vals, F, (const uint64_t*)W.hm.p,
                        W.amask.p, W.kcnt.p, W.isact.p);
-    scan_u32(W.mem, W.kcnt.p, W.csr_off.p, Count(T), W.csr_off.p + T, s);
-    scan_u32(W.mem, W.isact.p, W.rank.p, Count(T), ctr + M_ACTIVE, s);
+    {
+        // CSR offsets and active ranks in one pass (the third field is unused)
+        const uint32_t* in[3] = {W.kcnt.p, W.isact.p, W.isact.p};
+        uint32_t* out[3] = {W.csr_off.p, W.rank.p, W.tscan.p};
+        uint32_t* tot[3] = {W.csr_off.p + T, ctr + M_ACTIVE, nullptr};
+        scan3_u32(W.mem, in, 1, out, Count(T), tot, s);
+    }
     hipLaunchKernelGGL(k_mi_check, dim3(1), dim3(64), 0, s, ctr, (const uint32_t*)W.csr_off.p, T, c, W.errs.p);
     hipLaunchKernelGGL(k_mi_fill, dim3(blocks_for(T)), dim3(256), 0, s, T, (const uint64_t*)W.amask.p,
                        (const uint32_t*)W.csr_off.p, (const uint32_t*)W.rank.p, abort, W.fit.p, W.act.p,
@@ -408,12 +416,12 @@ void mi_path(MIWorkspace& W, const Grid& g, const Values& val, const MITablesDev
     hipLaunchKernelGGL(k_mi_count, dim3(blocks_for(c.active)), dim3(256), 0, s, A, (const uint64_t*)W.emask.p,
                        W.cnt.p);
     hipLaunchKernelGGL(k_mi_split_padded, dim3(blocks_for(c.active + 1)), dim3(256), 0, s, NA, (uint64_t)c.active,
-                       (const uint32_t*)W.cnt.p, W.cv.p, W.cf.p, W.cfv.p);
+                       (const uint32_t*)W.cnt.p, W.tv.p, W.tf.p, W.tfv.p);
     {
-        const uint32_t* in[3] = {W.cv.p, W.cf.p, W.cfv.p};
+        const uint32_t* in[3] = {W.tv.p, W.tf.p, W.tfv.p};
         uint32_t* out[3] = {W.cv.p, W.cf.p, W.cfv.p};
         uint32_t* tot[3] = {ctr + M_VO, ctr + M_FO, ctr + M_FV};
-        for (int q = 0; q < 3; ++q) scan_u32(W.mem, in[q], out[q], Count((uint64_t)c.active + 1), tot[q], s);
+        scan3_u32(W.mem, in, 1, out, Count((uint64_t)c.active + 1), tot, s);
     }
     hipLaunchKernelGGL(k_mi_check, dim3(1), dim3(64), 0, s, ctr, (const uint32_t*)W.csr_off.p, T, c, W.errs.p);
     hipLaunchKernelGGL(k_mi_extract, dim3(blocks_for(c.active)), dim3(256), 0, s, A, (const uint64_t*)W.emask.p,

@@ -134,8 +134,11 @@ __global__ void k_filter_ia(GridView g, const uint64_t* __restrict__ pos, const 
     }
 }
 
+// degen: per vertex (null: not written); total (null: not counted): the
+// number of tied vertices added per workgroup
 __global__ void k_highest_mi(const double* __restrict__ vals, uint64_t V, uint32_t F, uint64_t* __restrict__ hm,
-                             uint32_t* __restrict__ degen) {
+                             uint32_t* __restrict__ degen, uint32_t* __restrict__ total) {
+    uint32_t ties = 0;
     GRID_STRIDE(v, V) {
         double mx = vals[v];
         uint32_t id = 0, cnt = 1;
@@ -156,7 +159,19 @@ __global__ void k_highest_mi(const double* __restrict__ vals, uint64_t V, uint32
                 if (vals[(uint64_t)j * V + v] == mx) m |= 1ull << j;
         }
         hm[v] = m;
-        degen[v] = cnt > 1;
+        if (degen) degen[v] = cnt > 1;
+        ties += cnt > 1;
+    }
+    if (total) {
+        __shared__ uint32_t red[4];
+        for (int d = 32; d > 0; d >>= 1) ties += __shfl_xor(ties, d, 64);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = ties;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t t = 0;
+            for (uint32_t w = 0; w < (blockDim.x + 63) / 64; ++w) t += red[w];
+            if (t) atomicAdd(total, t);
+        }
     }
 }
 

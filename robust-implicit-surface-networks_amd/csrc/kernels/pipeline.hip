@@ -546,7 +546,7 @@ void run_pipeline(DevMem& mem, const Grid& g, const Values& val, uint32_t flags,
     } else {
         DBuf<uint64_t> hm(mem, V);
         hipLaunchKernelGGL(k_highest_mi, dim3(nblk(V)), dim3(256), 0, s, (const double*)val.vals.p, V, F, hm.p,
-                           vflag.p);
+                           vflag.p, (uint32_t*)nullptr);
         tm.mark("highest func");
         hipLaunchKernelGGL(k_filter_mi, dim3(nblk(T)), dim3(256), 0, s, g.view, (const double*)val.vals.p, F,
                            (const uint64_t*)hm.p, amask.p, kcnt.p, isact.p);
