@@ -372,10 +372,11 @@ void mi_path(MIWorkspace& W, const Grid& g, const Values& val, const MITablesDev
                            (const uint8_t*)W.fit.p, n2, 5, 8, (uint32_t)0, (const uint32_t*)W.rec_off.p, W.rec.p,
                            W.cnt.p, W.errs.p, (const uint32_t*)bl[2]);
     } else {
-        // two tets a wavefront up to four materials, one for five to eight
-        const unsigned g1 = (unsigned)std::min<uint64_t>((c.active + 1) / 2 + 1, 2048);
+        // a wavefront per tet: these lists are short and each tet's latency
+        // is the kernel's time
+        const unsigned g1 = (unsigned)std::min<uint64_t>(c.active + 1, 2048);
         const unsigned g2 = (unsigned)std::min<uint64_t>(c.active + 1, 1024);
-        hipLaunchKernelGGL((k_cut_group<CMed, 32, true>), dim3(g1), dim3(64), 0, s, G, vals,
+        hipLaunchKernelGGL((k_cut_group<CMed, 64, true>), dim3(g1), dim3(64), 0, s, G, vals,
                            (const uint32_t*)W.act.p, (const uint8_t*)W.act_k.p, (const uint32_t*)W.csr_off.p,
                            (const uint8_t*)W.fit.p, (const uint32_t*)bl[1], n1, (const uint32_t*)W.rec_off.p, W.rec.p,
                            W.cnt.p, W.errs.p, 0);

@@ -652,7 +652,7 @@ void run_pipeline(DevMem& mem, const Grid& g, const Values& val, uint32_t flags,
                                cnt.p, errs.p, 1);
         };
         if (q == 0)
-            MI ? go(k_cut_group<CMed, 32, true>, 2) : go(k_cut_group<CMed, 16, false>, 4);
+            MI ? go(k_cut_group<CMed, 64, true>, 1) : go(k_cut_group<CMed, 16, false>, 4);
         else
             MI ? go(k_cut_group<CBig, 64, true>, 1) : go(k_cut_group<CBig, 64, false>, 1);
         RIN_HIP(hipGetLastError());
