@@ -56,7 +56,7 @@ __host__ __device__ inline uint32_t spread_at(int counter, uint32_t slot) {
 }
 enum Ctr : int {
     C_ACTIVE = 0, C_FIT, C_VO, C_FO, C_FV, C_VERTS, C_FACES, C_FACEVERTS, C_B1, C_B2, C_B3, C_B5, C_BX, C_DEGEN,
-    C_Q2, C_Q3, C_Q5, C_ABORT, C_TETFACES, C_RECW, C_DFAIL, C_CUTK3, C_CUTW3, C_CUTK5, C_CUTW5, C_N
+    C_Q2, C_Q3, C_Q5, C_ABORT, C_TETFACES, C_RECW, C_DFAIL, C_CUTK3, C_CUTW3, C_CUTK5, C_CUTW5, C_E3, C_E5, C_N
 };
 
 struct K1Entry {
@@ -551,15 +551,30 @@ struct TileSums {
     uint32_t v[kTileFields];
 };
 
+// A tet with 3..8 active functions, listed by the count pass with its place
+// inside its tile of cubes: with the tile's prefixes (after the tile scan) it
+// gives the tet's active rank and bucket slot, so its cut can run beside the
+// fill pass instead of after it.
+struct EarlyEntry {
+    uint32_t t, tile;
+    uint32_t lo;   // active functions of the tile's tets before it
+    uint32_t lrb;  // active tets before it | tets of its bucket list before it << 16
+};
+
+// early3, early5 (nullable): lists of the k = 3..4 and 5..8 tets, at most
+// early_cap3 / early_cap5 entries, counted in ctr[C_E3] / ctr[C_E5] (zero at
+// the start of a run).  A tile's entries are contiguous and in tet order.
 __global__ void __launch_bounds__(kCubeTile) k_cube_count(GridView g, const uint64_t* __restrict__ pos,
                                                           const uint64_t* __restrict__ neg, uint64_t full,
-                                                          TileSums* __restrict__ tile_sums) {
+                                                          TileSums* __restrict__ tile_sums, EarlyEntry* early3,
+                                                          EarlyEntry* early5, uint32_t early_cap3,
+                                                          uint32_t early_cap5, uint32_t* ctr) {
     const uint32_t ncubes = (uint32_t)(g.T / 5);
     const uint32_t cube = blockIdx.x * kCubeTile + threadIdx.x;
     uint32_t v[kTileFields] = {0, 0, 0, 0, 0, 0, 0, 0};
+    uint64_t m[5] = {0, 0, 0, 0, 0};
     if (cube < ncubes) {
         uint32_t c[8];
-        uint64_t m[5];
         bool even;
         cube_masks(g, cube, pos, neg, full, c, m, even);
         for (int q = 0; q < 5; ++q) {
@@ -569,8 +584,60 @@ __global__ void __launch_bounds__(kCubeTile) k_cube_count(GridView g, const uint
             if (k) v[TF_B1 + bucket_of(k)]++;
         }
     }
-    __shared__ uint32_t red[kCubeTile / 64][kTileFields];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // tiles holding listed tets rank them inside the tile: exclusive scans of
+    // (active functions, active tets) packed as in the fill pass, and of the
+    // listed counts (k = 3..4 in the low half, 5..8 in the high half); one
+    // atomic add per list and tile reserves the tile's entries
+    const uint32_t want = (early3 ? v[TF_B3] : 0u) | (early5 ? v[TF_B5] << 16 : 0u);
+    if ((early3 || early5) && __syncthreads_or(want != 0)) {
+        __shared__ uint32_t esum[kCubeTile / 64], ebk[kCubeTile / 64], ebase[2];
+        constexpr int kAsBits = 13;
+        const uint32_t packed = v[TF_FIT] << kAsBits | v[TF_ACTIVE];
+        uint32_t inc = packed, binc = want;
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t o = __shfl_up(inc, d, 64), ob = __shfl_up(binc, d, 64);
+            if (lane >= d) {
+                inc += o;
+                binc += ob;
+            }
+        }
+        if (lane == 63) {
+            esum[wave] = inc;
+            ebk[wave] = binc;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t tot = 0;
+            for (int w = 0; w < kCubeTile / 64; ++w) tot += ebk[w];
+            ebase[0] = (tot & 0xffffu) ? atomicAdd(ctr + C_E3, tot & 0xffffu) : 0u;
+            ebase[1] = (tot >> 16) ? atomicAdd(ctr + C_E5, tot >> 16) : 0u;
+        }
+        uint32_t ex = inc - packed, bx = binc - want;
+        for (int w = 0; w < wave; ++w) {
+            ex += esum[w];
+            bx += ebk[w];
+        }
+        __syncthreads();
+        uint32_t lo = ex >> kAsBits, lr = ex & ((1u << kAsBits) - 1u);
+        uint32_t b3 = bx & 0xffffu, b5 = bx >> 16;
+        for (int q = 0; q < 5; ++q) {
+            const int k = __popcll(m[q]);
+            const uint32_t t = cube * 5u + (uint32_t)q;
+            if (k >= 3 && k <= 4 && early3) {
+                const uint32_t at = ebase[0] + b3;
+                if (at < early_cap3) early3[at] = {t, blockIdx.x, lo, lr | b3 << 16};
+                ++b3;
+            } else if (k >= 5 && k <= 8 && early5) {
+                const uint32_t at = ebase[1] + b5;
+                if (at < early_cap5) early5[at] = {t, blockIdx.x, lo, lr | b5 << 16};
+                ++b5;
+            }
+            lo += (uint32_t)k;
+            lr += k != 0;
+        }
+    }
+    __shared__ uint32_t red[kCubeTile / 64][kTileFields];
     for (int x = 0; x < kTileFields - 1; ++x) {
         for (int d = 32; d > 0; d >>= 1) v[x] += __shfl_xor(v[x], d, 64);
         if (lane == 0) red[wave][x] = v[x];
@@ -765,6 +832,16 @@ struct CutArgs {
     uint32_t* ctr;       // for the work statistics (spread counters)
     int stat_k, stat_w;  // spread counters of this bucket's functions and words (-1: none)
     uint8_t* ws;         // thread form: one LaneCut slot per lane of the grid
+    // early form (beside the fill pass): the count pass's list and the tile
+    // prefixes (tf: the tile field of the bucket's slots); the tet's
+    // functions come from its corner sign masks
+    const EarlyEntry* early;
+    const uint32_t* n_early;
+    const TileSums* tiles;
+    const uint64_t *pos, *neg;
+    uint64_t full;
+    uint32_t rec_base, cap_active;
+    int tf;
 };
 
 // persistent per-bucket cut: each wave takes 64 bucket entries at a time
@@ -818,33 +895,69 @@ __global__ void __launch_bounds__(64) k_cut_bucket(CutArgs A) {
 // wave-cooperative cut: one lane group of G lanes per bucket entry (a block
 // is a single wavefront carrying 64 / G entries), the complex in LDS; the
 // group's lanes share the predicates and the splits of each insertion
-template <class C, int G>
+//
+// Early: the tets come from the count pass's list (EarlyEntry) and the tile
+// prefixes, and nothing is read that the fill pass writes: the active rank
+// and bucket slot are the tile's prefixes plus the tet's place in its tile,
+// the functions are the set bits of the tet's corner sign masks (the CSR's
+// order), and the record goes to the slot the fill pass assigns the tet.
+template <class C, int G, bool Early = false>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_cut_wave(CutArgs A) {
     // 64 / G tets per wavefront, one lane group each
     constexpr int NG = 64 / G;
     __shared__ dev::WaveIA<C> Xs[NG];
     const int gid = threadIdx.x / G, lane = threadIdx.x & (G - 1);
     dev::WaveIA<C>& X = Xs[gid];
-    const uint32_t n = *A.abort ? 0u : min(*A.n, A.cap);
     const uint32_t i = blockIdx.x * NG + gid;
-    if (i >= n) return;
-    const uint32_t r = A.list[i];
-    const uint64_t ts = A.lts[i];
-    const int k = A.act_k[r];
-    // corner values: the loads need not wait for k (rows past the tet's
-    // functions read the CSR's next entries, inside its head-room of 64,
-    // with the function id clamped into range; their values go unused)
-    if (lane < 4 * (C::MP - 4)) {
-        const uint32_t t = (uint32_t)ts, s0 = (uint32_t)(ts >> 32);
+    uint32_t r, rec_at;
+    int k;
+    if constexpr (Early) {
+        // the fill pass shares the CUs: its waves wait on memory, these issue
+        // (the one-tet-a-wavefront k = 5..8 cut first)
+        __builtin_amdgcn_s_setprio(G == 64 ? 3 : 2);
+        const uint32_t n = min(*A.n_early, A.cap);
+        if (i >= n) return;
+        const EarlyEntry e = A.early[i];
+        const TileSums& pre = A.tiles[e.tile];
+        r = pre.v[TF_ACTIVE] + (e.lrb & 0xffffu);
+        const uint32_t slot = pre.v[A.tf] + (e.lrb >> 16);
+        if (r >= A.cap_active || slot >= A.cap) return;  // the fill pass flags the overflow
+        rec_at = A.rec_base + slot * A.slot;
         uint32_t v[4];
-        dev::tet_corners(A.g, t, v);
-        const uint32_t fr = A.fit[s0 + (lane >> 2)];
-        const uint32_t f = fr < A.F ? fr : 0u;
-        const double x = A.vals[(uint64_t)f * A.g.V + v[lane & 3]];
-        if (lane < 4 * k && 4 + k <= C::MP) X.P[lane >> 2][lane & 3] = x;
+        dev::tet_corners(A.g, e.t, v);
+        const uint64_t m = A.full & ~((A.pos[v[0]] & A.pos[v[1]] & A.pos[v[2]] & A.pos[v[3]]) |
+                                      (A.neg[v[0]] & A.neg[v[1]] & A.neg[v[2]] & A.neg[v[3]]));
+        k = __popcll(m);
+        if (lane < 4 * (C::MP - 4)) {
+            // row q: the q-th active function (rows past k load function 0, unused)
+            uint64_t mm = m;
+            for (int q = lane >> 2; q > 0 && mm; --q) mm &= mm - 1;
+            const uint32_t f = mm ? (uint32_t)(__ffsll((unsigned long long)mm) - 1) : 0u;
+            const double x = A.vals[(uint64_t)f * A.g.V + v[lane & 3]];
+            if (lane < 4 * k && 4 + k <= C::MP) X.P[lane >> 2][lane & 3] = x;
+        }
+    } else {
+        const uint32_t n = *A.abort ? 0u : min(*A.n, A.cap);
+        if (i >= n) return;
+        r = A.list[i];
+        const uint64_t ts = A.lts[i];
+        k = A.act_k[r];
+        rec_at = 0;
+        // corner values: the loads need not wait for k (rows past the tet's
+        // functions read the CSR's next entries, inside its head-room of 64,
+        // with the function id clamped into range; their values go unused)
+        if (lane < 4 * (C::MP - 4)) {
+            const uint32_t t = (uint32_t)ts, s0 = (uint32_t)(ts >> 32);
+            uint32_t v[4];
+            dev::tet_corners(A.g, t, v);
+            const uint32_t fr = A.fit[s0 + (lane >> 2)];
+            const uint32_t f = fr < A.F ? fr : 0u;
+            const double x = A.vals[(uint64_t)f * A.g.V + v[lane & 3]];
+            if (lane < 4 * k && 4 + k <= C::MP) X.P[lane >> 2][lane & 3] = x;
+        }
     }
     dev::wave_ia_cut<C, G>(X, k);
-    uint16_t* out = A.rec + A.rec_loc[r];
+    uint16_t* out = A.rec + (Early ? rec_at : A.rec_loc[r]);
     if (X.err) {
         if (lane == 0) {
             atomicOr(A.errs, 1u);
@@ -1511,6 +1624,7 @@ struct Workspace {
     DBuf<uint64_t> lts[3];  // per bucket slot: CSR offset << 32 | tet
     DBuf<uint32_t> act_off;  // CSR offset of each active tet, by active rank
     DBuf<TileSums> tile_sums;  // regular grids: per tile of cubes
+    DBuf<EarlyEntry> early[3];  // regular grids: the k = 3..4 and 5..8 tets ([1], [2]), listed by the count pass
     DBuf<uint8_t> fit, act_k;
     DBuf<uint16_t> rec;
     DBuf<uint32_t> cv, cf, cfv;
@@ -1554,10 +1668,10 @@ struct Workspace {
     // the same path as one graph: replays without stage times
     hipGraph_t whole_graph = nullptr;
     hipGraphExec_t whole = nullptr;
-    // side branch of the whole graph (the k = 5..8 and k = 2 cuts run beside
-    // the k = 3..4 cut): a stream to capture it on and the fork / join events
-    hipStream_t side = nullptr;
-    hipEvent_t fork = nullptr, join = nullptr;
+    // side branches of the whole graph (the cut kernels beside each other and
+    // beside the fill pass): streams to capture them on, fork and join events
+    hipStream_t side = nullptr, side2 = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr, join2 = nullptr;
     // pinned, device-mapped summary the closing kernel of a run writes (counters
     // and error flags), read by the host after the stream synchronises
     uint32_t* pub_h = nullptr;
@@ -1590,6 +1704,8 @@ struct Workspace {
         for (auto e : ev) (void)hipEventDestroy(e);
         if (fork) (void)hipEventDestroy(fork);
         if (join) (void)hipEventDestroy(join);
+        if (join2) (void)hipEventDestroy(join2);
+        if (side2) (void)hipStreamDestroy(side2);
         if (side) (void)hipStreamDestroy(side);
         if (pub_h) (void)hipHostFree(pub_h);
     }
@@ -1694,6 +1810,7 @@ void alloc_ws(Workspace& W, const Grid& g, const Caps_& c, hipStream_t s) {
     W.neg.alloc(m, V);
     if (g.view.regular) {
         W.tile_sums.alloc(m, cube_tiles(g) + 1);
+        for (int q = 1; q < 3; ++q) W.early[q].alloc(m, c.b[q] + 1);
     } else {
         W.kcnt.alloc(m, T + 1);
         W.isact.alloc(m, T + 1);
@@ -1792,7 +1909,9 @@ void alloc_ws(Workspace& W, const Grid& g, const Caps_& c, hipStream_t s) {
 // stage 1 (also run eagerly to size the workspace): signs, filter, and
 // either the per-tile sums (regular grids) or the per-tet CSR scans; zero:
 // clear the counters first (the whole-path graph leaves them clear instead)
-void stage_count(Workspace& W, const Grid& g, const Values& val, hipStream_t s, bool zero = true) {
+// early3, early5: list the k = 3..4 / 5..8 tets for launch_cut_early (regular grids)
+void stage_count(Workspace& W, const Grid& g, const Values& val, hipStream_t s, bool zero = true,
+                 bool early3 = false, bool early5 = false) {
     const uint64_t V = g.view.V, T = g.view.T;
     const uint32_t F = val.F;
     if (zero) hipLaunchKernelGGL(k_zero_ctr, dim3(1), dim3(64), 0, s, W.ctr.p, (int)C_N, W.errs.p);
@@ -1801,7 +1920,8 @@ void stage_count(Workspace& W, const Grid& g, const Values& val, hipStream_t s, 
     const uint64_t full = (F == 64) ? ~0ull : ((1ull << F) - 1);
     if (g.view.regular) {
         hipLaunchKernelGGL(k_cube_count, dim3(cube_tiles(g)), dim3(kCubeTile), 0, s, g.view, (const uint64_t*)W.pos.p,
-                           (const uint64_t*)W.neg.p, full, W.tile_sums.p);
+                           (const uint64_t*)W.neg.p, full, W.tile_sums.p, early3 ? W.early[1].p : nullptr,
+                           early5 ? W.early[2].p : nullptr, W.cap.b[1], W.cap.b[2], W.ctr.p);
         hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(kTileScanThreads), 0, s, W.tile_sums.p, cube_tiles(g), W.ctr.p);
         return;
     }
@@ -1880,6 +2000,40 @@ void launch_cut(Workspace& W, const Grid& g, const Values& val, int q, int grid,
     if (W.ws_thread[q].n < need) W.ws_thread[q].alloc(W.mem, need);
     A.ws = W.ws_thread[q].p;
     hipLaunchKernelGGL(k_cut_bucket<C>, dim3(grid), dim3(64), 0, s, A);
+}
+
+// the k = 3..4 (q = 1) or 5..8 (q = 2) cut from the count pass's list,
+// beside the fill pass (regular grids, lane-group forms: grid -1, or -3 for
+// four tets a wavefront)
+void launch_cut_early(Workspace& W, const Grid& g, const Values& val, int q, int grid, hipStream_t s) {
+    CutArgs A = {};
+    A.g = g.view;
+    A.vals = val.vals.p;
+    A.cap = W.cap.b[q];
+    A.slot = W.slot[q];
+    A.F = val.F;
+    A.rec = W.rec.p;
+    A.cnt = W.cnt.p;
+    A.errs = W.errs.p;
+    A.ctr = W.ctr.p;
+    A.stat_k = q == 1 ? SP_CUTK3 : SP_CUTK5;
+    A.stat_w = q == 1 ? SP_CUTW3 : SP_CUTW5;
+    A.early = W.early[q].p;
+    A.n_early = W.ctr.p + (q == 1 ? C_E3 : C_E5);
+    A.tiles = W.tile_sums.p;
+    A.pos = W.pos.p;
+    A.neg = W.neg.p;
+    A.full = (val.F == 64) ? ~0ull : ((1ull << val.F) - 1);
+    A.rec_base = W.rec_base[q];
+    A.cap_active = W.cap.active;
+    A.tf = q == 1 ? TF_B3 : TF_B5;
+    const uint32_t cap = W.cap.b[q];
+    if (q == 2)
+        hipLaunchKernelGGL((k_cut_wave<CBig, 64, true>), dim3(cap), dim3(64), 0, s, A);
+    else if (grid == -3)
+        hipLaunchKernelGGL((k_cut_wave<CMed, 16, true>), dim3((cap + 3) / 4), dim3(64), 0, s, A);
+    else
+        hipLaunchKernelGGL((k_cut_wave<CMed, 32, true>), dim3((cap + 1) / 2), dim3(64), 0, s, A);
 }
 
 void stage_cut(Workspace& W, const Grid& g, const Values& val, const int* grid, hipStream_t s) {
@@ -2227,7 +2381,9 @@ std::string cut_label(const char* base, const Workspace& w, const Plan& P, int q
 
 // the whole path as one graph: the k = 5..8 cut (few tets, long serial
 // insertions) and the k = 2 table lookups on a branch beside the k = 3..4 cut
-// (the cut kernels share no outputs); the three extraction passes write
+// (the cut kernels share no outputs; on regular grids the k = 3..4 and 5..8
+// cuts start beside the fill pass, from the count pass's lists); the three
+// extraction passes write
 // disjoint rows and run as one launch, and so do coordinates and the first
 // face pass
 void record_whole(Workspace& w, const Grid& g, const Values& val, hipStream_t s) {
@@ -2235,18 +2391,44 @@ void record_whole(Workspace& w, const Grid& g, const Values& val, hipStream_t s)
         RIN_HIP(hipStreamCreateWithFlags(&w.side, hipStreamNonBlocking));
         RIN_HIP(hipEventCreateWithFlags(&w.fork, hipEventDisableTiming));
         RIN_HIP(hipEventCreateWithFlags(&w.join, hipEventDisableTiming));
+        RIN_HIP(hipStreamCreateWithFlags(&w.side2, hipStreamNonBlocking));
+        RIN_HIP(hipEventCreateWithFlags(&w.join2, hipEventDisableTiming));
     }
     hipGraph_t gr = nullptr;
     RIN_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
-    stage_count(w, g, val, s, false);
-    stage_fill(w, g, val, s);
-    RIN_HIP(hipEventRecord(w.fork, s));
-    RIN_HIP(hipStreamWaitEvent(w.side, w.fork, 0));
-    if (w.cap.b[2]) launch_cut<CBig>(w, g, val, 2, w.grid[2], w.side);
-    if (w.cap.b[0]) launch_cut<CK2>(w, g, val, 0, w.grid[0], w.side);
-    RIN_HIP(hipEventRecord(w.join, w.side));
-    if (w.cap.b[1]) launch_cut<CMed>(w, g, val, 1, w.grid[1], s);
-    RIN_HIP(hipStreamWaitEvent(s, w.join, 0));
+    // regular grids with lane-group cuts start them right after the counts,
+    // beside the fill pass; the k = 2 lookups follow the fill pass
+    const bool e3 = g.view.regular && w.cap.b[1] && w.grid[1] < 0;
+    const bool e5 = g.view.regular && w.cap.b[2] && w.grid[2] < 0;
+    stage_count(w, g, val, s, false, e3, e5);
+    if (e3 || e5) {
+        // the longest branch (the k = 5..8 cut: one tet per wavefront, bound
+        // by its slowest tet) stays on the capturing stream, captured first
+        RIN_HIP(hipEventRecord(w.fork, s));
+        if (e5) launch_cut_early(w, g, val, 2, w.grid[2], s);
+        if (e3) {
+            RIN_HIP(hipStreamWaitEvent(w.side, w.fork, 0));
+            launch_cut_early(w, g, val, 1, w.grid[1], w.side);
+            RIN_HIP(hipEventRecord(w.join, w.side));
+        }
+        RIN_HIP(hipStreamWaitEvent(w.side2, w.fork, 0));
+        stage_fill(w, g, val, w.side2);
+        if (!e5 && w.cap.b[2]) launch_cut<CBig>(w, g, val, 2, w.grid[2], w.side2);
+        if (!e3 && w.cap.b[1]) launch_cut<CMed>(w, g, val, 1, w.grid[1], w.side2);
+        if (w.cap.b[0]) launch_cut<CK2>(w, g, val, 0, w.grid[0], w.side2);
+        RIN_HIP(hipEventRecord(w.join2, w.side2));
+        if (e3) RIN_HIP(hipStreamWaitEvent(s, w.join, 0));
+        RIN_HIP(hipStreamWaitEvent(s, w.join2, 0));
+    } else {
+        stage_fill(w, g, val, s);
+        RIN_HIP(hipEventRecord(w.fork, s));
+        RIN_HIP(hipStreamWaitEvent(w.side, w.fork, 0));
+        if (w.cap.b[2]) launch_cut<CBig>(w, g, val, 2, w.grid[2], w.side);
+        if (w.cap.b[0]) launch_cut<CK2>(w, g, val, 0, w.grid[0], w.side);
+        RIN_HIP(hipEventRecord(w.join, w.side));
+        if (w.cap.b[1]) launch_cut<CMed>(w, g, val, 1, w.grid[1], s);
+        RIN_HIP(hipStreamWaitEvent(s, w.join, 0));
+    }
     stage_rest(w, g, val, s, REST_SCANS);
     stage_rest(w, g, val, s, REST_EXTRACT_K1 | REST_EXTRACT_REC);
     stage_rest(w, g, val, s, REST_DEDUP);
