@@ -169,6 +169,16 @@ int rin_result_edges(rin_ctx* ctx, const rin_result* res, uint64_t* v12, uint64_
  * start[p + 1])). */
 int rin_result_patch_count(rin_ctx* ctx, const rin_result* res, uint64_t* n_patches);
 int rin_result_patches(rin_ctx* ctx, const rin_result* res, uint64_t* start, uint64_t* faces);
+/* Chains of the result's non-manifold iso-edges, computed on the device on
+ * first request (compute_chains, src/mesh_connectivity.cpp:196-241): edges
+ * with more than two face occurrences, joined across vertices that carry
+ * exactly two such edges; chains numbered by their smallest edge, each listed
+ * in breadth-first order from that edge (the neighbour across its first
+ * vertex, then across its second).  rin_result_chains fills
+ * start[n_chains + 1] and edges[start[n_chains]] (chain c is edges[start[c] ..
+ * start[c + 1]), edge ids as rin_result_edges numbers them). */
+int rin_result_chain_count(rin_ctx* ctx, const rin_result* res, uint64_t* n_chains);
+int rin_result_chains(rin_ctx* ctx, const rin_result* res, uint64_t* start, uint64_t* edges);
 /* Per-tet cut records (layout in include/rin/cut_record.h) of the given tets,
  * packed: rec_start[n+1] word offsets into words[cap_words].  Needs
  * RIN_FLAG_KEEP_RECORDS.  Empty tets get an empty record. */

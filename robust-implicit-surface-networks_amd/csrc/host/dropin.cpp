@@ -322,6 +322,16 @@ void device_patches(rin_ctx* c, const rin_result* r, const DeviceMesh& m, std::v
     for (uint64_t p = 0; p < np; ++p) patches[p].assign(fc.begin() + st[p], fc.begin() + st[p + 1]);
 }
 
+void device_chains(rin_ctx* c, const rin_result* r, std::vector<std::vector<size_t>>& chains) {
+    uint64_t nc = 0, ne = 0;
+    check(c, rin_result_chain_count(c, r, &nc), "rin_result_chain_count");
+    check(c, rin_result_edge_count(c, r, &ne), "rin_result_edge_count");
+    std::vector<uint64_t> st(nc + 1), ed(ne + 1);
+    check(c, rin_result_chains(c, r, st.data(), ed.data()), "rin_result_chains");
+    chains.resize(nc);
+    for (uint64_t i = 0; i < nc; ++i) chains[i].assign(ed.begin() + st[i], ed.begin() + st[i + 1]);
+}
+
 template <class T>
 void push_stat(std::vector<std::string>& sl, std::vector<size_t>& sv, const char* label, T v) {
     sl.emplace_back(label);
@@ -456,6 +466,7 @@ bool implicit_arrangement(bool robust_test, bool use_lookup, bool use_secondary_
         in.cuts = &cuts;
         in.edges_from = [&](std::vector<std::vector<size_t>>& eof, std::vector<Edge>& e) { device_edges(c, r, m, eof, e); };
         in.patches_from = [&](std::vector<std::vector<size_t>>& p) { device_patches(c, r, m, p); };
+        in.chains_from = [&](std::vector<std::vector<size_t>>& ch) { device_chains(c, r, ch); };
         rin::GraphOut g;
         const bool ok = rin::run_graph(in, g);
         for (auto& t : g.timings) {
@@ -585,6 +596,7 @@ bool material_interface(bool robust_test, bool use_lookup, bool use_secondary_lo
         in.cuts = &cuts;
         in.edges_from = [&](std::vector<std::vector<size_t>>& eof, std::vector<Edge>& e) { device_edges(c, r, m, eof, e); };
         in.patches_from = [&](std::vector<std::vector<size_t>>& p) { device_patches(c, r, m, p); };
+        in.chains_from = [&](std::vector<std::vector<size_t>>& ch) { device_chains(c, r, ch); };
         rin::GraphOut g;
         const bool ok = rin::run_graph(in, g);
         for (auto& t : g.timings) {

@@ -964,7 +964,10 @@ bool run_graph(const GraphIn& in, GraphOut& out) {
             out.nm_edges_of_vert[out.edges[i].v1].push_back(i);
             out.nm_edges_of_vert[out.edges[i].v2].push_back(i);
         }
-    compute_chains(out.edges, out.nm_edges_of_vert, out.chains);
+    if (in.chains_from)
+        in.chains_from(out.chains);
+    else
+        compute_chains(out.edges, out.nm_edges_of_vert, out.chains);
     lap("chains");
     out.stats.emplace_back("num_chains", out.chains.size());
 

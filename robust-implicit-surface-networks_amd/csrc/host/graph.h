@@ -71,6 +71,8 @@ struct GraphIn {
     std::function<void(std::vector<std::vector<size_t>>&, std::vector<Edge>&)> edges_from;
     // when set, the patches come from here instead of compute_patches
     std::function<void(std::vector<std::vector<size_t>>&)> patches_from;
+    // when set, the chains come from here instead of compute_chains
+    std::function<void(std::vector<std::vector<size_t>>&)> chains_from;
 };
 
 struct GraphOut {

@@ -759,19 +759,23 @@ __device__ inline uint32_t uf_find(uint32_t* parent, uint32_t x) {
     return x;
 }
 
+// joins the components of a and b: the larger root hooks under the smaller,
+// so every component's root is its smallest element
+__device__ inline void uf_union(uint32_t* parent, uint32_t a, uint32_t b) {
+    for (;;) {
+        a = uf_find(parent, a);
+        b = uf_find(parent, b);
+        if (a == b) return;
+        const uint32_t hi = a > b ? a : b, lo = a > b ? b : a;
+        if (atomicCAS(&parent[hi], hi, lo) == hi) return;
+    }
+}
+
 __global__ void k_uf_hook(Count no, const uint32_t* __restrict__ face_of, const uint32_t* __restrict__ adj,
                           uint32_t* parent) {
     GRID_STRIDE(o, no) {
         const uint32_t v = adj[o];
-        if (v == ~0u) continue;
-        uint32_t a = face_of[o], b = v;
-        for (;;) {
-            a = uf_find(parent, a);
-            b = uf_find(parent, b);
-            if (a == b) break;
-            const uint32_t hi = a > b ? a : b, lo = a > b ? b : a;
-            if (atomicCAS(&parent[hi], hi, lo) == hi) break;
-        }
+        if (v != ~0u) uf_union(parent, face_of[o], v);
     }
 }
 
@@ -781,6 +785,116 @@ __global__ void k_uf_roots(Count nf, uint32_t* parent, uint32_t* __restrict__ se
         const uint32_t r = uf_find(parent, (uint32_t)f);
         parent[f] = r;
         seed[f] = r == f ? 1u : 0u;
+    }
+}
+
+// Chains (compute_chains, src/mesh_connectivity.cpp:196-241): non-manifold
+// edges (more than two face occurrences) join across the vertices that carry
+// exactly two of them, so every component is a path or a cycle.  Its seed is
+// its smallest edge, and its order the breadth-first one from the seed,
+// neighbour across the first vertex before the one across the second.
+__device__ inline bool edge_non_manifold(const uint32_t* __restrict__ e_start, uint32_t i) {
+    return e_start[i + 1] - e_start[i] > 2;
+}
+
+// every non-manifold edge registers at its two vertices (the first two kept)
+__global__ void k_chain_vert(Count ne, const uint32_t* __restrict__ v12, const uint32_t* __restrict__ e_start,
+                             uint32_t* __restrict__ vdeg, uint32_t* __restrict__ vslot) {
+    GRID_STRIDE(i, ne) {
+        if (!edge_non_manifold(e_start, (uint32_t)i)) continue;
+        for (int s = 0; s < 2; ++s) {
+            const uint32_t v = v12[2 * i + s];
+            const uint32_t k = atomicAdd(&vdeg[v], 1u);
+            if (k < 2) vslot[2 * (uint64_t)v + k] = (uint32_t)i;
+        }
+    }
+}
+
+// each edge's neighbours across its first and second vertex (~0u: none), its
+// union-find parent and a cleared visit flag
+__global__ void k_chain_links(Count ne, const uint32_t* __restrict__ v12, const uint32_t* __restrict__ e_start,
+                              const uint32_t* __restrict__ vdeg, const uint32_t* __restrict__ vslot,
+                              uint32_t* __restrict__ nb, uint32_t* __restrict__ parent, uint32_t* __restrict__ seen) {
+    GRID_STRIDE(i, ne) {
+        uint32_t n[2] = {~0u, ~0u};
+        if (edge_non_manifold(e_start, (uint32_t)i))
+            for (int s = 0; s < 2; ++s) {
+                const uint32_t v = v12[2 * i + s];
+                if (vdeg[v] != 2) continue;  // a chain ends at a vertex of one or of three or more
+                const uint32_t a = vslot[2 * (uint64_t)v], b = vslot[2 * (uint64_t)v + 1];
+                n[s] = a == (uint32_t)i ? b : a;
+            }
+        nb[2 * i] = n[0];
+        nb[2 * i + 1] = n[1];
+        parent[i] = (uint32_t)i;
+        seen[i] = 0;
+    }
+}
+
+__global__ void k_chain_hook(Count nl, const uint32_t* __restrict__ nb, uint32_t* parent) {
+    GRID_STRIDE(o, nl) {
+        const uint32_t v = nb[o];
+        if (v != ~0u) uf_union(parent, (uint32_t)(o >> 1), v);
+    }
+}
+
+// roots, and seed flags: non-manifold edges that are their component's root
+__global__ void k_chain_roots(Count ne, const uint32_t* __restrict__ e_start, uint32_t* parent,
+                              uint32_t* __restrict__ seed) {
+    GRID_STRIDE(i, ne) {
+        const uint32_t r = uf_find(parent, (uint32_t)i);
+        parent[i] = r;
+        seed[i] = (r == (uint32_t)i && edge_non_manifold(e_start, (uint32_t)i)) ? 1u : 0u;
+    }
+}
+
+// chain sizes, and each chain's seed
+__global__ void k_chain_count(Count ne, const uint32_t* __restrict__ e_start, const uint32_t* __restrict__ root,
+                              const uint32_t* __restrict__ seed, const uint32_t* __restrict__ srank,
+                              uint32_t* __restrict__ size, uint32_t* __restrict__ seeds) {
+    GRID_STRIDE(i, ne) {
+        if (!edge_non_manifold(e_start, (uint32_t)i)) continue;
+        atomicAdd(&size[srank[root[i]]], 1u);
+        if (seed[i]) seeds[srank[i]] = (uint32_t)i;
+    }
+}
+
+// one lane per chain: the breadth-first walk from the seed.  The queue never
+// holds more than two edges (the seed adds at most two, every later edge at
+// most one: one of its neighbours is the edge it was reached from); err
+// counts chains whose walk does not fill their slice exactly.
+__global__ void k_chain_walk(Count nc, const uint32_t* __restrict__ seeds, const uint32_t* __restrict__ start,
+                             const uint32_t* __restrict__ nb, uint32_t* __restrict__ seen,
+                             uint32_t* __restrict__ out, uint32_t* __restrict__ err) {
+    GRID_STRIDE(c, nc) {
+        const uint32_t s = seeds[c], end = start[c + 1];
+        uint32_t o = start[c];
+        bool bad = false;
+        out[o++] = s;
+        seen[s] = 1;
+        uint32_t qa = s, qb = 0;
+        int qn = 1;
+        while (qn > 0 && !bad) {
+            const uint32_t e = qa;
+            qa = qb;
+            --qn;
+            for (int k = 0; k < 2; ++k) {
+                const uint32_t x = nb[2 * (uint64_t)e + k];
+                if (x == ~0u || seen[x]) continue;
+                if (o >= end || qn == 2) {
+                    bad = true;
+                    break;
+                }
+                seen[x] = 1;
+                out[o++] = x;
+                if (qn == 0)
+                    qa = x;
+                else
+                    qb = x;
+                ++qn;
+            }
+        }
+        if (bad || o != end) atomicAdd(err, 1u);
     }
 }
 

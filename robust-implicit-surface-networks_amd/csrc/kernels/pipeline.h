@@ -123,6 +123,11 @@ struct RunOut {
     uint64_t n_patches = 0;
     DBuf<uint32_t> p_start;  // n_patches + 1
     DBuf<uint32_t> p_faces;  // each patch's faces in breadth-first order
+    // chains of non-manifold edges (compute_chains), on first request
+    bool have_chains = false;
+    uint64_t n_chains = 0;
+    DBuf<uint32_t> c_start;  // n_chains + 1
+    DBuf<uint32_t> c_edges;  // each chain's edges in breadth-first order
 };
 
 void grid_regular(DevMem& mem, int res, const double bmin[3], const double bmax[3], Grid& g, hipStream_t s);
@@ -182,6 +187,8 @@ void gather_records(DevMem& mem, const RunOut& r, const uint64_t* tets, uint64_t
 void compute_edges(DevMem& mem, RunOut& r, hipStream_t s);
 // the result's patches on the device (src/mesh_connectivity.cpp:58-93), computing the edges first
 void compute_patches(DevMem& mem, RunOut& r, hipStream_t s);
+// chains of the result's non-manifold iso-edges (computes the edges first)
+void compute_chains(DevMem& mem, RunOut& r, hipStream_t s);
 
 // diagnostics: a plain device copy of `bytes` (reference kernel for
 // hardware-counter scaling); returns the average milliseconds per copy

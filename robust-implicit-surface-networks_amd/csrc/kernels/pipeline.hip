@@ -1212,6 +1212,52 @@ void compute_patches(DevMem& mem, RunOut& r, hipStream_t s) {
     r.have_patches = true;
 }
 
+void compute_chains(DevMem& mem, RunOut& r, hipStream_t s) {
+    if (r.have_chains) return;
+    compute_edges(mem, r, s);
+    const uint64_t NE = r.n_edges, NV = r.n_verts;
+    r.c_start.alloc(mem, NE + 2);
+    r.c_edges.alloc(mem, NE + 1);
+    if (NE == 0) {
+        r.n_chains = 0;
+        fill_u32(r.c_start.p, 0, 1, s);
+        RIN_HIP(hipStreamSynchronize(s));
+        r.have_chains = true;
+        return;
+    }
+    DBuf<uint32_t> vdeg(mem, NV + 1), vslot(mem, 2 * NV + 2), nb(mem, 2 * NE), parent(mem, NE), seen(mem, NE),
+        seed(mem, NE + 1), srank(mem, NE + 1), size(mem, NE + 1), seeds(mem, NE + 1), err(mem, 1);
+    fill_u32(vdeg.p, 0, NV + 1, s);
+    fill_u32(err.p, 0, 1, s);
+    const uint32_t* v12 = r.e_v12.p;
+    const uint32_t* es = r.e_start.p;
+    hipLaunchKernelGGL(k_chain_vert, dim3(nblk(NE)), dim3(256), 0, s, Count(NE), v12, es, vdeg.p, vslot.p);
+    hipLaunchKernelGGL(k_chain_links, dim3(nblk(NE)), dim3(256), 0, s, Count(NE), v12, es, (const uint32_t*)vdeg.p,
+                       (const uint32_t*)vslot.p, nb.p, parent.p, seen.p);
+    hipLaunchKernelGGL(k_chain_hook, dim3(nblk(2 * NE)), dim3(256), 0, s, Count(2 * NE), (const uint32_t*)nb.p,
+                       parent.p);
+    hipLaunchKernelGGL(k_chain_roots, dim3(nblk(NE)), dim3(256), 0, s, Count(NE), es, parent.p, seed.p);
+    scan_u32(mem, seed.p, srank.p, Count(NE), srank.p + NE, s);
+    const uint64_t NC = read1(srank.p + NE, s);
+    r.n_chains = NC;
+    if (NC == 0) {
+        fill_u32(r.c_start.p, 0, 1, s);
+        RIN_HIP(hipStreamSynchronize(s));
+        r.have_chains = true;
+        return;
+    }
+    fill_u32(size.p, 0, NC + 1, s);
+    hipLaunchKernelGGL(k_chain_count, dim3(nblk(NE)), dim3(256), 0, s, Count(NE), es, (const uint32_t*)parent.p,
+                       (const uint32_t*)seed.p, (const uint32_t*)srank.p, size.p, seeds.p);
+    scan_u32(mem, size.p, r.c_start.p, Count(NC), r.c_start.p + NC, s);
+    hipLaunchKernelGGL(k_chain_walk, dim3(nblk(NC)), dim3(256), 0, s, Count(NC), (const uint32_t*)seeds.p,
+                       (const uint32_t*)r.c_start.p, (const uint32_t*)nb.p, seen.p, r.c_edges.p, err.p);
+    RIN_HIP(hipGetLastError());
+    if (read1(err.p, s) != 0) throw std::runtime_error("chain walk did not cover its component");
+    RIN_HIP(hipStreamSynchronize(s));
+    r.have_chains = true;
+}
+
 float debug_copy(DevMem& mem, uint64_t bytes, int reps, hipStream_t s) {
     const uint64_t n = bytes / 8;
     if (n == 0 || reps < 1) throw std::invalid_argument("copy size and repetitions must be positive");

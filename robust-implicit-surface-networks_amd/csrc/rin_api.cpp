@@ -381,6 +381,29 @@ int rin_result_patches(rin_ctx* c, const rin_result* res, uint64_t* start, uint6
     });
 }
 
+int rin_result_chain_count(rin_ctx* c, const rin_result* res, uint64_t* n_chains) {
+    if (!res || !n_chains) return fail(c, RIN_ERR_ARG, "null argument");
+    return guarded(c, [&] {
+        auto& r = const_cast<rin_result*>(res)->r;  // the chains are cached in the result
+        rin::compute_chains(c->mem, r, c->stream);
+        *n_chains = r.n_chains;
+    });
+}
+
+int rin_result_chains(rin_ctx* c, const rin_result* res, uint64_t* start, uint64_t* edges) {
+    if (!res || !start) return fail(c, RIN_ERR_ARG, "null argument");
+    return guarded(c, [&] {
+        auto& r = const_cast<rin_result*>(res)->r;
+        rin::compute_chains(c->mem, r, c->stream);
+        auto a = fetch(r.c_start.p, r.n_chains + 1, c->stream);
+        const uint64_t n = a[r.n_chains];
+        if (n && !edges) throw std::invalid_argument("null argument");
+        auto b = fetch(r.c_edges.p, n, c->stream);
+        for (uint64_t i = 0; i <= r.n_chains; ++i) start[i] = a[i];
+        for (uint64_t i = 0; i < n; ++i) edges[i] = b[i];
+    });
+}
+
 int rin_result_records(rin_ctx* c, const rin_result* res, const uint64_t* tets, uint64_t n, uint64_t* rec_start,
                        uint16_t* words, uint64_t cap) {
     if (!res || (!tets && n) || !rec_start) return fail(c, RIN_ERR_ARG, "null argument");

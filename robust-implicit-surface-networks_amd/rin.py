@@ -82,6 +82,8 @@ def lib():
         L.rin_result_edges.argtypes = [_vp, _vp, _u64p, _u64p, _u64p, _u64p]
         L.rin_result_patch_count.argtypes = [_vp, _vp, _u64p]
         L.rin_result_patches.argtypes = [_vp, _vp, _u64p, _u64p]
+        L.rin_result_chain_count.argtypes = [_vp, _vp, _u64p]
+        L.rin_result_chains.argtypes = [_vp, _vp, _u64p, _u64p]
         L.rin_result_stage_times.argtypes = [_vp, _vp, _f64p, ctypes.c_int32]
         L.rin_result_stage_label.argtypes = [_vp, _vp, ctypes.c_int32]
         L.rin_result_stage_label.restype = ctypes.c_char_p
@@ -311,6 +313,17 @@ class Result:
         fc = np.zeros(max(1, self.sizes.n_faces), dtype=np.uint64)
         self.ctx.check(lib().rin_result_patches(self.ctx.h, self.h, _p(st, _u64p), _p(fc, _u64p)))
         return st, fc[:self.sizes.n_faces]
+
+    def chains(self):
+        """Chains of non-manifold edges on the device: (start (NC + 1), edges)."""
+        n = ctypes.c_uint64(0)
+        self.ctx.check(lib().rin_result_chain_count(self.ctx.h, self.h, ctypes.byref(n)))
+        ne = ctypes.c_uint64(0)
+        self.ctx.check(lib().rin_result_edge_count(self.ctx.h, self.h, ctypes.byref(ne)))
+        st = np.zeros(n.value + 1, dtype=np.uint64)
+        ed = np.zeros(max(1, ne.value), dtype=np.uint64)
+        self.ctx.check(lib().rin_result_chains(self.ctx.h, self.h, _p(st, _u64p), _p(ed, _u64p)))
+        return st, ed[:int(st[-1])]
 
     def cut_counts(self):
         n = self.sizes.num_intersecting

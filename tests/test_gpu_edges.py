@@ -1,9 +1,12 @@
-"""Iso-edges and patches on the device (rin_result_edges, rin_result_patches)
-against compute_mesh_edges and compute_patches (src/mesh_connectivity.cpp:
-10-93) restated over the oracle's faces: edge ids in order of first
-occurrence over (face, j), each edge's endpoints (smaller first), its (face,
-j) list in that order, and each face's edge list; patches seeded at their
-smallest unvisited face, faces in queue order across manifold edges."""
+"""Iso-edges, patches and chains on the device (rin_result_edges,
+rin_result_patches, rin_result_chains) against compute_mesh_edges,
+compute_patches and compute_chains (src/mesh_connectivity.cpp:10-93, 196-241)
+restated over the oracle's faces: edge ids in order of first occurrence over
+(face, j), each edge's endpoints (smaller first), its (face, j) list in that
+order, and each face's edge list; patches seeded at their smallest unvisited
+face, faces in queue order across manifold edges; chains seeded at their
+smallest unvisited non-manifold edge, edges in queue order across vertices
+with exactly two non-manifold edges."""
 from collections import deque
 
 import numpy as np
@@ -227,3 +230,113 @@ def test_patches_headline(ctx):
     vals.release()
     g.release()
 
+
+
+def _chains_of(v12, nocc):
+    """The reference's loop (src/mesh_connectivity.cpp:196-241) over edges with
+    endpoints v12 and nocc face occurrences each; the per-vertex lists of
+    non-manifold edges are built in edge order, as its caller builds them."""
+    nm = {}
+    for i, (a, b) in enumerate(v12):
+        if nocc[i] > 2:
+            nm.setdefault(int(a), []).append(i)
+            nm.setdefault(int(b), []).append(i)
+    seen = [False] * len(v12)
+    chains = []
+    for i in range(len(v12)):
+        if seen[i] or nocc[i] <= 2:
+            continue
+        chain = [i]
+        seen[i] = True
+        q = deque([i])
+        while q:
+            e = q.popleft()
+            for v in (int(v12[e][0]), int(v12[e][1])):
+                lst = nm.get(v, [])
+                if len(lst) != 2:
+                    continue
+                o = lst[1] if lst[0] == e else lst[0]
+                if not seen[o]:
+                    seen[o] = True
+                    q.append(o)
+                    chain.append(o)
+        chains.append(chain)
+    return chains
+
+
+def _check_chains(dev, want):
+    st, ed = dev.chains()
+    assert len(st) - 1 == len(want)
+    flat = np.array([e for c in want for e in c], dtype=np.uint64)
+    np.testing.assert_array_equal(ed, flat)
+    np.testing.assert_array_equal(np.diff(st.astype(np.int64)), [len(c) for c in want])
+    # cached: a second request returns the same lists
+    st2, ed2 = dev.chains()
+    np.testing.assert_array_equal(st2, st)
+    np.testing.assert_array_equal(ed2, ed)
+
+
+@pytest.mark.parametrize("mi,which", [(0, "config1"), (0, "config4"), (1, "config4"), (1, "config3_small"),
+                                      (0, "config2_res48")])
+def test_chains_match_reference_loop(ctx, mi, which):
+    """Chains over the oracle's faces (its own edges, restated), for both forms."""
+    import rin
+
+    if which == "config1":
+        res, fs = W.config1()
+    elif which == "config4":
+        res, fs = W.config4(shuffle_seed=44)
+    elif which == "config2_res48":
+        _, fs = W.config2()
+        res = 48
+    else:
+        _, fs = W.config3()
+        res = 24
+    types, params = fs.arrays()
+    g = ctx.grid_regular(res)
+    vals = g.eval(types, params)
+    pts, tets = S.grid(res)
+    ref = S.OracleRun(mi, pts, tets, S.evaluate(types, params, pts))
+    rv, rocc, _ = _edges_of_faces(ref.face_start, ref.face_verts)
+    want = _chains_of(rv, [len(o) for o in rocc])
+    assert len(want) > 0
+    for flags in (0, rin.FLAG_BATCHED):
+        dev = vals.run((rin.FLAG_MI if mi else 0) | flags)
+        _check_chains(dev, want)
+        dev.release()
+    vals.release()
+    g.release()
+
+
+def test_chains_empty(ctx):
+    """One function: no non-manifold edges, no chains; no faces at all: none."""
+    g = ctx.grid_regular(8)
+    for planes in ([[0.0, 0.0, 0.5]], [[0.0, 0.0, 5.0]]):
+        fs = W.mixed(np.random.default_rng(3), 0, 0, 0, 0)
+        fs.plane(planes[0], [0.0, 0.0, 1.0])
+        types, params = fs.arrays()
+        vals = g.eval(types, params)
+        dev = vals.run(0)
+        st, ed = dev.chains()
+        assert list(st) == [0] and len(ed) == 0
+        dev.release()
+        vals.release()
+
+
+def test_chains_headline(ctx):
+    """At the 101^3-vertex, 20-function size, the device chains equal the
+    reference loop run over the device's own edges; the workload has both
+    open chains and closed ones (cycles: the last edge meets the first)."""
+    import rin
+
+    res, fs = W.config2()
+    types, params = fs.arrays()
+    g = ctx.grid_regular(res)
+    vals = g.eval(types, params)
+    dev = vals.run(rin.FLAG_BATCHED)
+    v12, st, _, _ = dev.edges()
+    want = _chains_of(v12, np.diff(st.astype(np.int64)))
+    _check_chains(dev, want)
+    dev.release()
+    vals.release()
+    g.release()
