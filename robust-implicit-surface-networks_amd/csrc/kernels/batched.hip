@@ -570,7 +570,8 @@ __global__ void __launch_bounds__(kCubeTile) k_cube_count(GridView g, const uint
                                                           EarlyEntry* early5, uint32_t early_cap3,
                                                           uint32_t early_cap5, uint32_t* ctr) {
     const uint32_t ncubes = (uint32_t)(g.T / 5);
-    const uint32_t cube = blockIdx.x * kCubeTile + threadIdx.x;
+    const uint32_t tile = blockIdx.x;
+    const uint32_t cube = tile * kCubeTile + threadIdx.x;
     uint32_t v[kTileFields] = {0, 0, 0, 0, 0, 0, 0, 0};
     uint64_t m[5] = {0, 0, 0, 0, 0};
     if (cube < ncubes) {
@@ -626,11 +627,11 @@ __global__ void __launch_bounds__(kCubeTile) k_cube_count(GridView g, const uint
             const uint32_t t = cube * 5u + (uint32_t)q;
             if (k >= 3 && k <= 4 && early3) {
                 const uint32_t at = ebase[0] + b3;
-                if (at < early_cap3) early3[at] = {t, blockIdx.x, lo, lr | b3 << 16};
+                if (at < early_cap3) early3[at] = {t, tile, lo, lr | b3 << 16};
                 ++b3;
             } else if (k >= 5 && k <= 8 && early5) {
                 const uint32_t at = ebase[1] + b5;
-                if (at < early_cap5) early5[at] = {t, blockIdx.x, lo, lr | b5 << 16};
+                if (at < early_cap5) early5[at] = {t, tile, lo, lr | b5 << 16};
                 ++b5;
             }
             lo += (uint32_t)k;
@@ -647,7 +648,7 @@ __global__ void __launch_bounds__(kCubeTile) k_cube_count(GridView g, const uint
         uint32_t t = 0;
         if (threadIdx.x < kTileFields - 1)
             for (int w = 0; w < kCubeTile / 64; ++w) t += red[w][threadIdx.x];
-        tile_sums[blockIdx.x].v[threadIdx.x] = t;
+        tile_sums[tile].v[threadIdx.x] = t;
     }
 }
 
@@ -719,9 +720,10 @@ __global__ void __launch_bounds__(kCubeTile) k_cube_fill(FillArgs A, const TileS
     __shared__ uint32_t s_off[5 * kCubeTile], s_rank[5 * kCubeTile];  // per-tet CSR entries of the tile
     for (int i = threadIdx.x; i < 81; i += blockDim.x) tab[i] = A.k1[i];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const TileSums pre = tile_sums[blockIdx.x];
+    const uint32_t tile = blockIdx.x;
+    const TileSums pre = tile_sums[tile];
     const uint32_t ncubes = (uint32_t)(A.g.T / 5);
-    const uint32_t cube = blockIdx.x * kCubeTile + threadIdx.x;
+    const uint32_t cube = tile * kCubeTile + threadIdx.x;
     uint32_t c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     uint64_t m[5] = {0, 0, 0, 0, 0};
     bool even = true;
@@ -808,7 +810,7 @@ __global__ void __launch_bounds__(kCubeTile) k_cube_fill(FillArgs A, const TileS
     }
     // the tile's per-tet CSR entries, coalesced
     __syncthreads();
-    const uint64_t t0 = (uint64_t)blockIdx.x * (5 * kCubeTile), T = A.g.T;
+    const uint64_t t0 = (uint64_t)tile * (5 * kCubeTile), T = A.g.T;
     for (int i = threadIdx.x; i < 5 * kCubeTile; i += kCubeTile)
         if (t0 + i < T) {
             A.csr_off_w[t0 + i] = s_off[i];

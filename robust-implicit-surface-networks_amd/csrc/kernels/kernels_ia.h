@@ -41,6 +41,20 @@ __device__ inline uint64_t count_of(const Count& n) { return n.get(); }
 
 __device__ inline void flag_error(uint32_t* errs, uint32_t bit) { errs[threadIdx.x & 63] |= bit; }
 
+// The chip's 8 XCDs each have their own L2, and a launch's workgroups go to
+// them round-robin (workgroup b to XCD b mod 8).  Tiled kernels renumber
+// their workgroups so that each XCD takes one contiguous range of the n live
+// tiles: neighbouring tiles, which read the same lines (grid corners shared
+// by neighbouring cubes, the star tets of a vertex, the occurrences of
+// nearby tets), then meet in one L2 instead of being fetched by several.
+// A bijection of [0, n) for b < n: XCD x takes n / 8 tiles, one more if
+// x < n mod 8.
+constexpr uint32_t kXcds = 8;
+__device__ inline uint32_t xcd_tile(uint32_t b, uint32_t n) {
+    const uint32_t q = n / kXcds, r = n % kXcds, x = b % kXcds;
+    return x * q + (x < r ? x : r) + b / kXcds;
+}
+
 // One lane's thread-cut state (the complex and the insertion's working
 // arrays) in a device-memory workspace: lane i of a launch owns bytes
 // [i * stride, (i + 1) * stride).  Kept out of the private segment, so the
@@ -1063,8 +1077,10 @@ __device__ inline uint64_t star_lowest_tet(const StarArgs& A, uint32_t R, uint32
 __global__ void __launch_bounds__(256) k_dedup_star(Count n, StarArgs A) {
     const uint32_t R = (uint32_t)A.res, N = R + 1;
     const uint32_t NO = (uint32_t)count_of(n);
-    const uint32_t base = blockIdx.x * kDedupTile;
-    if (base >= NO) return;
+    const uint32_t nt = (NO + kDedupTile - 1) / kDedupTile;
+    if (blockIdx.x >= nt) return;
+    const uint32_t tile = xcd_tile(blockIdx.x, nt);
+    const uint32_t base = tile * kDedupTile;
     const uint32_t NA = *A.n_active;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     uint64_t K[kDedupU], ts[kDedupU];
@@ -1134,7 +1150,7 @@ __global__ void __launch_bounds__(256) k_dedup_star(Count n, StarArgs A) {
     for (int d = 32; d > 0; d >>= 1) nfirst += __shfl_xor(nfirst, d, 64);
     if (lane == 0) red[wave] = nfirst;
     __syncthreads();
-    if (threadIdx.x == 0) A.tile_first[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+    if (threadIdx.x == 0) A.tile_first[tile] = red[0] + red[1] + red[2] + red[3];
 }
 
 // Exclusive prefixes of per-tile counts in place, one workgroup; the tiles
@@ -1369,8 +1385,10 @@ __global__ void __launch_bounds__(256) k_dedup_rank(Count n, const uint32_t* __r
         }
     }
     const uint32_t NO = (uint32_t)count_of(n);
-    const uint32_t base = blockIdx.x * kDedupTile;
-    if (base >= NO) return;
+    const uint32_t nt = (NO + kDedupTile - 1) / kDedupTile;
+    if (blockIdx.x >= nt) return;
+    const uint32_t tile = xcd_tile(blockIdx.x, nt);
+    const uint32_t base = tile * kDedupTile;
     __shared__ uint32_t ws[kDedupU][256 / 64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     uint32_t f[kDedupU];
@@ -1386,7 +1404,7 @@ __global__ void __launch_bounds__(256) k_dedup_rank(Count n, const uint32_t* __r
         if (lane == 0) ws[u][wave] = (uint32_t)__popcll(m[u]);
     }
     __syncthreads();
-    uint32_t carry = tile_first[blockIdx.x];
+    uint32_t carry = tile_first[tile];
 #pragma unroll
     for (int u = 0; u < kDedupU; ++u) {
         uint32_t before = 0, tot = 0;
@@ -1786,8 +1804,10 @@ __device__ inline void face_first_role(const FaceTileArgs& A, uint32_t blk) {
         A.sizes_out[0] = NQ;
         A.sizes_out[1] = nfv;
     }
+    const uint32_t nt = (NQ + kFaceTile - 1) / kFaceTile;
+    if (blk >= nt) return;
+    blk = xcd_tile(blk, nt);
     const uint32_t base = blk * kFaceTile;
-    if (base >= NQ) return;
     const uint32_t NA = (uint32_t)A.n_active.get();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     uint32_t sf = 0, st = 0, sv = 0;
@@ -1882,8 +1902,10 @@ __device__ inline uint32_t wave_inclusive(uint32_t x, int lane) {
 // those of the tiles before its own
 __device__ inline void face_emit_role(const FaceTileArgs& A, uint32_t blk) {
     const uint32_t NQ = A.sizes ? A.sizes[0] : (uint32_t)A.nq.get();
+    const uint32_t nt = (NQ + kFaceTile - 1) / kFaceTile;
+    if (blk >= nt) return;
+    blk = xcd_tile(blk, nt);
     const uint32_t base = blk * kFaceTile;
-    if (base >= NQ) return;
     const uint32_t nfv = A.sizes ? A.sizes[1] : clamp_count(A.nfv_total, A.nfv_cap);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     __shared__ uint32_t ws[kFaceU][3][256 / 64];
