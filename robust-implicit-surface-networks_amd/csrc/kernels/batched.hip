@@ -170,7 +170,14 @@ struct K2Table {
     std::vector<uint32_t> off;  // word offset of each slot's record, or ~0u
     std::vector<uint32_t> cnt;  // iso vertices | iso faces << 8 | iso-face corners << 16
     std::vector<uint16_t> words;
+    // The complex after both planes, per slot (byte offset, or ~0u): the state
+    // the lane-group cuts of 3..8 functions continue from.  Bytes: nv, nf, nc,
+    // dup; grp[0..5]; ori[0..5]; per vertex its plane triple; per face fa, fb,
+    // fc, fn, fv[6], fe[6].
+    std::vector<uint32_t> snap_off;
+    std::vector<uint8_t> snap;
 };
+constexpr int kK2SnapHead = 16, kK2SnapFace = 16;
 constexpr uint32_t K2_NONE = 0xFFFFFFFFu;
 
 const K2Table& k2_table() {
@@ -214,6 +221,7 @@ const K2Table& k2_table() {
         T.nslots = base;
         T.off.assign(base, K2_NONE);
         T.cnt.assign(base, 0);
+        T.snap_off.assign(base, K2_NONE);
         for (int pat = 0; pat < 81; ++pat) {
             const K2Pat& K = T.pat[pat];
             if (!K.valid) continue;
@@ -263,11 +271,102 @@ const K2Table& k2_table() {
                     T.off[slot] = (uint32_t)T.words.size();
                     T.cnt[slot] = niv | nif << 8 | nfv << 16;
                     T.words.insert(T.words.end(), rec, rec + len);
+                    static_assert(CK2::MP == 6 && CK2::MFV == 6, "k2 snapshot layout");
+                    T.snap_off[slot] = (uint32_t)T.snap.size();
+                    const uint8_t head[4] = {(uint8_t)X2.nv, (uint8_t)X2.nf, (uint8_t)X2.nc, (uint8_t)X2.dup};
+                    T.snap.insert(T.snap.end(), head, head + 4);
+                    for (int p = 0; p < 6; ++p) T.snap.push_back(X2.grp[p]);
+                    for (int p = 0; p < 6; ++p) T.snap.push_back(X2.ori[p]);
+                    for (int v = 0; v < X2.nv; ++v)
+                        for (int q = 0; q < 3; ++q) T.snap.push_back(X2.vl[v][q]);
+                    for (int f = 0; f < X2.nf; ++f) {
+                        const uint8_t fh[4] = {X2.fa[f], X2.fb[f], X2.fc[f], X2.fn[f]};
+                        T.snap.insert(T.snap.end(), fh, fh + 4);
+                        for (int q = 0; q < 6; ++q) T.snap.push_back(q < X2.fn[f] ? X2.fv[f][q] : (uint8_t)0);
+                        for (int q = 0; q < 6; ++q) T.snap.push_back(q < X2.fn[f] ? X2.fe[f][q] : (uint8_t)0);
+                    }
                 }
             }
         }
     });
     return T;
+}
+
+// The lane-group cuts of 3..8 functions start from the complex after the
+// tet's first two planes, the k = 2 snapshot of the tet's slot: two of their
+// insertions become a table read.
+struct K2Snap {
+    const K2Pat* pat;
+    const uint32_t* off;  // per slot: byte offset of the snapshot, or K2_NONE (null: start from the tet)
+    const uint8_t* bytes;
+};
+
+// the slot of a tet whose first two rows are P[0], P[1], as k_k2_lookup
+// computes it (corner signs and at most four exact edge predicates), or K2_NONE
+__device__ inline uint32_t k2_slot_of(const K2Pat* __restrict__ pats, const double (*P)[4]) {
+    int pa = 0, sB[4];
+    for (int c = 0, w = 1; c < 4; ++c, w *= 3) {
+        const double a = P[0][c], b = P[1][c];
+        pa += (a > 0 ? 2 : (a < 0 ? 0 : 1)) * w;
+        sB[c] = b > 0 ? 1 : (b < 0 ? -1 : 0);
+    }
+    const K2Pat K = pats[pa];
+    if (!K.valid) return K2_NONE;
+    uint32_t slot = K.base, w = 81;
+    for (int c = 0, w3 = 1; c < 4; ++c, w3 *= 3) slot += (uint32_t)(sB[c] + 1) * w3;
+    bool err = false;
+    for (int q = 0; q + 4 < K.nv; ++q, w *= 3) {
+        const int c1 = K.e[q][0], c2 = K.e[q][1];
+        int sg;
+        if (sB[c1] * sB[c2] < 0)
+            sg = dev::orient_ia_edge(P[0][c1], P[0][c2], P[1][c1], P[1][c2], err);
+        else
+            sg = sB[c1] == sB[c2] ? sB[c1] : (sB[c1] != 0 ? sB[c1] : sB[c2]);
+        slot += (uint32_t)(sg + 1) * w;
+    }
+    return err ? K2_NONE : slot;
+}
+
+// Loads the group's complex after the first two planes of its tet (rows in
+// X.P, visible to the group) and returns true, or returns false (no snapshot:
+// the cut starts from the tet).  Uniform over the group.
+template <class C, int G>
+__device__ bool k2_start(dev::WaveIA<C>& X, const K2Snap& S, int k) {
+    const int lane = dev::Grp<G>::lane();
+    if (!S.off || k < 3 || 4 + k > C::MP) return false;
+    const uint32_t slot = k2_slot_of(S.pat, X.P);
+    if (slot == K2_NONE) return false;
+    const uint32_t off = S.off[slot];
+    if (off == K2_NONE) return false;
+    const uint8_t* __restrict__ b = S.bytes + off;
+    const int nv = b[0], nf = b[1];
+    if (lane == 0) {
+        X.nv = nv;
+        X.nf = nf;
+        X.nc = b[2];
+        X.dup = b[3];
+        X.err = 0;
+    }
+    if (lane < C::MP) {
+        X.grp[lane] = lane < 6 ? b[4 + lane] : dev::NIL;
+        X.ori[lane] = lane < 6 ? b[10 + lane] : (uint8_t)1;
+    }
+    const uint8_t* __restrict__ vb = b + kK2SnapHead;
+    for (int v = lane; v < nv; v += G)
+        for (int q = 0; q < 3; ++q) X.vl[v][q] = vb[3 * v + q];
+    const uint8_t* __restrict__ fb = vb + 3 * nv;
+    for (int f = lane; f < nf; f += G) {
+        const uint8_t* __restrict__ r = fb + kK2SnapFace * f;
+        X.fa[f] = r[0];
+        X.fb[f] = r[1];
+        X.fc[f] = r[2];
+        X.fn[f] = r[3];
+        for (int q = 0; q < C::MFV; ++q) {
+            X.fv[f][q] = q < 6 ? r[4 + q] : (uint8_t)0;
+            X.fe[f][q] = q < 6 ? r[10 + q] : (uint8_t)0;
+        }
+    }
+    return true;
 }
 
 // Extraction plans of the k = 2 table records: what the record pass would
@@ -844,6 +943,7 @@ struct CutArgs {
     uint64_t full;
     uint32_t rec_base, cap_active;
     int tf;
+    K2Snap k2s;  // lane-group forms: the complexes after two planes (off null: none)
 };
 
 // persistent per-bucket cut: each wave takes 64 bucket entries at a time
@@ -958,7 +1058,12 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
             if (lane < 4 * k && 4 + k <= C::MP) X.P[lane >> 2][lane & 3] = x;
         }
     }
-    dev::wave_ia_cut<C, G>(X, k);
+    int start = 0;
+    if (A.k2s.off) {
+        __syncthreads();  // the rows, for the slot
+        start = k2_start<C, G>(X, A.k2s, k) ? 2 : 0;
+    }
+    dev::wave_ia_cut<C, G>(X, k, start);
     uint16_t* out = A.rec + (Early ? rec_at : A.rec_loc[r]);
     if (X.err) {
         if (lane == 0) {
@@ -1658,6 +1763,9 @@ struct Workspace {
     DBuf<uint16_t> k1_words;  // table records: k = 1 patterns, then k = 2 slots
     DBuf<uint32_t> k1_off;    // word offset per table index (81 + k = 2 slots)
     DBuf<K2Pat> k2pat;
+    DBuf<uint32_t> k2snap_off;  // complexes after two planes (lane-group cuts of 3..8 functions)
+    DBuf<uint8_t> k2snap;
+    bool k2_start = true;       // those cuts start from them
     DBuf<uint32_t> k2plan_off;
     DBuf<uint8_t> k2plan;
     DBuf<uint32_t> k2cnt;
@@ -1897,6 +2005,13 @@ void alloc_ws(Workspace& W, const Grid& g, const Caps_& c, hipStream_t s) {
     RIN_HIP(hipMemcpyAsync(W.k1_off.p, off.data(), 4 * off.size(), hipMemcpyHostToDevice, s));
     RIN_HIP(hipMemcpyAsync(W.k2pat.p, K2.pat, sizeof(K2.pat), hipMemcpyHostToDevice, s));
     RIN_HIP(hipMemcpyAsync(W.k2cnt.p, K2.cnt.data(), 4 * K2.cnt.size(), hipMemcpyHostToDevice, s));
+    // RIN_K2_START=0: the lane-group cuts insert every plane themselves
+    const char* ks = std::getenv("RIN_K2_START");
+    W.k2_start = !(ks && !std::strcmp(ks, "0"));
+    W.k2snap_off.alloc(m, K2.snap_off.size() + 1);
+    W.k2snap.alloc(m, K2.snap.size() + 1);
+    RIN_HIP(hipMemcpyAsync(W.k2snap_off.p, K2.snap_off.data(), 4 * K2.snap_off.size(), hipMemcpyHostToDevice, s));
+    RIN_HIP(hipMemcpyAsync(W.k2snap.p, K2.snap.data(), K2.snap.size(), hipMemcpyHostToDevice, s));
     const K2Plans& KP = k2_plans();
     W.k2plan_off.alloc(m, KP.off.size() + 1);
     W.k2plan.alloc(m, KP.bytes.size() + 1);
@@ -1957,6 +2072,7 @@ void launch_cut(Workspace& W, const Grid& g, const Values& val, int q, int grid,
     A.ctr = W.ctr.p;
     A.stat_k = q == 1 ? SP_CUTK3 : (q == 2 ? SP_CUTK5 : -1);
     A.stat_w = q == 1 ? SP_CUTW3 : (q == 2 ? SP_CUTW5 : -1);
+    A.k2s = W.k2_start ? K2Snap{W.k2pat.p, W.k2snap_off.p, W.k2snap.p} : K2Snap{nullptr, nullptr, nullptr};
     if (grid == -2) {  // k = 2 table
         K2Args K;
         K.g = g.view;
@@ -2029,6 +2145,7 @@ void launch_cut_early(Workspace& W, const Grid& g, const Values& val, int q, int
     A.rec_base = W.rec_base[q];
     A.cap_active = W.cap.active;
     A.tf = q == 1 ? TF_B3 : TF_B5;
+    A.k2s = W.k2_start ? K2Snap{W.k2pat.p, W.k2snap_off.p, W.k2snap.p} : K2Snap{nullptr, nullptr, nullptr};
     const uint32_t cap = W.cap.b[q];
     if (q == 2)
         hipLaunchKernelGGL((k_cut_wave<CBig, 64, true>), dim3(cap), dim3(64), 0, s, A);

@@ -244,11 +244,13 @@ __device__ inline int split_side_regs(const uint8_t (&fvr)[MFV], const uint8_t (
 // Runs the insertion of k planes (X.P[0..k-1], loaded by the caller) into
 // the tet; on return X.err is set when a capacity or the predicates'
 // exponent range was exceeded.
+// start: planes already inserted (0, or 2 when the caller has loaded the
+// complex after the first two, which depends on their signs only)
 template <class C, int G = 64>
-__device__ void wave_ia_cut(WaveIA<C>& X, int k) {
+__device__ void wave_ia_cut(WaveIA<C>& X, int k, int start = 0) {
     using W = Grp<G>;
     const int lane = W::lane();
-    if (lane < 4) {
+    if (start == 0 && lane < 4) {
         const int i = lane;
         for (int q = 0; q < 3; ++q) X.vl[i][q] = (uint8_t)(q < i ? q : q + 1);  // corner i: planes != i
         X.fa[i] = (uint8_t)i;
@@ -256,7 +258,7 @@ __device__ void wave_ia_cut(WaveIA<C>& X, int k) {
         X.fc[i] = NIL;
         X.fn[i] = 3;
     }
-    if (lane == 0) {
+    if (start == 0 && lane == 0) {
         const uint8_t cyc[4][3] = {{1, 3, 2}, {0, 2, 3}, {0, 3, 1}, {0, 1, 2}};
         for (int i = 0; i < 4; ++i)
             for (int q = 0; q < 3; ++q) {
@@ -269,7 +271,7 @@ __device__ void wave_ia_cut(WaveIA<C>& X, int k) {
         X.dup = 0;
         X.err = (4 + k > C::MP) ? 1 : 0;
     }
-    if (lane < C::MP) {
+    if (start == 0 && lane < C::MP) {
         X.grp[lane] = lane < 4 ? (uint8_t)lane : NIL;
         X.ori[lane] = 1;
     }
@@ -279,7 +281,7 @@ __device__ void wave_ia_cut(WaveIA<C>& X, int k) {
         if (lane == 0) X.err = 1;
         __syncthreads();
     };
-    for (int j = 0; j < k; ++j) {
+    for (int j = start; j < k; ++j) {
         const uint8_t pid = (uint8_t)(4 + j);
         const int nv0 = X.nv, nf0 = X.nf, nc0 = X.nc;
 

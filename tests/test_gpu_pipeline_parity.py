@@ -205,16 +205,21 @@ def test_pipeline_mi_explicit_mesh(ctx):
     _compare(dev, ref)
 
 
-@pytest.mark.parametrize("form", ["auto", "wave", "wave16", "thread", "table"])
+@pytest.mark.parametrize("form", ["auto", "wave", "wave16", "thread", "table", "wave-scratch", "wave16-scratch"])
 @pytest.mark.parametrize("which", ["config1", "config2_small", "config4"])
 def test_batched_matches_oracle_and_replays(ctx, which, form, monkeypatch):
     """The batched form (k=1 table, bucket kernels, captured graph) gives the
     oracle's output bit for bit, on the first (capturing) run and on graph
     replays with plain-form runs between them; the bucket cut kernels are the
     autotuned choice, or forced to the wave-per-tet or thread-per-tet form, or
-    k = 2 by the sign-vector table."""
+    k = 2 by the sign-vector table.  The lane-group cuts of 3..8 functions
+    start from the k = 2 table's complex after two planes, or ("-scratch")
+    insert every plane themselves."""
     import rin
 
+    if form.endswith("-scratch"):
+        monkeypatch.setenv("RIN_K2_START", "0")
+        form = form[: -len("-scratch")]
     if form != "auto":
         monkeypatch.setenv("RIN_CUT_FORM", form)
 
@@ -332,12 +337,17 @@ def test_large_complexes_are_cut_again(ctx, mi, batched):
     g.release()
 
 
-@pytest.mark.parametrize("form", ["batched", "plain"])
-def test_headline_workload_full_size(ctx, form):
+@pytest.mark.parametrize("form", ["batched", "plain", "batched-scratch"])
+def test_headline_workload_full_size(ctx, form, monkeypatch):
     """The benchmark workload itself: 101^3 vertices (5,000,000 tets), 20
     functions (6 planes, 6 spheres, 4 cylinders, 4 tori), in both forms, bit
-    for bit against the oracle (counts, CSR, vertices and coordinates, faces)."""
+    for bit against the oracle (counts, CSR, vertices and coordinates, faces);
+    "-scratch": the lane-group cuts insert every plane themselves."""
     import rin
+
+    if form.endswith("-scratch"):
+        monkeypatch.setenv("RIN_K2_START", "0")
+        form = form[: -len("-scratch")]
 
     res, fs = W.config2()
     types, params = fs.arrays()
