@@ -157,12 +157,6 @@ const K1Table& k1_table() {
 // across that edge (then one exact predicate).  Every sign vector the
 // corner signs allow is enumerated once on the host, running the insertion
 // itself; a tet then needs at most four predicates and one table read.
-struct K2Pat {
-    uint32_t base;  // first slot of this first-plane pattern
-    uint8_t nv, valid, pad0, pad1;
-    uint8_t e[4][2];  // corners of the tet edge under vertex 4 + q
-};
-static_assert(sizeof(K2Pat) == 16, "k2 pattern layout");
 
 struct K2Table {
     K2Pat pat[81];
@@ -177,8 +171,7 @@ struct K2Table {
     std::vector<uint32_t> snap_off;
     std::vector<uint8_t> snap;
 };
-constexpr int kK2SnapHead = 16, kK2SnapFace = 16;
-constexpr uint32_t K2_NONE = 0xFFFFFFFFu;
+
 
 const K2Table& k2_table() {
     static K2Table T;
@@ -290,83 +283,6 @@ const K2Table& k2_table() {
         }
     });
     return T;
-}
-
-// The lane-group cuts of 3..8 functions start from the complex after the
-// tet's first two planes, the k = 2 snapshot of the tet's slot: two of their
-// insertions become a table read.
-struct K2Snap {
-    const K2Pat* pat;
-    const uint32_t* off;  // per slot: byte offset of the snapshot, or K2_NONE (null: start from the tet)
-    const uint8_t* bytes;
-};
-
-// the slot of a tet whose first two rows are P[0], P[1], as k_k2_lookup
-// computes it (corner signs and at most four exact edge predicates), or K2_NONE
-__device__ inline uint32_t k2_slot_of(const K2Pat* __restrict__ pats, const double (*P)[4]) {
-    int pa = 0, sB[4];
-    for (int c = 0, w = 1; c < 4; ++c, w *= 3) {
-        const double a = P[0][c], b = P[1][c];
-        pa += (a > 0 ? 2 : (a < 0 ? 0 : 1)) * w;
-        sB[c] = b > 0 ? 1 : (b < 0 ? -1 : 0);
-    }
-    const K2Pat K = pats[pa];
-    if (!K.valid) return K2_NONE;
-    uint32_t slot = K.base, w = 81;
-    for (int c = 0, w3 = 1; c < 4; ++c, w3 *= 3) slot += (uint32_t)(sB[c] + 1) * w3;
-    bool err = false;
-    for (int q = 0; q + 4 < K.nv; ++q, w *= 3) {
-        const int c1 = K.e[q][0], c2 = K.e[q][1];
-        int sg;
-        if (sB[c1] * sB[c2] < 0)
-            sg = dev::orient_ia_edge(P[0][c1], P[0][c2], P[1][c1], P[1][c2], err);
-        else
-            sg = sB[c1] == sB[c2] ? sB[c1] : (sB[c1] != 0 ? sB[c1] : sB[c2]);
-        slot += (uint32_t)(sg + 1) * w;
-    }
-    return err ? K2_NONE : slot;
-}
-
-// Loads the group's complex after the first two planes of its tet (rows in
-// X.P, visible to the group) and returns true, or returns false (no snapshot:
-// the cut starts from the tet).  Uniform over the group.
-template <class C, int G>
-__device__ bool k2_start(dev::WaveIA<C>& X, const K2Snap& S, int k) {
-    const int lane = dev::Grp<G>::lane();
-    if (!S.off || k < 3 || 4 + k > C::MP) return false;
-    const uint32_t slot = k2_slot_of(S.pat, X.P);
-    if (slot == K2_NONE) return false;
-    const uint32_t off = S.off[slot];
-    if (off == K2_NONE) return false;
-    const uint8_t* __restrict__ b = S.bytes + off;
-    const int nv = b[0], nf = b[1];
-    if (lane == 0) {
-        X.nv = nv;
-        X.nf = nf;
-        X.nc = b[2];
-        X.dup = b[3];
-        X.err = 0;
-    }
-    if (lane < C::MP) {
-        X.grp[lane] = lane < 6 ? b[4 + lane] : dev::NIL;
-        X.ori[lane] = lane < 6 ? b[10 + lane] : (uint8_t)1;
-    }
-    const uint8_t* __restrict__ vb = b + kK2SnapHead;
-    for (int v = lane; v < nv; v += G)
-        for (int q = 0; q < 3; ++q) X.vl[v][q] = vb[3 * v + q];
-    const uint8_t* __restrict__ fb = vb + 3 * nv;
-    for (int f = lane; f < nf; f += G) {
-        const uint8_t* __restrict__ r = fb + kK2SnapFace * f;
-        X.fa[f] = r[0];
-        X.fb[f] = r[1];
-        X.fc[f] = r[2];
-        X.fn[f] = r[3];
-        for (int q = 0; q < C::MFV; ++q) {
-            X.fv[f][q] = q < 6 ? r[4 + q] : (uint8_t)0;
-            X.fe[f][q] = q < 6 ? r[10 + q] : (uint8_t)0;
-        }
-    }
-    return true;
 }
 
 // Extraction plans of the k = 2 table records: what the record pass would
@@ -1166,6 +1082,8 @@ struct PlainTab {
     const K2Pat* pat;
     const uint32_t *off2, *cnt2;
     const uint16_t* w2;  // k = 2 records (off2 per slot; a record's word 6 is its length)
+    const uint32_t* snap_off;  // complexes after two planes (K2Table::snap_off, snap)
+    const uint8_t* snap;
 };
 
 __global__ void k_table_plain(GridView g, const double* __restrict__ vals, const uint32_t* __restrict__ act,
@@ -1697,9 +1615,18 @@ PlainTab plain_tables(hipStream_t s) {
     D.off2 = static_cast<const uint32_t*>(up(K2.off.data(), 4 * K2.off.size()));
     D.cnt2 = static_cast<const uint32_t*>(up(K2.cnt.data(), 4 * K2.cnt.size()));
     D.w2 = static_cast<const uint16_t*>(up(K2.words.data(), 2 * K2.words.size()));
+    D.snap_off = static_cast<const uint32_t*>(up(K2.snap_off.data(), 4 * K2.snap_off.size()));
+    D.snap = static_cast<const uint8_t*>(up(K2.snap.data(), K2.snap.size()));
     RIN_HIP(hipStreamSynchronize(s));
     per_device[dev] = D;
     return D;
+}
+
+void plain_k2_snapshots(hipStream_t s, const void** pat, const uint32_t** off, const uint8_t** bytes) {
+    const PlainTab T = plain_tables(s);
+    *pat = T.pat;
+    *off = T.snap_off;
+    *bytes = T.snap;
 }
 
 void table_cut_plain(const GridView& g, const double* vals, const uint32_t* act, const uint8_t* act_k,

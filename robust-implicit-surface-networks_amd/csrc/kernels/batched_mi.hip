@@ -379,11 +379,11 @@ void mi_path(MIWorkspace& W, const Grid& g, const Values& val, const MITablesDev
         hipLaunchKernelGGL((k_cut_group<CMed, 64, true>), dim3(g1), dim3(64), 0, s, G, vals,
                            (const uint32_t*)W.act.p, (const uint8_t*)W.act_k.p, (const uint32_t*)W.csr_off.p,
                            (const uint8_t*)W.fit.p, (const uint32_t*)bl[1], n1, (const uint32_t*)W.rec_off.p, W.rec.p,
-                           W.cnt.p, W.errs.p, 0);
+                           W.cnt.p, W.errs.p, 0, K2Snap{nullptr, nullptr, nullptr});
         hipLaunchKernelGGL((k_cut_group<CBig, 64, true>), dim3(g2), dim3(64), 0, s, G, vals,
                            (const uint32_t*)W.act.p, (const uint8_t*)W.act_k.p, (const uint32_t*)W.csr_off.p,
                            (const uint8_t*)W.fit.p, (const uint32_t*)bl[2], n2, (const uint32_t*)W.rec_off.p, W.rec.p,
-                           W.cnt.p, W.errs.p, 0);
+                           W.cnt.p, W.errs.p, 0, K2Snap{nullptr, nullptr, nullptr});
     }
     if (W.ws_huge.p)
         hipLaunchKernelGGL(k_cut_huge<true>, dim3(kHugeBlocks), dim3(64), 0, s, G, vals, (const uint32_t*)W.act.p,

@@ -394,6 +394,98 @@ __global__ void __launch_bounds__(64) k_cut_lds(GridView g, const double* __rest
 // of rec_off[a + 1] - rec_off[a] words.  A tet whose complex outgrows the
 // capacities is marked for the large-capacity retry (retry != 0) or sets the
 // run's capacity error.  The grid strides over the list.
+// k = 2 table, per first-plane sign pattern: the first slot of the pattern,
+// the first plane's complex's vertex count and, for each of its vertices on a
+// tet edge, that edge's corners.  A slot indexes the second plane's signs at
+// those vertices (base-3 digits, corners first).
+struct K2Pat {
+    uint32_t base;  // first slot of this first-plane pattern
+    uint8_t nv, valid, pad0, pad1;
+    uint8_t e[4][2];  // corners of the tet edge under vertex 4 + q
+};
+static_assert(sizeof(K2Pat) == 16, "k2 pattern layout");
+constexpr uint32_t K2_NONE = 0xFFFFFFFFu;
+// snapshot layout of the complex after two planes: a 16-byte head, 3 bytes
+// per vertex, 16 per face
+constexpr int kK2SnapHead = 16, kK2SnapFace = 16;
+
+// The lane-group cuts of 3..8 functions start from the complex after the
+// tet's first two planes, the k = 2 snapshot of the tet's slot: two of their
+// insertions become a table read.
+struct K2Snap {
+    const K2Pat* pat;
+    const uint32_t* off;  // per slot: byte offset of the snapshot, or K2_NONE (null: start from the tet)
+    const uint8_t* bytes;
+};
+
+// the slot of a tet whose first two rows are P[0], P[1], as k_k2_lookup
+// computes it (corner signs and at most four exact edge predicates), or K2_NONE
+__host__ __device__ inline uint32_t k2_slot_of(const K2Pat* __restrict__ pats, const double (*P)[4]) {
+    int pa = 0, sB[4];
+    for (int c = 0, w = 1; c < 4; ++c, w *= 3) {
+        const double a = P[0][c], b = P[1][c];
+        pa += (a > 0 ? 2 : (a < 0 ? 0 : 1)) * w;
+        sB[c] = b > 0 ? 1 : (b < 0 ? -1 : 0);
+    }
+    const K2Pat K = pats[pa];
+    if (!K.valid) return K2_NONE;
+    uint32_t slot = K.base, w = 81;
+    for (int c = 0, w3 = 1; c < 4; ++c, w3 *= 3) slot += (uint32_t)(sB[c] + 1) * w3;
+    bool err = false;
+    for (int q = 0; q + 4 < K.nv; ++q, w *= 3) {
+        const int c1 = K.e[q][0], c2 = K.e[q][1];
+        int sg;
+        if (sB[c1] * sB[c2] < 0)
+            sg = dev::orient_ia_edge(P[0][c1], P[0][c2], P[1][c1], P[1][c2], err);
+        else
+            sg = sB[c1] == sB[c2] ? sB[c1] : (sB[c1] != 0 ? sB[c1] : sB[c2]);
+        slot += (uint32_t)(sg + 1) * w;
+    }
+    return err ? K2_NONE : slot;
+}
+
+// Loads the group's complex after the first two planes of its tet (rows in
+// X.P, visible to the group) and returns true, or returns false (no snapshot:
+// the cut starts from the tet).  Uniform over the group.
+template <class C, int G>
+__device__ bool k2_start(dev::WaveIA<C>& X, const K2Snap& S, int k) {
+    const int lane = dev::Grp<G>::lane();
+    if (!S.off || k < 3 || 4 + k > C::MP) return false;
+    const uint32_t slot = k2_slot_of(S.pat, X.P);
+    if (slot == K2_NONE) return false;
+    const uint32_t off = S.off[slot];
+    if (off == K2_NONE) return false;
+    const uint8_t* __restrict__ b = S.bytes + off;
+    const int nv = b[0], nf = b[1];
+    if (lane == 0) {
+        X.nv = nv;
+        X.nf = nf;
+        X.nc = b[2];
+        X.dup = b[3];
+        X.err = 0;
+    }
+    if (lane < C::MP) {
+        X.grp[lane] = lane < 6 ? b[4 + lane] : dev::NIL;
+        X.ori[lane] = lane < 6 ? b[10 + lane] : (uint8_t)1;
+    }
+    const uint8_t* __restrict__ vb = b + kK2SnapHead;
+    for (int v = lane; v < nv; v += G)
+        for (int q = 0; q < 3; ++q) X.vl[v][q] = vb[3 * v + q];
+    const uint8_t* __restrict__ fb = vb + 3 * nv;
+    for (int f = lane; f < nf; f += G) {
+        const uint8_t* __restrict__ r = fb + kK2SnapFace * f;
+        X.fa[f] = r[0];
+        X.fb[f] = r[1];
+        X.fc[f] = r[2];
+        X.fn[f] = r[3];
+        for (int q = 0; q < C::MFV; ++q) {
+            X.fv[f][q] = q < 6 ? r[4 + q] : (uint8_t)0;
+            X.fe[f][q] = q < 6 ? r[10 + q] : (uint8_t)0;
+        }
+    }
+    return true;
+}
+
 template <class C, int G, bool MI>
 __global__ void __launch_bounds__(64) k_cut_group(GridView g, const double* __restrict__ vals,
                                                   const uint32_t* __restrict__ act, const uint8_t* __restrict__ act_k,
@@ -401,7 +493,7 @@ __global__ void __launch_bounds__(64) k_cut_group(GridView g, const double* __re
                                                   const uint8_t* __restrict__ fit, const uint32_t* __restrict__ list,
                                                   Count nlist, const uint32_t* __restrict__ rec_off,
                                                   uint16_t* __restrict__ rec, uint32_t* __restrict__ cnt,
-                                                  uint32_t* __restrict__ errs, int retry) {
+                                                  uint32_t* __restrict__ errs, int retry, K2Snap k2s) {
     constexpr int NG = 64 / G;
     using Cx = typename std::conditional<MI, dev::WaveMI<C>, dev::WaveIA<C>>::type;
     __shared__ Cx Xs[NG];
@@ -420,10 +512,12 @@ __global__ void __launch_bounds__(64) k_cut_group(GridView g, const double* __re
             X.P[lane >> 2][lane & 3] = vals[(uint64_t)f * V + v[lane & 3]];
         }
         __syncthreads();
-        if constexpr (MI)
+        if constexpr (MI) {
             dev::wave_mi_cut<C, G>(X, k);
-        else
-            dev::wave_ia_cut<C, G>(X, k);
+        } else {
+            const int start = k2_start<C, G>(X, k2s, k) ? 2 : 0;
+            dev::wave_ia_cut<C, G>(X, k, start);
+        }
         uint16_t* out = rec + rec_off[a];
         const int cap = (int)(rec_off[a + 1] - rec_off[a]);
         uint32_t a0 = 0, a1 = 0, a2 = 0;

@@ -199,6 +199,9 @@ float debug_copy(DevMem& mem, uint64_t bytes, int reps, hipStream_t s);
 void table_cut_plain(const GridView& g, const double* vals, const uint32_t* act, const uint8_t* act_k,
                      const uint32_t* csr_off, const uint8_t* fit, const uint32_t* list, uint64_t n,
                      const uint32_t* rec_off, uint16_t* rec, uint32_t* cnt, uint32_t* errs, hipStream_t s);
+// the k = 2 table's patterns and complexes after two planes on this device
+// (the plain form's lane-group cuts start from them; batched.hip)
+void plain_k2_snapshots(hipStream_t s, const void** pat, const uint32_t** off, const uint8_t** bytes);
 // the device scans (scan_u32; scan3_u32 over three fields) on host data
 void debug_scan(DevMem& mem, int fields, int stride, const uint32_t* in, uint64_t n, uint64_t count, uint32_t* out,
                 uint32_t* totals, hipStream_t s);

@@ -640,6 +640,15 @@ void run_pipeline(DevMem& mem, const Grid& g, const Values& val, uint32_t flags,
     // the cuts beyond the tables: lane groups with the complex in LDS (the
     // -R check's counts and reversed order take the thread cut)
     const bool groups = !(flags & (RUN_REVERSE | RUN_CUT_ONLY));
+    // the implicit-arrangement lane groups start from the k = 2 table's
+    // complex after two planes (RIN_K2_START=0: they insert every plane)
+    K2Snap k2s{nullptr, nullptr, nullptr};
+    const char* ks = std::getenv("RIN_K2_START");
+    if (!MI && groups && !(ks && !std::strcmp(ks, "0"))) {
+        const void* pat = nullptr;
+        plain_k2_snapshots(s, &pat, &k2s.off, &k2s.bytes);
+        k2s.pat = static_cast<const K2Pat*>(pat);
+    }
     for (int q = 0; q < 2 && NA && groups; ++q) {
         const uint64_t nl = bcnt[1 + q];
         if (!nl) continue;
@@ -649,7 +658,7 @@ void run_pipeline(DevMem& mem, const Grid& g, const Values& val, uint32_t flags,
                                (const uint32_t*)out.act.p, (const uint8_t*)out.act_k.p,
                                (const uint32_t*)out.csr_off.p, (const uint8_t*)out.fit.p,
                                (const uint32_t*)blist[1 + q], Count(nl), (const uint32_t*)out.rec_off.p, out.rec.p,
-                               cnt.p, errs.p, 1);
+                               cnt.p, errs.p, 1, k2s);
         };
         if (q == 0)
             MI ? go(k_cut_group<CMed, 64, true>, 1) : go(k_cut_group<CMed, 16, false>, 4);
