@@ -64,10 +64,10 @@ def cut_bytes(bucket):
 # single-kernel stages with an algorithmic byte count:
 # stage label prefix -> {form marker in the label: (kernel, file tag, bytes(sizes))}
 KERNEL_STAGES = {
-    "cut k=3..4": {"lane group of 16 per tet": ("k_cut_wave<Caps<40,64,8,8>, 16>", "k_cut_wave16_k3to4", cut_bytes(0)),
-                   "wave per tet": ("k_cut_wave<Caps<40,64,8,8>, 32>", "k_cut_wave_k3to4", cut_bytes(0)),
+    "cut k=3..4": {"lane group of 16 per tet": ("k_cut_wave<Caps<40,64,8,8>, 16, false>", "k_cut_wave16_k3to4", cut_bytes(0)),
+                   "wave per tet": ("k_cut_wave<Caps<40,64,8,8>, 32, false>", "k_cut_wave_k3to4", cut_bytes(0)),
                    "thread per tet": ("k_cut_bucket<Caps<40,64,8,8>>", "k_cut_bucket_k3to4", cut_bytes(0))},
-    "cut k=5..8": {"wave per tet": ("k_cut_wave<Caps<132,284,10,12>, 64>", "k_cut_wave_k5to8", cut_bytes(1)),
+    "cut k=5..8": {"wave per tet": ("k_cut_wave<Caps<132,284,10,12>, 64, false>", "k_cut_wave_k5to8", cut_bytes(1)),
                    "thread per tet": ("k_cut_bucket<Caps<132,284,10,12>>", "k_cut_bucket_k5to8", cut_bytes(1))},
 }
 
@@ -257,7 +257,7 @@ def traffic_for(tag):
 
 def _short(name):
     """Kernel name without namespaces, argument list and cap types' extra
-    parameters: 'k_cut_wave<Caps<40,64,8,8>, 32>'."""
+    parameters: 'k_cut_wave<Caps<40,64,8,8>, 32, false>'."""
     name = re.sub(r"rin::\(anonymous namespace\)::", "", name)
     name = re.sub(r"^void ", "", name.strip())
     name = re.sub(r"rin::dev::Caps<(\d+), (\d+), (\d+), (\d+)[^>]*>", r"Caps<\1,\2,\3,\4>", name)
