@@ -517,7 +517,9 @@ __global__ void k_signs_count(const double* __restrict__ vals, uint64_t V, uint3
 // places its tets in the CSR, the active list and the bucket lists from its
 // tile's prefixes, in tet order and without atomics.
 
-constexpr int kCubeTile = 512;  // cubes per tile (one workgroup)
+// cubes per tile (one workgroup): 256, 512 or 1024, picked per plan by the
+// autotuner (the results do not depend on it)
+constexpr int kCubeTile = 512;  // the default
 
 // pat (optional): per tet with one active function, its corner sign pattern
 // (sum of s_c 3^c, s_c = 0 negative, 1 zero, 2 positive), else 0
@@ -579,14 +581,15 @@ struct EarlyEntry {
 // early3, early5 (nullable): lists of the k = 3..4 and 5..8 tets, at most
 // early_cap3 / early_cap5 entries, counted in ctr[C_E3] / ctr[C_E5] (zero at
 // the start of a run).  A tile's entries are contiguous and in tet order.
-__global__ void __launch_bounds__(kCubeTile) k_cube_count(GridView g, const uint64_t* __restrict__ pos,
+template <int TILE>
+__global__ void __launch_bounds__(TILE) k_cube_count(GridView g, const uint64_t* __restrict__ pos,
                                                           const uint64_t* __restrict__ neg, uint64_t full,
                                                           TileSums* __restrict__ tile_sums, EarlyEntry* early3,
                                                           EarlyEntry* early5, uint32_t early_cap3,
                                                           uint32_t early_cap5, uint32_t* ctr) {
     const uint32_t ncubes = (uint32_t)(g.T / 5);
     const uint32_t tile = blockIdx.x;
-    const uint32_t cube = tile * kCubeTile + threadIdx.x;
+    const uint32_t cube = tile * TILE + threadIdx.x;
     uint32_t v[kTileFields] = {0, 0, 0, 0, 0, 0, 0, 0};
     uint64_t m[5] = {0, 0, 0, 0, 0};
     if (cube < ncubes) {
@@ -607,7 +610,7 @@ __global__ void __launch_bounds__(kCubeTile) k_cube_count(GridView g, const uint
     // atomic add per list and tile reserves the tile's entries
     const uint32_t want = (early3 ? v[TF_B3] : 0u) | (early5 ? v[TF_B5] << 16 : 0u);
     if ((early3 || early5) && __syncthreads_or(want != 0)) {
-        __shared__ uint32_t esum[kCubeTile / 64], ebk[kCubeTile / 64], ebase[2];
+        __shared__ uint32_t esum[TILE / 64], ebk[TILE / 64], ebase[2];
         constexpr int kAsBits = 13;
         const uint32_t packed = v[TF_FIT] << kAsBits | v[TF_ACTIVE];
         uint32_t inc = packed, binc = want;
@@ -625,7 +628,7 @@ __global__ void __launch_bounds__(kCubeTile) k_cube_count(GridView g, const uint
         __syncthreads();
         if (threadIdx.x == 0) {
             uint32_t tot = 0;
-            for (int w = 0; w < kCubeTile / 64; ++w) tot += ebk[w];
+            for (int w = 0; w < TILE / 64; ++w) tot += ebk[w];
             ebase[0] = (tot & 0xffffu) ? atomicAdd(ctr + C_E3, tot & 0xffffu) : 0u;
             ebase[1] = (tot >> 16) ? atomicAdd(ctr + C_E5, tot >> 16) : 0u;
         }
@@ -653,7 +656,7 @@ __global__ void __launch_bounds__(kCubeTile) k_cube_count(GridView g, const uint
             lr += k != 0;
         }
     }
-    __shared__ uint32_t red[kCubeTile / 64][kTileFields];
+    __shared__ uint32_t red[TILE / 64][kTileFields];
     for (int x = 0; x < kTileFields - 1; ++x) {
         for (int d = 32; d > 0; d >>= 1) v[x] += __shfl_xor(v[x], d, 64);
         if (lane == 0) red[wave][x] = v[x];
@@ -662,7 +665,7 @@ __global__ void __launch_bounds__(kCubeTile) k_cube_count(GridView g, const uint
     if (threadIdx.x < kTileFields) {
         uint32_t t = 0;
         if (threadIdx.x < kTileFields - 1)
-            for (int w = 0; w < kCubeTile / 64; ++w) t += red[w][threadIdx.x];
+            for (int w = 0; w < TILE / 64; ++w) t += red[w][threadIdx.x];
         tile_sums[tile].v[threadIdx.x] = t;
     }
 }
@@ -728,17 +731,18 @@ __global__ void __launch_bounds__(kTileScanThreads) k_tile_scan(TileSums* __rest
 }
 
 // tile_sums: exclusive prefixes (k_tile_scan)
-__global__ void __launch_bounds__(kCubeTile) k_cube_fill(FillArgs A, const TileSums* __restrict__ tile_sums) {
+template <int TILE>
+__global__ void __launch_bounds__(TILE) k_cube_fill(FillArgs A, const TileSums* __restrict__ tile_sums) {
     __shared__ K1Entry tab[81];
-    __shared__ uint32_t wsum[kCubeTile / 64];
-    __shared__ uint32_t wbk[kCubeTile / 64];  // per-wave bucket counts, 10 bits per list
-    __shared__ uint32_t s_off[5 * kCubeTile], s_rank[5 * kCubeTile];  // per-tet CSR entries of the tile
+    __shared__ uint32_t wsum[TILE / 64];
+    __shared__ uint32_t wbk[TILE / 64];  // per-wave bucket counts, 10 bits per list
+    __shared__ uint32_t s_off[5 * TILE], s_rank[5 * TILE];  // per-tet CSR entries of the tile
     for (int i = threadIdx.x; i < 81; i += blockDim.x) tab[i] = A.k1[i];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint32_t tile = blockIdx.x;
     const TileSums pre = tile_sums[tile];
     const uint32_t ncubes = (uint32_t)(A.g.T / 5);
-    const uint32_t cube = tile * kCubeTile + threadIdx.x;
+    const uint32_t cube = tile * TILE + threadIdx.x;
     uint32_t c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     uint64_t m[5] = {0, 0, 0, 0, 0};
     bool even = true;
@@ -762,9 +766,9 @@ __global__ void __launch_bounds__(kCubeTile) k_cube_fill(FillArgs A, const TileS
     }
     if (lane == 63) wbk[wave] = binc;
     // block exclusive scan of (ks, as) packed as ks << kAsBits | as (a tile
-    // holds at most 5 kCubeTile active tets and 64 times as many functions)
+    // holds at most 5 TILE active tets and 64 times as many functions)
     constexpr int kAsBits = 13;
-    static_assert(5 * kCubeTile < (1 << kAsBits) && 5 * 64 * kCubeTile < (1 << (32 - kAsBits)), "tile packing");
+    static_assert(5 * TILE < (1 << kAsBits) && 5 * 64 * TILE < (1 << (32 - kAsBits)), "tile packing");
     const uint32_t packed = ks << kAsBits | as;
     uint32_t inc = packed;
     for (int d = 1; d < 64; d <<= 1) {
@@ -825,8 +829,8 @@ __global__ void __launch_bounds__(kCubeTile) k_cube_fill(FillArgs A, const TileS
     }
     // the tile's per-tet CSR entries, coalesced
     __syncthreads();
-    const uint64_t t0 = (uint64_t)tile * (5 * kCubeTile), T = A.g.T;
-    for (int i = threadIdx.x; i < 5 * kCubeTile; i += kCubeTile)
+    const uint64_t t0 = (uint64_t)tile * (5 * TILE), T = A.g.T;
+    for (int i = threadIdx.x; i < 5 * TILE; i += TILE)
         if (t0 + i < T) {
             A.csr_off_w[t0 + i] = s_off[i];
             A.rank_w[t0 + i] = s_rank[i];
@@ -1658,6 +1662,7 @@ struct Workspace {
     DBuf<uint64_t> lts[3];  // per bucket slot: CSR offset << 32 | tet
     DBuf<uint32_t> act_off;  // CSR offset of each active tet, by active rank
     DBuf<TileSums> tile_sums;  // regular grids: per tile of cubes
+    int cube_tile = kCubeTile;  // cubes per tile of the count and fill passes
     DBuf<EarlyEntry> early[3];  // regular grids: the k = 3..4 and 5..8 tets ([1], [2]), listed by the count pass
     DBuf<uint8_t> fit, act_k;
     DBuf<uint16_t> rec;
@@ -1752,6 +1757,7 @@ struct Plan {
     std::mutex mu;
     std::vector<std::shared_ptr<Workspace>> ws;
     int blocks_per_cu[3] = {0, 0, 0};  // autotuned grid of each bucket's cut kernel
+    int cube_tile = 0;                  // autotuned cube tile of the count and fill passes (0: not yet)
     uint32_t hint_b[5] = {0, 0, 0, 0, 0};
 };
 
@@ -1834,9 +1840,9 @@ const StarTables& star_tables() {
     return T;
 }
 
-unsigned cube_tiles(const Grid& g) {
+unsigned cube_tiles(const Grid& g, int tile = kCubeTile) {
     const uint64_t cubes = g.view.T / 5;
-    return (unsigned)std::max<uint64_t>(1, (cubes + kCubeTile - 1) / kCubeTile);
+    return (unsigned)std::max<uint64_t>(1, (cubes + tile - 1) / tile);
 }
 
 void alloc_ws(Workspace& W, const Grid& g, const Caps_& c, hipStream_t s) {
@@ -1846,7 +1852,7 @@ void alloc_ws(Workspace& W, const Grid& g, const Caps_& c, hipStream_t s) {
     W.pos.alloc(m, V);
     W.neg.alloc(m, V);
     if (g.view.regular) {
-        W.tile_sums.alloc(m, cube_tiles(g) + 1);
+        W.tile_sums.alloc(m, cube_tiles(g, 256) + 1);  // the smallest tile has the most tiles
         for (int q = 1; q < 3; ++q) W.early[q].alloc(m, c.b[q] + 1);
     } else {
         W.kcnt.alloc(m, T + 1);
@@ -1963,10 +1969,20 @@ void stage_count(Workspace& W, const Grid& g, const Values& val, hipStream_t s, 
                        W.neg.p, W.ctr.p);
     const uint64_t full = (F == 64) ? ~0ull : ((1ull << F) - 1);
     if (g.view.regular) {
-        hipLaunchKernelGGL(k_cube_count, dim3(cube_tiles(g)), dim3(kCubeTile), 0, s, g.view, (const uint64_t*)W.pos.p,
-                           (const uint64_t*)W.neg.p, full, W.tile_sums.p, early3 ? W.early[1].p : nullptr,
-                           early5 ? W.early[2].p : nullptr, W.cap.b[1], W.cap.b[2], W.ctr.p);
-        hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(kTileScanThreads), 0, s, W.tile_sums.p, cube_tiles(g), W.ctr.p);
+        const int tile = W.cube_tile;
+        auto count = [&](auto kernel) {
+            hipLaunchKernelGGL(kernel, dim3(cube_tiles(g, tile)), dim3(tile), 0, s, g.view, (const uint64_t*)W.pos.p,
+                               (const uint64_t*)W.neg.p, full, W.tile_sums.p, early3 ? W.early[1].p : nullptr,
+                               early5 ? W.early[2].p : nullptr, W.cap.b[1], W.cap.b[2], W.ctr.p);
+        };
+        if (tile == 256)
+            count(k_cube_count<256>);
+        else if (tile == 1024)
+            count(k_cube_count<1024>);
+        else
+            count(k_cube_count<512>);
+        hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(kTileScanThreads), 0, s, W.tile_sums.p, cube_tiles(g, tile),
+                           W.ctr.p);
         return;
     }
     hipLaunchKernelGGL(k_filter_count, dim3(nblk(T)), dim3(256), 0, s, g.view, (const uint64_t*)W.pos.p,
@@ -2119,8 +2135,14 @@ void stage_fill(Workspace& W, const Grid& g, const Values& val, hipStream_t s) {
     A.csr_off_w = W.csr_off.p;
     A.rank_w = W.rank.p;
     if (g.view.regular) {
-        hipLaunchKernelGGL(k_cube_fill, dim3(cube_tiles(g)), dim3(kCubeTile), 0, s, A,
-                           (const TileSums*)W.tile_sums.p);
+        const int tile = W.cube_tile;
+        const TileSums* ts = W.tile_sums.p;
+        if (tile == 256)
+            hipLaunchKernelGGL(k_cube_fill<256>, dim3(cube_tiles(g, tile)), dim3(tile), 0, s, A, ts);
+        else if (tile == 1024)
+            hipLaunchKernelGGL(k_cube_fill<1024>, dim3(cube_tiles(g, tile)), dim3(tile), 0, s, A, ts);
+        else
+            hipLaunchKernelGGL(k_cube_fill<512>, dim3(cube_tiles(g, tile)), dim3(tile), 0, s, A, ts);
         return;
     }
     hipLaunchKernelGGL(k_fill_batched, dim3(nblk(T)), dim3(256), 0, s, A);
@@ -2613,6 +2635,34 @@ void run_batched(DevMem& ctx_mem, const Grid& g, const Values& val, uint32_t fla
                 P.blocks_per_cu[0] = -2;
                 P.blocks_per_cu[1] = P.blocks_per_cu[2] = -1;
             }
+            // autotune the cube tile of the count and fill passes once per plan
+            // (regular grids): every tile size gives the same lists, so the
+            // last pass timed leaves them as they must be
+            if (P.cube_tile == 0) {
+                P.cube_tile = kCubeTile;
+                // RIN_CUBE_TILE=256|512|1024 takes that tile without timing
+                const char* ct = std::getenv("RIN_CUBE_TILE");
+                const int forced = ct ? std::atoi(ct) : 0;
+                if (forced == 256 || forced == 512 || forced == 1024)
+                    P.cube_tile = forced;
+                else if (g.view.regular && !(flags & RUN_FIXED_FORMS)) {
+                    float best = 1e30f;
+                    for (int tile : {256, 512, 1024}) {
+                        w.cube_tile = tile;
+                        float ms = 0;
+                        for (int rep = 0; rep < 2; ++rep)  // the second pass, caches warm
+                            ms = time_ms(s, [&] {
+                                stage_count(w, g, val, s);
+                                stage_fill(w, g, val, s);
+                            });
+                        if (ms < best) {
+                            best = ms;
+                            P.cube_tile = tile;
+                        }
+                    }
+                }
+            }
+            w.cube_tile = P.cube_tile;
             // autotune the bucket cut grids once per plan
             if (P.blocks_per_cu[0] == 0) {
                 RIN_HIP(hipStreamSynchronize(s));

@@ -337,6 +337,29 @@ def test_large_complexes_are_cut_again(ctx, mi, batched):
     g.release()
 
 
+@pytest.mark.parametrize("tile", ["256", "512", "1024"])
+@pytest.mark.parametrize("which", ["config1", "config4"])
+def test_batched_cube_tiles(ctx, which, tile, monkeypatch):
+    """Every cube tile the autotuner can pick for the count and fill passes
+    (256, 512 or 1024 cubes a workgroup) gives the oracle's output bit for bit,
+    on the capturing run and on a replay."""
+    import rin
+
+    monkeypatch.setenv("RIN_CUBE_TILE", tile)
+    res, fs = W.config1() if which == "config1" else W.config4()
+    types, params = fs.arrays()
+    g = ctx.grid_regular(res)
+    vals = g.eval(types, params)
+    pts, tets = S.grid(res)
+    ref = S.OracleRun(0, pts, tets, S.evaluate(types, params, pts))
+    for rep in range(2):
+        dev = vals.run(rin.FLAG_BATCHED)
+        _compare(dev, ref)
+        dev.release()
+    vals.release()
+    g.release()
+
+
 @pytest.mark.parametrize("form", ["batched", "plain", "batched-scratch"])
 def test_headline_workload_full_size(ctx, form, monkeypatch):
     """The benchmark workload itself: 101^3 vertices (5,000,000 tets), 20
