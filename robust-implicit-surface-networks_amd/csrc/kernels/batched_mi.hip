@@ -373,17 +373,21 @@ void mi_path(MIWorkspace& W, const Grid& g, const Values& val, const MITablesDev
                            W.cnt.p, W.errs.p, (const uint32_t*)bl[2]);
     } else {
         // a wavefront per tet: these lists are short and each tet's latency
-        // is the kernel's time
+        // is the kernel's time; the cuts start from the 3-material table's
+        // complex after three materials (RIN_MI3_START=0: from the tet)
+        const char* ms = std::getenv("RIN_MI3_START");
+        const MI3Snap m3s = (ms && !std::strcmp(ms, "0")) ? MI3Snap{nullptr, nullptr, nullptr}
+                                                          : MI3Snap{TT.pat3, TT.snap_off3, TT.snap3};
         const unsigned g1 = (unsigned)std::min<uint64_t>(c.active + 1, 2048);
         const unsigned g2 = (unsigned)std::min<uint64_t>(c.active + 1, 1024);
         hipLaunchKernelGGL((k_cut_group<CMed, 64, true>), dim3(g1), dim3(64), 0, s, G, vals,
                            (const uint32_t*)W.act.p, (const uint8_t*)W.act_k.p, (const uint32_t*)W.csr_off.p,
                            (const uint8_t*)W.fit.p, (const uint32_t*)bl[1], n1, (const uint32_t*)W.rec_off.p, W.rec.p,
-                           W.cnt.p, W.errs.p, 0, K2Snap{nullptr, nullptr, nullptr});
+                           W.cnt.p, W.errs.p, 0, m3s);
         hipLaunchKernelGGL((k_cut_group<CBig, 64, true>), dim3(g2), dim3(64), 0, s, G, vals,
                            (const uint32_t*)W.act.p, (const uint8_t*)W.act_k.p, (const uint32_t*)W.csr_off.p,
                            (const uint8_t*)W.fit.p, (const uint32_t*)bl[2], n2, (const uint32_t*)W.rec_off.p, W.rec.p,
-                           W.cnt.p, W.errs.p, 0, K2Snap{nullptr, nullptr, nullptr});
+                           W.cnt.p, W.errs.p, 0, m3s);
     }
     if (W.ws_huge.p)
         hipLaunchKernelGGL(k_cut_huge<true>, dim3(kHugeBlocks), dim3(64), 0, s, G, vals, (const uint32_t*)W.act.p,

@@ -131,14 +131,16 @@ __device__ inline int wave_chain(const uint8_t* ea, const uint8_t* eb, const uin
 // Runs the insertion of materials 1..k-1 (X.P[0..k-1] loaded by the caller)
 // into the tet filled with material 0 (label 4); on return X.err is set when
 // a capacity or the predicates' exponent range was exceeded.
+// start: materials already in the complex (1, or 3 when the caller has
+// loaded the complex after the first three, which depends on their signs only)
 template <class C, int G = 64>
-__device__ void wave_mi_cut(WaveMI<C>& X, int k) {
+__device__ void wave_mi_cut(WaveMI<C>& X, int k, int start = 1) {
     using W = Grp<G>;
     constexpr int MFV = C::MFV, MV = C::MV, MF = C::MF, MP = C::MP;
     constexpr int CAND = WaveMI<C>::CAND, HT = WaveMI<C>::HT, PE = WaveMI<C>::PE, TE = WaveMI<C>::TE;
     const int lane = W::lane();
     // the tet filled with material 4 (mi_init)
-    if (lane < 4) {
+    if (start == 1 && lane < 4) {
         const int i = lane;
         int n = 0;
         for (int q = 0; q < 4; ++q)
@@ -153,8 +155,8 @@ __device__ void wave_mi_cut(WaveMI<C>& X, int k) {
             X.fe[i][q] = cyc[i][(q + 2) % 3];
         }
     }
-    if (lane < MP) X.grp[lane] = lane < 4 ? (uint8_t)lane : (lane == 4 ? (uint8_t)4 : NIL);
-    if (lane == 0) {
+    if (start == 1 && lane < MP) X.grp[lane] = lane < 4 ? (uint8_t)lane : (lane == 4 ? (uint8_t)4 : NIL);
+    if (start == 1 && lane == 0) {
         X.nv = 4;
         X.nf = 4;
         X.nc = 1;
@@ -168,7 +170,7 @@ __device__ void wave_mi_cut(WaveMI<C>& X, int k) {
         if (lane == 0) X.err = 1;
         __syncthreads();
     };
-    for (int j = 1; j < k; ++j) {
+    for (int j = start; j < k; ++j) {
         const uint8_t m = (uint8_t)(4 + j);
         const int nv0 = X.nv, nf0 = X.nf, nc0 = X.nc;
 

@@ -486,6 +486,86 @@ __device__ bool k2_start(dev::WaveIA<C>& X, const K2Snap& S, int k) {
     return true;
 }
 
+// 3 materials a < b < c (the material-interface table): per 2-material
+// pattern, the first slot of its sign vectors and its vertices' labels
+struct MI3Pat {
+    uint32_t base;
+    uint8_t nv, valid, pad0, pad1;
+    uint8_t lab[8][4];
+};
+static_assert(sizeof(MI3Pat) == 40, "3-material pattern layout");
+constexpr uint32_t MI3_NONE = 0xFFFFFFFFu;
+// snapshot layout of the complex after three materials: a 20-byte head
+// (nv, nf, nc, dup, grp[8], cells[8]), 4 bytes per vertex, 20 per face
+// (fa, fb, fn, 0, fv[8], fe[8])
+constexpr int kMI3SnapHead = 20, kMI3SnapFace = 20;
+
+// The material-interface lane-group cut of 4..8 materials starts from the
+// complex after the tet's first three, the 3-material table's snapshot of the
+// tet's slot: three insertions become a table read.
+struct MI3Snap {
+    const MI3Pat* pat;
+    const uint32_t* off;  // per slot: byte offset of the snapshot, or MI3_NONE (null: start from the tet)
+    const uint8_t* bytes;
+};
+
+// the 3-material slot of a tet whose first three rows are P[0..2], as
+// k_mi_table computes it, or MI3_NONE
+__device__ inline uint32_t mi3_slot_of(const MI3Pat* __restrict__ pats, const double (*P)[4]) {
+    int pat = 0;
+    for (int c = 0, w = 1; c < 4; ++c, w *= 3) pat += (P[1][c] > P[0][c] ? 2 : (P[1][c] < P[0][c] ? 0 : 1)) * w;
+    const MI3Pat& K = pats[pat];
+    if (!K.valid) return MI3_NONE;
+    bool err = false;
+    uint32_t slot = K.base, w = 1;
+    for (int q = 0; q < K.nv && !err; ++q, w *= 3) {
+        const int lab[4] = {K.lab[q][0], K.lab[q][1], K.lab[q][2], K.lab[q][3]};
+        slot += (uint32_t)(dev::orient_mi(P, lab, P[2], err) + 1) * w;
+    }
+    return err ? MI3_NONE : slot;
+}
+
+// Loads the group's complex after the first three materials of its tet (rows
+// in X.P, visible to the group) and returns true, or returns false (the cut
+// starts from the tet).  Uniform over the group.
+template <class C, int G>
+__device__ bool mi3_start(dev::WaveMI<C>& X, const MI3Snap& S, int k) {
+    const int lane = dev::Grp<G>::lane();
+    if (!S.off || k < 4 || 4 + k > C::MP) return false;
+    const uint32_t slot = mi3_slot_of(S.pat, X.P);
+    if (slot == MI3_NONE) return false;
+    const uint32_t off = S.off[slot];
+    if (off == MI3_NONE) return false;
+    const uint8_t* __restrict__ b = S.bytes + off;
+    const int nv = b[0], nf = b[1];
+    if (lane == 0) {
+        X.nv = nv;
+        X.nf = nf;
+        X.nc = b[2];
+        X.dup = b[3];
+        X.err = 0;
+    }
+    if (lane < C::MP) {
+        X.grp[lane] = lane < 8 ? b[4 + lane] : dev::NIL;
+        X.cells[lane] = lane < 8 ? b[12 + lane] : dev::NIL;
+    }
+    const uint8_t* __restrict__ vb = b + kMI3SnapHead;
+    for (int v = lane; v < nv; v += G)
+        for (int q = 0; q < 4; ++q) X.vl[v][q] = vb[4 * v + q];
+    const uint8_t* __restrict__ fb = vb + 4 * nv;
+    for (int f = lane; f < nf; f += G) {
+        const uint8_t* __restrict__ r = fb + kMI3SnapFace * f;
+        X.fa[f] = r[0];
+        X.fb[f] = r[1];
+        X.fn[f] = r[2];
+        for (int q = 0; q < C::MFV; ++q) {
+            X.fv[f][q] = q < 8 ? r[4 + q] : (uint8_t)0;
+            X.fe[f][q] = q < 8 ? r[12 + q] : (uint8_t)0;
+        }
+    }
+    return true;
+}
+
 template <class C, int G, bool MI>
 __global__ void __launch_bounds__(64) k_cut_group(GridView g, const double* __restrict__ vals,
                                                   const uint32_t* __restrict__ act, const uint8_t* __restrict__ act_k,
@@ -493,7 +573,8 @@ __global__ void __launch_bounds__(64) k_cut_group(GridView g, const double* __re
                                                   const uint8_t* __restrict__ fit, const uint32_t* __restrict__ list,
                                                   Count nlist, const uint32_t* __restrict__ rec_off,
                                                   uint16_t* __restrict__ rec, uint32_t* __restrict__ cnt,
-                                                  uint32_t* __restrict__ errs, int retry, K2Snap k2s) {
+                                                  uint32_t* __restrict__ errs, int retry,
+                                                  typename std::conditional<MI, MI3Snap, K2Snap>::type snap) {
     constexpr int NG = 64 / G;
     using Cx = typename std::conditional<MI, dev::WaveMI<C>, dev::WaveIA<C>>::type;
     __shared__ Cx Xs[NG];
@@ -513,9 +594,10 @@ __global__ void __launch_bounds__(64) k_cut_group(GridView g, const double* __re
         }
         __syncthreads();
         if constexpr (MI) {
-            dev::wave_mi_cut<C, G>(X, k);
+            const int start = mi3_start<C, G>(X, snap, k) ? 3 : 1;
+            dev::wave_mi_cut<C, G>(X, k, start);
         } else {
-            const int start = k2_start<C, G>(X, k2s, k) ? 2 : 0;
+            const int start = k2_start<C, G>(X, snap, k) ? 2 : 0;
             dev::wave_ia_cut<C, G>(X, k, start);
         }
         uint16_t* out = rec + rec_off[a];

@@ -81,13 +81,7 @@ struct MI2Entry {
 // exact comparison, at a vertex where a and b tie on a tet edge one exact
 // 3 x 3 predicate.  Every sign vector is cut once on the host.  Per pattern:
 // the first slot of its sign vectors and the labels of its vertices.
-struct MI3Pat {
-    uint32_t base;
-    uint8_t nv, valid, pad0, pad1;
-    uint8_t lab[8][4];
-};
-static_assert(sizeof(MI3Pat) == 40, "3-material pattern layout");
-constexpr uint32_t MI3_NONE = 0xFFFFFFFFu;
+// (MI3Pat and MI3_NONE: kernels_ia.h)
 
 // the tables on the device: 2-material records by pattern; 3-material
 // patterns (staged in LDS by the kernel), slots (word offset or MI3_NONE),
@@ -99,6 +93,8 @@ struct MITablesDev {
     const MI3Pat* pat3 = nullptr;
     const uint32_t *off3 = nullptr, *cnt3 = nullptr, *cut3 = nullptr;
     const uint16_t* w3 = nullptr;
+    const uint32_t* snap_off3 = nullptr;  // complexes after three materials (MI3Snap)
+    const uint8_t* snap3 = nullptr;
 };
 
 inline MITablesDev mi_tables_view(const MITablesRaw& r) {
@@ -110,6 +106,8 @@ inline MITablesDev mi_tables_view(const MITablesRaw& r) {
     d.cnt3 = (const uint32_t*)r.cnt3;
     d.cut3 = (const uint32_t*)r.cut3;
     d.w3 = (const uint16_t*)r.w3;
+    d.snap_off3 = (const uint32_t*)r.snap_off3;
+    d.snap3 = (const uint8_t*)r.snap3;
     return d;
 }
 

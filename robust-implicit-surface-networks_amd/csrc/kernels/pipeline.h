@@ -156,7 +156,7 @@ void run_batched(DevMem& ctx_mem, const Grid& g, const Values& v, uint32_t flags
 // device, uploaded once per process and device; plain pointers (the kernels'
 // view of them is MITablesDev in kernels_mi.h)
 struct MITablesRaw {
-    const void *e2, *w2, *pat3, *off3, *cnt3, *cut3, *w3;
+    const void *e2, *w2, *pat3, *off3, *cnt3, *cut3, *w3, *snap_off3, *snap3;
 };
 MITablesRaw mi_tables_device(hipStream_t s);
 

@@ -110,6 +110,11 @@ struct MI3Table {
     uint32_t nslots = 0;
     std::vector<uint32_t> off, cnt, cut;
     std::vector<uint16_t> words;
+    // the complex after the three materials, per slot (byte offset, or
+    // MI3_NONE; equal complexes stored once): the state the lane-group cut of
+    // 4..8 materials continues from (layout: kMI3SnapHead, kMI3SnapFace)
+    std::vector<uint32_t> snap_off;
+    std::vector<uint8_t> snap;
 };
 
 const MI3Table& mi3_table() {
@@ -145,7 +150,9 @@ const MI3Table& mi3_table() {
         T.off.assign(base, MI3_NONE);
         T.cnt.assign(base, 0);
         T.cut.assign(base, 0);
+        T.snap_off.assign(base, MI3_NONE);
         std::map<std::vector<uint16_t>, uint32_t> seen;
+        std::map<std::vector<uint8_t>, uint32_t> seen_snap;
         for (int pat = 0; pat < 81; ++pat) {
             const MI3Pat& K = T.pat[pat];
             two(pat);
@@ -176,6 +183,27 @@ const MI3Table& mi3_table() {
                 T.off[K.base + sv] = off;
                 T.cnt[K.base + sv] = a0 | a1 << 8 | a2 << 16;
                 T.cut[K.base + sv] = (uint32_t)X2.nv | (uint32_t)X2.nf << 8 | (uint32_t)X2.nc << 16;
+                static_assert(CMed::MP == 8 && CMed::MFV == 8, "3-material snapshot layout");
+                std::vector<uint8_t> sn = {(uint8_t)X2.nv, (uint8_t)X2.nf, (uint8_t)X2.nc, (uint8_t)X2.dup};
+                for (int p = 0; p < 8; ++p) sn.push_back(X2.grp[p]);
+                for (int p = 0; p < 8; ++p) sn.push_back(p < X2.nc ? X2.cells[p] : (uint8_t)dev::NIL);
+                for (int v = 0; v < X2.nv; ++v)
+                    for (int q = 0; q < 4; ++q) sn.push_back(X2.vl[v][q]);
+                for (int f = 0; f < X2.nf; ++f) {
+                    sn.push_back(X2.fa[f]);
+                    sn.push_back(X2.fb[f]);
+                    sn.push_back(X2.fn[f]);
+                    sn.push_back(0);
+                    for (int q = 0; q < 8; ++q) sn.push_back(q < X2.fn[f] ? X2.fv[f][q] : (uint8_t)0);
+                    for (int q = 0; q < 8; ++q) sn.push_back(q < X2.fn[f] ? X2.fe[f][q] : (uint8_t)0);
+                }
+                auto is = seen_snap.find(sn);
+                if (is == seen_snap.end()) {
+                    const uint32_t so = (uint32_t)T.snap.size();
+                    T.snap.insert(T.snap.end(), sn.begin(), sn.end());
+                    is = seen_snap.emplace(std::move(sn), so).first;
+                }
+                T.snap_off[K.base + sv] = is->second;
             }
         }
     });
@@ -210,6 +238,8 @@ MITablesRaw mi_tables_device(hipStream_t s) {
     D.cnt3 = up(T3.cnt.data(), 4 * T3.cnt.size());
     D.cut3 = up(T3.cut.data(), 4 * T3.cut.size());
     D.w3 = up(T3.words.data(), 2 * T3.words.size());
+    D.snap_off3 = up(T3.snap_off.data(), 4 * T3.snap_off.size());
+    D.snap3 = up(T3.snap.data(), T3.snap.size());
     RIN_HIP(hipStreamSynchronize(s));
     per_device[dev] = D;
     return D;
@@ -649,21 +679,29 @@ void run_pipeline(DevMem& mem, const Grid& g, const Values& val, uint32_t flags,
         plain_k2_snapshots(s, &pat, &k2s.off, &k2s.bytes);
         k2s.pat = static_cast<const K2Pat*>(pat);
     }
+    // the material-interface lane groups start from the 3-material table's
+    // complex after three materials (RIN_MI3_START=0: they insert every one)
+    MI3Snap m3s{nullptr, nullptr, nullptr};
+    const char* ms = std::getenv("RIN_MI3_START");
+    if (MI && groups && !(ms && !std::strcmp(ms, "0"))) {
+        const MITablesDev TT = mi_tables_dev(s);
+        m3s = MI3Snap{TT.pat3, TT.snap_off3, TT.snap3};
+    }
     for (int q = 0; q < 2 && NA && groups; ++q) {
         const uint64_t nl = bcnt[1 + q];
         if (!nl) continue;
-        auto go = [&](auto kernel, int per_wave) {
+        auto go = [&](auto kernel, int per_wave, auto snap) {
             const unsigned blocks = (unsigned)std::min<uint64_t>((nl + per_wave - 1) / per_wave, 4096);
             hipLaunchKernelGGL(kernel, dim3(blocks), dim3(64), 0, s, g.view, (const double*)val.vals.p,
                                (const uint32_t*)out.act.p, (const uint8_t*)out.act_k.p,
                                (const uint32_t*)out.csr_off.p, (const uint8_t*)out.fit.p,
                                (const uint32_t*)blist[1 + q], Count(nl), (const uint32_t*)out.rec_off.p, out.rec.p,
-                               cnt.p, errs.p, 1, k2s);
+                               cnt.p, errs.p, 1, snap);
         };
         if (q == 0)
-            MI ? go(k_cut_group<CMed, 64, true>, 1) : go(k_cut_group<CMed, 16, false>, 4);
+            MI ? go(k_cut_group<CMed, 64, true>, 1, m3s) : go(k_cut_group<CMed, 16, false>, 4, k2s);
         else
-            MI ? go(k_cut_group<CBig, 64, true>, 1) : go(k_cut_group<CBig, 64, false>, 1);
+            MI ? go(k_cut_group<CBig, 64, true>, 1, m3s) : go(k_cut_group<CBig, 64, false>, 1, k2s);
         RIN_HIP(hipGetLastError());
     }
     // each thread-per-tet cut kernel runs over its own k range only

@@ -553,14 +553,19 @@ def test_batched_mi_matches_oracle_and_replays(ctx, which):
     g.release()
 
 
-@pytest.mark.parametrize("form", ["wave", "thread"])
+@pytest.mark.parametrize("form", ["wave", "thread", "wave-scratch"])
 @pytest.mark.parametrize("which", ["config3_small", "config4", "config4_shuffled", "mesh"])
 def test_batched_mi_cut_forms(ctx, which, form, monkeypatch):
     """Every material-interface tet (tables off) through the lane-group cut or
     the thread cut gives the oracle's output bit for bit, and the records of
-    every active tet equal the oracle's."""
+    every active tet equal the oracle's.  The lane-group cut of four or more
+    materials starts from the 3-material table's complex after three
+    materials, or ("-scratch") inserts every material itself."""
     import rin
 
+    if form.endswith("-scratch"):
+        monkeypatch.setenv("RIN_MI3_START", "0")
+        form = form[: -len("-scratch")]
     monkeypatch.setenv("RIN_CUT_FORM", form)
     if which == "mesh":
         m = np.load(S.REF_TESTS + "/mesh.npz")
