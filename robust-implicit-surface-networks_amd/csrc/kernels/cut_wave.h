@@ -90,18 +90,38 @@ struct Grp {
         if (G == 64) return b;
         return (b >> ((threadIdx.x & 63) & ~(G - 1))) & ((1ull << (G & 63)) - 1);
     }
+    // exclusive prefix and sum over the group.  B > 0: x < 2^B on every lane,
+    // and bit b of the result comes from a ballot of bit b (a mask count, no
+    // cross-lane data movement); B = 0: any x, by log2(G) lane shuffles.
+    template <int B = 0>
     __device__ static uint32_t excl(uint32_t x) {
         const int l = lane();
-        uint32_t v = x;
-        for (int d = 1; d < G; d <<= 1) {
-            const uint32_t o = __shfl_up(v, d, G);
-            if (l >= d) v += o;
+        if constexpr (B > 0) {
+            const uint64_t below = (1ull << l) - 1u;  // the group's lanes before this one
+            uint32_t r = 0;
+#pragma unroll
+            for (int b = 0; b < B; ++b) r += (uint32_t)__popcll(ballot((x >> b) & 1u) & below) << b;
+            return r;
+        } else {
+            uint32_t v = x;
+            for (int d = 1; d < G; d <<= 1) {
+                const uint32_t o = __shfl_up(v, d, G);
+                if (l >= d) v += o;
+            }
+            return v - x;
         }
-        return v - x;
     }
+    template <int B = 0>
     __device__ static uint32_t sum(uint32_t x) {
-        for (int d = G / 2; d > 0; d >>= 1) x += __shfl_xor(x, d, G);
-        return x;
+        if constexpr (B > 0) {
+            uint32_t r = 0;
+#pragma unroll
+            for (int b = 0; b < B; ++b) r += (uint32_t)__popcll(ballot((x >> b) & 1u)) << b;
+            return r;
+        } else {
+            for (int d = G / 2; d > 0; d >>= 1) x += __shfl_xor(x, d, G);
+            return x;
+        }
     }
     __device__ static int min(int x) {
         for (int d = G / 2; d > 0; d >>= 1) {
@@ -114,16 +134,16 @@ struct Grp {
 
 // exclusive prefix of count(i) over items 0..n-1, item i on group lane
 // i % G; store(i, offset) runs on that same lane; returns the total (uniform
-// over the group)
-template <int G, class F, class H>
+// over the group).  B > 0: every count is below 2^B.
+template <int G, int B = 0, class F, class H>
 __device__ inline uint32_t wave_prefix(int n, F count, H store) {
     uint32_t base = 0;
     for (int r0 = 0; r0 < n; r0 += G) {
         const int i = r0 + Grp<G>::lane();
         const uint32_t c = i < n ? count(i) : 0u;
-        const uint32_t pre = Grp<G>::excl(c);
+        const uint32_t pre = Grp<G>::template excl<B>(c);
         if (i < n) store(i, base + pre);
-        base += Grp<G>::sum(c);
+        base += Grp<G>::template sum<B>(c);
     }
     return base;
 }
@@ -318,16 +338,22 @@ __device__ void wave_ia_cut(WaveIA<C>& X, int k, int start = 0) {
 
         // each face's sign flags (kept for the face and cell splits), and a
         // plane containing an existing face joins that face's group
+        // (the smallest such face: the first round with one, its first lane)
         int cop = INT_MAX;
-        for (int f = lane; f < nf0; f += G) {
-            uint8_t fvr[C::MFV];
-            int8_t sr[C::MFV];
-            (void)load_face<C>(X, S, f, fvr, sr);
-            const uint8_t fl = sign_flags<C::MFV>(sr);
-            X.fflag[f] = fl;
-            if (fl == 0 && f < cop) cop = f;
+        for (int r0 = 0; r0 < nf0; r0 += G) {
+            const int f = r0 + lane;
+            bool on = false;
+            if (f < nf0) {
+                uint8_t fvr[C::MFV];
+                int8_t sr[C::MFV];
+                (void)load_face<C>(X, S, f, fvr, sr);
+                const uint8_t fl = sign_flags<C::MFV>(sr);
+                X.fflag[f] = fl;
+                on = fl == 0;
+            }
+            const uint64_t b = W::ballot(on);
+            if (b && cop == INT_MAX) cop = r0 + __builtin_ctzll(b);
         }
-        cop = W::min(cop);
         if (cop != INT_MAX) {
             if (lane == 0) {
                 const uint8_t cell = X.fb[cop];
@@ -379,8 +405,9 @@ __device__ void wave_ia_cut(WaveIA<C>& X, int k, int start = 0) {
                     c += i < n && sr[i] * sw < 0;
                 }
             }
-            uint32_t o = ncand + W::excl(c);
-            const uint32_t m = nmix + W::excl(mixed ? 1u : 0u);
+            static_assert(MFV < 16, "crossings per face fit four bits");
+            uint32_t o = ncand + W::template excl<4>(c);
+            const uint32_t m = nmix + W::template excl<1>(mixed ? 1u : 0u);
             if (mixed) {
                 X.frank[f] = (uint16_t)m;
                 X.fcand[f] = (uint16_t)o;
@@ -399,8 +426,8 @@ __device__ void wave_ia_cut(WaveIA<C>& X, int k, int start = 0) {
                     ++o;
                 }
             }
-            ncand += W::sum(c);
-            nmix += W::sum(mixed ? 1u : 0u);
+            ncand += W::template sum<4>(c);
+            nmix += W::template sum<1>(mixed ? 1u : 0u);
         }
         if (ncand > (uint32_t)CAND) return fail();
         __syncthreads();
@@ -429,7 +456,7 @@ __device__ void wave_ia_cut(WaveIA<C>& X, int k, int start = 0) {
         __syncthreads();
         for (int c = lane; c < (int)ncand; c += G) X.u.c.cfirst[c] = first_of(X.u.c.ckey[c]) == (uint32_t)c;
         __syncthreads();
-        const uint32_t nnew = wave_prefix<G>(
+        const uint32_t nnew = wave_prefix<G, 1>(
             (int)ncand, [&](int c) { return (uint32_t)X.u.c.cfirst[c]; },
             [&](int c, uint32_t o) {
                 if (X.u.c.cfirst[c]) X.u.c.cvid[c] = (uint8_t)(nv0 + o);
@@ -512,7 +539,7 @@ __device__ void wave_ia_cut(WaveIA<C>& X, int k, int start = 0) {
             }
         }
         __syncthreads();
-        const uint32_t nsplit = wave_prefix<G>(
+        const uint32_t nsplit = wave_prefix<G, 1>(
             nc0, [&](int c) { return (uint32_t)(X.u.e.cflag[c] == 3); },
             [&](int c, uint32_t o) { X.u.e.crank[c] = X.u.e.cflag[c] == 3 ? (uint16_t)o : (uint16_t)0xFFFF; });
         if (nf1 + (int)nsplit > C::MF || nc0 + (int)nsplit > WaveIA<C>::MC) return fail();
@@ -659,7 +686,8 @@ __device__ int wave_ia_record(WaveIA<C>& X, int k, uint16_t* out, int cap, uint3
     using W = Grp<G>;
     const int lane = W::lane();
     const int L = 4 + k, nv = X.nv, nf = X.nf;
-    const uint32_t fwords = wave_prefix<G>(
+    static_assert(4 + C::MFV < 16, "face words fit four bits");
+    const uint32_t fwords = wave_prefix<G, 4>(
         nf, [&](int f) { return (uint32_t)(4 + X.fn[f]); }, [&](int f, uint32_t o) { X.u.r.foff[f] = (uint16_t)o; });
     const int total = 8 + 4 * nv + (int)fwords + (X.dup ? 2 * L : 0);
     for (int v = lane; v < nv; v += G) X.u.r.isov[v] = 0;

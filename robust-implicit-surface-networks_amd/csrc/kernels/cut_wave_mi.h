@@ -256,7 +256,8 @@ __device__ void wave_mi_cut(WaveMI<C>& X, int k, int start = 1) {
                     c += i < n && sr[i] * sw < 0;
                 }
             }
-            uint32_t o = ncand + W::excl(c);
+            static_assert(MFV < 16, "crossings per face fit four bits");
+            uint32_t o = ncand + W::template excl<4>(c);
             if (mixed) {
                 X.fcand[f] = (uint16_t)o;
 #pragma unroll
@@ -274,7 +275,7 @@ __device__ void wave_mi_cut(WaveMI<C>& X, int k, int start = 1) {
                     ++o;
                 }
             }
-            ncand += W::sum(c);
+            ncand += W::template sum<4>(c);
         }
         if (ncand > (uint32_t)CAND) return fail();
         __syncthreads();
@@ -300,7 +301,7 @@ __device__ void wave_mi_cut(WaveMI<C>& X, int k, int start = 1) {
         __syncthreads();
         for (int c = lane; c < (int)ncand; c += G) X.u.c.cfirst[c] = first_of(X.u.c.ckey[c]) == (uint32_t)c;
         __syncthreads();
-        const uint32_t nnew = wave_prefix<G>(
+        const uint32_t nnew = wave_prefix<G, 1>(
             (int)ncand, [&](int c) { return (uint32_t)X.u.c.cfirst[c]; },
             [&](int c, uint32_t o) {
                 if (X.u.c.cfirst[c]) X.u.c.cvid[c] = (uint8_t)(nv0 + o);
@@ -369,13 +370,13 @@ __device__ void wave_mi_cut(WaveMI<C>& X, int k, int start = 1) {
             const int f = base + lane;
             uint32_t emit = 0, npe = 0;
             if (f < nf0) plan(f, emit, npe);
-            const uint32_t oo = nout + W::excl(emit), op = npe_all + W::excl(npe);
+            const uint32_t oo = nout + W::template excl<1>(emit), op = npe_all + W::template excl<4>(npe);
             if (f < nf0) {
                 X.fout[f] = emit ? (uint16_t)oo : (uint16_t)0xFFFF;
                 X.fpe[f] = (uint16_t)op;
             }
-            nout += W::sum(emit);
-            npe_all += W::sum(npe);
+            nout += W::template sum<1>(emit);
+            npe_all += W::template sum<4>(npe);
         }
         if (npe_all > (uint32_t)PE) return fail();
         __syncthreads();
@@ -681,7 +682,7 @@ __device__ void wave_mi_cut(WaveMI<C>& X, int k, int start = 1) {
             for (int q = 0; q < n; ++q) X.used[X.fv[f][q]] = 1;
         }
         __syncthreads();
-        const uint32_t nvn = wave_prefix<G>(
+        const uint32_t nvn = wave_prefix<G, 1>(
             nv1, [&](int v) { return (uint32_t)X.used[v]; },
             [&](int v, uint32_t o) { X.remap[v] = X.used[v] ? (uint8_t)o : (uint8_t)NIL; });
         __syncthreads();
@@ -723,7 +724,8 @@ __device__ int wave_mi_record(WaveMI<C>& X, int k, uint16_t* out, int cap, uint3
     const int lane = W::lane();
     const int L = 4 + k, nv = X.nv, nf = X.nf, nc = X.nc;
     // face word offsets: fpe is free after the insertions
-    const uint32_t fwords = wave_prefix<G>(
+    static_assert(4 + C::MFV < 16, "face words fit four bits");
+    const uint32_t fwords = wave_prefix<G, 4>(
         nf, [&](int f) { return (uint32_t)(4 + X.fn[f]); }, [&](int f, uint32_t o) { X.fpe[f] = (uint16_t)o; });
     const int total = 8 + 4 * nv + (int)fwords + nc + (X.dup ? L : 0);
     for (int v = lane; v < nv; v += G) X.used[v] = 0;
