@@ -167,6 +167,11 @@ struct MIWorkspace {
     // per active tet
     DBuf<uint8_t> fit, act_k;
     DBuf<uint32_t> act, blists, bc, rbound, rec_off, cnt, fb;
+    // each active tet's record location: its slot (rec_off), or for a table
+    // tet the table's record, whose words follow the slots in rec (at
+    // tab_base2 and tab_base3)
+    DBuf<uint32_t> rec_loc;
+    uint32_t tab_base2 = 0, tab_base3 = 0;
     DBuf<uint16_t> rec;
     DBuf<uint64_t> emask;
     unsigned cut_blocks[2] = {0, 0};
@@ -253,7 +258,18 @@ void alloc_mi(MIWorkspace& W, const Grid& g, const MICaps& c, hipStream_t s) {
     W.bc.alloc(m, (uint64_t)kBucketWords * nch + kBucketWords);
     W.rbound.alloc(m, A + 2);
     W.rec_off.alloc(m, A + 2);
-    W.rec.alloc(m, (uint64_t)c.words + 1);
+    W.rec_loc.alloc(m, A + 2);
+    {
+        // the slots, then the two tables' record words (copied once per allocation)
+        const MITablesRaw TR = mi_tables_device(s);
+        if ((uint64_t)c.words + 1 + TR.n_w2 + TR.n_w3 >= (1ull << 32))
+            throw std::runtime_error("material-interface record words exceed 32-bit offsets");
+        W.tab_base2 = (uint32_t)c.words + 1;
+        W.tab_base3 = W.tab_base2 + (uint32_t)TR.n_w2;
+        W.rec.alloc(m, (uint64_t)W.tab_base3 + TR.n_w3 + 1);
+        RIN_HIP(hipMemcpyAsync(W.rec.p + W.tab_base2, TR.w2, 2 * TR.n_w2, hipMemcpyDeviceToDevice, s));
+        RIN_HIP(hipMemcpyAsync(W.rec.p + W.tab_base3, TR.w3, 2 * TR.n_w3, hipMemcpyDeviceToDevice, s));
+    }
     W.cnt.alloc(m, 3 * A + 3);
     W.fb.alloc(m, A + 2);
     W.emask.alloc(m, A + 1);
@@ -357,10 +373,12 @@ void mi_path(MIWorkspace& W, const Grid& g, const Values& val, const MITablesDev
     scan_u32(W.mem, W.rbound.p, W.rec_off.p, Count((uint64_t)c.active + 1), ctr + M_WORDS, s);
     hipLaunchKernelGGL(k_mi_check, dim3(1), dim3(64), 0, s, ctr, (const uint32_t*)W.csr_off.p, T, c, W.errs.p);
     // the cut: tables for two and three materials, thread cuts beyond
+    // record locations: each tet's slot, then table tets point at the table's record
+    copy_u32(W.rec_loc.p, W.rec_off.p, NA, s);
     hipLaunchKernelGGL(k_mi_table, dim3(blocks_for(c.active)), dim3(256), 0, s, G, vals, (const uint32_t*)W.act.p,
                        (const uint32_t*)W.csr_off.p, (const uint8_t*)W.fit.p, (const uint32_t*)bl[0],
                        (uint64_t)c.active, (const uint32_t*)(ctr + M_LIST), (uint32_t)0, (const uint32_t*)W.rec_off.p,
-                       W.rec.p, W.cnt.p, W.errs.p, TT);
+                       W.rec.p, W.cnt.p, W.errs.p, TT, W.rec_loc.p, W.tab_base2, W.tab_base3);
     const Count n1(ctr + M_LIST + 1, c.active, abort), n2(ctr + M_LIST + 2, c.active, abort);
     if (all_thread) {
         hipLaunchKernelGGL((k_cut_lds<CMed, true, 2>), dim3(W.cut_blocks[0]), dim3(2), 0, s, G, vals,
@@ -402,7 +420,7 @@ void mi_path(MIWorkspace& W, const Grid& g, const Values& val, const MITablesDev
     A.act = W.act.p;
     A.csr_off = W.csr_off.p;
     A.rank = W.rank.p;
-    A.rec_loc = W.rec_off.p;
+    A.rec_loc = W.rec_loc.p;
     A.nbr = g.nbr.p;
     A.fit = W.fit.p;
     A.rec = W.rec.p;
@@ -447,7 +465,7 @@ void mi_path(MIWorkspace& W, const Grid& g, const Values& val, const MITablesDev
     // coordinates and faces
     const Count NV(ctr + M_NV, c.vo, abort), NF(ctr + M_FO, c.fo, abort), NFV(ctr + M_FV, c.fv, abort);
     hipLaunchKernelGGL(k_vert_fields_mi, dim3(blocks_for(c.vo)), dim3(256), 0, s, G, vals, (const uint32_t*)W.act.p,
-                       (const uint32_t*)W.csr_off.p, (const uint8_t*)W.fit.p, (const uint32_t*)W.rec_off.p,
+                       (const uint32_t*)W.csr_off.p, (const uint8_t*)W.fit.p, (const uint32_t*)W.rec_loc.p,
                        (const uint16_t*)W.rec.p, (const uint32_t*)W.first_occ.p, (const uint32_t*)W.oa.p,
                        (const uint16_t*)W.oj.p, NV, W.v_tet.p, W.v_j.p, W.v_ss.p, W.v_sv.p, W.v_fn.p, W.xyz.p);
     hipLaunchKernelGGL(k_mi_face_out, dim3(blocks_for(c.fo)), dim3(256), 0, s, NF, (const uint32_t*)W.act.p,
@@ -627,7 +645,7 @@ void run_batched_mi(DevMem& ctx_mem, const Grid& g, const Values& val, uint32_t 
     vw.rank_of_tet = w.rank.p;
     vw.rec_off = w.rec_off.p;
     vw.rec = w.rec.p;
-    vw.rec_loc = w.rec_off.p;
+    vw.rec_loc = w.rec_loc.p;
     vw.v_tet = w.v_tet.p;
     vw.v_j = w.v_j.p;
     vw.v_ss = w.v_ss.p;

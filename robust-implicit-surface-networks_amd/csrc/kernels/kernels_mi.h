@@ -114,7 +114,9 @@ inline MITablesDev mi_tables_view(const MITablesRaw& r) {
 // Active tets with two or three materials by table (list: their active
 // ranks; the count at n_dev when given, else n).  A tet whose sign vector has
 // no table entry, or whose record outgrows its slot, is marked for the
-// general cut (CNT_RETRY).
+// general cut (CNT_RETRY).  rec_loc (optional): the record words hold the two
+// tables' words at base2 and base3, and the tet's record location becomes the
+// table record's instead of a copy into its slot.
 __global__ void __launch_bounds__(256) k_mi_table(GridView g, const double* __restrict__ vals,
                                                   const uint32_t* __restrict__ act,
                                                   const uint32_t* __restrict__ csr_off, const uint8_t* __restrict__ fit,
@@ -122,7 +124,8 @@ __global__ void __launch_bounds__(256) k_mi_table(GridView g, const double* __re
                                                   const uint32_t* __restrict__ n_dev, uint32_t flags,
                                                   const uint32_t* __restrict__ rec_off, uint16_t* __restrict__ rec,
                                                   uint32_t* __restrict__ cnt, uint32_t* __restrict__ errs,
-                                                  MITablesDev T) {
+                                                  MITablesDev T, uint32_t* __restrict__ rec_loc, uint32_t base2,
+                                                  uint32_t base3) {
     __shared__ MI3Pat pats[81];
     for (int i = threadIdx.x; i < 81 * (int)sizeof(MI3Pat) / 4; i += blockDim.x)
         reinterpret_cast<uint32_t*>(pats)[i] = reinterpret_cast<const uint32_t*>(T.pat3)[i];
@@ -144,10 +147,11 @@ __global__ void __launch_bounds__(256) k_mi_table(GridView g, const double* __re
         int pat = 0;
         for (int c = 0, w = 1; c < 4; ++c, w *= 3) pat += (P[1][c] > P[0][c] ? 2 : (P[1][c] < P[0][c] ? 0 : 1)) * w;
         const uint16_t* src = nullptr;
-        uint32_t len = 0, c3[3];
+        uint32_t len = 0, c3[3], at_table = 0;
         if (k == 2) {
             const MI2Entry E = T.e2[pat];
             src = T.w2 + E.off;
+            at_table = base2 + E.off;
             len = E.len;
             const uint32_t* cc = (flags & RUN_CUT_ONLY) ? E.cut : E.cnt;
             c3[0] = cc[0];
@@ -164,6 +168,7 @@ __global__ void __launch_bounds__(256) k_mi_table(GridView g, const double* __re
             const uint32_t off = err ? MI3_NONE : T.off3[slot];
             if (off != MI3_NONE) {
                 src = T.w3 + off;
+                at_table = base3 + off;
                 len = src[6];
                 const uint32_t pk = (flags & RUN_CUT_ONLY) ? T.cut3[slot] : T.cnt3[slot];
                 c3[0] = pk & 0xFF;
@@ -172,14 +177,17 @@ __global__ void __launch_bounds__(256) k_mi_table(GridView g, const double* __re
             }
         }
         const uint32_t at = rec_off[a];
-        if (!src || len > rec_off[a + 1] - at) {
+        if (!src || (!rec_loc && len > rec_off[a + 1] - at)) {
             flag_error(errs, ERR_RETRY);
             cnt[3 * (uint64_t)a] = CNT_RETRY;
             cnt[3 * (uint64_t)a + 1] = cnt[3 * (uint64_t)a + 2] = 0;
             rec[at] = 0;
             continue;
         }
-        for (uint32_t q = 0; q < len; ++q) rec[at + q] = src[q];
+        if (rec_loc)
+            rec_loc[a] = at_table;
+        else
+            for (uint32_t q = 0; q < len; ++q) rec[at + q] = src[q];
         for (int q = 0; q < 3; ++q) cnt[3 * (uint64_t)a + q] = c3[q];
     }
 }

@@ -157,6 +157,7 @@ void run_batched(DevMem& ctx_mem, const Grid& g, const Values& v, uint32_t flags
 // view of them is MITablesDev in kernels_mi.h)
 struct MITablesRaw {
     const void *e2, *w2, *pat3, *off3, *cnt3, *cut3, *w3, *snap_off3, *snap3;
+    uint64_t n_w2, n_w3;  // record words of the two- and three-material tables
 };
 MITablesRaw mi_tables_device(hipStream_t s);
 

@@ -240,6 +240,8 @@ MITablesRaw mi_tables_device(hipStream_t s) {
     D.w3 = up(T3.words.data(), 2 * T3.words.size());
     D.snap_off3 = up(T3.snap_off.data(), 4 * T3.snap_off.size());
     D.snap3 = up(T3.snap.data(), T3.snap.size());
+    D.n_w2 = T2.words.size();
+    D.n_w3 = T3.words.size();
     RIN_HIP(hipStreamSynchronize(s));
     per_device[dev] = D;
     return D;
@@ -664,7 +666,7 @@ void run_pipeline(DevMem& mem, const Grid& g, const Values& val, uint32_t flags,
         hipLaunchKernelGGL(k_mi_table, dim3(nblk(bcnt[0])), dim3(256), 0, s, g.view, (const double*)val.vals.p,
                            (const uint32_t*)out.act.p, (const uint32_t*)out.csr_off.p, (const uint8_t*)out.fit.p,
                            (const uint32_t*)blist[0], (uint64_t)bcnt[0], (const uint32_t*)nullptr, flags,
-                           (const uint32_t*)out.rec_off.p, out.rec.p, cnt.p, errs.p, TT);
+                           (const uint32_t*)out.rec_off.p, out.rec.p, cnt.p, errs.p, TT, (uint32_t*)nullptr, 0u, 0u);
         RIN_HIP(hipGetLastError());
     }
     // the cuts beyond the tables: lane groups with the complex in LDS (the

@@ -138,6 +138,8 @@ def test_mi_two_material_table(ctx, which):
     vals = g.eval(types, params)
     on = vals.run(rin.FLAG_MI | rin.FLAG_KEEP_RECORDS)
     off = vals.run(rin.FLAG_MI | rin.FLAG_NO_LOOKUP | rin.FLAG_KEEP_RECORDS)
+    # the batched form keeps no copy: a table tet's record is the table's
+    batched = vals.run(rin.FLAG_MI | rin.FLAG_BATCHED | rin.FLAG_KEEP_RECORDS)
     pts, tets = S.grid(res)
     ref = S.OracleRun(1, pts, tets, S.evaluate(types, params, pts))
     assert on.buckets[0] > 0  # two-material tets present
@@ -148,9 +150,13 @@ def test_mi_two_material_table(ctx, which):
     st2, w2 = off.records(ref.active, cap)
     np.testing.assert_array_equal(st2, st)
     np.testing.assert_array_equal(w2, w)
+    st3, w3 = batched.records(ref.active, cap)
+    np.testing.assert_array_equal(st3, st)
+    np.testing.assert_array_equal(w3, w)
     _compare(on, ref)
     _compare(off, ref)
-    for r in (on, off):
+    _compare(batched, ref)
+    for r in (on, off, batched):
         r.release()
     vals.release()
     g.release()
