@@ -265,9 +265,12 @@ __device__ inline int split_side_regs(const uint8_t (&fvr)[MFV], const uint8_t (
 // the tet; on return X.err is set when a capacity or the predicates'
 // exponent range was exceeded.
 // start: planes already inserted (0, or 2 when the caller has loaded the
-// complex after the first two, which depends on their signs only)
+// complex after the first two, which depends on their signs only).  Always
+// inlined: as a called function (several kernels instantiate it) the calling
+// convention would save and restore the live registers through scratch memory
+// on every call.
 template <class C, int G = 64>
-__device__ void wave_ia_cut(WaveIA<C>& X, int k, int start = 0) {
+__device__ __forceinline__ void wave_ia_cut(WaveIA<C>& X, int k, int start = 0) {
     using W = Grp<G>;
     const int lane = W::lane();
     if (start == 0 && lane < 4) {
@@ -681,7 +684,7 @@ __device__ void wave_ia_cut(WaveIA<C>& X, int k, int start = 0) {
 // record (nf words past its length), so that readers need not walk the
 // variable-length face list.
 template <class C, int G = 64>
-__device__ int wave_ia_record(WaveIA<C>& X, int k, uint16_t* out, int cap, uint32_t& niv, uint32_t& nif,
+__device__ __forceinline__ int wave_ia_record(WaveIA<C>& X, int k, uint16_t* out, int cap, uint32_t& niv, uint32_t& nif,
                               uint32_t& nfv) {
     using W = Grp<G>;
     const int lane = W::lane();

@@ -134,7 +134,7 @@ __device__ inline int wave_chain(const uint8_t* ea, const uint8_t* eb, const uin
 // start: materials already in the complex (1, or 3 when the caller has
 // loaded the complex after the first three, which depends on their signs only)
 template <class C, int G = 64>
-__device__ void wave_mi_cut(WaveMI<C>& X, int k, int start = 1) {
+__device__ __forceinline__ void wave_mi_cut(WaveMI<C>& X, int k, int start = 1) {
     using W = Grp<G>;
     constexpr int MFV = C::MFV, MV = C::MV, MF = C::MF, MP = C::MP;
     constexpr int CAND = WaveMI<C>::CAND, HT = WaveMI<C>::HT, PE = WaveMI<C>::PE, TE = WaveMI<C>::TE;
@@ -718,7 +718,7 @@ __device__ void wave_mi_cut(WaveMI<C>& X, int k, int start = 1) {
 // nfv are the tet's interface-vertex, face and face-corner counts
 // (mi_counts), uniform over the group.
 template <class C, int G = 64>
-__device__ int wave_mi_record(WaveMI<C>& X, int k, uint16_t* out, int cap, uint32_t& niv, uint32_t& nif,
+__device__ __forceinline__ int wave_mi_record(WaveMI<C>& X, int k, uint16_t* out, int cap, uint32_t& niv, uint32_t& nif,
                               uint32_t& nfv) {
     using W = Grp<G>;
     const int lane = W::lane();

@@ -420,7 +420,7 @@ struct K2Snap {
 
 // the slot of a tet whose first two rows are P[0], P[1], as k_k2_lookup
 // computes it (corner signs and at most four exact edge predicates), or K2_NONE
-__host__ __device__ inline uint32_t k2_slot_of(const K2Pat* __restrict__ pats, const double (*P)[4]) {
+__host__ __device__ __forceinline__ uint32_t k2_slot_of(const K2Pat* __restrict__ pats, const double (*P)[4]) {
     int pa = 0, sB[4];
     for (int c = 0, w = 1; c < 4; ++c, w *= 3) {
         const double a = P[0][c], b = P[1][c];
@@ -448,7 +448,7 @@ __host__ __device__ inline uint32_t k2_slot_of(const K2Pat* __restrict__ pats, c
 // X.P, visible to the group) and returns true, or returns false (no snapshot:
 // the cut starts from the tet).  Uniform over the group.
 template <class C, int G>
-__device__ bool k2_start(dev::WaveIA<C>& X, const K2Snap& S, int k) {
+__device__ __forceinline__ bool k2_start(dev::WaveIA<C>& X, const K2Snap& S, int k) {
     const int lane = dev::Grp<G>::lane();
     if (!S.off || k < 3 || 4 + k > C::MP) return false;
     const uint32_t slot = k2_slot_of(S.pat, X.P);
@@ -511,7 +511,7 @@ struct MI3Snap {
 
 // the 3-material slot of a tet whose first three rows are P[0..2], as
 // k_mi_table computes it, or MI3_NONE
-__device__ inline uint32_t mi3_slot_of(const MI3Pat* __restrict__ pats, const double (*P)[4]) {
+__device__ __forceinline__ uint32_t mi3_slot_of(const MI3Pat* __restrict__ pats, const double (*P)[4]) {
     int pat = 0;
     for (int c = 0, w = 1; c < 4; ++c, w *= 3) pat += (P[1][c] > P[0][c] ? 2 : (P[1][c] < P[0][c] ? 0 : 1)) * w;
     const MI3Pat& K = pats[pat];
@@ -529,7 +529,7 @@ __device__ inline uint32_t mi3_slot_of(const MI3Pat* __restrict__ pats, const do
 // in X.P, visible to the group) and returns true, or returns false (the cut
 // starts from the tet).  Uniform over the group.
 template <class C, int G>
-__device__ bool mi3_start(dev::WaveMI<C>& X, const MI3Snap& S, int k) {
+__device__ __forceinline__ bool mi3_start(dev::WaveMI<C>& X, const MI3Snap& S, int k) {
     const int lane = dev::Grp<G>::lane();
     if (!S.off || k < 4 || 4 + k > C::MP) return false;
     const uint32_t slot = mi3_slot_of(S.pat, X.P);
