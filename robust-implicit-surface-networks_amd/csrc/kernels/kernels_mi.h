@@ -518,7 +518,7 @@ __global__ void k_mi_count(MIArgs A, const uint64_t* __restrict__ emask, uint32_
     }
 }
 
-__global__ void k_mi_extract(MIArgs A, const uint64_t* __restrict__ emask, const uint32_t* __restrict__ voff,
+__global__ void __launch_bounds__(256) k_mi_extract(MIArgs A, const uint64_t* __restrict__ emask, const uint32_t* __restrict__ voff,
                              const uint32_t* __restrict__ foff, const uint32_t* __restrict__ fvoff,
                              uint64_t* __restrict__ okey,
                              uint32_t* __restrict__ oa, uint16_t* __restrict__ oj, uint32_t* __restrict__ fq_a,
