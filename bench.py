@@ -323,6 +323,9 @@ def _replay_timeline(trace_csv):
     rows = sorted(csv.DictReader(open(trace_csv)), key=lambda r: int(r["Start_Timestamp"]))
     starts = [i for i, r in enumerate(rows) if _short(r["Kernel_Name"]) == "k_signs_count"]
     segs = [rows[a:b] for a, b in zip(starts, starts[1:] + [len(rows)])]
+    # whole-path runs end with the faces' closing kernel (the autotuner's
+    # timing passes start with the signs pass too, and stop early)
+    segs = [x for x in segs if any(_short(r["Kernel_Name"]) == "k_face_emit_close" for r in x)]
     if not segs:
         return None
     counts = [len(x) for x in segs]
