@@ -225,6 +225,11 @@ int rin_debug_copy(rin_ctx* ctx, uint64_t bytes, int32_t reps, double* ms);
  * totals[f] receives each field's sum. */
 int rin_debug_scan(rin_ctx* ctx, int32_t fields, int32_t stride, const uint32_t* in, uint64_t n, uint64_t count,
                    uint32_t* out, uint32_t* totals);
+/* Test hook with no counterpart in the reference: the correctly rounded
+ * square root the device's function evaluation uses, on host data (n
+ * doubles in, n out), so that its exact fallback near rounding midpoints can
+ * be checked against the host's. */
+int rin_debug_sqrt(rin_ctx* ctx, const double* in, uint64_t n, double* out);
 
 #ifdef __cplusplus
 }

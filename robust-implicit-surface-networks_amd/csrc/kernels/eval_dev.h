@@ -49,6 +49,17 @@ __device__ inline double sqrt_rn(double x) {
         e -= 1;               // [1,4), even exponent
     }
     double y = __builtin_sqrt(m);
+    // Fast acceptance.  For the correctly rounded root the residual m - y*y
+    // is a double (Boldo and Daumas, "Representable correcting terms for
+    // possibly underflowing floating point operations", 2003), so one fused
+    // multiply-add gives it exactly; y in [1, 2] is the correctly rounded root
+    // when the residual lies within 2 h y of zero (h = 2^-53, half an ulp), and
+    // a root off by an ulp has a residual beyond (1 - 2^-52) of that band,
+    // exact or rounded, so a test with a margin of 2^-40 accepts only the
+    // correctly rounded root (and nearly every one: a lane that fails sends
+    // its whole wavefront through the slow path).  Roots within 2^-40 of a
+    // midpoint go on to the exact tests below.
+    if (__builtin_fabs(__builtin_fma(-y, y, m)) <= (1.0 - 0x1p-40) * 0x1p-52 * y) return ldexp(y, e / 2);
     const double u = 0x1p-52;
     for (int it = 0; it < 3; ++it) {
         if (y < 2.0 && mid_cmp(m, y, 0.5 * u) > 0) {

@@ -194,6 +194,8 @@ void compute_chains(DevMem& mem, RunOut& r, hipStream_t s);
 // diagnostics: a plain device copy of `bytes` (reference kernel for
 // hardware-counter scaling); returns the average milliseconds per copy
 float debug_copy(DevMem& mem, uint64_t bytes, int reps, hipStream_t s);
+// the function evaluation's correctly rounded square root on host data
+void debug_sqrt(DevMem& mem, const double* in, uint64_t n, double* out, hipStream_t s);
 // the plain form's k = 1 and k = 2 implicit-arrangement tets (list of active
 // ranks, n of them) by the lookup tables: each table record is copied into
 // the tet's record slot (batched.hip)

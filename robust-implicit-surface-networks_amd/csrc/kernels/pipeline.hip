@@ -1307,6 +1307,22 @@ void compute_chains(DevMem& mem, RunOut& r, hipStream_t s) {
     r.have_chains = true;
 }
 
+namespace {
+__global__ void k_debug_sqrt(const double* __restrict__ in, uint64_t n, double* __restrict__ out) {
+    GRID_STRIDE(i, n) out[i] = dev::sqrt_rn(in[i]);
+}
+}  // namespace
+
+void debug_sqrt(DevMem& mem, const double* in, uint64_t n, double* out, hipStream_t s) {
+    if (!n) return;
+    DBuf<double> din(mem, n), dout(mem, n);
+    RIN_HIP(hipMemcpyAsync(din.p, in, 8 * n, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_debug_sqrt, dim3(nblk(n)), dim3(256), 0, s, (const double*)din.p, n, dout.p);
+    RIN_HIP(hipGetLastError());
+    RIN_HIP(hipMemcpyAsync(out, dout.p, 8 * n, hipMemcpyDeviceToHost, s));
+    RIN_HIP(hipStreamSynchronize(s));
+}
+
 float debug_copy(DevMem& mem, uint64_t bytes, int reps, hipStream_t s) {
     const uint64_t n = bytes / 8;
     if (n == 0 || reps < 1) throw std::invalid_argument("copy size and repetitions must be positive");

@@ -474,6 +474,11 @@ int rin_debug_copy(rin_ctx* c, uint64_t bytes, int32_t reps, double* ms) {
     return guarded(c, [&] { *ms = rin::debug_copy(c->mem, bytes, reps, c->stream); });
 }
 
+int rin_debug_sqrt(rin_ctx* c, const double* in, uint64_t n, double* out) {
+    if ((!in || !out) && n) return fail(c, RIN_ERR_ARG, "null argument");
+    return guarded(c, [&] { rin::debug_sqrt(c->mem, in, n, out, c->stream); });
+}
+
 int rin_debug_scan(rin_ctx* c, int32_t fields, int32_t stride, const uint32_t* in, uint64_t n, uint64_t count,
                    uint32_t* out, uint32_t* totals) {
     if ((!in && n) || (!out && n) || !totals) return fail(c, RIN_ERR_ARG, "null argument");

@@ -91,6 +91,7 @@ def lib():
         L.rin_debug_cuts.argtypes = [_vp, ctypes.c_int32, _f64p, _i32p, ctypes.c_int32, ctypes.c_int32, _u16p,
                                      _i32p]
         L.rin_debug_copy.argtypes = [_vp, ctypes.c_uint64, ctypes.c_int32, ctypes.POINTER(ctypes.c_double)]
+        L.rin_debug_sqrt.argtypes = [_vp, _f64p, ctypes.c_uint64, _f64p]
         L.rin_debug_scan.argtypes = [_vp, ctypes.c_int32, ctypes.c_int32, _u32p, ctypes.c_uint64, ctypes.c_uint64,
                                      _u32p, _u32p]
         L.rin_debug_table_cuts.argtypes = [ctypes.c_int32, _f64p, ctypes.c_int32, ctypes.c_int32, _u16p, _i32p]
@@ -176,6 +177,13 @@ class Context:
         self.check(lib().rin_debug_scan(self.h, fields, stride, _p(data, _u32p), n, n if count is None else count,
                                         _p(out, _u32p), _p(tot, _u32p)))
         return out.reshape(fields, n), tot
+
+    def debug_sqrt(self, x):
+        """The device evaluation's correctly rounded square root of float64 x."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        out = np.zeros_like(x)
+        self.check(lib().rin_debug_sqrt(self.h, _p(x, _f64p), x.size, _p(out, _f64p)))
+        return out
 
     def debug_copy(self, nbytes, reps=5):
         """Average milliseconds of one plain device copy of nbytes."""
