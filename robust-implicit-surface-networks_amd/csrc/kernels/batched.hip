@@ -56,7 +56,7 @@ __host__ __device__ inline uint32_t spread_at(int counter, uint32_t slot) {
 }
 enum Ctr : int {
     C_ACTIVE = 0, C_FIT, C_VO, C_FO, C_FV, C_VERTS, C_FACES, C_FACEVERTS, C_B1, C_B2, C_B3, C_B5, C_BX, C_DEGEN,
-    C_Q2, C_Q3, C_Q5, C_ABORT, C_TETFACES, C_RECW, C_DFAIL, C_CUTK3, C_CUTW3, C_CUTK5, C_CUTW5, C_E3, C_E5, C_N
+    C_Q2, C_Q3, C_Q5, C_ABORT, C_TETFACES, C_RECW, C_DFAIL, C_CUTK3, C_CUTW3, C_CUTK5, C_CUTW5, C_E3, C_E5, C_E3B, C_N
 };
 
 struct K1Entry {
@@ -580,7 +580,11 @@ struct EarlyEntry {
 
 // early3, early5 (nullable): lists of the k = 3..4 and 5..8 tets, at most
 // early_cap3 / early_cap5 entries, counted in ctr[C_E3] / ctr[C_E5] (zero at
-// the start of a run).  A tile's entries are contiguous and in tet order.
+// the start of a run).  The k = 3..4 list holds the k = 4 tets from its front
+// (ctr[C_E3] of them) and the k = 3 tets from its back (ctr[C_E3B]), so that a
+// wavefront of the cut mostly holds tets of one k (two insertions past the
+// two-plane start, or one) and the longer ones start first.  A tile's entries
+// are contiguous and in tet order (the back part reversed).
 template <int TILE>
 __global__ void __launch_bounds__(TILE) k_cube_count(GridView g, const uint64_t* __restrict__ pos,
                                                           const uint64_t* __restrict__ neg, uint64_t full,
@@ -592,6 +596,7 @@ __global__ void __launch_bounds__(TILE) k_cube_count(GridView g, const uint64_t*
     const uint32_t cube = tile * TILE + threadIdx.x;
     uint32_t v[kTileFields] = {0, 0, 0, 0, 0, 0, 0, 0};
     uint64_t m[5] = {0, 0, 0, 0, 0};
+    uint32_t n4 = 0;  // k = 4 tets (of the k = 3..4 bucket)
     if (cube < ncubes) {
         uint32_t c[8];
         bool even;
@@ -601,6 +606,7 @@ __global__ void __launch_bounds__(TILE) k_cube_count(GridView g, const uint64_t*
             v[TF_FIT] += (uint32_t)k;
             v[TF_ACTIVE] += k != 0;
             if (k) v[TF_B1 + bucket_of(k)]++;
+            n4 += k == 4;
         }
     }
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -608,14 +614,19 @@ __global__ void __launch_bounds__(TILE) k_cube_count(GridView g, const uint64_t*
     // (active functions, active tets) packed as in the fill pass, and of the
     // listed counts (k = 3..4 in the low half, 5..8 in the high half); one
     // atomic add per list and tile reserves the tile's entries
-    const uint32_t want = (early3 ? v[TF_B3] : 0u) | (early5 ? v[TF_B5] << 16 : 0u);
+    // listed counts: k = 4 | k = 3 << 16 | k = 5..8 << 32
+    const uint64_t want = (early3 ? (uint64_t)n4 | (uint64_t)(v[TF_B3] - n4) << 16 : 0ull) |
+                          (early5 ? (uint64_t)v[TF_B5] << 32 : 0ull);
     if ((early3 || early5) && __syncthreads_or(want != 0)) {
-        __shared__ uint32_t esum[TILE / 64], ebk[TILE / 64], ebase[2];
+        __shared__ uint32_t esum[TILE / 64], ebase[3];
+        __shared__ uint64_t ebk[TILE / 64];
         constexpr int kAsBits = 13;
         const uint32_t packed = v[TF_FIT] << kAsBits | v[TF_ACTIVE];
-        uint32_t inc = packed, binc = want;
+        uint32_t inc = packed;
+        uint64_t binc = want;
         for (int d = 1; d < 64; d <<= 1) {
-            const uint32_t o = __shfl_up(inc, d, 64), ob = __shfl_up(binc, d, 64);
+            const uint32_t o = __shfl_up(inc, d, 64);
+            const uint64_t ob = __shfl_up(binc, d, 64);
             if (lane >= d) {
                 inc += o;
                 binc += ob;
@@ -627,26 +638,36 @@ __global__ void __launch_bounds__(TILE) k_cube_count(GridView g, const uint64_t*
         }
         __syncthreads();
         if (threadIdx.x == 0) {
-            uint32_t tot = 0;
+            uint64_t tot = 0;
             for (int w = 0; w < TILE / 64; ++w) tot += ebk[w];
-            ebase[0] = (tot & 0xffffu) ? atomicAdd(ctr + C_E3, tot & 0xffffu) : 0u;
-            ebase[1] = (tot >> 16) ? atomicAdd(ctr + C_E5, tot >> 16) : 0u;
+            const uint32_t t4 = (uint32_t)tot & 0xffffu, t3 = (uint32_t)(tot >> 16) & 0xffffu,
+                           t5 = (uint32_t)(tot >> 32);
+            ebase[0] = t4 ? atomicAdd(ctr + C_E3, t4) : 0u;
+            ebase[1] = t5 ? atomicAdd(ctr + C_E5, t5) : 0u;
+            ebase[2] = t3 ? atomicAdd(ctr + C_E3B, t3) : 0u;
         }
-        uint32_t ex = inc - packed, bx = binc - want;
+        uint32_t ex = inc - packed;
+        uint64_t bx = binc - want;
         for (int w = 0; w < wave; ++w) {
             ex += esum[w];
             bx += ebk[w];
         }
         __syncthreads();
         uint32_t lo = ex >> kAsBits, lr = ex & ((1u << kAsBits) - 1u);
-        uint32_t b3 = bx & 0xffffu, b5 = bx >> 16;
+        // the tile's k = 4, k = 3 and k = 5..8 tets before this one
+        uint32_t b4 = (uint32_t)bx & 0xffffu, b3 = (uint32_t)(bx >> 16) & 0xffffu, b5 = (uint32_t)(bx >> 32);
         for (int q = 0; q < 5; ++q) {
             const int k = __popcll(m[q]);
             const uint32_t t = cube * 5u + (uint32_t)q;
             if (k >= 3 && k <= 4 && early3) {
-                const uint32_t at = ebase[0] + b3;
-                if (at < early_cap3) early3[at] = {t, tile, lo, lr | b3 << 16};
-                ++b3;
+                const uint32_t rank = b3 + b4;  // place in the tile's bucket list
+                if (k == 4) {
+                    const uint32_t at = ebase[0] + b4++;
+                    if (at < early_cap3) early3[at] = {t, tile, lo, lr | rank << 16};
+                } else {
+                    const uint32_t back = ebase[2] + b3++;
+                    if (back < early_cap3) early3[early_cap3 - 1 - back] = {t, tile, lo, lr | rank << 16};
+                }
             } else if (k >= 5 && k <= 8 && early5) {
                 const uint32_t at = ebase[1] + b5;
                 if (at < early_cap5) early5[at] = {t, tile, lo, lr | b5 << 16};
@@ -857,7 +878,7 @@ struct CutArgs {
     // prefixes (tf: the tile field of the bucket's slots); the tet's
     // functions come from its corner sign masks
     const EarlyEntry* early;
-    const uint32_t* n_early;
+    const uint32_t *n_early, *n_back;  // the list's front and back parts (n_back nullable)
     const TileSums* tiles;
     const uint64_t *pos, *neg;
     uint64_t full;
@@ -937,9 +958,12 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
         // the fill pass shares the CUs: its waves wait on memory, these issue
         // (the one-tet-a-wavefront k = 5..8 cut first)
         __builtin_amdgcn_s_setprio(G == 64 ? 3 : 2);
-        const uint32_t n = min(*A.n_early, A.cap);
-        if (i >= n) return;
-        const EarlyEntry e = A.early[i];
+        // the list's front part, then its back part read from the end; a
+        // list past the bucket's capacity is cut by no one (the fill pass
+        // flags the overflow)
+        const uint32_t nf = *A.n_early, nb = A.n_back ? *A.n_back : 0u;
+        if (nf + nb > A.cap || i >= nf + nb) return;
+        const EarlyEntry e = A.early[i < nf ? i : A.cap - 1 - (i - nf)];
         const TileSums& pre = A.tiles[e.tile];
         r = pre.v[TF_ACTIVE] + (e.lrb & 0xffffu);
         const uint32_t slot = pre.v[A.tf] + (e.lrb >> 16);
@@ -1027,53 +1051,69 @@ struct K2Args {
 
 // k = 2 by table: sign vector of the second plane over the first plane's
 // complex -> slot -> record and counts (no per-tet record is written)
+// the k = 2 table slot of tet t (corners v) whose active functions are
+// fa < fb: the first plane's sign pattern picks the pattern, the second
+// plane's signs at the corners and at the first plane's edge points the
+// slot; K2_NONE when a sign pattern or point falls outside the table
+__device__ __forceinline__ uint32_t k2_table_slot(const K2Args& A, const K2Pat* pats, const uint32_t* v, uint32_t fa,
+                                                  uint32_t fb) {
+    const uint64_t V = A.g.V;
+    int pa = 0, sB[4];
+    for (int c = 0, w = 1; c < 4; ++c, w *= 3) {
+        const uint64_t p = A.pos[v[c]], q = A.neg[v[c]];
+        pa += (((p >> fa) & 1) ? 2 : (((q >> fa) & 1) ? 0 : 1)) * w;
+        sB[c] = ((p >> fb) & 1) ? 1 : (((q >> fb) & 1) ? -1 : 0);
+    }
+    const K2Pat& K = pats[pa];
+    uint32_t slot = K.base, w = 81;
+    for (int c = 0, w3 = 1; c < 4; ++c, w3 *= 3) slot += (uint32_t)(sB[c] + 1) * w3;
+    bool err = !K.valid;
+    for (int q = 0; q + 4 < K.nv; ++q, w *= 3) {
+        const int c1 = K.e[q][0], c2 = K.e[q][1];
+        int sg;
+        if (sB[c1] * sB[c2] < 0) {
+            // second plane at the point of edge c1 c2 where the first vanishes
+            sg = dev::orient_ia_edge(A.vals[(uint64_t)fa * V + v[c1]], A.vals[(uint64_t)fa * V + v[c2]],
+                                     A.vals[(uint64_t)fb * V + v[c1]], A.vals[(uint64_t)fb * V + v[c2]], err);
+        } else {
+            sg = sB[c1] == sB[c2] ? sB[c1] : (sB[c1] != 0 ? sB[c1] : sB[c2]);
+        }
+        slot += (uint32_t)(sg + 1) * w;
+    }
+    return err ? K2_NONE : slot;
+}
+
+// points active tet r at table slot `slot` with its counts; false (flagged,
+// zero counts) when the slot holds no record
+__device__ __forceinline__ bool k2_point(const K2Args& A, uint32_t slot, uint32_t r) {
+    if (slot == K2_NONE || A.slot_off[slot] == K2_NONE) {
+        atomicOr(A.errs, (uint32_t)ERR_CAPACITY);
+        A.cnt[3 * r] = A.cnt[3 * r + 1] = A.cnt[3 * r + 2] = 0;
+        return false;
+    }
+    const uint32_t c3 = A.slot_cnt[slot];
+    A.rec_loc[r] = K1_FLAG | (81u + slot);
+    A.cnt[3 * r] = c3 & 0xFF;
+    A.cnt[3 * r + 1] = (c3 >> 8) & 0xFF;
+    A.cnt[3 * r + 2] = c3 >> 16;
+    return true;
+}
+
+// k = 2 by table: sign vector of the second plane over the first plane's
+// complex -> slot -> record and counts (no per-tet record is written)
 __global__ void k_k2_lookup(K2Args A) {
     __shared__ K2Pat pats[81];
     for (int i = threadIdx.x; i < 81; i += blockDim.x) pats[i] = A.pat[i];
     __syncthreads();
     const uint32_t n = *A.abort ? 0u : min(*A.n, A.cap);
-    const uint64_t V = A.g.V;
     GRID_STRIDE(i, n) {
         const uint32_t r = A.list[i];
         const uint64_t ts = A.lts[i];
         const uint32_t t = (uint32_t)ts, s0 = (uint32_t)(ts >> 32);
-        const uint32_t fa = A.fit[s0], fb = A.fit[s0 + 1];
         uint32_t v[4];
         dev::tet_corners(A.g, t, v);
-        int pa = 0, sB[4];
-        for (int c = 0, w = 1; c < 4; ++c, w *= 3) {
-            const uint64_t p = A.pos[v[c]], q = A.neg[v[c]];
-            pa += (((p >> fa) & 1) ? 2 : (((q >> fa) & 1) ? 0 : 1)) * w;
-            sB[c] = ((p >> fb) & 1) ? 1 : (((q >> fb) & 1) ? -1 : 0);
-        }
-        const K2Pat& K = pats[pa];
-        uint32_t slot = K.base, w = 81;
-        for (int c = 0, w3 = 1; c < 4; ++c, w3 *= 3) slot += (uint32_t)(sB[c] + 1) * w3;
-        bool err = !K.valid;
-        for (int q = 0; q + 4 < K.nv; ++q, w *= 3) {
-            const int c1 = K.e[q][0], c2 = K.e[q][1];
-            int sg;
-            if (sB[c1] * sB[c2] < 0) {
-                // second plane at the point of edge c1 c2 where the first vanishes
-                sg = dev::orient_ia_edge(A.vals[(uint64_t)fa * V + v[c1]], A.vals[(uint64_t)fa * V + v[c2]],
-                                         A.vals[(uint64_t)fb * V + v[c1]], A.vals[(uint64_t)fb * V + v[c2]], err);
-            } else {
-                sg = sB[c1] == sB[c2] ? sB[c1] : (sB[c1] != 0 ? sB[c1] : sB[c2]);
-            }
-            slot += (uint32_t)(sg + 1) * w;
-        }
-        const uint32_t off = err ? K2_NONE : A.slot_off[slot];
-        if (off == K2_NONE) {
-            atomicOr(A.errs, (uint32_t)ERR_CAPACITY);
-            A.rec[A.rec_loc[r]] = 0;
-            A.cnt[3 * r] = A.cnt[3 * r + 1] = A.cnt[3 * r + 2] = 0;
-            continue;
-        }
-        const uint32_t c3 = A.slot_cnt[slot];
-        A.rec_loc[r] = K1_FLAG | (81u + slot);
-        A.cnt[3 * r] = c3 & 0xFF;
-        A.cnt[3 * r + 1] = (c3 >> 8) & 0xFF;
-        A.cnt[3 * r + 2] = c3 >> 16;
+        const uint32_t slot = k2_table_slot(A, pats, v, A.fit[s0], A.fit[s0 + 1]);
+        if (!k2_point(A, slot, r)) A.rec[A.rec_loc[r]] = 0;
     }
 }
 
@@ -1991,6 +2031,30 @@ void stage_count(Workspace& W, const Grid& g, const Values& val, hipStream_t s, 
     scan_u32(W.mem, W.isact.p, W.rank.p, Count(T), W.ctr.p + C_ACTIVE, s);
 }
 
+// the k = 2 table lookups' arguments (the bucket list's size and the abort
+// flag are the caller's)
+K2Args k2_args(Workspace& W, const Grid& g, const Values& val) {
+    K2Args K = {};
+    K.g = g.view;
+    K.vals = val.vals.p;
+    K.pos = W.pos.p;
+    K.neg = W.neg.p;
+    K.act = W.act.p;
+    K.csr_off = W.csr_off.p;
+    K.list = W.list[0].p;
+    K.lts = W.lts[0].p;
+    K.fit = W.fit.p;
+    K.cap = W.cap.b[0];
+    K.pat = W.k2pat.p;
+    K.slot_off = W.k1_off.p + 81;
+    K.slot_cnt = W.k2cnt.p;
+    K.rec_loc = W.rec_loc.p;
+    K.cnt = W.cnt.p;
+    K.errs = W.errs.p;
+    K.rec = W.rec.p;
+    return K;
+}
+
 template <class C>
 void launch_cut(Workspace& W, const Grid& g, const Values& val, int q, int grid, hipStream_t s) {
     CutArgs A;
@@ -2017,26 +2081,9 @@ void launch_cut(Workspace& W, const Grid& g, const Values& val, int q, int grid,
     A.stat_w = q == 1 ? SP_CUTW3 : (q == 2 ? SP_CUTW5 : -1);
     A.k2s = W.k2_start ? K2Snap{W.k2pat.p, W.k2snap_off.p, W.k2snap.p} : K2Snap{nullptr, nullptr, nullptr};
     if (grid == -2) {  // k = 2 table
-        K2Args K;
-        K.g = g.view;
-        K.vals = val.vals.p;
-        K.pos = W.pos.p;
-        K.neg = W.neg.p;
-        K.act = W.act.p;
-        K.csr_off = W.csr_off.p;
-        K.list = W.list[q].p;
-        K.lts = W.lts[q].p;
-        K.fit = W.fit.p;
+        K2Args K = k2_args(W, g, val);
         K.n = A.n;
         K.abort = A.abort;
-        K.cap = A.cap;
-        K.pat = W.k2pat.p;
-        K.slot_off = W.k1_off.p + 81;
-        K.slot_cnt = W.k2cnt.p;
-        K.rec_loc = W.rec_loc.p;
-        K.cnt = W.cnt.p;
-        K.errs = W.errs.p;
-        K.rec = W.rec.p;
         hipLaunchKernelGGL(k_k2_lookup, dim3(nblk(W.cap.b[q])), dim3(256), 0, s, K);
         return;
     }
@@ -2081,6 +2128,7 @@ void launch_cut_early(Workspace& W, const Grid& g, const Values& val, int q, int
     A.stat_w = q == 1 ? SP_CUTW3 : SP_CUTW5;
     A.early = W.early[q].p;
     A.n_early = W.ctr.p + (q == 1 ? C_E3 : C_E5);
+    A.n_back = q == 1 ? W.ctr.p + C_E3B : nullptr;
     A.tiles = W.tile_sums.p;
     A.pos = W.pos.p;
     A.neg = W.neg.p;
@@ -2650,9 +2698,11 @@ void run_batched(DevMem& ctx_mem, const Grid& g, const Values& val, uint32_t fla
                     for (int tile : {256, 512, 1024}) {
                         w.cube_tile = tile;
                         float ms = 0;
+                        // the count pass as the whole path runs it with lane-group
+                        // cuts: listing the k = 3..8 tets (the lists go unused here)
                         for (int rep = 0; rep < 2; ++rep)  // the second pass, caches warm
                             ms = time_ms(s, [&] {
-                                stage_count(w, g, val, s);
+                                stage_count(w, g, val, s, true, w.cap.b[1] != 0, w.cap.b[2] != 0);
                                 stage_fill(w, g, val, s);
                             });
                         if (ms < best) {
