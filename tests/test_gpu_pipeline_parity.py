@@ -366,6 +366,42 @@ def test_batched_cube_tiles(ctx, which, tile, monkeypatch):
     g.release()
 
 
+@pytest.mark.parametrize("form", ["wave16", "wave"])
+def test_batched_bucket_lists_outgrow_the_plan(ctx, form, monkeypatch):
+    """A second value set of the same size with many more tets of 2..5
+    active functions than the first overflows the plan's bucket capacities
+    (the count pass's k = 3..4 list, with its k = 4 front and k = 3 back
+    parts, included): the run grows the workspace and still matches the
+    oracle; both sets, in turn, keep matching it afterwards.  Counts per k,
+    res 32: first set 1804 / 47 / 0 / 0 tets of k = 2 / 3 / 4 / 5, second
+    set 4372 / 446 / 44 / 12."""
+    import rin
+
+    monkeypatch.setenv("RIN_CUT_FORM", form)
+    res = 32
+    few = W.FuncSet()
+    few.sphere([0.1, -0.2, 0.15], 0.5)
+    few.sphere([-0.2, 0.1, 0.0], 0.45)
+    for i in range(6):
+        few.plane([-0.75 + 0.3 * i, 0, 0], [1, 0, 0])
+    many = W.mixed(np.random.default_rng(33), 2, 2, 2, 2)
+    g = ctx.grid_regular(res)
+    pts, tets = S.grid(res)
+    sets = []
+    for fs in (few, many):
+        types, params = fs.arrays()
+        sets.append((g.eval(types, params), S.OracleRun(0, pts, tets, S.evaluate(types, params, pts))))
+    assert sets[0][1].buckets != sets[1][1].buckets
+    for i in (0, 1, 0, 1):
+        vals, ref = sets[i]
+        dev = vals.run(rin.FLAG_BATCHED)
+        _compare(dev, ref)
+        dev.release()
+    for vals, _ in sets:
+        vals.release()
+    g.release()
+
+
 @pytest.mark.parametrize("form", ["batched", "plain", "batched-scratch"])
 def test_headline_workload_full_size(ctx, form, monkeypatch):
     """The benchmark workload itself: 101^3 vertices (5,000,000 tets), 20
