@@ -615,9 +615,9 @@ __global__ void __launch_bounds__(TILE) k_cube_count(GridView g, const uint64_t*
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     // tiles holding listed tets rank them inside the tile: exclusive scans of
     // (active functions, active tets) packed as in the fill pass, and of the
-    // listed counts (k = 3..4 in the low half, 5..8 in the high half); one
-    // atomic add per list and tile reserves the tile's entries
-    // listed counts: k = 4 | k = 3 << 16 | k = 5..8 << 32
+    // listed counts in 16-bit fields (k = 4 | k = 3 << 16 | k = 5..8 << 32;
+    // a tile holds at most 5 TILE tets); one atomic add per list part and
+    // tile reserves the tile's entries
     const uint64_t want = (early3 ? (uint64_t)n4 | (uint64_t)(v[TF_B3] - n4) << 16 : 0ull) |
                           (early5 ? (uint64_t)v[TF_B5] << 32 : 0ull);
     if ((early3 || early5) && __syncthreads_or(want != 0)) {
