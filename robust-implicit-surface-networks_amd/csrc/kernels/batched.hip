@@ -1040,6 +1040,7 @@ struct K2Args {
     GridView g;
     const double* vals;
     const uint64_t *pos, *neg;
+    uint64_t full;  // the functions' bits
     const uint32_t *act, *csr_off, *list;
     const uint64_t* lts;  // per slot: CSR offset << 32 | tet
     const uint8_t* fit;
@@ -1052,8 +1053,6 @@ struct K2Args {
     uint16_t* rec;
 };
 
-// k = 2 by table: sign vector of the second plane over the first plane's
-// complex -> slot -> record and counts (no per-tet record is written)
 // the k = 2 table slot of tet t (corners v) whose active functions are
 // fa < fb: the first plane's sign pattern picks the pattern, the second
 // plane's signs at the corners and at the first plane's edge points the
@@ -1111,11 +1110,16 @@ __global__ void k_k2_lookup(K2Args A) {
     const uint32_t n = *A.abort ? 0u : min(*A.n, A.cap);
     GRID_STRIDE(i, n) {
         const uint32_t r = A.list[i];
-        const uint64_t ts = A.lts[i];
-        const uint32_t t = (uint32_t)ts, s0 = (uint32_t)(ts >> 32);
+        const uint32_t t = (uint32_t)A.lts[i];
         uint32_t v[4];
         dev::tet_corners(A.g, t, v);
-        const uint32_t slot = k2_table_slot(A, pats, v, A.fit[s0], A.fit[s0 + 1]);
+        // the two active functions from the corner masks (the CSR's order,
+        // one dependent load fewer than reading the CSR)
+        const uint64_t m = A.full & ~((A.pos[v[0]] & A.pos[v[1]] & A.pos[v[2]] & A.pos[v[3]]) |
+                                      (A.neg[v[0]] & A.neg[v[1]] & A.neg[v[2]] & A.neg[v[3]]));
+        const uint32_t fa = (uint32_t)(__ffsll((unsigned long long)m) - 1);
+        const uint32_t fb = (uint32_t)(__ffsll((unsigned long long)(m & (m - 1))) - 1);
+        const uint32_t slot = k2_table_slot(A, pats, v, fa, fb);
         if (!k2_point(A, slot, r)) A.rec[A.rec_loc[r]] = 0;
     }
 }
@@ -2042,6 +2046,7 @@ K2Args k2_args(Workspace& W, const Grid& g, const Values& val) {
     K.vals = val.vals.p;
     K.pos = W.pos.p;
     K.neg = W.neg.p;
+    K.full = (val.F == 64) ? ~0ull : ((1ull << val.F) - 1);
     K.act = W.act.p;
     K.csr_off = W.csr_off.p;
     K.list = W.list[0].p;
