@@ -2526,13 +2526,16 @@ void record_whole(Workspace& w, const Grid& g, const Values& val, hipStream_t s)
     const bool e5 = g.view.regular && w.cap.b[2] && w.grid[2] < 0;
     stage_count(w, g, val, s, false, e3, e5);
     if (e3 || e5) {
-        // the longest branch (the k = 5..8 cut: one tet per wavefront, bound
-        // by its slowest tet) stays on the capturing stream, captured first
+        // the k = 3..4 cut, captured first on the capturing stream, starts
+        // a few microseconds before the other branches: its wavefronts (all
+        // resident at once when they find the CUs free) take their place
+        // before the fill pass's workgroups fill the CUs; the k = 5..8 cut
+        // (tens of wavefronts) and the fill pass follow on side streams
         RIN_HIP(hipEventRecord(w.fork, s));
-        if (e5) launch_cut_early(w, g, val, 2, w.grid[2], s);
-        if (e3) {
+        if (e3) launch_cut_early(w, g, val, 1, w.grid[1], s);
+        if (e5) {
             RIN_HIP(hipStreamWaitEvent(w.side, w.fork, 0));
-            launch_cut_early(w, g, val, 1, w.grid[1], w.side);
+            launch_cut_early(w, g, val, 2, w.grid[2], w.side);
             RIN_HIP(hipEventRecord(w.join, w.side));
         }
         RIN_HIP(hipStreamWaitEvent(w.side2, w.fork, 0));
@@ -2541,7 +2544,7 @@ void record_whole(Workspace& w, const Grid& g, const Values& val, hipStream_t s)
         if (!e3 && w.cap.b[1]) launch_cut<CMed>(w, g, val, 1, w.grid[1], w.side2);
         if (w.cap.b[0]) launch_cut<CK2>(w, g, val, 0, w.grid[0], w.side2);
         RIN_HIP(hipEventRecord(w.join2, w.side2));
-        if (e3) RIN_HIP(hipStreamWaitEvent(s, w.join, 0));
+        if (e5) RIN_HIP(hipStreamWaitEvent(s, w.join, 0));
         RIN_HIP(hipStreamWaitEvent(s, w.join2, 0));
     } else {
         stage_fill(w, g, val, s);
