@@ -402,6 +402,37 @@ def test_batched_bucket_lists_outgrow_the_plan(ctx, form, monkeypatch):
     g.release()
 
 
+@pytest.mark.parametrize("form", ["wave16", "wave", "thread", "plain"])
+def test_sixty_four_functions(ctx, form, monkeypatch):
+    """64 functions, the widest sign masks (every bit a function): the
+    batched form's count-pass lists, the k = 2 lookups and the cuts, which
+    take a tet's functions from its corner masks, and the plain form match
+    the oracle.  64 small spheres at res 24: 14898 / 4287 / 1160 / 380 / 128
+    / 23 / 8 tets of k = 1..7, every function active somewhere."""
+    import rin
+
+    if form != "plain":
+        monkeypatch.setenv("RIN_CUT_FORM", form)
+    res = 24
+    rng = np.random.default_rng(64)
+    fs = W.FuncSet()
+    for _ in range(64):
+        fs.sphere(rng.uniform(-0.8, 0.8, 3), rng.uniform(0.1, 0.3))
+    types, params = fs.arrays()
+    g = ctx.grid_regular(res)
+    vals = g.eval(types, params)
+    pts, tets = S.grid(res)
+    ref = S.OracleRun(0, pts, tets, S.evaluate(types, params, pts))
+    assert len(np.unique(ref.func_in_tet)) == 64
+    flags = 0 if form == "plain" else rin.FLAG_BATCHED
+    for _ in range(2):
+        dev = vals.run(flags)
+        _compare(dev, ref)
+        dev.release()
+    vals.release()
+    g.release()
+
+
 @pytest.mark.parametrize("form", ["batched", "plain", "batched-scratch"])
 def test_headline_workload_full_size(ctx, form, monkeypatch):
     """The benchmark workload itself: 101^3 vertices (5,000,000 tets), 20
