@@ -433,6 +433,38 @@ def test_sixty_four_functions(ctx, form, monkeypatch):
     g.release()
 
 
+@pytest.mark.parametrize("batched", [True, False])
+def test_mi_sixty_four_materials(ctx, batched):
+    """64 materials (every mask bit a material) through the material
+    interface, batched and plain, against the oracle: Voronoi-like
+    materials -|x - p_i| + w_i at res 24, 16626 / 6337 / 3089 tets of 2 / 3
+    / 4 or more materials (up to 9: the batched form takes those with its
+    large-capacity cut and stays one graph)."""
+    import rin
+
+    res = 24
+    rng = np.random.default_rng(164)
+    fs = W.FuncSet()
+    for _ in range(64):
+        fs.sphere(rng.uniform(-0.9, 0.9, 3), rng.uniform(0.0, 0.1))
+    types, params = fs.arrays()
+    g = ctx.grid_regular(res)
+    vals = g.eval(types, params)
+    pts, tets = S.grid(res)
+    ref = S.OracleRun(1, pts, tets, S.evaluate(types, params, pts))
+    assert ref.num_intersecting > 0
+    flags = rin.FLAG_MI | (rin.FLAG_BATCHED if batched else 0)
+    for _ in range(2):
+        dev = vals.run(flags)
+        if batched:
+            label = dev.stage_times()[0][0]
+            assert label == "batched path (one graph)", label
+        _compare(dev, ref)
+        dev.release()
+    vals.release()
+    g.release()
+
+
 @pytest.mark.parametrize("form", ["batched", "plain", "batched-scratch"])
 def test_headline_workload_full_size(ctx, form, monkeypatch):
     """The benchmark workload itself: 101^3 vertices (5,000,000 tets), 20
