@@ -465,6 +465,45 @@ def test_mi_sixty_four_materials(ctx, batched):
     g.release()
 
 
+@pytest.mark.parametrize("mi", [0, 1])
+def test_no_active_tets_then_some(ctx, mi):
+    """Functions that cross no tet (planes outside the grid's box; for the
+    material interface, one material above the others everywhere) give the
+    oracle's empty output in every form, on the capturing run and a replay;
+    a value set of the same size with active tets then grows the same plan
+    from empty buckets and still matches."""
+    import rin
+
+    res = 16
+    empty = W.FuncSet()
+    for j in range(4):
+        empty.plane([1.5 + 0.1 * j, 0, 0], [1, 0.2 * j, 0])
+    if mi:
+        # material 0 is above the others everywhere in the box
+        empty = W.FuncSet()
+        empty.plane([5.0, 0, 0], [-1, 0, 0])
+        for j in range(3):
+            empty.plane([-3.0 - j, 0, 0], [-1, 0, 0])
+    some = W.mixed(np.random.default_rng(17), 1, 1, 1, 1)
+    g = ctx.grid_regular(res)
+    pts, tets = S.grid(res)
+    sets = []
+    for fs in (empty, some):
+        types, params = fs.arrays()
+        sets.append((g.eval(types, params), S.OracleRun(mi, pts, tets, S.evaluate(types, params, pts))))
+    assert sets[0][1].num_intersecting == 0 and sets[1][1].num_intersecting > 0
+    base = rin.FLAG_MI if mi else 0
+    for flags in (base, base | rin.FLAG_BATCHED):
+        for i in (0, 0, 1, 0):
+            vals, ref = sets[i]
+            dev = vals.run(flags)
+            _compare(dev, ref)
+            dev.release()
+    for vals, _ in sets:
+        vals.release()
+    g.release()
+
+
 @pytest.mark.parametrize("form", ["batched", "plain", "batched-scratch"])
 def test_headline_workload_full_size(ctx, form, monkeypatch):
     """The benchmark workload itself: 101^3 vertices (5,000,000 tets), 20
