@@ -504,6 +504,29 @@ def test_no_active_tets_then_some(ctx, mi):
     g.release()
 
 
+@pytest.mark.parametrize("res", [1, 2, 3])
+@pytest.mark.parametrize("mi", [0, 1])
+def test_tiny_grids(ctx, res, mi):
+    """Grids of one, eight and 27 cubes (a single, partly filled cube tile)
+    with functions crossing them, in every form, against the oracle."""
+    import rin
+
+    fs = W.mixed(np.random.default_rng(100 + res), 2, 2, 1, 1)
+    types, params = fs.arrays()
+    g = ctx.grid_regular(res)
+    vals = g.eval(types, params)
+    pts, tets = S.grid(res)
+    ref = S.OracleRun(mi, pts, tets, S.evaluate(types, params, pts))
+    assert ref.num_intersecting > 0
+    base = rin.FLAG_MI if mi else 0
+    for flags in (base, base | rin.FLAG_BATCHED, base | rin.FLAG_BATCHED):
+        dev = vals.run(flags)
+        _compare(dev, ref)
+        dev.release()
+    vals.release()
+    g.release()
+
+
 @pytest.mark.parametrize("form", ["batched", "plain", "batched-scratch"])
 def test_headline_workload_full_size(ctx, form, monkeypatch):
     """The benchmark workload itself: 101^3 vertices (5,000,000 tets), 20
