@@ -2276,9 +2276,12 @@ void stage_rest(Workspace& W, const Grid& g, const Values& val, hipStream_t s, i
         E.plan = W.k2plan.p;
         E.n_plans = (uint32_t)k2_plans().off.size();
         if ((part & REST_EXTRACT_K1) && (part & REST_EXTRACT_REC)) {
-            const uint32_t nb_rec = std::min<uint32_t>((c.b[0] + 63) / 64, 8192u) + 1;
+            // up to 16384 workgroups a lane-per-tet role (one lane per active
+            // tet at 101^3: 0.375 ms/step against 0.381 with 8192)
+            const uint32_t ecap = 16384u;
+            const uint32_t nb_rec = std::min<uint32_t>((c.b[0] + 63) / 64, ecap) + 1;
             const uint32_t nb_wave = std::max(1u, c.b[1] + c.b[2]);
-            const uint32_t nb_k1 = std::min<uint32_t>((c.active + 63) / 64, 8192u) + 1;
+            const uint32_t nb_k1 = std::min<uint32_t>((c.active + 63) / 64, ecap) + 1;
             hipLaunchKernelGGL(k_extract_all, dim3(nb_rec + nb_wave + nb_k1), dim3(64), 0, s, E, nb_rec, nb_wave,
                                nb_k1);
         } else {
