@@ -2401,9 +2401,11 @@ void stage_rest(Workspace& W, const Grid& g, const Values& val, hipStream_t s, i
         FA.sizes = nullptr;
         FA.sizes_out = W.fsizes.p;
         const unsigned fgrid = (unsigned)((c.fo + kFaceTile - 1) / kFaceTile);
+        // the coordinates role: up to 8192 workgroups (at 101^3 0.374 ms/step
+        // against 0.381 with 2048)
+        const unsigned xgrid = std::min(nblk(c.vo), 8192u);
         if (xyz_with_faces)
-            hipLaunchKernelGGL(k_faces_and_xyz, dim3(fgrid + cgrid(c.vo)), dim3(256), 0, s, FA, VF, fgrid,
-                               cgrid(c.vo));
+            hipLaunchKernelGGL(k_faces_and_xyz, dim3(fgrid + xgrid), dim3(256), 0, s, FA, VF, fgrid, xgrid);
         else
             hipLaunchKernelGGL(k_face_first_tiled, dim3(fgrid), dim3(256), 0, s, FA);
         // the emit launch closes the run (counters published and cleared); it
