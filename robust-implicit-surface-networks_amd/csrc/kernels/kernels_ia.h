@@ -1219,7 +1219,7 @@ struct StarArgs {
 // chains of dependent loads (key, star tet, its rank and offsets, its keys)
 // overlap; a workgroup takes one tile of consecutive occurrences, lane-major
 // within each of the kDedupU rows.
-constexpr int kDedupU = 2;  // measured against 4 and 8: more waves beat more loads in flight per lane
+constexpr int kDedupU = 2;  // measured against 1, 4 and 8 occurrences a lane
 constexpr uint32_t kDedupTile = 256u * kDedupU;
 
 // lowest tet of the star of key K's simplex inside the grid (~0: none)
